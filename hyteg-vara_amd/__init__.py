@@ -1,0 +1,261 @@
+"""ctypes binding of libhyteg_amd.so (the C ABI declared in include/hyteg_amd.h).
+
+The binding is plumbing for tests and benchmarks: every operation runs in the HIP library.
+Loading fails loudly when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libhyteg_amd.so")
+HEADER = os.path.join(ROOT, "include", "hyteg_amd.h")
+MESHES = os.path.join(ROOT, "meshes")
+
+OK, ERR_ARG, ERR_HIP, ERR_IO, ERR_NUMERIC, ERR_STATE = 0, 1, 2, 3, 4, 5
+NONE, INNER, DIRICHLET, NEUMANN, FREESLIP, ALL = 0, 1, 2, 4, 8, 15
+REPLACE, ADD = 0, 1
+F64, F32 = 8, 4
+
+_lib = None
+
+
+class HygError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hyg error {code}: {msg}")
+        self.code = code
+
+
+def _sig(lib):
+    vp, i, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+    P = np.ctypeslib.ndpointer
+    pvp = C.POINTER(C.c_void_p)
+    table = {
+        "hyg_last_error": (C.c_char_p, []),
+        "hyg_init": (i, [i]),
+        "hyg_mesh_create_from_msh": (i, [C.c_char_p, pvp]),
+        "hyg_mesh_destroy": (i, [vp]),
+        "hyg_mesh_info": (i, [vp, P(np.int64)]),
+        "hyg_mesh_num_dofs": (i, [vp, i, C.POINTER(C.c_int64)]),
+        "hyg_mesh_dof_coords": (i, [vp, i, P(np.float64), P(np.uint8)]),
+        "hyg_func_create": (i, [vp, i, i, i, pvp]),
+        "hyg_func_destroy": (i, [vp]),
+        "hyg_func_upload": (i, [vp, i, P(np.float64), vp]),
+        "hyg_func_download": (i, [vp, i, P(np.float64), vp]),
+        "hyg_interpolate_const": (i, [vp, d, i, i, vp]),
+        "hyg_assign": (i, [vp, i, P(np.float64), pvp, i, i, vp]),
+        "hyg_add": (i, [vp, i, P(np.float64), pvp, i, i, vp]),
+        "hyg_mult_elementwise": (i, [vp, vp, vp, i, i, vp]),
+        "hyg_dot": (i, [vp, vp, i, i, C.POINTER(C.c_double), vp]),
+        "hyg_convert": (i, [vp, vp, i, vp]),
+        "hyg_op_create_p1_const": (i, [vp, i, i, pvp]),
+        "hyg_op_destroy": (i, [vp]),
+        "hyg_op_cell_stencil": (i, [vp, i, i, P(np.float64)]),
+        "hyg_apply": (i, [vp, vp, vp, i, i, i, vp]),
+        "hyg_smooth_jac": (i, [vp, vp, vp, vp, d, i, i, vp]),
+        "hyg_inverse_diagonal": (i, [vp, vp, i, vp]),
+        "hyg_residual": (i, [vp, vp, vp, vp, i, i, vp]),
+        "hyg_restrict": (i, [vp, i, i, vp]),
+        "hyg_prolongate": (i, [vp, i, i, i, vp]),
+        "hyg_cheb_coefficients": (i, [i, d, d, P(np.float64)]),
+        "hyg_power_iteration": (i, [vp, vp, vp, i, i, C.POINTER(C.c_double), vp]),
+        "hyg_gmg_create": (i, [vp, i, P(np.int32), P(np.float64), pvp]),
+        "hyg_gmg_destroy": (i, [vp]),
+        "hyg_gmg_vcycle": (i, [vp, vp, vp, i, vp]),
+        "hyg_gmg_solve": (i, [vp, vp, vp, i, i, P(np.float64), vp]),
+        "hyg_ir_solve": (i, [vp, vp, vp, vp, i, i, i, d, P(np.float64), C.POINTER(C.c_int), vp]),
+    }
+    for name, (res, args) in table.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return table
+
+
+def header_symbols():
+    """Entry points declared in include/hyteg_amd.h."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(hyg_\w+)\s*\(", text, re.M)))
+
+
+def load():
+    """Load the HIP library (no device calls)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(LIB_PATH)
+        _sig(lib)
+        _lib = lib
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        raise HygError(rc, load().hyg_last_error().decode())
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        check(load().hyg_init(device))
+        _initialised = True
+
+
+def _out():
+    return C.c_void_p()
+
+
+class Mesh:
+    def __init__(self, name):
+        init()
+        self.path = name if os.path.isabs(name) else os.path.join(MESHES, name)
+        h = _out()
+        check(load().hyg_mesh_create_from_msh(self.path.encode(), C.byref(h)))
+        self.h = h
+
+    def info(self):
+        out = np.zeros(7, dtype=np.int64)
+        check(load().hyg_mesh_info(self.h, out))
+        return out
+
+    def ndofs(self, L):
+        n = C.c_int64()
+        check(load().hyg_mesh_num_dofs(self.h, L, C.byref(n)))
+        return n.value
+
+    def coords(self, L):
+        n = self.ndofs(L)
+        xyz = np.zeros(3 * n)
+        bnd = np.zeros(n, dtype=np.uint8)
+        check(load().hyg_mesh_dof_coords(self.h, L, xyz, bnd))
+        return xyz.reshape(n, 3), bnd.astype(bool)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.hyg_mesh_destroy(self.h)
+
+
+class Function:
+    def __init__(self, mesh, minL, maxL, dtype=F64):
+        self.mesh, self.minL, self.maxL, self.dtype = mesh, minL, maxL, dtype
+        h = _out()
+        check(load().hyg_func_create(mesh.h, minL, maxL, dtype, C.byref(h)))
+        self.h = h
+
+    def upload(self, L, v):
+        check(load().hyg_func_upload(self.h, L, np.ascontiguousarray(v, dtype=np.float64), None))
+
+    def download(self, L):
+        out = np.zeros(self.mesh.ndofs(L))
+        check(load().hyg_func_download(self.h, L, out, None))
+        return out
+
+    def interpolate(self, value, L, flag=ALL):
+        check(load().hyg_interpolate_const(self.h, value, L, flag, None))
+
+    def assign(self, scalars, funcs, L, flag=ALL):
+        arr = (C.c_void_p * len(funcs))(*[f.h.value for f in funcs])
+        check(load().hyg_assign(self.h, len(funcs), np.asarray(scalars, dtype=np.float64), arr, L, flag, None))
+
+    def add(self, scalars, funcs, L, flag=ALL):
+        arr = (C.c_void_p * len(funcs))(*[f.h.value for f in funcs])
+        check(load().hyg_add(self.h, len(funcs), np.asarray(scalars, dtype=np.float64), arr, L, flag, None))
+
+    def mult(self, a, b, L, flag=ALL):
+        check(load().hyg_mult_elementwise(self.h, a.h, b.h, L, flag, None))
+
+    def dot(self, other, L, flag=ALL):
+        out = C.c_double()
+        check(load().hyg_dot(self.h, other.h, L, flag, C.byref(out), None))
+        return out.value
+
+    def convert_from(self, src, L):
+        check(load().hyg_convert(self.h, src.h, L, None))
+
+    def restrict(self, L, flag=INNER):
+        check(load().hyg_restrict(self.h, L, flag, None))
+
+    def prolongate(self, L, flag=INNER, update=REPLACE):
+        check(load().hyg_prolongate(self.h, L, flag, update, None))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.hyg_func_destroy(self.h)
+
+
+class P1ConstantOperator:
+    def __init__(self, mesh, minL, maxL):
+        self.mesh = mesh
+        h = _out()
+        check(load().hyg_op_create_p1_const(mesh.h, minL, maxL, C.byref(h)))
+        self.h = h
+
+    def cell_stencil(self, L, c):
+        out = np.zeros(15)
+        check(load().hyg_op_cell_stencil(self.h, L, c, out))
+        return out
+
+    def apply(self, src, dst, L, flag=INNER, update=REPLACE):
+        check(load().hyg_apply(self.h, src.h, dst.h, L, flag, update, None))
+
+    def smooth_jac(self, dst, rhs, src, relax, L, flag=INNER):
+        check(load().hyg_smooth_jac(self.h, dst.h, rhs.h, src.h, relax, L, flag, None))
+
+    def inverse_diagonal(self, dst, L):
+        check(load().hyg_inverse_diagonal(self.h, dst.h, L, None))
+
+    def residual(self, r, b, x, L, flag=INNER):
+        check(load().hyg_residual(self.h, r.h, b.h, x.h, L, flag, None))
+
+    def power_iteration(self, x, tmp, iters, L):
+        out = C.c_double()
+        check(load().hyg_power_iteration(self.h, x.h, tmp.h, iters, L, C.byref(out), None))
+        return out.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.hyg_op_destroy(self.h)
+
+
+def cheb_coefficients(order, lower, upper):
+    out = np.zeros(5)
+    check(load().hyg_cheb_coefficients(order, lower, upper, out))
+    return out[:order]
+
+
+class GMG:
+    def __init__(self, op, minL, maxL, dtype=F64, pre=2, post=2, inc=0, smoother="jacobi", relax=2.0 / 3.0,
+                 cheb_order=2, cheb_bounds=(0.0, 0.0), cg_max_iter=10000, cg_tol=1e-16):
+        self.op, self.maxL = op, maxL
+        ip = np.array([minL, maxL, pre, post, inc, 0 if smoother == "jacobi" else 1, cheb_order, cg_max_iter],
+                      dtype=np.int32)
+        dp = np.array([relax, cheb_bounds[0], cheb_bounds[1], cg_tol])
+        h = _out()
+        check(load().hyg_gmg_create(op.h, dtype, ip, dp, C.byref(h)))
+        self.h = h
+
+    def vcycle(self, x, b, L):
+        check(load().hyg_gmg_vcycle(self.h, x.h, b.h, L, None))
+
+    def solve(self, x, b, L, cycles):
+        res = np.zeros(cycles + 1)
+        check(load().hyg_gmg_solve(self.h, x.h, b.h, L, cycles, res, None))
+        return res
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.hyg_gmg_destroy(self.h)
+
+
+def ir_solve(op64, gmg32, x, b, L, inner=1, max_outer=30, rel_tol=1e-10):
+    res = np.zeros(max_outer + 1)
+    steps = C.c_int()
+    check(load().hyg_ir_solve(op64.h, gmg32.h, x.h, b.h, L, inner, max_outer, rel_tol, res, C.byref(steps), None))
+    return res[: steps.value + 1]

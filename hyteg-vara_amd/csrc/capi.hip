@@ -1,0 +1,992 @@
+// C ABI of the library (declared in include/hyteg_amd.h): handles, host<->device transfer,
+// operators, grid transfer and the multigrid / defect-correction drivers.
+#include "../../include/hyteg_amd.h"
+#include "kernels.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+namespace hyg {
+void assembleP1Const( hyg_op_s& op );
+void primEntriesPos( const LowerEntry& e, int N, int i, int j, int& qx, int& qy, int& qz );
+} // namespace hyg
+
+using namespace hyg;
+
+namespace {
+
+thread_local std::string g_err;
+int                      g_device = -1;
+
+// per-process reduction scratch (dot products); calls that overlap on one device race on it
+struct DotScratch
+{
+   double* partials = nullptr;
+   size_t  cap      = 0;
+   double* result   = nullptr;
+   double* host     = nullptr; // pinned
+} g_dot;
+
+int fail( int code, const std::string& msg )
+{
+   g_err = msg;
+   return code;
+}
+
+#define HYG_HIP( call )                                                                                                         \
+   do                                                                                                                          \
+   {                                                                                                                           \
+      hipError_t e_ = ( call );                                                                                                \
+      if ( e_ != hipSuccess )                                                                                                  \
+         return fail( HYG_ERR_HIP, std::string( #call ) + ": " + hipGetErrorString( e_ ) );                                   \
+   } while ( 0 )
+
+#define HYG_TRY( expr )                                                                                                         \
+   do                                                                                                                          \
+   {                                                                                                                           \
+      int rc_ = ( expr );                                                                                                      \
+      if ( rc_ != HYG_OK )                                                                                                     \
+         return rc_;                                                                                                           \
+   } while ( 0 )
+
+inline hipStream_t S( void* s ) { return static_cast< hipStream_t >( s ); }
+
+int checkInit()
+{
+   if ( g_device < 0 )
+      return fail( HYG_ERR_STATE, "hyg_init has not been called" );
+   return HYG_OK;
+}
+
+int launched( const char* what )
+{
+   hipError_t e = hipGetLastError();
+   if ( e != hipSuccess )
+      return fail( HYG_ERR_HIP, std::string( what ) + ": kernel launch failed: " + hipGetErrorString( e ) );
+   return HYG_OK;
+}
+
+int checkLevel( const hyg_func_s* f, int L )
+{
+   if ( !f )
+      return fail( HYG_ERR_ARG, "null function handle" );
+   if ( L < f->minL || L > f->maxL )
+      return fail( HYG_ERR_ARG, "level " + std::to_string( L ) + " outside the function's range [" + std::to_string( f->minL ) +
+                                     ", " + std::to_string( f->maxL ) + "]" );
+   return HYG_OK;
+}
+
+int checkOpLevel( const hyg_op_s* op, int L )
+{
+   if ( !op )
+      return fail( HYG_ERR_ARG, "null operator handle" );
+   if ( L < op->minL || L > op->maxL )
+      return fail( HYG_ERR_ARG, "level " + std::to_string( L ) + " outside the operator's range" );
+   return HYG_OK;
+}
+
+int checkPair( const hyg_func_s* a, const hyg_func_s* b, int L )
+{
+   HYG_TRY( checkLevel( a, L ) );
+   HYG_TRY( checkLevel( b, L ) );
+   if ( a->mesh != b->mesh )
+      return fail( HYG_ERR_ARG, "functions live on different meshes" );
+   if ( a->dtype != b->dtype )
+      return fail( HYG_ERR_ARG, "functions have different value types" );
+   return HYG_OK;
+}
+
+i64 levelValues( const hyg_func_s* f, int L ) { return i64( f->mesh->ctx.mesh->nC() ) * cellStride( L ); }
+
+int createFunc( hyg_mesh_s* mesh, int minL, int maxL, int dtype, hyg_func_s** out )
+{
+   auto* f  = new ( std::nothrow ) hyg_func_s;
+   if ( !f )
+      return fail( HYG_ERR_ARG, "out of host memory" );
+   f->mesh  = mesh;
+   f->minL  = minL;
+   f->maxL  = maxL;
+   f->dtype = dtype;
+   f->data.assign( maxL + 1, nullptr );
+   for ( int L = minL; L <= maxL; ++L )
+   {
+      mesh->ctx.geom( L );
+      const size_t bytes = size_t( levelValues( f, L ) ) * f->elem();
+      hipError_t   e     = hipMalloc( &f->data[L], bytes );
+      if ( e == hipSuccess )
+         e = hipMemset( f->data[L], 0, bytes );
+      if ( e != hipSuccess )
+      {
+         for ( void* p : f->data )
+            (void) hipFree( p );
+         delete f;
+         return fail( HYG_ERR_HIP, std::string( "allocating function memory: " ) + hipGetErrorString( e ) );
+      }
+   }
+   *out = f;
+   return HYG_OK;
+}
+
+void destroyFunc( hyg_func_s* f )
+{
+   if ( !f )
+      return;
+   for ( void* p : f->data )
+      (void) hipFree( p );
+   delete f;
+}
+
+struct FuncGuard
+{
+   hyg_func_s* f = nullptr;
+   ~FuncGuard() { destroyFunc( f ); }
+};
+
+// visit every canonical DoF: fn(canonicalIndex, prim or -1, entry list / cell, i, j, k)
+template < typename Fn >
+void forEachCanonical( const MeshCtx& mc, int L, Fn&& fn )
+{
+   const Mesh& m = *mc.mesh;
+   const int   N = 1 << L;
+   i64         n = 0;
+   for ( size_t p = 0; p < mc.primsHost.size(); ++p )
+   {
+      const LowerPrim& pr = mc.primsHost[p];
+      if ( pr.kind == 0 )
+         fn( n++, int( p ), 0, 0, 0 );
+      else if ( pr.kind == 1 )
+         for ( int i = 1; i <= N - 1; ++i )
+            fn( n++, int( p ), i, 0, 0 );
+      else
+         for ( int j = 1; j <= N - 2; ++j )
+            for ( int i = 1; i + j <= N - 1; ++i )
+               fn( n++, int( p ), i, j, 0 );
+   }
+   for ( int c = 0; c < m.nC(); ++c )
+      for ( int z = 1; z <= N - 3; ++z )
+         for ( int y = 1; y + z <= N - 2; ++y )
+            for ( int x = 1; x + y + z <= N - 1; ++x )
+               fn( n++, -1 - c, x, y, z );
+}
+
+int ensureDotScratch( size_t n )
+{
+   if ( g_dot.cap >= n )
+      return HYG_OK;
+   (void) hipFree( g_dot.partials );
+   g_dot.partials = nullptr;
+   HYG_HIP( hipMalloc( &g_dot.partials, n * sizeof( double ) ) );
+   g_dot.cap = n;
+   if ( !g_dot.result )
+      HYG_HIP( hipMalloc( &g_dot.result, sizeof( double ) ) );
+   if ( !g_dot.host )
+      HYG_HIP( hipHostMalloc( (void**) &g_dot.host, sizeof( double ), hipHostMallocDefault ) );
+   return HYG_OK;
+}
+
+int dotImpl( hyg_func_s* a, hyg_func_s* b, int L, int flag, double* out, hipStream_t st )
+{
+   MeshCtx&   mc = a->mesh->ctx;
+   LevelGeom& g  = mc.geom( L );
+   HYG_TRY( ensureDotScratch( size_t( dotPartialsNeeded( mc, g ) ) ) );
+   launchDot( mc, g, L, a->dtype, a->data[L], b->data[L], flag, g_dot.partials, g_dot.result, st );
+   HYG_TRY( launched( "dot" ) );
+   HYG_HIP( hipMemcpyAsync( g_dot.host, g_dot.result, sizeof( double ), hipMemcpyDeviceToHost, st ) );
+   HYG_HIP( hipStreamSynchronize( st ) );
+   *out = *g_dot.host;
+   return HYG_OK;
+}
+
+int lincombImpl( hyg_func_s* dst, int n, const double* s, hyg_func_s* const* src, bool acc, double value, int L, int flag,
+                 hipStream_t st )
+{
+   MeshCtx&    mc = dst->mesh->ctx;
+   const void* p[3] = { nullptr, nullptr, nullptr };
+   for ( int k = 0; k < n; ++k )
+      p[k] = src[k]->data[L];
+   launchLincomb( mc, mc.geom( L ), L, dst->dtype, dst->data[L], n, s, p, acc, value, flag, st );
+   return launched( "lincomb" );
+}
+
+int stencilImpl( hyg_op_s* op, int L, int dtype, const StencilCall& c, hipStream_t st )
+{
+   MeshCtx& mc = op->mesh->ctx;
+   launchStencil( mc, mc.geom( L ), op->levels[L], L, dtype, c, st );
+   return launched( "stencil" );
+}
+
+} // namespace
+
+// ================================================================== GMG object
+struct hyg_gmg_s
+{
+   hyg_op_s*             op;
+   int                   dtype;
+   int                   minL, maxL, pre, post, inc, smoother, chebOrder, cgMaxIter;
+   double                relax, chebLo, chebHi, cgTol;
+   double                cheb[5];
+   hyg_func_s*           tmp = nullptr, *jt = nullptr, *t1 = nullptr, *t2 = nullptr, *cy = nullptr;
+   hyg_func_s*           cgp = nullptr, *cgz = nullptr, *cgap = nullptr, *cgr = nullptr;
+   hyg_func_s*           x32 = nullptr, *b32 = nullptr, *r64 = nullptr, *t64 = nullptr;
+   ~hyg_gmg_s()
+   {
+      for ( hyg_func_s* f : { tmp, jt, t1, t2, cy, cgp, cgz, cgap, cgr, x32, b32, r64, t64 } )
+         destroyFunc( f );
+   }
+};
+
+namespace {
+
+const int kGmgFlag = HYG_INNER | HYG_NEUMANN;
+
+int smoothImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, hipStream_t st )
+{
+   StencilCall c;
+   c.flag = kGmgFlag;
+   if ( g->smoother == 0 )
+   {
+      // WeightedJacobiSmoother::solve: tmp := x (All); smooth_jac(x, b, tmp)
+      launchCopy( levelValues( x, L ), g->dtype, g->jt->data[L], x->data[L], st );
+      HYG_TRY( launched( "copy" ) );
+      c.mode = MODE_JACOBI;
+      c.x    = g->jt->data[L];
+      c.in1  = b->data[L];
+      c.out0 = x->data[L];
+      c.a    = g->relax;
+      return stencilImpl( g->op, L, g->dtype, c, st );
+   }
+   // ChebyshevSmoother::solve, fused: t0 = D^-1 (b - A x), y = x + c0 t0; t_k = D^-1 A t_{k-1}, y += c_k t_k; x = y
+   const int K = g->chebOrder;
+   c.mode      = MODE_CHEB0;
+   c.x         = x->data[L];
+   c.in1       = b->data[L];
+   c.out0      = g->t1->data[L];
+   c.out1      = g->cy->data[L];
+   c.a         = g->cheb[0];
+   HYG_TRY( stencilImpl( g->op, L, g->dtype, c, st ) );
+   hyg_func_s* tin  = g->t1;
+   hyg_func_s* tout = g->t2;
+   for ( int k = 1; k < K; ++k )
+   {
+      c.mode = MODE_CHEBK;
+      c.x    = tin->data[L];
+      c.in1  = g->cy->data[L];
+      c.out0 = tout->data[L];
+      c.out1 = k == K - 1 ? x->data[L] : g->cy->data[L];
+      c.a    = g->cheb[k];
+      HYG_TRY( stencilImpl( g->op, L, g->dtype, c, st ) );
+      std::swap( tin, tout );
+   }
+   if ( K == 1 )
+   {
+      const double one = 1.0;
+      hyg_func_s*  src = g->cy;
+      return lincombImpl( x, 1, &one, &src, false, 0.0, L, kGmgFlag, st );
+   }
+   return HYG_OK;
+}
+
+// CGSolver::solve with the identity preconditioner (z = r)
+int cgImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, hipStream_t st )
+{
+   const int   flag = HYG_INNER | HYG_NEUMANN | HYG_FREESLIP;
+   StencilCall c;
+   c.flag = flag;
+   c.mode = MODE_RESIDUAL;
+   c.x    = x->data[L];
+   c.in1  = b->data[L];
+   c.out0 = g->cgr->data[L]; // r = b - A x
+   HYG_TRY( stencilImpl( g->op, L, g->dtype, c, st ) );
+   const double one = 1.0;
+   hyg_func_s*  r   = g->cgr;
+   HYG_TRY( lincombImpl( g->cgp, 1, &one, &r, false, 0.0, L, flag, st ) ); // p = z = r
+   double prsold = 0.0;
+   HYG_TRY( dotImpl( g->cgr, g->cgr, L, flag, &prsold, st ) );
+   if ( std::sqrt( prsold ) < g->cgTol )
+      return HYG_OK;
+   for ( int it = 0; it < g->cgMaxIter; ++it )
+   {
+      c.mode = MODE_APPLY;
+      c.x    = g->cgp->data[L];
+      c.out0 = g->cgap->data[L];
+      HYG_TRY( stencilImpl( g->op, L, g->dtype, c, st ) );
+      double pAp = 0.0;
+      HYG_TRY( dotImpl( g->cgp, g->cgap, L, flag, &pAp, st ) );
+      const double alpha = prsold / pAp;
+      hyg_func_s*  p     = g->cgp;
+      hyg_func_s*  ap    = g->cgap;
+      HYG_TRY( lincombImpl( x, 1, &alpha, &p, true, 0.0, L, flag, st ) );
+      const double malpha = -alpha;
+      HYG_TRY( lincombImpl( g->cgr, 1, &malpha, &ap, true, 0.0, L, flag, st ) );
+      double rsnew = 0.0;
+      HYG_TRY( dotImpl( g->cgr, g->cgr, L, flag, &rsnew, st ) );
+      if ( std::sqrt( rsnew ) < g->cgTol )
+         break;
+      const double beta = rsnew / prsold;
+      // p = z + beta p with z = r
+      const double sc[2]  = { 1.0, beta };
+      hyg_func_s*  srcs[2] = { g->cgr, g->cgp };
+      HYG_TRY( lincombImpl( g->cgp, 2, sc, srcs, false, 0.0, L, flag, st ) );
+      prsold = rsnew;
+   }
+   return HYG_OK;
+}
+
+// GeometricMultigridSolver::solveRecursively, V-cycle
+int vcycleImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, int invoked, hipStream_t st )
+{
+   if ( L == g->minL )
+      return cgImpl( g, x, b, L, st );
+   MeshCtx&  mc   = g->op->mesh->ctx;
+   const int preS = g->pre + g->inc * ( invoked - L );
+   for ( int i = 0; i < preS; ++i )
+      HYG_TRY( smoothImpl( g, x, b, L, st ) );
+   StencilCall c;
+   c.flag = kGmgFlag;
+   c.mode = MODE_RESIDUAL;
+   c.x    = x->data[L];
+   c.in1  = b->data[L];
+   c.out0 = g->tmp->data[L];
+   HYG_TRY( stencilImpl( g->op, L, g->dtype, c, st ) );
+   launchRestrict( mc, mc.geom( L - 1 ), L - 1, g->dtype, g->tmp->data[L - 1], g->tmp->data[L], kGmgFlag, st );
+   HYG_TRY( launched( "restrict" ) );
+   const double one = 1.0;
+   hyg_func_s*  t   = g->tmp;
+   HYG_TRY( lincombImpl( b, 1, &one, &t, false, 0.0, L - 1, kGmgFlag, st ) );
+   HYG_TRY( lincombImpl( x, 0, nullptr, nullptr, false, 0.0, L - 1, HYG_ALL, st ) );
+   HYG_TRY( vcycleImpl( g, x, b, L - 1, invoked, st ) );
+   launchProlongate( mc, mc.geom( L ), L, g->dtype, x->data[L], x->data[L - 1], kGmgFlag, true, st );
+   HYG_TRY( launched( "prolongate" ) );
+   const int postS = g->post + g->inc * ( invoked - L );
+   for ( int i = 0; i < postS; ++i )
+      HYG_TRY( smoothImpl( g, x, b, L, st ) );
+   return HYG_OK;
+}
+
+int residualNorm( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, hyg_func_s* r, int L, double* out, hipStream_t st )
+{
+   StencilCall c;
+   c.flag = kGmgFlag;
+   c.mode = MODE_RESIDUAL;
+   c.x    = x->data[L];
+   c.in1  = b->data[L];
+   c.out0 = r->data[L];
+   HYG_TRY( stencilImpl( g->op, L, r->dtype, c, st ) );
+   double d = 0.0;
+   HYG_TRY( dotImpl( r, r, L, kGmgFlag, &d, st ) );
+   *out = std::sqrt( d );
+   if ( !std::isfinite( *out ) )
+      return fail( HYG_ERR_NUMERIC, "non-finite residual norm" );
+   return HYG_OK;
+}
+
+int checkGmgArgs( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L )
+{
+   if ( !g )
+      return fail( HYG_ERR_ARG, "null solver handle" );
+   HYG_TRY( checkPair( x, b, L ) );
+   if ( x->dtype != g->dtype )
+      return fail( HYG_ERR_ARG, "function value type differs from the solver's" );
+   if ( L != g->maxL || x->minL > g->minL || b->minL > g->minL )
+      return fail( HYG_ERR_ARG, "solve level must be the solver's max level and functions must cover its levels" );
+   if ( x->mesh != g->op->mesh )
+      return fail( HYG_ERR_ARG, "functions and solver live on different meshes" );
+   return HYG_OK;
+}
+
+} // namespace
+
+// ================================================================== C ABI
+extern "C" {
+
+const char* hyg_last_error( void ) { return g_err.c_str(); }
+
+int hyg_init( int device )
+{
+   int n = 0;
+   HYG_HIP( hipGetDeviceCount( &n ) );
+   if ( device < 0 || device >= n )
+      return fail( HYG_ERR_ARG, "device index out of range" );
+   HYG_HIP( hipSetDevice( device ) );
+   g_device = device;
+   return HYG_OK;
+}
+
+int hyg_mesh_create_from_msh( const char* path, hyg_mesh* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !path || !out )
+      return fail( HYG_ERR_ARG, "null argument" );
+   std::string err;
+   auto        m = readGmsh( path, err );
+   if ( !m )
+      return fail( HYG_ERR_IO, err );
+   auto* h = new hyg_mesh_s;
+   h->ctx.mesh   = std::move( m );
+   h->ctx.device = g_device;
+   try
+   {
+      h->ctx.buildTopology();
+   } catch ( const std::exception& e )
+   {
+      delete h;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   *out = h;
+   return HYG_OK;
+}
+
+int hyg_mesh_destroy( hyg_mesh mesh )
+{
+   delete mesh;
+   return HYG_OK;
+}
+
+int hyg_mesh_info( hyg_mesh mesh, int64_t* out )
+{
+   if ( !mesh || !out )
+      return fail( HYG_ERR_ARG, "null argument" );
+   const Mesh& m = *mesh->ctx.mesh;
+   out[0]        = m.nV();
+   out[1]        = m.nE();
+   out[2]        = m.nF();
+   out[3]        = m.nC();
+   out[4]        = std::count( m.vB.begin(), m.vB.end(), 1 );
+   out[5]        = std::count( m.eB.begin(), m.eB.end(), 1 );
+   out[6]        = std::count( m.fB.begin(), m.fB.end(), 1 );
+   return HYG_OK;
+}
+
+int hyg_mesh_num_dofs( hyg_mesh mesh, int level, int64_t* out )
+{
+   if ( !mesh || !out || level < 0 || level > 10 )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   *out = numDofs( *mesh->ctx.mesh, level );
+   return HYG_OK;
+}
+
+int hyg_mesh_dof_coords( hyg_mesh mesh, int level, double* xyz, uint8_t* boundary )
+{
+   if ( !mesh || !xyz || level < 0 || level > 10 )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   const MeshCtx& mc = mesh->ctx;
+   const Mesh&    m  = *mc.mesh;
+   const int      N  = 1 << level;
+   const double   h  = 1.0 / double( N );
+   forEachCanonical( mc, level, [&]( i64 n, int p, int i, int j, int k ) {
+      double r[3];
+      int    bnd = 0;
+      if ( p >= 0 )
+      {
+         const LowerPrim& pr = mc.primsHost[p];
+         int              v[3];
+         if ( pr.kind == 0 )
+            v[0] = v[1] = v[2] = p;
+         else if ( pr.kind == 1 )
+         {
+            const auto& e = m.E[p - m.nV()];
+            v[0] = e[0], v[1] = e[1], v[2] = e[0];
+         }
+         else
+         {
+            const auto& f = m.F[p - m.nV() - m.nE()];
+            v[0] = f[0], v[1] = f[1], v[2] = f[2];
+         }
+         for ( int a = 0; a < 3; ++a )
+            r[a] = m.X[v[0]][a] + ( m.X[v[1]][a] - m.X[v[0]][a] ) * h * i + ( m.X[v[2]][a] - m.X[v[0]][a] ) * h * j;
+         bnd = pr.bc == HYG_DIRICHLET;
+      }
+      else
+      {
+         const auto& cv = m.Cv[-1 - p];
+         for ( int a = 0; a < 3; ++a )
+            r[a] = m.X[cv[0]][a] + ( m.X[cv[1]][a] - m.X[cv[0]][a] ) * h * i + ( m.X[cv[2]][a] - m.X[cv[0]][a] ) * h * j +
+                   ( m.X[cv[3]][a] - m.X[cv[0]][a] ) * h * k;
+      }
+      xyz[3 * n] = r[0], xyz[3 * n + 1] = r[1], xyz[3 * n + 2] = r[2];
+      if ( boundary )
+         boundary[n] = uint8_t( bnd );
+   } );
+   return HYG_OK;
+}
+
+int hyg_func_create( hyg_mesh mesh, int min_level, int max_level, int dtype, hyg_func* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh || !out )
+      return fail( HYG_ERR_ARG, "null argument" );
+   if ( min_level < 0 || max_level < min_level || max_level > 10 )
+      return fail( HYG_ERR_ARG, "bad level range" );
+   if ( dtype != HYG_F64 && dtype != HYG_F32 )
+      return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
+   return createFunc( mesh, min_level, max_level, dtype, out );
+}
+
+int hyg_func_destroy( hyg_func f )
+{
+   destroyFunc( f );
+   return HYG_OK;
+}
+
+int hyg_func_upload( hyg_func f, int level, const double* canonical, void* stream )
+{
+   HYG_TRY( checkLevel( f, level ) );
+   if ( !canonical )
+      return fail( HYG_ERR_ARG, "null data" );
+   const MeshCtx&        mc = f->mesh->ctx;
+   const int             N = 1 << level, w = N + 1;
+   const i64             stride = cellStride( level );
+   std::vector< double > h( size_t( levelValues( f, level ) ), 0.0 );
+   forEachCanonical( mc, level, [&]( i64 n, int p, int i, int j, int k ) {
+      if ( p >= 0 )
+      {
+         const LowerPrim& pr = mc.primsHost[p];
+         for ( int e = 0; e < pr.nCells; ++e )
+         {
+            const LowerEntry& en = mc.entriesHost[pr.entry0 + e];
+            int               qx, qy, qz;
+            primEntriesPos( en, N, i, j, qx, qy, qz );
+            h[size_t( en.cell * stride + cidx( w, qx, qy, qz ) )] = canonical[n];
+         }
+      }
+      else
+         h[size_t( ( -1 - p ) * stride + cidx( w, i, j, k ) )] = canonical[n];
+   } );
+   if ( f->dtype == HYG_F64 )
+   {
+      HYG_HIP( hipMemcpyAsync( f->data[level], h.data(), h.size() * 8, hipMemcpyHostToDevice, S( stream ) ) );
+      HYG_HIP( hipStreamSynchronize( S( stream ) ) );
+   }
+   else
+   {
+      std::vector< float > hf( h.begin(), h.end() );
+      HYG_HIP( hipMemcpyAsync( f->data[level], hf.data(), hf.size() * 4, hipMemcpyHostToDevice, S( stream ) ) );
+      HYG_HIP( hipStreamSynchronize( S( stream ) ) );
+   }
+   return HYG_OK;
+}
+
+int hyg_func_download( hyg_func f, int level, double* canonical, void* stream )
+{
+   HYG_TRY( checkLevel( f, level ) );
+   if ( !canonical )
+      return fail( HYG_ERR_ARG, "null data" );
+   const MeshCtx&        mc = f->mesh->ctx;
+   const int             N = 1 << level, w = N + 1;
+   const i64             stride = cellStride( level );
+   const size_t          nv     = size_t( levelValues( f, level ) );
+   std::vector< double > h( nv );
+   if ( f->dtype == HYG_F64 )
+   {
+      HYG_HIP( hipMemcpyAsync( h.data(), f->data[level], nv * 8, hipMemcpyDeviceToHost, S( stream ) ) );
+      HYG_HIP( hipStreamSynchronize( S( stream ) ) );
+   }
+   else
+   {
+      std::vector< float > hf( nv );
+      HYG_HIP( hipMemcpyAsync( hf.data(), f->data[level], nv * 4, hipMemcpyDeviceToHost, S( stream ) ) );
+      HYG_HIP( hipStreamSynchronize( S( stream ) ) );
+      std::copy( hf.begin(), hf.end(), h.begin() );
+   }
+   forEachCanonical( mc, level, [&]( i64 n, int p, int i, int j, int k ) {
+      if ( p >= 0 )
+      {
+         const LowerPrim&  pr = mc.primsHost[p];
+         const LowerEntry& en = mc.entriesHost[pr.entry0];
+         int               qx, qy, qz;
+         primEntriesPos( en, N, i, j, qx, qy, qz );
+         canonical[n] = h[size_t( en.cell * stride + cidx( w, qx, qy, qz ) )];
+      }
+      else
+         canonical[n] = h[size_t( ( -1 - p ) * stride + cidx( w, i, j, k ) )];
+   } );
+   return HYG_OK;
+}
+
+int hyg_interpolate_const( hyg_func f, double value, int level, int flag, void* stream )
+{
+   HYG_TRY( checkLevel( f, level ) );
+   return lincombImpl( f, 0, nullptr, nullptr, false, value, level, flag, S( stream ) );
+}
+
+int hyg_assign( hyg_func dst, int n, const double* scalars, const hyg_func* srcs, int level, int flag, void* stream )
+{
+   HYG_TRY( checkLevel( dst, level ) );
+   if ( n < 1 || n > 3 || !scalars || !srcs )
+      return fail( HYG_ERR_ARG, "assign takes 1 to 3 sources" );
+   for ( int k = 0; k < n; ++k )
+      HYG_TRY( checkPair( dst, srcs[k], level ) );
+   return lincombImpl( dst, n, scalars, srcs, false, 0.0, level, flag, S( stream ) );
+}
+
+int hyg_add( hyg_func dst, int n, const double* scalars, const hyg_func* srcs, int level, int flag, void* stream )
+{
+   HYG_TRY( checkLevel( dst, level ) );
+   if ( n < 1 || n > 3 || !scalars || !srcs )
+      return fail( HYG_ERR_ARG, "add takes 1 to 3 sources" );
+   for ( int k = 0; k < n; ++k )
+      HYG_TRY( checkPair( dst, srcs[k], level ) );
+   return lincombImpl( dst, n, scalars, srcs, true, 0.0, level, flag, S( stream ) );
+}
+
+int hyg_mult_elementwise( hyg_func dst, hyg_func a, hyg_func b, int level, int flag, void* stream )
+{
+   HYG_TRY( checkPair( dst, a, level ) );
+   HYG_TRY( checkPair( dst, b, level ) );
+   MeshCtx& mc = dst->mesh->ctx;
+   launchMult( mc, mc.geom( level ), level, dst->dtype, dst->data[level], a->data[level], b->data[level], flag, S( stream ) );
+   return launched( "multElementwise" );
+}
+
+int hyg_dot( hyg_func a, hyg_func b, int level, int flag, double* out, void* stream )
+{
+   HYG_TRY( checkPair( a, b, level ) );
+   if ( !out )
+      return fail( HYG_ERR_ARG, "null output" );
+   return dotImpl( a, b, level, flag, out, S( stream ) );
+}
+
+int hyg_convert( hyg_func dst, hyg_func src, int level, void* stream )
+{
+   HYG_TRY( checkLevel( dst, level ) );
+   HYG_TRY( checkLevel( src, level ) );
+   if ( dst->mesh != src->mesh )
+      return fail( HYG_ERR_ARG, "functions live on different meshes" );
+   launchConvert( levelValues( dst, level ), dst->dtype, dst->data[level], src->dtype, src->data[level], S( stream ) );
+   return launched( "convert" );
+}
+
+int hyg_op_create_p1_const( hyg_mesh mesh, int min_level, int max_level, hyg_op* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh || !out || min_level < 0 || max_level < min_level || max_level > 10 )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   auto* op = new hyg_op_s;
+   op->mesh = mesh;
+   op->minL = min_level;
+   op->maxL = max_level;
+   try
+   {
+      assembleP1Const( *op );
+      for ( int L = min_level; L <= max_level; ++L )
+         mesh->ctx.geom( L );
+   } catch ( const std::exception& e )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   for ( int L = min_level; L <= max_level; ++L )
+   {
+      const auto& ol = op->levels[L];
+      for ( int c = 0; L >= 2 && c < mesh->ctx.mesh->nC(); ++c )
+         if ( !std::isfinite( ol.cellStHost[16 * c + 15] ) )
+         {
+            delete op;
+            return fail( HYG_ERR_NUMERIC, "zero diagonal in cell stencil (degenerate macro-cell)" );
+         }
+   }
+   *out = op;
+   return HYG_OK;
+}
+
+int hyg_op_destroy( hyg_op op )
+{
+   delete op;
+   return HYG_OK;
+}
+
+int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 )
+{
+   HYG_TRY( checkOpLevel( op, level ) );
+   if ( cell < 0 || cell >= op->mesh->ctx.mesh->nC() || !out15 )
+      return fail( HYG_ERR_ARG, "bad cell index" );
+   std::memcpy( out15, &op->levels[level].cellStHost[16 * cell], 15 * sizeof( double ) );
+   return HYG_OK;
+}
+
+int hyg_apply( hyg_op op, hyg_func src, hyg_func dst, int level, int flag, int update, void* stream )
+{
+   HYG_TRY( checkOpLevel( op, level ) );
+   HYG_TRY( checkPair( src, dst, level ) );
+   if ( src == dst )
+      return fail( HYG_ERR_ARG, "apply requires src != dst" );
+   if ( src->mesh != op->mesh )
+      return fail( HYG_ERR_ARG, "function and operator live on different meshes" );
+   if ( update != HYG_REPLACE && update != HYG_ADD )
+      return fail( HYG_ERR_ARG, "update must be HYG_REPLACE or HYG_ADD" );
+   StencilCall c;
+   c.mode = update == HYG_ADD ? MODE_APPLY_ADD : MODE_APPLY;
+   c.flag = flag;
+   c.x    = src->data[level];
+   c.out0 = dst->data[level];
+   return stencilImpl( op, level, src->dtype, c, S( stream ) );
+}
+
+int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double relax, int level, int flag, void* stream )
+{
+   HYG_TRY( checkOpLevel( op, level ) );
+   HYG_TRY( checkPair( src, dst, level ) );
+   HYG_TRY( checkPair( src, rhs, level ) );
+   if ( src == dst || rhs == dst )
+      return fail( HYG_ERR_ARG, "smooth_jac requires dst different from src and rhs" );
+   StencilCall c;
+   c.mode = MODE_JACOBI;
+   c.flag = flag;
+   c.x    = src->data[level];
+   c.in1  = rhs->data[level];
+   c.out0 = dst->data[level];
+   c.a    = relax;
+   return stencilImpl( op, level, src->dtype, c, S( stream ) );
+}
+
+int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream )
+{
+   HYG_TRY( checkOpLevel( op, level ) );
+   HYG_TRY( checkLevel( dst, level ) );
+   MeshCtx& mc = op->mesh->ctx;
+   launchInvDiag( mc, mc.geom( level ), op->levels[level], level, dst->dtype, dst->data[level], S( stream ) );
+   return launched( "inverse diagonal" );
+}
+
+int hyg_residual( hyg_op op, hyg_func r, hyg_func b, hyg_func x, int level, int flag, void* stream )
+{
+   HYG_TRY( checkOpLevel( op, level ) );
+   HYG_TRY( checkPair( r, b, level ) );
+   HYG_TRY( checkPair( r, x, level ) );
+   if ( r == x )
+      return fail( HYG_ERR_ARG, "residual requires r != x" );
+   StencilCall c;
+   c.mode = MODE_RESIDUAL;
+   c.flag = flag;
+   c.x    = x->data[level];
+   c.in1  = b->data[level];
+   c.out0 = r->data[level];
+   return stencilImpl( op, level, x->dtype, c, S( stream ) );
+}
+
+int hyg_restrict( hyg_func f, int source_level, int flag, void* stream )
+{
+   HYG_TRY( checkLevel( f, source_level ) );
+   HYG_TRY( checkLevel( f, source_level - 1 ) );
+   MeshCtx& mc = f->mesh->ctx;
+   launchRestrict( mc, mc.geom( source_level - 1 ), source_level - 1, f->dtype, f->data[source_level - 1], f->data[source_level],
+                   flag, S( stream ) );
+   return launched( "restrict" );
+}
+
+int hyg_prolongate( hyg_func f, int source_level, int flag, int update, void* stream )
+{
+   HYG_TRY( checkLevel( f, source_level ) );
+   HYG_TRY( checkLevel( f, source_level + 1 ) );
+   MeshCtx& mc = f->mesh->ctx;
+   launchProlongate( mc, mc.geom( source_level + 1 ), source_level + 1, f->dtype, f->data[source_level + 1], f->data[source_level],
+                     flag, update == HYG_ADD, S( stream ) );
+   return launched( "prolongate" );
+}
+
+int hyg_cheb_coefficients( int order, double lower, double upper, double* c )
+{
+   if ( !c || order < 1 || order > 5 )
+      return fail( HYG_ERR_ARG, "Chebyshev order must be 1..5" );
+   // coefficients of q with x <- x + q(D^-1 A) D^-1 r, 1 - t q(t) = T_k((theta - t)/delta) / T_k(theta/delta)
+   const double th = 0.5 * ( upper + lower ), de = 0.5 * ( upper - lower );
+   const double th2 = th * th, de2 = de * de;
+   switch ( order )
+   {
+   case 1:
+      c[0] = 1.0 / th;
+      break;
+   case 2: {
+      const double q = 1.0 / ( de2 - 2.0 * th2 );
+      c[0]           = -4.0 * th * q;
+      c[1]           = 2.0 * q;
+      break;
+   }
+   case 3: {
+      const double q = 1.0 / ( th * ( 3.0 * de2 - 4.0 * th2 ) );
+      c[0]           = q * ( 3.0 * de2 - 12.0 * th2 );
+      c[1]           = 12.0 / ( 3.0 * de2 - 4.0 * th2 );
+      c[2]           = -4.0 * q;
+      break;
+   }
+   case 4: {
+      const double q = 1.0 / ( de2 * de2 + 8.0 * th2 * th2 - 8.0 * de2 * th2 );
+      c[0]           = q * ( 32.0 * th2 * th - 16.0 * th * de2 );
+      c[1]           = q * ( 8.0 * de2 - 48.0 * th2 );
+      c[2]           = 32.0 * th * q;
+      c[3]           = -8.0 * q;
+      break;
+   }
+   default: {
+      const double d4 = de2 * de2, t4 = th2 * th2;
+      const double q5 = 1.0 / ( th * ( 16.0 * t4 - 20.0 * de2 * th2 + 5.0 * d4 ) );
+      const double q4 = 1.0 / ( 5.0 * d4 + 16.0 * t4 - 20.0 * de2 * th2 );
+      c[0]            = q5 * ( 5.0 * d4 + 80.0 * t4 - 60.0 * de2 * th2 );
+      c[1]            = q4 * ( 60.0 * de2 - 160.0 * th2 );
+      c[2]            = q5 * ( 160.0 * th2 - 20.0 * de2 );
+      c[3]            = -80.0 * q4;
+      c[4]            = 16.0 * q5;
+      break;
+   }
+   }
+   return HYG_OK;
+}
+
+int hyg_power_iteration( hyg_op op, hyg_func x, hyg_func tmp, int iterations, int level, double* radius, void* stream )
+{
+   HYG_TRY( checkOpLevel( op, level ) );
+   HYG_TRY( checkPair( x, tmp, level ) );
+   if ( x == tmp || !radius || iterations < 1 )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   hipStream_t st = S( stream );
+   FuncGuard   dinv;
+   HYG_TRY( createFunc( x->mesh, level, level, x->dtype, &dinv.f ) );
+   HYG_TRY( hyg_inverse_diagonal( op, dinv.f, level, stream ) );
+   auto invDiagApply = [&]( hyg_func_s* s, hyg_func_s* d ) -> int {
+      HYG_TRY( hyg_apply( op, s, d, level, HYG_ALL, HYG_REPLACE, stream ) );
+      return hyg_mult_elementwise( d, dinv.f, d, level, HYG_ALL, stream );
+   };
+   double nrm = 0.0;
+   HYG_TRY( dotImpl( x, x, level, HYG_ALL, &nrm, st ) );
+   double s = 1.0 / std::sqrt( nrm );
+   HYG_TRY( lincombImpl( x, 1, &s, &x, false, 0.0, level, HYG_ALL, st ) );
+   HYG_TRY( invDiagApply( x, tmp ) );
+   double r = 0.0;
+   for ( int it = 1; it <= iterations; ++it )
+   {
+      HYG_TRY( dotImpl( tmp, tmp, level, HYG_ALL, &nrm, st ) );
+      s = 1.0 / std::sqrt( nrm );
+      HYG_TRY( lincombImpl( x, 1, &s, &tmp, false, 0.0, level, HYG_ALL, st ) );
+      HYG_TRY( invDiagApply( x, tmp ) );
+      HYG_TRY( dotImpl( x, tmp, level, HYG_ALL, &r, st ) );
+   }
+   *radius = r;
+   if ( !std::isfinite( r ) )
+      return fail( HYG_ERR_NUMERIC, "power iteration produced a non-finite value" );
+   return HYG_OK;
+}
+
+int hyg_gmg_create( hyg_op op, int dtype, const int* ip, const double* dp, hyg_gmg* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !op || !ip || !dp || !out )
+      return fail( HYG_ERR_ARG, "null argument" );
+   if ( dtype != HYG_F64 && dtype != HYG_F32 )
+      return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
+   if ( ip[0] < op->minL || ip[1] > op->maxL || ip[0] > ip[1] )
+      return fail( HYG_ERR_ARG, "solver levels outside the operator's range" );
+   if ( ip[5] != 0 && ip[5] != 1 )
+      return fail( HYG_ERR_ARG, "smoother must be 0 (weighted Jacobi) or 1 (Chebyshev)" );
+   if ( ip[5] == 1 && ( ip[6] < 1 || ip[6] > 5 ) )
+      return fail( HYG_ERR_ARG, "Chebyshev order must be 1..5" );
+   auto* g      = new hyg_gmg_s;
+   g->op        = op;
+   g->dtype     = dtype;
+   g->minL      = ip[0];
+   g->maxL      = ip[1];
+   g->pre       = ip[2];
+   g->post      = ip[3];
+   g->inc       = ip[4];
+   g->smoother  = ip[5];
+   g->chebOrder = ip[6];
+   g->cgMaxIter = ip[7];
+   g->relax     = dp[0];
+   g->chebLo    = dp[1];
+   g->chebHi    = dp[2];
+   g->cgTol     = dp[3];
+   if ( g->smoother == 1 )
+      hyg_cheb_coefficients( g->chebOrder, g->chebLo, g->chebHi, g->cheb );
+   hyg_mesh_s* m  = op->mesh;
+   int         rc = HYG_OK;
+   for ( hyg_func_s** f : { &g->tmp, &g->jt, &g->t1, &g->t2, &g->cy, &g->cgp, &g->cgz, &g->cgap, &g->cgr } )
+      if ( rc == HYG_OK )
+         rc = createFunc( m, g->minL, g->maxL, dtype, f );
+   if ( rc != HYG_OK )
+   {
+      delete g;
+      return rc;
+   }
+   *out = g;
+   return HYG_OK;
+}
+
+int hyg_gmg_destroy( hyg_gmg g )
+{
+   delete g;
+   return HYG_OK;
+}
+
+int hyg_gmg_vcycle( hyg_gmg g, hyg_func x, hyg_func b, int level, void* stream )
+{
+   HYG_TRY( checkGmgArgs( g, x, b, level ) );
+   HYG_TRY( vcycleImpl( g, x, b, level, level, S( stream ) ) );
+   return HYG_OK;
+}
+
+int hyg_gmg_solve( hyg_gmg g, hyg_func x, hyg_func b, int level, int cycles, double* res, void* stream )
+{
+   HYG_TRY( checkGmgArgs( g, x, b, level ) );
+   if ( cycles < 0 || !res )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   hipStream_t st = S( stream );
+   HYG_TRY( residualNorm( g, x, b, g->tmp, level, &res[0], st ) );
+   for ( int k = 0; k < cycles; ++k )
+   {
+      HYG_TRY( vcycleImpl( g, x, b, level, level, st ) );
+      HYG_TRY( residualNorm( g, x, b, g->tmp, level, &res[k + 1], st ) );
+   }
+   return HYG_OK;
+}
+
+int hyg_ir_solve( hyg_op op64, hyg_gmg g32, hyg_func x, hyg_func b, int level, int inner, int max_outer, double rel_tol,
+                  double* res, int* steps, void* stream )
+{
+   HYG_TRY( checkOpLevel( op64, level ) );
+   if ( !g32 || g32->dtype != HYG_F32 || g32->maxL != level )
+      return fail( HYG_ERR_ARG, "defect correction needs an FP32 solver whose max level is the solve level" );
+   HYG_TRY( checkPair( x, b, level ) );
+   if ( x->dtype != HYG_F64 )
+      return fail( HYG_ERR_ARG, "defect correction works on FP64 x and b" );
+   if ( !res || !steps || inner < 1 || max_outer < 0 )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   hipStream_t st = S( stream );
+   hyg_mesh_s* m  = op64->mesh;
+   if ( !g32->x32 )
+   {
+      HYG_TRY( createFunc( m, g32->minL, g32->maxL, HYG_F32, &g32->x32 ) );
+      HYG_TRY( createFunc( m, g32->minL, g32->maxL, HYG_F32, &g32->b32 ) );
+      HYG_TRY( createFunc( m, level, level, HYG_F64, &g32->r64 ) );
+      HYG_TRY( createFunc( m, level, level, HYG_F64, &g32->t64 ) );
+   }
+   // residual in FP64 (the FP64 operator shares the FP32 one's mesh)
+   hyg_gmg_s dummy{};
+   dummy.op    = op64;
+   dummy.dtype = HYG_F64;
+   double r0   = 0.0;
+   HYG_TRY( residualNorm( &dummy, x, b, g32->r64, level, &r0, st ) );
+   res[0]   = r0;
+   *steps   = 0;
+   for ( int it = 0; it < max_outer; ++it )
+   {
+      if ( r0 == 0.0 || res[it] / r0 <= rel_tol )
+         break;
+      launchConvert( levelValues( x, level ), HYG_F32, g32->b32->data[level], HYG_F64, g32->r64->data[level], st );
+      HYG_TRY( launched( "convert" ) );
+      HYG_TRY( lincombImpl( g32->x32, 0, nullptr, nullptr, false, 0.0, level, HYG_ALL, st ) );
+      for ( int k = 0; k < inner; ++k )
+         HYG_TRY( vcycleImpl( g32, g32->x32, g32->b32, level, level, st ) );
+      launchConvert( levelValues( x, level ), HYG_F64, g32->t64->data[level], HYG_F32, g32->x32->data[level], st );
+      HYG_TRY( launched( "convert" ) );
+      const double one = 1.0;
+      hyg_func_s*  t   = g32->t64;
+      HYG_TRY( lincombImpl( x, 1, &one, &t, true, 0.0, level, kGmgFlag, st ) );
+      HYG_TRY( residualNorm( &dummy, x, b, g32->r64, level, &res[it + 1], st ) );
+      *steps = it + 1;
+   }
+   return HYG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,221 @@
+// Internal declarations of the MI355X P1 multigrid library (host + device).
+//
+// Data layout in HBM. A function at level L stores, for every macro-cell, one array of
+// S(w) = w(w+1)(w+2)/6 values (w = 2^L + 1) in the reference's tetrahedral linearisation
+// index(x,y,z) = S(w) - S(w-z) + y(w-z+1) - y(y+1)/2 + x
+// (src/hyteg/indexing/MacroCellIndexing.hpp:40-53), padded to a multiple of 64 values so
+// every cell array starts 512-byte aligned. The cell arrays of all macro-cells of a level are
+// contiguous. Values on macro-faces, -edges and -vertices are held by every incident cell
+// (a replica per cell) and every kernel that writes such a value writes all replicas, so a
+// function is always consistent and no halo refresh precedes an operator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace hyg {
+
+using i64 = int64_t;
+
+// ------------------------------------------------------------------ indexing
+__host__ __device__ inline i64 S3( i64 w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
+__host__ __device__ inline i64 T2( i64 w ) { return w * ( w + 1 ) / 2; }
+__host__ __device__ inline int cidx( int w, int x, int y, int z )
+{
+   // all terms fit 32 bits up to level 10
+   return int( S3( w ) - S3( w - z ) ) + y * ( w - z + 1 ) - ( y * ( y + 1 ) ) / 2 + x;
+}
+// start of row y in a plane whose row 0 has W points
+__host__ __device__ inline int rowStart( int W, int y ) { return y * W - ( y * ( y - 1 ) ) / 2; }
+
+inline i64 cellStride( int L )
+{
+   const i64 w = ( i64( 1 ) << L ) + 1;
+   return ( S3( w ) + 63 ) / 64 * 64;
+}
+
+// stencil directions in the reference order C W E N S NW SE TC TW TS TSE BC BN BE BNW
+// (src/hyteg/p1functionspace/VertexDoFIndexing.cpp:484-520)
+constexpr int NDIR = 15;
+__host__ __device__ constexpr int dirX( int d )
+{
+   constexpr int t[NDIR] = { 0, -1, 1, 0, 0, -1, 1, 0, -1, 0, 1, 0, 0, 1, -1 };
+   return t[d];
+}
+__host__ __device__ constexpr int dirY( int d )
+{
+   constexpr int t[NDIR] = { 0, 0, 0, 1, -1, 1, -1, 0, 0, -1, -1, 0, 1, 0, 1 };
+   return t[d];
+}
+__host__ __device__ constexpr int dirZ( int d )
+{
+   constexpr int t[NDIR] = { 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1 };
+   return t[d];
+}
+
+// Local primitive slot of a cell point: 0 interior, 1..4 faces 0..3, 5..10 edges 0..5, 11..14 vertices 0..3.
+// Cell-local faces: f0=(0,1,2) f1=(0,1,3) f2=(0,2,3) f3=(1,2,3); edges e0=(0,1) e1=(0,2) e2=(1,2) e3=(0,3) e4=(1,3) e5=(2,3).
+__host__ __device__ inline int classifySlot( int N, int x, int y, int z )
+{
+   const int b0 = N - x - y - z;
+   const int zm = ( b0 == 0 ) | ( ( x == 0 ) << 1 ) | ( ( y == 0 ) << 2 ) | ( ( z == 0 ) << 3 );
+   // lookup by zero-mask of the barycentric coordinates (b0, x, y, z)
+   // one zero -> face opposite; two zeros -> edge of the two non-zero ones; three zeros -> vertex
+   constexpr signed char lut[16] = {
+       0,      // none zero: interior
+       1 + 3,  // b0=0  -> face 3 (1,2,3)
+       1 + 2,  // x=0   -> face 2 (0,2,3)
+       5 + 5,  // b0,x  -> edge (2,3) = e5
+       1 + 1,  // y=0   -> face 1 (0,1,3)
+       5 + 4,  // b0,y  -> edge (1,3) = e4
+       5 + 3,  // x,y   -> edge (0,3) = e3
+       11 + 3, // b0,x,y -> vertex 3
+       1 + 0,  // z=0   -> face 0 (0,1,2)
+       5 + 2,  // b0,z  -> edge (1,2) = e2
+       5 + 1,  // x,z   -> edge (0,2) = e1
+       11 + 2, // b0,x,z -> vertex 2
+       5 + 0,  // y,z   -> edge (0,1) = e0
+       11 + 1, // b0,y,z -> vertex 1
+       11 + 0, // x,y,z -> vertex 0
+       -1 };
+   return lut[zm];
+}
+
+__host__ __device__ inline bool inCell( int N, int x, int y, int z ) { return x >= 0 && y >= 0 && z >= 0 && x + y + z <= N; }
+
+// ------------------------------------------------------------------ device tables
+// A tile of 256 consecutive points of one z-plane of a cell array.
+struct PlaneTile
+{
+   int z;
+   int s0;
+};
+
+// Lower-dimensional primitive (macro-vertex, -edge or -face); order: vertices, edges, faces.
+struct LowerPrim
+{
+   int kind;   // 0 vertex, 1 edge, 2 face
+   int bc;     // DoFType of the primitive
+   int nCells; // incident macro-cells
+   int entry0; // first LowerEntry
+};
+
+// One (lower-dimensional primitive, incident cell) pair. The primitive's vertices are the cell-local
+// vertices l0, l1, l2 (l1/l2 unused for vertices, l2 for edges); its point (i, j) has cell coordinates
+// N*e(l0) + i*(e(l1) - e(l0)) + j*(e(l2) - e(l0)), e(0) = 0, e(1..3) = unit vectors.
+struct LowerEntry
+{
+   int cell;
+   int l0, l1, l2;
+};
+
+// A run of up to 64 points of one lower-dimensional primitive: face row j (i from i0), edge (i from i0), vertex.
+struct LowerChunk
+{
+   int prim;
+   int j;
+   int i0;
+   int n;
+};
+
+// Per-cell data used by pointwise kernels.
+struct CellInfo
+{
+   uint64_t bcword;    // DoFType (4 bits) of local slots 0..14
+   uint32_t ownmask;   // bit k: this cell is the designated owner of local slot k (bit 0 always)
+   int      prim[15];  // LowerPrim index of local slot k >= 1 (slot 0: -1)
+   double   invNC[16]; // 1 / #cells of the local slot's primitive (slot 0: 1)
+};
+
+// ------------------------------------------------------------------ host objects
+struct Mesh
+{
+   std::vector< std::array< double, 3 > > X;
+   std::vector< std::array< int, 2 > >    E;
+   std::vector< std::array< int, 3 > >    F;
+   std::vector< std::array< int, 4 > >    Cv;
+   std::vector< std::array< int, 4 > >    cF;
+   std::vector< std::array< int, 6 > >    cE;
+   std::vector< std::vector< int > >      vC, eC, fC;
+   std::vector< uint8_t >                 vB, eB, fB;
+   int nV() const { return (int) X.size(); }
+   int nE() const { return (int) E.size(); }
+   int nF() const { return (int) F.size(); }
+   int nC() const { return (int) Cv.size(); }
+};
+
+std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err );
+
+// canonical DoF count, V + E(N-1) + F(N-1)(N-2)/2 + C(N-1)(N-2)(N-3)/6
+i64 numDofs( const Mesh& m, int L );
+
+// Per-level device geometry shared by all functions and operators of a mesh.
+struct LevelGeom
+{
+   int                        L = -1;
+   i64                        stride = 0; // per-cell array stride (values)
+   std::vector< PlaneTile >   tilesHost;  // all planes z = 0..N, z-major
+   PlaneTile*                 tiles = nullptr;
+   int                        nTiles = 0;
+   int                        interiorTile0 = 0; // tiles of planes 1..N-3 (points off the cell boundary)
+   int                        nInteriorTiles = 0;
+   std::vector< LowerChunk >  chunksHost;
+   LowerChunk*                chunks = nullptr;
+   int                        nChunks = 0;
+};
+
+struct MeshCtx
+{
+   std::unique_ptr< Mesh >     mesh;
+   std::vector< LevelGeom >    levels; // indexed by level (lazily built)
+   std::vector< CellInfo >     cellInfoHost;
+   std::vector< LowerPrim >    primsHost;
+   std::vector< LowerEntry >   entriesHost;
+   LowerPrim*                  prims    = nullptr;
+   LowerEntry*                 entries  = nullptr;
+   CellInfo*                   cellInfo = nullptr;
+   int                         device   = 0;
+   void                        buildTopology(); // prims, entries, cell info (host + device)
+   LevelGeom&                  geom( int L );
+   ~MeshCtx();
+};
+
+} // namespace hyg
+
+struct hyg_mesh_s
+{
+   hyg::MeshCtx ctx;
+};
+
+struct hyg_func_s
+{
+   hyg_mesh_s*           mesh;
+   int                   minL, maxL, dtype;
+   std::vector< void* >  data; // per level (nullptr below minL)
+   size_t                elem() const { return dtype == 8 ? 8 : 4; }
+};
+
+struct hyg_op_s
+{
+   hyg_mesh_s* mesh;
+   int         minL, maxL;
+   struct Level
+   {
+      std::vector< double > cellStHost;  // nC x 16: 15 weights, inverse centre weight
+      std::vector< double > entryWHost;  // nEntries x 16: one-sided weights of each (primitive, cell)
+      std::vector< double > primInvHost; // nPrims: inverse centre weight (sum over incident cells)
+      double*               cellSt  = nullptr;
+      float*                cellStF = nullptr;
+      double*               entryW  = nullptr;
+      float*                entryWF = nullptr;
+      double*               primInv = nullptr;
+      float*                primInvF = nullptr;
+   };
+   std::vector< Level > levels;
+   ~hyg_op_s();
+};

@@ -1,0 +1,59 @@
+// Launch wrappers of the gfx950 kernels (implemented in kernels.hip).
+#pragma once
+
+#include "hyg_internal.hpp"
+
+namespace hyg {
+
+// Stencil modes; x = stencil input, A x computed on the fly, d = inverse diagonal
+enum StencilMode : int
+{
+   MODE_APPLY     = 0, // out0 = A x
+   MODE_APPLY_ADD = 1, // out0 += A x
+   MODE_RESIDUAL  = 2, // out0 = in1 - A x
+   MODE_JACOBI    = 3, // out0 = x + a * d * (in1 - A x)
+   MODE_CHEB0     = 4, // t = d * (in1 - A x); out0 = t; out1 = x + a * t
+   MODE_CHEBK     = 5, // t = d * A x;         out0 = t; out1 = in1 + a * t
+};
+
+struct StencilCall
+{
+   const void* x    = nullptr; // stencil input
+   const void* in1  = nullptr; // rhs (residual / Jacobi / Cheb0) or accumulator (ChebK)
+   void*       out0 = nullptr;
+   void*       out1 = nullptr;
+   double      a    = 0.0;
+   int         flag = 0;
+   int         mode = 0;
+};
+
+// Runs the cell-interior kernel and the macro-face/edge/vertex kernel of one stencil operation.
+void launchStencil( const MeshCtx& m, const LevelGeom& g, const hyg_op_s::Level& ol, int L, int dtype, const StencilCall& c,
+                    hipStream_t s );
+
+// Pointwise linear combination over all cell slots selected by flag:
+// dst = (accumulate ? dst : 0) + sum_k s_k src_k  (nsrc <= 3), or dst = value if nsrc == 0.
+void launchLincomb( const MeshCtx& m, const LevelGeom& g, int L, int dtype, void* dst, int nsrc, const double* s,
+                    const void* const* src, bool accumulate, double value, int flag, hipStream_t st );
+// dst = a * b pointwise (flag)
+void launchMult( const MeshCtx& m, const LevelGeom& g, int L, int dtype, void* dst, const void* a, const void* b, int flag,
+                 hipStream_t st );
+// Dot product over owned DoFs selected by flag; the FP64 result is left in result[0] (device).
+// `partials` must hold dotPartialsNeeded() doubles.
+void launchDot( const MeshCtx& m, const LevelGeom& g, int L, int dtype, const void* a, const void* b, int flag, double* partials,
+                double* result, hipStream_t st );
+int  dotPartialsNeeded( const MeshCtx& m, const LevelGeom& g );
+// value-type conversion over n values
+void launchConvert( i64 n, int dstType, void* dst, int srcType, const void* src, hipStream_t st );
+// plain copy of n values
+void launchCopy( i64 n, int dtype, void* dst, const void* src, hipStream_t st );
+// inverse diagonal written into a function (all slots)
+void launchInvDiag( const MeshCtx& m, const LevelGeom& g, const hyg_op_s::Level& ol, int L, int dtype, void* dst, hipStream_t st );
+
+// grid transfer: restriction level Lc+1 -> Lc, prolongation Lf-1 -> Lf
+void launchRestrict( const MeshCtx& m, const LevelGeom& gc, int Lc, int dtype, void* coarse, const void* fine, int flag,
+                     hipStream_t st );
+void launchProlongate( const MeshCtx& m, const LevelGeom& gf, int Lf, int dtype, void* fine, const void* coarse, int flag,
+                       bool add, hipStream_t st );
+
+} // namespace hyg

@@ -1,0 +1,347 @@
+// Host-side setup: topology tables, per-level work lists and stencil assembly.
+//
+// Stencils. The P1 Laplace stiffness of a uniformly refined affine macro-cell is a sum over
+// the micro-tetrahedra around a micro-vertex (src/hyteg/p1functionspace/P1Elements.hpp:89-143,
+// 463-527). All micro-tetrahedra of one of the six types (CellDoFIndexing.hpp:155-198) are
+// translates of each other, so one 4x4 element matrix per type and cell suffices. The element
+// matrix is K_ij = vol * grad(l_i) . grad(l_j) with grad(l_i) = n_i / (n_i . (P_i - P_a)),
+// n_i a normal of the face opposite P_i and P_a one of its vertices. The stencil of a point
+// in a cell is the sum, over the (type, corner) pairs whose tetrahedron lies inside the cell,
+// of the corresponding element-matrix row; at a macro-face, -edge or -vertex only the
+// micro-tetrahedra inside the cell contribute (one-sided stencil), and the full row is the sum
+// over the incident cells (P1Operator.hpp:1792-2161).
+#include "hyg_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace hyg {
+
+namespace {
+
+using V3 = std::array< double, 3 >;
+V3     sub( const V3& a, const V3& b ) { return { a[0] - b[0], a[1] - b[1], a[2] - b[2] }; }
+V3     cross( const V3& a, const V3& b ) { return { a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0] }; }
+double dot3( const V3& a, const V3& b ) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// lattice offsets of the vertices of the six micro-tetrahedron types
+const int kMicroTet[6][4][3] = { { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } },   // white up
+                                 { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },   // blue up
+                                 { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } },   // green up
+                                 { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },   // white down
+                                 { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } },   // blue down
+                                 { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } }; // green down
+
+int dirIndex( int dx, int dy, int dz )
+{
+   for ( int d = 0; d < NDIR; ++d )
+      if ( dirX( d ) == dx && dirY( d ) == dy && dirZ( d ) == dz )
+         return d;
+   throw std::logic_error( "micro-tetrahedron edge is not a stencil direction" );
+}
+
+// element matrices of the six micro-tetrahedron types of cell c at level L
+void elementMatrices( const Mesh& m, int c, int L, double K[6][4][4] )
+{
+   const double N  = double( 1 << L );
+   const V3&    P0 = m.X[m.Cv[c][0]];
+   const V3     a1 = sub( m.X[m.Cv[c][1]], P0 ), a2 = sub( m.X[m.Cv[c][2]], P0 ), a3 = sub( m.X[m.Cv[c][3]], P0 );
+   for ( int t = 0; t < 6; ++t )
+   {
+      V3 P[4];
+      for ( int k = 0; k < 4; ++k )
+         for ( int r = 0; r < 3; ++r )
+            P[k][r] = ( kMicroTet[t][k][0] * a1[r] + kMicroTet[t][k][1] * a2[r] + kMicroTet[t][k][2] * a3[r] ) / N;
+      const double vol = std::fabs( dot3( sub( P[1], P[0] ), cross( sub( P[2], P[0] ), sub( P[3], P[0] ) ) ) ) / 6.0;
+      V3           g[4];
+      for ( int i = 0; i < 4; ++i )
+      {
+         const int a = ( i + 1 ) % 4, b = ( i + 2 ) % 4, d = ( i + 3 ) % 4;
+         const V3  n = cross( sub( P[b], P[a] ), sub( P[d], P[a] ) );
+         const double s = dot3( n, sub( P[i], P[a] ) );
+         for ( int r = 0; r < 3; ++r )
+            g[i][r] = n[r] / s;
+      }
+      for ( int i = 0; i < 4; ++i )
+         for ( int j = 0; j < 4; ++j )
+            K[t][i][j] = vol * dot3( g[i], g[j] );
+   }
+}
+
+// one-sided stencil of cell c at lattice point (qx, qy, qz)
+void stencilAt( const double K[6][4][4], int N, int qx, int qy, int qz, double* w )
+{
+   for ( int d = 0; d < NDIR; ++d )
+      w[d] = 0.0;
+   for ( int t = 0; t < 6; ++t )
+      for ( int k = 0; k < 4; ++k )
+      {
+         // tetrahedron of type t whose corner k sits at q
+         bool inside = true;
+         for ( int j = 0; j < 4; ++j )
+         {
+            const int x = qx - kMicroTet[t][k][0] + kMicroTet[t][j][0];
+            const int y = qy - kMicroTet[t][k][1] + kMicroTet[t][j][1];
+            const int z = qz - kMicroTet[t][k][2] + kMicroTet[t][j][2];
+            inside      = inside && inCell( N, x, y, z );
+         }
+         if ( !inside )
+            continue;
+         for ( int j = 0; j < 4; ++j )
+            w[dirIndex( kMicroTet[t][j][0] - kMicroTet[t][k][0], kMicroTet[t][j][1] - kMicroTet[t][k][1],
+                        kMicroTet[t][j][2] - kMicroTet[t][k][2] )] += K[t][k][j];
+      }
+}
+
+void entryPosHost( const LowerEntry& e, int N, int i, int j, int& qx, int& qy, int& qz )
+{
+   auto u = []( int l, int r ) { return l == r + 1 ? 1 : 0; };
+   qx     = N * u( e.l0, 0 ) + i * ( u( e.l1, 0 ) - u( e.l0, 0 ) ) + j * ( u( e.l2, 0 ) - u( e.l0, 0 ) );
+   qy     = N * u( e.l0, 1 ) + i * ( u( e.l1, 1 ) - u( e.l0, 1 ) ) + j * ( u( e.l2, 1 ) - u( e.l0, 1 ) );
+   qz     = N * u( e.l0, 2 ) + i * ( u( e.l1, 2 ) - u( e.l0, 2 ) ) + j * ( u( e.l2, 2 ) - u( e.l0, 2 ) );
+}
+
+template < typename T >
+T* upload( const std::vector< T >& h )
+{
+   if ( h.empty() )
+      return nullptr;
+   void* d = nullptr;
+   if ( hipMalloc( &d, h.size() * sizeof( T ) ) != hipSuccess )
+      throw std::runtime_error( "hipMalloc failed for setup table" );
+   if ( hipMemcpy( d, h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice ) != hipSuccess )
+      throw std::runtime_error( "hipMemcpy failed for setup table" );
+   return static_cast< T* >( d );
+}
+
+} // namespace
+
+void primEntriesPos( const LowerEntry& e, int N, int i, int j, int& qx, int& qy, int& qz ) { entryPosHost( e, N, i, j, qx, qy, qz ); }
+
+// Topology tables: LowerPrim order = vertices, edges, faces; CellInfo per cell.
+void MeshCtx::buildTopology()
+{
+   const Mesh& m = *mesh;
+   auto        localOf = [&]( int c, int v ) {
+      for ( int l = 0; l < 4; ++l )
+         if ( m.Cv[c][l] == v )
+            return l;
+      throw std::logic_error( "vertex not in cell" );
+   };
+   auto addPrim = [&]( int kind, int bnd, const std::vector< int >& cells, const int* pv, int nv ) {
+      LowerPrim p;
+      p.kind   = kind;
+      p.bc     = bnd ? 2 : 1; // create0123BC: boundary flag 1 -> DirichletBoundary, 0 -> Inner
+      p.nCells = (int) cells.size();
+      p.entry0 = (int) entriesHost.size();
+      primsHost.push_back( p );
+      for ( int c : cells )
+      {
+         LowerEntry e;
+         e.cell = c;
+         e.l0   = localOf( c, pv[0] );
+         e.l1   = nv > 1 ? localOf( c, pv[1] ) : e.l0;
+         e.l2   = nv > 2 ? localOf( c, pv[2] ) : e.l0;
+         entriesHost.push_back( e );
+      }
+   };
+   for ( int v = 0; v < m.nV(); ++v )
+      addPrim( 0, m.vB[v], m.vC[v], &v, 1 );
+   for ( int e = 0; e < m.nE(); ++e )
+      addPrim( 1, m.eB[e], m.eC[e], m.E[e].data(), 2 );
+   for ( int f = 0; f < m.nF(); ++f )
+      addPrim( 2, m.fB[f], m.fC[f], m.F[f].data(), 3 );
+
+   cellInfoHost.resize( m.nC() );
+   for ( int c = 0; c < m.nC(); ++c )
+   {
+      CellInfo& ci = cellInfoHost[c];
+      ci.bcword    = 1; // slot 0: cells are always inner
+      ci.ownmask   = 1;
+      ci.prim[0]   = -1;
+      ci.invNC[0]  = 1.0;
+      ci.invNC[15] = 0.0;
+      auto setSlot = [&]( int slot, int prim, int bnd, const std::vector< int >& cells ) {
+         ci.bcword |= uint64_t( bnd ? 2 : 1 ) << ( 4 * slot );
+         if ( cells.front() == c )
+            ci.ownmask |= 1u << slot;
+         ci.prim[slot]  = prim;
+         ci.invNC[slot] = 1.0 / double( cells.size() );
+      };
+      for ( int k = 0; k < 4; ++k )
+      {
+         const int f = m.cF[c][k];
+         setSlot( 1 + k, m.nV() + m.nE() + f, m.fB[f], m.fC[f] );
+      }
+      for ( int k = 0; k < 6; ++k )
+      {
+         const int e = m.cE[c][k];
+         setSlot( 5 + k, m.nV() + e, m.eB[e], m.eC[e] );
+      }
+      for ( int k = 0; k < 4; ++k )
+      {
+         const int v = m.Cv[c][k];
+         setSlot( 11 + k, v, m.vB[v], m.vC[v] );
+      }
+   }
+   prims    = upload( primsHost );
+   entries  = upload( entriesHost );
+   cellInfo = upload( cellInfoHost );
+}
+
+LevelGeom& MeshCtx::geom( int L )
+{
+   if ( (int) levels.size() <= L )
+      levels.resize( L + 1 );
+   LevelGeom& g = levels[L];
+   if ( g.L == L )
+      return g;
+   const int N = 1 << L, w = N + 1;
+   g.stride    = cellStride( L );
+   g.tilesHost.clear();
+   g.interiorTile0  = -1;
+   g.nInteriorTiles = 0;
+   for ( int z = 0; z <= N; ++z )
+   {
+      const int W = w - z, np = W * ( W + 1 ) / 2;
+      for ( int s0 = 0; s0 < np; s0 += 256 )
+      {
+         if ( z >= 1 && z <= N - 3 )
+         {
+            if ( g.interiorTile0 < 0 )
+               g.interiorTile0 = (int) g.tilesHost.size();
+            ++g.nInteriorTiles;
+         }
+         g.tilesHost.push_back( { z, s0 } );
+      }
+   }
+   if ( g.interiorTile0 < 0 )
+      g.interiorTile0 = 0;
+   g.nTiles = (int) g.tilesHost.size();
+   g.chunksHost.clear();
+   for ( int p = 0; p < (int) primsHost.size(); ++p )
+   {
+      const int kind = primsHost[p].kind;
+      if ( kind == 0 )
+         g.chunksHost.push_back( { p, 0, 0, 1 } );
+      else if ( kind == 1 )
+      {
+         for ( int i0 = 1; i0 <= N - 1; i0 += 64 )
+            g.chunksHost.push_back( { p, 0, i0, std::min( 64, N - i0 ) } );
+      }
+      else
+      {
+         for ( int j = 1; j <= N - 2; ++j )
+            for ( int i0 = 1; i0 <= N - 1 - j; i0 += 64 )
+               g.chunksHost.push_back( { p, j, i0, std::min( 64, N - j - i0 ) } );
+      }
+   }
+   g.nChunks = (int) g.chunksHost.size();
+   g.tiles   = upload( g.tilesHost );
+   g.chunks  = upload( g.chunksHost );
+   g.L       = L;
+   return g;
+}
+
+MeshCtx::~MeshCtx()
+{
+   for ( auto& g : levels )
+   {
+      (void) hipFree( g.tiles );
+      (void) hipFree( g.chunks );
+   }
+   (void) hipFree( prims );
+   (void) hipFree( entries );
+   (void) hipFree( cellInfo );
+}
+
+// Assemble the per-level stencil tables of the constant P1 Laplace operator.
+void assembleP1Const( hyg_op_s& op )
+{
+   const MeshCtx& mc = op.mesh->ctx;
+   const Mesh&    m  = *mc.mesh;
+   op.levels.resize( op.maxL + 1 );
+   for ( int L = op.minL; L <= op.maxL; ++L )
+   {
+      hyg_op_s::Level& ol = op.levels[L];
+      const int        N  = 1 << L;
+      ol.cellStHost.assign( size_t( m.nC() ) * 16, 0.0 );
+      ol.entryWHost.assign( mc.entriesHost.size() * 16, 0.0 );
+      ol.primInvHost.assign( mc.primsHost.size(), 0.0 );
+      std::vector< std::array< std::array< std::array< double, 4 >, 4 >, 6 > > Ks( m.nC() );
+      for ( int c = 0; c < m.nC(); ++c )
+      {
+         double K[6][4][4];
+         elementMatrices( m, c, L, K );
+         for ( int t = 0; t < 6; ++t )
+            for ( int i = 0; i < 4; ++i )
+               for ( int j = 0; j < 4; ++j )
+                  Ks[c][t][i][j] = K[t][i][j];
+         if ( L >= 2 )
+         {
+            double* w = &ol.cellStHost[16 * c];
+            stencilAt( K, N, 1, 1, 1, w );
+            w[15] = 1.0 / w[0];
+         }
+      }
+      for ( size_t p = 0; p < mc.primsHost.size(); ++p )
+      {
+         const LowerPrim& pr = mc.primsHost[p];
+         // representative interior point of the primitive at this level
+         int i = 0, j = 0;
+         if ( pr.kind == 1 )
+         {
+            if ( L < 1 )
+               continue;
+            i = 1;
+         }
+         else if ( pr.kind == 2 )
+         {
+            if ( L < 2 )
+               continue;
+            i = j = 1;
+         }
+         double centre = 0.0;
+         for ( int k = 0; k < pr.nCells; ++k )
+         {
+            const LowerEntry& e = mc.entriesHost[pr.entry0 + k];
+            double            K[6][4][4];
+            for ( int t = 0; t < 6; ++t )
+               for ( int a = 0; a < 4; ++a )
+                  for ( int b = 0; b < 4; ++b )
+                     K[t][a][b] = Ks[e.cell][t][a][b];
+            int qx, qy, qz;
+            entryPosHost( e, N, i, j, qx, qy, qz );
+            double* w = &ol.entryWHost[16 * size_t( pr.entry0 + k )];
+            stencilAt( K, N, qx, qy, qz, w );
+            centre += w[0];
+         }
+         ol.primInvHost[p] = 1.0 / centre;
+      }
+      std::vector< float > f1( ol.cellStHost.begin(), ol.cellStHost.end() );
+      std::vector< float > f2( ol.entryWHost.begin(), ol.entryWHost.end() );
+      std::vector< float > f3( ol.primInvHost.begin(), ol.primInvHost.end() );
+      ol.cellSt   = upload( ol.cellStHost );
+      ol.entryW   = upload( ol.entryWHost );
+      ol.primInv  = upload( ol.primInvHost );
+      ol.cellStF  = upload( f1 );
+      ol.entryWF  = upload( f2 );
+      ol.primInvF = upload( f3 );
+   }
+}
+
+} // namespace hyg
+
+hyg_op_s::~hyg_op_s()
+{
+   for ( auto& l : levels )
+   {
+      (void) hipFree( l.cellSt );
+      (void) hipFree( l.cellStF );
+      (void) hipFree( l.entryW );
+      (void) hipFree( l.entryWF );
+      (void) hipFree( l.primInv );
+      (void) hipFree( l.primInvF );
+   }
+}

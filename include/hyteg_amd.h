@@ -1,0 +1,152 @@
+/*
+ * hyteg_amd.h — C ABI of the MI355X (gfx950) matrix-free P1 multigrid library.
+ *
+ * Every entry point returns an int status (HYG_OK = 0) and, on failure, stores a message
+ * retrievable with hyg_last_error() (thread-local). No exception crosses this boundary.
+ * Streams are passed as `void*` holding a hipStream_t (NULL = the default stream).
+ * Handles are opaque. Functions and operators live on the device selected by hyg_init().
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference repository root, src/hyteg/...).
+ */
+#ifndef HYTEG_AMD_H
+#define HYTEG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define HYG_OK 0
+#define HYG_ERR_ARG 1     /* bad handle, level out of range, size mismatch, unsupported option */
+#define HYG_ERR_HIP 2     /* a HIP runtime call failed */
+#define HYG_ERR_IO 3      /* mesh file missing or malformed */
+#define HYG_ERR_NUMERIC 4 /* non-finite value detected (zero diagonal, NaN residual) */
+#define HYG_ERR_STATE 5   /* library not initialised or object in the wrong state */
+
+/* DoFType bit flags, types/types.hpp:29-59 */
+#define HYG_NONE 0
+#define HYG_INNER 1
+#define HYG_DIRICHLET 2
+#define HYG_NEUMANN 4
+#define HYG_FREESLIP 8
+#define HYG_ALL 15
+
+/* UpdateType, types/types.hpp:61-72 */
+#define HYG_REPLACE 0
+#define HYG_ADD 1
+
+/* value types of a function */
+#define HYG_F64 8
+#define HYG_F32 4
+
+typedef struct hyg_mesh_s* hyg_mesh;
+typedef struct hyg_func_s* hyg_func;
+typedef struct hyg_op_s*   hyg_op;
+typedef struct hyg_gmg_s*  hyg_gmg;
+
+/* Last error message of the calling thread ("" if none). */
+const char* hyg_last_error( void );
+
+/* Select the device and create the library's per-device state. Must precede all other calls. */
+int hyg_init( int device );
+
+/* ---------------------------------------------------------------- mesh
+ * Replaces MeshInfo::fromGmshFile (mesh/MeshGenFromGmshFile.cpp:32) followed by
+ * SetupPrimitiveStorage(meshInfo, 1) + setMeshBoundaryFlagsOnBoundary(1, 0, true)
+ * (primitivestorage/SetupPrimitiveStorage.cpp:40-365, 781-792) and PrimitiveStorage.
+ * Boundary primitives get mesh flag 1 = Dirichlet (boundary/BoundaryConditions.hpp create0123BC). */
+int hyg_mesh_create_from_msh( const char* path, hyg_mesh* out );
+int hyg_mesh_destroy( hyg_mesh mesh );
+/* out[0..6] = #vertices, #edges, #faces, #cells, #boundary vertices, #boundary edges, #boundary faces */
+int hyg_mesh_info( hyg_mesh mesh, int64_t* out );
+/* Number of vertex DoFs at `level` in the canonical order (numberOfGlobalDoFs<P1FunctionTag>). */
+int hyg_mesh_num_dofs( hyg_mesh mesh, int level, int64_t* out );
+/* xyz[3*n], boundary[n] (1 = on a Dirichlet primitive) of every DoF in canonical order:
+ * macro-vertices, then edge interiors, face interiors, cell interiors, each primitive in
+ * storage order and its points in the reference's own linearisation. */
+int hyg_mesh_dof_coords( hyg_mesh mesh, int level, double* xyz, uint8_t* boundary );
+
+/* ---------------------------------------------------------------- functions
+ * Replaces P1Function<ValueType>(name, storage, minLevel, maxLevel)
+ * (p1functionspace/VertexDoFFunction.cpp:84-169; memory/FunctionMemory.hpp:46-240). */
+int hyg_func_create( hyg_mesh mesh, int min_level, int max_level, int dtype, hyg_func* out );
+int hyg_func_destroy( hyg_func f );
+/* Host <-> device copy of one level in canonical order (double on the host side).
+ * Upload also writes every copy of a value that is shared between macro-cells. */
+int hyg_func_upload( hyg_func f, int level, const double* canonical, void* stream );
+int hyg_func_download( hyg_func f, int level, double* canonical, void* stream );
+
+/* VertexDoFFunction::interpolate(constant, level, flag)  (VertexDoFFunction.cpp:382-394) */
+int hyg_interpolate_const( hyg_func f, double value, int level, int flag, void* stream );
+/* VertexDoFFunction::assign({scalars}, {functions}, level, flag), 1 to 3 sources
+ * (VertexDoFFunction.cpp:1107-1199) */
+int hyg_assign( hyg_func dst, int n, const double* scalars, const hyg_func* srcs, int level, int flag, void* stream );
+/* VertexDoFFunction::add({scalars}, {functions}, level, flag): dst += sum_k s_k f_k (VertexDoFFunction.cpp:1366-1444) */
+int hyg_add( hyg_func dst, int n, const double* scalars, const hyg_func* srcs, int level, int flag, void* stream );
+/* VertexDoFFunction::multElementwise({a, b}, level, flag) (VertexDoFFunction.cpp:1445-1523) */
+int hyg_mult_elementwise( hyg_func dst, hyg_func a, hyg_func b, int level, int flag, void* stream );
+/* VertexDoFFunction::dotGlobal(other, level, flag) (VertexDoFFunction.cpp:1668-1753); synchronises `stream`. */
+int hyg_dot( hyg_func a, hyg_func b, int level, int flag, double* out, void* stream );
+/* P1Function<T>::copyFrom(P1Function<U>) between value types (VertexDoFFunction.hpp:511-530, FunctionMemory.hpp:117) */
+int hyg_convert( hyg_func dst, hyg_func src, int level, void* stream );
+
+/* ---------------------------------------------------------------- operators
+ * Replaces P1ConstantLaplaceOperator(storage, minLevel, maxLevel)
+ * (p1functionspace/P1ConstantOperator.cpp:77, 602-655) incl.
+ * computeInverseDiagonalOperatorValues() (P1Operator.hpp:465-469, 645-915). */
+int hyg_op_create_p1_const( hyg_mesh mesh, int min_level, int max_level, hyg_op* out );
+int hyg_op_destroy( hyg_op op );
+/* Cell stencil (15 weights in the order C W E N S NW SE TC TW TS TSE BC BN BE BNW) of cell `cell`. */
+int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 );
+/* Operator::apply(src, dst, level, flag, updateType) (operators/Operator.hpp:73-80; P1Operator.hpp:196-324).
+ * src and dst must differ and have the same value type. */
+int hyg_apply( hyg_op op, hyg_func src, hyg_func dst, int level, int flag, int update, void* stream );
+/* WeightedJacobiSmoothable::smooth_jac(dst, rhs, src, relax, level, flag) (solvers/Smoothables.hpp:32-43;
+ * P1Operator.hpp:433-451): dst = src + relax * D^-1 (rhs - A src), fused into one pass. */
+int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double relax, int level, int flag,
+                    void* stream );
+/* OperatorWithInverseDiagonal::getInverseDiagonalValues (Smoothables.hpp:99-106) written into `dst`. */
+int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream );
+/* residual r = b - A x on `flag` (GeometricMultigridSolver.hpp:185-186 apply + assign), fused. */
+int hyg_residual( hyg_op op, hyg_func r, hyg_func b, hyg_func x, int level, int flag, void* stream );
+
+/* ---------------------------------------------------------------- grid transfer
+ * P1toP1LinearRestriction::restrict(function, sourceLevel, flag) (gridtransferoperators/P1toP1LinearRestriction.cpp:168-345) */
+int hyg_restrict( hyg_func f, int source_level, int flag, void* stream );
+/* P1toP1LinearProlongation::prolongate / prolongateAndAdd (gridtransferoperators/P1toP1LinearProlongation.cpp:194-398) */
+int hyg_prolongate( hyg_func f, int source_level, int flag, int update, void* stream );
+
+/* ---------------------------------------------------------------- solvers
+ * ChebyshevSmoother::setupCoefficients(order, lower, upper) (solvers/ChebyshevSmoother.hpp:128-190), order 1..5. */
+int hyg_cheb_coefficients( int order, double lower, double upper, double* out );
+/* chebyshev::estimateRadius -> estimateSpectralRadiusWithPowerIteration on D^-1 A, flag All
+ * (ChebyshevSmoother.hpp:255-265, numerictools/SpectrumEstimation.hpp:57-80). x holds the start vector. */
+int hyg_power_iteration( hyg_op op, hyg_func x, hyg_func tmp, int iterations, int level, double* radius, void* stream );
+
+/* GeometricMultigridSolver (solvers/GeometricMultigridSolver.hpp:42-235) with a WeightedJacobiSmoother
+ * (smoother = 0, solvers/WeightedJacobiSmoother.hpp:42-62) or ChebyshevSmoother (smoother = 1,
+ * ChebyshevSmoother.hpp:48-97) and a CGSolver coarse solver (solvers/CGSolver.hpp:66-134).
+ * iparams = {min_level, max_level, pre, post, increment, smoother, cheb_order, cg_max_iter}
+ * dparams = {relax, cheb_lower, cheb_upper, cg_tolerance}
+ * The solver works on functions of the operator's value type (HYG_F64 or HYG_F32). */
+int hyg_gmg_create( hyg_op op, int dtype, const int* iparams, const double* dparams, hyg_gmg* out );
+int hyg_gmg_destroy( hyg_gmg g );
+/* Solver::solve(A, x, b, level) once = one V-cycle (GeometricMultigridSolver.hpp:127-134). */
+int hyg_gmg_vcycle( hyg_gmg g, hyg_func x, hyg_func b, int level, void* stream );
+/* `cycles` V-cycles; res[0] = ||b - Ax|| over Inner|Neumann DoFs before, res[k] after cycle k (FP64 norms). */
+int hyg_gmg_solve( hyg_gmg g, hyg_func x, hyg_func b, int level, int cycles, double* res, void* stream );
+/* Mixed-precision defect correction (tests/hyteg/mixedPrecision/mixedPrecisionIterativeRefinement.cpp:162-172):
+ * FP64 x, b; each outer step r = b - A x (FP64), FP32 correction by `inner` FP32 V-cycles of g32, x += c (FP64).
+ * Stops after `max_outer` steps or when ||r||/||r0|| <= rel_tol. res[k] = FP64 residual norm after step k
+ * (res[0] initial); *steps = number of outer steps taken. */
+int hyg_ir_solve( hyg_op op64, hyg_gmg g32, hyg_func x, hyg_func b, int level, int inner, int max_outer,
+                  double rel_tol, double* res, int* steps, void* stream );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1525 @@
+// ORACLE — CPU restatement of HyTeG's P1 matrix-free geometric-multigrid path.
+//
+// This file is TEST INFRASTRUCTURE. It is built into oracle/liboracle.so and is only
+// loaded by tests/, by __graft_entry__.smoke() (as the checker) and by bench.py's
+// cpu_baseline leg. The product library never links, loads or calls it.
+//
+// It follows the reference's algorithms, data layout and loop order function by
+// function (citations are relative to the reference repository root):
+//   * macro-mesh construction  src/hyteg/mesh/MeshGenFromGmshFile.cpp:32-200,
+//     src/hyteg/primitivestorage/SetupPrimitiveStorage.cpp:40-365 (primitive order,
+//     face vertex order from sorted edges, cell vertex order = Gmsh element order),
+//     boundary flags SetupPrimitiveStorage.cpp:781-792 + onBoundary :895-945;
+//   * per-primitive arrays    src/hyteg/indexing/MacroCellIndexing.hpp:40-53,
+//     MacroFaceIndexing.hpp:39-48 (owned data incl. boundary copies; the ghost layers
+//     of faces/edges/vertices are replaced by direct reads of the neighbour cell arrays,
+//     which hold identical values after the downward communication);
+//   * downward halo refresh   P1Operator.hpp:205-211 (V->E, E->F, F->C chain);
+//   * stencil assembly        P1Elements.hpp:89-143 (24 micro-tets at a vertex),
+//     :463-527 calculateStencilInMacroCellForm_new, FEniCS p1_tet_diffusion
+//     tabulate_tensor (src/hyteg/forms/form_fenics_generated/p1_tet_diffusion.h:4113);
+//   * apply                   P1Operator.hpp:196-324, generated cell kernel
+//     p1functionspace/generatedKernels/apply_3D_macrocell_vertexdof_to_vertexdof_replace.cpp:32-87;
+//   * smooth_jac              P1Operator.hpp:433-451, WeightedJacobiSmoother.hpp:42-62;
+//   * inverse diagonal        P1Operator.hpp:645-915, VertexDoFFunction.cpp:1524-1666;
+//   * restriction             P1toP1LinearRestriction.cpp:168-345 with the leaf-location
+//     scaling of generatedKernels/restrict_3D_macrocell_P1_pull_additive.cpp;
+//   * prolongation            P1toP1LinearProlongation.cpp:194-398;
+//   * additive communication  VertexDoFFunction.hpp:322-415;
+//   * vector ops              VertexDoFFunction.cpp:1107-1199, 1366-1523, 1668-1753;
+//   * Chebyshev smoother      solvers/ChebyshevSmoother.hpp:48-190, power iteration
+//     numerictools/SpectrumEstimation.hpp:57-80;
+//   * CG                      solvers/CGSolver.hpp:66-134, 237-248;
+//   * GMG V-cycle             solvers/GeometricMultigridSolver.hpp:127-235;
+//   * mixed-precision IR      tests/hyteg/mixedPrecision/mixedPrecisionIterativeRefinement.cpp:162-172.
+//
+// How each function is pinned (tests/test_oracle_*.py): the apply, smoother, transfer
+// and V-cycle against an independent scipy global assembly of the FE matrices on the
+// refined meshes (brute force), the stencils against A*1 = 0 / A*linear = 0, row sums
+// and h-scaling, the transfer against exactness on linear functions and R = P^T,
+// the Chebyshev coefficients against the closed-form Chebyshev polynomial, CG against
+// scipy's direct solve. Parity unpinned: none of the functions exported here.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace orc {
+
+using idx = int64_t;
+
+// ---------------------------------------------------------------- indexing
+inline idx S3( idx w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
+inline idx T2( idx w ) { return w * ( w + 1 ) / 2; }
+// MacroCellIndexing.hpp:46-53
+inline idx cidx( idx w, idx x, idx y, idx z ) { return S3( w ) - S3( w - z ) + y * ( w - z + 1 ) - y * ( y + 1 ) / 2 + x; }
+// MacroFaceIndexing.hpp:39-48
+inline idx fidx( idx w, idx x, idx y ) { return y * ( w + 1 ) - y * ( y + 1 ) / 2 + x; }
+
+enum DoFType : int { None = 0, Inner = 1, DirichletBoundary = 2, NeumannBoundary = 4, FreeslipBoundary = 8, All = 15 };
+
+using I3 = std::array< idx, 3 >;
+using P3 = std::array< double, 3 >;
+
+// Stencil directions, VertexDoFIndexing.cpp:484-520
+enum Dir { V_C, V_W, V_E, V_N, V_S, V_NW, V_SE, V_TC, V_TW, V_TS, V_TSE, V_BC, V_BN, V_BE, V_BNW, NDIR };
+static const int DOFF[NDIR][3] = { { 0, 0, 0 },
+                                   { -1, 0, 0 },
+                                   { 1, 0, 0 },
+                                   { 0, 1, 0 },
+                                   { 0, -1, 0 },
+                                   { -1, 1, 0 },
+                                   { 1, -1, 0 },
+                                   { 0, 0, 1 },
+                                   { -1, 0, 1 },
+                                   { 0, -1, 1 },
+                                   { 1, -1, 1 },
+                                   { 0, 0, -1 },
+                                   { 0, 1, -1 },
+                                   { 1, 0, -1 },
+                                   { -1, 1, -1 } };
+
+// The 24 micro-tets around an inner micro-vertex, P1Elements.hpp:93-143 (each starts with C).
+static const int CELLS_AT_VERTEX[24][4] = {
+    // white up
+    { V_C, V_BC, V_BE, V_BN },
+    { V_C, V_S, V_SE, V_TS },
+    { V_C, V_W, V_NW, V_TW },
+    { V_C, V_N, V_E, V_TC },
+    // white down
+    { V_C, V_W, V_BC, V_S },
+    { V_C, V_E, V_SE, V_BE },
+    { V_C, V_N, V_NW, V_BN },
+    { V_C, V_TS, V_TC, V_TW },
+    // blue up
+    { V_C, V_BC, V_BN, V_BNW },
+    { V_C, V_W, V_S, V_TS },
+    { V_C, V_E, V_SE, V_TSE },
+    { V_C, V_NW, V_N, V_TC },
+    // blue down
+    { V_C, V_BC, V_S, V_SE },
+    { V_C, V_W, V_NW, V_BNW },
+    { V_C, V_E, V_BN, V_N },
+    { V_C, V_TC, V_TS, V_TSE },
+    // green up
+    { V_C, V_W, V_BC, V_BNW },
+    { V_C, V_E, V_BE, V_BN },
+    { V_C, V_TC, V_TW, V_NW },
+    { V_C, V_SE, V_TS, V_TSE },
+    // green down
+    { V_C, V_BC, V_BE, V_SE },
+    { V_C, V_BN, V_BNW, V_NW },
+    { V_C, V_E, V_TSE, V_TC },
+    { V_C, V_W, V_TS, V_TW } };
+
+// ---------------------------------------------------------------- mesh / storage
+struct Mesh
+{
+   std::vector< P3 >                   X;     // vertices ordered by Gmsh id
+   std::vector< std::array< int, 2 > > E;     // sorted vertex pairs, map order
+   std::vector< std::array< int, 3 > > F;     // sorted vertex triples, map order
+   std::vector< std::array< int, 4 > > Cl;    // cell vertices in Gmsh element order, map order
+   std::vector< std::array< int, 4 > > cF;    // cell-local face k -> face id
+   std::vector< std::array< int, 6 > > cE;    // cell-local edge k -> edge id
+   std::vector< std::vector< int > >   vC, eC, fC, vE, vF, eF;
+   std::vector< int >                  vB, eB, fB; // 1 = on boundary (Dirichlet), 0 = inner
+};
+
+static const int CELL_FACE_VERTS[4][3] = { { 0, 1, 2 }, { 0, 1, 3 }, { 0, 2, 3 }, { 1, 2, 3 } };
+static const int CELL_EDGE_VERTS[6][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 }, { 0, 3 }, { 1, 3 }, { 2, 3 } };
+
+static std::unique_ptr< Mesh > loadGmsh( const std::string& path )
+{
+   std::ifstream in( path );
+   if ( !in )
+      return nullptr;
+   std::string tok;
+   in >> tok >> tok >> tok >> tok >> tok >> tok; // $MeshFormat 2.2 0 8 $EndMeshFormat $Nodes
+   if ( tok != "$Nodes" )
+      return nullptr;
+   idx nv;
+   in >> nv;
+   std::map< idx, P3 > nodes;
+   for ( idx i = 0; i < nv; ++i )
+   {
+      idx id;
+      P3  x;
+      in >> id >> x[0] >> x[1] >> x[2];
+      nodes[id] = x;
+   }
+   in >> tok >> tok; // $EndNodes $Elements
+   idx ne;
+   in >> ne;
+   std::set< std::vector< idx > > cellsGmsh; // keyed like MeshInfo::CellContainer
+   std::vector< std::vector< idx > > cellList;
+   for ( idx i = 0; i < ne; ++i )
+   {
+      idx id, type, ntags;
+      in >> id >> type >> ntags;
+      std::vector< idx > tags( ntags );
+      for ( auto& t : tags )
+         in >> t;
+      int nn = type == 15 ? 1 : type == 1 ? 2 : type == 2 ? 3 : type == 4 ? 4 : -1;
+      if ( nn < 0 )
+         return nullptr;
+      std::vector< idx > vs( nn );
+      for ( auto& v : vs )
+         in >> v;
+      if ( type == 4 )
+         cellsGmsh.insert( vs );
+   }
+   auto m = std::make_unique< Mesh >();
+   std::map< idx, int > vid;
+   for ( auto& [id, x] : nodes )
+   {
+      vid[id] = (int) m->X.size();
+      m->X.push_back( x );
+   }
+   std::set< std::array< int, 2 > > es;
+   std::set< std::array< int, 3 > > fs;
+   for ( auto& c : cellsGmsh )
+   {
+      std::array< int, 4 > cv{ vid[c[0]], vid[c[1]], vid[c[2]], vid[c[3]] };
+      m->Cl.push_back( cv );
+      for ( auto& e : CELL_EDGE_VERTS )
+      {
+         std::array< int, 2 > k{ cv[e[0]], cv[e[1]] };
+         std::sort( k.begin(), k.end() );
+         es.insert( k );
+      }
+      for ( auto& f : CELL_FACE_VERTS )
+      {
+         std::array< int, 3 > k{ cv[f[0]], cv[f[1]], cv[f[2]] };
+         std::sort( k.begin(), k.end() );
+         fs.insert( k );
+      }
+   }
+   m->E.assign( es.begin(), es.end() );
+   m->F.assign( fs.begin(), fs.end() );
+   std::map< std::array< int, 2 >, int > eid;
+   std::map< std::array< int, 3 >, int > fid;
+   for ( size_t i = 0; i < m->E.size(); ++i )
+      eid[m->E[i]] = (int) i;
+   for ( size_t i = 0; i < m->F.size(); ++i )
+      fid[m->F[i]] = (int) i;
+   size_t nV = m->X.size(), nE = m->E.size(), nF = m->F.size(), nC = m->Cl.size();
+   m->vC.resize( nV );
+   m->eC.resize( nE );
+   m->fC.resize( nF );
+   m->vE.resize( nV );
+   m->vF.resize( nV );
+   m->eF.resize( nE );
+   for ( size_t e = 0; e < nE; ++e )
+      for ( int v : m->E[e] )
+         m->vE[v].push_back( (int) e );
+   for ( size_t f = 0; f < nF; ++f )
+   {
+      auto& t = m->F[f];
+      for ( int v : t )
+         m->vF[v].push_back( (int) f );
+      m->eF[eid[{ t[0], t[1] }]].push_back( (int) f );
+      m->eF[eid[{ t[0], t[2] }]].push_back( (int) f );
+      m->eF[eid[{ t[1], t[2] }]].push_back( (int) f );
+   }
+   m->cF.resize( nC );
+   m->cE.resize( nC );
+   for ( size_t c = 0; c < nC; ++c )
+   {
+      auto& cv = m->Cl[c];
+      for ( int k = 0; k < 4; ++k )
+      {
+         std::array< int, 3 > key{ cv[CELL_FACE_VERTS[k][0]], cv[CELL_FACE_VERTS[k][1]], cv[CELL_FACE_VERTS[k][2]] };
+         std::sort( key.begin(), key.end() );
+         m->cF[c][k] = fid[key];
+         m->fC[fid[key]].push_back( (int) c );
+      }
+      for ( int k = 0; k < 6; ++k )
+      {
+         std::array< int, 2 > key{ cv[CELL_EDGE_VERTS[k][0]], cv[CELL_EDGE_VERTS[k][1]] };
+         std::sort( key.begin(), key.end() );
+         m->cE[c][k] = eid[key];
+         m->eC[eid[key]].push_back( (int) c );
+      }
+      for ( int v : cv )
+         m->vC[v].push_back( (int) c );
+   }
+   // onBoundary (SetupPrimitiveStorage.cpp:895-945): face with one cell; edge/vertex touching such a face.
+   m->fB.assign( nF, 0 );
+   m->eB.assign( nE, 0 );
+   m->vB.assign( nV, 0 );
+   for ( size_t f = 0; f < nF; ++f )
+      if ( m->fC[f].size() == 1 )
+      {
+         m->fB[f] = 1;
+         auto& t  = m->F[f];
+         m->eB[eid[{ t[0], t[1] }]] = 1;
+         m->eB[eid[{ t[0], t[2] }]] = 1;
+         m->eB[eid[{ t[1], t[2] }]] = 1;
+         for ( int v : t )
+            m->vB[v] = 1;
+      }
+   return m;
+}
+
+// create0123BC: mesh flag 1 -> Dirichlet, 0 -> Inner
+inline int bcType( int onBoundary ) { return onBoundary ? DirichletBoundary : Inner; }
+
+// Cell coordinates of a point of a lower-dimensional primitive given by global vertex ids pv[0..nv-1]
+// and barycentric weights bw[0..nv-1] (sum N); VertexDoFMacroCell / Cell local vertex maps.
+inline I3 cellCoords( const Mesh& m, int c, const int* pv, const idx* bw, int nv )
+{
+   idx b[4] = { 0, 0, 0, 0 };
+   for ( int k = 0; k < nv; ++k )
+      for ( int l = 0; l < 4; ++l )
+         if ( m.Cl[c][l] == pv[k] )
+            b[l] = bw[k];
+   return { b[1], b[2], b[3] };
+}
+
+inline bool inCell( idx N, idx x, idx y, idx z ) { return x >= 0 && y >= 0 && z >= 0 && x + y + z <= N; }
+
+// ---------------------------------------------------------------- function memory
+template < typename T >
+struct Function
+{
+   const Mesh* m;
+   int         minL, maxL;
+   // [level][primitive] -> owned array (incl. boundary copies for edges/faces/cells)
+   std::vector< std::vector< std::vector< T > > > v, e, f, c;
+   Function( const Mesh* mesh, int lo, int hi )
+   : m( mesh )
+   , minL( lo )
+   , maxL( hi )
+   {
+      v.resize( hi + 1 );
+      e.resize( hi + 1 );
+      f.resize( hi + 1 );
+      c.resize( hi + 1 );
+      for ( int L = lo; L <= hi; ++L )
+      {
+         idx w = ( idx( 1 ) << L ) + 1;
+         v[L].assign( m->X.size(), std::vector< T >( 1, T( 0 ) ) );
+         e[L].assign( m->E.size(), std::vector< T >( w, T( 0 ) ) );
+         f[L].assign( m->F.size(), std::vector< T >( T2( w ), T( 0 ) ) );
+         c[L].assign( m->Cl.size(), std::vector< T >( S3( w ), T( 0 ) ) );
+      }
+   }
+   // communicate<Vertex,Edge>, <Edge,Face>, <Face,Cell> (VertexDoFPackInfo.hpp:161-201, 250-292, 411-505)
+   void commDown( int L ) const
+   {
+      auto&     self = const_cast< Function& >( *this );
+      const idx N    = idx( 1 ) << L;
+      const idx w    = N + 1;
+      for ( size_t ei = 0; ei < m->E.size(); ++ei )
+      {
+         self.e[L][ei][0] = v[L][m->E[ei][0]][0];
+         self.e[L][ei][N] = v[L][m->E[ei][1]][0];
+      }
+      for ( size_t fi = 0; fi < m->F.size(); ++fi )
+      {
+         auto& t = m->F[fi];
+         // face edges: (t0,t1) y=0 ; (t0,t2) x=0 ; (t1,t2) x+y=N
+         for ( idx i = 0; i <= N; ++i )
+         {
+            self.f[L][fi][fidx( w, i, 0 )]     = edgeValue( L, t[0], t[1], i );
+            self.f[L][fi][fidx( w, 0, i )]     = edgeValue( L, t[0], t[2], i );
+            self.f[L][fi][fidx( w, N - i, i )] = edgeValue( L, t[1], t[2], i );
+         }
+      }
+#pragma omp parallel for schedule( dynamic )
+      for ( idx ci = 0; ci < (idx) m->Cl.size(); ++ci )
+      {
+         for ( int k = 0; k < 4; ++k )
+         {
+            int  fi = m->cF[ci][k];
+            auto tv = m->F[fi];
+            int  pv[3] = { tv[0], tv[1], tv[2] };
+            for ( idx j = 0; j <= N; ++j )
+               for ( idx i = 0; i + j <= N; ++i )
+               {
+                  idx bw[3] = { N - i - j, i, j };
+                  I3  q     = cellCoords( *m, (int) ci, pv, bw, 3 );
+                  self.c[L][ci][cidx( w, q[0], q[1], q[2] )] = f[L][fi][fidx( w, i, j )];
+               }
+         }
+      }
+   }
+   // value at edge-local position i measured from vertex a towards vertex b
+   T edgeValue( int L, int a, int b, idx i ) const
+   {
+      const idx N = idx( 1 ) << L;
+      for ( size_t ei = 0; ei < m->E.size(); ++ei )
+      {
+         if ( m->E[ei][0] == a && m->E[ei][1] == b )
+            return e[L][ei][i];
+         if ( m->E[ei][0] == b && m->E[ei][1] == a )
+            return e[L][ei][N - i];
+      }
+      return T( 0 );
+   }
+};
+
+// canonical DoF order: vertices, edge interiors, face interiors, cell interiors
+inline idx numDofs( const Mesh& m, int L )
+{
+   idx N = idx( 1 ) << L;
+   return idx( m.X.size() ) + idx( m.E.size() ) * ( N - 1 ) + idx( m.F.size() ) * ( ( N - 1 ) * ( N - 2 ) / 2 ) +
+          idx( m.Cl.size() ) * ( ( N - 1 ) * ( N - 2 ) * ( N - 3 ) / 6 );
+}
+
+template < typename T, typename Fn >
+void forEachDof( const Mesh& m, int L, Fn&& fn )
+{
+   // fn(kind, primitive, i, j, k, canonicalIndex)
+   const idx N = idx( 1 ) << L;
+   idx       k = 0;
+   for ( size_t vi = 0; vi < m.X.size(); ++vi )
+      fn( 0, (int) vi, 0, 0, 0, k++ );
+   for ( size_t ei = 0; ei < m.E.size(); ++ei )
+      for ( idx i = 1; i <= N - 1; ++i )
+         fn( 1, (int) ei, i, 0, 0, k++ );
+   for ( size_t fi = 0; fi < m.F.size(); ++fi )
+      for ( idx j = 1; j <= N - 2; ++j )
+         for ( idx i = 1; i + j <= N - 1; ++i )
+            fn( 2, (int) fi, i, j, 0, k++ );
+   for ( size_t ci = 0; ci < m.Cl.size(); ++ci )
+      for ( idx z = 1; z <= N - 3; ++z )
+         for ( idx y = 1; y + z <= N - 2; ++y )
+            for ( idx x = 1; x + y + z <= N - 1; ++x )
+               fn( 3, (int) ci, x, y, z, k++ );
+}
+
+// ---------------------------------------------------------------- local form (FEniCS p1_tet_diffusion)
+// Row 0 of the P1 Laplace element matrix: A_0j = |det J|/6 * (J^-T g0).(J^-T gj), g = reference gradients.
+inline void p1DiffusionRow0( const P3* x, double* row )
+{
+   double J[3][3];
+   for ( int r = 0; r < 3; ++r )
+      for ( int k = 0; k < 3; ++k )
+         J[r][k] = x[k + 1][r] - x[0][r];
+   double det = J[0][0] * ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) - J[0][1] * ( J[1][0] * J[2][2] - J[1][2] * J[2][0] ) +
+                J[0][2] * ( J[1][0] * J[2][1] - J[1][1] * J[2][0] );
+   double K[3][3]; // K = J^-1
+   K[0][0] = ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) / det;
+   K[0][1] = ( J[0][2] * J[2][1] - J[0][1] * J[2][2] ) / det;
+   K[0][2] = ( J[0][1] * J[1][2] - J[0][2] * J[1][1] ) / det;
+   K[1][0] = ( J[1][2] * J[2][0] - J[1][0] * J[2][2] ) / det;
+   K[1][1] = ( J[0][0] * J[2][2] - J[0][2] * J[2][0] ) / det;
+   K[1][2] = ( J[0][2] * J[1][0] - J[0][0] * J[1][2] ) / det;
+   K[2][0] = ( J[1][0] * J[2][1] - J[1][1] * J[2][0] ) / det;
+   K[2][1] = ( J[0][1] * J[2][0] - J[0][0] * J[2][1] ) / det;
+   K[2][2] = ( J[0][0] * J[1][1] - J[0][1] * J[1][0] ) / det;
+   const double g[4][3] = { { -1, -1, -1 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } };
+   double       G[4][3];
+   for ( int i = 0; i < 4; ++i )
+      for ( int r = 0; r < 3; ++r )
+         G[i][r] = K[0][r] * g[i][0] + K[1][r] * g[i][1] + K[2][r] * g[i][2];
+   const double vol = std::fabs( det ) / 6.0;
+   for ( int j = 0; j < 4; ++j )
+      row[j] = vol * ( G[0][0] * G[j][0] + G[0][1] * G[j][1] + G[0][2] * G[j][2] );
+}
+
+// VertexDoFMacroCell.hpp:68-75
+inline P3 microCoord( const Mesh& m, int c, int L, idx x, idx y, idx z )
+{
+   const double h  = 1.0 / double( idx( 1 ) << L );
+   const P3&    v0 = m.X[m.Cl[c][0]];
+   const P3&    v1 = m.X[m.Cl[c][1]];
+   const P3&    v2 = m.X[m.Cl[c][2]];
+   const P3&    v3 = m.X[m.Cl[c][3]];
+   P3           r;
+   for ( int k = 0; k < 3; ++k )
+      r[k] = v0[k] + ( v1[k] - v0[k] ) * h * double( x ) + ( v2[k] - v0[k] ) * h * double( y ) + ( v3[k] - v0[k] ) * h * double( z );
+   return r;
+}
+
+// calculateStencilInMacroCellForm_new (P1Elements.hpp:463-527): sum row 0 over the micro-tets at q inside cell c.
+inline std::array< double, NDIR > stencilInCell( const Mesh& m, int c, int L, const I3& q )
+{
+   std::array< double, NDIR > s{};
+   const idx                  N = idx( 1 ) << L;
+   for ( auto& el : CELLS_AT_VERTEX )
+   {
+      bool inside = true;
+      P3   xs[4];
+      for ( int k = 0; k < 4; ++k )
+      {
+         idx x = q[0] + DOFF[el[k]][0], y = q[1] + DOFF[el[k]][1], z = q[2] + DOFF[el[k]][2];
+         if ( !inCell( N, x, y, z ) )
+            inside = false;
+         else
+            xs[k] = microCoord( m, c, L, x, y, z );
+      }
+      if ( !inside )
+         continue;
+      double row[4];
+      p1DiffusionRow0( xs, row );
+      for ( int k = 0; k < 4; ++k )
+         s[el[k]] += row[k];
+   }
+   return s;
+}
+
+// ---------------------------------------------------------------- P1 constant operator
+struct LowerStencil
+{
+   int                                       nCells;
+   std::vector< int >                        cells;
+   std::vector< std::array< double, NDIR > > w; // per neighbour cell
+};
+
+template < typename T >
+struct P1ConstantOperator
+{
+   const Mesh* m;
+   int         minL, maxL;
+   // [level]
+   std::vector< std::vector< std::array< double, NDIR > > > cellSt;            // per cell
+   std::vector< std::vector< LowerStencil > >               vSt, eSt, fSt;     // per primitive
+   std::unique_ptr< Function< T > >                         invDiag;
+
+   P1ConstantOperator( const Mesh* mesh, int lo, int hi )
+   : m( mesh )
+   , minL( lo )
+   , maxL( hi )
+   {
+      cellSt.resize( hi + 1 );
+      vSt.resize( hi + 1 );
+      eSt.resize( hi + 1 );
+      fSt.resize( hi + 1 );
+      for ( int L = lo; L <= hi; ++L )
+         assemble( L );
+      invDiag = std::make_unique< Function< T > >( m, lo, hi );
+      for ( int L = lo; L <= hi; ++L )
+         computeInverseDiagonal( L );
+   }
+
+   LowerStencil lowerStencil( int L, const std::vector< int >& cells, const int* pv, const idx* bw, int nv ) const
+   {
+      LowerStencil ls;
+      ls.nCells = (int) cells.size();
+      ls.cells  = cells;
+      for ( int c : cells )
+      {
+         I3 q = cellCoords( *m, c, pv, bw, nv );
+         ls.w.push_back( stencilInCell( *m, c, L, q ) );
+      }
+      return ls;
+   }
+
+   // P1ConstantOperator::assembleStencils (P1ConstantOperator.cpp:602-655), P1Operator.hpp:510-552, 1792-1950, 2071-2161
+   void assemble( int L )
+   {
+      const idx N = idx( 1 ) << L;
+      cellSt[L].resize( m->Cl.size() );
+      if ( L >= 2 )
+         for ( size_t c = 0; c < m->Cl.size(); ++c )
+            cellSt[L][c] = stencilInCell( *m, (int) c, L, { 1, 1, 1 } );
+      for ( size_t vi = 0; vi < m->X.size(); ++vi )
+      {
+         int pv[1] = { (int) vi };
+         idx bw[1] = { N };
+         vSt[L].push_back( lowerStencil( L, m->vC[vi], pv, bw, 1 ) );
+      }
+      for ( size_t ei = 0; ei < m->E.size(); ++ei )
+      {
+         int pv[2] = { m->E[ei][0], m->E[ei][1] };
+         idx bw[2] = { N - 1, 1 };
+         eSt[L].push_back( L >= 1 ? lowerStencil( L, m->eC[ei], pv, bw, 2 ) : LowerStencil{} );
+      }
+      for ( size_t fi = 0; fi < m->F.size(); ++fi )
+      {
+         int pv[3] = { m->F[fi][0], m->F[fi][1], m->F[fi][2] };
+         idx bw[3] = { N - 2, 1, 1 };
+         fSt[L].push_back( L >= 2 ? lowerStencil( L, m->fC[fi], pv, bw, 3 ) : LowerStencil{} );
+      }
+   }
+
+   // value of stencil sum at a lower-dim point (reads the neighbour cells' arrays)
+   T lowerApply( const Function< T >& src, int L, const LowerStencil& ls, const int* pv, const idx* bw, int nv ) const
+   {
+      const idx w   = ( idx( 1 ) << L ) + 1;
+      T         sum = T( 0 );
+      for ( int k = 0; k < ls.nCells; ++k )
+      {
+         int         c   = ls.cells[k];
+         I3          q   = cellCoords( *m, c, pv, bw, nv );
+         const auto& arr = src.c[L][c];
+         const idx   N   = w - 1;
+         for ( int d = 0; d < NDIR; ++d )
+         {
+            if ( ls.w[k][d] == 0.0 )
+               continue;
+            idx x = q[0] + DOFF[d][0], y = q[1] + DOFF[d][1], z = q[2] + DOFF[d][2];
+            if ( !inCell( N, x, y, z ) )
+               continue;
+            sum += T( ls.w[k][d] ) * arr[cidx( w, x, y, z )];
+         }
+      }
+      return sum;
+   }
+
+   void apply( const Function< T >& src, Function< T >& dst, int L, int flag, bool add ) const
+   {
+      src.commDown( L );
+      const idx N = idx( 1 ) << L;
+      const idx w = N + 1;
+      for ( size_t vi = 0; vi < m->X.size(); ++vi )
+      {
+         if ( !( bcType( m->vB[vi] ) & flag ) )
+            continue;
+         int pv[1] = { (int) vi };
+         idx bw[1] = { N };
+         T   r     = lowerApply( src, L, vSt[L][vi], pv, bw, 1 );
+         dst.v[L][vi][0] = add ? dst.v[L][vi][0] + r : r;
+      }
+      if ( L >= 1 )
+         for ( size_t ei = 0; ei < m->E.size(); ++ei )
+         {
+            if ( !( bcType( m->eB[ei] ) & flag ) )
+               continue;
+            int pv[2] = { m->E[ei][0], m->E[ei][1] };
+            for ( idx i = 1; i <= N - 1; ++i )
+            {
+               idx bw[2] = { N - i, i };
+               T   r     = lowerApply( src, L, eSt[L][ei], pv, bw, 2 );
+               dst.e[L][ei][i] = add ? dst.e[L][ei][i] + r : r;
+            }
+         }
+      if ( L >= 2 )
+      {
+         for ( size_t fi = 0; fi < m->F.size(); ++fi )
+         {
+            if ( !( bcType( m->fB[fi] ) & flag ) )
+               continue;
+            int pv[3] = { m->F[fi][0], m->F[fi][1], m->F[fi][2] };
+            for ( idx j = 1; j <= N - 2; ++j )
+               for ( idx i = 1; i + j <= N - 1; ++i )
+               {
+                  idx bw[3] = { N - i - j, i, j };
+                  T   r     = lowerApply( src, L, fSt[L][fi], pv, bw, 3 );
+                  auto& d   = dst.f[L][fi][fidx( w, i, j )];
+                  d         = add ? d + r : r;
+               }
+         }
+         if ( flag & Inner )
+         {
+            const idx nC = (idx) m->Cl.size();
+#pragma omp parallel for collapse( 2 ) schedule( dynamic, 4 )
+            for ( idx ci = 0; ci < nC; ++ci )
+               for ( idx z = 1; z <= N - 3; ++z )
+                  applyCellPlane( src.c[L][ci].data(), dst.c[L][ci].data(), cellSt[L][ci], L, z, add );
+         }
+      }
+   }
+
+   // generated kernel apply_3D_macrocell_vertexdof_to_vertexdof_{replace,add}.cpp:32-87, one z-plane:
+   // the 15 coefficients are hoisted, the row offsets computed once per (y, z) row.
+   static void applyCellPlane( const T* s, T* d, const std::array< double, NDIR >& st, int L, idx z, bool add )
+   {
+      const idx N = idx( 1 ) << L;
+      const idx w = N + 1;
+      T         c[NDIR];
+      for ( int k = 0; k < NDIR; ++k )
+         c[k] = T( st[k] );
+      for ( idx y = 1; y + z <= N - 2; ++y )
+      {
+         const T* rB0 = s + cidx( w, 0, y, z - 1 );     // (y,   z-1)
+         const T* rB1 = s + cidx( w, 0, y + 1, z - 1 ); // (y+1, z-1)
+         const T* rS  = s + cidx( w, 0, y - 1, z );     // (y-1, z)
+         const T* rC  = s + cidx( w, 0, y, z );         // (y,   z)
+         const T* rN  = s + cidx( w, 0, y + 1, z );     // (y+1, z)
+         const T* rTS = s + cidx( w, 0, y - 1, z + 1 ); // (y-1, z+1)
+         const T* rT  = s + cidx( w, 0, y, z + 1 );     // (y,   z+1)
+         T*       o   = d + cidx( w, 0, y, z );
+         for ( idx x = 1; x + y + z <= N - 1; ++x )
+         {
+            T r = c[V_C] * rC[x] + c[V_W] * rC[x - 1] + c[V_E] * rC[x + 1] + c[V_N] * rN[x] + c[V_S] * rS[x] + c[V_NW] * rN[x - 1] +
+                  c[V_SE] * rS[x + 1] + c[V_TC] * rT[x] + c[V_TW] * rT[x - 1] + c[V_TS] * rTS[x] + c[V_TSE] * rTS[x + 1] +
+                  c[V_BC] * rB0[x] + c[V_BN] * rB1[x] + c[V_BE] * rB0[x + 1] + c[V_BNW] * rB1[x - 1];
+            o[x] = add ? o[x] + r : r;
+         }
+      }
+   }
+
+   // computeDiagonalOperatorValues(invert=true), P1Operator.hpp:645-915 + invertElementwise
+   void computeInverseDiagonal( int L )
+   {
+      const idx N = idx( 1 ) << L;
+      const idx w = N + 1;
+      auto&     D = *invDiag;
+      for ( size_t vi = 0; vi < m->X.size(); ++vi )
+      {
+         double s = 0;
+         for ( auto& ww : vSt[L][vi].w )
+            s += ww[V_C];
+         D.v[L][vi][0] = T( 1.0 / s );
+      }
+      if ( L >= 1 )
+         for ( size_t ei = 0; ei < m->E.size(); ++ei )
+         {
+            double s = 0;
+            for ( auto& ww : eSt[L][ei].w )
+               s += ww[V_C];
+            for ( idx i = 1; i <= N - 1; ++i )
+               D.e[L][ei][i] = T( 1.0 / s );
+         }
+      if ( L >= 2 )
+      {
+         for ( size_t fi = 0; fi < m->F.size(); ++fi )
+         {
+            double s = 0;
+            for ( auto& ww : fSt[L][fi].w )
+               s += ww[V_C];
+            for ( idx j = 1; j <= N - 2; ++j )
+               for ( idx i = 1; i + j <= N - 1; ++i )
+                  D.f[L][fi][fidx( w, i, j )] = T( 1.0 / s );
+         }
+         for ( size_t ci = 0; ci < m->Cl.size(); ++ci )
+            for ( idx z = 1; z <= N - 3; ++z )
+               for ( idx y = 1; y + z <= N - 2; ++y )
+                  for ( idx x = 1; x + y + z <= N - 1; ++x )
+                     D.c[L][ci][cidx( w, x, y, z )] = T( 1.0 / cellSt[L][ci][V_C] );
+      }
+   }
+};
+
+// ---------------------------------------------------------------- vector operations (owned points, flag-filtered)
+template < typename T, typename Fn >
+void forOwned( const Mesh& m, int L, int flag, Fn&& fn )
+{
+   // fn(kind, prim, arrayIndex)
+   const idx N = idx( 1 ) << L;
+   const idx w = N + 1;
+   for ( size_t vi = 0; vi < m.X.size(); ++vi )
+      if ( bcType( m.vB[vi] ) & flag )
+         fn( 0, (int) vi, 0 );
+   for ( size_t ei = 0; ei < m.E.size(); ++ei )
+      if ( bcType( m.eB[ei] ) & flag )
+         for ( idx i = 1; i <= N - 1; ++i )
+            fn( 1, (int) ei, i );
+   for ( size_t fi = 0; fi < m.F.size(); ++fi )
+      if ( bcType( m.fB[fi] ) & flag )
+         for ( idx j = 1; j <= N - 2; ++j )
+            for ( idx i = 1; i + j <= N - 1; ++i )
+               fn( 2, (int) fi, fidx( w, i, j ) );
+   if ( flag & Inner )
+      for ( size_t ci = 0; ci < m.Cl.size(); ++ci )
+         for ( idx z = 1; z <= N - 3; ++z )
+            for ( idx y = 1; y + z <= N - 2; ++y )
+               for ( idx x = 1; x + y + z <= N - 1; ++x )
+                  fn( 3, (int) ci, cidx( w, x, y, z ) );
+}
+
+template < typename T >
+inline T& at( Function< T >& f, int L, int kind, int p, idx i )
+{
+   switch ( kind )
+   {
+   case 0:
+      return f.v[L][p][i];
+   case 1:
+      return f.e[L][p][i];
+   case 2:
+      return f.f[L][p][i];
+   default:
+      return f.c[L][p][i];
+   }
+}
+
+template < typename T >
+void assign( Function< T >& dst, const std::vector< double >& s, const std::vector< const Function< T >* >& src, int L, int flag )
+{
+   forOwned< T >( *dst.m, L, flag, [&]( int kind, int p, idx i ) {
+      T r = T( 0 );
+      for ( size_t k = 0; k < s.size(); ++k )
+         r += T( s[k] ) * at( const_cast< Function< T >& >( *src[k] ), L, kind, p, i );
+      at( dst, L, kind, p, i ) = r;
+   } );
+}
+
+template < typename T >
+void addScaled( Function< T >& dst, double s, const Function< T >& src, int L, int flag )
+{
+   forOwned< T >( *dst.m, L, flag, [&]( int kind, int p, idx i ) {
+      at( dst, L, kind, p, i ) += T( s ) * at( const_cast< Function< T >& >( src ), L, kind, p, i );
+   } );
+}
+
+template < typename T >
+void multElementwise( Function< T >& dst, const Function< T >& a, const Function< T >& b, int L, int flag )
+{
+   forOwned< T >( *dst.m, L, flag, [&]( int kind, int p, idx i ) {
+      at( dst, L, kind, p, i ) = at( const_cast< Function< T >& >( a ), L, kind, p, i ) *
+                                 at( const_cast< Function< T >& >( b ), L, kind, p, i );
+   } );
+}
+
+template < typename T >
+void interpolate( Function< T >& dst, double val, int L, int flag )
+{
+   forOwned< T >( *dst.m, L, flag, [&]( int kind, int p, idx i ) { at( dst, L, kind, p, i ) = T( val ); } );
+}
+
+template < typename T >
+double dot( const Function< T >& a, const Function< T >& b, int L, int flag )
+{
+   T s = T( 0 );
+   forOwned< T >( *a.m, L, flag, [&]( int kind, int p, idx i ) {
+      s += at( const_cast< Function< T >& >( a ), L, kind, p, i ) * at( const_cast< Function< T >& >( b ), L, kind, p, i );
+   } );
+   return double( s );
+}
+
+template < typename T >
+void copyFrom( Function< T >& dst, const Function< double >& src, int L )
+{
+   forOwned< T >( *dst.m, L, All, [&]( int kind, int p, idx i ) {
+      at( dst, L, kind, p, i ) = T( at( const_cast< Function< double >& >( src ), L, kind, p, i ) );
+   } );
+}
+
+template < typename T >
+void copyTo( Function< double >& dst, const Function< T >& src, int L )
+{
+   forOwned< T >( *dst.m, L, All, [&]( int kind, int p, idx i ) {
+      at( dst, L, kind, p, i ) = double( at( const_cast< Function< T >& >( src ), L, kind, p, i ) );
+   } );
+}
+
+// ---------------------------------------------------------------- grid transfer
+// classification of a cell-local point: returns (kind, local id): kind 3 interior, 2 face, 1 edge, 0 vertex
+inline std::pair< int, int > classify( idx N, idx x, idx y, idx z )
+{
+   idx b[4] = { N - x - y - z, x, y, z };
+   int nz = 0, zmask = 0;
+   for ( int k = 0; k < 4; ++k )
+      if ( b[k] == 0 )
+      {
+         ++nz;
+         zmask |= 1 << k;
+      }
+   if ( nz == 0 )
+      return { 3, 0 };
+   if ( nz == 1 )
+   {
+      // the face opposite to the zero coordinate: b0=0 -> face3 (1,2,3), x=0 -> face2 (0,2,3), y=0 -> face1, z=0 -> face0
+      int zk = __builtin_ctz( zmask );
+      return { 2, 3 - zk };
+   }
+   if ( nz == 2 )
+   {
+      int nzm = ( ~zmask ) & 15;
+      int a   = __builtin_ctz( nzm );
+      int bb  = 31 - __builtin_clz( nzm );
+      for ( int k = 0; k < 6; ++k )
+         if ( CELL_EDGE_VERTS[k][0] == a && CELL_EDGE_VERTS[k][1] == bb )
+            return { 1, k };
+   }
+   int nzm = ( ~zmask ) & 15;
+   return { 0, __builtin_ctz( nzm ) };
+}
+
+inline double invNumCells( const Mesh& m, int c, const std::pair< int, int >& cl )
+{
+   switch ( cl.first )
+   {
+   case 3:
+      return 1.0;
+   case 2:
+      return 1.0 / double( m.fC[m.cF[c][cl.second]].size() );
+   case 1:
+      return 1.0 / double( m.eC[m.cE[c][cl.second]].size() );
+   default:
+      return 1.0 / double( m.vC[m.Cl[c][cl.second]].size() );
+   }
+}
+
+// additive communication Cell -> {Vertex, Edge, Face} (VertexDoFAdditivePackInfo.hpp:378-571):
+// lower-dim owners receive the sum of the cells' boundary values; excluded primitives are skipped.
+template < typename T >
+void commAdditiveCellToLower( Function< T >& fn, int L, int excludeFlag, bool zero )
+{
+   const Mesh& m = *fn.m;
+   const idx   N = idx( 1 ) << L;
+   const idx   w = N + 1;
+   for ( size_t vi = 0; vi < m.X.size(); ++vi )
+   {
+      if ( bcType( m.vB[vi] ) & excludeFlag )
+         continue;
+      T   s     = zero ? T( 0 ) : fn.v[L][vi][0];
+      int pv[1] = { (int) vi };
+      idx bw[1] = { N };
+      for ( int c : m.vC[vi] )
+      {
+         I3 q = cellCoords( m, c, pv, bw, 1 );
+         s += fn.c[L][c][cidx( w, q[0], q[1], q[2] )];
+      }
+      fn.v[L][vi][0] = s;
+   }
+   for ( size_t ei = 0; ei < m.E.size(); ++ei )
+   {
+      if ( bcType( m.eB[ei] ) & excludeFlag )
+         continue;
+      int pv[2] = { m.E[ei][0], m.E[ei][1] };
+      for ( idx i = 1; i <= N - 1; ++i )
+      {
+         T   s     = zero ? T( 0 ) : fn.e[L][ei][i];
+         idx bw[2] = { N - i, i };
+         for ( int c : m.eC[ei] )
+         {
+            I3 q = cellCoords( m, c, pv, bw, 2 );
+            s += fn.c[L][c][cidx( w, q[0], q[1], q[2] )];
+         }
+         fn.e[L][ei][i] = s;
+      }
+   }
+   for ( size_t fi = 0; fi < m.F.size(); ++fi )
+   {
+      if ( bcType( m.fB[fi] ) & excludeFlag )
+         continue;
+      int pv[3] = { m.F[fi][0], m.F[fi][1], m.F[fi][2] };
+      for ( idx j = 1; j <= N - 2; ++j )
+         for ( idx i = 1; i + j <= N - 1; ++i )
+         {
+            T&  d     = fn.f[L][fi][fidx( w, i, j )];
+            T   s     = zero ? T( 0 ) : d;
+            idx bw[3] = { N - i - j, i, j };
+            for ( int c : m.fC[fi] )
+            {
+               I3 q = cellCoords( m, c, pv, bw, 3 );
+               s += fn.c[L][c][cidx( w, q[0], q[1], q[2] )];
+            }
+            d = s;
+         }
+   }
+}
+
+// P1toP1LinearRestriction::restrict3D with the generated kernel's per-leaf scaling
+template < typename T >
+void restrict3D( Function< T >& fn, int srcL, int flag )
+{
+   const Mesh& m  = *fn.m;
+   const int   dL = srcL - 1;
+   fn.commDown( srcL );
+   const idx Nc = idx( 1 ) << dL, wc = Nc + 1;
+   const idx Nf = idx( 1 ) << srcL, wf = Nf + 1;
+#pragma omp parallel for schedule( dynamic )
+   for ( idx ci = 0; ci < (idx) m.Cl.size(); ++ci )
+   {
+      const auto& src = fn.c[srcL][ci];
+      auto&       dst = fn.c[dL][ci];
+      for ( idx z = 0; z <= Nc; ++z )
+         for ( idx y = 0; y + z <= Nc; ++y )
+            for ( idx x = 0; x + y + z <= Nc; ++x )
+            {
+               T   r  = T( 0 );
+               idx fx = 2 * x, fy = 2 * y, fz = 2 * z;
+               for ( int d = 0; d < NDIR; ++d )
+               {
+                  idx lx = fx + DOFF[d][0], ly = fy + DOFF[d][1], lz = fz + DOFF[d][2];
+                  if ( !inCell( Nf, lx, ly, lz ) )
+                     continue;
+                  double wgt = ( d == V_C ? 1.0 : 0.5 ) * invNumCells( m, (int) ci, classify( Nf, lx, ly, lz ) );
+                  r += T( wgt ) * src[cidx( wf, lx, ly, lz )];
+               }
+               dst[cidx( wc, x, y, z )] = r;
+            }
+   }
+   commAdditiveCellToLower( fn, dL, flag ^ All, true );
+}
+
+// P1toP1LinearProlongation::prolongate3DAdditively
+template < typename T >
+void prolongate3D( Function< T >& fn, int srcL, int flag, bool add )
+{
+   const Mesh& m  = *fn.m;
+   const int   dL = srcL + 1;
+   fn.commDown( srcL );
+   const idx Nc = idx( 1 ) << srcL, wc = Nc + 1;
+   const idx Nf = idx( 1 ) << dL, wf = Nf + 1;
+#pragma omp parallel for schedule( dynamic )
+   for ( idx ci = 0; ci < (idx) m.Cl.size(); ++ci )
+   {
+      const auto& src = fn.c[srcL][ci];
+      auto&       dst = fn.c[dL][ci];
+      for ( idx z = 0; z <= Nf; ++z )
+         for ( idx y = 0; y + z <= Nf; ++y )
+            for ( idx x = 0; x + y + z <= Nf; ++x )
+            {
+               bool onFace = x == 0 || y == 0 || z == 0 || x + y + z == Nf;
+               if ( !add || onFace )
+                  dst[cidx( wf, x, y, z )] = T( 0 );
+            }
+      for ( idx z = 0; z <= Nc; ++z )
+         for ( idx y = 0; y + z <= Nc; ++y )
+            for ( idx x = 0; x + y + z <= Nc; ++x )
+            {
+               const T   v  = src[cidx( wc, x, y, z )];
+               const idx fx = 2 * x, fy = 2 * y, fz = 2 * z;
+               for ( int d = 0; d < NDIR; ++d )
+               {
+                  idx lx = fx + DOFF[d][0], ly = fy + DOFF[d][1], lz = fz + DOFF[d][2];
+                  if ( !inCell( Nf, lx, ly, lz ) )
+                     continue;
+                  double wgt = ( d == V_C ? 1.0 : 0.5 ) * invNumCells( m, (int) ci, classify( Nf, lx, ly, lz ) );
+                  dst[cidx( wf, lx, ly, lz )] += T( wgt ) * v;
+               }
+            }
+   }
+   commAdditiveCellToLower( fn, dL, flag ^ All, !add );
+}
+
+// ---------------------------------------------------------------- smoothers and solvers
+template < typename T >
+struct Solvers
+{
+   const Mesh*                       m;
+   const P1ConstantOperator< T >*    A;
+   int                               minL, maxL;
+   std::unique_ptr< Function< T > >  tmp, jtmp, ct1, ct2, cgp, cgz, cgap, cgr;
+   std::vector< double >             chebCoeffs;
+   int                               smoother = 0; // 0 = weighted Jacobi, 1 = Chebyshev
+   double                            relax    = 2.0 / 3.0;
+   int                               pre = 2, post = 2, inc = 0;
+   int                               cgMaxIter = 10000;
+   double                            cgTol     = 1e-16;
+   const int                         gmgFlag   = Inner | NeumannBoundary;
+
+   Solvers( const Mesh* mesh, const P1ConstantOperator< T >* op, int lo, int hi )
+   : m( mesh )
+   , A( op )
+   , minL( lo )
+   , maxL( hi )
+   {
+      tmp  = std::make_unique< Function< T > >( m, lo, hi );
+      jtmp = std::make_unique< Function< T > >( m, lo, hi );
+      ct1  = std::make_unique< Function< T > >( m, lo, hi );
+      ct2  = std::make_unique< Function< T > >( m, lo, hi );
+      cgp  = std::make_unique< Function< T > >( m, lo, hi );
+      cgz  = std::make_unique< Function< T > >( m, lo, hi );
+      cgap = std::make_unique< Function< T > >( m, lo, hi );
+      cgr  = std::make_unique< Function< T > >( m, lo, hi );
+   }
+
+   // P1Operator::smooth_jac
+   void smoothJac( Function< T >& dst, const Function< T >& rhs, const Function< T >& src, double w, int L, int flag ) const
+   {
+      A->apply( src, dst, L, flag, false );
+      assign< T >( dst, { 1.0, -1.0 }, { &rhs, &dst }, L, flag );
+      multElementwise< T >( dst, *A->invDiag, dst, L, flag );
+      assign< T >( dst, { 1.0, w }, { &src, &dst }, L, flag );
+   }
+   void jacobiSolve( Function< T >& x, const Function< T >& b, int L )
+   {
+      assign< T >( *jtmp, { 1.0 }, { &x }, L, All );
+      smoothJac( x, b, *jtmp, relax, L, gmgFlag );
+   }
+   // ChebyshevSmoother::solve
+   void chebyshevSolve( Function< T >& x, const Function< T >& b, int L )
+   {
+      const int flag = gmgFlag;
+      A->apply( x, *ct1, L, flag, false );
+      assign< T >( *ct1, { 1.0, -1.0 }, { &b, ct1.get() }, L, flag );
+      multElementwise< T >( *ct1, *A->invDiag, *ct1, L, flag );
+      assign< T >( x, { 1.0, chebCoeffs[0] }, { &x, ct1.get() }, L, flag );
+      for ( size_t k = 1; k < chebCoeffs.size(); ++k )
+      {
+         A->apply( *ct1, *ct2, L, flag, false );
+         multElementwise< T >( *ct1, *A->invDiag, *ct2, L, flag );
+         assign< T >( x, { 1.0, chebCoeffs[k] }, { &x, ct1.get() }, L, flag );
+      }
+   }
+   void smooth( Function< T >& x, const Function< T >& b, int L )
+   {
+      if ( smoother == 0 )
+         jacobiSolve( x, b, L );
+      else
+         chebyshevSolve( x, b, L );
+   }
+   // CGSolver::solve with the identity preconditioner
+   int cgSolve( Function< T >& x, const Function< T >& b, int L )
+   {
+      const int flag = Inner | NeumannBoundary | FreeslipBoundary;
+      A->apply( x, *cgp, L, flag, false );
+      assign< T >( *cgr, { 1.0, -1.0 }, { &b, cgp.get() }, L, flag );
+      assign< T >( *cgz, { 1.0 }, { cgr.get() }, L, flag );
+      assign< T >( *cgp, { 1.0 }, { cgz.get() }, L, flag );
+      double prsold = dot< T >( *cgr, *cgz, L, flag );
+      double res0   = std::sqrt( dot< T >( *cgr, *cgr, L, flag ) );
+      if ( res0 < cgTol )
+         return 0;
+      int it = 0;
+      for ( ; it < cgMaxIter; ++it )
+      {
+         A->apply( *cgp, *cgap, L, flag, false );
+         double pAp   = dot< T >( *cgp, *cgap, L, flag );
+         double alpha = prsold / pAp;
+         addScaled< T >( x, alpha, *cgp, L, flag );
+         addScaled< T >( *cgr, -alpha, *cgap, L, flag );
+         double rsnew = dot< T >( *cgr, *cgr, L, flag );
+         if ( std::sqrt( rsnew ) < cgTol )
+            break;
+         assign< T >( *cgz, { 1.0 }, { cgr.get() }, L, flag );
+         double prsnew = dot< T >( *cgr, *cgz, L, flag );
+         double beta   = prsnew / prsold;
+         assign< T >( *cgp, { 1.0, beta }, { cgz.get(), cgp.get() }, L, flag );
+         prsold = prsnew;
+      }
+      return it + 1;
+   }
+   // GeometricMultigridSolver::solveRecursively (V-cycle)
+   void vcycle( Function< T >& x, Function< T >& b, int L, int invoked )
+   {
+      if ( L == minL )
+      {
+         cgSolve( x, b, L );
+         return;
+      }
+      const int preS = pre + inc * ( invoked - L );
+      for ( int i = 0; i < preS; ++i )
+         smooth( x, b, L );
+      A->apply( x, *tmp, L, gmgFlag, false );
+      assign< T >( *tmp, { 1.0, -1.0 }, { &b, tmp.get() }, L, gmgFlag );
+      restrict3D< T >( *tmp, L, gmgFlag );
+      assign< T >( b, { 1.0 }, { tmp.get() }, L - 1, gmgFlag );
+      interpolate< T >( x, 0.0, L - 1, All );
+      vcycle( x, b, L - 1, invoked );
+      prolongate3D< T >( x, L - 1, gmgFlag, true );
+      const int postS = post + inc * ( invoked - L );
+      for ( int i = 0; i < postS; ++i )
+         smooth( x, b, L );
+   }
+   // relative residual ||b - Ax|| over Inner DoFs
+   double residualNorm( const Function< T >& x, const Function< T >& b, int L )
+   {
+      A->apply( x, *tmp, L, gmgFlag, false );
+      assign< T >( *tmp, { 1.0, -1.0 }, { &b, tmp.get() }, L, gmgFlag );
+      return std::sqrt( dot< T >( *tmp, *tmp, L, gmgFlag ) );
+   }
+};
+
+// ChebyshevSmoother::setupCoefficients (order 1..5, mfem form)
+inline int chebCoefficients( int order, double lower, double upper, double* c )
+{
+   const double theta = 0.5 * ( upper + lower );
+   const double delta = 0.5 * ( upper - lower );
+   switch ( order - 1 )
+   {
+   case 0:
+      c[0] = 1.0 / theta;
+      return 0;
+   case 1: {
+      const double t0 = 1.0 / ( std::pow( delta, 2 ) - 2 * std::pow( theta, 2 ) );
+      c[0]            = -4 * theta * t0;
+      c[1]            = 2 * t0;
+      return 0;
+   }
+   case 2: {
+      const double t0 = 3 * std::pow( delta, 2 );
+      const double t1 = std::pow( theta, 2 );
+      const double t2 = 1.0 / ( -4 * std::pow( theta, 3 ) + theta * t0 );
+      c[0]            = t2 * ( t0 - 12 * t1 );
+      c[1]            = 12 / ( t0 - 4 * t1 );
+      c[2]            = -4 * t2;
+      return 0;
+   }
+   case 3: {
+      const double t0 = std::pow( delta, 2 );
+      const double t1 = std::pow( theta, 2 );
+      const double t2 = 8 * t0;
+      const double t3 = 1.0 / ( std::pow( delta, 4 ) + 8 * std::pow( theta, 4 ) - t1 * t2 );
+      c[0]            = t3 * ( 32 * std::pow( theta, 3 ) - 16 * theta * t0 );
+      c[1]            = t3 * ( -48 * t1 + t2 );
+      c[2]            = 32 * theta * t3;
+      c[3]            = -8 * t3;
+      return 0;
+   }
+   case 4: {
+      const double t0 = 5 * std::pow( delta, 4 );
+      const double t1 = std::pow( theta, 4 );
+      const double t2 = std::pow( theta, 2 );
+      const double t3 = std::pow( delta, 2 );
+      const double t4 = 60 * t3;
+      const double t5 = 20 * t3;
+      const double t6 = 1.0 / ( 16 * std::pow( theta, 5 ) - std::pow( theta, 3 ) * t5 + theta * t0 );
+      const double t7 = 160 * t2;
+      const double t8 = 1.0 / ( t0 + 16 * t1 - t2 * t5 );
+      c[0]            = t6 * ( t0 + 80 * t1 - t2 * t4 );
+      c[1]            = t8 * ( t4 - t7 );
+      c[2]            = t6 * ( -t5 + t7 );
+      c[3]            = -80 * t8;
+      c[4]            = 16 * t6;
+      return 0;
+   }
+   default:
+      return -1;
+   }
+}
+
+} // namespace orc
+
+// ================================================================ C ABI for ctypes (tests / bench only)
+using namespace orc;
+
+struct OrFunc
+{
+   int                                   prec; // 8 or 4
+   std::unique_ptr< Function< double > > d;
+   std::unique_ptr< Function< float > >  f;
+};
+
+struct OrOp
+{
+   int                                             prec;
+   const Mesh*                                     m;
+   std::unique_ptr< P1ConstantOperator< double > > d;
+   std::unique_ptr< P1ConstantOperator< float > >  f;
+};
+
+template < typename T >
+static Function< T >& fget( OrFunc* h );
+template <>
+Function< double >& fget< double >( OrFunc* h )
+{
+   return *h->d;
+}
+template <>
+Function< float >& fget< float >( OrFunc* h )
+{
+   return *h->f;
+}
+
+template < typename T >
+static void setCanonical( Function< T >& fn, int L, const double* in )
+{
+   const Mesh& m = *fn.m;
+   const idx   w = ( idx( 1 ) << L ) + 1;
+   forEachDof< T >( m, L, [&]( int kind, int p, idx i, idx j, idx k, idx n ) {
+      switch ( kind )
+      {
+      case 0:
+         fn.v[L][p][0] = T( in[n] );
+         break;
+      case 1:
+         fn.e[L][p][i] = T( in[n] );
+         break;
+      case 2:
+         fn.f[L][p][fidx( w, i, j )] = T( in[n] );
+         break;
+      default:
+         fn.c[L][p][cidx( w, i, j, k )] = T( in[n] );
+      }
+   } );
+   fn.commDown( L );
+}
+
+template < typename T >
+static void getCanonical( const Function< T >& fn, int L, double* out )
+{
+   const Mesh& m = *fn.m;
+   const idx   w = ( idx( 1 ) << L ) + 1;
+   forEachDof< T >( m, L, [&]( int kind, int p, idx i, idx j, idx k, idx n ) {
+      switch ( kind )
+      {
+      case 0:
+         out[n] = double( fn.v[L][p][0] );
+         break;
+      case 1:
+         out[n] = double( fn.e[L][p][i] );
+         break;
+      case 2:
+         out[n] = double( fn.f[L][p][fidx( w, i, j )] );
+         break;
+      default:
+         out[n] = double( fn.c[L][p][cidx( w, i, j, k )] );
+      }
+   } );
+}
+
+// GMG parameters: ip = {minL, maxL, pre, post, inc, smoother(0 jac,1 cheb), chebOrder, cgMaxIter}
+//                 dp = {relax, chebLower, chebUpper, cgTol}
+// Runs `cycles` V-cycles on x with rhs b at level maxL; res[0] = initial residual norm, res[k] after cycle k.
+template < typename T >
+static void gmgRun( OrOp* o, P1ConstantOperator< T >* A, Function< T >& x, Function< T >& b, const int* ip, const double* dp,
+                    int cycles, double* res )
+{
+   Solvers< T > S( o->m, A, ip[0], ip[1] );
+   S.pre       = ip[2];
+   S.post      = ip[3];
+   S.inc       = ip[4];
+   S.smoother  = ip[5];
+   S.cgMaxIter = ip[7];
+   S.relax     = dp[0];
+   S.cgTol     = dp[3];
+   if ( S.smoother == 1 )
+   {
+      S.chebCoeffs.resize( ip[6] );
+      chebCoefficients( ip[6], dp[1], dp[2], S.chebCoeffs.data() );
+   }
+   const int L = ip[1];
+   res[0]      = S.residualNorm( x, b, L );
+   for ( int c = 0; c < cycles; ++c )
+   {
+      S.vcycle( x, b, L, L );
+      res[c + 1] = S.residualNorm( x, b, L );
+   }
+}
+
+extern "C" {
+
+void* or_mesh_load( const char* path ) { return loadGmsh( path ).release(); }
+void  or_mesh_free( void* m ) { delete static_cast< Mesh* >( m ); }
+
+// V, E, F, C, boundary V, boundary E, boundary F
+void or_mesh_info( void* mh, int64_t* out )
+{
+   auto* m = static_cast< Mesh* >( mh );
+   out[0]  = (int64_t) m->X.size();
+   out[1]  = (int64_t) m->E.size();
+   out[2]  = (int64_t) m->F.size();
+   out[3]  = (int64_t) m->Cl.size();
+   out[4]  = std::count( m->vB.begin(), m->vB.end(), 1 );
+   out[5]  = std::count( m->eB.begin(), m->eB.end(), 1 );
+   out[6]  = std::count( m->fB.begin(), m->fB.end(), 1 );
+}
+
+int64_t or_ndofs( void* mh, int L ) { return numDofs( *static_cast< Mesh* >( mh ), L ); }
+
+// coordinates (3 per DoF) and boundary marker (1 = Dirichlet) in canonical order
+void or_dof_coords( void* mh, int L, double* xyz, uint8_t* bnd )
+{
+   auto&     m = *static_cast< Mesh* >( mh );
+   const idx N = idx( 1 ) << L;
+   forEachDof< double >( m, L, [&]( int kind, int p, idx i, idx j, idx k, idx n ) {
+      P3  x{};
+      int b = 0;
+      if ( kind == 0 )
+      {
+         x = m.X[p];
+         b = m.vB[p];
+      }
+      else if ( kind == 1 )
+      {
+         const P3 &a = m.X[m.E[p][0]], &c = m.X[m.E[p][1]];
+         for ( int r = 0; r < 3; ++r )
+            x[r] = a[r] + ( c[r] - a[r] ) * double( i ) / double( N );
+         b = m.eB[p];
+      }
+      else if ( kind == 2 )
+      {
+         const P3 &a = m.X[m.F[p][0]], &c1 = m.X[m.F[p][1]], &c2 = m.X[m.F[p][2]];
+         for ( int r = 0; r < 3; ++r )
+            x[r] = a[r] + ( c1[r] - a[r] ) * double( i ) / double( N ) + ( c2[r] - a[r] ) * double( j ) / double( N );
+         b = m.fB[p];
+      }
+      else
+      {
+         x = microCoord( m, p, L, i, j, k );
+         b = 0;
+      }
+      xyz[3 * n + 0] = x[0];
+      xyz[3 * n + 1] = x[1];
+      xyz[3 * n + 2] = x[2];
+      if ( bnd )
+         bnd[n] = (uint8_t) b;
+   } );
+}
+
+void* or_func_new( void* mh, int minL, int maxL, int prec )
+{
+   auto* h = new OrFunc;
+   h->prec = prec;
+   if ( prec == 4 )
+      h->f = std::make_unique< Function< float > >( static_cast< Mesh* >( mh ), minL, maxL );
+   else
+      h->d = std::make_unique< Function< double > >( static_cast< Mesh* >( mh ), minL, maxL );
+   return h;
+}
+void or_func_free( void* h ) { delete static_cast< OrFunc* >( h ); }
+
+void or_func_set( void* fh, int L, const double* canon )
+{
+   auto* h = static_cast< OrFunc* >( fh );
+   if ( h->prec == 4 )
+      setCanonical( *h->f, L, canon );
+   else
+      setCanonical( *h->d, L, canon );
+}
+void or_func_get( void* fh, int L, double* canon )
+{
+   auto* h = static_cast< OrFunc* >( fh );
+   if ( h->prec == 4 )
+      getCanonical( *h->f, L, canon );
+   else
+      getCanonical( *h->d, L, canon );
+}
+
+void* or_op_new( void* mh, int minL, int maxL, int prec )
+{
+   auto* o = new OrOp;
+   o->prec = prec;
+   o->m    = static_cast< Mesh* >( mh );
+   if ( prec == 4 )
+      o->f = std::make_unique< P1ConstantOperator< float > >( o->m, minL, maxL );
+   else
+      o->d = std::make_unique< P1ConstantOperator< double > >( o->m, minL, maxL );
+   return o;
+}
+void or_op_free( void* o ) { delete static_cast< OrOp* >( o ); }
+
+// cell stencil (15 weights, C W E N S NW SE TC TW TS TSE BC BN BE BNW) of cell c at level L
+void or_op_cell_stencil( void* oh, int L, int c, double* out )
+{
+   auto* o = static_cast< OrOp* >( oh );
+   auto& s = o->prec == 4 ? o->f->cellSt[L][c] : o->d->cellSt[L][c];
+   for ( int k = 0; k < NDIR; ++k )
+      out[k] = s[k];
+}
+
+void or_apply( void* oh, void* src, void* dst, int L, int flag, int add )
+{
+   auto* o = static_cast< OrOp* >( oh );
+   if ( o->prec == 4 )
+      o->f->apply( fget< float >( (OrFunc*) src ), fget< float >( (OrFunc*) dst ), L, flag, add != 0 );
+   else
+      o->d->apply( fget< double >( (OrFunc*) src ), fget< double >( (OrFunc*) dst ), L, flag, add != 0 );
+}
+
+void or_inv_diag( void* oh, int L, double* canon )
+{
+   auto* o = static_cast< OrOp* >( oh );
+   if ( o->prec == 4 )
+      getCanonical( *o->f->invDiag, L, canon );
+   else
+      getCanonical( *o->d->invDiag, L, canon );
+}
+
+void or_smooth_jac( void* oh, void* dst, void* rhs, void* src, double relax, int L, int flag )
+{
+   auto* o = static_cast< OrOp* >( oh );
+   if ( o->prec == 4 )
+   {
+      Solvers< float > s( o->m, o->f.get(), L, L );
+      s.smoothJac( fget< float >( (OrFunc*) dst ), fget< float >( (OrFunc*) rhs ), fget< float >( (OrFunc*) src ), relax, L, flag );
+   }
+   else
+   {
+      Solvers< double > s( o->m, o->d.get(), L, L );
+      s.smoothJac( fget< double >( (OrFunc*) dst ), fget< double >( (OrFunc*) rhs ), fget< double >( (OrFunc*) src ), relax, L, flag );
+   }
+}
+
+void or_restrict( void* fh, int srcL, int flag )
+{
+   auto* h = static_cast< OrFunc* >( fh );
+   if ( h->prec == 4 )
+      restrict3D( *h->f, srcL, flag );
+   else
+      restrict3D( *h->d, srcL, flag );
+}
+
+void or_prolongate( void* fh, int srcL, int flag, int add )
+{
+   auto* h = static_cast< OrFunc* >( fh );
+   if ( h->prec == 4 )
+      prolongate3D( *h->f, srcL, flag, add != 0 );
+   else
+      prolongate3D( *h->d, srcL, flag, add != 0 );
+}
+
+double or_dot( void* a, void* b, int L, int flag )
+{
+   auto* ha = static_cast< OrFunc* >( a );
+   if ( ha->prec == 4 )
+      return dot( *ha->f, fget< float >( (OrFunc*) b ), L, flag );
+   return dot( *ha->d, fget< double >( (OrFunc*) b ), L, flag );
+}
+
+int or_cheb_coefficients( int order, double lower, double upper, double* out ) { return chebCoefficients( order, lower, upper, out ); }
+
+// estimateSpectralRadiusWithPowerIteration on D^-1 A (ChebyshevSmoother.hpp:255-265, SpectrumEstimation.hpp:57-80);
+// x holds the start vector and is overwritten.
+double or_power_iteration( void* oh, void* xh, int iters, int L )
+{
+   auto*              o = static_cast< OrOp* >( oh );
+   auto&              A = *o->d;
+   Function< double >& x = fget< double >( (OrFunc*) xh );
+   Function< double >  aux( o->m, L, L );
+   auto               invDiagApply = [&]( const Function< double >& s, Function< double >& d ) {
+      A.apply( s, d, L, All, false );
+      multElementwise< double >( d, *A.invDiag, d, L, All );
+   };
+   double norm = std::sqrt( dot( x, x, L, All ) );
+   assign< double >( x, { 1.0 / norm }, { &x }, L, All );
+   invDiagApply( x, aux );
+   double radius = 0.0;
+   for ( int it = 1; it <= iters; ++it )
+   {
+      norm = std::sqrt( dot( aux, aux, L, All ) );
+      assign< double >( x, { 1.0 / norm }, { &aux }, L, All );
+      invDiagApply( x, aux );
+      radius = dot( x, aux, L, All );
+   }
+   return radius;
+}
+
+void or_gmg( void* oh, void* xh, void* bh, const int* ip, const double* dp, int cycles, double* res )
+{
+   auto* o = static_cast< OrOp* >( oh );
+   if ( o->prec == 4 )
+      gmgRun< float >( o, o->f.get(), fget< float >( (OrFunc*) xh ), fget< float >( (OrFunc*) bh ), ip, dp, cycles, res );
+   else
+      gmgRun< double >( o, o->d.get(), fget< double >( (OrFunc*) xh ), fget< double >( (OrFunc*) bh ), ip, dp, cycles, res );
+}
+
+// Mixed-precision iterative refinement (mixedPrecisionIterativeRefinement.cpp:162-172): FP64 residual, FP32 V-cycles
+// on the correction, FP64 update. opd/ops are FP64/FP32 operators on the same mesh and levels.
+void or_ir( void* opd, void* ops, void* xh, void* bh, const int* ip, const double* dp, int outer, int inner, double* res )
+{
+   auto* od = static_cast< OrOp* >( opd );
+   auto* os = static_cast< OrOp* >( ops );
+   auto& x  = fget< double >( (OrFunc*) xh );
+   auto& b  = fget< double >( (OrFunc*) bh );
+   const int L = ip[1];
+   Solvers< double > Sd( od->m, od->d.get(), ip[0], ip[1] );
+   Solvers< float >  Ss( os->m, os->f.get(), ip[0], ip[1] );
+   Ss.pre = ip[2];
+   Ss.post = ip[3];
+   Ss.inc = ip[4];
+   Ss.smoother = ip[5];
+   Ss.cgMaxIter = ip[7];
+   Ss.relax = dp[0];
+   Ss.cgTol = dp[3];
+   if ( Ss.smoother == 1 )
+   {
+      Ss.chebCoeffs.resize( ip[6] );
+      chebCoefficients( ip[6], dp[1], dp[2], Ss.chebCoeffs.data() );
+   }
+   Function< double > r( od->m, ip[0], ip[1] ), t( od->m, ip[0], ip[1] );
+   Function< float >  rs( od->m, ip[0], ip[1] ), cs( od->m, ip[0], ip[1] );
+   const int flag = Inner | NeumannBoundary;
+   res[0] = Sd.residualNorm( x, b, L );
+   for ( int it = 0; it < outer; ++it )
+   {
+      od->d->apply( x, t, L, flag, false );
+      assign< double >( r, { 1.0, -1.0 }, { &b, &t }, L, flag );
+      copyFrom< float >( rs, r, L );
+      interpolate< float >( cs, 0.0, L, All );
+      for ( int k = 0; k < inner; ++k )
+         Ss.vcycle( cs, rs, L, L );
+      copyTo< float >( t, cs, L );
+      addScaled< double >( x, 1.0, t, L, flag );
+      res[it + 1] = Sd.residualNorm( x, b, L );
+   }
+}
+
+} // extern "C"

@@ -1,0 +1,189 @@
+"""Independent brute-force model of the P1 finite-element problem, used to pin the oracle.
+
+Nothing here shares code with the oracle or the HIP library. The refined mesh is built
+element by element from the Gmsh macro mesh, using the six micro-tetrahedron types of a
+uniformly refined macro-cell (reference: src/hyteg/volumedofspace/CellDoFIndexing.hpp:155-198),
+micro-vertices are identified globally by their exact coordinates, and the global P1
+stiffness matrix is assembled with scipy from the closed-form element matrix
+K_T = vol(T) * G G^T (G = gradients of the barycentric coordinates). Linear interpolation
+between levels is assembled from the coarse micro-edges (a fine vertex is either a coarse
+vertex or the midpoint of a coarse micro-edge).
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+# lattice offsets of the four vertices of the six micro-cell types (WHITE_UP, BLUE_UP,
+# GREEN_UP, WHITE_DOWN, BLUE_DOWN, GREEN_DOWN)
+MICRO_TETS = np.array([
+    [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)],
+    [(1, 0, 0), (1, 1, 0), (0, 1, 0), (1, 0, 1)],
+    [(1, 0, 0), (0, 1, 0), (1, 0, 1), (0, 0, 1)],
+    [(1, 1, 0), (1, 1, 1), (0, 1, 1), (1, 0, 1)],
+    [(1, 0, 1), (0, 1, 1), (0, 0, 1), (0, 1, 0)],
+    [(0, 1, 0), (1, 1, 0), (1, 0, 1), (0, 1, 1)],
+], dtype=np.int64)
+
+
+def read_gmsh(path):
+    """Nodes (dict id -> xyz) and tetrahedra (vertex id lists, Gmsh order) of a v2.2 file."""
+    with open(path) as fh:
+        tok = fh.read().split()
+    i = tok.index("$Nodes") + 1
+    nn = int(tok[i]); i += 1
+    nodes = {}
+    for _ in range(nn):
+        nodes[int(tok[i])] = np.array([float(t) for t in tok[i + 1:i + 4]])
+        i += 4
+    i = tok.index("$Elements") + 1
+    ne = int(tok[i]); i += 1
+    tets = []
+    nper = {15: 1, 1: 2, 2: 3, 4: 4}
+    for _ in range(ne):
+        typ, ntag = int(tok[i + 1]), int(tok[i + 2])
+        i += 3 + ntag
+        vs = [int(t) for t in tok[i:i + nper[typ]]]
+        i += nper[typ]
+        if typ == 4:
+            tets.append(vs)
+    return nodes, tets
+
+
+def _lattice(N):
+    pts = [(x, y, z) for z in range(N + 1) for y in range(N + 1 - z) for x in range(N + 1 - y - z)]
+    return np.array(pts, dtype=np.int64)
+
+
+class RefinedMesh:
+    """Uniformly refined macro mesh at level L with globally numbered micro-vertices."""
+
+    def __init__(self, path, L, scale_key=None):
+        nodes, tets = read_gmsh(path)
+        self.N = N = 1 << L
+        self.L = L
+        keyscale = 4 * N if scale_key is None else scale_key
+        allkeys = []
+        elems = []
+        off = 0
+        w = N + 1
+        lat = _lattice(N)
+        for cv in tets:
+            V = np.array([nodes[v] for v in cv])
+            xyz = V[0] + (lat @ (V[1:] - V[0])) / N
+            allkeys.append(np.rint(xyz * keyscale).astype(np.int64))
+            # micro-tets: for every type, every lattice base with all four vertices inside
+            for t in MICRO_TETS:
+                verts = lat[:, None, :] + t[None, :, :]
+                ok = np.all(verts.sum(axis=2) <= N, axis=1)
+                vv = verts[ok]
+                x, y, z = vv[..., 0], vv[..., 1], vv[..., 2]
+                s3 = lambda q: q * (q + 1) * (q + 2) // 6
+                ids = s3(w) - s3(w - z) + y * (w - z + 1) - y * (y + 1) // 2 + x
+                elems.append(ids + off)
+            off += len(lat)
+        keys = np.concatenate(allkeys)
+        uniq, inv = np.unique(keys, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        self.keys = uniq
+        self.xyz = uniq / keyscale
+        self.keyscale = keyscale
+        self.tets = inv[np.concatenate(elems)]
+        self.n = len(uniq)
+        # boundary micro-vertices: vertices of micro-faces that belong to one micro-tet only
+        faces = np.sort(np.concatenate([self.tets[:, [0, 1, 2]], self.tets[:, [0, 1, 3]],
+                                        self.tets[:, [0, 2, 3]], self.tets[:, [1, 2, 3]]]), axis=1)
+        uf, cnt = np.unique(faces, axis=0, return_counts=True)
+        self.boundary = np.zeros(self.n, dtype=bool)
+        self.boundary[uf[cnt == 1].reshape(-1)] = True
+
+    def key_index(self, xyz):
+        """Global indices of the given coordinates."""
+        k = np.rint(np.asarray(xyz) * self.keyscale).astype(np.int64)
+        lookup = {tuple(r): i for i, r in enumerate(self.keys)}
+        return np.array([lookup[tuple(r)] for r in k], dtype=np.int64)
+
+    def stiffness(self, kfun=None):
+        X = self.xyz[self.tets]                     # (ne, 4, 3)
+        J = (X[:, 1:, :] - X[:, :1, :])             # rows = edge vectors x_k - x_0
+        # lambda_k(x) = e_k . J^{-T} (x - x_0): the gradient of lambda_k is row k of J^{-1} transposed
+        Jinv = np.linalg.inv(J)
+        g = np.empty((len(X), 4, 3))
+        g[:, 1:, :] = Jinv.transpose(0, 2, 1)
+        g[:, 0, :] = -g[:, 1:, :].sum(axis=1)
+        vol = np.abs(np.linalg.det(J)) / 6.0
+        if kfun is not None:
+            vol = vol * kfun(X)
+        K = vol[:, None, None] * np.einsum("eik,ejk->eij", g, g)
+        rows = np.repeat(self.tets, 4, axis=1).reshape(-1)
+        cols = np.tile(self.tets, (1, 4)).reshape(-1)
+        return sp.csr_matrix((K.reshape(-1), (rows, cols)), shape=(self.n, self.n))
+
+
+def prolongation(coarse, fine):
+    """Linear interpolation matrix (fine.n x coarse.n)."""
+    # map coarse vertices into fine numbering via coordinates
+    cf = fine.key_index(coarse.xyz)
+    rows, cols, vals = list(cf), list(range(coarse.n)), [1.0] * coarse.n
+    edges = set()
+    for t in coarse.tets:
+        for a in range(4):
+            for b in range(a + 1, 4):
+                i, j = int(t[a]), int(t[b])
+                edges.add((min(i, j), max(i, j)))
+    edges = np.array(sorted(edges))
+    mids = 0.5 * (coarse.xyz[edges[:, 0]] + coarse.xyz[edges[:, 1]])
+    mf = fine.key_index(mids)
+    for (i, j), f in zip(edges, mf):
+        rows += [f, f]
+        cols += [i, j]
+        vals += [0.5, 0.5]
+    P = sp.csr_matrix((vals, (rows, cols)), shape=(fine.n, coarse.n))
+    assert np.allclose(np.asarray(P.sum(axis=1)).ravel(), 1.0)
+    return P
+
+
+class GMGModel:
+    """V-cycle on inner DoFs with global sparse matrices (zero Dirichlet boundary)."""
+
+    def __init__(self, path, minL, maxL):
+        self.minL, self.maxL = minL, maxL
+        self.mesh = {L: RefinedMesh(path, L) for L in range(minL, maxL + 1)}
+        self.inner = {L: np.flatnonzero(~m.boundary) for L, m in self.mesh.items()}
+        self.A = {}
+        self.P = {}
+        for L, m in self.mesh.items():
+            Af = m.stiffness()
+            ii = self.inner[L]
+            self.A[L] = Af[ii][:, ii].tocsr()
+        for L in range(minL + 1, maxL + 1):
+            P = prolongation(self.mesh[L - 1], self.mesh[L])
+            self.P[L] = P[self.inner[L]][:, self.inner[L - 1]].tocsr()
+
+    def jacobi(self, L, x, b, omega):
+        A = self.A[L]
+        return x + omega * (b - A @ x) / A.diagonal()
+
+    def chebyshev(self, L, x, b, coeffs):
+        A = self.A[L]
+        dinv = 1.0 / A.diagonal()
+        t = dinv * (b - A @ x)
+        x = x + coeffs[0] * t
+        for c in coeffs[1:]:
+            t = dinv * (A @ t)
+            x = x + c * t
+        return x
+
+    def vcycle(self, L, x, b, smooth, pre, post):
+        if L == self.minL:
+            if self.A[L].shape[0] == 0:
+                return x
+            return spla.spsolve(self.A[L].tocsc(), b) if self.A[L].shape[0] > 1 else b / self.A[L].toarray()[0]
+        for _ in range(pre):
+            x = smooth(L, x, b)
+        r = b - self.A[L] @ x
+        bc = self.P[L].T @ r
+        xc = self.vcycle(L - 1, np.zeros_like(bc), bc, smooth, pre, post)
+        x = x + self.P[L] @ xc
+        for _ in range(post):
+            x = smooth(L, x, b)
+        return x

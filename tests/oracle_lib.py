@@ -1,0 +1,141 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so). Test infrastructure only."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+MESHES = os.path.join(ROOT, "meshes")
+
+INNER, DIRICHLET, NEUMANN, FREESLIP, ALL = 1, 2, 4, 8, 15
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(os.path.join(ROOT, "oracle", "oracle.cpp")):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+        l = C.CDLL(LIB)
+        vp, i, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+        P = np.ctypeslib.ndpointer
+        sig = {
+            "or_mesh_load": (vp, [C.c_char_p]),
+            "or_mesh_free": (None, [vp]),
+            "or_mesh_info": (None, [vp, P(np.int64)]),
+            "or_ndofs": (i64, [vp, i]),
+            "or_dof_coords": (None, [vp, i, P(np.float64), P(np.uint8)]),
+            "or_func_new": (vp, [vp, i, i, i]),
+            "or_func_free": (None, [vp]),
+            "or_func_set": (None, [vp, i, P(np.float64)]),
+            "or_func_get": (None, [vp, i, P(np.float64)]),
+            "or_op_new": (vp, [vp, i, i, i]),
+            "or_op_free": (None, [vp]),
+            "or_op_cell_stencil": (None, [vp, i, i, P(np.float64)]),
+            "or_apply": (None, [vp, vp, vp, i, i, i]),
+            "or_inv_diag": (None, [vp, i, P(np.float64)]),
+            "or_smooth_jac": (None, [vp, vp, vp, vp, d, i, i]),
+            "or_restrict": (None, [vp, i, i]),
+            "or_prolongate": (None, [vp, i, i, i]),
+            "or_dot": (d, [vp, vp, i, i]),
+            "or_cheb_coefficients": (i, [i, d, d, P(np.float64)]),
+            "or_power_iteration": (d, [vp, vp, i, i]),
+            "or_gmg": (None, [vp, vp, vp, P(np.int32), P(np.float64), i, P(np.float64)]),
+            "or_ir": (None, [vp, vp, vp, vp, P(np.int32), P(np.float64), i, i, P(np.float64)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+class Mesh:
+    def __init__(self, name):
+        path = name if os.path.isabs(name) else os.path.join(MESHES, name)
+        self.path = path
+        self.h = lib().or_mesh_load(path.encode())
+        if not self.h:
+            raise RuntimeError("cannot load mesh " + path)
+
+    def info(self):
+        out = np.zeros(8, dtype=np.int64)
+        lib().or_mesh_info(self.h, out)
+        return out[:7]
+
+    def ndofs(self, L):
+        return lib().or_ndofs(self.h, L)
+
+    def coords(self, L):
+        n = self.ndofs(L)
+        xyz = np.zeros(3 * n)
+        bnd = np.zeros(n, dtype=np.uint8)
+        lib().or_dof_coords(self.h, L, xyz, bnd)
+        return xyz.reshape(n, 3), bnd.astype(bool)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.or_mesh_free(self.h)
+
+
+class Function:
+    def __init__(self, mesh, minL, maxL, prec=8):
+        self.mesh = mesh
+        self.h = lib().or_func_new(mesh.h, minL, maxL, prec)
+
+    def set(self, L, v):
+        lib().or_func_set(self.h, L, np.ascontiguousarray(v, dtype=np.float64))
+
+    def get(self, L):
+        out = np.zeros(self.mesh.ndofs(L))
+        lib().or_func_get(self.h, L, out)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.or_func_free(self.h)
+
+
+class Operator:
+    def __init__(self, mesh, minL, maxL, prec=8):
+        self.mesh = mesh
+        self.h = lib().or_op_new(mesh.h, minL, maxL, prec)
+
+    def apply(self, src, dst, L, flag=INNER, add=False):
+        lib().or_apply(self.h, src.h, dst.h, L, flag, int(add))
+
+    def cell_stencil(self, L, c):
+        out = np.zeros(15)
+        lib().or_op_cell_stencil(self.h, L, c, out)
+        return out
+
+    def inv_diag(self, L):
+        out = np.zeros(self.mesh.ndofs(L))
+        lib().or_inv_diag(self.h, L, out)
+        return out
+
+    def smooth_jac(self, dst, rhs, src, relax, L, flag=INNER):
+        lib().or_smooth_jac(self.h, dst.h, rhs.h, src.h, relax, L, flag)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.or_op_free(self.h)
+
+
+def cheb_coefficients(order, lower, upper):
+    out = np.zeros(order)
+    assert lib().or_cheb_coefficients(order, lower, upper, out) == 0
+    return out
+
+
+def gmg(op, x, b, minL, maxL, cycles, pre=2, post=2, inc=0, smoother="jacobi", relax=2.0 / 3.0,
+        cheb_order=2, cheb_bounds=(0.0, 0.0), cg_max_iter=10000, cg_tol=1e-16):
+    ip = np.array([minL, maxL, pre, post, inc, 0 if smoother == "jacobi" else 1, cheb_order, cg_max_iter], dtype=np.int32)
+    dp = np.array([relax, cheb_bounds[0], cheb_bounds[1], cg_tol])
+    res = np.zeros(cycles + 1)
+    lib().or_gmg(op.h, x.h, b.h, ip, dp, cycles, res)
+    return res
