@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X P1 matrix-free multigrid path.
+
+Workload (one GPU): P1 constant-coefficient Laplace on the 24-tetrahedron unit cube
+(meshes/cube_24el.msh) refined to level 8 (67,502,849 DoFs, 66,716,415 inner).
+A "step" is one FP64 operator apply y = A x over all inner DoFs, inputs resident in HBM.
+value = inner DoFs x steps / time (GDoF/s), timed with HIP events on the launch stream.
+
+Also reported:
+  roofline      algorithmic bytes of the apply kernel (16 B per inner DoF: read x, write y)
+                over its average launch duration, against the 8 TB/s HBM peak; `traffic`
+                from the committed rocprofv3 counter summary when present;
+  solve         V(3,3) Chebyshev-smoothed V-cycles to relative residual 1e-10, plain FP64 and
+                mixed precision (FP32 V-cycles inside FP64 defect correction), with the error of
+                the mixed solution against the FP64 one;
+  cpu_baseline  the CPU oracle (C++/OpenMP restatement of the reference's apply loops) timed on
+                this host on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PEAK_HBM_GBS = 8000.0
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("hyteg_vara_amd", os.path.join(ROOT, "hyteg-vara_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["hyteg_vara_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class Hip:
+    """Minimal HIP runtime access for streams and events."""
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so")
+        self.lib.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def stream(self):
+        s = C.c_void_p()
+        self._ok(self.lib.hipStreamCreate(C.byref(s)), "hipStreamCreate")
+        return s
+
+    def event(self):
+        e = C.c_void_p()
+        self._ok(self.lib.hipEventCreate(C.byref(e)), "hipEventCreate")
+        return e
+
+    def record(self, e, s):
+        self._ok(self.lib.hipEventRecord(e, s), "hipEventRecord")
+
+    def elapsed_ms(self, a, b):
+        self._ok(self.lib.hipEventSynchronize(b), "hipEventSynchronize")
+        ms = C.c_float()
+        self._ok(self.lib.hipEventElapsedTime(C.byref(ms), a, b), "hipEventElapsedTime")
+        return float(ms.value)
+
+    def sync(self, s):
+        self._ok(self.lib.hipStreamSynchronize(s), "hipStreamSynchronize")
+
+
+def timed(hip, stream, fn, reps):
+    e0, e1 = hip.event(), hip.event()
+    hip.sync(stream)
+    hip.record(e0, stream)
+    for _ in range(reps):
+        fn()
+    hip.record(e1, stream)
+    return hip.elapsed_ms(e0, e1)
+
+
+def profile_traffic():
+    """HBM bytes per apply launch from the committed rocprofv3 counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "apply_cube24_L8_f64_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            return json.load(fh).get("traffic_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(mesh_name, L, target_s, inner):
+    """Times the CPU oracle's apply on this host (bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol  # test infrastructure, used here only as the CPU baseline
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    m = ol.Mesh(mesh_name)
+    n = m.ndofs(L)
+    op = ol.Operator(m, L, L)
+    s, d = ol.Function(m, L, L), ol.Function(m, L, L)
+    s.set(L, np.random.default_rng(7).uniform(-1, 1, n))
+    op.apply(s, d, L, ol.INNER)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        op.apply(s, d, L, ol.INNER)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= target_s or reps >= 200:
+            break
+    return {"value": inner * reps / el / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "sample": f"{mesh_name} L{L} FP64 P1 constant apply (inner DoFs, incl. downward halo refresh), "
+                      f"{reps} applies in {el:.1f} s on {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--mesh", default="cube_24el.msh")
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cheb-order", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        return 2
+
+    hyg = load_pkg()
+    hyg.init(0)
+    hip = Hip()
+    st = hip.stream()
+    L = args.level
+    mesh = hyg.Mesh(args.mesh)
+    n = mesh.ndofs(L)
+    bnd = mesh.boundary(L)
+    inner = int((~bnd).sum())
+    rng = np.random.default_rng(2024)
+    op = hyg.P1ConstantOperator(mesh, 0, L)
+    lib = hyg.load()
+
+    # ---------------- apply: the timed steps
+    x = hyg.Function(mesh, L, L)
+    y = hyg.Function(mesh, L, L)
+    x.upload(L, rng.uniform(-1.0, 1.0, n))
+
+    def apply_once():
+        hyg.check(lib.hyg_apply(op.h, x.h, y.h, L, hyg.INNER, hyg.REPLACE, st))
+
+    for _ in range(args.warmup):
+        apply_once()
+    ms = timed(hip, st, apply_once, args.steps)
+    ms_step = ms / args.steps
+    gdofs = inner * args.steps / (ms / 1e3) / 1e9
+    alg_bytes = 16.0 * inner
+    achieved = alg_bytes / (ms_step / 1e3) / 1e9
+
+    # copy kernel of known size on the same stream (bandwidth reference)
+    nval = int(mesh.info()[3]) * ((((1 << L) + 1) * ((1 << L) + 2) * ((1 << L) + 3) // 6 + 63) // 64 * 64)
+
+    def copy_once():
+        hyg.check(lib.hyg_convert(y.h, x.h, L, st))
+
+    for _ in range(3):
+        copy_once()
+    ms_copy = timed(hip, st, copy_once, 20) / 20
+    copy_gbs = 16.0 * nval / (ms_copy / 1e3) / 1e9
+
+    out = {
+        "metric": "GDoF/s matrix-free apply + time-to-1e-10 V-cycle solve, mixed vs FP64, 1 GPU",
+        "value": round(gdofs, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform[-1,1] source vector)",
+        "config": {"workload": f"P1 constant-coefficient Laplace apply, {args.mesh} level {L}, FP64, Inner DoFs",
+                   "mesh": args.mesh, "level": L, "dofs_total": n, "dofs_inner": inner, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": profile_traffic(),
+                     "kernel": "kStencil<double, MODE_APPLY>", "algorithmic_bytes_per_launch": alg_bytes},
+        "copy_reference": {"bytes_per_launch": 16.0 * nval, "ms": round(ms_copy, 5), "GB/s": round(copy_gbs, 1)},
+    }
+
+    # ---------------- time to 1e-10: FP64 vs mixed precision
+    if not args.no_solve:
+        b = hyg.Function(mesh, 0, L)
+        bv = rng.uniform(-1.0, 1.0, n)
+        bv[bnd] = 0.0
+        b.upload(L, bv)
+        # spectral radius of D^-1 A (power iteration, fixed start vector and 40 iterations), Chebyshev bounds [0.3, 1.2] rho
+        Lr = min(L, 6)
+        pv, pt = hyg.Function(mesh, Lr, Lr), hyg.Function(mesh, Lr, Lr)
+        pv.upload(Lr, np.random.default_rng(5).uniform(-1, 1, mesh.ndofs(Lr)))
+        rho = op.power_iteration(pv, pt, 40, Lr)
+        kw = dict(pre=3, post=3, smoother="cheb", cheb_order=args.cheb_order, cheb_bounds=(0.3 * rho, 1.2 * rho))
+        g64 = hyg.GMG(op, 0, L, hyg.F64, **kw)
+        g32 = hyg.GMG(op, 0, L, hyg.F32, **kw)
+        x64 = hyg.Function(mesh, 0, L)
+        xmx = hyg.Function(mesh, 0, L)
+        # warm-up solves (first-touch of all levels), then timed solves from x = 0
+        g64.solve_tol(x64, b, L, 1, 1e-10, st)
+        hyg.ir_solve(op, g32, xmx, b, L, inner=1, max_outer=1, rel_tol=1e-10, stream=st)
+        for f in (x64, xmx):
+            f.interpolate(0.0, L)
+        b.upload(L, bv)
+        hip.sync(st)
+        t0 = time.perf_counter()
+        res64 = g64.solve_tol(x64, b, L, 60, 1e-10, st)
+        t64 = time.perf_counter() - t0
+        b.upload(L, bv)
+        hip.sync(st)
+        t0 = time.perf_counter()
+        resmx = hyg.ir_solve(op, g32, xmx, b, L, inner=1, max_outer=60, rel_tol=1e-10, stream=st)
+        tmx = time.perf_counter() - t0
+        u64, umx = x64.download(L), xmx.download(L)
+        out["solve"] = {
+            "problem": f"-Laplace u = f, {args.mesh} L{L}, zero Dirichlet, f ~ U[-1,1] seeded, u0 = 0",
+            "cycle": f"V(3,3), Chebyshev order {args.cheb_order}, bounds [0.3, 1.2] x rho(D^-1 A) = {rho:.6f}, "
+                     f"coarse level 0 CG",
+            "fp64": {"seconds": round(t64, 5), "cycles": len(res64) - 1,
+                     "rel_residual": float(res64[-1] / res64[0])},
+            "mixed": {"seconds": round(tmx, 5), "outer_steps": len(resmx) - 1,
+                      "rel_residual": float(resmx[-1] / resmx[0]),
+                      "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
+            "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
+        }
+
+    # ---------------- CPU baseline (oracle), rank 0, N = 1
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(args.mesh, L, args.cpu_seconds, inner)
+        except Exception as e:  # the baseline must not hide the GPU result
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -39,7 +39,7 @@ def _sig(lib):
         "hyg_mesh_destroy": (i, [vp]),
         "hyg_mesh_info": (i, [vp, P(np.int64)]),
         "hyg_mesh_num_dofs": (i, [vp, i, C.POINTER(C.c_int64)]),
-        "hyg_mesh_dof_coords": (i, [vp, i, P(np.float64), P(np.uint8)]),
+        "hyg_mesh_dof_coords": (i, [vp, i, vp, vp]),
         "hyg_func_create": (i, [vp, i, i, i, pvp]),
         "hyg_func_destroy": (i, [vp]),
         "hyg_func_upload": (i, [vp, i, P(np.float64), vp]),
@@ -65,6 +65,7 @@ def _sig(lib):
         "hyg_gmg_destroy": (i, [vp]),
         "hyg_gmg_vcycle": (i, [vp, vp, vp, i, vp]),
         "hyg_gmg_solve": (i, [vp, vp, vp, i, i, P(np.float64), vp]),
+        "hyg_gmg_solve_tol": (i, [vp, vp, vp, i, i, d, P(np.float64), C.POINTER(C.c_int), vp]),
         "hyg_ir_solve": (i, [vp, vp, vp, vp, i, i, i, d, P(np.float64), C.POINTER(C.c_int), vp]),
     }
     for name, (res, args) in table.items():
@@ -134,8 +135,13 @@ class Mesh:
         n = self.ndofs(L)
         xyz = np.zeros(3 * n)
         bnd = np.zeros(n, dtype=np.uint8)
-        check(load().hyg_mesh_dof_coords(self.h, L, xyz, bnd))
+        check(load().hyg_mesh_dof_coords(self.h, L, xyz.ctypes.data, bnd.ctypes.data))
         return xyz.reshape(n, 3), bnd.astype(bool)
+
+    def boundary(self, L):
+        bnd = np.zeros(self.ndofs(L), dtype=np.uint8)
+        check(load().hyg_mesh_dof_coords(self.h, L, None, bnd.ctypes.data))
+        return bnd.astype(bool)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -249,13 +255,19 @@ class GMG:
         check(load().hyg_gmg_solve(self.h, x.h, b.h, L, cycles, res, None))
         return res
 
+    def solve_tol(self, x, b, L, max_cycles, rel_tol, stream=None):
+        res = np.zeros(max_cycles + 1)
+        n = C.c_int()
+        check(load().hyg_gmg_solve_tol(self.h, x.h, b.h, L, max_cycles, rel_tol, res, C.byref(n), stream))
+        return res[: n.value + 1]
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.hyg_gmg_destroy(self.h)
 
 
-def ir_solve(op64, gmg32, x, b, L, inner=1, max_outer=30, rel_tol=1e-10):
+def ir_solve(op64, gmg32, x, b, L, inner=1, max_outer=30, rel_tol=1e-10, stream=None):
     res = np.zeros(max_outer + 1)
     steps = C.c_int()
-    check(load().hyg_ir_solve(op64.h, gmg32.h, x.h, b.h, L, inner, max_outer, rel_tol, res, C.byref(steps), None))
+    check(load().hyg_ir_solve(op64.h, gmg32.h, x.h, b.h, L, inner, max_outer, rel_tol, res, C.byref(steps), stream))
     return res[: steps.value + 1]

@@ -470,7 +470,7 @@ int hyg_mesh_num_dofs( hyg_mesh mesh, int level, int64_t* out )
 
 int hyg_mesh_dof_coords( hyg_mesh mesh, int level, double* xyz, uint8_t* boundary )
 {
-   if ( !mesh || !xyz || level < 0 || level > 10 )
+   if ( !mesh || ( !xyz && !boundary ) || level < 0 || level > 10 )
       return fail( HYG_ERR_ARG, "bad argument" );
    const MeshCtx& mc = mesh->ctx;
    const Mesh&    m  = *mc.mesh;
@@ -506,7 +506,8 @@ int hyg_mesh_dof_coords( hyg_mesh mesh, int level, double* xyz, uint8_t* boundar
             r[a] = m.X[cv[0]][a] + ( m.X[cv[1]][a] - m.X[cv[0]][a] ) * h * i + ( m.X[cv[2]][a] - m.X[cv[0]][a] ) * h * j +
                    ( m.X[cv[3]][a] - m.X[cv[0]][a] ) * h * k;
       }
-      xyz[3 * n] = r[0], xyz[3 * n + 1] = r[1], xyz[3 * n + 2] = r[2];
+      if ( xyz )
+         xyz[3 * n] = r[0], xyz[3 * n + 1] = r[1], xyz[3 * n + 2] = r[2];
       if ( boundary )
          boundary[n] = uint8_t( bnd );
    } );
@@ -937,6 +938,24 @@ int hyg_gmg_solve( hyg_gmg g, hyg_func x, hyg_func b, int level, int cycles, dou
    {
       HYG_TRY( vcycleImpl( g, x, b, level, level, st ) );
       HYG_TRY( residualNorm( g, x, b, g->tmp, level, &res[k + 1], st ) );
+   }
+   return HYG_OK;
+}
+
+int hyg_gmg_solve_tol( hyg_gmg g, hyg_func x, hyg_func b, int level, int max_cycles, double rel_tol, double* res, int* cycles,
+                       void* stream )
+{
+   HYG_TRY( checkGmgArgs( g, x, b, level ) );
+   if ( max_cycles < 0 || !res || !cycles )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   hipStream_t st = S( stream );
+   HYG_TRY( residualNorm( g, x, b, g->tmp, level, &res[0], st ) );
+   *cycles = 0;
+   for ( int k = 0; k < max_cycles && res[k] > rel_tol * res[0]; ++k )
+   {
+      HYG_TRY( vcycleImpl( g, x, b, level, level, st ) );
+      HYG_TRY( residualNorm( g, x, b, g->tmp, level, &res[k + 1], st ) );
+      *cycles = k + 1;
    }
    return HYG_OK;
 }

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <map>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -25,10 +26,11 @@ using i64 = int64_t;
 // ------------------------------------------------------------------ indexing
 __host__ __device__ inline i64 S3( i64 w ) { return w * ( w + 1 ) * ( w + 2 ) / 6; }
 __host__ __device__ inline i64 T2( i64 w ) { return w * ( w + 1 ) / 2; }
+// 32-bit variant: w (w + 1) (w + 2) < 2^31 up to w = 1025 (level 10)
+__host__ __device__ inline int S3i( int w ) { return ( w * ( w + 1 ) * ( w + 2 ) ) / 6; }
 __host__ __device__ inline int cidx( int w, int x, int y, int z )
 {
-   // all terms fit 32 bits up to level 10
-   return int( S3( w ) - S3( w - z ) ) + y * ( w - z + 1 ) - ( y * ( y + 1 ) ) / 2 + x;
+   return S3i( w ) - S3i( w - z ) + y * ( w - z + 1 ) - ( y * ( y + 1 ) ) / 2 + x;
 }
 // start of row y in a plane whose row 0 has W points
 __host__ __device__ inline int rowStart( int W, int y ) { return y * W - ( y * ( y - 1 ) ) / 2; }
@@ -95,6 +97,37 @@ struct PlaneTile
    int z;
    int s0;
 };
+
+// Work of one flat stencil block: 4 waves x 4 windows of 62 consecutive output points of plane z,
+// starting at plane index s0; p0 = offset of plane z in the cell array.
+struct StencilTile
+{
+   int z;
+   int s0;
+   int p0;
+   int pad;
+};
+
+// Work of one LDS-staged stencil block: interior rows [y0, y0 + rows) of planes [z0, z1).
+struct BandTile
+{
+   int y0;
+   int z0;
+   int z1;
+   int pad;
+};
+constexpr int kBandZChunk   = 32; // planes per band work item
+
+// Work of one z-marching wavefront: row y, points x0-1 .. x0+62 (outputs x0 .. x0+61), planes [z0, z1).
+struct ZSeg
+{
+   int y;
+   int x0;
+   int z0;
+   int z1;
+};
+constexpr int kZSegChunk = 64; // planes per z-marching segment
+constexpr int kBandMaxLevel = 8;  // the band kernel's register prefetch is sized for levels <= 8
 
 // Lower-dimensional primitive (macro-vertex, -edge or -face); order: vertices, edges, faces.
 struct LowerPrim
@@ -164,9 +197,20 @@ struct LevelGeom
    int                        nTiles = 0;
    int                        interiorTile0 = 0; // tiles of planes 1..N-3 (points off the cell boundary)
    int                        nInteriorTiles = 0;
-   std::vector< LowerChunk >  chunksHost;
-   LowerChunk*                chunks = nullptr;
-   int                        nChunks = 0;
+   std::vector< StencilTile > stTilesHost; // flat stencil tiles over the interior points of planes 1..N-3
+   StencilTile*               stTiles  = nullptr;
+   int                        nStTiles = 0;
+   // band work items per band height (same for every cell), built on first use
+   std::map< int, std::vector< BandTile > > bandTilesHost;
+   std::map< int, BandTile* >               bandTilesDevice;
+   const std::vector< BandTile >&           bandTilesFor( int rows );
+   const BandTile*                          bandTilesDev( int rows );
+   std::vector< ZSeg >                      zSegsHost; // z-marching segments (same for every cell)
+   ZSeg*                                    zSegs  = nullptr;
+   int                                      nZSegs = 0;
+   std::vector< LowerChunk >                chunksHost; // macro-face/-edge/-vertex work
+   LowerChunk*                              chunks  = nullptr;
+   int                                      nChunks = 0;
 };
 
 struct MeshCtx

@@ -66,7 +66,7 @@ int hyg_mesh_info( hyg_mesh mesh, int64_t* out );
 int hyg_mesh_num_dofs( hyg_mesh mesh, int level, int64_t* out );
 /* xyz[3*n], boundary[n] (1 = on a Dirichlet primitive) of every DoF in canonical order:
  * macro-vertices, then edge interiors, face interiors, cell interiors, each primitive in
- * storage order and its points in the reference's own linearisation. */
+ * storage order and its points in the reference's own linearisation. Either pointer may be NULL. */
 int hyg_mesh_dof_coords( hyg_mesh mesh, int level, double* xyz, uint8_t* boundary );
 
 /* ---------------------------------------------------------------- functions
@@ -138,6 +138,11 @@ int hyg_gmg_destroy( hyg_gmg g );
 int hyg_gmg_vcycle( hyg_gmg g, hyg_func x, hyg_func b, int level, void* stream );
 /* `cycles` V-cycles; res[0] = ||b - Ax|| over Inner|Neumann DoFs before, res[k] after cycle k (FP64 norms). */
 int hyg_gmg_solve( hyg_gmg g, hyg_func x, hyg_func b, int level, int cycles, double* res, void* stream );
+/* V-cycles until ||b - Ax|| <= rel_tol * ||b - Ax0|| or `max_cycles` cycles (the convergence loop around
+ * Solver::solve in the reference's drivers, e.g. apps/MultigridStudies/MultigridStudies.cpp:575-660).
+ * res[0..*cycles] as in hyg_gmg_solve; res must hold max_cycles + 1 values. */
+int hyg_gmg_solve_tol( hyg_gmg g, hyg_func x, hyg_func b, int level, int max_cycles, double rel_tol, double* res, int* cycles,
+                       void* stream );
 /* Mixed-precision defect correction (tests/hyteg/mixedPrecision/mixedPrecisionIterativeRefinement.cpp:162-172):
  * FP64 x, b; each outer step r = b - A x (FP64), FP32 correction by `inner` FP32 V-cycles of g32, x += c (FP64).
  * Stops after `max_outer` steps or when ||r||/||r0|| <= rel_tol. res[k] = FP64 residual norm after step k

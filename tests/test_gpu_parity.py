@@ -194,3 +194,29 @@ def test_vcycle_residual_history_parity(hyg, meshes, name, smoother):
     ref = ol.gmg(oop, ox, ob, minL, maxL, cycles, **kw)
     assert np.all(np.abs(got - ref) / ref <= 1e-10), (got, ref)
     assert rel_inf(x.download(maxL), ox.get(maxL)) <= 1e-10
+
+
+@pytest.mark.parametrize("L", [6, 7, 8])
+def test_apply_parity_large(hyg, meshes, L):
+    """Full-size apply on the 24-tet cube against the oracle, plus A*linear = 0 on inner DoFs."""
+    gm, om = meshes["cube_24el.msh"]
+    n = gm.ndofs(L)
+    src = rnd(n, 1000 + L)
+    op = hyg.P1ConstantOperator(gm, L, L)
+    s, d = hyg.Function(gm, L, L), hyg.Function(gm, L, L)
+    s.upload(L, src)
+    op.apply(s, d, L, hyg.INNER)
+    got = d.download(L)
+    oop = ol.Operator(om, L, L)
+    os_, od = ol.Function(om, L, L), ol.Function(om, L, L)
+    os_.set(L, src)
+    oop.apply(os_, od, L, ol.INNER, False)
+    assert rel_inf(got, od.get(L)) <= TOL64
+    del os_, od, oop
+    # the discrete Laplacian annihilates linear functions on inner DoFs (P1LaplaceOperator3DTest)
+    xyz, bnd = gm.coords(L)
+    lin = 0.3 + xyz @ np.array([1.0, -2.0, 0.5])
+    s.upload(L, lin)
+    op.apply(s, d, L, hyg.INNER)
+    r = d.download(L)[~bnd]
+    assert np.abs(r).max() <= 1e-13 * np.abs(lin).max() * (1 << L)
