@@ -114,7 +114,7 @@ int createFunc( hyg_mesh_s* mesh, int minL, int maxL, int dtype, hyg_func_s** ou
    for ( int L = minL; L <= maxL; ++L )
    {
       mesh->ctx.geom( L );
-      const size_t bytes = size_t( levelValues( f, L ) ) * f->elem();
+      const size_t bytes = size_t( levelValues( f, L ) + kTailSlack ) * f->elem();
       hipError_t   e     = hipMalloc( &f->data[L], bytes );
       if ( e == hipSuccess )
          e = hipMemset( f->data[L], 0, bytes );

@@ -56,6 +56,41 @@ __device__ __forceinline__ void storeStream( T* p, T v )
    __builtin_nontemporal_store( v, p );
 }
 
+// ------------------------------------------------------------------ raw buffer access
+// A buffer descriptor over `bytes` bytes from p (gfx9 family: word 3 = 0x00020000, untyped access).
+// Accesses give `voff` (per lane) and `soff` (wave-uniform) byte offsets; a store whose voff is
+// kDropOffset lies outside every descriptor's range and is dropped by the hardware, so masked
+// lanes need no branch around the store.
+constexpr unsigned kDropOffset = 0x80000000u;
+typedef unsigned int u32x2 __attribute__( ( ext_vector_type( 2 ) ) );
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t makeRsrc( const void* p, unsigned bytes )
+{
+   return __builtin_amdgcn_make_buffer_rsrc( const_cast< void* >( p ), 0, int( bytes ), 0x00020000 );
+}
+__device__ __forceinline__ double bufLoad( __amdgpu_buffer_rsrc_t r, unsigned voff, int soff, double* )
+{
+   return __builtin_bit_cast( double, __builtin_amdgcn_raw_buffer_load_b64( r, voff, soff, 0 ) );
+}
+__device__ __forceinline__ float bufLoad( __amdgpu_buffer_rsrc_t r, unsigned voff, int soff, float* )
+{
+   return __builtin_bit_cast( float, __builtin_amdgcn_raw_buffer_load_b32( r, voff, soff, 0 ) );
+}
+template < typename T >
+__device__ __forceinline__ T bufLoad( __amdgpu_buffer_rsrc_t r, unsigned voff, int soff )
+{
+   return bufLoad( r, voff, soff, static_cast< T* >( nullptr ) );
+}
+// streaming (non-temporal) stores
+__device__ __forceinline__ void bufStore( double v, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff )
+{
+   __builtin_amdgcn_raw_buffer_store_b64( __builtin_bit_cast( u32x2, v ), r, voff, soff, 2 );
+}
+__device__ __forceinline__ void bufStore( float v, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff )
+{
+   __builtin_amdgcn_raw_buffer_store_b32( __builtin_bit_cast( unsigned, v ), r, voff, soff, 2 );
+}
+
 // ------------------------------------------------------------------ stencil epilogue
 template < typename T, int MODE >
 __device__ __forceinline__ void epilogue( T Ax, T xc, T in1v, T out0v, T dinv, T a, T& o0, T& o1 )
@@ -98,6 +133,19 @@ template < int CTRL >
 __device__ __forceinline__ float laneShift( float v )
 {
    return __int_as_float( __builtin_amdgcn_update_dpp( 0, __float_as_int( v ), CTRL, 0xf, 0xf, false ) );
+}
+// the same shifts where the lane without a source (lane 0 for shr, lane 63 for shl) may receive 0
+template < int CTRL >
+__device__ __forceinline__ double laneMove( double v )
+{
+   const int lo = __builtin_amdgcn_mov_dpp( __double2loint( v ), CTRL, 0xf, 0xf, true );
+   const int hi = __builtin_amdgcn_mov_dpp( __double2hiint( v ), CTRL, 0xf, 0xf, true );
+   return __hiloint2double( hi, lo );
+}
+template < int CTRL >
+__device__ __forceinline__ float laneMove( float v )
+{
+   return __int_as_float( __builtin_amdgcn_mov_dpp( __float_as_int( v ), CTRL, 0xf, 0xf, true ) );
 }
 
 

@@ -13,7 +13,6 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
-#include <map>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -28,6 +27,7 @@ __host__ __device__ inline i64 S3( i64 w ) { return w * ( w + 1 ) * ( w + 2 ) / 
 __host__ __device__ inline i64 T2( i64 w ) { return w * ( w + 1 ) / 2; }
 // 32-bit variant: w (w + 1) (w + 2) < 2^31 up to w = 1025 (level 10)
 __host__ __device__ inline int S3i( int w ) { return ( w * ( w + 1 ) * ( w + 2 ) ) / 6; }
+__host__ __device__ inline int T2i( int w ) { return ( w * ( w + 1 ) ) >> 1; }
 __host__ __device__ inline int cidx( int w, int x, int y, int z )
 {
    return S3i( w ) - S3i( w - z ) + y * ( w - z + 1 ) - ( y * ( y + 1 ) ) / 2 + x;
@@ -40,6 +40,9 @@ inline i64 cellStride( int L )
    const i64 w = ( i64( 1 ) << L ) + 1;
    return ( S3( w ) + 63 ) / 64 * 64;
 }
+// values allocated after the last cell array of a level: row-window kernels read up to 64 values
+// past a cell's last point (and discard them)
+constexpr i64 kTailSlack = 128;
 
 // stencil directions in the reference order C W E N S NW SE TC TW TS TSE BC BN BE BNW
 // (src/hyteg/p1functionspace/VertexDoFIndexing.cpp:484-520)
@@ -91,34 +94,15 @@ __host__ __device__ inline int classifySlot( int N, int x, int y, int z )
 __host__ __device__ inline bool inCell( int N, int x, int y, int z ) { return x >= 0 && y >= 0 && z >= 0 && x + y + z <= N; }
 
 // ------------------------------------------------------------------ device tables
-// A tile of 256 consecutive points of one z-plane of a cell array.
+// A tile of 1024 consecutive points of one z-plane of a cell array (256 threads x 4 points).
 struct PlaneTile
 {
    int z;
    int s0;
 };
 
-// Work of one flat stencil block: 4 waves x 4 windows of 62 consecutive output points of plane z,
-// starting at plane index s0; p0 = offset of plane z in the cell array.
-struct StencilTile
-{
-   int z;
-   int s0;
-   int p0;
-   int pad;
-};
-
-// Work of one LDS-staged stencil block: interior rows [y0, y0 + rows) of planes [z0, z1).
-struct BandTile
-{
-   int y0;
-   int z0;
-   int z1;
-   int pad;
-};
-constexpr int kBandZChunk   = 32; // planes per band work item
-
-// Work of one z-marching wavefront: row y, points x0-1 .. x0+62 (outputs x0 .. x0+61), planes [z0, z1).
+// Work of one z-marching wavefront: rows y .. y+R-1 (R = rows per wavefront of the table), points
+// x0-1 .. x0+62 (outputs x0 .. x0+61), planes [z0, z1).
 struct ZSeg
 {
    int y;
@@ -127,7 +111,9 @@ struct ZSeg
    int z1;
 };
 constexpr int kZSegChunk = 64; // planes per z-marching segment
-constexpr int kBandMaxLevel = 8;  // the band kernel's register prefetch is sized for levels <= 8
+// cell-interior kernel configurations: rows per wavefront
+constexpr int kCellRows[]   = { 1, 2, 4 };
+constexpr int kNumCellCfgs  = 3;
 
 // Lower-dimensional primitive (macro-vertex, -edge or -face); order: vertices, edges, faces.
 struct LowerPrim
@@ -197,17 +183,11 @@ struct LevelGeom
    int                        nTiles = 0;
    int                        interiorTile0 = 0; // tiles of planes 1..N-3 (points off the cell boundary)
    int                        nInteriorTiles = 0;
-   std::vector< StencilTile > stTilesHost; // flat stencil tiles over the interior points of planes 1..N-3
-   StencilTile*               stTiles  = nullptr;
-   int                        nStTiles = 0;
-   // band work items per band height (same for every cell), built on first use
-   std::map< int, std::vector< BandTile > > bandTilesHost;
-   std::map< int, BandTile* >               bandTilesDevice;
-   const std::vector< BandTile >&           bandTilesFor( int rows );
-   const BandTile*                          bandTilesDev( int rows );
-   std::vector< ZSeg >                      zSegsHost; // z-marching segments (same for every cell)
-   ZSeg*                                    zSegs  = nullptr;
-   int                                      nZSegs = 0;
+   // z-marching segments per configuration (same for every cell)
+   std::vector< ZSeg >                      zSegsHost[kNumCellCfgs];
+   ZSeg*                                    zSegs[kNumCellCfgs]  = {};
+   int                                      nZSegs[kNumCellCfgs] = {};
+   int                                      cellCfg[2]           = { -1, -1 }; // chosen configuration (FP64, FP32)
    std::vector< LowerChunk >                chunksHost; // macro-face/-edge/-vertex work
    LowerChunk*                              chunks  = nullptr;
    int                                      nChunks = 0;

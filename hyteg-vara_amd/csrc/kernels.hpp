@@ -30,10 +30,10 @@ struct StencilCall
 // Runs the cell-interior kernel and the macro-face/edge/vertex kernel of one stencil operation.
 void launchStencil( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, int dtype, const StencilCall& c,
                     hipStream_t s );
-// cell-interior kernel configuration (band height x block size), chosen per value type
-int  stencilConfigCount();
-void setStencilConfig( int dtype, int cfg );
-int  getStencilConfig( int dtype );
+
+// cell-interior kernel configuration (index into kCellRows) of a level and value type
+int  cellConfig( LevelGeom& g, int dtype );
+void setCellConfig( LevelGeom& g, int dtype, int cfg );
 
 // Pointwise linear combination over all cell slots selected by flag:
 // dst = (accumulate ? dst : 0) + sum_k s_k src_k  (nsrc <= 3), or dst = value if nsrc == 0.

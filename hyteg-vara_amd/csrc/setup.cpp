@@ -219,27 +219,23 @@ LevelGeom& MeshCtx::geom( int L )
    if ( g.interiorTile0 < 0 )
       g.interiorTile0 = 0;
    g.nTiles = (int) g.tilesHost.size();
-   g.stTilesHost.clear();
-   for ( int z = 1; z <= N - 3; ++z )
+   // z-chunk major, then segment column, then row group: the four waves of a block march adjacent rows
+   for ( int k = 0; k < kNumCellCfgs; ++k )
    {
-      const int W = w - z, p0 = S3i( w ) - S3i( w - z ), yl = N - 2 - z;
-      const int first = W + 1, last = yl * W - ( yl * ( yl - 1 ) ) / 2 + 1;
-      for ( int s0 = first; s0 <= last; s0 += 4 * 4 * 62 ) // 4 waves x 4 windows of 62 outputs
-         g.stTilesHost.push_back( { z, s0, p0, 0 } );
+      const int R = kCellRows[k];
+      auto&     v = g.zSegsHost[k];
+      v.clear();
+      for ( int z0 = 1; z0 <= N - 3; z0 += kZSegChunk )
+         for ( int x0 = 1; x0 <= N - 3; x0 += 62 )
+            for ( int y = 1; y <= N - 3; y += R )
+            {
+               const int zEnd = N - y - x0; // outputs of row y exist for z < N - y - x0
+               if ( z0 < zEnd )
+                  v.push_back( { y, x0, z0, std::min( z0 + kZSegChunk, zEnd ) } );
+            }
+      g.nZSegs[k] = (int) v.size();
+      g.zSegs[k]  = upload( v );
    }
-   g.nStTiles = (int) g.stTilesHost.size();
-   g.zSegsHost.clear();
-   // z-chunk major, then segment column, then row: the four waves of a block march adjacent rows
-   for ( int z0 = 1; z0 <= N - 3; z0 += kZSegChunk )
-      for ( int x0 = 1; x0 <= N - 3; x0 += 62 )
-         for ( int y = 1; y <= N - 3; ++y )
-         {
-            const int zEnd = N - y - x0; // outputs exist for z < N - y - x0
-            if ( z0 < zEnd )
-               g.zSegsHost.push_back( { y, x0, z0, std::min( z0 + kZSegChunk, zEnd ) } );
-         }
-   g.nZSegs = (int) g.zSegsHost.size();
-   g.zSegs  = upload( g.zSegsHost );
    g.chunksHost.clear();
    for ( int p = 0; p < (int) primsHost.size(); ++p )
    {
@@ -265,36 +261,13 @@ LevelGeom& MeshCtx::geom( int L )
    return g;
 }
 
-const std::vector< BandTile >& LevelGeom::bandTilesFor( int rows )
-{
-   auto it = bandTilesHost.find( rows );
-   if ( it != bandTilesHost.end() )
-      return it->second;
-   const int               N = 1 << L;
-   std::vector< BandTile > t;
-   // z-chunk major, then bands: the bands of a plane range are dispatched next to each other
-   for ( int z0 = 1; z0 <= N - 3; z0 += kBandZChunk )
-      for ( int y0 = 1; y0 <= N - 2 - z0; y0 += rows )
-         t.push_back( { y0, z0, std::min( z0 + kBandZChunk, N - 1 - y0 ), 0 } );
-   bandTilesDevice[rows] = upload( t );
-   return bandTilesHost[rows] = t;
-}
-
-const BandTile* LevelGeom::bandTilesDev( int rows )
-{
-   bandTilesFor( rows );
-   return bandTilesDevice[rows];
-}
-
 MeshCtx::~MeshCtx()
 {
    for ( auto& g : levels )
    {
       (void) hipFree( g.tiles );
-      (void) hipFree( g.stTiles );
-      (void) hipFree( g.zSegs );
-      for ( auto& kv : g.bandTilesDevice )
-         (void) hipFree( kv.second );
+      for ( auto* zs : g.zSegs )
+         (void) hipFree( zs );
       (void) hipFree( g.chunks );
    }
    (void) hipFree( prims );

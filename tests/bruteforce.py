@@ -63,6 +63,7 @@ class RefinedMesh:
         self.L = L
         keyscale = 4 * N if scale_key is None else scale_key
         allkeys = []
+        allxyz = []
         elems = []
         off = 0
         w = N + 1
@@ -71,6 +72,7 @@ class RefinedMesh:
             V = np.array([nodes[v] for v in cv])
             xyz = V[0] + (lat @ (V[1:] - V[0])) / N
             allkeys.append(np.rint(xyz * keyscale).astype(np.int64))
+            allxyz.append(xyz)
             # micro-tets: for every type, every lattice base with all four vertices inside
             for t in MICRO_TETS:
                 verts = lat[:, None, :] + t[None, :, :]
@@ -82,10 +84,10 @@ class RefinedMesh:
                 elems.append(ids + off)
             off += len(lat)
         keys = np.concatenate(allkeys)
-        uniq, inv = np.unique(keys, axis=0, return_inverse=True)
+        uniq, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
         inv = inv.reshape(-1)
         self.keys = uniq
-        self.xyz = uniq / keyscale
+        self.xyz = np.concatenate(allxyz)[first]  # coordinates of the first cell that produced the vertex
         self.keyscale = keyscale
         self.tets = inv[np.concatenate(elems)]
         self.n = len(uniq)
