@@ -188,9 +188,10 @@ struct LevelGeom
    ZSeg*                                    zSegs[kNumCellCfgs]  = {};
    int                                      nZSegs[kNumCellCfgs] = {};
    int                                      cellCfg[2]           = { -1, -1 }; // chosen configuration (FP64, FP32)
-   std::vector< LowerChunk >                chunksHost; // macro-face/-edge/-vertex work
+   std::vector< LowerChunk >                chunksHost; // macro-vertex, then -edge, then -face work
    LowerChunk*                              chunks  = nullptr;
    int                                      nChunks = 0;
+   int                                      firstFaceChunk = 0; // chunks [firstFaceChunk, nChunks) are face rows
 };
 
 struct MeshCtx

@@ -254,7 +254,14 @@ LevelGeom& MeshCtx::geom( int L )
                g.chunksHost.push_back( { p, j, i0, std::min( 64, N - j - i0 ) } );
       }
    }
-   g.nChunks = (int) g.chunksHost.size();
+   g.nChunks        = (int) g.chunksHost.size();
+   g.firstFaceChunk = g.nChunks;
+   for ( int k = 0; k < g.nChunks; ++k )
+      if ( primsHost[g.chunksHost[k].prim].kind == 2 )
+      {
+         g.firstFaceChunk = k;
+         break;
+      }
    g.tiles   = upload( g.tilesHost );
    g.chunks  = upload( g.chunksHost );
    g.L       = L;

@@ -73,16 +73,19 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    int       pl    = z0 - 1;
    int       Wl    = N + 1 - pl;
    int       off   = S3i( N + 1 ) - S3i( Wl ) + rowStart( Wl, y0 - 1 ) + sg.x0 - 1;
+   // lanes past the end of a row take the out-of-range offset: they read 0 and fetch nothing
+   const int laneEnd = N + 2 - sg.x0 - y0; // last lane of row y0-1 in plane 0
    auto      loadPlane = [&]( PlaneRows< T, R >& P ) {
       int o = off;
 #pragma unroll
       for ( int k = 0; k < R + 2; ++k )
       {
-         P.r[k] = bufLoad< T >( rx, vo, o * B );
+         const int le = laneEnd - pl - k; // last lane holding a point of row y0-1+k in plane pl
+         P.r[k]       = bufLoad< T >( rx, int( lane ) <= le ? vo : kDropOffset, o * B );
          if constexpr ( kE )
          {
             if ( k >= 1 && k <= R )
-               P.e[k - 1] = bufLoad< T >( re, vo, o * B );
+               P.e[k - 1] = bufLoad< T >( re, int( lane ) < le ? vo : kDropOffset, o * B );
          }
          o += Wl - ( y0 - 1 + k ); // start of the next row
       }
@@ -234,6 +237,77 @@ __global__ __launch_bounds__( kBlock ) void kLower( const LowerChunk* __restrict
    }
 }
 
+// Macro-face points: the two incident cells' one-sided stencils, with the 15 neighbour offsets of a
+// point derived incrementally from its own index (no per-direction index arithmetic). Directions
+// with a zero weight (those leaving the cell) read the point itself.
+template < typename T, int MODE >
+__global__ __launch_bounds__( kBlock ) void kFace( const LowerChunk* __restrict__ chunks, int nChunks,
+                                                   const LowerPrim* __restrict__ prims, const LowerEntry* __restrict__ entries,
+                                                   const T* __restrict__ entryW, const T* __restrict__ primInv, int L,
+                                                   i64 stride, int flag, const T* __restrict__ x, const T* __restrict__ in1,
+                                                   T* __restrict__ out0, T* __restrict__ out1, T a )
+{
+   const int nb = ( nChunks + 3 ) / 4;
+   const int ci = remapXcd( int( blockIdx.x ), nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
+   if ( ci >= nChunks )
+      return;
+   const LowerChunk ch = chunks[ci];
+   const LowerPrim  p  = prims[ch.prim];
+   if ( !( p.bc & flag ) )
+      return;
+   const int  lane = int( threadIdx.x & 63 );
+   const bool on   = lane < ch.n;
+   const int  N = 1 << L, w = N + 1;
+   const int  i = ch.i0 + ( on ? lane : 0 ), j = ch.j;
+   T          v[2][NDIR];
+   i64        o[2];
+#pragma unroll
+   for ( int k = 0; k < 2; ++k )
+   {
+      const int        ek = p.entry0 + ( k < p.nCells ? k : 0 );
+      const LowerEntry e  = entries[ek];
+      int              qx, qy, qz;
+      entryPos( e, N, i, j, qx, qy, qz );
+      const int W  = w - qz;
+      const int tc = T2i( W ) - qy, bc = T2i( W + 1 ) - qy, nn = W - qy;
+      const int off[NDIR] = { 0,  -1,      1,          nn,     -( nn + 1 ),      nn - 1, -nn, tc, tc - 1, tc - nn, tc - nn + 1,
+                              -bc, -bc + nn + 1, -bc + 1, -bc + nn };
+      o[k]                = i64( e.cell ) * stride + cidx( w, qx, qy, qz );
+      const T* xs         = x + o[k];
+      const T* wv         = entryW + i64( ek ) * 16;
+#pragma unroll
+      for ( int d = 0; d < NDIR; ++d )
+         v[k][d] = xs[wv[d] != T( 0 ) ? off[d] : 0];
+   }
+   T Ax = T( 0 );
+#pragma unroll
+   for ( int k = 0; k < 2; ++k )
+   {
+      const T* wv = entryW + i64( p.entry0 + ( k < p.nCells ? k : 0 ) ) * 16;
+      const T  s  = k < p.nCells ? T( 1 ) : T( 0 );
+#pragma unroll
+      for ( int d = 0; d < NDIR; ++d )
+         Ax += ( s * wv[d] ) * v[k][d];
+   }
+   if ( !on )
+      return;
+   T i1 = T( 0 ), ov = T( 0 ), r0, r1;
+   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      i1 = in1[o[0]];
+   if constexpr ( MODE == MODE_APPLY_ADD )
+      ov = out0[o[0]];
+   epilogue< T, MODE >( Ax, v[0][0], i1, ov, primInv[ch.prim], a, r0, r1 );
+   out0[o[0]] = r0;
+   if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      out1[o[0]] = r1;
+   if ( p.nCells > 1 )
+   {
+      out0[o[1]] = r0;
+      if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+         out1[o[1]] = r1;
+   }
+}
+
 // ------------------------------------------------------------------ launch
 // cell-interior configuration of a level: set by the tuner, else HYG_CELL_CFG, else rows = 2
 int cellConfig( LevelGeom& g, int dtype )
@@ -263,9 +337,14 @@ static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol,
       hipLaunchKernelGGL( kf, dim3( xcdGrid( nb ) ), dim3( kBlock ), 0, s, g.zSegs[cfg], g.nZSegs[cfg], nC, L, g.stride, cst,
                           (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
    }
-   if ( g.nChunks > 0 )
-      hipLaunchKernelGGL( ( kLower< T, MODE > ), dim3( xcdGrid( ceilDiv( g.nChunks, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
-                          g.chunks, g.nChunks, m.prims, m.entries, ew, pi, L, g.stride, c.flag, (const T*) c.x,
+   const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
+   if ( nLow > 0 )
+      hipLaunchKernelGGL( ( kLower< T, MODE > ), dim3( xcdGrid( ceilDiv( nLow, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, g.chunks,
+                          nLow, m.prims, m.entries, ew, pi, L, g.stride, c.flag, (const T*) c.x, (const T*) c.in1,
+                          (T*) c.out0, (T*) c.out1, T( c.a ) );
+   if ( nFace > 0 )
+      hipLaunchKernelGGL( ( kFace< T, MODE > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
+                          g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, ew, pi, L, g.stride, c.flag, (const T*) c.x,
                           (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
 }
 

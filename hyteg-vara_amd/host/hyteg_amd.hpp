@@ -1,0 +1,303 @@
+// C++ host classes over the C ABI of include/hyteg_amd.h, shaped like the reference's interfaces so that
+// code written against HyTeG's operator/solver API reads the same:
+//   Operator::apply(src, dst, level, flag, updateType)          operators/Operator.hpp:73-80
+//   WeightedJacobiSmoothable::smooth_jac(dst, rhs, tmp, relax, level, flag)   solvers/Smoothables.hpp:32-43
+//   Solver<Op>::solve(A, x, b, level)                            solvers/Solver.hpp:26-36
+//   ProlongationOperator::prolongate / prolongateAndAdd          gridtransferoperators/ProlongationOperator.hpp:30-39
+//   RestrictionOperator::restrict                                gridtransferoperators/RestrictionOperator.hpp:29-35
+//   GeometricMultigridSolver(storage, smoother, coarse, restriction, prolongation, minL, maxL, pre, post, inc)
+//                                                                solvers/GeometricMultigridSolver.hpp:42-121
+// Functions are passed by const& and mutated, as in the reference. A failing call throws
+// std::runtime_error carrying hyg_last_error() (the reference aborts or throws from its smoothers).
+#pragma once
+
+#include "../../include/hyteg_amd.h"
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace hyteg_amd {
+
+enum DoFType : int
+{
+   None              = HYG_NONE,
+   Inner             = HYG_INNER,
+   DirichletBoundary = HYG_DIRICHLET,
+   NeumannBoundary   = HYG_NEUMANN,
+   FreeslipBoundary  = HYG_FREESLIP,
+   All               = HYG_ALL,
+};
+enum UpdateType : int
+{
+   Replace = HYG_REPLACE,
+   Add     = HYG_ADD,
+};
+
+inline void check( int rc )
+{
+   if ( rc != HYG_OK )
+      throw std::runtime_error( std::string( "hyteg_amd: " ) + hyg_last_error() );
+}
+
+// Device selection; call once per process before anything else.
+inline void initialise( int device = 0 ) { check( hyg_init( device ) ); }
+
+// Macro mesh + boundary flags (MeshInfo::fromGmshFile + SetupPrimitiveStorage + PrimitiveStorage).
+class PrimitiveStorage
+{
+ public:
+   explicit PrimitiveStorage( const std::string& gmshFile )
+   {
+      hyg_mesh m = nullptr;
+      check( hyg_mesh_create_from_msh( gmshFile.c_str(), &m ) );
+      h_.reset( m, []( hyg_mesh p ) { hyg_mesh_destroy( p ); } );
+   }
+   hyg_mesh handle() const { return h_.get(); }
+   int64_t  numberOfDoFs( int level ) const
+   {
+      int64_t n = 0;
+      check( hyg_mesh_num_dofs( handle(), level, &n ) );
+      return n;
+   }
+
+ private:
+   std::shared_ptr< hyg_mesh_s > h_;
+};
+
+// P1Function<ValueType> (double or float): device arrays for levels minLevel..maxLevel.
+template < typename ValueType >
+class P1Function
+{
+   static_assert( std::is_same< ValueType, double >::value || std::is_same< ValueType, float >::value, "double or float" );
+
+ public:
+   using valueType = ValueType;
+   P1Function( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
+   : name_( name )
+   , storage_( storage )
+   {
+      hyg_func f = nullptr;
+      check( hyg_func_create( storage->handle(), minLevel, maxLevel, sizeof( ValueType ) == 8 ? HYG_F64 : HYG_F32, &f ) );
+      h_.reset( f, []( hyg_func p ) { hyg_func_destroy( p ); } );
+   }
+   hyg_func           handle() const { return h_.get(); }
+   const std::string& getFunctionName() const { return name_; }
+
+   void interpolate( double constant, int level, DoFType flag = All ) const
+   {
+      check( hyg_interpolate_const( handle(), constant, level, flag, stream_ ) );
+   }
+   void assign( const std::vector< double >& scalars, const std::vector< P1Function* >& functions, int level,
+                DoFType flag = All ) const
+   {
+      std::vector< hyg_func > f;
+      for ( auto* p : functions )
+         f.push_back( p->handle() );
+      check( hyg_assign( handle(), int( f.size() ), scalars.data(), f.data(), level, flag, stream_ ) );
+   }
+   void add( const std::vector< double >& scalars, const std::vector< P1Function* >& functions, int level,
+             DoFType flag = All ) const
+   {
+      std::vector< hyg_func > f;
+      for ( auto* p : functions )
+         f.push_back( p->handle() );
+      check( hyg_add( handle(), int( f.size() ), scalars.data(), f.data(), level, flag, stream_ ) );
+   }
+   void multElementwise( const P1Function& a, const P1Function& b, int level, DoFType flag = All ) const
+   {
+      check( hyg_mult_elementwise( handle(), a.handle(), b.handle(), level, flag, stream_ ) );
+   }
+   double dotGlobal( const P1Function& other, int level, DoFType flag = All ) const
+   {
+      double r = 0.0;
+      check( hyg_dot( handle(), other.handle(), level, flag, &r, stream_ ) );
+      return r;
+   }
+   template < typename Other >
+   void copyFrom( const P1Function< Other >& other, int level ) const
+   {
+      check( hyg_convert( handle(), other.handle(), level, stream_ ) );
+   }
+   // canonical-order host copies (double on the host side)
+   void setValues( int level, const std::vector< double >& v ) const { check( hyg_func_upload( handle(), level, v.data(), stream_ ) ); }
+   std::vector< double > getValues( int level ) const
+   {
+      std::vector< double > v( size_t( storage_->numberOfDoFs( level ) ) );
+      check( hyg_func_download( handle(), level, v.data(), stream_ ) );
+      return v;
+   }
+   void setStream( void* s ) { stream_ = s; }
+
+ private:
+   std::string                         name_;
+   std::shared_ptr< PrimitiveStorage > storage_;
+   std::shared_ptr< hyg_func_s >       h_;
+   void*                               stream_ = nullptr;
+};
+
+// P1ConstantLaplaceOperator (P1ConstantOperator<P1FenicsForm<p1_diffusion, p1_tet_diffusion>>), one value type.
+template < typename ValueType >
+class P1ConstantLaplaceOperator
+{
+ public:
+   using srcType = P1Function< ValueType >;
+   using dstType = P1Function< ValueType >;
+   P1ConstantLaplaceOperator( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
+   : storage_( storage )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   {
+      hyg_op op = nullptr;
+      check( hyg_op_create_p1_const( storage->handle(), minLevel, maxLevel, &op ) );
+      h_.reset( op, []( hyg_op p ) { hyg_op_destroy( p ); } );
+   }
+   hyg_op handle() const { return h_.get(); }
+   int    minLevel() const { return minLevel_; }
+   int    maxLevel() const { return maxLevel_; }
+
+   void apply( const srcType& src, const dstType& dst, int level, DoFType flag, UpdateType updateType = Replace ) const
+   {
+      check( hyg_apply( handle(), src.handle(), dst.handle(), level, flag, updateType, stream_ ) );
+   }
+   void smooth_jac( const dstType& dst, const dstType& rhs, const srcType& tmp, double relax, int level, DoFType flag ) const
+   {
+      check( hyg_smooth_jac( handle(), dst.handle(), rhs.handle(), tmp.handle(), relax, level, flag, stream_ ) );
+   }
+   // writes D^-1 into `dst` (getInverseDiagonalValues; the values are computed at construction)
+   void getInverseDiagonalValues( const dstType& dst, int level ) const
+   {
+      check( hyg_inverse_diagonal( handle(), dst.handle(), level, stream_ ) );
+   }
+   void setStream( void* s ) { stream_ = s; }
+
+ private:
+   std::shared_ptr< PrimitiveStorage > storage_;
+   std::shared_ptr< hyg_op_s >         h_;
+   int                                 minLevel_, maxLevel_;
+   void*                               stream_ = nullptr;
+};
+
+class P1toP1LinearRestriction
+{
+ public:
+   template < typename F >
+   void restrict( const F& function, int sourceLevel, DoFType flag ) const
+   {
+      check( hyg_restrict( function.handle(), sourceLevel, flag, nullptr ) );
+   }
+};
+
+class P1toP1LinearProlongation
+{
+ public:
+   template < typename F >
+   void prolongate( const F& function, int sourceLevel, DoFType flag ) const
+   {
+      check( hyg_prolongate( function.handle(), sourceLevel, flag, HYG_REPLACE, nullptr ) );
+   }
+   template < typename F >
+   void prolongateAndAdd( const F& function, int sourceLevel, DoFType flag ) const
+   {
+      check( hyg_prolongate( function.handle(), sourceLevel, flag, HYG_ADD, nullptr ) );
+   }
+};
+
+// Smoother / coarse-solver descriptions consumed by GeometricMultigridSolver (the library runs them).
+struct WeightedJacobiSmoother
+{
+   double relax = 2.0 / 3.0;
+};
+struct ChebyshevSmoother
+{
+   int    order = 2;
+   double lower = 0.0, upper = 0.0;
+   // setupCoefficients(order, spectralRadius): bounds [0.3, 1.2] x spectralRadius (ChebyshevSmoother.hpp:109-190)
+   void setupCoefficients( int ord, double spectralRadius )
+   {
+      order = ord;
+      lower = 0.3 * spectralRadius;
+      upper = 1.2 * spectralRadius;
+   }
+};
+struct CGSolver
+{
+   int    maxIter   = 10000;
+   double tolerance = 1e-16;
+};
+
+// chebyshev::estimateRadius: power iteration on D^-1 A with `tmp` as workspace; x holds the start vector.
+template < typename Op, typename F >
+double estimateRadius( const Op& A, int level, int iterations, const F& x, const F& tmp )
+{
+   double rho = 0.0;
+   check( hyg_power_iteration( A.handle(), x.handle(), tmp.handle(), iterations, level, &rho, nullptr ) );
+   return rho;
+}
+
+template < typename Op >
+class GeometricMultigridSolver
+{
+ public:
+   using FunctionType = typename Op::srcType;
+   template < typename Smoother >
+   GeometricMultigridSolver( const std::shared_ptr< PrimitiveStorage >&, const Op& A, const Smoother& smoother, const CGSolver& coarse,
+                             int minLevel, int maxLevel, int preSmoothSteps = 3, int postSmoothSteps = 3,
+                             int smoothIncrementOnCoarserGrids = 0 )
+   {
+      int    ip[8] = { minLevel, maxLevel, preSmoothSteps, postSmoothSteps, smoothIncrementOnCoarserGrids, 0, 1, coarse.maxIter };
+      double dp[4] = { 2.0 / 3.0, 0.0, 0.0, coarse.tolerance };
+      setSmoother( smoother, ip, dp );
+      hyg_gmg g = nullptr;
+      check( hyg_gmg_create( A.handle(), sizeof( typename FunctionType::valueType ) == 8 ? HYG_F64 : HYG_F32, ip, dp, &g ) );
+      h_.reset( g, []( hyg_gmg p ) { hyg_gmg_destroy( p ); } );
+   }
+   hyg_gmg handle() const { return h_.get(); }
+   // one V-cycle (Solver::solve)
+   void solve( const Op&, const FunctionType& x, const FunctionType& b, int level ) const
+   {
+      check( hyg_gmg_vcycle( handle(), x.handle(), b.handle(), level, nullptr ) );
+   }
+   // V-cycles until the residual dropped by relTol; returns the residual norms (FP64)
+   std::vector< double > solveTo( const FunctionType& x, const FunctionType& b, int level, int maxCycles, double relTol ) const
+   {
+      std::vector< double > res( size_t( maxCycles ) + 1 );
+      int                   n = 0;
+      check( hyg_gmg_solve_tol( handle(), x.handle(), b.handle(), level, maxCycles, relTol, res.data(), &n, nullptr ) );
+      res.resize( size_t( n ) + 1 );
+      return res;
+   }
+
+ private:
+   static void setSmoother( const WeightedJacobiSmoother& s, int* ip, double* dp )
+   {
+      ip[5] = 0;
+      dp[0] = s.relax;
+   }
+   static void setSmoother( const ChebyshevSmoother& s, int* ip, double* dp )
+   {
+      ip[5] = 1;
+      ip[6] = s.order;
+      dp[1] = s.lower;
+      dp[2] = s.upper;
+   }
+   std::shared_ptr< hyg_gmg_s > h_;
+};
+
+// Mixed-precision defect correction: FP64 residual and update, `inner` FP32 V-cycles per step.
+inline std::vector< double > mixedPrecisionSolve( const P1ConstantLaplaceOperator< double >&       A64,
+                                                  const GeometricMultigridSolver< P1ConstantLaplaceOperator< float > >& gmg32,
+                                                  const P1Function< double >& x, const P1Function< double >& b, int level,
+                                                  int inner, int maxOuter, double relTol )
+{
+   std::vector< double > res( size_t( maxOuter ) + 1 );
+   int                   n = 0;
+   check( hyg_ir_solve( A64.handle(), gmg32.handle(), x.handle(), b.handle(), level, inner, maxOuter, relTol, res.data(), &n,
+                        nullptr ) );
+   res.resize( size_t( n ) + 1 );
+   return res;
+}
+
+} // namespace hyteg_amd
