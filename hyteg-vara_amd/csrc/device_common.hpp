@@ -56,6 +56,21 @@ __device__ __forceinline__ void storeStream( T* p, T v )
    __builtin_nontemporal_store( v, p );
 }
 
+// ------------------------------------------------------------------ single-lane moves
+// returns `old` with lane `dst` replaced by lane `src` of v (src, dst wave-uniform)
+__device__ __forceinline__ double moveLane( double v, int src, int dst, double old )
+{
+   const int    lo = __builtin_amdgcn_readlane( __double2loint( v ), src );
+   const int    hi = __builtin_amdgcn_readlane( __double2hiint( v ), src );
+   const double s  = __hiloint2double( hi, lo );
+   return int( threadIdx.x & 63u ) == dst ? s : old;
+}
+__device__ __forceinline__ float moveLane( float v, int src, int dst, float old )
+{
+   const float s = __int_as_float( __builtin_amdgcn_readlane( __float_as_int( v ), src ) );
+   return int( threadIdx.x & 63u ) == dst ? s : old;
+}
+
 // ------------------------------------------------------------------ raw buffer access
 // A buffer descriptor over `bytes` bytes from p (gfx9 family: word 3 = 0x00020000, untyped access).
 // Accesses give `voff` (per lane) and `soff` (wave-uniform) byte offsets; a store whose voff is

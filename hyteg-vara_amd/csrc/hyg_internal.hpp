@@ -142,6 +142,24 @@ struct LowerChunk
    int n;
 };
 
+// One face of one macro-cell, for the stencil kernels that compute one-sided face sums. Local face slots
+// in the cell's layout: 0 z = 0, 1 y = 0, 2 x = 0, 3 x+y+z = N.
+struct CellFaceSlot
+{
+   int entry; // LowerEntry of this (face, cell) pair: its one-sided weights are entryW[16 * entry ..]
+   int prim;  // LowerPrim index of the face
+   int block; // 2 * face index + side (position of the cell in the face's cell list): partial-sum block
+   int bc;    // DoFType of the face
+};
+// Partial sums of a face side at level L are stored in a block of (N+1)^2 values, indexed by the cell
+// coordinates of the point: slot 0 x + y (N+1), slot 1 x + z (N+1), slots 2 and 3 z + y (N+1).
+__host__ __device__ inline int facePartIndex( int slot, int N, int x, int y, int z )
+{
+   return slot == 0 ? x + y * ( N + 1 ) : ( slot == 1 ? x + z * ( N + 1 ) : z + y * ( N + 1 ) );
+}
+// local face slot of a face entry (the slot is the face opposite the local vertex not among l0, l1, l2)
+__host__ __device__ inline int entrySlot( const LowerEntry& e ) { return e.l0 + e.l1 + e.l2 - 3; }
+
 // Per-cell data used by pointwise kernels.
 struct CellInfo
 {
@@ -157,7 +175,7 @@ struct Mesh
    std::vector< std::array< double, 3 > > X;
    std::vector< std::array< int, 2 > >    E;
    std::vector< std::array< int, 3 > >    F;
-   std::vector< std::array< int, 4 > >    Cv;
+   std::vector< std::array< int, 4 > >    Cv; // cell vertices in the reference's local order
    std::vector< std::array< int, 4 > >    cF;
    std::vector< std::array< int, 6 > >    cE;
    std::vector< std::vector< int > >      vC, eC, fC;
@@ -169,6 +187,11 @@ struct Mesh
 };
 
 std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err );
+
+// element matrices of the six micro-tetrahedron types of a cell with vertices cv at level L, and the
+// one-sided 15-point stencil of that cell at lattice point (qx, qy, qz) (w[0..14])
+void elementMatrices( const Mesh& m, const std::array< int, 4 >& cv, int L, double K[6][4][4] );
+void stencilAt( const double K[6][4][4], int N, int qx, int qy, int qz, double* w );
 
 // canonical DoF count, V + E(N-1) + F(N-1)(N-2)/2 + C(N-1)(N-2)(N-3)/6
 i64 numDofs( const Mesh& m, int L );
@@ -188,6 +211,9 @@ struct LevelGeom
    ZSeg*                                    zSegs[kNumCellCfgs]  = {};
    int                                      nZSegs[kNumCellCfgs] = {};
    int                                      cellCfg[2]           = { -1, -1 }; // chosen configuration (FP64, FP32)
+   std::vector< int4 >                      flatItemsHost; // z = 0 / y = 0 face rows: (cell, slot, line, x0)
+   int4*                                    flatItems  = nullptr;
+   int                                      nFlatItems = 0;
    std::vector< LowerChunk >                chunksHost; // macro-vertex, then -edge, then -face work
    LowerChunk*                              chunks  = nullptr;
    int                                      nChunks = 0;
@@ -201,9 +227,11 @@ struct MeshCtx
    std::vector< CellInfo >     cellInfoHost;
    std::vector< LowerPrim >    primsHost;
    std::vector< LowerEntry >   entriesHost;
+   std::vector< CellFaceSlot > cellFacesHost; // 4 per cell
    LowerPrim*                  prims    = nullptr;
    LowerEntry*                 entries  = nullptr;
    CellInfo*                   cellInfo = nullptr;
+   CellFaceSlot*               cellFaces = nullptr;
    int                         device   = 0;
    void                        buildTopology(); // prims, entries, cell info (host + device)
    LevelGeom&                  geom( int L );
@@ -240,6 +268,9 @@ struct hyg_op_s
       float*                entryWF = nullptr;
       double*               primInv = nullptr;
       float*                primInvF = nullptr;
+      // one-sided face partial sums (2 sides x (N+1)^2 values per face), FP64 and FP32
+      double*               facePart  = nullptr;
+      float*                facePartF = nullptr;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

@@ -41,12 +41,14 @@ int dirIndex( int dx, int dy, int dz )
    throw std::logic_error( "micro-tetrahedron edge is not a stencil direction" );
 }
 
-// element matrices of the six micro-tetrahedron types of cell c at level L
-void elementMatrices( const Mesh& m, int c, int L, double K[6][4][4] )
+} // namespace
+
+// element matrices of the six micro-tetrahedron types of a cell with vertices cv (local order) at level L
+void elementMatrices( const Mesh& m, const std::array< int, 4 >& cv, int L, double K[6][4][4] )
 {
    const double N  = double( 1 << L );
-   const V3&    P0 = m.X[m.Cv[c][0]];
-   const V3     a1 = sub( m.X[m.Cv[c][1]], P0 ), a2 = sub( m.X[m.Cv[c][2]], P0 ), a3 = sub( m.X[m.Cv[c][3]], P0 );
+   const V3&    P0 = m.X[cv[0]];
+   const V3     a1 = sub( m.X[cv[1]], P0 ), a2 = sub( m.X[cv[2]], P0 ), a3 = sub( m.X[cv[3]], P0 );
    for ( int t = 0; t < 6; ++t )
    {
       V3 P[4];
@@ -93,6 +95,8 @@ void stencilAt( const double K[6][4][4], int N, int qx, int qy, int qz, double* 
                         kMicroTet[t][j][2] - kMicroTet[t][k][2] )] += K[t][k][j];
       }
 }
+
+namespace {
 
 void entryPosHost( const LowerEntry& e, int N, int i, int j, int& qx, int& qy, int& qz )
 {
@@ -185,9 +189,26 @@ void MeshCtx::buildTopology()
          setSlot( 11 + k, v, m.vB[v], m.vC[v] );
       }
    }
-   prims    = upload( primsHost );
-   entries  = upload( entriesHost );
-   cellInfo = upload( cellInfoHost );
+   // face slots of every cell (layout order), with the entry of the (face, cell) pair
+   cellFacesHost.assign( size_t( m.nC() ) * 4, CellFaceSlot{ -1, -1, -1, 0 } );
+   for ( int f = 0; f < m.nF(); ++f )
+   {
+      const int       p  = m.nV() + m.nE() + f;
+      const LowerPrim& pr = primsHost[p];
+      for ( int k = 0; k < pr.nCells; ++k )
+      {
+         const LowerEntry& e = entriesHost[pr.entry0 + k];
+         CellFaceSlot&     s = cellFacesHost[4 * e.cell + entrySlot( e )];
+         s.entry             = pr.entry0 + k;
+         s.prim              = p;
+         s.block             = 2 * f + k;
+         s.bc                = pr.bc;
+      }
+   }
+   prims     = upload( primsHost );
+   entries   = upload( entriesHost );
+   cellInfo  = upload( cellInfoHost );
+   cellFaces = upload( cellFacesHost );
 }
 
 LevelGeom& MeshCtx::geom( int L )
@@ -219,23 +240,35 @@ LevelGeom& MeshCtx::geom( int L )
    if ( g.interiorTile0 < 0 )
       g.interiorTile0 = 0;
    g.nTiles = (int) g.tilesHost.size();
-   // z-chunk major, then segment column, then row group: the four waves of a block march adjacent rows
+   // z-chunk major, then segment column, then row group: the four waves of a block march adjacent rows.
+   // A segment of rows y.. starting at column x0 covers the planes where row y still has a point at
+   // x >= x0 (z <= N - y - x0): the last of them holds no interior output, only the row end that lies
+   // on the diagonal face.
    for ( int k = 0; k < kNumCellCfgs; ++k )
    {
       const int R = kCellRows[k];
       auto&     v = g.zSegsHost[k];
       v.clear();
-      for ( int z0 = 1; z0 <= N - 3; z0 += kZSegChunk )
-         for ( int x0 = 1; x0 <= N - 3; x0 += 62 )
-            for ( int y = 1; y <= N - 3; y += R )
+      for ( int z0 = 1; z0 <= N - 2; z0 += kZSegChunk )
+         for ( int x0 = 1; x0 <= N - 2; x0 += 62 )
+            for ( int y = 1; y <= N - 2; y += R )
             {
-               const int zEnd = N - y - x0; // outputs of row y exist for z < N - y - x0
+               const int zEnd = N - y - x0 + 1;
                if ( z0 < zEnd )
                   v.push_back( { y, x0, z0, std::min( z0 + kZSegChunk, zEnd ) } );
             }
       g.nZSegs[k] = (int) v.size();
       g.zSegs[k]  = upload( v );
    }
+   // z = 0 and y = 0 faces of every cell: rows of 62 points
+   g.flatItemsHost.clear();
+   for ( int c = 0; c < mesh->nC(); ++c )
+      for ( int slot = 0; slot < 2; ++slot )
+         for ( int line = 1; line <= N - 2; ++line )
+            for ( int x0 = 1; x0 <= N - 1 - line; x0 += 62 )
+               g.flatItemsHost.push_back( int4{ c, slot, line, x0 } );
+   g.nFlatItems = (int) g.flatItemsHost.size();
+   g.flatItems  = upload( g.flatItemsHost );
    g.chunksHost.clear();
    for ( int p = 0; p < (int) primsHost.size(); ++p )
    {
@@ -275,9 +308,11 @@ MeshCtx::~MeshCtx()
       (void) hipFree( g.tiles );
       for ( auto* zs : g.zSegs )
          (void) hipFree( zs );
+      (void) hipFree( g.flatItems );
       (void) hipFree( g.chunks );
    }
    (void) hipFree( prims );
+   (void) hipFree( cellFaces );
    (void) hipFree( entries );
    (void) hipFree( cellInfo );
 }
@@ -299,7 +334,7 @@ void assembleP1Const( hyg_op_s& op )
       for ( int c = 0; c < m.nC(); ++c )
       {
          double K[6][4][4];
-         elementMatrices( m, c, L, K );
+         elementMatrices( m, m.Cv[c], L, K );
          for ( int t = 0; t < 6; ++t )
             for ( int i = 0; i < 4; ++i )
                for ( int j = 0; j < 4; ++j )
@@ -354,6 +389,20 @@ void assembleP1Const( hyg_op_s& op )
       ol.cellStF  = upload( f1 );
       ol.entryWF  = upload( f2 );
       ol.primInvF = upload( f3 );
+      // face partial-sum blocks (allocated here so that no stencil call allocates)
+      const size_t np = size_t( m.nF() ) * 2 * size_t( N + 1 ) * size_t( N + 1 );
+      if ( np > 0 )
+      {
+         void* d64 = nullptr;
+         void* d32 = nullptr;
+         if ( hipMalloc( &d64, np * sizeof( double ) ) != hipSuccess || hipMalloc( &d32, np * sizeof( float ) ) != hipSuccess )
+         {
+            (void) hipFree( d64 );
+            throw std::runtime_error( "hipMalloc failed for face partial sums" );
+         }
+         ol.facePart  = static_cast< double* >( d64 );
+         ol.facePartF = static_cast< float* >( d32 );
+      }
    }
 }
 
@@ -369,5 +418,7 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.entryWF );
       (void) hipFree( l.primInv );
       (void) hipFree( l.primInvF );
+      (void) hipFree( l.facePart );
+      (void) hipFree( l.facePartF );
    }
 }

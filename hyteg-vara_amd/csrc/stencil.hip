@@ -1,23 +1,27 @@
 // gfx950 kernels of the 15-point P1 stencil operations (apply, residual, Jacobi, Chebyshev steps).
 //
-// Cell interior: z-marching wavefronts. A wavefront owns the row segment (y, x0-1 .. x0+62) of one
-// macro-cell and marches up the planes z0 .. z1-1. It keeps rows y-1, y, y+1 of planes z-1, z, z+1
-// in registers and has the rows of planes z+2 and z+3 in flight, so each plane step issues three
-// coalesced row loads (plus one row of the epilogue input, also prefetched) and no barrier. Row and
-// plane offsets are wave-uniform and advanced incrementally in scalar registers, so every load and
-// store is `uniform base + lane`. The x-1 / x+1 neighbours come from the adjacent lanes: the terms
-// needing lane-1 (W, NW, TW, BNW) and lane+1 (E, SE, TSE, BE) are summed per lane first and each sum
-// is moved with one DPP shift (the coefficients are uniform over a macro-cell). Lanes 1..62 produce
-// results. Windows may run past a row end; such lanes read other points of the array (function
-// arrays carry kTailSlack values after the last cell) and their results are discarded. A valid
-// output only reads points of its own cell. Loads and stores go through per-cell buffer
-// descriptors with a uniform byte offset, so a step has no per-lane address arithmetic, and the
-// stores of discarded lanes are dropped by the descriptor's range check instead of a branch.
+// Cell interior: z-marching wavefronts (kCellZ). A wavefront owns R adjacent rows (y0 .. y0+R-1) and
+// a 64-point x-window (x0-1 .. x0+62, outputs x0 .. x0+61) of one macro-cell and marches up the
+// planes z0 .. z1-1. A five-plane register ring holds rows y0-1 .. y0+R of planes z-1, z, z+1 and
+// has planes z+2, z+3 in flight, so each plane step issues R+2 coalesced row loads (plus R rows of the
+// epilogue input) and no barrier. Row and plane offsets are wave-uniform and advanced incrementally
+// in scalar registers; loads and stores go through per-cell buffer descriptors with a uniform byte
+// offset, lanes past a row end take an out-of-range offset (they read 0 and fetch nothing), and the
+// stores of lanes without an output are dropped the same way, so a step has no branch around a
+// memory operation. The x-1 / x+1 neighbours come from the adjacent lanes: the terms needing lane-1
+// (W, NW, TW, BNW) and lane+1 (E, SE, TSE, BE) are summed per lane first and moved with one DPP shift
+// each.
 //
-// Macro-faces, -edges and -vertices. One wavefront per chunk of up to 64 points of a primitive;
-// each point sums the one-sided stencils of its incident cells and writes the result to every
-// cell's replica of the point. Chunks of one primitive are consecutive and stay on one XCD, so the
-// cache lines around neighbouring chunk rows are shared through that XCD's L2.
+// Macro-faces. A face value is the sum of one-sided stencils of its (one or two) cells. Each cell
+// computes its own one-sided sums where its rows are already streamed: the x = 0 face is lane 0 of
+// the windows starting at a row start (per-lane weights), the diagonal face x+y+z = N is the last
+// point of each row (a second weight set, evaluated by the wavefronts whose window holds row ends);
+// both are gathered in registers over the march and stored once per segment. The z = 0 and y = 0
+// faces are whole planes/rows and are computed by kFaceFlat with coalesced loads. kFaceCombine adds
+// the partial sums of the two sides, applies the epilogue and writes every replica.
+//
+// Macro-edges and -vertices (kLower): one lane per point gathers the one-sided stencils of all
+// incident cells.
 #include "device_common.hpp"
 
 namespace hyg {
@@ -32,10 +36,16 @@ struct PlaneRows
    T e[R];
 };
 
+// directions without an x shift and with x+1 (E group)
+constexpr int kDirU[7] = { 0, 3, 4, 7, 9, 11, 12 };
+constexpr int kDirE[4] = { 2, 6, 10, 13 };
+
 template < typename T, int MODE, int R >
 __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ segs, int nSegs, int nC, int L, i64 stride,
-                                                    const T* __restrict__ cellSt, const T* __restrict__ x,
-                                                    const T* __restrict__ in1, T* __restrict__ out0, T* __restrict__ out1, T a )
+                                                    const T* __restrict__ cellSt, const CellFaceSlot* __restrict__ cfs,
+                                                    const T* __restrict__ entryW, T* __restrict__ part, int flag,
+                                                    const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
+                                                    T* __restrict__ out1, T a )
 {
    constexpr bool kIn1 = MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK;
    constexpr bool kE   = kIn1 || MODE == MODE_APPLY_ADD;
@@ -51,7 +61,7 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    if ( si >= nSegs )
       return;
    const ZSeg     sg    = segs[si];
-   const int      N     = 1 << L, y0 = sg.y, z0 = sg.z0, z1 = sg.z1;
+   const int      N     = 1 << L, NP = N + 1, y0 = sg.y, z0 = sg.z0, z1 = sg.z1, x0 = sg.x0;
    const unsigned lane  = threadIdx.x & 63u;
    const unsigned vo    = lane * unsigned( B );
    const i64      cb    = i64( c ) * stride;
@@ -65,6 +75,38 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
 #pragma unroll
    for ( int d = 0; d < 16; ++d )
       w[d] = st[d];
+   const bool interior = ( flag & 1 ) != 0;
+
+   // x = 0 face (slot 2): lane 0 of the windows starting at a row start, with that face's one-sided
+   // weights; the E-group terms lane 0 needs are formed on lane 1, so lane 1 holds the face weights
+   // of that group.
+   const CellFaceSlot fx   = cfs[4 * c + 2];
+   const bool         onX0 = x0 == 1 && ( fx.bc & flag ) != 0;
+   T                  wu[7], we[4];
+   {
+      const T* wf = entryW + 16 * i64( onX0 ? fx.entry : 0 );
+#pragma unroll
+      for ( int k = 0; k < 7; ++k )
+         wu[k] = ( onX0 && lane == 0u ) ? wf[kDirU[k]] : w[kDirU[k]];
+#pragma unroll
+      for ( int k = 0; k < 4; ++k )
+         we[k] = ( onX0 && lane == 1u ) ? wf[kDirE[k]] : w[kDirE[k]];
+   }
+   // diagonal face (slot 3): the row ends x = N-y-z of rows y0..y0+R-1 inside the output window
+   const CellFaceSlot fd     = cfs[4 * c + 3];
+   const int          xeHi   = N - y0 - z0, xeLo = N - ( y0 + R - 1 ) - ( z1 - 1 );
+   const bool         onDiag = ( fd.bc & flag ) != 0 && xeHi >= x0 && xeLo <= x0 + 61;
+   T                  wd[15];
+   {
+      const T* wf = entryW + 16 * i64( onDiag ? fd.entry : 0 );
+#pragma unroll
+      for ( int d = 0; d < NDIR; ++d )
+         wd[d] = onDiag ? wf[d] : T( 0 );
+   }
+   T f2v[R], dg[R]; // face partial sums gathered over the march
+#pragma unroll
+   for ( int r = 0; r < R; ++r )
+      f2v[r] = dg[r] = T( 0 );
 
    // load front: offset of (x0-1, y0-1) in plane pl, whose row 0 has Wl points. Every step issues the
    // same loads (the front stops at the last plane needed, min(z1, N), and repeats it), so the
@@ -72,9 +114,9 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    const int plMax = min( z1, N );
    int       pl    = z0 - 1;
    int       Wl    = N + 1 - pl;
-   int       off   = S3i( N + 1 ) - S3i( Wl ) + rowStart( Wl, y0 - 1 ) + sg.x0 - 1;
+   int       off   = S3i( N + 1 ) - S3i( Wl ) + rowStart( Wl, y0 - 1 ) + x0 - 1;
    // lanes past the end of a row take the out-of-range offset: they read 0 and fetch nothing
-   const int laneEnd = N + 2 - sg.x0 - y0; // last lane of row y0-1 in plane 0
+   const int laneEnd = N + 2 - x0 - y0; // last lane of row y0-1 in plane 0
    auto      loadPlane = [&]( PlaneRows< T, R >& P ) {
       int o = off;
 #pragma unroll
@@ -98,7 +140,7 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    };
    // output plane z: offset oz of (x0-1, y0-1), row 0 of the plane has Wz points
    int       z = z0, oz = 0, Wz = N + 1 - z0;
-   const int lastLane = N - y0 - sg.x0; // lane of x = N-1-y-z is lastLane - (y - y0) - z
+   const int lastLane = N - y0 - x0; // lane of x = N-1-y-z is lastLane - (y - y0) - z
    // A, B, C: planes z-1, z, z+1 (resident); plane z+2 is in flight; E receives plane z+3
    auto step = [&]( const PlaneRows< T, R >& A, const PlaneRows< T, R >& Bp, const PlaneRows< T, R >& C, PlaneRows< T, R >& E ) {
       loadPlane( E );
@@ -108,30 +150,43 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
       {
          // plane z: C = row r+1, N = row r+2, S = row r; plane z+1: TC = row r+1, TS = row r;
          // plane z-1: BC = row r+1, BN = row r+2
-         T Ax = w[0] * Bp.r[r + 1];
-         Ax += w[3] * Bp.r[r + 2];
-         Ax += w[4] * Bp.r[r];
-         Ax += w[7] * C.r[r + 1];
-         Ax += w[9] * C.r[r];
-         Ax += w[11] * A.r[r + 1];
-         Ax += w[12] * A.r[r + 2];
+         const T bc = Bp.r[r + 1], bn = Bp.r[r + 2], bs = Bp.r[r], cc = C.r[r + 1], cs = C.r[r], ac = A.r[r + 1],
+                 an = A.r[r + 2];
+         T Ax = wu[0] * bc;
+         Ax += wu[1] * bn;
+         Ax += wu[2] * bs;
+         Ax += wu[3] * cc;
+         Ax += wu[4] * cs;
+         Ax += wu[5] * ac;
+         Ax += wu[6] * an;
          // terms read at x-1 (W, NW, TW, BNW) and at x+1 (E, SE, TSE, BE)
-         T sr = w[1] * Bp.r[r + 1];
-         sr += w[5] * Bp.r[r + 2];
-         sr += w[8] * C.r[r + 1];
-         sr += w[14] * A.r[r + 2];
-         T sl = w[2] * Bp.r[r + 1];
-         sl += w[6] * Bp.r[r];
-         sl += w[10] * C.r[r];
-         sl += w[13] * A.r[r + 1];
+         T sr = w[1] * bc;
+         sr += w[5] * bn;
+         sr += w[8] * cc;
+         sr += w[14] * an;
+         T sl = we[0] * bc;
+         sl += we[1] * bs;
+         sl += we[2] * cs;
+         sl += we[3] * ac;
          Ax += laneMove< kDppWaveShr1 >( sr );
          Ax += laneMove< kDppWaveShl1 >( sl );
          T o0, o1;
-         epilogue< T, MODE >( Ax, Bp.r[r + 1], Bp.e[r], Bp.e[r], w[15], a, o0, o1 );
-         const unsigned so = ( lane - 1u < 62u && int( lane ) <= lastLane - r - z ) ? vo : kDropOffset;
+         epilogue< T, MODE >( Ax, bc, Bp.e[r], Bp.e[r], w[15], a, o0, o1 );
+         const unsigned so = ( interior && lane - 1u < 62u && int( lane ) <= lastLane - r - z ) ? vo : kDropOffset;
          bufStore( o0, r0, so, o * B );
          if constexpr ( kTwo )
             bufStore( o1, r1, so, o * B );
+         if ( onX0 ) // lane 0 holds the x = 0 face sum of row y0+r, plane z
+            f2v[r] = moveLane( Ax, 0, z - z0, f2v[r] );
+         if ( onDiag )
+         {
+            T ad = wd[0] * bc + wd[3] * bn + wd[4] * bs + wd[7] * cc + wd[9] * cs + wd[11] * ac + wd[12] * an;
+            T dr = wd[1] * bc + wd[5] * bn + wd[8] * cc + wd[14] * an;
+            T dl = wd[2] * bc + wd[6] * bs + wd[10] * cs + wd[13] * ac;
+            ad += laneMove< kDppWaveShr1 >( dr );
+            ad += laneMove< kDppWaveShl1 >( dl );
+            dg[r] = int( lane ) == lastLane + 1 - r - z ? ad : dg[r];
+         }
          o += Wz - ( y0 + r );
       }
       oz += T2i( Wz ) - ( y0 - 1 );
@@ -166,9 +221,130 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
          break;
       step( R0, R1, R2, R4 );
    }
+   // face partial sums of this segment
+   const unsigned pb = unsigned( NP * NP * B );
+   if ( onX0 )
+   {
+      const auto rp = makeRsrc( part + i64( fx.block ) * NP * NP, pb );
+#pragma unroll
+      for ( int r = 0; r < R; ++r )
+      {
+         // lane k: point (0, y0+r, z0+k) of the face interior (y + z <= N-1)
+         const int  zz = z0 + int( lane );
+         const bool ok = zz < z1 && y0 + r + zz <= N - 1;
+         bufStore( f2v[r], rp, ok ? unsigned( facePartIndex( 2, N, 0, y0 + r, zz ) * B ) : kDropOffset, 0 );
+      }
+   }
+   if ( onDiag )
+   {
+      const auto rp = makeRsrc( part + i64( fd.block ) * NP * NP, pb );
+#pragma unroll
+      for ( int r = 0; r < R; ++r )
+      {
+         // lane l held the row end x = x0-1+l of row y0+r at plane z = N - y - x
+         const int  zz = N - ( y0 + r ) - ( x0 - 1 + int( lane ) );
+         const bool ok = lane - 1u < 62u && zz >= z0 && zz < z1 && zz >= 1;
+         bufStore( dg[r], rp, ok ? unsigned( facePartIndex( 3, N, 0, y0 + r, zz ) * B ) : kDropOffset, 0 );
+      }
+   }
 }
 
-// ------------------------------------------------------------------ macro-face / -edge / -vertex points
+// ------------------------------------------------------------------ z = 0 and y = 0 faces: one-sided sums
+// Work item: cell, slot (0: plane z = 0, row `line`; 1: row y = 0 of plane `line`), window start x0.
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kFaceFlat( const int4* __restrict__ items, int nItems, int L, i64 stride,
+                                                       const CellFaceSlot* __restrict__ cfs, const T* __restrict__ entryW,
+                                                       T* __restrict__ part, int flag, const T* __restrict__ x )
+{
+   constexpr int B  = int( sizeof( T ) );
+   const int     nb = ( nItems + 3 ) / 4;
+   const int     wi = remapXcd( int( blockIdx.x ), nb ) * 4 + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
+   if ( wi >= nItems )
+      return;
+   const int4         it = items[wi];
+   const int          c = it.x, slot = it.y, line = it.z, x0 = it.w;
+   const CellFaceSlot fs = cfs[4 * c + slot];
+   if ( !( fs.bc & flag ) )
+      return;
+   const int      N = 1 << L, NP = N + 1, w1 = N + 1;
+   const unsigned lane = threadIdx.x & 63u, vo = lane * unsigned( B );
+   const auto     rx   = makeRsrc( x + i64( c ) * stride, unsigned( ( stride + kTailSlack ) * B ) );
+   const T*       wf   = entryW + 16 * i64( fs.entry );
+   T              w[15];
+#pragma unroll
+   for ( int d = 0; d < NDIR; ++d )
+      w[d] = wf[d];
+   // row (yy, zz) at x = x0-1+lane, 0 past its end
+   auto row = [&]( int yy, int zz ) {
+      const int W  = w1 - zz;
+      const int le = N - zz - yy - ( x0 - 1 ); // last lane of the row
+      if ( yy < 0 || zz < 0 || le < 0 )
+         return T( 0 );
+      const int o = S3i( w1 ) - S3i( W ) + rowStart( W, yy ) + x0 - 1;
+      return bufLoad< T >( rx, int( lane ) <= le ? vo : kDropOffset, o * B );
+   };
+   // the rows of the stencil around (x, yc, zc); rows outside the cell are 0
+   const int yc = slot == 0 ? line : 0, zc = slot == 0 ? 0 : line;
+   const T   bc = row( yc, zc ), bn = row( yc + 1, zc ), bs = row( yc - 1, zc ), cc = row( yc, zc + 1 ), cs = row( yc - 1, zc + 1 ),
+           ac = row( yc, zc - 1 ), an = row( yc + 1, zc - 1 );
+   T Ax = w[0] * bc + w[3] * bn + w[4] * bs + w[7] * cc + w[9] * cs + w[11] * ac + w[12] * an;
+   T sr = w[1] * bc + w[5] * bn + w[8] * cc + w[14] * an;
+   T sl = w[2] * bc + w[6] * bs + w[10] * cs + w[13] * ac;
+   Ax += laneMove< kDppWaveShr1 >( sr );
+   Ax += laneMove< kDppWaveShl1 >( sl );
+   const int  xx = x0 - 1 + int( lane );
+   const bool ok = lane - 1u < 62u && xx + yc + zc <= N - 1;
+   const auto rp = makeRsrc( part + i64( fs.block ) * NP * NP, unsigned( NP * NP * B ) );
+   bufStore( Ax, rp, ok ? unsigned( facePartIndex( slot, N, xx, yc, zc ) * B ) : kDropOffset, 0 );
+}
+
+// ------------------------------------------------------------------ macro-face points: combine
+template < typename T, int MODE >
+__global__ __launch_bounds__( kBlock ) void kFaceCombine( const LowerChunk* __restrict__ chunks, int nChunks,
+                                                          const LowerPrim* __restrict__ prims,
+                                                          const LowerEntry* __restrict__ entries, int firstFacePrim,
+                                                          const T* __restrict__ part, const T* __restrict__ primInv, int L,
+                                                          i64 stride, int flag, const T* __restrict__ x,
+                                                          const T* __restrict__ in1, T* __restrict__ out0, T* __restrict__ out1, T a )
+{
+   const int nb = ( nChunks + 3 ) / 4;
+   const int ci = remapXcd( int( blockIdx.x ), nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
+   if ( ci >= nChunks )
+      return;
+   const LowerChunk ch = chunks[ci];
+   const LowerPrim  p  = prims[ch.prim];
+   const int        lane = int( threadIdx.x & 63 );
+   if ( !( p.bc & flag ) || lane >= ch.n )
+      return;
+   const int N = 1 << L, NP = N + 1, w = N + 1;
+   const int i = ch.i0 + lane, j = ch.j, f = ch.prim - firstFacePrim;
+   T         Ax = T( 0 );
+   i64       o[2];
+   for ( int k = 0; k < p.nCells; ++k )
+   {
+      const LowerEntry e = entries[p.entry0 + k];
+      int              qx, qy, qz;
+      entryPos( e, N, i, j, qx, qy, qz );
+      o[k] = i64( e.cell ) * stride + cidx( w, qx, qy, qz );
+      Ax += part[i64( 2 * f + k ) * NP * NP + facePartIndex( entrySlot( e ), N, qx, qy, qz )];
+   }
+   T xc = T( 0 ), i1 = T( 0 ), ov = T( 0 ), r0, r1;
+   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
+      xc = x[o[0]];
+   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      i1 = in1[o[0]];
+   if constexpr ( MODE == MODE_APPLY_ADD )
+      ov = out0[o[0]];
+   epilogue< T, MODE >( Ax, xc, i1, ov, primInv[ch.prim], a, r0, r1 );
+   for ( int k = 0; k < p.nCells; ++k )
+   {
+      out0[o[k]] = r0;
+      if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+         out1[o[k]] = r1;
+   }
+}
+
+// ------------------------------------------------------------------ macro-edge / -vertex points
 // Each lane gathers the one-sided stencil of every incident cell. Directions leaving a cell carry a
 // zero weight in that cell's one-sided stencil; they read the point itself instead, so all loads of a
 // lane are unconditional and independent and issue back to back.
@@ -237,77 +413,6 @@ __global__ __launch_bounds__( kBlock ) void kLower( const LowerChunk* __restrict
    }
 }
 
-// Macro-face points: the two incident cells' one-sided stencils, with the 15 neighbour offsets of a
-// point derived incrementally from its own index (no per-direction index arithmetic). Directions
-// with a zero weight (those leaving the cell) read the point itself.
-template < typename T, int MODE >
-__global__ __launch_bounds__( kBlock ) void kFace( const LowerChunk* __restrict__ chunks, int nChunks,
-                                                   const LowerPrim* __restrict__ prims, const LowerEntry* __restrict__ entries,
-                                                   const T* __restrict__ entryW, const T* __restrict__ primInv, int L,
-                                                   i64 stride, int flag, const T* __restrict__ x, const T* __restrict__ in1,
-                                                   T* __restrict__ out0, T* __restrict__ out1, T a )
-{
-   const int nb = ( nChunks + 3 ) / 4;
-   const int ci = remapXcd( int( blockIdx.x ), nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
-   if ( ci >= nChunks )
-      return;
-   const LowerChunk ch = chunks[ci];
-   const LowerPrim  p  = prims[ch.prim];
-   if ( !( p.bc & flag ) )
-      return;
-   const int  lane = int( threadIdx.x & 63 );
-   const bool on   = lane < ch.n;
-   const int  N = 1 << L, w = N + 1;
-   const int  i = ch.i0 + ( on ? lane : 0 ), j = ch.j;
-   T          v[2][NDIR];
-   i64        o[2];
-#pragma unroll
-   for ( int k = 0; k < 2; ++k )
-   {
-      const int        ek = p.entry0 + ( k < p.nCells ? k : 0 );
-      const LowerEntry e  = entries[ek];
-      int              qx, qy, qz;
-      entryPos( e, N, i, j, qx, qy, qz );
-      const int W  = w - qz;
-      const int tc = T2i( W ) - qy, bc = T2i( W + 1 ) - qy, nn = W - qy;
-      const int off[NDIR] = { 0,  -1,      1,          nn,     -( nn + 1 ),      nn - 1, -nn, tc, tc - 1, tc - nn, tc - nn + 1,
-                              -bc, -bc + nn + 1, -bc + 1, -bc + nn };
-      o[k]                = i64( e.cell ) * stride + cidx( w, qx, qy, qz );
-      const T* xs         = x + o[k];
-      const T* wv         = entryW + i64( ek ) * 16;
-#pragma unroll
-      for ( int d = 0; d < NDIR; ++d )
-         v[k][d] = xs[wv[d] != T( 0 ) ? off[d] : 0];
-   }
-   T Ax = T( 0 );
-#pragma unroll
-   for ( int k = 0; k < 2; ++k )
-   {
-      const T* wv = entryW + i64( p.entry0 + ( k < p.nCells ? k : 0 ) ) * 16;
-      const T  s  = k < p.nCells ? T( 1 ) : T( 0 );
-#pragma unroll
-      for ( int d = 0; d < NDIR; ++d )
-         Ax += ( s * wv[d] ) * v[k][d];
-   }
-   if ( !on )
-      return;
-   T i1 = T( 0 ), ov = T( 0 ), r0, r1;
-   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-      i1 = in1[o[0]];
-   if constexpr ( MODE == MODE_APPLY_ADD )
-      ov = out0[o[0]];
-   epilogue< T, MODE >( Ax, v[0][0], i1, ov, primInv[ch.prim], a, r0, r1 );
-   out0[o[0]] = r0;
-   if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-      out1[o[0]] = r1;
-   if ( p.nCells > 1 )
-   {
-      out0[o[1]] = r0;
-      if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-         out1[o[1]] = r1;
-   }
-}
-
 // ------------------------------------------------------------------ launch
 // cell-interior configuration of a level: set by the tuner, else HYG_CELL_CFG, else rows = 2
 int cellConfig( LevelGeom& g, int dtype )
@@ -325,27 +430,40 @@ void setCellConfig( LevelGeom& g, int dtype, int cfg ) { g.cellCfg[dtype == 8 ? 
 template < typename T, int MODE >
 static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, const StencilCall& c, hipStream_t s )
 {
-   const int nC  = m.mesh->nC();
-   const T*  cst = std::is_same< T, double >::value ? (const T*) ol.cellSt : (const T*) ol.cellStF;
-   const T*  ew  = std::is_same< T, double >::value ? (const T*) ol.entryW : (const T*) ol.entryWF;
-   const T*  pi  = std::is_same< T, double >::value ? (const T*) ol.primInv : (const T*) ol.primInvF;
-   const int cfg = cellConfig( g, std::is_same< T, double >::value ? 8 : 4 );
-   if ( ( c.flag & 1 ) && g.nZSegs[cfg] > 0 )
+   constexpr bool dbl  = std::is_same< T, double >::value;
+   const int      nC   = m.mesh->nC();
+   const T*       cst  = dbl ? (const T*) ol.cellSt : (const T*) ol.cellStF;
+   const T*       ew   = dbl ? (const T*) ol.entryW : (const T*) ol.entryWF;
+   const T*       pi   = dbl ? (const T*) ol.primInv : (const T*) ol.primInvF;
+   T*             part = dbl ? (T*) ol.facePart : (T*) ol.facePartF;
+   const int      cfg  = cellConfig( g, dbl ? 8 : 4 );
+   const T*       x    = (const T*) c.x;
+   const T*       in1  = (const T*) c.in1;
+   T*             out0 = (T*) c.out0;
+   T*             out1 = (T*) c.out1;
+   // 1. cell interiors, and the one-sided sums of the x = 0 and diagonal faces
+   if ( g.nZSegs[cfg] > 0 )
    {
       const int nb = ( ( g.nZSegs[cfg] + 3 ) / 4 ) * nC;
       auto      kf = cfg == 0 ? kCellZ< T, MODE, 1 > : ( cfg == 1 ? kCellZ< T, MODE, 2 > : kCellZ< T, MODE, 4 > );
       hipLaunchKernelGGL( kf, dim3( xcdGrid( nb ) ), dim3( kBlock ), 0, s, g.zSegs[cfg], g.nZSegs[cfg], nC, L, g.stride, cst,
-                          (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
+                          m.cellFaces, ew, part, c.flag, x, in1, out0, out1, T( c.a ) );
    }
    const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
+   if ( nFace > 0 )
+   {
+      // 2. one-sided sums of the z = 0 and y = 0 faces; 3. face values from the partial sums
+      if ( g.nFlatItems > 0 )
+         hipLaunchKernelGGL( ( kFaceFlat< T > ), dim3( xcdGrid( ceilDiv( g.nFlatItems, 4 ) ) ), dim3( kBlock ), 0, s, g.flatItems,
+                             g.nFlatItems, L, g.stride, m.cellFaces, ew, part, c.flag, x );
+      hipLaunchKernelGGL( ( kFaceCombine< T, MODE > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
+                          g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, m.mesh->nV() + m.mesh->nE(), part, pi, L,
+                          g.stride, c.flag, x, in1, out0, out1, T( c.a ) );
+   }
+   // 4. macro-edges and -vertices
    if ( nLow > 0 )
       hipLaunchKernelGGL( ( kLower< T, MODE > ), dim3( xcdGrid( ceilDiv( nLow, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, g.chunks,
-                          nLow, m.prims, m.entries, ew, pi, L, g.stride, c.flag, (const T*) c.x, (const T*) c.in1,
-                          (T*) c.out0, (T*) c.out1, T( c.a ) );
-   if ( nFace > 0 )
-      hipLaunchKernelGGL( ( kFace< T, MODE > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
-                          g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, ew, pi, L, g.stride, c.flag, (const T*) c.x,
-                          (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
+                          nLow, m.prims, m.entries, ew, pi, L, g.stride, c.flag, x, in1, out0, out1, T( c.a ) );
 }
 
 template < typename T >
