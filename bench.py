@@ -129,6 +129,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cheb-order", type=int, default=2)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default cell-kernel configuration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -147,6 +148,12 @@ def main():
     rng = np.random.default_rng(2024)
     op = hyg.P1ConstantOperator(mesh, 0, L)
     lib = hyg.load()
+    # per-level choice of the cell-kernel configuration (rows x points per lane) for both value types
+    tuned = {}
+    if not args.no_tune:
+        for lev in range(max(3, L - 4), L + 1):
+            for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):
+                tuned[f"L{lev}_{name}"] = op.tune(lev, dt, reps=3 if lev >= 7 else 10, stream=st)
 
     # ---------------- apply: the timed steps
     x = hyg.Function(mesh, L, L)
@@ -192,8 +199,11 @@ def main():
                    "mesh": args.mesh, "level": L, "dofs_total": n, "dofs_inner": inner, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": profile_traffic(),
-                     "kernel": "kStencil<double, MODE_APPLY>", "algorithmic_bytes_per_launch": alg_bytes},
+                     "kernel": "P1 apply (kCellZ + kFaceFlat + kFaceCombine + kLower, one launch each)",
+                     "algorithmic_bytes_per_launch": alg_bytes},
         "copy_reference": {"bytes_per_launch": 16.0 * nval, "ms": round(ms_copy, 5), "GB/s": round(copy_gbs, 1)},
+        "cell_kernel_config": {"apply": "rows %d x points %d" % op.config(L, hyg.F64, 0)[1:],
+                               "tuned_per_mode": tuned},
     }
 
     # ---------------- time to 1e-10: FP64 vs mixed precision

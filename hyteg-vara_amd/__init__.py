@@ -54,6 +54,9 @@ def _sig(lib):
         "hyg_op_destroy": (i, [vp]),
         "hyg_op_cell_stencil": (i, [vp, i, i, P(np.float64)]),
         "hyg_apply": (i, [vp, vp, vp, i, i, i, vp]),
+        "hyg_op_tune": (i, [vp, i, i, i, vp, vp]),
+        "hyg_op_set_config": (i, [vp, i, i, i, i]),
+        "hyg_op_get_config": (i, [vp, i, i, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "hyg_smooth_jac": (i, [vp, vp, vp, vp, d, i, i, vp]),
         "hyg_inverse_diagonal": (i, [vp, vp, i, vp]),
         "hyg_residual": (i, [vp, vp, vp, vp, i, i, vp]),
@@ -219,6 +222,20 @@ class P1ConstantOperator:
 
     def residual(self, r, b, x, L, flag=INNER):
         check(load().hyg_residual(self.h, r.h, b.h, x.h, L, flag, None))
+
+    def tune(self, L, dtype=F64, reps=5, stream=None):
+        """Time every cell-kernel configuration per stencil mode at level L and keep the fastest."""
+        chosen = (C.c_int * 6)()
+        check(load().hyg_op_tune(self.h, L, dtype, reps, chosen, stream))
+        return list(chosen)
+
+    def config(self, L, dtype=F64, mode=0):
+        c, r, p = C.c_int(), C.c_int(), C.c_int()
+        check(load().hyg_op_get_config(self.h, L, dtype, mode, C.byref(c), C.byref(r), C.byref(p)))
+        return c.value, r.value, p.value
+
+    def set_config(self, L, dtype, mode, cfg):
+        check(load().hyg_op_set_config(self.h, L, dtype, mode, cfg))
 
     def power_iteration(self, x, tmp, iters, L):
         out = C.c_double()

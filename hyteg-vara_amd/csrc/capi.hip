@@ -708,6 +708,110 @@ int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 )
    return HYG_OK;
 }
 
+int hyg_op_tune( hyg_op op, int level, int dtype, int reps, int* chosen, void* stream )
+{
+   HYG_TRY( checkOpLevel( op, level ) );
+   if ( dtype != HYG_F64 && dtype != HYG_F32 )
+      return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
+   if ( reps < 1 )
+      return fail( HYG_ERR_ARG, "reps must be positive" );
+   MeshCtx&    mc = op->mesh->ctx;
+   LevelGeom&  g  = mc.geom( level );
+   hipStream_t st = S( stream );
+   hyg_func_s *a = nullptr, *b = nullptr, *r = nullptr, *t = nullptr;
+   auto        release = [&]() {
+      for ( hyg_func_s* f : { a, b, r, t } )
+         destroyFunc( f );
+   };
+   int rc = createFunc( op->mesh, level, level, dtype, &a );
+   if ( rc == HYG_OK )
+      rc = createFunc( op->mesh, level, level, dtype, &b );
+   if ( rc == HYG_OK )
+      rc = createFunc( op->mesh, level, level, dtype, &r );
+   if ( rc == HYG_OK )
+      rc = createFunc( op->mesh, level, level, dtype, &t );
+   if ( rc != HYG_OK )
+   {
+      release();
+      return rc;
+   }
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   if ( hipEventCreate( &e0 ) != hipSuccess || hipEventCreate( &e1 ) != hipSuccess )
+   {
+      release();
+      return fail( HYG_ERR_HIP, "hipEventCreate failed" );
+   }
+   // inputs: 1 everywhere (the timing does not depend on the values)
+   const double one = 1.0;
+   for ( hyg_func_s* f : { a, b, r, t } )
+      lincombImpl( f, 0, nullptr, nullptr, false, one, level, HYG_ALL, st );
+   int status = HYG_OK;
+   for ( int mode = 0; mode < 6 && status == HYG_OK; ++mode )
+   {
+      StencilCall c;
+      c.mode = mode;
+      c.flag = HYG_INNER;
+      c.a    = 0.5;
+      c.x    = a->data[level];
+      c.in1  = b->data[level];
+      c.out0 = r->data[level];
+      c.out1 = t->data[level];
+      float best = 0.0f;
+      int   bestCfg = 0;
+      for ( int cfg = 0; cfg < kNumCellCfgs && status == HYG_OK; ++cfg )
+      {
+         setCellConfig( g, dtype, mode, cfg );
+         for ( int k = 0; k < 2; ++k )
+            launchStencil( mc, g, op->levels[level], level, dtype, c, st );
+         (void) hipEventRecord( e0, st );
+         for ( int k = 0; k < reps; ++k )
+            launchStencil( mc, g, op->levels[level], level, dtype, c, st );
+         (void) hipEventRecord( e1, st );
+         float ms = 0.0f;
+         if ( hipEventSynchronize( e1 ) != hipSuccess || hipEventElapsedTime( &ms, e0, e1 ) != hipSuccess )
+            status = fail( HYG_ERR_HIP, "timing the stencil configurations failed" );
+         else if ( cfg == 0 || ms < best )
+         {
+            best    = ms;
+            bestCfg = cfg;
+         }
+      }
+      setCellConfig( g, dtype, mode, bestCfg );
+      if ( chosen )
+         chosen[mode] = bestCfg;
+   }
+   (void) hipEventDestroy( e0 );
+   (void) hipEventDestroy( e1 );
+   release();
+   if ( status != HYG_OK )
+      return status;
+   return launched( "tune" );
+}
+
+int hyg_op_set_config( hyg_op op, int level, int dtype, int mode, int config )
+{
+   HYG_TRY( checkOpLevel( op, level ) );
+   if ( ( dtype != HYG_F64 && dtype != HYG_F32 ) || mode < 0 || mode > 5 || config < 0 || config >= kNumCellCfgs )
+      return fail( HYG_ERR_ARG, "bad dtype, mode or configuration" );
+   setCellConfig( op->mesh->ctx.geom( level ), dtype, mode, config );
+   return HYG_OK;
+}
+
+int hyg_op_get_config( hyg_op op, int level, int dtype, int mode, int* config, int* rows, int* points )
+{
+   HYG_TRY( checkOpLevel( op, level ) );
+   if ( ( dtype != HYG_F64 && dtype != HYG_F32 ) || mode < 0 || mode > 5 )
+      return fail( HYG_ERR_ARG, "bad dtype or mode" );
+   const int c = cellConfig( op->mesh->ctx.geom( level ), dtype, mode );
+   if ( config )
+      *config = c;
+   if ( rows )
+      *rows = kCellRows[c];
+   if ( points )
+      *points = kCellPoints[c];
+   return HYG_OK;
+}
+
 int hyg_apply( hyg_op op, hyg_func src, hyg_func dst, int level, int flag, int update, void* stream )
 {
    HYG_TRY( checkOpLevel( op, level ) );

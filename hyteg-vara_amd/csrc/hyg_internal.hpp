@@ -40,9 +40,9 @@ inline i64 cellStride( int L )
    const i64 w = ( i64( 1 ) << L ) + 1;
    return ( S3( w ) + 63 ) / 64 * 64;
 }
-// values allocated after the last cell array of a level: row-window kernels read up to 64 values
+// values allocated after the last cell array of a level: row-window kernels read up to 128 values
 // past a cell's last point (and discard them)
-constexpr i64 kTailSlack = 128;
+constexpr i64 kTailSlack = 256;
 
 // stencil directions in the reference order C W E N S NW SE TC TW TS TSE BC BN BE BNW
 // (src/hyteg/p1functionspace/VertexDoFIndexing.cpp:484-520)
@@ -101,8 +101,8 @@ struct PlaneTile
    int s0;
 };
 
-// Work of one z-marching wavefront: rows y .. y+R-1 (R = rows per wavefront of the table), points
-// x0-1 .. x0+62 (outputs x0 .. x0+61), planes [z0, z1).
+// Work of one z-marching wavefront: rows y .. y+R-1 and points x0-1 .. x0+64P-2 (outputs
+// x0 .. x0+64P-3), R rows and P points per lane of the table's configuration, planes [z0, z1).
 struct ZSeg
 {
    int y;
@@ -111,9 +111,10 @@ struct ZSeg
    int z1;
 };
 constexpr int kZSegChunk = 64; // planes per z-marching segment
-// cell-interior kernel configurations: rows per wavefront
-constexpr int kCellRows[]   = { 1, 2, 4 };
-constexpr int kNumCellCfgs  = 3;
+// cell-interior kernel configurations: rows per wavefront, points per lane
+constexpr int kCellRows[]   = { 1, 2, 4, 1, 2, 4 };
+constexpr int kCellPoints[] = { 1, 1, 1, 2, 2, 2 };
+constexpr int kNumCellCfgs  = 6;
 
 // Lower-dimensional primitive (macro-vertex, -edge or -face); order: vertices, edges, faces.
 struct LowerPrim
@@ -210,7 +211,13 @@ struct LevelGeom
    std::vector< ZSeg >                      zSegsHost[kNumCellCfgs];
    ZSeg*                                    zSegs[kNumCellCfgs]  = {};
    int                                      nZSegs[kNumCellCfgs] = {};
-   int                                      cellCfg[2]           = { -1, -1 }; // chosen configuration (FP64, FP32)
+   int                                      cellCfg[2][6]; // chosen configuration per value type (FP64, FP32) and stencil mode, -1 = default
+   LevelGeom()
+   {
+      for ( auto& t : cellCfg )
+         for ( int& v : t )
+            v = -1;
+   }
    std::vector< int4 >                      flatItemsHost; // z = 0 / y = 0 face rows: (cell, slot, line, x0)
    int4*                                    flatItems  = nullptr;
    int                                      nFlatItems = 0;

@@ -31,9 +31,9 @@ struct StencilCall
 void launchStencil( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, int dtype, const StencilCall& c,
                     hipStream_t s );
 
-// cell-interior kernel configuration (index into kCellRows) of a level and value type
-int  cellConfig( LevelGeom& g, int dtype );
-void setCellConfig( LevelGeom& g, int dtype, int cfg );
+// cell-interior kernel configuration (index into kCellRows / kCellPoints) of a level, value type and mode
+int  cellConfig( LevelGeom& g, int dtype, int mode );
+void setCellConfig( LevelGeom& g, int dtype, int mode, int cfg );
 
 // Pointwise linear combination over all cell slots selected by flag:
 // dst = (accumulate ? dst : 0) + sum_k s_k src_k  (nsrc <= 3), or dst = value if nsrc == 0.

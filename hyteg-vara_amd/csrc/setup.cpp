@@ -246,11 +246,11 @@ LevelGeom& MeshCtx::geom( int L )
    // on the diagonal face.
    for ( int k = 0; k < kNumCellCfgs; ++k )
    {
-      const int R = kCellRows[k];
+      const int R = kCellRows[k], WO = 64 * kCellPoints[k] - 2;
       auto&     v = g.zSegsHost[k];
       v.clear();
       for ( int z0 = 1; z0 <= N - 2; z0 += kZSegChunk )
-         for ( int x0 = 1; x0 <= N - 2; x0 += 62 )
+         for ( int x0 = 1; x0 <= N - 2; x0 += WO )
             for ( int y = 1; y <= N - 2; y += R )
             {
                const int zEnd = N - y - x0 + 1;

@@ -27,20 +27,88 @@
 namespace hyg {
 
 // ------------------------------------------------------------------ cell interior, z-marching wavefronts
-// R output rows per wavefront: one plane of the register ring holds rows y0-1 .. y0+R of x and
-// rows y0 .. y0+R-1 of the epilogue input.
-template < typename T, int R >
+// R output rows and P points per lane: one plane of the register ring holds rows y0-1 .. y0+R of x
+// and rows y0 .. y0+R-1 of the epilogue input; lane l holds x = x0-1+P*l .. x0-1+P*l+P-1.
+template < typename T, int R, int P >
 struct PlaneRows
 {
-   T r[R + 2];
-   T e[R];
+   T r[R + 2][P];
+   T e[R][P];
 };
 
 // directions without an x shift and with x+1 (E group)
 constexpr int kDirU[7] = { 0, 3, 4, 7, 9, 11, 12 };
 constexpr int kDirE[4] = { 2, 6, 10, 13 };
 
-template < typename T, int MODE, int R >
+// one row of a stencil sweep: the 7 unshifted, 4 x-1 (W, NW, TW, BNW) and 4 x+1 (E, SE, TSE, BE)
+// terms for the P points of a lane; wu / we are the per-lane weights of the first point's
+// unshifted terms and of the E-group terms formed for the first point (see kCellZ).
+template < typename T, int P >
+__device__ __forceinline__ void stencilRow( const T* w, const T* wu, const T* we, const T ( &bc )[P], const T ( &bn )[P],
+                                            const T ( &bs )[P], const T ( &cc )[P], const T ( &cs )[P], const T ( &ac )[P],
+                                            const T ( &an )[P], T ( &ax )[P] )
+{
+   if constexpr ( P == 1 )
+   {
+      T a = wu[0] * bc[0];
+      a += wu[1] * bn[0];
+      a += wu[2] * bs[0];
+      a += wu[3] * cc[0];
+      a += wu[4] * cs[0];
+      a += wu[5] * ac[0];
+      a += wu[6] * an[0];
+      T sr = w[1] * bc[0];
+      sr += w[5] * bn[0];
+      sr += w[8] * cc[0];
+      sr += w[14] * an[0];
+      T sl = we[0] * bc[0];
+      sl += we[1] * bs[0];
+      sl += we[2] * cs[0];
+      sl += we[3] * ac[0];
+      a += laneMove< kDppWaveShr1 >( sr );
+      a += laneMove< kDppWaveShl1 >( sl );
+      ax[0] = a;
+   }
+   else
+   {
+      // point 0: x-1 is point 1 of lane-1 (shifted), x+1 is point 1 of this lane;
+      // point 1: x-1 is point 0 of this lane, x+1 is point 0 of lane+1 (shifted)
+      T a0 = wu[0] * bc[0];
+      a0 += wu[1] * bn[0];
+      a0 += wu[2] * bs[0];
+      a0 += wu[3] * cc[0];
+      a0 += wu[4] * cs[0];
+      a0 += wu[5] * ac[0];
+      a0 += wu[6] * an[0];
+      a0 += we[0] * bc[1];
+      a0 += we[1] * bs[1];
+      a0 += we[2] * cs[1];
+      a0 += we[3] * ac[1];
+      T a1 = w[0] * bc[1];
+      a1 += w[3] * bn[1];
+      a1 += w[4] * bs[1];
+      a1 += w[7] * cc[1];
+      a1 += w[9] * cs[1];
+      a1 += w[11] * ac[1];
+      a1 += w[12] * an[1];
+      a1 += w[1] * bc[0];
+      a1 += w[5] * bn[0];
+      a1 += w[8] * cc[0];
+      a1 += w[14] * an[0];
+      T sr = w[1] * bc[1];
+      sr += w[5] * bn[1];
+      sr += w[8] * cc[1];
+      sr += w[14] * an[1];
+      T sl = w[2] * bc[0];
+      sl += w[6] * bs[0];
+      sl += w[10] * cs[0];
+      sl += w[13] * ac[0];
+      ax[0] = a0 + laneMove< kDppWaveShr1 >( sr );
+      ax[1] = a1 + laneMove< kDppWaveShl1 >( sl );
+   }
+}
+
+template < typename T, int MODE, int R, int P >
 __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ segs, int nSegs, int nC, int L, i64 stride,
                                                     const T* __restrict__ cellSt, const CellFaceSlot* __restrict__ cfs,
                                                     const T* __restrict__ entryW, T* __restrict__ part, int flag,
@@ -51,6 +119,7 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    constexpr bool kE   = kIn1 || MODE == MODE_APPLY_ADD;
    constexpr bool kTwo = MODE == MODE_CHEB0 || MODE == MODE_CHEBK;
    constexpr int  B    = int( sizeof( T ) );
+   constexpr int  WO   = 64 * P - 2; // outputs per window: x0 .. x0+WO-1
    const int      nbx  = ( nSegs + 3 ) / 4; // blocks per cell
    const int      nb   = nbx * nC;
    const int      lb   = remapXcd( int( blockIdx.x ), nb );
@@ -63,7 +132,8 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    const ZSeg     sg    = segs[si];
    const int      N     = 1 << L, NP = N + 1, y0 = sg.y, z0 = sg.z0, z1 = sg.z1, x0 = sg.x0;
    const unsigned lane  = threadIdx.x & 63u;
-   const unsigned vo    = lane * unsigned( B );
+   const int      d0    = P * int( lane ); // window offset of the lane's first point (x = x0-1+d0)
+   const unsigned vo    = unsigned( d0 * B );
    const i64      cb    = i64( c ) * stride;
    const unsigned bytes = unsigned( ( stride + kTailSlack ) * B ); // cell array plus the read-past tail
    const auto     rx    = makeRsrc( x + cb, bytes );
@@ -77,9 +147,9 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
       w[d] = st[d];
    const bool interior = ( flag & 1 ) != 0;
 
-   // x = 0 face (slot 2): lane 0 of the windows starting at a row start, with that face's one-sided
-   // weights; the E-group terms lane 0 needs are formed on lane 1, so lane 1 holds the face weights
-   // of that group.
+   // x = 0 face (slot 2): the first point of lane 0 in the windows starting at a row start, with that
+   // face's one-sided weights. Its E-group terms are formed on lane 1 (P = 1, then shifted) or on
+   // lane 0 itself (P = 2), which therefore holds that face's E-group weights.
    const CellFaceSlot fx   = cfs[4 * c + 2];
    const bool         onX0 = x0 == 1 && ( fx.bc & flag ) != 0;
    T                  wu[7], we[4];
@@ -90,23 +160,34 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
          wu[k] = ( onX0 && lane == 0u ) ? wf[kDirU[k]] : w[kDirU[k]];
 #pragma unroll
       for ( int k = 0; k < 4; ++k )
-         we[k] = ( onX0 && lane == 1u ) ? wf[kDirE[k]] : w[kDirE[k]];
+         we[k] = ( onX0 && lane == ( P == 1 ? 1u : 0u ) ) ? wf[kDirE[k]] : w[kDirE[k]];
    }
    // diagonal face (slot 3): the row ends x = N-y-z of rows y0..y0+R-1 inside the output window
    const CellFaceSlot fd     = cfs[4 * c + 3];
    const int          xeHi   = N - y0 - z0, xeLo = N - ( y0 + R - 1 ) - ( z1 - 1 );
-   const bool         onDiag = ( fd.bc & flag ) != 0 && xeHi >= x0 && xeLo <= x0 + 61;
-   T                  wd[15];
+   const bool         onDiag = ( fd.bc & flag ) != 0 && xeHi >= x0 && xeLo <= x0 + WO - 1;
+   T                  wd[15], wdu[7], wde[4];
    {
       const T* wf = entryW + 16 * i64( onDiag ? fd.entry : 0 );
 #pragma unroll
       for ( int d = 0; d < NDIR; ++d )
          wd[d] = onDiag ? wf[d] : T( 0 );
+#pragma unroll
+      for ( int k = 0; k < 7; ++k )
+         wdu[k] = wd[kDirU[k]];
+#pragma unroll
+      for ( int k = 0; k < 4; ++k )
+         wde[k] = wd[kDirE[k]];
    }
-   T f2v[R], dg[R]; // face partial sums gathered over the march
+   T f2v[R], dg[R][P]; // face partial sums gathered over the march
 #pragma unroll
    for ( int r = 0; r < R; ++r )
-      f2v[r] = dg[r] = T( 0 );
+   {
+      f2v[r] = T( 0 );
+#pragma unroll
+      for ( int k = 0; k < P; ++k )
+         dg[r][k] = T( 0 );
+   }
 
    // load front: offset of (x0-1, y0-1) in plane pl, whose row 0 has Wl points. Every step issues the
    // same loads (the front stops at the last plane needed, min(z1, N), and repeats it), so the
@@ -115,19 +196,28 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    int       pl    = z0 - 1;
    int       Wl    = N + 1 - pl;
    int       off   = S3i( N + 1 ) - S3i( Wl ) + rowStart( Wl, y0 - 1 ) + x0 - 1;
-   // lanes past the end of a row take the out-of-range offset: they read 0 and fetch nothing
-   const int laneEnd = N + 2 - x0 - y0; // last lane of row y0-1 in plane 0
-   auto      loadPlane = [&]( PlaneRows< T, R >& P ) {
+   // lanes whose first point lies past the end of a row take the out-of-range offset: they read 0 and
+   // fetch nothing (a second point past the end reads a value of the next row and is never used)
+   const int laneEnd = N + 2 - x0 - y0; // window offset of the last point of row y0-1 in plane 0
+   auto      loadPlane = [&]( PlaneRows< T, R, P >& Q ) {
       int o = off;
 #pragma unroll
       for ( int k = 0; k < R + 2; ++k )
       {
-         const int le = laneEnd - pl - k; // last lane holding a point of row y0-1+k in plane pl
-         P.r[k]       = bufLoad< T >( rx, int( lane ) <= le ? vo : kDropOffset, o * B );
+         const int le = laneEnd - pl - k; // window offset of the last point of row y0-1+k in plane pl
+         if constexpr ( P == 1 )
+            Q.r[k][0] = bufLoad< T >( rx, d0 <= le ? vo : kDropOffset, o * B );
+         else
+            bufLoad2( rx, d0 <= le ? vo : kDropOffset, o * B, Q.r[k][0], Q.r[k][1] );
          if constexpr ( kE )
          {
             if ( k >= 1 && k <= R )
-               P.e[k - 1] = bufLoad< T >( re, int( lane ) < le ? vo : kDropOffset, o * B );
+            {
+               if constexpr ( P == 1 )
+                  Q.e[k - 1][0] = bufLoad< T >( re, d0 < le ? vo : kDropOffset, o * B );
+               else
+                  bufLoad2( re, d0 < le ? vo : kDropOffset, o * B, Q.e[k - 1][0], Q.e[k - 1][1] );
+            }
          }
          o += Wl - ( y0 - 1 + k ); // start of the next row
       }
@@ -140,9 +230,10 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    };
    // output plane z: offset oz of (x0-1, y0-1), row 0 of the plane has Wz points
    int       z = z0, oz = 0, Wz = N + 1 - z0;
-   const int lastLane = N - y0 - x0; // lane of x = N-1-y-z is lastLane - (y - y0) - z
+   const int lastD = N - y0 - x0; // window offset of x = N-1-y-z is lastD - (y - y0) - z
    // A, B, C: planes z-1, z, z+1 (resident); plane z+2 is in flight; E receives plane z+3
-   auto step = [&]( const PlaneRows< T, R >& A, const PlaneRows< T, R >& Bp, const PlaneRows< T, R >& C, PlaneRows< T, R >& E ) {
+   auto step = [&]( const PlaneRows< T, R, P >& A, const PlaneRows< T, R, P >& Bp, const PlaneRows< T, R, P >& C,
+                    PlaneRows< T, R, P >& E ) {
       loadPlane( E );
       int o = oz + ( Wz - ( y0 - 1 ) ); // row y0 of plane z
 #pragma unroll
@@ -150,42 +241,49 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
       {
          // plane z: C = row r+1, N = row r+2, S = row r; plane z+1: TC = row r+1, TS = row r;
          // plane z-1: BC = row r+1, BN = row r+2
-         const T bc = Bp.r[r + 1], bn = Bp.r[r + 2], bs = Bp.r[r], cc = C.r[r + 1], cs = C.r[r], ac = A.r[r + 1],
-                 an = A.r[r + 2];
-         T Ax = wu[0] * bc;
-         Ax += wu[1] * bn;
-         Ax += wu[2] * bs;
-         Ax += wu[3] * cc;
-         Ax += wu[4] * cs;
-         Ax += wu[5] * ac;
-         Ax += wu[6] * an;
-         // terms read at x-1 (W, NW, TW, BNW) and at x+1 (E, SE, TSE, BE)
-         T sr = w[1] * bc;
-         sr += w[5] * bn;
-         sr += w[8] * cc;
-         sr += w[14] * an;
-         T sl = we[0] * bc;
-         sl += we[1] * bs;
-         sl += we[2] * cs;
-         sl += we[3] * ac;
-         Ax += laneMove< kDppWaveShr1 >( sr );
-         Ax += laneMove< kDppWaveShl1 >( sl );
-         T o0, o1;
-         epilogue< T, MODE >( Ax, bc, Bp.e[r], Bp.e[r], w[15], a, o0, o1 );
-         const unsigned so = ( interior && lane - 1u < 62u && int( lane ) <= lastLane - r - z ) ? vo : kDropOffset;
-         bufStore( o0, r0, so, o * B );
-         if constexpr ( kTwo )
-            bufStore( o1, r1, so, o * B );
-         if ( onX0 ) // lane 0 holds the x = 0 face sum of row y0+r, plane z
-            f2v[r] = moveLane( Ax, 0, z - z0, f2v[r] );
+         T ax[P];
+         stencilRow< T, P >( w, wu, we, Bp.r[r + 1], Bp.r[r + 2], Bp.r[r], C.r[r + 1], C.r[r], A.r[r + 1], A.r[r + 2], ax );
+         T o0[P], o1[P];
+         bool ok[P];
+#pragma unroll
+         for ( int k = 0; k < P; ++k )
+         {
+            epilogue< T, MODE >( ax[k], Bp.r[r + 1][k], Bp.e[r][k], Bp.e[r][k], w[15], a, o0[k], o1[k] );
+            const int d = d0 + k;
+            ok[k]       = interior && d >= 1 && d <= WO && d <= lastD - r - z;
+         }
+         if constexpr ( P == 1 )
+         {
+            const unsigned so = ok[0] ? vo : kDropOffset;
+            bufStore( o0[0], r0, so, o * B );
+            if constexpr ( kTwo )
+               bufStore( o1[0], r1, so, o * B );
+         }
+         else
+         {
+            // both points of the lane in one store; a lane with one output point (window start, row
+            // end) stores it with a single-value store
+            const unsigned s2 = ( ok[0] && ok[1] ) ? vo : kDropOffset;
+            const unsigned s1 = ( ok[0] != ok[1] ) ? ( ok[0] ? vo : vo + unsigned( B ) ) : kDropOffset;
+            bufStore2( o0[0], o0[1], r0, s2, o * B );
+            bufStore( ok[0] ? o0[0] : o0[1], r0, s1, o * B );
+            if constexpr ( kTwo )
+            {
+               bufStore2( o1[0], o1[1], r1, s2, o * B );
+               bufStore( ok[0] ? o1[0] : o1[1], r1, s1, o * B );
+            }
+         }
+         if ( onX0 ) // the first point of lane 0 is the x = 0 face point of row y0+r, plane z
+            f2v[r] = moveLane( ax[0], 0, z - z0, f2v[r] );
          if ( onDiag )
          {
-            T ad = wd[0] * bc + wd[3] * bn + wd[4] * bs + wd[7] * cc + wd[9] * cs + wd[11] * ac + wd[12] * an;
-            T dr = wd[1] * bc + wd[5] * bn + wd[8] * cc + wd[14] * an;
-            T dl = wd[2] * bc + wd[6] * bs + wd[10] * cs + wd[13] * ac;
-            ad += laneMove< kDppWaveShr1 >( dr );
-            ad += laneMove< kDppWaveShl1 >( dl );
-            dg[r] = int( lane ) == lastLane + 1 - r - z ? ad : dg[r];
+            T ad[P];
+            stencilRow< T, P >( wd, wdu, wde, Bp.r[r + 1], Bp.r[r + 2], Bp.r[r], C.r[r + 1], C.r[r], A.r[r + 1], A.r[r + 2],
+                                ad );
+            const int dEnd = lastD + 1 - r - z; // window offset of the row end
+#pragma unroll
+            for ( int k = 0; k < P; ++k )
+               dg[r][k] = d0 + k == dEnd ? ad[k] : dg[r][k];
          }
          o += Wz - ( y0 + r );
       }
@@ -193,10 +291,12 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
       --Wz;
       ++z;
    };
-   PlaneRows< T, R > R0, R1, R2, R3, R4;
+   PlaneRows< T, R, P > R0, R1, R2, R3, R4;
 #pragma unroll
    for ( int k = 0; k < R; ++k )
-      R0.e[k] = R1.e[k] = R2.e[k] = R3.e[k] = R4.e[k] = T( 0 );
+#pragma unroll
+      for ( int q = 0; q < P; ++q )
+         R0.e[k][q] = R1.e[k][q] = R2.e[k][q] = R3.e[k][q] = R4.e[k][q] = T( 0 );
    loadPlane( R0 ); // plane z0-1
    oz = off;        // plane z0 (z0 < z1 <= plMax, so the front advanced)
    loadPlane( R1 );
@@ -240,12 +340,15 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
       const auto rp = makeRsrc( part + i64( fd.block ) * NP * NP, pb );
 #pragma unroll
       for ( int r = 0; r < R; ++r )
-      {
-         // lane l held the row end x = x0-1+l of row y0+r at plane z = N - y - x
-         const int  zz = N - ( y0 + r ) - ( x0 - 1 + int( lane ) );
-         const bool ok = lane - 1u < 62u && zz >= z0 && zz < z1 && zz >= 1;
-         bufStore( dg[r], rp, ok ? unsigned( facePartIndex( 3, N, 0, y0 + r, zz ) * B ) : kDropOffset, 0 );
-      }
+#pragma unroll
+         for ( int k = 0; k < P; ++k )
+         {
+            // this point held the row end x = x0-1+d of row y0+r at plane z = N - y - x
+            const int  d  = d0 + k;
+            const int  zz = N - ( y0 + r ) - ( x0 - 1 + d );
+            const bool ok = d >= 1 && d <= WO && zz >= z0 && zz < z1 && zz >= 1;
+            bufStore( dg[r][k], rp, ok ? unsigned( facePartIndex( 3, N, 0, y0 + r, zz ) * B ) : kDropOffset, 0 );
+         }
    }
 }
 
@@ -414,10 +517,11 @@ __global__ __launch_bounds__( kBlock ) void kLower( const LowerChunk* __restrict
 }
 
 // ------------------------------------------------------------------ launch
-// cell-interior configuration of a level: set by the tuner, else HYG_CELL_CFG, else rows = 2
-int cellConfig( LevelGeom& g, int dtype )
+// cell-interior configuration of a level, value type and stencil mode: set by the tuner, else
+// HYG_CELL_CFG, else 2 rows x 1 point per lane
+int cellConfig( LevelGeom& g, int dtype, int mode )
 {
-   int& c = g.cellCfg[dtype == 8 ? 0 : 1];
+   int& c = g.cellCfg[dtype == 8 ? 0 : 1][mode];
    if ( c < 0 )
    {
       const char* e = getenv( "HYG_CELL_CFG" );
@@ -425,7 +529,7 @@ int cellConfig( LevelGeom& g, int dtype )
    }
    return c;
 }
-void setCellConfig( LevelGeom& g, int dtype, int cfg ) { g.cellCfg[dtype == 8 ? 0 : 1] = cfg; }
+void setCellConfig( LevelGeom& g, int dtype, int mode, int cfg ) { g.cellCfg[dtype == 8 ? 0 : 1][mode] = cfg; }
 
 template < typename T, int MODE >
 static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, const StencilCall& c, hipStream_t s )
@@ -436,7 +540,7 @@ static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol,
    const T*       ew   = dbl ? (const T*) ol.entryW : (const T*) ol.entryWF;
    const T*       pi   = dbl ? (const T*) ol.primInv : (const T*) ol.primInvF;
    T*             part = dbl ? (T*) ol.facePart : (T*) ol.facePartF;
-   const int      cfg  = cellConfig( g, dbl ? 8 : 4 );
+   const int      cfg  = cellConfig( g, dbl ? 8 : 4, MODE );
    const T*       x    = (const T*) c.x;
    const T*       in1  = (const T*) c.in1;
    T*             out0 = (T*) c.out0;
@@ -445,7 +549,16 @@ static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol,
    if ( g.nZSegs[cfg] > 0 )
    {
       const int nb = ( ( g.nZSegs[cfg] + 3 ) / 4 ) * nC;
-      auto      kf = cfg == 0 ? kCellZ< T, MODE, 1 > : ( cfg == 1 ? kCellZ< T, MODE, 2 > : kCellZ< T, MODE, 4 > );
+      decltype( &kCellZ< T, MODE, 1, 1 > ) kf;
+      switch ( cfg )
+      {
+      case 0: kf = kCellZ< T, MODE, 1, 1 >; break;
+      case 1: kf = kCellZ< T, MODE, 2, 1 >; break;
+      case 2: kf = kCellZ< T, MODE, 4, 1 >; break;
+      case 3: kf = kCellZ< T, MODE, 1, 2 >; break;
+      case 4: kf = kCellZ< T, MODE, 2, 2 >; break;
+      default: kf = kCellZ< T, MODE, 4, 2 >; break;
+      }
       hipLaunchKernelGGL( kf, dim3( xcdGrid( nb ) ), dim3( kBlock ), 0, s, g.zSegs[cfg], g.nZSegs[cfg], nC, L, g.stride, cst,
                           m.cellFaces, ew, part, c.flag, x, in1, out0, out1, T( c.a ) );
    }

@@ -113,6 +113,17 @@ int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream );
 /* residual r = b - A x on `flag` (GeometricMultigridSolver.hpp:185-186 apply + assign), fused. */
 int hyg_residual( hyg_op op, hyg_func r, hyg_func b, hyg_func x, int level, int flag, void* stream );
 
+/* Stencil kernel configurations (no counterpart in the reference, whose generated kernels have one
+ * fixed loop nest). The cell-interior kernel marches z-planes with R rows per wavefront and P points
+ * per lane; configuration k has R = {1,2,4,1,2,4}[k], P = {1,1,1,2,2,2}[k]. Modes: 0 apply,
+ * 1 apply-add, 2 residual, 3 Jacobi sweep, 4 first Chebyshev step, 5 later Chebyshev steps.
+ * hyg_op_tune times every configuration for every mode at `level` on the operator's mesh (`reps`
+ * sweeps each, on scratch functions it allocates and frees) and keeps the fastest; chosen[0..5]
+ * (may be NULL) receives the choices. The choice is shared by all operators of the mesh. */
+int hyg_op_tune( hyg_op op, int level, int dtype, int reps, int* chosen, void* stream );
+int hyg_op_set_config( hyg_op op, int level, int dtype, int mode, int config );
+int hyg_op_get_config( hyg_op op, int level, int dtype, int mode, int* config, int* rows, int* points );
+
 /* ---------------------------------------------------------------- grid transfer
  * P1toP1LinearRestriction::restrict(function, sourceLevel, flag) (gridtransferoperators/P1toP1LinearRestriction.cpp:168-345) */
 int hyg_restrict( hyg_func f, int source_level, int flag, void* stream );

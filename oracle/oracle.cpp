@@ -476,21 +476,31 @@ struct LowerStencil
    std::vector< std::array< double, NDIR > > w; // per neighbour cell
 };
 
+// Operator<Src, Dst>::apply (operators/Operator.hpp:73-80) plus the inverse diagonal used by the smoothers
 template < typename T >
-struct P1ConstantOperator
+struct OperatorBase
 {
-   const Mesh* m;
-   int         minL, maxL;
+   const Mesh*                      m = nullptr;
+   int                              minL = 0, maxL = 0;
+   std::unique_ptr< Function< T > > invDiag;
+   virtual ~OperatorBase() = default;
+   virtual void apply( const Function< T >& src, Function< T >& dst, int L, int flag, bool add ) const = 0;
+};
+
+template < typename T >
+struct P1ConstantOperator : OperatorBase< T >
+{
+   using OperatorBase< T >::m;
+   using OperatorBase< T >::invDiag;
    // [level]
    std::vector< std::vector< std::array< double, NDIR > > > cellSt;            // per cell
    std::vector< std::vector< LowerStencil > >               vSt, eSt, fSt;     // per primitive
-   std::unique_ptr< Function< T > >                         invDiag;
 
    P1ConstantOperator( const Mesh* mesh, int lo, int hi )
-   : m( mesh )
-   , minL( lo )
-   , maxL( hi )
    {
+      this->m    = mesh;
+      this->minL = lo;
+      this->maxL = hi;
       cellSt.resize( hi + 1 );
       vSt.resize( hi + 1 );
       eSt.resize( hi + 1 );
@@ -567,7 +577,7 @@ struct P1ConstantOperator
       return sum;
    }
 
-   void apply( const Function< T >& src, Function< T >& dst, int L, int flag, bool add ) const
+   void apply( const Function< T >& src, Function< T >& dst, int L, int flag, bool add ) const override
    {
       src.commDown( L );
       const idx N = idx( 1 ) << L;
@@ -903,6 +913,193 @@ void commAdditiveCellToLower( Function< T >& fn, int L, int excludeFlag, bool ze
    }
 }
 
+// ---------------------------------------------------------------- variable-coefficient elementwise operator
+// Coefficient k(x) of -div(k grad u): kind 0: k0 exp( sum_m a_m cos(2 pi kappa_m . x + phi_m) ),
+// kind 1: tanh(alpha (x - 1/2)) + 2 (the coefficient of ElementwiseDivKGradCGConvergenceTest.cpp:60-62).
+struct KField
+{
+   int                                   kind  = 0;
+   double                                k0    = 1.0;
+   double                                alpha = 2.0;
+   std::vector< std::array< double, 5 > > modes; // kappa_x, kappa_y, kappa_z, phi, a
+   double operator()( const P3& x ) const
+   {
+      if ( kind == 1 )
+         return std::tanh( alpha * ( x[0] - 0.5 ) ) + 2.0;
+      double g = 0.0;
+      for ( auto& md : modes )
+         g += md[4] * std::cos( 2.0 * M_PI * ( md[0] * x[0] + md[1] * x[1] + md[2] * x[2] ) + md[3] );
+      return k0 * std::exp( g );
+   }
+};
+
+// The six micro-tetrahedron types of a refined macro-cell (WHITE_UP, BLUE_UP, GREEN_UP, WHITE_DOWN,
+// BLUE_DOWN, GREEN_DOWN), lattice offsets of their vertices from the micro-cell index
+// (volumedofspace/CellDoFIndexing.hpp:155-198, vertexdof getVertexDoFDataIndicesFromMicroCell).
+static const int MICRO_CELL[6][4][3] = {
+    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } }, { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
+    { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } }, { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
+    { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } }, { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } };
+
+// p1_div_k_grad_affine_q3::integrateAll (3D) (forms/form_hyteg_generated/p1/p1_div_k_grad_affine_q3.cpp:179-381):
+// a_ij = sum_q w_q k(x_q) |det J| grad phi_i . grad phi_j with the 5-point degree-3 rule: the centroid with
+// weight -2/15 and the four points with barycentric coordinates (1/2, 1/6, 1/6, 1/6) with weight 3/40.
+inline void divKGradElement( const P3* x, const KField& k, double a[4][4] )
+{
+   double J[3][3];
+   for ( int r = 0; r < 3; ++r )
+      for ( int c = 0; c < 3; ++c )
+         J[r][c] = x[c + 1][r] - x[0][r];
+   const double det = J[0][0] * ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) - J[0][1] * ( J[1][0] * J[2][2] - J[1][2] * J[2][0] ) +
+                      J[0][2] * ( J[1][0] * J[2][1] - J[1][1] * J[2][0] );
+   // rows of J^-1 are the gradients of the barycentric coordinates 1..3
+   double Ji[3][3];
+   Ji[0][0] = ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) / det;
+   Ji[0][1] = ( J[0][2] * J[2][1] - J[0][1] * J[2][2] ) / det;
+   Ji[0][2] = ( J[0][1] * J[1][2] - J[0][2] * J[1][1] ) / det;
+   Ji[1][0] = ( J[1][2] * J[2][0] - J[1][0] * J[2][2] ) / det;
+   Ji[1][1] = ( J[0][0] * J[2][2] - J[0][2] * J[2][0] ) / det;
+   Ji[1][2] = ( J[0][2] * J[1][0] - J[0][0] * J[1][2] ) / det;
+   Ji[2][0] = ( J[1][0] * J[2][1] - J[1][1] * J[2][0] ) / det;
+   Ji[2][1] = ( J[0][1] * J[2][0] - J[0][0] * J[2][1] ) / det;
+   Ji[2][2] = ( J[0][0] * J[1][1] - J[0][1] * J[1][0] ) / det;
+   double G[4][3];
+   for ( int c = 0; c < 3; ++c )
+   {
+      G[1][c] = Ji[0][c];
+      G[2][c] = Ji[1][c];
+      G[3][c] = Ji[2][c];
+      G[0][c] = -( G[1][c] + G[2][c] + G[3][c] );
+   }
+   const double lam[5][4] = { { 0.25, 0.25, 0.25, 0.25 },
+                              { 1.0 / 6, 1.0 / 6, 1.0 / 6, 0.5 },
+                              { 1.0 / 6, 1.0 / 6, 0.5, 1.0 / 6 },
+                              { 1.0 / 6, 0.5, 1.0 / 6, 1.0 / 6 },
+                              { 0.5, 1.0 / 6, 1.0 / 6, 1.0 / 6 } };
+   const double wq[5]     = { -2.0 / 15.0, 3.0 / 40.0, 3.0 / 40.0, 3.0 / 40.0, 3.0 / 40.0 };
+   double       kbar      = 0.0;
+   for ( int q = 0; q < 5; ++q )
+   {
+      P3 xq{ 0, 0, 0 };
+      for ( int v = 0; v < 4; ++v )
+         for ( int c = 0; c < 3; ++c )
+            xq[c] += lam[q][v] * x[v][c];
+      kbar += wq[q] * k( xq );
+   }
+   const double s = kbar * std::fabs( det );
+   for ( int i = 0; i < 4; ++i )
+      for ( int j = 0; j < 4; ++j )
+         a[i][j] = s * ( G[i][0] * G[j][0] + G[i][1] * G[j][1] + G[i][2] * G[j][2] );
+}
+
+// P1ElementwiseOperator<p1_div_k_grad_affine_q3> (elementwiseoperators/P1ElementwiseOperator.cpp:64-220, 275-301,
+// computeDiagonalOperatorValues :304-415): element loops over the micro-cells of every macro-cell accumulate into
+// the cell arrays, whose boundary values are then summed into the lower-dimensional primitives.
+template < typename T >
+struct P1ElementwiseDivKGrad : OperatorBase< T >
+{
+   using OperatorBase< T >::m;
+   using OperatorBase< T >::invDiag;
+   KField k;
+
+   P1ElementwiseDivKGrad( const Mesh* mesh, int lo, int hi, const KField& kf )
+   : k( kf )
+   {
+      this->m    = mesh;
+      this->minL = lo;
+      this->maxL = hi;
+      invDiag    = std::make_unique< Function< T > >( m, lo, hi );
+      for ( int L = lo; L <= hi; ++L )
+         computeInverseDiagonal( L );
+   }
+
+   // fn( cellIndex, vertex array indices, element matrix ) for every micro-cell of cell c
+   template < typename Fn >
+   void forElements( int c, int L, Fn&& fn ) const
+   {
+      const idx N = idx( 1 ) << L;
+      const idx w = N + 1;
+      for ( int t = 0; t < 6; ++t )
+         for ( idx z = 0; z <= N; ++z )
+            for ( idx y = 0; y + z <= N; ++y )
+               for ( idx x = 0; x + y + z <= N; ++x )
+               {
+                  idx vi[4];
+                  P3  xc[4];
+                  bool inside = true;
+                  for ( int v = 0; v < 4; ++v )
+                  {
+                     const idx a = x + MICRO_CELL[t][v][0], b = y + MICRO_CELL[t][v][1], d = z + MICRO_CELL[t][v][2];
+                     if ( !inCell( N, a, b, d ) )
+                     {
+                        inside = false;
+                        break;
+                     }
+                     vi[v] = cidx( w, a, b, d );
+                     xc[v] = microCoord( *m, c, L, a, b, d );
+                  }
+                  if ( !inside )
+                     continue;
+                  double el[4][4];
+                  divKGradElement( xc, k, el );
+                  fn( vi, el );
+               }
+   }
+
+   void apply( const Function< T >& src, Function< T >& dst, int L, int flag, bool add ) const override
+   {
+      src.commDown( L );
+      if ( !add )
+         interpolate< T >( dst, 0.0, L, flag );
+      const idx N = idx( 1 ) << L;
+      const idx w = N + 1;
+#pragma omp parallel for schedule( dynamic )
+      for ( idx c = 0; c < (idx) m->Cl.size(); ++c )
+      {
+         auto&       dc = dst.c[L][c];
+         const auto& sc = src.c[L][c];
+         // zero the cell's copies of face, edge and vertex values only
+         for ( idx z = 0; z <= N; ++z )
+            for ( idx y = 0; y + z <= N; ++y )
+               for ( idx x = 0; x + y + z <= N; ++x )
+                  if ( x == 0 || y == 0 || z == 0 || x + y + z == N )
+                     dc[cidx( w, x, y, z )] = T( 0 );
+         forElements( (int) c, L, [&]( const idx* vi, const double( *el )[4] ) {
+            for ( int i = 0; i < 4; ++i )
+            {
+               double s = 0.0;
+               for ( int j = 0; j < 4; ++j )
+                  s += el[i][j] * double( sc[vi[j]] );
+               dc[vi[i]] += T( s );
+            }
+         } );
+      }
+      commAdditiveCellToLower( dst, L, All ^ flag, !add );
+   }
+
+   void computeInverseDiagonal( int L )
+   {
+      auto&     D = *invDiag;
+      const idx N = idx( 1 ) << L;
+      interpolate< T >( D, 0.0, L, All );
+      for ( idx c = 0; c < (idx) m->Cl.size(); ++c )
+      {
+         auto& dc = D.c[L][c];
+         std::fill( dc.begin(), dc.end(), T( 0 ) );
+         forElements( (int) c, L, [&]( const idx* vi, const double( *el )[4] ) {
+            for ( int i = 0; i < 4; ++i )
+               dc[vi[i]] += T( el[i][i] );
+         } );
+      }
+      commAdditiveCellToLower( D, L, None, true );
+      (void) N;
+      forOwned< T >( *m, L, All, [&]( int kind, int p, idx i ) {
+         T& v = at( D, L, kind, p, i );
+         v    = T( 1.0 ) / v;
+      } );
+   }
+};
+
 // P1toP1LinearRestriction::restrict3D with the generated kernel's per-leaf scaling
 template < typename T >
 void restrict3D( Function< T >& fn, int srcL, int flag )
@@ -983,7 +1180,7 @@ template < typename T >
 struct Solvers
 {
    const Mesh*                       m;
-   const P1ConstantOperator< T >*    A;
+   const OperatorBase< T >*          A;
    int                               minL, maxL;
    std::unique_ptr< Function< T > >  tmp, jtmp, ct1, ct2, cgp, cgz, cgap, cgr;
    std::vector< double >             chebCoeffs;
@@ -994,7 +1191,7 @@ struct Solvers
    double                            cgTol     = 1e-16;
    const int                         gmgFlag   = Inner | NeumannBoundary;
 
-   Solvers( const Mesh* mesh, const P1ConstantOperator< T >* op, int lo, int hi )
+   Solvers( const Mesh* mesh, const OperatorBase< T >* op, int lo, int hi )
    : m( mesh )
    , A( op )
    , minL( lo )
@@ -1181,8 +1378,8 @@ struct OrOp
 {
    int                                             prec;
    const Mesh*                                     m;
-   std::unique_ptr< P1ConstantOperator< double > > d;
-   std::unique_ptr< P1ConstantOperator< float > >  f;
+   std::unique_ptr< OperatorBase< double > >        d;
+   std::unique_ptr< OperatorBase< float > >         f;
 };
 
 template < typename T >
@@ -1249,7 +1446,7 @@ static void getCanonical( const Function< T >& fn, int L, double* out )
 //                 dp = {relax, chebLower, chebUpper, cgTol}
 // Runs `cycles` V-cycles on x with rhs b at level maxL; res[0] = initial residual norm, res[k] after cycle k.
 template < typename T >
-static void gmgRun( OrOp* o, P1ConstantOperator< T >* A, Function< T >& x, Function< T >& b, const int* ip, const double* dp,
+static void gmgRun( OrOp* o, OperatorBase< T >* A, Function< T >& x, Function< T >& b, const int* ip, const double* dp,
                     int cycles, double* res )
 {
    Solvers< T > S( o->m, A, ip[0], ip[1] );
@@ -1376,13 +1573,42 @@ void* or_op_new( void* mh, int minL, int maxL, int prec )
 }
 void or_op_free( void* o ) { delete static_cast< OrOp* >( o ); }
 
+// P1ElementwiseOperator<p1_div_k_grad_affine_q3> with the coefficient of KField (modes: nmodes x 5 values
+// kappa_x, kappa_y, kappa_z, phi, a)
+void* or_op_new_divkgrad( void* mh, int minL, int maxL, int prec, int kind, double k0, double alpha, const double* modes,
+                          int nmodes )
+{
+   KField kf;
+   kf.kind  = kind;
+   kf.k0    = k0;
+   kf.alpha = alpha;
+   for ( int i = 0; i < nmodes; ++i )
+      kf.modes.push_back( { modes[5 * i], modes[5 * i + 1], modes[5 * i + 2], modes[5 * i + 3], modes[5 * i + 4] } );
+   auto* o = new OrOp;
+   o->prec = prec;
+   o->m    = static_cast< Mesh* >( mh );
+   if ( prec == 4 )
+      o->f = std::make_unique< P1ElementwiseDivKGrad< float > >( o->m, minL, maxL, kf );
+   else
+      o->d = std::make_unique< P1ElementwiseDivKGrad< double > >( o->m, minL, maxL, kf );
+   return o;
+}
+
 // cell stencil (15 weights, C W E N S NW SE TC TW TS TSE BC BN BE BNW) of cell c at level L
 void or_op_cell_stencil( void* oh, int L, int c, double* out )
 {
    auto* o = static_cast< OrOp* >( oh );
-   auto& s = o->prec == 4 ? o->f->cellSt[L][c] : o->d->cellSt[L][c];
    for ( int k = 0; k < NDIR; ++k )
-      out[k] = s[k];
+      out[k] = 0.0;
+   if ( o->prec == 4 )
+   {
+      if ( auto* cop = dynamic_cast< P1ConstantOperator< float >* >( o->f.get() ) )
+         for ( int k = 0; k < NDIR; ++k )
+            out[k] = cop->cellSt[L][c][k];
+   }
+   else if ( auto* cop = dynamic_cast< P1ConstantOperator< double >* >( o->d.get() ) )
+      for ( int k = 0; k < NDIR; ++k )
+         out[k] = cop->cellSt[L][c][k];
 }
 
 void or_apply( void* oh, void* src, void* dst, int L, int flag, int add )

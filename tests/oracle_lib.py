@@ -34,6 +34,7 @@ def lib():
             "or_func_get": (None, [vp, i, P(np.float64)]),
             "or_op_new": (vp, [vp, i, i, i]),
             "or_op_free": (None, [vp]),
+            "or_op_new_divkgrad": (vp, [vp, i, i, i, i, d, d, P(np.float64), i]),
             "or_op_cell_stencil": (None, [vp, i, i, P(np.float64)]),
             "or_apply": (None, [vp, vp, vp, i, i, i]),
             "or_inv_diag": (None, [vp, i, P(np.float64)]),
@@ -124,6 +125,32 @@ class Operator:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.or_op_free(self.h)
+
+
+class DivKGradOperator(Operator):
+    """P1 elementwise -div(k grad) operator; kfield = ("fourier", k0, modes[M x 5]) or ("tanh", alpha)."""
+
+    def __init__(self, mesh, minL, maxL, kfield, prec=8):
+        self.mesh = mesh
+        if kfield[0] == "fourier":
+            modes = np.ascontiguousarray(np.asarray(kfield[2], dtype=np.float64).reshape(-1, 5))
+            self.h = lib().or_op_new_divkgrad(mesh.h, minL, maxL, prec, 0, float(kfield[1]), 0.0,
+                                               modes if len(modes) else np.zeros((1, 5)), len(modes))
+        else:
+            self.h = lib().or_op_new_divkgrad(mesh.h, minL, maxL, prec, 1, 1.0, float(kfield[1]), np.zeros((1, 5)), 0)
+
+
+def random_field(seed, modes=32, kmax=8, std=1.0):
+    """Fourier modes of k = exp(g), g = sum_m a cos(2 pi kappa_m.x + phi_m), integer kappa with
+    1 <= max-norm <= kmax, phi ~ U[0, 2 pi), a = std sqrt(2 / modes) (g has standard deviation std)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < modes:
+        kap = rng.integers(-kmax, kmax + 1, 3)
+        if np.abs(kap).max() == 0:
+            continue
+        out.append([kap[0], kap[1], kap[2], rng.uniform(0, 2 * np.pi), std * np.sqrt(2.0 / modes)])
+    return np.array(out, dtype=np.float64)
 
 
 def cheb_coefficients(order, lower, upper):

@@ -220,3 +220,46 @@ def test_apply_parity_large(hyg, meshes, L):
     op.apply(s, d, L, hyg.INNER)
     r = d.download(L)[~bnd]
     assert np.abs(r).max() <= 1e-13 * np.abs(lin).max() * (1 << L)
+
+
+@pytest.mark.parametrize("cfg", range(6))
+@pytest.mark.parametrize("L", [3, 5, 6])
+def test_every_cell_kernel_configuration(hyg, meshes, cfg, L):
+    """Apply, residual and Jacobi sweep under each cell-kernel configuration (rows x points per lane),
+    FP64 against the oracle and FP32 against FP64 (relative 1e-5)."""
+    gm, om = meshes["cube_24el.msh"]
+    n = gm.ndofs(L)
+    src, rhs = rnd(n, 300 + L), rnd(n, 400 + L)
+    op = hyg.P1ConstantOperator(gm, L, L)
+    for mode in range(6):
+        op.set_config(L, hyg.F64, mode, cfg)
+        op.set_config(L, hyg.F32, mode, cfg)
+    oop = ol.Operator(om, L, L)
+    os_, ob, od = ol.Function(om, L, L), ol.Function(om, L, L), ol.Function(om, L, L)
+    os_.set(L, src)
+    ob.set(L, rhs)
+    for flag in (hyg.INNER, hyg.ALL):
+        s, d = hyg.Function(gm, L, L), hyg.Function(gm, L, L)
+        s.upload(L, src)
+        op.apply(s, d, L, flag)
+        od.set(L, np.zeros(n))
+        oop.apply(os_, od, L, flag, False)
+        assert rel_inf(d.download(L), od.get(L)) <= TOL64, (cfg, L, flag)
+        s32, d32 = hyg.Function(gm, L, L, hyg.F32), hyg.Function(gm, L, L, hyg.F32)
+        s32.upload(L, src)
+        op.apply(s32, d32, L, flag)
+        assert rel_inf(d32.download(L), od.get(L)) <= 1e-5, (cfg, L, flag, "fp32")
+    # Jacobi sweep and residual on inner DoFs
+    x, b, r = hyg.Function(gm, L, L), hyg.Function(gm, L, L), hyg.Function(gm, L, L)
+    x.upload(L, src)
+    b.upload(L, rhs)
+    op.residual(r, b, x, L, hyg.INNER)
+    od.set(L, np.zeros(n))
+    oop.apply(os_, od, L, ol.INNER, False)
+    _, bnd = om.coords(L)
+    assert rel_inf(r.download(L), np.where(bnd, 0.0, rhs - od.get(L))) <= TOL64
+    d = hyg.Function(gm, L, L)
+    op.smooth_jac(d, b, x, 0.6, L, hyg.INNER)
+    od.set(L, np.zeros(n))
+    oop.smooth_jac(od, ob, os_, 0.6, L, ol.INNER)
+    assert rel_inf(d.download(L), od.get(L)) <= TOL64
