@@ -51,6 +51,7 @@ def _sig(lib):
         "hyg_dot": (i, [vp, vp, i, i, C.POINTER(C.c_double), vp]),
         "hyg_convert": (i, [vp, vp, i, vp]),
         "hyg_op_create_p1_const": (i, [vp, i, i, pvp]),
+        "hyg_op_create_p1_divkgrad": (i, [vp, i, i, d, P(np.float64), i, pvp]),
         "hyg_op_destroy": (i, [vp]),
         "hyg_op_cell_stencil": (i, [vp, i, i, P(np.float64)]),
         "hyg_apply": (i, [vp, vp, vp, i, i, i, vp]),
@@ -245,6 +246,18 @@ class P1ConstantOperator:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.hyg_op_destroy(self.h)
+
+
+class P1DivKGradOperator(P1ConstantOperator):
+    """-div(k grad u), k = k0 exp(sum_m a_m cos(2 pi kappa_m . x + phi_m)), modes: M x 5 array."""
+
+    def __init__(self, mesh, minL, maxL, k0=1.0, modes=None):
+        self.mesh = mesh
+        md = np.zeros((1, 5)) if modes is None or len(modes) == 0 else np.ascontiguousarray(modes, dtype=np.float64)
+        nm = 0 if modes is None else len(modes)
+        h = _out()
+        check(load().hyg_op_create_p1_divkgrad(mesh.h, minL, maxL, float(k0), md, nm, C.byref(h)))
+        self.h = h
 
 
 def cheb_coefficients(order, lower, upper):

@@ -11,6 +11,7 @@
 
 namespace hyg {
 void assembleP1Const( hyg_op_s& op );
+void assembleP1DivKGrad( hyg_op_s& op );
 void primEntriesPos( const LowerEntry& e, int N, int i, int j, int& qx, int& qy, int& qz );
 } // namespace hyg
 
@@ -214,7 +215,10 @@ int lincombImpl( hyg_func_s* dst, int n, const double* s, hyg_func_s* const* src
 int stencilImpl( hyg_op_s* op, int L, int dtype, const StencilCall& c, hipStream_t st )
 {
    MeshCtx& mc = op->mesh->ctx;
-   launchStencil( mc, mc.geom( L ), op->levels[L], L, dtype, c, st );
+   if ( op->kind == 1 )
+      launchElementwise( mc, mc.geom( L ), *op, L, dtype, c, st );
+   else
+      launchStencil( mc, mc.geom( L ), op->levels[L], L, dtype, c, st );
    return launched( "stencil" );
 }
 
@@ -693,6 +697,46 @@ int hyg_op_create_p1_const( hyg_mesh mesh, int min_level, int max_level, hyg_op*
    return HYG_OK;
 }
 
+int hyg_op_create_p1_divkgrad( hyg_mesh mesh, int min_level, int max_level, double k0, const double* modes, int n_modes,
+                                hyg_op* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh || !out || min_level < 0 || max_level < min_level || max_level > 10 || n_modes < 0 || n_modes > 64 ||
+        ( n_modes > 0 && !modes ) || !std::isfinite( k0 ) )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   auto* op = new hyg_op_s;
+   op->mesh = mesh;
+   op->minL = min_level;
+   op->maxL = max_level;
+   op->kind = 1;
+   op->k0   = k0;
+   op->modesHost.assign( modes, modes + 5 * size_t( n_modes ) );
+   try
+   {
+      for ( int L = min_level; L <= max_level; ++L )
+         mesh->ctx.geom( L );
+      assembleP1DivKGrad( *op );
+      for ( int L = min_level; L <= max_level; ++L )
+         launchElementwiseInverseDiagonal( mesh->ctx, mesh->ctx.geom( L ), *op, L, nullptr );
+   } catch ( const std::exception& e )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   if ( int rc = launched( "elementwise operator setup" ) )
+   {
+      delete op;
+      return rc;
+   }
+   if ( hipDeviceSynchronize() != hipSuccess )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, "elementwise operator setup failed" );
+   }
+   *out = op;
+   return HYG_OK;
+}
+
 int hyg_op_destroy( hyg_op op )
 {
    delete op;
@@ -704,6 +748,8 @@ int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 )
    HYG_TRY( checkOpLevel( op, level ) );
    if ( cell < 0 || cell >= op->mesh->ctx.mesh->nC() || !out15 )
       return fail( HYG_ERR_ARG, "bad cell index" );
+   if ( op->kind != 0 )
+      return fail( HYG_ERR_ARG, "the operator has no constant stencil" );
    std::memcpy( out15, &op->levels[level].cellStHost[16 * cell], 15 * sizeof( double ) );
    return HYG_OK;
 }
@@ -711,6 +757,8 @@ int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 )
 int hyg_op_tune( hyg_op op, int level, int dtype, int reps, int* chosen, void* stream )
 {
    HYG_TRY( checkOpLevel( op, level ) );
+   if ( op->kind != 0 )
+      return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operator only" );
    if ( dtype != HYG_F64 && dtype != HYG_F32 )
       return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
    if ( reps < 1 )
@@ -852,7 +900,18 @@ int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream )
    HYG_TRY( checkOpLevel( op, level ) );
    HYG_TRY( checkLevel( dst, level ) );
    MeshCtx& mc = op->mesh->ctx;
-   launchInvDiag( mc, mc.geom( level ), op->levels[level], level, dst->dtype, dst->data[level], S( stream ) );
+   if ( op->kind == 1 )
+   {
+      // the elementwise operator keeps its inverse diagonal as a function array
+      const auto& ol = op->levels[level];
+      const i64   n  = levelValues( dst, level );
+      if ( dst->dtype == HYG_F64 )
+         launchCopy( n, 8, dst->data[level], ol.dinv, S( stream ) );
+      else
+         launchCopy( n, 4, dst->data[level], ol.dinvF, S( stream ) );
+   }
+   else
+      launchInvDiag( mc, mc.geom( level ), op->levels[level], level, dst->dtype, dst->data[level], S( stream ) );
    return launched( "inverse diagonal" );
 }
 

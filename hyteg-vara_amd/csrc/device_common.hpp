@@ -110,18 +110,6 @@ __device__ __forceinline__ void bufLoad2( __amdgpu_buffer_rsrc_t r, unsigned vof
    a             = __uint_as_float( v.x );
    b             = __uint_as_float( v.y );
 }
-// streaming (non-temporal) stores of two consecutive values
-__device__ __forceinline__ void bufStore2( double a, double b, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff )
-{
-   const u32x4 v = { unsigned( __double2loint( a ) ), unsigned( __double2hiint( a ) ), unsigned( __double2loint( b ) ),
-                     unsigned( __double2hiint( b ) ) };
-   __builtin_amdgcn_raw_buffer_store_b128( v, r, voff, soff, 2 );
-}
-__device__ __forceinline__ void bufStore2( float a, float b, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff )
-{
-   const u32x2 v = { __float_as_uint( a ), __float_as_uint( b ) };
-   __builtin_amdgcn_raw_buffer_store_b64( v, r, voff, soff, 2 );
-}
 // streaming (non-temporal) stores
 __device__ __forceinline__ void bufStore( double v, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff )
 {

@@ -167,7 +167,16 @@ struct CellInfo
    uint64_t bcword;    // DoFType (4 bits) of local slots 0..14
    uint32_t ownmask;   // bit k: this cell is the designated owner of local slot k (bit 0 always)
    int      prim[15];  // LowerPrim index of local slot k >= 1 (slot 0: -1)
+   int      entry[15]; // LowerEntry of the (primitive, this cell) pair of local slot k >= 1 (slot 0: -1)
    double   invNC[16]; // 1 / #cells of the local slot's primitive (slot 0: 1)
+};
+
+// Geometry of one macro-cell for the variable-coefficient elementwise operator at one level.
+struct ElemCell
+{
+   double v0[3];       // position of local vertex 0
+   double a[3][3];     // a[r][k]: component r of (v_{k+1} - v0) / N, the lattice step along local axis k
+   double mt[6][16];   // element matrix of each micro-cell type for k = 1 (vertex order of the type)
 };
 
 // ------------------------------------------------------------------ host objects
@@ -264,6 +273,13 @@ struct hyg_op_s
 {
    hyg_mesh_s* mesh;
    int         minL, maxL;
+   // 0: constant-coefficient stencils (P1ConstantLaplaceOperator)
+   // 1: elementwise -div(k grad) with k = k0 exp( sum_m a_m cos(2 pi kappa_m . x + phi_m) )
+   int                   kind = 0;
+   double                k0   = 1.0;
+   std::vector< double > modesHost; // nModes x 5: kappa_x, kappa_y, kappa_z, phi, a
+   double*               modes  = nullptr;
+   int                   nModes = 0;
    struct Level
    {
       std::vector< double > cellStHost;  // nC x 16: 15 weights, inverse centre weight
@@ -278,6 +294,16 @@ struct hyg_op_s
       // one-sided face partial sums (2 sides x (N+1)^2 values per face), FP64 and FP32
       double*               facePart  = nullptr;
       float*                facePartF = nullptr;
+      // elementwise operator: cell geometry, per-element coefficient integrals (6 types x S(N) per
+      // cell), one-sided sums of edge and vertex points (N+1 per primitive entry), inverse diagonal
+      // (a function array with replicas)
+      hyg::ElemCell*        elem    = nullptr;
+      double*               kb      = nullptr;
+      float*                kbF     = nullptr;
+      double*               lowPart = nullptr;
+      float*                lowPartF = nullptr;
+      double*               dinv    = nullptr;
+      float*                dinvF   = nullptr;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

@@ -31,6 +31,12 @@ struct StencilCall
 void launchStencil( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, int dtype, const StencilCall& c,
                     hipStream_t s );
 
+// Variable-coefficient elementwise operator (op.kind == 1): one stencil operation, and the inverse
+// diagonal of level L written into op.levels[L].dinv / dinvF (setup).
+void launchElementwise( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c,
+                        hipStream_t s );
+void launchElementwiseInverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );
+
 // cell-interior kernel configuration (index into kCellRows / kCellPoints) of a level, value type and mode
 int  cellConfig( LevelGeom& g, int dtype, int mode );
 void setCellConfig( LevelGeom& g, int dtype, int mode, int cfg );

@@ -166,6 +166,8 @@ void MeshCtx::buildTopology()
       ci.prim[0]   = -1;
       ci.invNC[0]  = 1.0;
       ci.invNC[15] = 0.0;
+      for ( int k = 0; k < 15; ++k )
+         ci.entry[k] = -1;
       auto setSlot = [&]( int slot, int prim, int bnd, const std::vector< int >& cells ) {
          ci.bcword |= uint64_t( bnd ? 2 : 1 ) << ( 4 * slot );
          if ( cells.front() == c )
@@ -187,6 +189,29 @@ void MeshCtx::buildTopology()
       {
          const int v = m.Cv[c][k];
          setSlot( 11 + k, v, m.vB[v], m.vC[v] );
+      }
+   }
+   // entry of every (primitive, cell) pair in the cell's slot table
+   for ( size_t p = 0; p < primsHost.size(); ++p )
+   {
+      const LowerPrim& pr = primsHost[p];
+      for ( int k = 0; k < pr.nCells; ++k )
+      {
+         const LowerEntry& e    = entriesHost[pr.entry0 + k];
+         CellInfo&         ci   = cellInfoHost[e.cell];
+         int               slot = -1;
+         if ( pr.kind == 2 )
+            slot = 1 + entrySlot( e );
+         else if ( pr.kind == 0 )
+            slot = 11 + e.l0;
+         else
+         {
+            // local edge of the vertex pair (l0, l1): e0..e5 = (0,1) (0,2) (1,2) (0,3) (1,3) (2,3)
+            const int a = std::min( e.l0, e.l1 ), b = std::max( e.l0, e.l1 );
+            const int le = a == 0 ? ( b == 1 ? 0 : ( b == 2 ? 1 : 3 ) ) : ( a == 1 ? ( b == 2 ? 2 : 4 ) : 5 );
+            slot         = 5 + le;
+         }
+         ci.entry[slot] = pr.entry0 + k;
       }
    }
    // face slots of every cell (layout order), with the entry of the (face, cell) pair
@@ -406,6 +431,60 @@ void assembleP1Const( hyg_op_s& op )
    }
 }
 
+// Variable-coefficient elementwise operator: per level the geometry of every cell (affine map and the
+// element matrices of the six micro-cell types for k = 1), the buffers of the per-element coefficient
+// integrals and of the one-sided face/edge/vertex sums, and the inverse diagonal, computed on the device.
+void assembleP1DivKGrad( hyg_op_s& op )
+{
+   const MeshCtx& mc = op.mesh->ctx;
+   const Mesh&    m  = *mc.mesh;
+   op.levels.resize( op.maxL + 1 );
+   auto alloc = []( size_t bytes ) {
+      void* d = nullptr;
+      if ( bytes > 0 && hipMalloc( &d, bytes ) != hipSuccess )
+         throw std::runtime_error( "hipMalloc failed for the elementwise operator" );
+      return d;
+   };
+   if ( !op.modesHost.empty() )
+      op.modes = upload( op.modesHost );
+   op.nModes = int( op.modesHost.size() / 5 );
+   for ( int L = op.minL; L <= op.maxL; ++L )
+   {
+      hyg_op_s::Level& ol = op.levels[L];
+      const int        N  = 1 << L;
+      std::vector< ElemCell > ec( m.nC() );
+      for ( int c = 0; c < m.nC(); ++c )
+      {
+         const auto& cv = m.Cv[c];
+         for ( int r = 0; r < 3; ++r )
+         {
+            ec[c].v0[r] = m.X[cv[0]][r];
+            for ( int k = 0; k < 3; ++k )
+               ec[c].a[r][k] = ( m.X[cv[k + 1]][r] - m.X[cv[0]][r] ) / double( N );
+         }
+         double K[6][4][4];
+         elementMatrices( m, cv, L, K );
+         for ( int t = 0; t < 6; ++t )
+            for ( int i = 0; i < 4; ++i )
+               for ( int j = 0; j < 4; ++j )
+                  ec[c].mt[t][4 * i + j] = K[t][i][j];
+      }
+      ol.elem                = upload( ec );
+      const size_t nKb       = size_t( m.nC() ) * 6 * size_t( S3( N ) );
+      const size_t nLow      = mc.entriesHost.size() * size_t( N + 1 );
+      const size_t nFace     = size_t( m.nF() ) * 2 * size_t( N + 1 ) * size_t( N + 1 );
+      const size_t nFunc     = size_t( m.nC() ) * size_t( cellStride( L ) ) + size_t( kTailSlack );
+      ol.kb        = static_cast< double* >( alloc( nKb * sizeof( double ) ) );
+      ol.kbF       = static_cast< float* >( alloc( nKb * sizeof( float ) ) );
+      ol.lowPart   = static_cast< double* >( alloc( nLow * sizeof( double ) ) );
+      ol.lowPartF  = static_cast< float* >( alloc( nLow * sizeof( float ) ) );
+      ol.facePart  = static_cast< double* >( alloc( nFace * sizeof( double ) ) );
+      ol.facePartF = static_cast< float* >( alloc( nFace * sizeof( float ) ) );
+      ol.dinv      = static_cast< double* >( alloc( nFunc * sizeof( double ) ) );
+      ol.dinvF     = static_cast< float* >( alloc( nFunc * sizeof( float ) ) );
+   }
+}
+
 } // namespace hyg
 
 hyg_op_s::~hyg_op_s()
@@ -420,5 +499,13 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.primInvF );
       (void) hipFree( l.facePart );
       (void) hipFree( l.facePartF );
+      (void) hipFree( l.elem );
+      (void) hipFree( l.kb );
+      (void) hipFree( l.kbF );
+      (void) hipFree( l.lowPart );
+      (void) hipFree( l.lowPartF );
+      (void) hipFree( l.dinv );
+      (void) hipFree( l.dinvF );
    }
+   (void) hipFree( modes );
 }

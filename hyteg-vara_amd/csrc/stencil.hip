@@ -22,7 +22,7 @@
 //
 // Macro-edges and -vertices (kLower): one lane per point gathers the one-sided stencils of all
 // incident cells.
-#include "device_common.hpp"
+#include "face_combine.hpp"
 
 namespace hyg {
 
@@ -243,35 +243,16 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
          // plane z-1: BC = row r+1, BN = row r+2
          T ax[P];
          stencilRow< T, P >( w, wu, we, Bp.r[r + 1], Bp.r[r + 2], Bp.r[r], C.r[r + 1], C.r[r], A.r[r + 1], A.r[r + 2], ax );
-         T o0[P], o1[P];
-         bool ok[P];
 #pragma unroll
          for ( int k = 0; k < P; ++k )
          {
-            epilogue< T, MODE >( ax[k], Bp.r[r + 1][k], Bp.e[r][k], Bp.e[r][k], w[15], a, o0[k], o1[k] );
-            const int d = d0 + k;
-            ok[k]       = interior && d >= 1 && d <= WO && d <= lastD - r - z;
-         }
-         if constexpr ( P == 1 )
-         {
-            const unsigned so = ok[0] ? vo : kDropOffset;
-            bufStore( o0[0], r0, so, o * B );
+            T o0, o1;
+            epilogue< T, MODE >( ax[k], Bp.r[r + 1][k], Bp.e[r][k], Bp.e[r][k], w[15], a, o0, o1 );
+            const int      d  = d0 + k;
+            const unsigned so = ( interior && d >= 1 && d <= WO && d <= lastD - r - z ) ? unsigned( d * B ) : kDropOffset;
+            bufStore( o0, r0, so, o * B );
             if constexpr ( kTwo )
-               bufStore( o1[0], r1, so, o * B );
-         }
-         else
-         {
-            // both points of the lane in one store; a lane with one output point (window start, row
-            // end) stores it with a single-value store
-            const unsigned s2 = ( ok[0] && ok[1] ) ? vo : kDropOffset;
-            const unsigned s1 = ( ok[0] != ok[1] ) ? ( ok[0] ? vo : vo + unsigned( B ) ) : kDropOffset;
-            bufStore2( o0[0], o0[1], r0, s2, o * B );
-            bufStore( ok[0] ? o0[0] : o0[1], r0, s1, o * B );
-            if constexpr ( kTwo )
-            {
-               bufStore2( o1[0], o1[1], r1, s2, o * B );
-               bufStore( ok[0] ? o1[0] : o1[1], r1, s1, o * B );
-            }
+               bufStore( o1, r1, so, o * B );
          }
          if ( onX0 ) // the first point of lane 0 is the x = 0 face point of row y0+r, plane z
             f2v[r] = moveLane( ax[0], 0, z - z0, f2v[r] );
@@ -401,52 +382,6 @@ __global__ __launch_bounds__( kBlock ) void kFaceFlat( const int4* __restrict__ 
    bufStore( Ax, rp, ok ? unsigned( facePartIndex( slot, N, xx, yc, zc ) * B ) : kDropOffset, 0 );
 }
 
-// ------------------------------------------------------------------ macro-face points: combine
-template < typename T, int MODE >
-__global__ __launch_bounds__( kBlock ) void kFaceCombine( const LowerChunk* __restrict__ chunks, int nChunks,
-                                                          const LowerPrim* __restrict__ prims,
-                                                          const LowerEntry* __restrict__ entries, int firstFacePrim,
-                                                          const T* __restrict__ part, const T* __restrict__ primInv, int L,
-                                                          i64 stride, int flag, const T* __restrict__ x,
-                                                          const T* __restrict__ in1, T* __restrict__ out0, T* __restrict__ out1, T a )
-{
-   const int nb = ( nChunks + 3 ) / 4;
-   const int ci = remapXcd( int( blockIdx.x ), nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
-   if ( ci >= nChunks )
-      return;
-   const LowerChunk ch = chunks[ci];
-   const LowerPrim  p  = prims[ch.prim];
-   const int        lane = int( threadIdx.x & 63 );
-   if ( !( p.bc & flag ) || lane >= ch.n )
-      return;
-   const int N = 1 << L, NP = N + 1, w = N + 1;
-   const int i = ch.i0 + lane, j = ch.j, f = ch.prim - firstFacePrim;
-   T         Ax = T( 0 );
-   i64       o[2];
-   for ( int k = 0; k < p.nCells; ++k )
-   {
-      const LowerEntry e = entries[p.entry0 + k];
-      int              qx, qy, qz;
-      entryPos( e, N, i, j, qx, qy, qz );
-      o[k] = i64( e.cell ) * stride + cidx( w, qx, qy, qz );
-      Ax += part[i64( 2 * f + k ) * NP * NP + facePartIndex( entrySlot( e ), N, qx, qy, qz )];
-   }
-   T xc = T( 0 ), i1 = T( 0 ), ov = T( 0 ), r0, r1;
-   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
-      xc = x[o[0]];
-   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-      i1 = in1[o[0]];
-   if constexpr ( MODE == MODE_APPLY_ADD )
-      ov = out0[o[0]];
-   epilogue< T, MODE >( Ax, xc, i1, ov, primInv[ch.prim], a, r0, r1 );
-   for ( int k = 0; k < p.nCells; ++k )
-   {
-      out0[o[k]] = r0;
-      if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-         out1[o[k]] = r1;
-   }
-}
-
 // ------------------------------------------------------------------ macro-edge / -vertex points
 // Each lane gathers the one-sided stencil of every incident cell. Directions leaving a cell carry a
 // zero weight in that cell's one-sided stencil; they read the point itself instead, so all loads of a
@@ -570,8 +505,8 @@ static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol,
          hipLaunchKernelGGL( ( kFaceFlat< T > ), dim3( xcdGrid( ceilDiv( g.nFlatItems, 4 ) ) ), dim3( kBlock ), 0, s, g.flatItems,
                              g.nFlatItems, L, g.stride, m.cellFaces, ew, part, c.flag, x );
       hipLaunchKernelGGL( ( kFaceCombine< T, MODE > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
-                          g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, m.mesh->nV() + m.mesh->nE(), part, pi, L,
-                          g.stride, c.flag, x, in1, out0, out1, T( c.a ) );
+                          g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, m.mesh->nV() + m.mesh->nE(), part, pi,
+                          (const T*) nullptr, L, g.stride, c.flag, x, in1, out0, out1, T( c.a ) );
    }
    // 4. macro-edges and -vertices
    if ( nLow > 0 )

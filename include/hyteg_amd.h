@@ -99,7 +99,18 @@ int hyg_convert( hyg_func dst, hyg_func src, int level, void* stream );
  * computeInverseDiagonalOperatorValues() (P1Operator.hpp:465-469, 645-915). */
 int hyg_op_create_p1_const( hyg_mesh mesh, int min_level, int max_level, hyg_op* out );
 int hyg_op_destroy( hyg_op op );
-/* Cell stencil (15 weights in the order C W E N S NW SE TC TW TS TSE BC BN BE BNW) of cell `cell`. */
+/* Replaces P1ElementwiseOperator<forms::p1_div_k_grad_affine_q3>(storage, minLevel, maxLevel, form)
+ * (elementwiseoperators/P1ElementwiseOperator.cpp:64-415; forms/form_hyteg_generated/p1/
+ * p1_div_k_grad_affine_q3.cpp:179-381) for -div(k grad u) with the coefficient
+ *   k(x) = k0 exp( sum_m a_m cos(2 pi kappa_m . x + phi_m) ),
+ * modes[5*m .. 5*m+4] = kappa_x, kappa_y, kappa_z, phi, a (0 <= n_modes <= 64; n_modes = 0: k = k0).
+ * k is evaluated at the 5 quadrature points of every element on the fly in each apply. The operator
+ * supports hyg_apply, hyg_smooth_jac, hyg_residual, hyg_inverse_diagonal, hyg_power_iteration and GMG
+ * in FP64 and FP32. */
+int hyg_op_create_p1_divkgrad( hyg_mesh mesh, int min_level, int max_level, double k0, const double* modes, int n_modes,
+                               hyg_op* out );
+/* Cell stencil (15 weights in the order C W E N S NW SE TC TW TS TSE BC BN BE BNW) of cell `cell`
+ * (constant-coefficient operators only). */
 int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 );
 /* Operator::apply(src, dst, level, flag, updateType) (operators/Operator.hpp:73-80; P1Operator.hpp:196-324).
  * src and dst must differ and have the same value type. */
