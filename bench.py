@@ -118,6 +118,70 @@ def cpu_baseline(mesh_name, L, target_s, inner):
                       f"{reps} applies in {el:.1f} s on {threads} OpenMP threads"}
 
 
+def bench_divkgrad(hyg, hip, st, steps, level, seed):
+    """Variable-coefficient workload: -div(k grad u) on the 24-tet cube, k = exp(g), g a sum of 32
+    seeded Fourier modes of wavenumber <= 8 (standard deviation 1), k evaluated at the 5 quadrature
+    points of every element in every apply. Apply throughput in FP64 and FP32; V(3,3) Chebyshev
+    cycles to 1e-10 in FP64 and as FP32 cycles inside FP64 defect correction."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol  # only for the seeded mode table (host-side numpy)
+    modes = ol.random_field(seed, modes=32, kmax=8)
+    mesh = hyg.Mesh("cube_24el.msh")
+    L = level
+    n = mesh.ndofs(L)
+    bnd = mesh.boundary(L)
+    inner = int((~bnd).sum())
+    lib = hyg.load()
+    op = hyg.P1DivKGradOperator(mesh, 0, L, 1.0, modes)
+    out = {"workload": f"P1 -div(k grad u), cube_24el.msh level {L}, k = exp(32 Fourier modes, |kappa| <= 8, seed {seed})",
+           "dofs_inner": inner}
+    rng = np.random.default_rng(seed)
+    for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):
+        x, y = hyg.Function(mesh, L, L, dt), hyg.Function(mesh, L, L, dt)
+        x.upload(L, rng.uniform(-1, 1, n))
+
+        def ap():
+            hyg.check(lib.hyg_apply(op.h, x.h, y.h, L, hyg.INNER, hyg.REPLACE, st))
+
+        for _ in range(2):
+            ap()
+        ms = timed(hip, st, ap, steps) / steps
+        out[f"apply_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3)}
+    # coefficient evaluation: per lattice anchor 32 modes x (3 FMA phase + 30 x 2 FMA) + 30 exp + 6 x 5 FMA
+    anchors = int(mesh.info()[3]) * ((1 << L) * ((1 << L) + 1) * ((1 << L) + 2) // 6)
+    out["kbar_flops_per_apply"] = anchors * (32 * (6 + 120) + 60)
+    b = hyg.Function(mesh, 0, L)
+    bv = rng.uniform(-1.0, 1.0, n)
+    bv[bnd] = 0.0
+    b.upload(L, bv)
+    Lr = min(L, 5)
+    pv, pt = hyg.Function(mesh, Lr, Lr), hyg.Function(mesh, Lr, Lr)
+    pv.upload(Lr, np.random.default_rng(5).uniform(-1, 1, mesh.ndofs(Lr)))
+    rho = op.power_iteration(pv, pt, 40, Lr)
+    kw = dict(pre=3, post=3, smoother="cheb", cheb_order=2, cheb_bounds=(0.3 * rho, 1.2 * rho))
+    g64 = hyg.GMG(op, 0, L, hyg.F64, **kw)
+    g32 = hyg.GMG(op, 0, L, hyg.F32, **kw)
+    x64, xmx = hyg.Function(mesh, 0, L), hyg.Function(mesh, 0, L)
+    g64.solve_tol(x64, b, L, 1, 1e-10, st)
+    hyg.ir_solve(op, g32, xmx, b, L, inner=1, max_outer=1, rel_tol=1e-10, stream=st)
+    for f in (x64, xmx):
+        f.interpolate(0.0, L)
+    hip.sync(st)
+    t0 = time.perf_counter()
+    r64 = g64.solve_tol(x64, b, L, 100, 1e-10, st)
+    t64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rmx = hyg.ir_solve(op, g32, xmx, b, L, inner=1, max_outer=100, rel_tol=1e-10, stream=st)
+    tmx = time.perf_counter() - t0
+    u64, umx = x64.download(L), xmx.download(L)
+    out["solve"] = {"cycle": f"V(3,3) Chebyshev order 2, rho(D^-1 A) = {rho:.5f}",
+                    "fp64": {"seconds": round(t64, 4), "cycles": len(r64) - 1, "rel_residual": float(r64[-1] / r64[0])},
+                    "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "rel_residual": float(rmx[-1] / rmx[0]),
+                              "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
+                    "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +194,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cheb-order", type=int, default=2)
     ap.add_argument("--no-tune", action="store_true", help="keep the default cell-kernel configuration")
+    ap.add_argument("--no-divkgrad", action="store_true", help="skip the variable-coefficient workload")
+    ap.add_argument("--divkgrad-level", type=int, default=7)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -249,6 +315,10 @@ def main():
                       "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
             "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
         }
+
+    # ---------------- variable-coefficient workload
+    if not args.no_divkgrad:
+        out["divkgrad"] = bench_divkgrad(hyg, hip, st, max(3, args.steps // 10), args.divkgrad_level, 2024)
 
     # ---------------- CPU baseline (oracle), rank 0, N = 1
     if not args.no_cpu:

@@ -68,6 +68,7 @@ def _sig(lib):
         "hyg_gmg_create": (i, [vp, i, P(np.int32), P(np.float64), pvp]),
         "hyg_gmg_destroy": (i, [vp]),
         "hyg_gmg_vcycle": (i, [vp, vp, vp, i, vp]),
+        "hyg_gmg_vcycle_graph": (i, [vp, vp, vp, i, i, vp]),
         "hyg_gmg_solve": (i, [vp, vp, vp, i, i, P(np.float64), vp]),
         "hyg_gmg_solve_tol": (i, [vp, vp, vp, i, i, d, P(np.float64), C.POINTER(C.c_int), vp]),
         "hyg_ir_solve": (i, [vp, vp, vp, vp, i, i, i, d, P(np.float64), C.POINTER(C.c_int), vp]),
@@ -279,6 +280,10 @@ class GMG:
 
     def vcycle(self, x, b, L):
         check(load().hyg_gmg_vcycle(self.h, x.h, b.h, L, None))
+
+    def vcycle_graph(self, x, b, L, cg_iterations, stream):
+        """One V-cycle replayed from a captured hipGraph (captured on the first call)."""
+        check(load().hyg_gmg_vcycle_graph(self.h, x.h, b.h, L, cg_iterations, stream))
 
     def solve(self, x, b, L, cycles):
         res = np.zeros(cycles + 1)

@@ -235,10 +235,24 @@ struct hyg_gmg_s
    hyg_func_s*           tmp = nullptr, *jt = nullptr, *t1 = nullptr, *t2 = nullptr, *cy = nullptr;
    hyg_func_s*           cgp = nullptr, *cgz = nullptr, *cgap = nullptr, *cgr = nullptr;
    hyg_func_s*           x32 = nullptr, *b32 = nullptr, *r64 = nullptr, *t64 = nullptr;
+   // captured V-cycle: device-side coarse CG scalars and partial sums, the graph and its arguments
+   int                   cgGraphIters = 0;
+   double*               cgDev        = nullptr; // rs (two slots), pAp
+   double*               cgPartials   = nullptr;
+   hipGraph_t            graph        = nullptr;
+   hipGraphExec_t        graphExec    = nullptr;
+   hyg_func_s*           graphX = nullptr, *graphB = nullptr;
+   int                   graphLevel   = -1;
    ~hyg_gmg_s()
    {
       for ( hyg_func_s* f : { tmp, jt, t1, t2, cy, cgp, cgz, cgap, cgr, x32, b32, r64, t64 } )
          destroyFunc( f );
+      if ( graphExec )
+         (void) hipGraphExecDestroy( graphExec );
+      if ( graph )
+         (void) hipGraphDestroy( graph );
+      (void) hipFree( cgDev );
+      (void) hipFree( cgPartials );
    }
 };
 
@@ -339,11 +353,48 @@ int cgImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, hipStream_t st )
    return HYG_OK;
 }
 
-// GeometricMultigridSolver::solveRecursively, V-cycle
-int vcycleImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, int invoked, hipStream_t st )
+// CGSolver::solve for a captured V-cycle: a fixed number of iterations (cgGraphIters) whose step
+// lengths come from device memory; once the residual norm is below the tolerance the remaining
+// iterations take no step, which reproduces the early exit of cgImpl.
+int cgDeviceImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, hipStream_t st )
+{
+   MeshCtx&     mc   = g->op->mesh->ctx;
+   LevelGeom&   gl   = mc.geom( L );
+   const int    flag = HYG_INNER | HYG_NEUMANN | HYG_FREESLIP;
+   const double tol2 = g->cgTol * g->cgTol;
+   StencilCall  c;
+   c.flag = flag;
+   c.mode = MODE_RESIDUAL;
+   c.x    = x->data[L];
+   c.in1  = b->data[L];
+   c.out0 = g->cgr->data[L];
+   HYG_TRY( stencilImpl( g->op, L, g->dtype, c, st ) );
+   const double one = 1.0;
+   hyg_func_s*  r   = g->cgr;
+   HYG_TRY( lincombImpl( g->cgp, 1, &one, &r, false, 0.0, L, flag, st ) );
+   launchDot( mc, gl, L, g->dtype, g->cgr->data[L], g->cgr->data[L], flag, g->cgPartials, g->cgDev + 0, st );
+   for ( int it = 0; it < g->cgGraphIters; ++it )
+   {
+      const int o = it & 1, nw = 1 - o;
+      c.mode      = MODE_APPLY;
+      c.x         = g->cgp->data[L];
+      c.out0      = g->cgap->data[L];
+      HYG_TRY( stencilImpl( g->op, L, g->dtype, c, st ) );
+      launchDot( mc, gl, L, g->dtype, g->cgp->data[L], g->cgap->data[L], flag, g->cgPartials, g->cgDev + 2, st );
+      launchCgUpdate( mc, gl, L, g->dtype, x->data[L], g->cgr->data[L], g->cgp->data[L], g->cgap->data[L], g->cgDev, o, 2, tol2,
+                      flag, st );
+      launchDot( mc, gl, L, g->dtype, g->cgr->data[L], g->cgr->data[L], flag, g->cgPartials, g->cgDev + nw, st );
+      launchCgDirection( mc, gl, L, g->dtype, g->cgp->data[L], g->cgr->data[L], g->cgDev, o, nw, tol2, flag, st );
+      HYG_TRY( launched( "coarse CG" ) );
+   }
+   return HYG_OK;
+}
+
+// GeometricMultigridSolver::solveRecursively, V-cycle (deviceCg: coarse CG without host round trips)
+int vcycleImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, int invoked, hipStream_t st, bool deviceCg = false )
 {
    if ( L == g->minL )
-      return cgImpl( g, x, b, L, st );
+      return deviceCg ? cgDeviceImpl( g, x, b, L, st ) : cgImpl( g, x, b, L, st );
    MeshCtx&  mc   = g->op->mesh->ctx;
    const int preS = g->pre + g->inc * ( invoked - L );
    for ( int i = 0; i < preS; ++i )
@@ -361,7 +412,7 @@ int vcycleImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, int invoked, 
    hyg_func_s*  t   = g->tmp;
    HYG_TRY( lincombImpl( b, 1, &one, &t, false, 0.0, L - 1, kGmgFlag, st ) );
    HYG_TRY( lincombImpl( x, 0, nullptr, nullptr, false, 0.0, L - 1, HYG_ALL, st ) );
-   HYG_TRY( vcycleImpl( g, x, b, L - 1, invoked, st ) );
+   HYG_TRY( vcycleImpl( g, x, b, L - 1, invoked, st, deviceCg ) );
    launchProlongate( mc, mc.geom( L ), L, g->dtype, x->data[L], x->data[L - 1], kGmgFlag, true, st );
    HYG_TRY( launched( "prolongate" ) );
    const int postS = g->post + g->inc * ( invoked - L );
@@ -1087,6 +1138,62 @@ int hyg_gmg_vcycle( hyg_gmg g, hyg_func x, hyg_func b, int level, void* stream )
 {
    HYG_TRY( checkGmgArgs( g, x, b, level ) );
    HYG_TRY( vcycleImpl( g, x, b, level, level, S( stream ) ) );
+   return HYG_OK;
+}
+
+int hyg_gmg_vcycle_graph( hyg_gmg g, hyg_func x, hyg_func b, int level, int cg_iterations, void* stream )
+{
+   if ( !g )
+      return fail( HYG_ERR_ARG, "null solver handle" );
+   HYG_TRY( checkLevel( x, level ) );
+   HYG_TRY( checkLevel( b, level ) );
+   if ( !stream )
+      return fail( HYG_ERR_ARG, "capturing a V-cycle needs a stream other than the default stream" );
+   if ( level < g->minL || level > g->maxL || x->dtype != g->dtype || b->dtype != g->dtype || x->minL > g->minL ||
+        b->minL > g->minL )
+      return fail( HYG_ERR_ARG, "functions must hold all solver levels in the solver's value type" );
+   if ( cg_iterations < 1 )
+      return fail( HYG_ERR_ARG, "cg_iterations must be positive" );
+   hipStream_t st = S( stream );
+   if ( !g->graphExec || g->graphX != x || g->graphB != b || g->graphLevel != level || g->cgGraphIters != cg_iterations )
+   {
+      if ( g->graphExec )
+         (void) hipGraphExecDestroy( g->graphExec );
+      if ( g->graph )
+         (void) hipGraphDestroy( g->graph );
+      g->graphExec = nullptr;
+      g->graph     = nullptr;
+      MeshCtx& mc  = g->op->mesh->ctx;
+      if ( !g->cgDev )
+         HYG_HIP( hipMalloc( &g->cgDev, 4 * sizeof( double ) ) );
+      if ( !g->cgPartials )
+         HYG_HIP( hipMalloc( &g->cgPartials, size_t( dotPartialsNeeded( mc, mc.geom( g->minL ) ) ) * sizeof( double ) ) );
+      for ( int L = g->minL; L <= level; ++L )
+         mc.geom( L );
+      g->cgGraphIters = cg_iterations;
+      HYG_HIP( hipStreamBeginCapture( st, hipStreamCaptureModeThreadLocal ) );
+      const int  rc = vcycleImpl( g, x, b, level, level, st, true );
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture( st, &graph );
+      if ( rc != HYG_OK || e != hipSuccess )
+      {
+         if ( graph )
+            (void) hipGraphDestroy( graph );
+         return rc != HYG_OK ? rc : fail( HYG_ERR_HIP, std::string( "capturing the V-cycle: " ) + hipGetErrorString( e ) );
+      }
+      hipGraphExec_t exec = nullptr;
+      if ( hipGraphInstantiate( &exec, graph, nullptr, nullptr, 0 ) != hipSuccess )
+      {
+         (void) hipGraphDestroy( graph );
+         return fail( HYG_ERR_HIP, "instantiating the V-cycle graph failed" );
+      }
+      g->graph      = graph;
+      g->graphExec  = exec;
+      g->graphX     = x;
+      g->graphB     = b;
+      g->graphLevel = level;
+   }
+   HYG_HIP( hipGraphLaunch( g->graphExec, st ) );
    return HYG_OK;
 }
 

@@ -88,6 +88,61 @@ __global__ __launch_bounds__( kBlock ) void kLincomb( const PlaneTile* __restric
          dst[sp[u].o] = r[u];
 }
 
+// Conjugate-gradient updates with scalars held in device memory (so that a captured graph needs no
+// host round trip): sc[rsOld], sc[pAp] from the dot kernels. A step is taken only while the residual
+// norm is not below tol (tol2 = tol^2), matching CGSolver's early exit.
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kCgUpdate( const PlaneTile* __restrict__ tiles, int nTiles, int nC, int L, i64 stride,
+                                                       const CellInfo* __restrict__ cinfo, int flag, T* __restrict__ x,
+                                                       T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ ap,
+                                                       const double* __restrict__ sc, int rsOld, int pAp, double tol2 )
+{
+   const int lb = remapXcd( blockIdx.x, nTiles * nC );
+   if ( lb >= nTiles * nC )
+      return;
+   const double rs = sc[rsOld];
+   const T      al = rs >= tol2 && sc[pAp] != 0.0 ? T( rs / sc[pAp] ) : T( 0 );
+   PlaneTile    tile;
+   const int    c  = tileCell( tiles, nTiles, lb, tile );
+   const uint64_t bw = cinfo[c].bcword;
+#pragma unroll
+   for ( int u = 0; u < kPtsPerThread; ++u )
+   {
+      SlotPos sp;
+      slotAt( c, tile, 1 << L, stride, tile.s0 + int( threadIdx.x ) + kBlock * u, sp );
+      if ( sp.ok && ( int( ( bw >> ( 4 * sp.slot ) ) & 15u ) & flag ) )
+      {
+         x[sp.o] += al * p[sp.o];
+         r[sp.o] -= al * ap[sp.o];
+      }
+   }
+}
+
+// p = r + beta p, beta = sc[rsNew] / sc[rsOld] (0 once converged)
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kCgDirection( const PlaneTile* __restrict__ tiles, int nTiles, int nC, int L, i64 stride,
+                                                          const CellInfo* __restrict__ cinfo, int flag, T* __restrict__ p,
+                                                          const T* __restrict__ r, const double* __restrict__ sc, int rsOld,
+                                                          int rsNew, double tol2 )
+{
+   const int lb = remapXcd( blockIdx.x, nTiles * nC );
+   if ( lb >= nTiles * nC )
+      return;
+   const double o  = sc[rsOld], nw = sc[rsNew];
+   const T      be = nw >= tol2 && o > 0.0 ? T( nw / o ) : T( 0 );
+   PlaneTile    tile;
+   const int    c  = tileCell( tiles, nTiles, lb, tile );
+   const uint64_t bw = cinfo[c].bcword;
+#pragma unroll
+   for ( int u = 0; u < kPtsPerThread; ++u )
+   {
+      SlotPos sp;
+      slotAt( c, tile, 1 << L, stride, tile.s0 + int( threadIdx.x ) + kBlock * u, sp );
+      if ( sp.ok && ( int( ( bw >> ( 4 * sp.slot ) ) & 15u ) & flag ) )
+         p[sp.o] = r[sp.o] + be * p[sp.o];
+   }
+}
+
 template < typename T >
 __global__ __launch_bounds__( kBlock ) void kMult( const PlaneTile* __restrict__ tiles, int nTiles, int nC, int L, i64 stride,
                                                    const CellInfo* __restrict__ cinfo, int flag, T* dst, const T* a, const T* b )
@@ -356,6 +411,39 @@ void launchMult( const MeshCtx& m, const LevelGeom& g, int L, int dtype, void* d
    else
       hipLaunchKernelGGL( ( kMult< float > ), dim3( xcdGrid( nb ) ), dim3( kBlock ), 0, st, g.tiles, g.nTiles, nC, L, g.stride,
                           m.cellInfo, flag, (float*) dst, (const float*) a, (const float*) b );
+}
+
+template < typename T >
+static void cgUpdT( const MeshCtx& m, const LevelGeom& g, int L, void* x, void* r, const void* p, const void* ap,
+                    const double* sc, int rsOld, int pAp, double tol2, int flag, hipStream_t st )
+{
+   const int nC = m.mesh->nC(), nb = g.nTiles * nC;
+   hipLaunchKernelGGL( ( kCgUpdate< T > ), dim3( xcdGrid( nb ) ), dim3( kBlock ), 0, st, g.tiles, g.nTiles, nC, L, g.stride,
+                       m.cellInfo, flag, (T*) x, (T*) r, (const T*) p, (const T*) ap, sc, rsOld, pAp, tol2 );
+}
+template < typename T >
+static void cgDirT( const MeshCtx& m, const LevelGeom& g, int L, void* p, const void* r, const double* sc, int rsOld, int rsNew,
+                    double tol2, int flag, hipStream_t st )
+{
+   const int nC = m.mesh->nC(), nb = g.nTiles * nC;
+   hipLaunchKernelGGL( ( kCgDirection< T > ), dim3( xcdGrid( nb ) ), dim3( kBlock ), 0, st, g.tiles, g.nTiles, nC, L, g.stride,
+                       m.cellInfo, flag, (T*) p, (const T*) r, sc, rsOld, rsNew, tol2 );
+}
+void launchCgUpdate( const MeshCtx& m, const LevelGeom& g, int L, int dtype, void* x, void* r, const void* p, const void* ap,
+                     const double* sc, int rsOld, int pAp, double tol2, int flag, hipStream_t st )
+{
+   if ( dtype == 8 )
+      cgUpdT< double >( m, g, L, x, r, p, ap, sc, rsOld, pAp, tol2, flag, st );
+   else
+      cgUpdT< float >( m, g, L, x, r, p, ap, sc, rsOld, pAp, tol2, flag, st );
+}
+void launchCgDirection( const MeshCtx& m, const LevelGeom& g, int L, int dtype, void* p, const void* r, const double* sc, int rsOld,
+                        int rsNew, double tol2, int flag, hipStream_t st )
+{
+   if ( dtype == 8 )
+      cgDirT< double >( m, g, L, p, r, sc, rsOld, rsNew, tol2, flag, st );
+   else
+      cgDirT< float >( m, g, L, p, r, sc, rsOld, rsNew, tol2, flag, st );
 }
 
 int dotPartialsNeeded( const MeshCtx& m, const LevelGeom& g ) { return g.nTiles * m.mesh->nC(); }

@@ -53,6 +53,12 @@ void launchMult( const MeshCtx& m, const LevelGeom& g, int L, int dtype, void* d
 void launchDot( const MeshCtx& m, const LevelGeom& g, int L, int dtype, const void* a, const void* b, int flag, double* partials,
                 double* result, hipStream_t st );
 int  dotPartialsNeeded( const MeshCtx& m, const LevelGeom& g );
+// conjugate-gradient steps with device-resident scalars sc[]: x += a p, r -= a Ap with a = sc[rsOld] / sc[pAp];
+// p = r + (sc[rsNew] / sc[rsOld]) p; no step once sc[rsOld] < tol2
+void launchCgUpdate( const MeshCtx& m, const LevelGeom& g, int L, int dtype, void* x, void* r, const void* p, const void* ap,
+                     const double* sc, int rsOld, int pAp, double tol2, int flag, hipStream_t st );
+void launchCgDirection( const MeshCtx& m, const LevelGeom& g, int L, int dtype, void* p, const void* r, const double* sc, int rsOld,
+                        int rsNew, double tol2, int flag, hipStream_t st );
 // value-type conversion over n values
 void launchConvert( i64 n, int dstType, void* dst, int srcType, const void* src, hipStream_t st );
 // plain copy of n values

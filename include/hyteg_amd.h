@@ -158,6 +158,13 @@ int hyg_gmg_create( hyg_op op, int dtype, const int* iparams, const double* dpar
 int hyg_gmg_destroy( hyg_gmg g );
 /* Solver::solve(A, x, b, level) once = one V-cycle (GeometricMultigridSolver.hpp:127-134). */
 int hyg_gmg_vcycle( hyg_gmg g, hyg_func x, hyg_func b, int level, void* stream );
+/* One V-cycle as a replayed hipGraph: the first call for (x, b, level, cg_iterations) captures the whole
+ * cycle on `stream` (which must not be the default stream) and instantiates it; later calls only launch
+ * it. The coarse CG runs `cg_iterations` iterations whose step lengths stay in device memory (no
+ * host round trip); once its residual norm is below the solver's CG tolerance the remaining iterations
+ * take no step, so with enough iterations the result equals hyg_gmg_vcycle. No counterpart in the
+ * reference (its V-cycle is host-driven). */
+int hyg_gmg_vcycle_graph( hyg_gmg g, hyg_func x, hyg_func b, int level, int cg_iterations, void* stream );
 /* `cycles` V-cycles; res[0] = ||b - Ax|| over Inner|Neumann DoFs before, res[k] after cycle k (FP64 norms). */
 int hyg_gmg_solve( hyg_gmg g, hyg_func x, hyg_func b, int level, int cycles, double* res, void* stream );
 /* V-cycles until ||b - Ax|| <= rel_tol * ||b - Ax0|| or `max_cycles` cycles (the convergence loop around

@@ -30,3 +30,14 @@ def load_hyg():
 @pytest.fixture(scope="session")
 def hyg():
     return load_hyg()
+
+
+@pytest.fixture(scope="session")
+def hip_stream():
+    """A HIP stream (non-default) for calls that need one, e.g. graph capture."""
+    import ctypes as C
+    lib = C.CDLL("libamdhip64.so")
+    s = C.c_void_p()
+    assert lib.hipStreamCreate(C.byref(s)) == 0
+    yield s
+    lib.hipStreamDestroy(s)
