@@ -123,9 +123,7 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
     seeded Fourier modes of wavenumber <= 8 (standard deviation 1), k evaluated at the 5 quadrature
     points of every element in every apply. Apply throughput in FP64 and FP32; V(3,3) Chebyshev
     cycles to 1e-10 in FP64 and as FP32 cycles inside FP64 defect correction."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as ol  # only for the seeded mode table (host-side numpy)
-    modes = ol.random_field(seed, modes=32, kmax=8)
+    modes = hyg.fourier_field(seed, modes=32, kmax=8)
     mesh = hyg.Mesh("cube_24el.msh")
     L = level
     n = mesh.ndofs(L)

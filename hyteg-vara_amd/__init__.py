@@ -306,3 +306,17 @@ def ir_solve(op64, gmg32, x, b, L, inner=1, max_outer=30, rel_tol=1e-10, stream=
     steps = C.c_int()
     check(load().hyg_ir_solve(op64.h, gmg32.h, x.h, b.h, L, inner, max_outer, rel_tol, res, C.byref(steps), stream))
     return res[: steps.value + 1]
+
+
+def fourier_field(seed, modes=32, kmax=8, std=1.0):
+    """Mode table of a seeded log-normal coefficient k = k0 exp(g), g = sum_m a cos(2 pi kappa_m . x + phi_m):
+    rows (kappa_x, kappa_y, kappa_z, phi, a) with integer kappa, 1 <= max-norm(kappa) <= kmax,
+    phi ~ U[0, 2 pi) and a = std sqrt(2 / modes), so that g has standard deviation std."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < modes:
+        kap = rng.integers(-kmax, kmax + 1, 3)
+        if np.abs(kap).max() == 0:
+            continue
+        out.append([kap[0], kap[1], kap[2], rng.uniform(0, 2 * np.pi), std * np.sqrt(2.0 / modes)])
+    return np.array(out, dtype=np.float64)

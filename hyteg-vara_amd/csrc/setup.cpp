@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <stdexcept>
 
 namespace hyg {
@@ -269,19 +270,35 @@ LevelGeom& MeshCtx::geom( int L )
    // A segment of rows y.. starting at column x0 covers the planes where row y still has a point at
    // x >= x0 (z <= N - y - x0): the last of them holds no interior output, only the row end that lies
    // on the diagonal face.
+   // HYG_ZCHUNK (planes per segment) and HYG_SEG_ORDER (1: columns vary fastest) override the
+   // defaults for experiments.
+   const char* ec    = std::getenv( "HYG_ZCHUNK" );
+   const char* eo    = std::getenv( "HYG_SEG_ORDER" );
+   const int   chunk = ec ? std::max( 4, std::min( 1024, std::atoi( ec ) ) ) : kZSegChunk;
+   const bool  xFast = eo && std::atoi( eo ) == 1;
    for ( int k = 0; k < kNumCellCfgs; ++k )
    {
       const int R = kCellRows[k], WO = 64 * kCellPoints[k] - 2;
       auto&     v = g.zSegsHost[k];
       v.clear();
-      for ( int z0 = 1; z0 <= N - 2; z0 += kZSegChunk )
-         for ( int x0 = 1; x0 <= N - 2; x0 += WO )
+      auto add = [&]( int y, int x0, int z0 ) {
+         const int zEnd = N - y - x0 + 1;
+         if ( z0 < zEnd )
+            v.push_back( { y, x0, z0, std::min( z0 + chunk, zEnd ) } );
+      };
+      for ( int z0 = 1; z0 <= N - 2; z0 += chunk )
+         if ( xFast )
+         {
             for ( int y = 1; y <= N - 2; y += R )
-            {
-               const int zEnd = N - y - x0 + 1;
-               if ( z0 < zEnd )
-                  v.push_back( { y, x0, z0, std::min( z0 + kZSegChunk, zEnd ) } );
-            }
+               for ( int x0 = 1; x0 <= N - 2; x0 += WO )
+                  add( y, x0, z0 );
+         }
+         else
+         {
+            for ( int x0 = 1; x0 <= N - 2; x0 += WO )
+               for ( int y = 1; y <= N - 2; y += R )
+                  add( y, x0, z0 );
+         }
       g.nZSegs[k] = (int) v.size();
       g.zSegs[k]  = upload( v );
    }

@@ -55,3 +55,10 @@ def test_chebyshev_coefficients_match_the_oracle(hyg, order):
 def test_chebyshev_order_out_of_range_is_an_argument_error(hyg):
     out = np.zeros(8)
     assert hyg.load().hyg_cheb_coefficients(9, 0.5, 2.0, out) == hyg.ERR_ARG
+
+
+def test_fourier_field_matches_the_oracle_helper(hyg):
+    for seed, m, k in ((2024, 32, 8), (3, 8, 3)):
+        a, b = hyg.fourier_field(seed, modes=m, kmax=k), ol.random_field(seed, modes=m, kmax=k)
+        assert a.shape == (m, 5) and np.array_equal(a, b)
+        assert np.all(np.abs(a[:, :3]).max(axis=1) >= 1) and np.all(np.abs(a[:, :3]) <= k)
