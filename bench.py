@@ -263,7 +263,7 @@ def main():
                    "mesh": args.mesh, "level": L, "dofs_total": n, "dofs_inner": inner, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": profile_traffic(),
-                     "kernel": "P1 apply (kCellZ + kFaceFlat + kFaceCombine + kLower, one launch each)",
+                     "kernel": "P1 apply: kSweep (cell interiors, one-sided face sums, edges, vertices) + kFaceCombine",
                      "algorithmic_bytes_per_launch": alg_bytes},
         "copy_reference": {"bytes_per_launch": 16.0 * nval, "ms": round(ms_copy, 5), "GB/s": round(copy_gbs, 1)},
         "cell_kernel_config": {"apply": "rows %d x points %d" % op.config(L, hyg.F64, 0)[1:],

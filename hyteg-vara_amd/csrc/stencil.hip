@@ -1,6 +1,6 @@
 // gfx950 kernels of the 15-point P1 stencil operations (apply, residual, Jacobi, Chebyshev steps).
 //
-// Cell interior: z-marching wavefronts (kCellZ). A wavefront owns R adjacent rows (y0 .. y0+R-1) and
+// Cell interior: z-marching wavefronts (cellZ). A wavefront owns R adjacent rows (y0 .. y0+R-1) and
 // a 64-point x-window (x0-1 .. x0+62, outputs x0 .. x0+61) of one macro-cell and marches up the
 // planes z0 .. z1-1. A five-plane register ring holds rows y0-1 .. y0+R of planes z-1, z, z+1 and
 // has planes z+2, z+3 in flight, so each plane step issues R+2 coalesced row loads (plus R rows of the
@@ -17,10 +17,10 @@
 // the windows starting at a row start (per-lane weights), the diagonal face x+y+z = N is the last
 // point of each row (a second weight set, evaluated by the wavefronts whose window holds row ends);
 // both are gathered in registers over the march and stored once per segment. The z = 0 and y = 0
-// faces are whole planes/rows and are computed by kFaceFlat with coalesced loads. kFaceCombine adds
+// faces are whole planes/rows and are computed by faceFlat with coalesced loads. kFaceCombine adds
 // the partial sums of the two sides, applies the epilogue and writes every replica.
 //
-// Macro-edges and -vertices (kLower): one lane per point gathers the one-sided stencils of all
+// Macro-edges and -vertices (lower): one lane per point gathers the one-sided stencils of all
 // incident cells.
 #include "face_combine.hpp"
 
@@ -42,7 +42,7 @@ constexpr int kDirE[4] = { 2, 6, 10, 13 };
 
 // one row of a stencil sweep: the 7 unshifted, 4 x-1 (W, NW, TW, BNW) and 4 x+1 (E, SE, TSE, BE)
 // terms for the P points of a lane; wu / we are the per-lane weights of the first point's
-// unshifted terms and of the E-group terms formed for the first point (see kCellZ).
+// unshifted terms and of the E-group terms formed for the first point (see cellZ).
 template < typename T, int P >
 __device__ __forceinline__ void stencilRow( const T* w, const T* wu, const T* we, const T ( &bc )[P], const T ( &bn )[P],
                                             const T ( &bs )[P], const T ( &cc )[P], const T ( &cs )[P], const T ( &ac )[P],
@@ -109,7 +109,7 @@ __device__ __forceinline__ void stencilRow( const T* w, const T* wu, const T* we
 }
 
 template < typename T, int MODE, int R, int P >
-__global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ segs, int nSegs, int nC, int L, i64 stride,
+__device__ __forceinline__ void cellZ( int bid, const ZSeg* __restrict__ segs, int nSegs, int nC, int L, i64 stride,
                                                     const T* __restrict__ cellSt, const CellFaceSlot* __restrict__ cfs,
                                                     const T* __restrict__ entryW, T* __restrict__ part, int flag,
                                                     const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
@@ -122,7 +122,7 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    constexpr int  WO   = 64 * P - 2; // outputs per window: x0 .. x0+WO-1
    const int      nbx  = ( nSegs + 3 ) / 4; // blocks per cell
    const int      nb   = nbx * nC;
-   const int      lb   = remapXcd( int( blockIdx.x ), nb );
+   const int      lb   = remapXcd( bid, nb );
    if ( lb >= nb )
       return;
    const int c  = lb / nbx;
@@ -196,27 +196,27 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    int       pl    = z0 - 1;
    int       Wl    = N + 1 - pl;
    int       off   = S3i( N + 1 ) - S3i( Wl ) + rowStart( Wl, y0 - 1 ) + x0 - 1;
-   // lanes whose first point lies past the end of a row take the out-of-range offset: they read 0 and
-   // fetch nothing (a second point past the end reads a value of the next row and is never used)
-   const int laneEnd = N + 2 - x0 - y0; // window offset of the last point of row y0-1 in plane 0
-   auto      loadPlane = [&]( PlaneRows< T, R, P >& Q ) {
+   // Lanes past the end of a row (and rows past the end of a plane) read whatever follows in the
+   // array, within the descriptor's range (cell array plus tail slack): no output of the cell
+   // interior uses such a lane (every neighbour of an interior point lies in the cell), and the
+   // diagonal face's one-sided sums weight them by 0. The loads therefore need no per-lane masks.
+   auto loadPlane = [&]( PlaneRows< T, R, P >& Q ) {
       int o = off;
 #pragma unroll
       for ( int k = 0; k < R + 2; ++k )
       {
-         const int le = laneEnd - pl - k; // window offset of the last point of row y0-1+k in plane pl
          if constexpr ( P == 1 )
-            Q.r[k][0] = bufLoad< T >( rx, d0 <= le ? vo : kDropOffset, o * B );
+            Q.r[k][0] = bufLoad< T >( rx, vo, o * B );
          else
-            bufLoad2( rx, d0 <= le ? vo : kDropOffset, o * B, Q.r[k][0], Q.r[k][1] );
+            bufLoad2( rx, vo, o * B, Q.r[k][0], Q.r[k][1] );
          if constexpr ( kE )
          {
             if ( k >= 1 && k <= R )
             {
                if constexpr ( P == 1 )
-                  Q.e[k - 1][0] = bufLoad< T >( re, d0 < le ? vo : kDropOffset, o * B );
+                  Q.e[k - 1][0] = bufLoad< T >( re, vo, o * B );
                else
-                  bufLoad2( re, d0 < le ? vo : kDropOffset, o * B, Q.e[k - 1][0], Q.e[k - 1][1] );
+                  bufLoad2( re, vo, o * B, Q.e[k - 1][0], Q.e[k - 1][1] );
             }
          }
          o += Wl - ( y0 - 1 + k ); // start of the next row
@@ -231,6 +231,15 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
    // output plane z: offset oz of (x0-1, y0-1), row 0 of the plane has Wz points
    int       z = z0, oz = 0, Wz = N + 1 - z0;
    const int lastD = N - y0 - x0; // window offset of x = N-1-y-z is lastD - (y - y0) - z
+   // point k of the lane stores when it is an output of the window and inside the row:
+   // d <= lastD - r - z, i.e. sd[k] + r + z <= lastD (sd = d, or a large value when never)
+   int sd[P];
+#pragma unroll
+   for ( int k = 0; k < P; ++k )
+   {
+      const int d = d0 + k;
+      sd[k]       = ( interior && d >= 1 && d <= WO ) ? d : ( 1 << 28 );
+   }
    // A, B, C: planes z-1, z, z+1 (resident); plane z+2 is in flight; E receives plane z+3
    auto step = [&]( const PlaneRows< T, R, P >& A, const PlaneRows< T, R, P >& Bp, const PlaneRows< T, R, P >& C,
                     PlaneRows< T, R, P >& E ) {
@@ -248,20 +257,19 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
          {
             T o0, o1;
             epilogue< T, MODE >( ax[k], Bp.r[r + 1][k], Bp.e[r][k], Bp.e[r][k], w[15], a, o0, o1 );
-            const int      d  = d0 + k;
-            const unsigned so = ( interior && d >= 1 && d <= WO && d <= lastD - r - z ) ? unsigned( d * B ) : kDropOffset;
+            const unsigned so = sd[k] <= lastD - r - z ? unsigned( ( d0 + k ) * B ) : kDropOffset;
             bufStore( o0, r0, so, o * B );
             if constexpr ( kTwo )
                bufStore( o1, r1, so, o * B );
          }
          if ( onX0 ) // the first point of lane 0 is the x = 0 face point of row y0+r, plane z
             f2v[r] = moveLane( ax[0], 0, z - z0, f2v[r] );
-         if ( onDiag )
+         const int dEnd = lastD + 1 - r - z; // window offset of the row end
+         if ( onDiag && dEnd >= 0 && dEnd < 64 * P ) // (wave-uniform) the row end is in the window
          {
             T ad[P];
             stencilRow< T, P >( wd, wdu, wde, Bp.r[r + 1], Bp.r[r + 2], Bp.r[r], C.r[r + 1], C.r[r], A.r[r + 1], A.r[r + 2],
                                 ad );
-            const int dEnd = lastD + 1 - r - z; // window offset of the row end
 #pragma unroll
             for ( int k = 0; k < P; ++k )
                dg[r][k] = d0 + k == dEnd ? ad[k] : dg[r][k];
@@ -336,13 +344,13 @@ __global__ __launch_bounds__( kBlock ) void kCellZ( const ZSeg* __restrict__ seg
 // ------------------------------------------------------------------ z = 0 and y = 0 faces: one-sided sums
 // Work item: cell, slot (0: plane z = 0, row `line`; 1: row y = 0 of plane `line`), window start x0.
 template < typename T >
-__global__ __launch_bounds__( kBlock ) void kFaceFlat( const int4* __restrict__ items, int nItems, int L, i64 stride,
+__device__ __forceinline__ void faceFlat( int bid, const int4* __restrict__ items, int nItems, int L, i64 stride,
                                                        const CellFaceSlot* __restrict__ cfs, const T* __restrict__ entryW,
                                                        T* __restrict__ part, int flag, const T* __restrict__ x )
 {
    constexpr int B  = int( sizeof( T ) );
    const int     nb = ( nItems + 3 ) / 4;
-   const int     wi = remapXcd( int( blockIdx.x ), nb ) * 4 + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
+   const int     wi = remapXcd( bid, nb ) * 4 + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
    if ( wi >= nItems )
       return;
    const int4         it = items[wi];
@@ -387,14 +395,14 @@ __global__ __launch_bounds__( kBlock ) void kFaceFlat( const int4* __restrict__ 
 // zero weight in that cell's one-sided stencil; they read the point itself instead, so all loads of a
 // lane are unconditional and independent and issue back to back.
 template < typename T, int MODE >
-__global__ __launch_bounds__( kBlock ) void kLower( const LowerChunk* __restrict__ chunks, int nChunks,
+__device__ __forceinline__ void lower( int bid, const LowerChunk* __restrict__ chunks, int nChunks,
                                                     const LowerPrim* __restrict__ prims, const LowerEntry* __restrict__ entries,
                                                     const T* __restrict__ entryW, const T* __restrict__ primInv, int L,
                                                     i64 stride, int flag, const T* __restrict__ x, const T* __restrict__ in1,
                                                     T* __restrict__ out0, T* __restrict__ out1, T a )
 {
    const int nb = ( nChunks + 3 ) / 4;
-   const int ci = remapXcd( int( blockIdx.x ), nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
+   const int ci = remapXcd( bid, nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
    if ( ci >= nChunks )
       return;
    const int        lane = int( threadIdx.x & 63 );
@@ -451,6 +459,50 @@ __global__ __launch_bounds__( kBlock ) void kLower( const LowerChunk* __restrict
    }
 }
 
+// ------------------------------------------------------------------ one launch per sweep
+// The cell interiors (with the one-sided sums of the x = 0 and diagonal faces), the one-sided sums of
+// the z = 0 and y = 0 faces and the macro-edge/-vertex points read only x and in1 and write disjoint
+// points, so they share one launch: block ranges [0, gC), [gC, gC + gF) and the rest. The small
+// parts are dispatched last and run in the tail of the cell part.
+template < typename T >
+struct SweepArgs
+{
+   const ZSeg*         segs;
+   int                 nSegs, nC, L, flag;
+   i64                 stride;
+   const T*            cellSt;
+   const CellFaceSlot* cfs;
+   const T*            entryW;
+   const T*            primInv;
+   T*                  part;
+   const T*            x;
+   const T*            in1;
+   T*                  out0;
+   T*                  out1;
+   T                   a;
+   const int4*         items;
+   int                 nItems;
+   const LowerChunk*   chunks;
+   int                 nChunks;
+   const LowerPrim*    prims;
+   const LowerEntry*   entries;
+   int                 gC, gF;
+};
+
+template < typename T, int MODE, int R, int P >
+__global__ __launch_bounds__( kBlock ) void kSweep( SweepArgs< T > p )
+{
+   const int b = int( blockIdx.x );
+   if ( b < p.gC )
+      cellZ< T, MODE, R, P >( b, p.segs, p.nSegs, p.nC, p.L, p.stride, p.cellSt, p.cfs, p.entryW, p.part, p.flag, p.x, p.in1,
+                              p.out0, p.out1, p.a );
+   else if ( b < p.gC + p.gF )
+      faceFlat< T >( b - p.gC, p.items, p.nItems, p.L, p.stride, p.cfs, p.entryW, p.part, p.flag, p.x );
+   else
+      lower< T, MODE >( b - p.gC - p.gF, p.chunks, p.nChunks, p.prims, p.entries, p.entryW, p.primInv, p.L, p.stride, p.flag,
+                        p.x, p.in1, p.out0, p.out1, p.a );
+}
+
 // ------------------------------------------------------------------ launch
 // cell-interior configuration of a level, value type and stencil mode: set by the tuner, else
 // HYG_CELL_CFG, else 2 rows x 1 point per lane
@@ -480,38 +532,54 @@ static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol,
    const T*       in1  = (const T*) c.in1;
    T*             out0 = (T*) c.out0;
    T*             out1 = (T*) c.out1;
-   // 1. cell interiors, and the one-sided sums of the x = 0 and diagonal faces
-   if ( g.nZSegs[cfg] > 0 )
+   // 1. cell interiors with the one-sided sums of the x = 0 and diagonal faces, the one-sided sums
+   // of the z = 0 and y = 0 faces, and macro-edges and -vertices: one launch
+   const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
+   SweepArgs< T > p;
+   p.segs    = g.zSegs[cfg];
+   p.nSegs   = g.nZSegs[cfg];
+   p.nC      = nC;
+   p.L       = L;
+   p.flag    = c.flag;
+   p.stride  = g.stride;
+   p.cellSt  = cst;
+   p.cfs     = m.cellFaces;
+   p.entryW  = ew;
+   p.primInv = pi;
+   p.part    = part;
+   p.x       = x;
+   p.in1     = in1;
+   p.out0    = out0;
+   p.out1    = out1;
+   p.a       = T( c.a );
+   p.items   = g.flatItems;
+   p.nItems  = nFace > 0 ? g.nFlatItems : 0;
+   p.chunks  = g.chunks;
+   p.nChunks = nLow;
+   p.prims   = m.prims;
+   p.entries = m.entries;
+   p.gC      = p.nSegs > 0 ? xcdGrid( ( ( p.nSegs + 3 ) / 4 ) * nC ) : 0;
+   p.gF      = p.nItems > 0 ? xcdGrid( ceilDiv( p.nItems, 4 ) ) : 0;
+   const int gL = nLow > 0 ? xcdGrid( ceilDiv( nLow, kBlock / 64 ) ) : 0;
+   if ( p.gC + p.gF + gL > 0 )
    {
-      const int nb = ( ( g.nZSegs[cfg] + 3 ) / 4 ) * nC;
-      decltype( &kCellZ< T, MODE, 1, 1 > ) kf;
+      decltype( &kSweep< T, MODE, 1, 1 > ) kf;
       switch ( cfg )
       {
-      case 0: kf = kCellZ< T, MODE, 1, 1 >; break;
-      case 1: kf = kCellZ< T, MODE, 2, 1 >; break;
-      case 2: kf = kCellZ< T, MODE, 4, 1 >; break;
-      case 3: kf = kCellZ< T, MODE, 1, 2 >; break;
-      case 4: kf = kCellZ< T, MODE, 2, 2 >; break;
-      default: kf = kCellZ< T, MODE, 4, 2 >; break;
+      case 0: kf = kSweep< T, MODE, 1, 1 >; break;
+      case 1: kf = kSweep< T, MODE, 2, 1 >; break;
+      case 2: kf = kSweep< T, MODE, 4, 1 >; break;
+      case 3: kf = kSweep< T, MODE, 1, 2 >; break;
+      case 4: kf = kSweep< T, MODE, 2, 2 >; break;
+      default: kf = kSweep< T, MODE, 4, 2 >; break;
       }
-      hipLaunchKernelGGL( kf, dim3( xcdGrid( nb ) ), dim3( kBlock ), 0, s, g.zSegs[cfg], g.nZSegs[cfg], nC, L, g.stride, cst,
-                          m.cellFaces, ew, part, c.flag, x, in1, out0, out1, T( c.a ) );
+      hipLaunchKernelGGL( kf, dim3( p.gC + p.gF + gL ), dim3( kBlock ), 0, s, p );
    }
-   const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
+   // 2. face values from the partial sums
    if ( nFace > 0 )
-   {
-      // 2. one-sided sums of the z = 0 and y = 0 faces; 3. face values from the partial sums
-      if ( g.nFlatItems > 0 )
-         hipLaunchKernelGGL( ( kFaceFlat< T > ), dim3( xcdGrid( ceilDiv( g.nFlatItems, 4 ) ) ), dim3( kBlock ), 0, s, g.flatItems,
-                             g.nFlatItems, L, g.stride, m.cellFaces, ew, part, c.flag, x );
       hipLaunchKernelGGL( ( kFaceCombine< T, MODE > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
                           g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, m.mesh->nV() + m.mesh->nE(), part, pi,
                           (const T*) nullptr, L, g.stride, c.flag, x, in1, out0, out1, T( c.a ) );
-   }
-   // 4. macro-edges and -vertices
-   if ( nLow > 0 )
-      hipLaunchKernelGGL( ( kLower< T, MODE > ), dim3( xcdGrid( ceilDiv( nLow, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, g.chunks,
-                          nLow, m.prims, m.entries, ew, pi, L, g.stride, c.flag, x, in1, out0, out1, T( c.a ) );
 }
 
 template < typename T >

@@ -31,7 +31,10 @@
 //     numerictools/SpectrumEstimation.hpp:57-80;
 //   * CG                      solvers/CGSolver.hpp:66-134, 237-248;
 //   * GMG V-cycle             solvers/GeometricMultigridSolver.hpp:127-235;
-//   * mixed-precision IR      tests/hyteg/mixedPrecision/mixedPrecisionIterativeRefinement.cpp:162-172.
+//   * mixed-precision IR      tests/hyteg/mixedPrecision/mixedPrecisionIterativeRefinement.cpp:162-172;
+//   * P2 (quadratic) path      P2ConstantOperator.cpp:99-113 (restated element by element, as
+//     P2ElementwiseOperator.cpp:100-310), p2_tet_diffusion, P2toP2QuadraticProlongation.cpp:217-424,
+//     P2toP2QuadraticRestriction.cpp:131-286, on the level-(L+1) vertex lattice.
 //
 // How each function is pinned (tests/test_oracle_*.py): the apply, smoother, transfer
 // and V-cycle against an independent scipy global assembly of the FE matrices on the
@@ -49,6 +52,7 @@
 #include <memory>
 #include <set>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -485,6 +489,9 @@ struct OperatorBase
    std::unique_ptr< Function< T > > invDiag;
    virtual ~OperatorBase() = default;
    virtual void apply( const Function< T >& src, Function< T >& dst, int L, int flag, bool add ) const = 0;
+   // grid transfers of the operator's function space (P1 linear by default, defined after the transfers)
+   virtual void restrictOp( Function< T >& f, int srcL, int flag ) const;
+   virtual void prolongateOp( Function< T >& f, int srcL, int flag, bool add ) const;
 };
 
 template < typename T >
@@ -944,33 +951,11 @@ static const int MICRO_CELL[6][4][3] = {
 // p1_div_k_grad_affine_q3::integrateAll (3D) (forms/form_hyteg_generated/p1/p1_div_k_grad_affine_q3.cpp:179-381):
 // a_ij = sum_q w_q k(x_q) |det J| grad phi_i . grad phi_j with the 5-point degree-3 rule: the centroid with
 // weight -2/15 and the four points with barycentric coordinates (1/2, 1/6, 1/6, 1/6) with weight 3/40.
+inline double baryGrads( const P3* x, double G[4][3] );
 inline void divKGradElement( const P3* x, const KField& k, double a[4][4] )
 {
-   double J[3][3];
-   for ( int r = 0; r < 3; ++r )
-      for ( int c = 0; c < 3; ++c )
-         J[r][c] = x[c + 1][r] - x[0][r];
-   const double det = J[0][0] * ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) - J[0][1] * ( J[1][0] * J[2][2] - J[1][2] * J[2][0] ) +
-                      J[0][2] * ( J[1][0] * J[2][1] - J[1][1] * J[2][0] );
-   // rows of J^-1 are the gradients of the barycentric coordinates 1..3
-   double Ji[3][3];
-   Ji[0][0] = ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) / det;
-   Ji[0][1] = ( J[0][2] * J[2][1] - J[0][1] * J[2][2] ) / det;
-   Ji[0][2] = ( J[0][1] * J[1][2] - J[0][2] * J[1][1] ) / det;
-   Ji[1][0] = ( J[1][2] * J[2][0] - J[1][0] * J[2][2] ) / det;
-   Ji[1][1] = ( J[0][0] * J[2][2] - J[0][2] * J[2][0] ) / det;
-   Ji[1][2] = ( J[0][2] * J[1][0] - J[0][0] * J[1][2] ) / det;
-   Ji[2][0] = ( J[1][0] * J[2][1] - J[1][1] * J[2][0] ) / det;
-   Ji[2][1] = ( J[0][1] * J[2][0] - J[0][0] * J[2][1] ) / det;
-   Ji[2][2] = ( J[0][0] * J[1][1] - J[0][1] * J[1][0] ) / det;
-   double G[4][3];
-   for ( int c = 0; c < 3; ++c )
-   {
-      G[1][c] = Ji[0][c];
-      G[2][c] = Ji[1][c];
-      G[3][c] = Ji[2][c];
-      G[0][c] = -( G[1][c] + G[2][c] + G[3][c] );
-   }
+   double       G[4][3];
+   const double det = baryGrads( x, G );
    const double lam[5][4] = { { 0.25, 0.25, 0.25, 0.25 },
                               { 1.0 / 6, 1.0 / 6, 1.0 / 6, 0.5 },
                               { 1.0 / 6, 1.0 / 6, 0.5, 1.0 / 6 },
@@ -990,6 +975,36 @@ inline void divKGradElement( const P3* x, const KField& k, double a[4][4] )
    for ( int i = 0; i < 4; ++i )
       for ( int j = 0; j < 4; ++j )
          a[i][j] = s * ( G[i][0] * G[j][0] + G[i][1] * G[j][1] + G[i][2] * G[j][2] );
+}
+
+// gradients of the barycentric coordinates of tetrahedron x (rows of J^-1, J = [x1-x0 x2-x0 x3-x0]); returns det J
+inline double baryGrads( const P3* x, double G[4][3] )
+{
+   double J[3][3];
+   for ( int r = 0; r < 3; ++r )
+      for ( int c = 0; c < 3; ++c )
+         J[r][c] = x[c + 1][r] - x[0][r];
+   const double det = J[0][0] * ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) - J[0][1] * ( J[1][0] * J[2][2] - J[1][2] * J[2][0] ) +
+                      J[0][2] * ( J[1][0] * J[2][1] - J[1][1] * J[2][0] );
+   // rows of J^-1 are the gradients of the barycentric coordinates 1..3
+   double Ji[3][3];
+   Ji[0][0] = ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) / det;
+   Ji[0][1] = ( J[0][2] * J[2][1] - J[0][1] * J[2][2] ) / det;
+   Ji[0][2] = ( J[0][1] * J[1][2] - J[0][2] * J[1][1] ) / det;
+   Ji[1][0] = ( J[1][2] * J[2][0] - J[1][0] * J[2][2] ) / det;
+   Ji[1][1] = ( J[0][0] * J[2][2] - J[0][2] * J[2][0] ) / det;
+   Ji[1][2] = ( J[0][2] * J[1][0] - J[0][0] * J[1][2] ) / det;
+   Ji[2][0] = ( J[1][0] * J[2][1] - J[1][1] * J[2][0] ) / det;
+   Ji[2][1] = ( J[0][1] * J[2][0] - J[0][0] * J[2][1] ) / det;
+   Ji[2][2] = ( J[0][0] * J[1][1] - J[0][1] * J[1][0] ) / det;
+   for ( int c = 0; c < 3; ++c )
+   {
+      G[1][c] = Ji[0][c];
+      G[2][c] = Ji[1][c];
+      G[3][c] = Ji[2][c];
+      G[0][c] = -( G[1][c] + G[2][c] + G[3][c] );
+   }
+   return det;
 }
 
 // P1ElementwiseOperator<p1_div_k_grad_affine_q3> (elementwiseoperators/P1ElementwiseOperator.cpp:64-220, 275-301,
@@ -1175,6 +1190,368 @@ void prolongate3D( Function< T >& fn, int srcL, int flag, bool add )
    commAdditiveCellToLower( fn, dL, flag ^ All, !add );
 }
 
+template < typename T >
+void OperatorBase< T >::restrictOp( Function< T >& f, int srcL, int flag ) const
+{
+   restrict3D< T >( f, srcL, flag );
+}
+template < typename T >
+void OperatorBase< T >::prolongateOp( Function< T >& f, int srcL, int flag, bool add ) const
+{
+   prolongate3D< T >( f, srcL, flag, add );
+}
+
+// ---------------------------------------------------------------- P2 (quadratic Lagrange) path
+// A P2Function of level L (p2functionspace/P2Function.hpp) holds vertex DoFs at the vertices of the level-L
+// micro-mesh and edge DoFs at the midpoints of its micro-edges (edgedofspace/EdgeDoFIndexing.hpp:919-1000, the
+// seven orientations X Y Z XY XZ YZ XYZ). Together these points are exactly the vertices of level L+1, so the
+// oracle stores a P2 function of level L as a vertex-DoF function of "lattice" level L+1 (vertex DoF (i,j,k) at
+// lattice point (2i,2j,2k), the edge DoF between vertices a and b at a+b). Every P2 routine below takes lattice
+// levels and works on the micro-cells of level lattice-1.
+
+// local nodes of a P2 tetrahedron: vertices 0..3, then the edges (2,3) (1,3) (1,2) (0,3) (0,2) (0,1)
+// (the FEniCS P2 tetrahedron of form_fenics_generated/p2_tet_diffusion.h)
+static const int P2_EDGE[6][2] = { { 2, 3 }, { 1, 3 }, { 1, 2 }, { 0, 3 }, { 0, 2 }, { 0, 1 } };
+
+inline void p2Basis( const double lam[4], double phi[10] )
+{
+   for ( int v = 0; v < 4; ++v )
+      phi[v] = lam[v] * ( 2.0 * lam[v] - 1.0 );
+   for ( int e = 0; e < 6; ++e )
+      phi[4 + e] = 4.0 * lam[P2_EDGE[e][0]] * lam[P2_EDGE[e][1]];
+}
+
+inline void p2Grads( const double lam[4], const double G[4][3], double d[10][3] )
+{
+   for ( int v = 0; v < 4; ++v )
+      for ( int c = 0; c < 3; ++c )
+         d[v][c] = ( 4.0 * lam[v] - 1.0 ) * G[v][c];
+   for ( int e = 0; e < 6; ++e )
+   {
+      const int a = P2_EDGE[e][0], b = P2_EDGE[e][1];
+      for ( int c = 0; c < 3; ++c )
+         d[4 + e][c] = 4.0 * ( lam[a] * G[b][c] + lam[b] * G[a][c] );
+   }
+}
+
+// p2_tet_diffusion tabulate_tensor: a_ij = integral over the tetrahedron of grad phi_i . grad phi_j. The
+// integrand is quadratic, so the 4-point rule with barycentric coordinates (a, b, b, b), a = (5 + 3 sqrt 5) / 20,
+// b = (5 - sqrt 5) / 20, and weights vol / 4 is exact.
+inline void p2LaplaceElement( const P3* x, double a[10][10] )
+{
+   double       G[4][3];
+   const double vol = std::fabs( baryGrads( x, G ) ) / 6.0;
+   const double qa = ( 5.0 + 3.0 * std::sqrt( 5.0 ) ) / 20.0, qb = ( 5.0 - std::sqrt( 5.0 ) ) / 20.0;
+   for ( int i = 0; i < 10; ++i )
+      for ( int j = 0; j < 10; ++j )
+         a[i][j] = 0.0;
+   for ( int q = 0; q < 4; ++q )
+   {
+      double lam[4] = { qb, qb, qb, qb };
+      lam[q]        = qa;
+      double d[10][3];
+      p2Grads( lam, G, d );
+      for ( int i = 0; i < 10; ++i )
+         for ( int j = 0; j < 10; ++j )
+            a[i][j] += 0.25 * vol * ( d[i][0] * d[j][0] + d[i][1] * d[j][1] + d[i][2] * d[j][2] );
+   }
+}
+
+// lattice coordinates (level lat) of the 10 nodes of micro-cell (t, anchor) of level lat-1
+inline void p2Nodes( int t, idx ax, idx ay, idx az, idx out[10][3] )
+{
+   idx v[4][3];
+   for ( int k = 0; k < 4; ++k )
+   {
+      v[k][0] = ax + MICRO_CELL[t][k][0];
+      v[k][1] = ay + MICRO_CELL[t][k][1];
+      v[k][2] = az + MICRO_CELL[t][k][2];
+      for ( int c = 0; c < 3; ++c )
+         out[k][c] = 2 * v[k][c];
+   }
+   for ( int e = 0; e < 6; ++e )
+      for ( int c = 0; c < 3; ++c )
+         out[4 + e][c] = v[P2_EDGE[e][0]][c] + v[P2_EDGE[e][1]][c];
+}
+
+// fn( t, node lattice coordinates ) for every micro-cell of level lat-1 of a macro-cell
+template < typename Fn >
+void forP2Cells( int lat, Fn&& fn )
+{
+   const idx Nc = idx( 1 ) << ( lat - 1 );
+   for ( int t = 0; t < 6; ++t )
+      for ( idx z = 0; z <= Nc; ++z )
+         for ( idx y = 0; y + z <= Nc; ++y )
+            for ( idx x = 0; x + y + z <= Nc; ++x )
+            {
+               bool inside = true;
+               for ( int v = 0; v < 4 && inside; ++v )
+                  inside = inCell( Nc, x + MICRO_CELL[t][v][0], y + MICRO_CELL[t][v][1], z + MICRO_CELL[t][v][2] );
+               if ( !inside )
+                  continue;
+               idx nd[10][3];
+               p2Nodes( t, x, y, z, nd );
+               fn( t, nd );
+            }
+}
+
+// P2ConstantOperator< p2_tet_diffusion > (p2functionspace/P2ConstantOperator.cpp:99-113 with the stencils of
+// P2Elements3D, i.e. the sums of element-matrix entries over the micro-cells at a DoF), restated element by
+// element: for every micro-cell, y_T += A_t x_T with the element matrix of its type in its macro-cell
+// (P2ElementwiseOperator::apply, elementwiseoperators/P2ElementwiseOperator.cpp:100-310, performs the same sums),
+// followed by the additive Cell -> Face/Edge/Vertex reduction. Levels are lattice levels.
+template < typename T >
+struct P2ConstantLaplace : OperatorBase< T >
+{
+   using OperatorBase< T >::m;
+   using OperatorBase< T >::invDiag;
+   // [lat][6 * cell + type] element matrices of the micro-cells of level lat-1
+   std::vector< std::vector< std::array< std::array< double, 10 >, 10 > > > el;
+
+   P2ConstantLaplace( const Mesh* mesh, int lo, int hi ) // P2 levels
+   {
+      this->m    = mesh;
+      this->minL = lo + 1;
+      this->maxL = hi + 1;
+      el.resize( hi + 2 );
+      for ( int lat = lo + 1; lat <= hi + 1; ++lat )
+      {
+         el[lat].resize( 6 * m->Cl.size() );
+         for ( size_t c = 0; c < m->Cl.size(); ++c )
+            for ( int t = 0; t < 6; ++t )
+            {
+               P3 xc[4];
+               for ( int v = 0; v < 4; ++v )
+                  xc[v] = microCoord( *m, (int) c, lat - 1, MICRO_CELL[t][v][0], MICRO_CELL[t][v][1], MICRO_CELL[t][v][2] );
+               double a[10][10];
+               p2LaplaceElement( xc, a );
+               for ( int i = 0; i < 10; ++i )
+                  for ( int j = 0; j < 10; ++j )
+                     el[lat][6 * c + t][i][j] = a[i][j];
+            }
+      }
+      invDiag = std::make_unique< Function< T > >( m, this->minL, this->maxL );
+      for ( int lat = this->minL; lat <= this->maxL; ++lat )
+         computeInverseDiagonal( lat );
+   }
+
+   void apply( const Function< T >& src, Function< T >& dst, int lat, int flag, bool add ) const override
+   {
+      src.commDown( lat );
+      if ( !add )
+         interpolate< T >( dst, 0.0, lat, flag );
+      const idx N = idx( 1 ) << lat;
+      const idx w = N + 1;
+#pragma omp parallel for schedule( dynamic )
+      for ( idx c = 0; c < (idx) m->Cl.size(); ++c )
+      {
+         auto&       dc = dst.c[lat][c];
+         const auto& sc = src.c[lat][c];
+         for ( idx z = 0; z <= N; ++z )
+            for ( idx y = 0; y + z <= N; ++y )
+               for ( idx x = 0; x + y + z <= N; ++x )
+                  if ( x == 0 || y == 0 || z == 0 || x + y + z == N )
+                     dc[cidx( w, x, y, z )] = T( 0 );
+         forP2Cells( lat, [&]( int t, const idx( *nd )[3] ) {
+            const auto& A = el[lat][6 * c + t];
+            idx         vi[10];
+            for ( int i = 0; i < 10; ++i )
+               vi[i] = cidx( w, nd[i][0], nd[i][1], nd[i][2] );
+            for ( int i = 0; i < 10; ++i )
+            {
+               double s = 0.0;
+               for ( int j = 0; j < 10; ++j )
+                  s += A[i][j] * double( sc[vi[j]] );
+               dc[vi[i]] += T( s );
+            }
+         } );
+      }
+      commAdditiveCellToLower( dst, lat, All ^ flag, !add );
+   }
+
+   // P2ConstantOperator::computeInverseDiagonalOperatorValues: sums of the element diagonals, inverted
+   void computeInverseDiagonal( int lat )
+   {
+      auto&     D = *invDiag;
+      const idx w = ( idx( 1 ) << lat ) + 1;
+      interpolate< T >( D, 0.0, lat, All );
+      for ( idx c = 0; c < (idx) m->Cl.size(); ++c )
+      {
+         auto& dc = D.c[lat][c];
+         std::fill( dc.begin(), dc.end(), T( 0 ) );
+         forP2Cells( lat, [&]( int t, const idx( *nd )[3] ) {
+            const auto& A = el[lat][6 * c + t];
+            for ( int i = 0; i < 10; ++i )
+               dc[cidx( w, nd[i][0], nd[i][1], nd[i][2] )] += T( A[i][i] );
+         } );
+      }
+      commAdditiveCellToLower( D, lat, None, true );
+      forOwned< T >( *m, lat, All, [&]( int kind, int p, idx i ) {
+         T& v = at( D, lat, kind, p, i );
+         v    = T( 1.0 ) / v;
+      } );
+   }
+
+   void restrictOp( Function< T >& f, int srcL, int flag ) const override;
+   void prolongateOp( Function< T >& f, int srcL, int flag, bool add ) const override;
+};
+
+// A micro-cell of level lat-2 containing lattice point p of level lat (the nodes of the level-(lat-1) lattice
+// are those of the P2 function of level lat-2): type, anchor and 4x the barycentric coordinates of p.
+inline void p2Locate( idx Nc, const idx p[3], int& t, idx a[3], idx lam4[4] )
+{
+   t = -1;
+   for ( int c = 0; c < 3; ++c )
+      a[c] = 0;
+   for ( int k = 0; k < 4; ++k )
+      lam4[k] = 0;
+   const idx q[3] = { p[0] / 4, p[1] / 4, p[2] / 4 };
+   for ( int tt = 0; tt < 6; ++tt )
+      for ( int o = 0; o < 8; ++o )
+      {
+         const idx an[3] = { q[0] - ( o & 1 ), q[1] - ( ( o >> 1 ) & 1 ), q[2] - ( ( o >> 2 ) & 1 ) };
+         idx       v[4][3];
+         bool      inside = true;
+         for ( int k = 0; k < 4 && inside; ++k )
+         {
+            for ( int c = 0; c < 3; ++c )
+               v[k][c] = an[c] + MICRO_CELL[tt][k][c];
+            inside = inCell( Nc, v[k][0], v[k][1], v[k][2] );
+         }
+         if ( !inside )
+            continue;
+         // 4 lambda_1..3 = J^-1 (p - 4 v0), J = [v1-v0 v2-v0 v3-v0] (integer, det +-1)
+         long J[3][3];
+         for ( int r = 0; r < 3; ++r )
+            for ( int c = 0; c < 3; ++c )
+               J[r][c] = long( v[c + 1][r] - v[0][r] );
+         const long det = J[0][0] * ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) - J[0][1] * ( J[1][0] * J[2][2] - J[1][2] * J[2][0] ) +
+                          J[0][2] * ( J[1][0] * J[2][1] - J[1][1] * J[2][0] );
+         const long Ji[3][3] = { { J[1][1] * J[2][2] - J[1][2] * J[2][1], J[0][2] * J[2][1] - J[0][1] * J[2][2],
+                                   J[0][1] * J[1][2] - J[0][2] * J[1][1] },
+                                 { J[1][2] * J[2][0] - J[1][0] * J[2][2], J[0][0] * J[2][2] - J[0][2] * J[2][0],
+                                   J[0][2] * J[1][0] - J[0][0] * J[1][2] },
+                                 { J[1][0] * J[2][1] - J[1][1] * J[2][0], J[0][1] * J[2][0] - J[0][0] * J[2][1],
+                                   J[0][0] * J[1][1] - J[0][1] * J[1][0] } };
+         const long d[3] = { long( p[0] - 4 * v[0][0] ), long( p[1] - 4 * v[0][1] ), long( p[2] - 4 * v[0][2] ) };
+         long       l[4];
+         l[0] = 4;
+         bool ok = true;
+         for ( int r = 0; r < 3; ++r )
+         {
+            l[r + 1] = ( Ji[r][0] * d[0] + Ji[r][1] * d[1] + Ji[r][2] * d[2] ) / det;
+            ok       = ok && l[r + 1] >= 0;
+            l[0] -= l[r + 1];
+         }
+         if ( !ok || l[0] < 0 )
+            continue;
+         t = tt;
+         for ( int c = 0; c < 3; ++c )
+            a[c] = an[c];
+         for ( int k = 0; k < 4; ++k )
+            lam4[k] = l[k];
+         return;
+      }
+   throw std::runtime_error( "p2Locate: lattice point outside the macro-cell" );
+}
+
+// P2toP2QuadraticProlongation::prolongateAdditively3D (gridtransferoperators/P2toP2QuadraticProlongation.cpp:217-424):
+// every DoF of level src+1 gets the value of the quadratic interpolant of the level-src function at its point.
+// The reference pushes the coarse DoFs with the weights phi_j(p) (3/8, -1/8, 3/4, 1/4, ...) cell by cell, scaling
+// points on macro-faces/-edges/-vertices by 1 / (number of cells sharing them), then sums additively; the oracle
+// pulls the same weighted sum per fine point. srcLat is the lattice level of the coarse function.
+template < typename T >
+void prolongateP2( Function< T >& fn, int srcLat, int flag, bool add )
+{
+   const Mesh& m  = *fn.m;
+   const int   dL = srcLat + 1;
+   fn.commDown( srcLat );
+   const idx Nf = idx( 1 ) << dL, wf = Nf + 1;
+   const idx Nc = idx( 1 ) << ( srcLat - 1 ), wc = ( idx( 1 ) << srcLat ) + 1;
+#pragma omp parallel for schedule( dynamic )
+   for ( idx ci = 0; ci < (idx) m.Cl.size(); ++ci )
+   {
+      const auto& src = fn.c[srcLat][ci];
+      auto&       dst = fn.c[dL][ci];
+      for ( idx z = 0; z <= Nf; ++z )
+         for ( idx y = 0; y + z <= Nf; ++y )
+            for ( idx x = 0; x + y + z <= Nf; ++x )
+            {
+               const bool onFace = x == 0 || y == 0 || z == 0 || x + y + z == Nf;
+               T&         d      = dst[cidx( wf, x, y, z )];
+               if ( !add || onFace )
+                  d = T( 0 );
+               const idx p[3] = { x, y, z };
+               int       t;
+               idx       a[3], l4[4];
+               p2Locate( Nc, p, t, a, l4 );
+               idx nd[10][3];
+               p2Nodes( t, a[0], a[1], a[2], nd );
+               const double lam[4] = { l4[0] / 4.0, l4[1] / 4.0, l4[2] / 4.0, l4[3] / 4.0 };
+               double       phi[10];
+               p2Basis( lam, phi );
+               double v = 0.0;
+               for ( int j = 0; j < 10; ++j )
+                  if ( phi[j] != 0.0 )
+                     v += phi[j] * double( src[cidx( wc, nd[j][0], nd[j][1], nd[j][2] )] );
+               d += T( invNumCells( m, (int) ci, classify( Nf, x, y, z ) ) * v );
+            }
+   }
+   commAdditiveCellToLower( fn, dL, flag ^ All, !add );
+}
+
+// P2toP2QuadraticRestriction::restrictAdditively3D (P2toP2QuadraticRestriction.cpp:131-286): the transpose of
+// the prolongation, coarse_j = sum over the fine points p of phi_j(p) fine_p (fine points on macro-faces/-edges/
+// -vertices scaled by 1 / (number of cells sharing them) per cell), then the additive Cell -> lower reduction.
+template < typename T >
+void restrictP2( Function< T >& fn, int srcLat, int flag )
+{
+   const Mesh& m  = *fn.m;
+   const int   dL = srcLat - 1;
+   fn.commDown( srcLat );
+   const idx Nf = idx( 1 ) << srcLat, wf = Nf + 1;
+   const idx Nc = idx( 1 ) << ( dL - 1 ), wc = ( idx( 1 ) << dL ) + 1;
+#pragma omp parallel for schedule( dynamic )
+   for ( idx ci = 0; ci < (idx) m.Cl.size(); ++ci )
+   {
+      const auto&           src = fn.c[srcLat][ci];
+      auto&                 dst = fn.c[dL][ci];
+      std::vector< double > acc( dst.size(), 0.0 );
+      for ( idx z = 0; z <= Nf; ++z )
+         for ( idx y = 0; y + z <= Nf; ++y )
+            for ( idx x = 0; x + y + z <= Nf; ++x )
+            {
+               const idx p[3] = { x, y, z };
+               int       t;
+               idx       a[3], l4[4];
+               p2Locate( Nc, p, t, a, l4 );
+               idx nd[10][3];
+               p2Nodes( t, a[0], a[1], a[2], nd );
+               const double lam[4] = { l4[0] / 4.0, l4[1] / 4.0, l4[2] / 4.0, l4[3] / 4.0 };
+               double       phi[10];
+               p2Basis( lam, phi );
+               const double s = invNumCells( m, (int) ci, classify( Nf, x, y, z ) ) * double( src[cidx( wf, x, y, z )] );
+               for ( int j = 0; j < 10; ++j )
+                  if ( phi[j] != 0.0 )
+                     acc[cidx( wc, nd[j][0], nd[j][1], nd[j][2] )] += phi[j] * s;
+            }
+      for ( size_t i = 0; i < dst.size(); ++i )
+         dst[i] = T( acc[i] );
+   }
+   commAdditiveCellToLower( fn, dL, flag ^ All, true );
+}
+
+template < typename T >
+void P2ConstantLaplace< T >::restrictOp( Function< T >& f, int srcL, int flag ) const
+{
+   restrictP2< T >( f, srcL, flag );
+}
+template < typename T >
+void P2ConstantLaplace< T >::prolongateOp( Function< T >& f, int srcL, int flag, bool add ) const
+{
+   prolongateP2< T >( f, srcL, flag, add );
+}
+
 // ---------------------------------------------------------------- smoothers and solvers
 template < typename T >
 struct Solvers
@@ -1286,11 +1663,11 @@ struct Solvers
          smooth( x, b, L );
       A->apply( x, *tmp, L, gmgFlag, false );
       assign< T >( *tmp, { 1.0, -1.0 }, { &b, tmp.get() }, L, gmgFlag );
-      restrict3D< T >( *tmp, L, gmgFlag );
+      A->restrictOp( *tmp, L, gmgFlag );
       assign< T >( b, { 1.0 }, { tmp.get() }, L - 1, gmgFlag );
       interpolate< T >( x, 0.0, L - 1, All );
       vcycle( x, b, L - 1, invoked );
-      prolongate3D< T >( x, L - 1, gmgFlag, true );
+      A->prolongateOp( x, L - 1, gmgFlag, true );
       const int postS = post + inc * ( invoked - L );
       for ( int i = 0; i < postS; ++i )
          smooth( x, b, L );
@@ -1592,6 +1969,51 @@ void* or_op_new_divkgrad( void* mh, int minL, int maxL, int prec, int kind, doub
    else
       o->d = std::make_unique< P1ElementwiseDivKGrad< double > >( o->m, minL, maxL, kf );
    return o;
+}
+
+// P2ConstantOperator< p2_tet_diffusion > on P2 levels minL..maxL; functions used with it hold lattice levels
+// minL+1..maxL+1 and every level argument of the operator calls below is a lattice level
+void* or_op_new_p2( void* mh, int minL, int maxL, int prec )
+{
+   auto* o = new OrOp;
+   o->prec = prec;
+   o->m    = static_cast< Mesh* >( mh );
+   if ( prec == 4 )
+      o->f = std::make_unique< P2ConstantLaplace< float > >( o->m, minL, maxL );
+   else
+      o->d = std::make_unique< P2ConstantLaplace< double > >( o->m, minL, maxL );
+   return o;
+}
+
+// P2 element matrix of -Laplace on a tetrahedron (12 coordinates) -> 100 values, row major
+void or_p2_element( const double* x, double* out )
+{
+   P3 xc[4];
+   for ( int v = 0; v < 4; ++v )
+      xc[v] = { x[3 * v], x[3 * v + 1], x[3 * v + 2] };
+   double a[10][10];
+   p2LaplaceElement( xc, a );
+   for ( int i = 0; i < 10; ++i )
+      for ( int j = 0; j < 10; ++j )
+         out[10 * i + j] = a[i][j];
+}
+
+void or_p2_restrict( void* fh, int srcLat, int flag )
+{
+   auto* h = static_cast< OrFunc* >( fh );
+   if ( h->prec == 4 )
+      restrictP2( *h->f, srcLat, flag );
+   else
+      restrictP2( *h->d, srcLat, flag );
+}
+
+void or_p2_prolongate( void* fh, int srcLat, int flag, int add )
+{
+   auto* h = static_cast< OrFunc* >( fh );
+   if ( h->prec == 4 )
+      prolongateP2( *h->f, srcLat, flag, add != 0 );
+   else
+      prolongateP2( *h->d, srcLat, flag, add != 0 );
 }
 
 // cell stencil (15 weights, C W E N S NW SE TC TW TS TSE BC BN BE BNW) of cell c at level L

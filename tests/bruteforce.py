@@ -189,3 +189,99 @@ class GMGModel:
         for _ in range(post):
             x = smooth(L, x, b)
         return x
+
+
+# ---------------------------------------------------------------- P2 (quadratic) model
+# local P2 nodes: vertices 0..3, then edges (2,3) (1,3) (1,2) (0,3) (0,2) (0,1)
+P2_EDGES = [(2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1)]
+
+
+def p2_element_exact(X):
+    """P2 stiffness matrices of tetrahedra X (ne, 4, 3) by exact integration of the barycentric monomials:
+    grad phi_i = sum_k lambda_k a_ik + b_i, int lambda_k lambda_l = vol (1 + delta_kl) / 20, int lambda_k = vol / 4."""
+    J = X[:, 1:, :] - X[:, :1, :]
+    Jinv = np.linalg.inv(J)
+    g = np.empty((len(X), 4, 3))
+    g[:, 1:, :] = Jinv.transpose(0, 2, 1)
+    g[:, 0, :] = -g[:, 1:, :].sum(axis=1)
+    vol = np.abs(np.linalg.det(J)) / 6.0
+    ne = len(X)
+    a = np.zeros((ne, 10, 4, 3))
+    b = np.zeros((ne, 10, 3))
+    for v in range(4):
+        a[:, v, v, :] = 4.0 * g[:, v, :]
+        b[:, v, :] = -g[:, v, :]
+    for e, (p, q) in enumerate(P2_EDGES):
+        a[:, 4 + e, p, :] = 4.0 * g[:, q, :]
+        a[:, 4 + e, q, :] = 4.0 * g[:, p, :]
+    mll = (np.ones((4, 4)) + np.eye(4)) / 20.0
+    K = np.einsum("eikc,ejlc,kl->eij", a, a, mll)
+    K += np.einsum("eikc,ejc->eij", a, b) / 4.0 + np.einsum("eic,ejkc->eij", b, a) / 4.0
+    K += np.einsum("eic,ejc->eij", b, b)
+    return vol[:, None, None] * K
+
+
+class P2Model:
+    """P2 of level L on the lattice of level L+1 (global numbering by exact coordinates)."""
+
+    def __init__(self, path, L):
+        self.coarse = RefinedMesh(path, L)
+        self.lat = RefinedMesh(path, L + 1)
+        self.n = self.lat.n
+        self.boundary = self.lat.boundary
+        X = self.coarse.xyz[self.coarse.tets]
+        nodes = np.concatenate([X, np.stack([(X[:, p] + X[:, q]) / 2 for p, q in P2_EDGES], axis=1)], axis=1)
+        self.X = X
+        lookup = {tuple(r): i for i, r in enumerate(self.lat.keys)}
+        k = np.rint(nodes.reshape(-1, 3) * self.lat.keyscale).astype(np.int64)
+        self.nodes = np.array([lookup[tuple(r)] for r in k], dtype=np.int64).reshape(-1, 10)
+
+    def stiffness(self):
+        K = p2_element_exact(self.X)
+        rows = np.repeat(self.nodes, 10, axis=1).reshape(-1)
+        cols = np.tile(self.nodes, (1, 10)).reshape(-1)
+        return sp.csr_matrix((K.reshape(-1), (rows, cols)), shape=(self.n, self.n))
+
+
+def p2_basis(lam):
+    """P2 basis values (npts, 10) at barycentric coordinates lam (npts, 4)."""
+    out = [lam[:, v] * (2 * lam[:, v] - 1) for v in range(4)]
+    out += [4 * lam[:, p] * lam[:, q] for p, q in P2_EDGES]
+    return np.stack(out, axis=1)
+
+
+def prolongation_p2(coarse, fine):
+    """Quadratic interpolation matrix (fine.n x coarse.n) from P2Model coarse (level L) to fine (level L+1)."""
+    pts4 = np.array([(i, j, k, 4 - i - j - k) for i in range(5) for j in range(5 - i) for k in range(5 - i - j)], float)
+    lam = np.stack([pts4[:, 3], pts4[:, 0], pts4[:, 1], pts4[:, 2]], axis=1) / 4.0  # lambda_0..3
+    phi = p2_basis(lam)                                                             # (35, 10)
+    X = coarse.X                                                                    # (ne, 4, 3)
+    P = np.einsum("pk,ekc->epc", lam, X)                                            # points (ne, 35, 3)
+    lookup = {tuple(r): i for i, r in enumerate(fine.lat.keys)}
+    k = np.rint(P.reshape(-1, 3) * fine.lat.keyscale).astype(np.int64)
+    fidx = np.array([lookup[tuple(r)] for r in k], dtype=np.int64).reshape(len(X), 35)
+    rows = np.repeat(fidx[:, :, None], 10, axis=2).reshape(-1)
+    cols = np.repeat(coarse.nodes[:, None, :], 35, axis=1).reshape(-1)
+    vals = np.broadcast_to(phi[None], (len(X), 35, 10)).reshape(-1)
+    keep = vals != 0.0
+    rc = np.stack([rows[keep], cols[keep]], axis=1)
+    uniq, first = np.unique(rc, axis=0, return_index=True)
+    Pm = sp.csr_matrix((vals[keep][first], (uniq[:, 0], uniq[:, 1])), shape=(fine.n, coarse.n))
+    assert np.allclose(np.asarray(Pm.sum(axis=1)).ravel(), 1.0)
+    return Pm
+
+
+class P2GMGModel(GMGModel):
+    """V-cycle for P2 on inner DoFs with global sparse matrices (zero Dirichlet boundary)."""
+
+    def __init__(self, path, minL, maxL):
+        self.minL, self.maxL = minL, maxL
+        self.mesh = {L: P2Model(path, L) for L in range(minL, maxL + 1)}
+        self.inner = {L: np.flatnonzero(~m.boundary) for L, m in self.mesh.items()}
+        self.A, self.P = {}, {}
+        for L, m in self.mesh.items():
+            ii = self.inner[L]
+            self.A[L] = m.stiffness()[ii][:, ii].tocsr()
+        for L in range(minL + 1, maxL + 1):
+            P = prolongation_p2(self.mesh[L - 1], self.mesh[L])
+            self.P[L] = P[self.inner[L]][:, self.inner[L - 1]].tocsr()

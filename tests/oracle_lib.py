@@ -46,6 +46,10 @@ def lib():
             "or_power_iteration": (d, [vp, vp, i, i]),
             "or_gmg": (None, [vp, vp, vp, P(np.int32), P(np.float64), i, P(np.float64)]),
             "or_ir": (None, [vp, vp, vp, vp, P(np.int32), P(np.float64), i, i, P(np.float64)]),
+            "or_op_new_p2": (vp, [vp, i, i, i]),
+            "or_p2_element": (None, [P(np.float64), P(np.float64)]),
+            "or_p2_restrict": (None, [vp, i, i]),
+            "or_p2_prolongate": (None, [vp, i, i, i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)
@@ -125,6 +129,31 @@ class Operator:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.or_op_free(self.h)
+
+
+class P2Operator(Operator):
+    """P2 constant -Laplace on P2 levels minL..maxL. A P2 function of level L is a Function holding lattice
+    level L+1 (vertex DoFs at even lattice points, edge DoFs at the midpoints); the apply / inv_diag /
+    smooth_jac level arguments are lattice levels."""
+
+    def __init__(self, mesh, minL, maxL, prec=8):
+        self.mesh = mesh
+        self.h = lib().or_op_new_p2(mesh.h, minL, maxL, prec)
+
+
+def p2_element(x):
+    """P2 element matrix of -Laplace on the tetrahedron with vertices x (4 x 3)."""
+    out = np.zeros(100)
+    lib().or_p2_element(np.ascontiguousarray(x, dtype=np.float64).ravel(), out)
+    return out.reshape(10, 10)
+
+
+def p2_restrict(f, src_lat, flag=INNER):
+    lib().or_p2_restrict(f.h, src_lat, flag)
+
+
+def p2_prolongate(f, src_lat, flag=INNER, add=False):
+    lib().or_p2_prolongate(f.h, src_lat, flag, int(add))
 
 
 class DivKGradOperator(Operator):
