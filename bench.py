@@ -120,10 +120,12 @@ def cpu_baseline(mesh_name, L, target_s, inner):
 
 def bench_divkgrad(hyg, hip, st, steps, level, seed):
     """Variable-coefficient workload: -div(k grad u) on the 24-tet cube, k = exp(g), g a sum of 32
-    seeded Fourier modes of wavenumber <= 8 (standard deviation 1), k evaluated at the 5 quadrature
-    points of every element in every apply. Apply throughput in FP64 and FP32; V(3,3) Chebyshev
-    cycles to 1e-10 in FP64 and as FP32 cycles inside FP64 defect correction."""
-    modes = hyg.fourier_field(seed, modes=32, kmax=8)
+    seeded Fourier modes of wavenumber <= 8, k evaluated at the 5 quadrature points of every element in
+    every apply. Apply throughput in FP64 and FP32; V(3,3) Chebyshev cycles to 1e-10 in FP64 and as FP32
+    cycles inside FP64 defect correction. g has standard deviation 0.5 (k within about e^-1.5 .. e^1.5):
+    with standard deviation 1 the multigrid of rediscretised coarse operators diverges on this mesh, in
+    the oracle as on the device."""
+    modes = hyg.fourier_field(seed, modes=32, kmax=8, std=0.5)
     mesh = hyg.Mesh("cube_24el.msh")
     L = level
     n = mesh.ndofs(L)
@@ -131,7 +133,7 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
     inner = int((~bnd).sum())
     lib = hyg.load()
     op = hyg.P1DivKGradOperator(mesh, 0, L, 1.0, modes)
-    out = {"workload": f"P1 -div(k grad u), cube_24el.msh level {L}, k = exp(32 Fourier modes, |kappa| <= 8, seed {seed})",
+    out = {"workload": f"P1 -div(k grad u), cube_24el.msh level {L}, k = exp(32 Fourier modes, |kappa| <= 8, std 0.5, seed {seed})",
            "dofs_inner": inner}
     rng = np.random.default_rng(seed)
     for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):

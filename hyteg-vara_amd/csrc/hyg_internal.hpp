@@ -110,7 +110,7 @@ struct ZSeg
    int z0;
    int z1;
 };
-constexpr int kZSegChunk = 64; // planes per z-marching segment
+constexpr int kZSegChunk = 24; // planes per z-marching segment (short segments balance the tail of the launch)
 // cell-interior kernel configurations: rows per wavefront, points per lane
 constexpr int kCellRows[]   = { 1, 2, 4, 1, 2, 4 };
 constexpr int kCellPoints[] = { 1, 1, 1, 2, 2, 2 };
