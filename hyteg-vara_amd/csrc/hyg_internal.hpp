@@ -179,6 +179,38 @@ struct ElemCell
    double mt[6][16];   // element matrix of each micro-cell type for k = 1 (vertex order of the type)
 };
 
+// P2 (quadratic Lagrange) on the refined vertex lattice: a P2 function of level L keeps its vertex DoFs at
+// the even points and its edge DoFs at the odd points of the vertex lattice of level L+1 ("lattice level").
+// Tables shared by the P2 kernels (built on the host once per operator). Local node order of a micro-cell:
+// vertices 0..3, edges (2,3) (1,3) (1,2) (0,3) (0,2) (0,1).
+struct P2Tables
+{
+   // lattice offset of node l of a micro-cell of type t from 2 * anchor (vertices 2 e_v, edges e_a + e_b)
+   int nd[6][10][3];
+   // per parity class px + 2 py + 4 pz of a lattice point: the (type, node) pairs (16 t + l) it can be
+   int nEnt[8];
+   int ent[8][24];
+   // prolongation: per residue (px & 3) + 4 (py & 3) + 16 (pz & 3) of a point of the fine lattice, the
+   // coarse micro-cells containing it (anchor = (p >> 2) - offset bits o) with the basis values there
+   struct Cand
+   {
+      int    t, o;
+      double phi[10];
+   };
+   int  nCand[64];
+   Cand cand[64][6];
+   // restriction: per parity class of a coarse lattice point n, the fine lattice points 2 n + d in the
+   // support of its basis function with the weights phi_n
+   struct Sup
+   {
+      int    d[3];
+      double w;
+   };
+   int nSup[8];
+   Sup sup[8][128];
+};
+void buildP2Tables( P2Tables& t );
+
 // ------------------------------------------------------------------ host objects
 struct Mesh
 {
@@ -264,7 +296,8 @@ struct hyg_mesh_s
 struct hyg_func_s
 {
    hyg_mesh_s*           mesh;
-   int                   minL, maxL, dtype;
+   int                   minL, maxL, dtype; // lattice levels (a P2 function of level L holds lattice L+1)
+   int                   lat = 0;           // 0: P1 function, 1: P2 function
    std::vector< void* >  data; // per level (nullptr below minL)
    size_t                elem() const { return dtype == 8 ? 8 : 4; }
 };
@@ -272,10 +305,13 @@ struct hyg_func_s
 struct hyg_op_s
 {
    hyg_mesh_s* mesh;
-   int         minL, maxL;
+   int         minL, maxL; // lattice levels
+   int         lat = 0;    // 0: P1 operator, 1: P2 operator (its functions hold lattice level L+1)
    // 0: constant-coefficient stencils (P1ConstantLaplaceOperator)
    // 1: elementwise -div(k grad) with k = k0 exp( sum_m a_m cos(2 pi kappa_m . x + phi_m) )
+   // 2: P2 constant-coefficient -Laplace (P2ConstantLaplaceOperator), element by element
    int                   kind = 0;
+   hyg::P2Tables*        p2tab = nullptr; // device copy (kind 2)
    double                k0   = 1.0;
    std::vector< double > modesHost; // nModes x 5: kappa_x, kappa_y, kappa_z, phi, a
    double*               modes  = nullptr;
@@ -304,6 +340,9 @@ struct hyg_op_s
       float*                lowPartF = nullptr;
       double*               dinv    = nullptr;
       float*                dinvF   = nullptr;
+      // P2: element matrices (6 types x 10 x 10 per cell) of the micro-cells of level lattice-1
+      double*               p2el    = nullptr;
+      float*                p2elF   = nullptr;
    };
    std::vector< Level > levels;
    ~hyg_op_s();
