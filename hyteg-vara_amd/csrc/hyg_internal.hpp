@@ -111,9 +111,10 @@ struct ZSeg
    int z1;
 };
 constexpr int kZSegChunk = 24; // planes per z-marching segment (short segments balance the tail of the launch)
-// cell-interior kernel configurations: rows per wavefront, points per lane
+// cell-interior kernel configurations: rows per wavefront, points per lane, planes in flight
 constexpr int kCellRows[]   = { 1, 2, 4, 1, 2, 4 };
-constexpr int kCellPoints[] = { 1, 1, 1, 2, 2, 2 };
+constexpr int kCellPoints[] = { 1, 1, 1, 1, 1, 1 };
+constexpr int kCellDepth[]  = { 2, 2, 2, 4, 4, 4 };
 constexpr int kNumCellCfgs  = 6;
 
 // Lower-dimensional primitive (macro-vertex, -edge or -face); order: vertices, edges, faces.

@@ -108,7 +108,7 @@ __device__ __forceinline__ void stencilRow( const T* w, const T* wu, const T* we
    }
 }
 
-template < typename T, int MODE, int R, int P >
+template < typename T, int MODE, int R, int P, int D >
 __device__ __forceinline__ void cellZ( int bid, const ZSeg* __restrict__ segs, int nSegs, int nC, int L, i64 stride,
                                                     const T* __restrict__ cellSt, const CellFaceSlot* __restrict__ cfs,
                                                     const T* __restrict__ entryW, T* __restrict__ part, int flag,
@@ -240,7 +240,7 @@ __device__ __forceinline__ void cellZ( int bid, const ZSeg* __restrict__ segs, i
       const int d = d0 + k;
       sd[k]       = ( interior && d >= 1 && d <= WO ) ? d : ( 1 << 28 );
    }
-   // A, B, C: planes z-1, z, z+1 (resident); plane z+2 is in flight; E receives plane z+3
+   // A, B, C: planes z-1, z, z+1 (resident); planes z+2 .. z+D are in flight; E receives plane z+D+1
    auto step = [&]( const PlaneRows< T, R, P >& A, const PlaneRows< T, R, P >& Bp, const PlaneRows< T, R, P >& C,
                     PlaneRows< T, R, P >& E ) {
       loadPlane( E );
@@ -280,35 +280,32 @@ __device__ __forceinline__ void cellZ( int bid, const ZSeg* __restrict__ segs, i
       --Wz;
       ++z;
    };
-   PlaneRows< T, R, P > R0, R1, R2, R3, R4;
+   // register ring of NR = D + 3 planes: z-1, z, z+1 resident, D planes in flight. The loop over the
+   // ring slots is unrolled, so every slot is a fixed set of registers (no register copies); every path
+   // through a step issues the same memory operations, so the loads of D planes stay in flight.
+   constexpr int        NR = D + 3;
+   PlaneRows< T, R, P > ring[NR];
 #pragma unroll
-   for ( int k = 0; k < R; ++k )
+   for ( int u = 0; u < NR; ++u )
 #pragma unroll
-      for ( int q = 0; q < P; ++q )
-         R0.e[k][q] = R1.e[k][q] = R2.e[k][q] = R3.e[k][q] = R4.e[k][q] = T( 0 );
-   loadPlane( R0 ); // plane z0-1
-   oz = off;        // plane z0 (z0 < z1 <= plMax, so the front advanced)
-   loadPlane( R1 );
-   loadPlane( R2 );
-   loadPlane( R3 ); // plane z0+2; the front is now at z0+3
-   // five-step register ring: no register copies; every path through a step issues the same
-   // memory operations, so loads of two planes stay in flight at each step
-   step( R0, R1, R2, R4 );
+      for ( int k = 0; k < R; ++k )
+#pragma unroll
+         for ( int q = 0; q < P; ++q )
+            ring[u].e[k][q] = T( 0 );
+   loadPlane( ring[0] ); // plane z0-1
+   oz = off;             // plane z0 (z0 < z1 <= plMax, so the front advanced)
+#pragma unroll
+   for ( int u = 1; u <= D + 1; ++u )
+      loadPlane( ring[u] ); // planes z0 .. z0+D; the front is now at z0+D+1
    while ( z < z1 )
    {
-      step( R1, R2, R3, R0 );
-      if ( z >= z1 )
-         break;
-      step( R2, R3, R4, R1 );
-      if ( z >= z1 )
-         break;
-      step( R3, R4, R0, R2 );
-      if ( z >= z1 )
-         break;
-      step( R4, R0, R1, R3 );
-      if ( z >= z1 )
-         break;
-      step( R0, R1, R2, R4 );
+#pragma unroll
+      for ( int u = 0; u < NR; ++u )
+      {
+         step( ring[u], ring[( u + 1 ) % NR], ring[( u + 2 ) % NR], ring[( u + NR - 1 ) % NR] );
+         if ( z >= z1 )
+            break;
+      }
    }
    // face partial sums of this segment
    const unsigned pb = unsigned( NP * NP * B );
@@ -489,12 +486,12 @@ struct SweepArgs
    int                 gC, gF;
 };
 
-template < typename T, int MODE, int R, int P >
+template < typename T, int MODE, int R, int P, int D >
 __global__ __launch_bounds__( kBlock ) void kSweep( SweepArgs< T > p )
 {
    const int b = int( blockIdx.x );
    if ( b < p.gC )
-      cellZ< T, MODE, R, P >( b, p.segs, p.nSegs, p.nC, p.L, p.stride, p.cellSt, p.cfs, p.entryW, p.part, p.flag, p.x, p.in1,
+      cellZ< T, MODE, R, P, D >( b, p.segs, p.nSegs, p.nC, p.L, p.stride, p.cellSt, p.cfs, p.entryW, p.part, p.flag, p.x, p.in1,
                               p.out0, p.out1, p.a );
    else if ( b < p.gC + p.gF )
       faceFlat< T >( b - p.gC, p.items, p.nItems, p.L, p.stride, p.cfs, p.entryW, p.part, p.flag, p.x );
@@ -563,15 +560,15 @@ static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol,
    const int gL = nLow > 0 ? xcdGrid( ceilDiv( nLow, kBlock / 64 ) ) : 0;
    if ( p.gC + p.gF + gL > 0 )
    {
-      decltype( &kSweep< T, MODE, 1, 1 > ) kf;
+      decltype( &kSweep< T, MODE, 1, 1, 2 > ) kf;
       switch ( cfg )
       {
-      case 0: kf = kSweep< T, MODE, 1, 1 >; break;
-      case 1: kf = kSweep< T, MODE, 2, 1 >; break;
-      case 2: kf = kSweep< T, MODE, 4, 1 >; break;
-      case 3: kf = kSweep< T, MODE, 1, 2 >; break;
-      case 4: kf = kSweep< T, MODE, 2, 2 >; break;
-      default: kf = kSweep< T, MODE, 4, 2 >; break;
+      case 0: kf = kSweep< T, MODE, 1, 1, 2 >; break;
+      case 1: kf = kSweep< T, MODE, 2, 1, 2 >; break;
+      case 2: kf = kSweep< T, MODE, 4, 1, 2 >; break;
+      case 3: kf = kSweep< T, MODE, 1, 1, 4 >; break;
+      case 4: kf = kSweep< T, MODE, 2, 1, 4 >; break;
+      default: kf = kSweep< T, MODE, 4, 1, 4 >; break;
       }
       hipLaunchKernelGGL( kf, dim3( p.gC + p.gF + gL ), dim3( kBlock ), 0, s, p );
    }
