@@ -41,6 +41,7 @@ def _sig(lib):
         "hyg_mesh_num_dofs": (i, [vp, i, C.POINTER(C.c_int64)]),
         "hyg_mesh_dof_coords": (i, [vp, i, vp, vp]),
         "hyg_func_create": (i, [vp, i, i, i, pvp]),
+        "hyg_func_create_p2": (i, [vp, i, i, i, pvp]),
         "hyg_func_destroy": (i, [vp]),
         "hyg_func_upload": (i, [vp, i, P(np.float64), vp]),
         "hyg_func_download": (i, [vp, i, P(np.float64), vp]),
@@ -52,6 +53,7 @@ def _sig(lib):
         "hyg_convert": (i, [vp, vp, i, vp]),
         "hyg_op_create_p1_const": (i, [vp, i, i, pvp]),
         "hyg_op_create_p1_divkgrad": (i, [vp, i, i, d, P(np.float64), i, pvp]),
+        "hyg_op_create_p2_const": (i, [vp, i, i, pvp]),
         "hyg_op_destroy": (i, [vp]),
         "hyg_op_cell_stencil": (i, [vp, i, i, P(np.float64)]),
         "hyg_apply": (i, [vp, vp, vp, i, i, i, vp]),
@@ -154,17 +156,22 @@ class Mesh:
 
 
 class Function:
-    def __init__(self, mesh, minL, maxL, dtype=F64):
+    """P1 function (space="p1") or P2 function (space="p2"); a P2 function of level L exchanges the values
+    of the vertex lattice of level L+1 (mesh.ndofs(L + 1) values, canonical order of mesh.coords(L + 1))."""
+
+    def __init__(self, mesh, minL, maxL, dtype=F64, space="p1"):
         self.mesh, self.minL, self.maxL, self.dtype = mesh, minL, maxL, dtype
+        self.lat = 1 if space == "p2" else 0
         h = _out()
-        check(load().hyg_func_create(mesh.h, minL, maxL, dtype, C.byref(h)))
+        create = load().hyg_func_create_p2 if self.lat else load().hyg_func_create
+        check(create(mesh.h, minL, maxL, dtype, C.byref(h)))
         self.h = h
 
     def upload(self, L, v):
         check(load().hyg_func_upload(self.h, L, np.ascontiguousarray(v, dtype=np.float64), None))
 
     def download(self, L):
-        out = np.zeros(self.mesh.ndofs(L))
+        out = np.zeros(self.mesh.ndofs(L + self.lat))
         check(load().hyg_func_download(self.h, L, out, None))
         return out
 
@@ -258,6 +265,16 @@ class P1DivKGradOperator(P1ConstantOperator):
         nm = 0 if modes is None else len(modes)
         h = _out()
         check(load().hyg_op_create_p1_divkgrad(mesh.h, minL, maxL, float(k0), md, nm, C.byref(h)))
+        self.h = h
+
+
+class P2ConstantOperator(P1ConstantOperator):
+    """P2 -Laplace on P2 levels minL..maxL (functions created with space="p2")."""
+
+    def __init__(self, mesh, minL, maxL):
+        self.mesh = mesh
+        h = _out()
+        check(load().hyg_op_create_p2_const(mesh.h, minL, maxL, C.byref(h)))
         self.h = h
 
 

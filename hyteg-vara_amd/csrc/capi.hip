@@ -12,6 +12,7 @@
 namespace hyg {
 void assembleP1Const( hyg_op_s& op );
 void assembleP1DivKGrad( hyg_op_s& op );
+void assembleP2Const( hyg_op_s& op );
 void primEntriesPos( const LowerEntry& e, int N, int i, int j, int& qx, int& qy, int& qz );
 } // namespace hyg
 
@@ -97,16 +98,31 @@ int checkPair( const hyg_func_s* a, const hyg_func_s* b, int L )
       return fail( HYG_ERR_ARG, "functions live on different meshes" );
    if ( a->dtype != b->dtype )
       return fail( HYG_ERR_ARG, "functions have different value types" );
+   if ( a->lat != b->lat )
+      return fail( HYG_ERR_ARG, "P1 and P2 functions cannot be combined" );
+   return HYG_OK;
+}
+
+// Level arguments of the C ABI are levels of the handle's space; a P2 function or operator of level L
+// works on the vertex lattice of level L+1, which every internal routine takes.
+inline int lat( const hyg_func_s* f, int level ) { return f ? level + f->lat : level; }
+inline int lat( const hyg_op_s* op, int level ) { return op ? level + op->lat : level; }
+
+int checkSpace( const hyg_op_s* op, const hyg_func_s* f )
+{
+   if ( op && f && op->lat != f->lat )
+      return fail( HYG_ERR_ARG, "P1 and P2 operators and functions cannot be combined" );
    return HYG_OK;
 }
 
 i64 levelValues( const hyg_func_s* f, int L ) { return i64( f->mesh->ctx.mesh->nC() ) * cellStride( L ); }
 
-int createFunc( hyg_mesh_s* mesh, int minL, int maxL, int dtype, hyg_func_s** out )
+int createFunc( hyg_mesh_s* mesh, int minL, int maxL, int dtype, hyg_func_s** out, int space = 0 )
 {
    auto* f  = new ( std::nothrow ) hyg_func_s;
    if ( !f )
       return fail( HYG_ERR_ARG, "out of host memory" );
+   f->lat   = space;
    f->mesh  = mesh;
    f->minL  = minL;
    f->maxL  = maxL;
@@ -217,6 +233,8 @@ int stencilImpl( hyg_op_s* op, int L, int dtype, const StencilCall& c, hipStream
    MeshCtx& mc = op->mesh->ctx;
    if ( op->kind == 1 )
       launchElementwise( mc, mc.geom( L ), *op, L, dtype, c, st );
+   else if ( op->kind == 2 )
+      launchP2( mc, mc.geom( L ), *op, L, dtype, c, st );
    else
       launchStencil( mc, mc.geom( L ), op->levels[L], L, dtype, c, st );
    return launched( "stencil" );
@@ -406,14 +424,20 @@ int vcycleImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, int invoked, 
    c.in1  = b->data[L];
    c.out0 = g->tmp->data[L];
    HYG_TRY( stencilImpl( g->op, L, g->dtype, c, st ) );
-   launchRestrict( mc, mc.geom( L - 1 ), L - 1, g->dtype, g->tmp->data[L - 1], g->tmp->data[L], kGmgFlag, st );
+   if ( g->op->lat )
+      launchP2Restrict( mc, mc.geom( L - 1 ), mc.p2Tables(), L - 1, g->dtype, g->tmp->data[L - 1], g->tmp->data[L], kGmgFlag, st );
+   else
+      launchRestrict( mc, mc.geom( L - 1 ), L - 1, g->dtype, g->tmp->data[L - 1], g->tmp->data[L], kGmgFlag, st );
    HYG_TRY( launched( "restrict" ) );
    const double one = 1.0;
    hyg_func_s*  t   = g->tmp;
    HYG_TRY( lincombImpl( b, 1, &one, &t, false, 0.0, L - 1, kGmgFlag, st ) );
    HYG_TRY( lincombImpl( x, 0, nullptr, nullptr, false, 0.0, L - 1, HYG_ALL, st ) );
    HYG_TRY( vcycleImpl( g, x, b, L - 1, invoked, st, deviceCg ) );
-   launchProlongate( mc, mc.geom( L ), L, g->dtype, x->data[L], x->data[L - 1], kGmgFlag, true, st );
+   if ( g->op->lat )
+      launchP2Prolongate( mc, mc.geom( L ), mc.p2Tables(), L, g->dtype, x->data[L], x->data[L - 1], kGmgFlag, true, st );
+   else
+      launchProlongate( mc, mc.geom( L ), L, g->dtype, x->data[L], x->data[L - 1], kGmgFlag, true, st );
    HYG_TRY( launched( "prolongate" ) );
    const int postS = g->post + g->inc * ( invoked - L );
    for ( int i = 0; i < postS; ++i )
@@ -449,6 +473,8 @@ int checkGmgArgs( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L )
       return fail( HYG_ERR_ARG, "solve level must be the solver's max level and functions must cover its levels" );
    if ( x->mesh != g->op->mesh )
       return fail( HYG_ERR_ARG, "functions and solver live on different meshes" );
+   HYG_TRY( checkSpace( g->op, x ) );
+   HYG_TRY( checkSpace( g->op, b ) );
    return HYG_OK;
 }
 
@@ -581,6 +607,18 @@ int hyg_func_create( hyg_mesh mesh, int min_level, int max_level, int dtype, hyg
    return createFunc( mesh, min_level, max_level, dtype, out );
 }
 
+int hyg_func_create_p2( hyg_mesh mesh, int min_level, int max_level, int dtype, hyg_func* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh || !out )
+      return fail( HYG_ERR_ARG, "null argument" );
+   if ( min_level < 0 || max_level < min_level || max_level > 9 )
+      return fail( HYG_ERR_ARG, "bad level range" );
+   if ( dtype != HYG_F64 && dtype != HYG_F32 )
+      return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
+   return createFunc( mesh, min_level + 1, max_level + 1, dtype, out, 1 );
+}
+
 int hyg_func_destroy( hyg_func f )
 {
    destroyFunc( f );
@@ -589,6 +627,7 @@ int hyg_func_destroy( hyg_func f )
 
 int hyg_func_upload( hyg_func f, int level, const double* canonical, void* stream )
 {
+   level = lat( f, level );
    HYG_TRY( checkLevel( f, level ) );
    if ( !canonical )
       return fail( HYG_ERR_ARG, "null data" );
@@ -627,6 +666,7 @@ int hyg_func_upload( hyg_func f, int level, const double* canonical, void* strea
 
 int hyg_func_download( hyg_func f, int level, double* canonical, void* stream )
 {
+   level = lat( f, level );
    HYG_TRY( checkLevel( f, level ) );
    if ( !canonical )
       return fail( HYG_ERR_ARG, "null data" );
@@ -664,12 +704,14 @@ int hyg_func_download( hyg_func f, int level, double* canonical, void* stream )
 
 int hyg_interpolate_const( hyg_func f, double value, int level, int flag, void* stream )
 {
+   level = lat( f, level );
    HYG_TRY( checkLevel( f, level ) );
    return lincombImpl( f, 0, nullptr, nullptr, false, value, level, flag, S( stream ) );
 }
 
 int hyg_assign( hyg_func dst, int n, const double* scalars, const hyg_func* srcs, int level, int flag, void* stream )
 {
+   level = lat( dst, level );
    HYG_TRY( checkLevel( dst, level ) );
    if ( n < 1 || n > 3 || !scalars || !srcs )
       return fail( HYG_ERR_ARG, "assign takes 1 to 3 sources" );
@@ -680,6 +722,7 @@ int hyg_assign( hyg_func dst, int n, const double* scalars, const hyg_func* srcs
 
 int hyg_add( hyg_func dst, int n, const double* scalars, const hyg_func* srcs, int level, int flag, void* stream )
 {
+   level = lat( dst, level );
    HYG_TRY( checkLevel( dst, level ) );
    if ( n < 1 || n > 3 || !scalars || !srcs )
       return fail( HYG_ERR_ARG, "add takes 1 to 3 sources" );
@@ -690,6 +733,7 @@ int hyg_add( hyg_func dst, int n, const double* scalars, const hyg_func* srcs, i
 
 int hyg_mult_elementwise( hyg_func dst, hyg_func a, hyg_func b, int level, int flag, void* stream )
 {
+   level = lat( dst, level );
    HYG_TRY( checkPair( dst, a, level ) );
    HYG_TRY( checkPair( dst, b, level ) );
    MeshCtx& mc = dst->mesh->ctx;
@@ -699,6 +743,7 @@ int hyg_mult_elementwise( hyg_func dst, hyg_func a, hyg_func b, int level, int f
 
 int hyg_dot( hyg_func a, hyg_func b, int level, int flag, double* out, void* stream )
 {
+   level = lat( a, level );
    HYG_TRY( checkPair( a, b, level ) );
    if ( !out )
       return fail( HYG_ERR_ARG, "null output" );
@@ -707,10 +752,13 @@ int hyg_dot( hyg_func a, hyg_func b, int level, int flag, double* out, void* str
 
 int hyg_convert( hyg_func dst, hyg_func src, int level, void* stream )
 {
+   level = lat( dst, level );
    HYG_TRY( checkLevel( dst, level ) );
    HYG_TRY( checkLevel( src, level ) );
    if ( dst->mesh != src->mesh )
       return fail( HYG_ERR_ARG, "functions live on different meshes" );
+   if ( dst->lat != src->lat )
+      return fail( HYG_ERR_ARG, "P1 and P2 functions cannot be combined" );
    launchConvert( levelValues( dst, level ), dst->dtype, dst->data[level], src->dtype, src->data[level], S( stream ) );
    return launched( "convert" );
 }
@@ -788,6 +836,44 @@ int hyg_op_create_p1_divkgrad( hyg_mesh mesh, int min_level, int max_level, doub
    return HYG_OK;
 }
 
+int hyg_op_create_p2_const( hyg_mesh mesh, int min_level, int max_level, hyg_op* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh || !out || min_level < 0 || max_level < min_level || max_level > 9 )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   auto* op = new hyg_op_s;
+   op->mesh = mesh;
+   op->lat  = 1;
+   op->minL = min_level + 1;
+   op->maxL = max_level + 1;
+   op->kind = 2;
+   try
+   {
+      mesh->ctx.p2Tables();
+      for ( int L = op->minL; L <= op->maxL; ++L )
+         mesh->ctx.geom( L );
+      assembleP2Const( *op );
+      for ( int L = op->minL; L <= op->maxL; ++L )
+         launchP2InverseDiagonal( mesh->ctx, mesh->ctx.geom( L ), *op, L, nullptr );
+   } catch ( const std::exception& e )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   if ( int rc = launched( "P2 operator setup" ) )
+   {
+      delete op;
+      return rc;
+   }
+   if ( hipDeviceSynchronize() != hipSuccess )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, "P2 operator setup failed" );
+   }
+   *out = op;
+   return HYG_OK;
+}
+
 int hyg_op_destroy( hyg_op op )
 {
    delete op;
@@ -796,6 +882,7 @@ int hyg_op_destroy( hyg_op op )
 
 int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 )
 {
+   level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    if ( cell < 0 || cell >= op->mesh->ctx.mesh->nC() || !out15 )
       return fail( HYG_ERR_ARG, "bad cell index" );
@@ -807,6 +894,7 @@ int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 )
 
 int hyg_op_tune( hyg_op op, int level, int dtype, int reps, int* chosen, void* stream )
 {
+   level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    if ( op->kind != 0 )
       return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operator only" );
@@ -889,7 +977,10 @@ int hyg_op_tune( hyg_op op, int level, int dtype, int reps, int* chosen, void* s
 
 int hyg_op_set_config( hyg_op op, int level, int dtype, int mode, int config )
 {
+   level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
+   if ( op->kind != 0 )
+      return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operator only" );
    if ( ( dtype != HYG_F64 && dtype != HYG_F32 ) || mode < 0 || mode > 5 || config < 0 || config >= kNumCellCfgs )
       return fail( HYG_ERR_ARG, "bad dtype, mode or configuration" );
    setCellConfig( op->mesh->ctx.geom( level ), dtype, mode, config );
@@ -898,7 +989,10 @@ int hyg_op_set_config( hyg_op op, int level, int dtype, int mode, int config )
 
 int hyg_op_get_config( hyg_op op, int level, int dtype, int mode, int* config, int* rows, int* points )
 {
+   level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
+   if ( op->kind != 0 )
+      return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operator only" );
    if ( ( dtype != HYG_F64 && dtype != HYG_F32 ) || mode < 0 || mode > 5 )
       return fail( HYG_ERR_ARG, "bad dtype or mode" );
    const int c = cellConfig( op->mesh->ctx.geom( level ), dtype, mode );
@@ -913,8 +1007,10 @@ int hyg_op_get_config( hyg_op op, int level, int dtype, int mode, int* config, i
 
 int hyg_apply( hyg_op op, hyg_func src, hyg_func dst, int level, int flag, int update, void* stream )
 {
+   level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    HYG_TRY( checkPair( src, dst, level ) );
+   HYG_TRY( checkSpace( op, src ) );
    if ( src == dst )
       return fail( HYG_ERR_ARG, "apply requires src != dst" );
    if ( src->mesh != op->mesh )
@@ -931,8 +1027,10 @@ int hyg_apply( hyg_op op, hyg_func src, hyg_func dst, int level, int flag, int u
 
 int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double relax, int level, int flag, void* stream )
 {
+   level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    HYG_TRY( checkPair( src, dst, level ) );
+   HYG_TRY( checkSpace( op, src ) );
    HYG_TRY( checkPair( src, rhs, level ) );
    if ( src == dst || rhs == dst )
       return fail( HYG_ERR_ARG, "smooth_jac requires dst different from src and rhs" );
@@ -948,12 +1046,14 @@ int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double 
 
 int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream )
 {
+   level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    HYG_TRY( checkLevel( dst, level ) );
+   HYG_TRY( checkSpace( op, dst ) );
    MeshCtx& mc = op->mesh->ctx;
-   if ( op->kind == 1 )
+   if ( op->kind == 1 || op->kind == 2 )
    {
-      // the elementwise operator keeps its inverse diagonal as a function array
+      // the elementwise and P2 operators keep their inverse diagonal as a function array
       const auto& ol = op->levels[level];
       const i64   n  = levelValues( dst, level );
       if ( dst->dtype == HYG_F64 )
@@ -968,7 +1068,9 @@ int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream )
 
 int hyg_residual( hyg_op op, hyg_func r, hyg_func b, hyg_func x, int level, int flag, void* stream )
 {
+   level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
+   HYG_TRY( checkSpace( op, x ) );
    HYG_TRY( checkPair( r, b, level ) );
    HYG_TRY( checkPair( r, x, level ) );
    if ( r == x )
@@ -984,9 +1086,16 @@ int hyg_residual( hyg_op op, hyg_func r, hyg_func b, hyg_func x, int level, int 
 
 int hyg_restrict( hyg_func f, int source_level, int flag, void* stream )
 {
+   source_level = lat( f, source_level );
    HYG_TRY( checkLevel( f, source_level ) );
    HYG_TRY( checkLevel( f, source_level - 1 ) );
    MeshCtx& mc = f->mesh->ctx;
+   if ( f->lat )
+   {
+      launchP2Restrict( mc, mc.geom( source_level - 1 ), mc.p2Tables(), source_level - 1, f->dtype, f->data[source_level - 1],
+                        f->data[source_level], flag, S( stream ) );
+      return launched( "restrict" );
+   }
    launchRestrict( mc, mc.geom( source_level - 1 ), source_level - 1, f->dtype, f->data[source_level - 1], f->data[source_level],
                    flag, S( stream ) );
    return launched( "restrict" );
@@ -994,9 +1103,16 @@ int hyg_restrict( hyg_func f, int source_level, int flag, void* stream )
 
 int hyg_prolongate( hyg_func f, int source_level, int flag, int update, void* stream )
 {
+   source_level = lat( f, source_level );
    HYG_TRY( checkLevel( f, source_level ) );
    HYG_TRY( checkLevel( f, source_level + 1 ) );
    MeshCtx& mc = f->mesh->ctx;
+   if ( f->lat )
+   {
+      launchP2Prolongate( mc, mc.geom( source_level + 1 ), mc.p2Tables(), source_level + 1, f->dtype, f->data[source_level + 1],
+                          f->data[source_level], flag, update == HYG_ADD, S( stream ) );
+      return launched( "prolongate" );
+   }
    launchProlongate( mc, mc.geom( source_level + 1 ), source_level + 1, f->dtype, f->data[source_level + 1], f->data[source_level],
                      flag, update == HYG_ADD, S( stream ) );
    return launched( "prolongate" );
@@ -1052,17 +1168,21 @@ int hyg_cheb_coefficients( int order, double lower, double upper, double* c )
 
 int hyg_power_iteration( hyg_op op, hyg_func x, hyg_func tmp, int iterations, int level, double* radius, void* stream )
 {
+   // the nested C ABI calls take the level of the operator's space, the internal routines the lattice level
+   const int user = level;
+   level          = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    HYG_TRY( checkPair( x, tmp, level ) );
+   HYG_TRY( checkSpace( op, x ) );
    if ( x == tmp || !radius || iterations < 1 )
       return fail( HYG_ERR_ARG, "bad argument" );
    hipStream_t st = S( stream );
    FuncGuard   dinv;
-   HYG_TRY( createFunc( x->mesh, level, level, x->dtype, &dinv.f ) );
-   HYG_TRY( hyg_inverse_diagonal( op, dinv.f, level, stream ) );
+   HYG_TRY( createFunc( x->mesh, level, level, x->dtype, &dinv.f, x->lat ) );
+   HYG_TRY( hyg_inverse_diagonal( op, dinv.f, user, stream ) );
    auto invDiagApply = [&]( hyg_func_s* s, hyg_func_s* d ) -> int {
-      HYG_TRY( hyg_apply( op, s, d, level, HYG_ALL, HYG_REPLACE, stream ) );
-      return hyg_mult_elementwise( d, dinv.f, d, level, HYG_ALL, stream );
+      HYG_TRY( hyg_apply( op, s, d, user, HYG_ALL, HYG_REPLACE, stream ) );
+      return hyg_mult_elementwise( d, dinv.f, d, user, HYG_ALL, stream );
    };
    double nrm = 0.0;
    HYG_TRY( dotImpl( x, x, level, HYG_ALL, &nrm, st ) );
@@ -1091,7 +1211,7 @@ int hyg_gmg_create( hyg_op op, int dtype, const int* ip, const double* dp, hyg_g
       return fail( HYG_ERR_ARG, "null argument" );
    if ( dtype != HYG_F64 && dtype != HYG_F32 )
       return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
-   if ( ip[0] < op->minL || ip[1] > op->maxL || ip[0] > ip[1] )
+   if ( lat( op, ip[0] ) < op->minL || lat( op, ip[1] ) > op->maxL || ip[0] > ip[1] )
       return fail( HYG_ERR_ARG, "solver levels outside the operator's range" );
    if ( ip[5] != 0 && ip[5] != 1 )
       return fail( HYG_ERR_ARG, "smoother must be 0 (weighted Jacobi) or 1 (Chebyshev)" );
@@ -1100,8 +1220,8 @@ int hyg_gmg_create( hyg_op op, int dtype, const int* ip, const double* dp, hyg_g
    auto* g      = new hyg_gmg_s;
    g->op        = op;
    g->dtype     = dtype;
-   g->minL      = ip[0];
-   g->maxL      = ip[1];
+   g->minL      = lat( op, ip[0] );
+   g->maxL      = lat( op, ip[1] );
    g->pre       = ip[2];
    g->post      = ip[3];
    g->inc       = ip[4];
@@ -1118,7 +1238,7 @@ int hyg_gmg_create( hyg_op op, int dtype, const int* ip, const double* dp, hyg_g
    int         rc = HYG_OK;
    for ( hyg_func_s** f : { &g->tmp, &g->jt, &g->t1, &g->t2, &g->cy, &g->cgp, &g->cgz, &g->cgap, &g->cgr } )
       if ( rc == HYG_OK )
-         rc = createFunc( m, g->minL, g->maxL, dtype, f );
+         rc = createFunc( m, g->minL, g->maxL, dtype, f, op->lat );
    if ( rc != HYG_OK )
    {
       delete g;
@@ -1136,6 +1256,7 @@ int hyg_gmg_destroy( hyg_gmg g )
 
 int hyg_gmg_vcycle( hyg_gmg g, hyg_func x, hyg_func b, int level, void* stream )
 {
+   level = lat( g ? g->op : nullptr, level );
    HYG_TRY( checkGmgArgs( g, x, b, level ) );
    HYG_TRY( vcycleImpl( g, x, b, level, level, S( stream ) ) );
    return HYG_OK;
@@ -1145,6 +1266,9 @@ int hyg_gmg_vcycle_graph( hyg_gmg g, hyg_func x, hyg_func b, int level, int cg_i
 {
    if ( !g )
       return fail( HYG_ERR_ARG, "null solver handle" );
+   level = lat( g->op, level );
+   HYG_TRY( checkSpace( g->op, x ) );
+   HYG_TRY( checkSpace( g->op, b ) );
    HYG_TRY( checkLevel( x, level ) );
    HYG_TRY( checkLevel( b, level ) );
    if ( !stream )
@@ -1199,6 +1323,7 @@ int hyg_gmg_vcycle_graph( hyg_gmg g, hyg_func x, hyg_func b, int level, int cg_i
 
 int hyg_gmg_solve( hyg_gmg g, hyg_func x, hyg_func b, int level, int cycles, double* res, void* stream )
 {
+   level = lat( g ? g->op : nullptr, level );
    HYG_TRY( checkGmgArgs( g, x, b, level ) );
    if ( cycles < 0 || !res )
       return fail( HYG_ERR_ARG, "bad argument" );
@@ -1215,6 +1340,7 @@ int hyg_gmg_solve( hyg_gmg g, hyg_func x, hyg_func b, int level, int cycles, dou
 int hyg_gmg_solve_tol( hyg_gmg g, hyg_func x, hyg_func b, int level, int max_cycles, double rel_tol, double* res, int* cycles,
                        void* stream )
 {
+   level = lat( g ? g->op : nullptr, level );
    HYG_TRY( checkGmgArgs( g, x, b, level ) );
    if ( max_cycles < 0 || !res || !cycles )
       return fail( HYG_ERR_ARG, "bad argument" );
@@ -1233,10 +1359,14 @@ int hyg_gmg_solve_tol( hyg_gmg g, hyg_func x, hyg_func b, int level, int max_cyc
 int hyg_ir_solve( hyg_op op64, hyg_gmg g32, hyg_func x, hyg_func b, int level, int inner, int max_outer, double rel_tol,
                   double* res, int* steps, void* stream )
 {
+   level = lat( op64, level );
    HYG_TRY( checkOpLevel( op64, level ) );
    if ( !g32 || g32->dtype != HYG_F32 || g32->maxL != level )
       return fail( HYG_ERR_ARG, "defect correction needs an FP32 solver whose max level is the solve level" );
    HYG_TRY( checkPair( x, b, level ) );
+   HYG_TRY( checkSpace( op64, x ) );
+   if ( g32->op->lat != op64->lat )
+      return fail( HYG_ERR_ARG, "P1 and P2 operators and solvers cannot be combined" );
    if ( x->dtype != HYG_F64 )
       return fail( HYG_ERR_ARG, "defect correction works on FP64 x and b" );
    if ( !res || !steps || inner < 1 || max_outer < 0 )
@@ -1245,10 +1375,10 @@ int hyg_ir_solve( hyg_op op64, hyg_gmg g32, hyg_func x, hyg_func b, int level, i
    hyg_mesh_s* m  = op64->mesh;
    if ( !g32->x32 )
    {
-      HYG_TRY( createFunc( m, g32->minL, g32->maxL, HYG_F32, &g32->x32 ) );
-      HYG_TRY( createFunc( m, g32->minL, g32->maxL, HYG_F32, &g32->b32 ) );
-      HYG_TRY( createFunc( m, level, level, HYG_F64, &g32->r64 ) );
-      HYG_TRY( createFunc( m, level, level, HYG_F64, &g32->t64 ) );
+      HYG_TRY( createFunc( m, g32->minL, g32->maxL, HYG_F32, &g32->x32, op64->lat ) );
+      HYG_TRY( createFunc( m, g32->minL, g32->maxL, HYG_F32, &g32->b32, op64->lat ) );
+      HYG_TRY( createFunc( m, level, level, HYG_F64, &g32->r64, op64->lat ) );
+      HYG_TRY( createFunc( m, level, level, HYG_F64, &g32->t64, op64->lat ) );
    }
    // residual in FP64 (the FP64 operator shares the FP32 one's mesh)
    hyg_gmg_s dummy{};

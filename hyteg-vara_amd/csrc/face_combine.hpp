@@ -1,5 +1,6 @@
-// Face values from the one-sided partial sums of their cells (shared by the constant-coefficient and
-// the elementwise operators): sum the partial sums, apply the epilogue, write every replica.
+// Kernels and helpers shared by the constant-coefficient, the elementwise and the P2 operators: face and
+// edge/vertex values from the one-sided partial sums of their cells (sum, epilogue, write every replica),
+// the pointwise inverse, and the enumeration of the points of a macro-cell.
 #pragma once
 
 #include "device_common.hpp"
@@ -51,6 +52,79 @@ __global__ __launch_bounds__( kBlock ) void kFaceCombine( const LowerChunk* __re
       if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
          out1[o[k]] = r1;
    }
+}
+
+// largest coordinate sum of a type's vertex offsets: the element anchored at a lies in the cell iff
+// a >= 0 and |a| + kMCMax[t] <= N
+constexpr int kMCMax[6] = { 1, 2, 2, 3, 2, 2 };
+
+// (x, y, z) of the s-th point of a tetrahedral lattice with w points per edge
+__device__ __forceinline__ void tetXYZ( int w, int s, int& x, int& y, int& z )
+{
+   // plane z: S3(w) - S3(w - z) <= s < S3(w) - S3(w - z - 1)
+   int lo = 0, hi = w - 1;
+   const int S = S3i( w );
+   while ( lo < hi )
+   {
+      const int mid = ( lo + hi + 1 ) >> 1;
+      if ( S - S3i( w - mid ) <= s )
+         lo = mid;
+      else
+         hi = mid - 1;
+   }
+   z = lo;
+   planeXY( w - z, s - ( S - S3i( w - z ) ), x, y );
+}
+
+// ------------------------------------------------------------------ edges and vertices: combine
+template < typename T, int MODE >
+__global__ __launch_bounds__( kBlock ) void kLowCombine( const LowerChunk* __restrict__ chunks, int nChunks,
+                                                         const LowerPrim* __restrict__ prims, const LowerEntry* __restrict__ entries,
+                                                         const T* __restrict__ lowPart, const T* __restrict__ dinv, int L,
+                                                         i64 stride, int flag, const T* __restrict__ x, const T* __restrict__ in1,
+                                                         T* __restrict__ out0, T* __restrict__ out1, T a )
+{
+   const int ci = int( blockIdx.x ) * ( kBlock / 64 ) + int( threadIdx.x >> 6 );
+   if ( ci >= nChunks )
+      return;
+   const LowerChunk ch   = chunks[ci];
+   const LowerPrim  p    = prims[ch.prim];
+   const int        lane = int( threadIdx.x & 63 );
+   if ( !( p.bc & flag ) || lane >= ch.n )
+      return;
+   const int N = 1 << L, w = N + 1, NP = N + 1;
+   const int i = ch.i0 + lane;
+   T         Ax = T( 0 );
+   for ( int k = 0; k < p.nCells; ++k )
+      Ax += lowPart[i64( p.entry0 + k ) * NP + i];
+   int qx, qy, qz;
+   entryPos( entries[p.entry0], N, i, 0, qx, qy, qz );
+   const i64 o0 = i64( entries[p.entry0].cell ) * stride + cidx( w, qx, qy, qz );
+   T         xc = T( 0 ), i1 = T( 0 ), r0, r1;
+   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
+      xc = x[o0];
+   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      i1 = in1[o0];
+   if constexpr ( MODE == MODE_APPLY_ADD )
+      i1 = out0[o0];
+   epilogue< T, MODE >( Ax, xc, i1, i1, dinv ? dinv[o0] : T( 1 ), a, r0, r1 );
+   for ( int k = 0; k < p.nCells; ++k )
+   {
+      const LowerEntry e = entries[p.entry0 + k];
+      entryPos( e, N, i, 0, qx, qy, qz );
+      const i64 o = i64( e.cell ) * stride + cidx( w, qx, qy, qz );
+      out0[o]     = r0;
+      if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+         out1[o] = r1;
+   }
+}
+
+// 1 / d of every value (0 stays 0: padding and unused slots)
+template < typename T >
+__global__ void kInvertAll( i64 n, T* __restrict__ d )
+{
+   for ( i64 i = i64( blockIdx.x ) * blockDim.x + threadIdx.x; i < n; i += i64( gridDim.x ) * blockDim.x )
+      d[i] = d[i] != T( 0 ) ? T( 1 ) / d[i] : T( 0 );
 }
 
 } // namespace hyg

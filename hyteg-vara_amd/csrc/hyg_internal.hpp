@@ -282,8 +282,10 @@ struct MeshCtx
    CellInfo*                   cellInfo = nullptr;
    CellFaceSlot*               cellFaces = nullptr;
    int                         device   = 0;
+   P2Tables*                   p2tab    = nullptr; // device copy, built on first use
    void                        buildTopology(); // prims, entries, cell info (host + device)
    LevelGeom&                  geom( int L );
+   const P2Tables*             p2Tables();
    ~MeshCtx();
 };
 
@@ -312,7 +314,6 @@ struct hyg_op_s
    // 1: elementwise -div(k grad) with k = k0 exp( sum_m a_m cos(2 pi kappa_m . x + phi_m) )
    // 2: P2 constant-coefficient -Laplace (P2ConstantLaplaceOperator), element by element
    int                   kind = 0;
-   hyg::P2Tables*        p2tab = nullptr; // device copy (kind 2)
    double                k0   = 1.0;
    std::vector< double > modesHost; // nModes x 5: kappa_x, kappa_y, kappa_z, phi, a
    double*               modes  = nullptr;

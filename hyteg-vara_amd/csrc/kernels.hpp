@@ -36,6 +36,14 @@ void launchStencil( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, i
 void launchElementwise( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c,
                         hipStream_t s );
 void launchElementwiseInverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );
+// P2 operator (kind 2) at lattice level L: the stencil modes, the inverse diagonal, and the quadratic transfers
+// (fine lattice Lf from Lf-1; coarse lattice Lc from Lc+1)
+void launchP2( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c, hipStream_t s );
+void launchP2InverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );
+void launchP2Prolongate( const MeshCtx& m, const LevelGeom& gf, const P2Tables* tb, int Lf, int dtype, void* fine,
+                         const void* coarse, int flag, bool add, hipStream_t st );
+void launchP2Restrict( const MeshCtx& m, const LevelGeom& gc, const P2Tables* tb, int Lc, int dtype, void* coarse, const void* fine,
+                       int flag, hipStream_t st );
 
 // cell-interior kernel configuration (index into kCellRows / kCellPoints) of a level, value type and mode
 int  cellConfig( LevelGeom& g, int dtype, int mode );

@@ -1,0 +1,390 @@
+// gfx950 kernels of the P2 (quadratic Lagrange) path: the constant-coefficient -Laplace operator
+// (P2ConstantOperator< p2_tet_diffusion >, p2functionspace/P2ConstantOperator.cpp:99-113), its inverse
+// diagonal, and the quadratic grid transfers (P2toP2QuadraticProlongation.cpp:217-424,
+// P2toP2QuadraticRestriction.cpp:131-286).
+//
+// A P2 function of level L holds its vertex DoFs and the DoFs at the midpoints of the level-L micro-edges;
+// together they are the vertices of level L+1, so it is stored like a P1 function of "lattice" level L+1
+// (same per-cell arrays, replicas of face/edge/vertex values in every incident cell). Every routine here
+// takes lattice levels; the micro-cells of the P2 elements are those of level lattice-1, whose ten nodes
+// are lattice points (P2Tables::nd).
+//
+// The operator is evaluated element by element per point: a point sums, over the micro-cells of its
+// macro-cell that have it as a node, the row of the cell-type's 10x10 element matrix times the ten node
+// values. Cell-interior points get the mode's epilogue; points on faces, edges and vertices store the
+// cell's one-sided sum for kFaceCombine / kLowCombine, exactly as the P1 elementwise operator does.
+#include "face_combine.hpp"
+
+namespace hyg {
+
+// ------------------------------------------------------------------ operator application
+template < typename T, int MODE, bool DIAG >
+__global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i64 stride, const T* __restrict__ el,
+                                                      const P2Tables* __restrict__ tb, const CellInfo* __restrict__ cinfo,
+                                                      const CellFaceSlot* __restrict__ cfs, const LowerPrim* __restrict__ prims,
+                                                      const LowerEntry* __restrict__ entries, T* __restrict__ part,
+                                                      T* __restrict__ lowPart, const T* __restrict__ dinv, int flag,
+                                                      const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
+                                                      T* __restrict__ out1, T a )
+{
+   const int c    = int( blockIdx.x ) / tilesPerCell;
+   const int tile = int( blockIdx.x ) - c * tilesPerCell;
+   const int N = 1 << L, w = N + 1, NP = N + 1, Nc = N >> 1;
+   const int s = tile * kBlock + int( threadIdx.x );
+   if ( s >= S3i( w ) )
+      return;
+   int px, py, pz;
+   tetXYZ( w, s, px, py, pz );
+   const int slot = classifySlot( N, px, py, pz );
+   if ( slot == 0 && !( flag & 1 ) )
+      return;
+   const int cls = ( px & 1 ) | ( ( py & 1 ) << 1 ) | ( ( pz & 1 ) << 2 );
+   const T*  ec  = el + i64( c ) * 600;
+   const T*  xc  = x + i64( c ) * stride;
+   T         val = T( 0 );
+   const int ne  = tb->nEnt[cls];
+   for ( int k = 0; k < ne; ++k )
+   {
+      const int tl = tb->ent[cls][k];
+      const int t = tl >> 4, l = tl & 15;
+      // anchor of the micro-cell of type t that has this point as node l
+      const int ax = ( px - tb->nd[t][l][0] ) >> 1, ay = ( py - tb->nd[t][l][1] ) >> 1, az = ( pz - tb->nd[t][l][2] ) >> 1;
+      if ( ax < 0 || ay < 0 || az < 0 || ax + ay + az + kMCMax[t] > Nc )
+         continue;
+      const T* row = ec + 100 * t + 10 * l;
+      if constexpr ( DIAG )
+         val += row[l];
+      else
+      {
+#pragma unroll
+         for ( int j = 0; j < 10; ++j )
+            val += row[j] * xc[cidx( w, 2 * ax + tb->nd[t][j][0], 2 * ay + tb->nd[t][j][1], 2 * az + tb->nd[t][j][2] )];
+      }
+   }
+   const i64 q = i64( c ) * stride + cidx( w, px, py, pz );
+   if ( slot == 0 )
+   {
+      if constexpr ( DIAG )
+         out0[q] = val;
+      else
+      {
+         T i1 = T( 0 ), r0, r1;
+         if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+            i1 = in1[q];
+         if constexpr ( MODE == MODE_APPLY_ADD )
+            i1 = out0[q];
+         epilogue< T, MODE >( val, x[q], i1, i1, dinv[q], a, r0, r1 );
+         out0[q] = r0;
+         if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+            out1[q] = r1;
+      }
+   }
+   else if ( slot <= 4 )
+   {
+      const CellFaceSlot fs = cfs[4 * c + slot - 1];
+      if ( fs.bc & flag )
+         part[i64( fs.block ) * NP * NP + facePartIndex( slot - 1, N, px, py, pz )] = val;
+   }
+   else
+   {
+      const CellInfo& ci = cinfo[c];
+      if ( prims[ci.prim[slot]].bc & flag )
+      {
+         const int e = ci.entry[slot];
+         int       i = 0;
+         if ( slot <= 10 )
+         {
+            const int b[4] = { N - px - py - pz, px, py, pz };
+            i              = b[entries[e].l1];
+         }
+         lowPart[i64( e ) * NP + i] = val;
+      }
+   }
+}
+
+// ------------------------------------------------------------------ prolongation
+// Value at point p (fine lattice, N = 2^Lf) of the quadratic interpolant of the coarse function cs
+// (lattice Lf-1, w = Nf/2 + 1) on a micro-cell of level Lf-2 of the macro-cell containing p.
+template < typename T >
+__device__ __forceinline__ T p2Interp( const P2Tables* __restrict__ tb, int Nf, const T* __restrict__ cs, int px, int py, int pz )
+{
+   const int wc = ( Nf >> 1 ) + 1;
+   if ( ( ( px | py | pz ) & 1 ) == 0 )
+      return cs[cidx( wc, px >> 1, py >> 1, pz >> 1 )];
+   const int Ncc = Nf >> 2;
+   const int r   = ( px & 3 ) + 4 * ( py & 3 ) + 16 * ( pz & 3 );
+   const int qx = px >> 2, qy = py >> 2, qz = pz >> 2;
+   const int nc  = tb->nCand[r];
+   for ( int k = 0; k < nc; ++k )
+   {
+      const P2Tables::Cand& cd = tb->cand[r][k];
+      const int             ax = qx - ( cd.o & 1 ), ay = qy - ( ( cd.o >> 1 ) & 1 ), az = qz - ( ( cd.o >> 2 ) & 1 );
+      if ( ax < 0 || ay < 0 || az < 0 || ax + ay + az + kMCMax[cd.t] > Ncc )
+         continue;
+      T v = T( 0 );
+#pragma unroll
+      for ( int j = 0; j < 10; ++j )
+         if ( cd.phi[j] != 0.0 )
+            v += T( cd.phi[j] ) * cs[cidx( wc, 2 * ax + tb->nd[cd.t][j][0], 2 * ay + tb->nd[cd.t][j][1], 2 * az + tb->nd[cd.t][j][2] )];
+      return v;
+   }
+   return T( 0 );
+}
+
+// cell-interior points of the fine lattice of every cell
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kP2ProlongCell( int Lf, int tilesPerCell, i64 strideF, i64 strideC,
+                                                            const P2Tables* __restrict__ tb, int flag, T* __restrict__ fine,
+                                                            const T* __restrict__ coarse, int add )
+{
+   if ( !( flag & 1 ) )
+      return;
+   const int c    = int( blockIdx.x ) / tilesPerCell;
+   const int tile = int( blockIdx.x ) - c * tilesPerCell;
+   const int N = 1 << Lf, w = N + 1;
+   const int s = tile * kBlock + int( threadIdx.x );
+   if ( s >= S3i( w ) )
+      return;
+   int px, py, pz;
+   tetXYZ( w, s, px, py, pz );
+   if ( classifySlot( N, px, py, pz ) != 0 )
+      return;
+   const T   v = p2Interp< T >( tb, N, coarse + i64( c ) * strideC, px, py, pz );
+   const i64 o = i64( c ) * strideF + cidx( w, px, py, pz );
+   fine[o]     = add ? fine[o] + v : v;
+}
+
+// points of macro-faces, -edges and -vertices: interpolated in the first incident cell, written to every replica
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kP2ProlongLower( const LowerChunk* __restrict__ chunks, int nChunks,
+                                                             const LowerPrim* __restrict__ prims,
+                                                             const LowerEntry* __restrict__ entries, const P2Tables* __restrict__ tb,
+                                                             int Lf, i64 strideF, i64 strideC, int flag, T* __restrict__ fine,
+                                                             const T* __restrict__ coarse, int add )
+{
+   const int ci = int( blockIdx.x ) * ( kBlock / 64 ) + int( threadIdx.x >> 6 );
+   if ( ci >= nChunks )
+      return;
+   const int        lane = int( threadIdx.x & 63 );
+   const LowerChunk ch   = chunks[ci];
+   const LowerPrim  p    = prims[ch.prim];
+   if ( !( p.bc & flag ) || lane >= ch.n )
+      return;
+   const int N = 1 << Lf, w = N + 1;
+   const int i = ch.i0 + lane;
+   int       qx, qy, qz;
+   const LowerEntry e0 = entries[p.entry0];
+   entryPos( e0, N, i, ch.j, qx, qy, qz );
+   T v = p2Interp< T >( tb, N, coarse + i64( e0.cell ) * strideC, qx, qy, qz );
+   if ( add )
+      v += fine[i64( e0.cell ) * strideF + cidx( w, qx, qy, qz )];
+   for ( int k = 0; k < p.nCells; ++k )
+   {
+      const LowerEntry e = entries[p.entry0 + k];
+      entryPos( e, N, i, ch.j, qx, qy, qz );
+      fine[i64( e.cell ) * strideF + cidx( w, qx, qy, qz )] = v;
+   }
+}
+
+// ------------------------------------------------------------------ restriction (transpose of the prolongation)
+// cell-interior coarse points: the weighted sum over the support of their basis function, which lies in the cell
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kP2RestrictCell( int Lc, int tilesPerCell, i64 strideC, i64 strideF,
+                                                             const P2Tables* __restrict__ tb, T* __restrict__ coarse,
+                                                             const T* __restrict__ fine )
+{
+   const int c    = int( blockIdx.x ) / tilesPerCell;
+   const int tile = int( blockIdx.x ) - c * tilesPerCell;
+   const int Nc = 1 << Lc, wc = Nc + 1, wf = 2 * Nc + 1;
+   const int s = tile * kBlock + int( threadIdx.x );
+   if ( s >= S3i( wc ) )
+      return;
+   int nx, ny, nz;
+   tetXYZ( wc, s, nx, ny, nz );
+   if ( classifySlot( Nc, nx, ny, nz ) != 0 )
+      return;
+   const int cls = ( nx & 1 ) | ( ( ny & 1 ) << 1 ) | ( ( nz & 1 ) << 2 );
+   const T*  fs  = fine + i64( c ) * strideF;
+   T         v   = T( 0 );
+   const int ns  = tb->nSup[cls];
+   for ( int k = 0; k < ns; ++k )
+   {
+      const P2Tables::Sup& sp = tb->sup[cls][k];
+      v += T( sp.w ) * fs[cidx( wf, 2 * nx + sp.d[0], 2 * ny + sp.d[1], 2 * nz + sp.d[2] )];
+   }
+   coarse[i64( c ) * strideC + cidx( wc, nx, ny, nz )] = v;
+}
+
+// coarse points of macro-faces, -edges and -vertices: sum over the incident cells of the in-cell support,
+// fine points on macro-primitives scaled by 1 / (#cells sharing them); written to every replica
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kP2RestrictLower( const LowerChunk* __restrict__ chunks, int nChunks,
+                                                              const LowerPrim* __restrict__ prims,
+                                                              const LowerEntry* __restrict__ entries, const CellInfo* __restrict__ cinfo,
+                                                              const P2Tables* __restrict__ tb, int Lc, i64 strideC, i64 strideF,
+                                                              int flag, T* __restrict__ coarse, const T* __restrict__ fine )
+{
+   const int ci = int( blockIdx.x ) * ( kBlock / 64 ) + int( threadIdx.x >> 6 );
+   if ( ci >= nChunks )
+      return;
+   const int        lane = int( threadIdx.x & 63 );
+   const LowerChunk ch   = chunks[ci];
+   const LowerPrim  p    = prims[ch.prim];
+   if ( !( p.bc & flag ) || lane >= ch.n )
+      return;
+   const int Nc = 1 << Lc, wc = Nc + 1, Nf = 2 * Nc, wf = Nf + 1;
+   const int i = ch.i0 + lane;
+   T         v = T( 0 );
+   for ( int k = 0; k < p.nCells; ++k )
+   {
+      const LowerEntry e = entries[p.entry0 + k];
+      int              nx, ny, nz;
+      entryPos( e, Nc, i, ch.j, nx, ny, nz );
+      const int       cls = ( nx & 1 ) | ( ( ny & 1 ) << 1 ) | ( ( nz & 1 ) << 2 );
+      const T*        fs  = fine + i64( e.cell ) * strideF;
+      const CellInfo& cf  = cinfo[e.cell];
+      const int       ns  = tb->nSup[cls];
+      for ( int m = 0; m < ns; ++m )
+      {
+         const P2Tables::Sup& sp = tb->sup[cls][m];
+         const int            fx = 2 * nx + sp.d[0], fy = 2 * ny + sp.d[1], fz = 2 * nz + sp.d[2];
+         if ( !inCell( Nf, fx, fy, fz ) )
+            continue;
+         v += T( sp.w * cf.invNC[classifySlot( Nf, fx, fy, fz )] ) * fs[cidx( wf, fx, fy, fz )];
+      }
+   }
+   for ( int k = 0; k < p.nCells; ++k )
+   {
+      const LowerEntry e = entries[p.entry0 + k];
+      int              nx, ny, nz;
+      entryPos( e, Nc, i, ch.j, nx, ny, nz );
+      coarse[i64( e.cell ) * strideC + cidx( wc, nx, ny, nz )] = v;
+   }
+}
+
+// ------------------------------------------------------------------ launch
+template < typename T, int MODE, bool DIAG >
+static void p2T( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s )
+{
+   constexpr bool         dbl  = std::is_same< T, double >::value;
+   const hyg_op_s::Level& ol   = op.levels[L];
+   const int              nC   = m.mesh->nC();
+   const int              w    = ( 1 << L ) + 1;
+   const T*               el   = dbl ? (const T*) ol.p2el : (const T*) ol.p2elF;
+   T*                     part = dbl ? (T*) ol.facePart : (T*) ol.facePartF;
+   T*                     low  = dbl ? (T*) ol.lowPart : (T*) ol.lowPartF;
+   const T*               dv   = DIAG ? nullptr : ( dbl ? (const T*) ol.dinv : (const T*) ol.dinvF );
+   const int              tA   = ceilDiv( S3i( w ), kBlock );
+   hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, el, m.p2tab,
+                       m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x, (const T*) c.in1,
+                       (T*) c.out0, (T*) c.out1, T( c.a ) );
+   const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
+   if ( nFace > 0 )
+      hipLaunchKernelGGL( ( kFaceCombine< T, MODE > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
+                          g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, m.mesh->nV() + m.mesh->nE(), part,
+                          (const T*) nullptr, dv, L, g.stride, c.flag, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
+                          (T*) c.out1, T( c.a ) );
+   if ( nLow > 0 )
+      hipLaunchKernelGGL( ( kLowCombine< T, MODE > ), dim3( ceilDiv( nLow, kBlock / 64 ) ), dim3( kBlock ), 0, s, g.chunks, nLow,
+                          m.prims, m.entries, low, dv, L, g.stride, c.flag, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
+                          (T*) c.out1, T( c.a ) );
+}
+
+template < typename T >
+static void p2Mode( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s )
+{
+   switch ( c.mode )
+   {
+   case MODE_APPLY:
+      p2T< T, MODE_APPLY, false >( m, g, op, L, c, s );
+      break;
+   case MODE_APPLY_ADD:
+      p2T< T, MODE_APPLY_ADD, false >( m, g, op, L, c, s );
+      break;
+   case MODE_RESIDUAL:
+      p2T< T, MODE_RESIDUAL, false >( m, g, op, L, c, s );
+      break;
+   case MODE_JACOBI:
+      p2T< T, MODE_JACOBI, false >( m, g, op, L, c, s );
+      break;
+   case MODE_CHEB0:
+      p2T< T, MODE_CHEB0, false >( m, g, op, L, c, s );
+      break;
+   default:
+      p2T< T, MODE_CHEBK, false >( m, g, op, L, c, s );
+      break;
+   }
+}
+
+void launchP2( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c, hipStream_t s )
+{
+   if ( dtype == 8 )
+      p2Mode< double >( m, g, op, L, c, s );
+   else
+      p2Mode< float >( m, g, op, L, c, s );
+}
+
+// inverse diagonal of a P2 operator at lattice level L into its per-point arrays (FP64 and FP32 copy)
+void launchP2InverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s )
+{
+   const hyg_op_s::Level& ol = op.levels[L];
+   const i64              n  = i64( m.mesh->nC() ) * g.stride + kTailSlack;
+   (void) hipMemsetAsync( ol.dinv, 0, size_t( n ) * sizeof( double ), s );
+   StencilCall c;
+   c.mode = MODE_APPLY;
+   c.flag = 15;
+   c.out0 = ol.dinv;
+   p2T< double, MODE_APPLY, true >( m, g, op, L, c, s );
+   hipLaunchKernelGGL( ( kInvertAll< double > ), dim3( 1024 ), dim3( kBlock ), 0, s, n, ol.dinv );
+   launchConvert( n, 4, ol.dinvF, 8, ol.dinv, s );
+}
+
+template < typename T >
+static void p2ProlongT( const MeshCtx& m, const LevelGeom& gf, const P2Tables* tb, int Lf, void* fine, const void* coarse, int flag,
+                        bool add, hipStream_t st )
+{
+   const int nC      = m.mesh->nC();
+   const int tA      = ceilDiv( S3i( ( 1 << Lf ) + 1 ), kBlock );
+   const i64 strideC = cellStride( Lf - 1 );
+   hipLaunchKernelGGL( ( kP2ProlongCell< T > ), dim3( nC * tA ), dim3( kBlock ), 0, st, Lf, tA, gf.stride, strideC, tb, flag, (T*) fine,
+                       (const T*) coarse, add ? 1 : 0 );
+   if ( gf.nChunks > 0 )
+      hipLaunchKernelGGL( ( kP2ProlongLower< T > ), dim3( ceilDiv( gf.nChunks, kBlock / 64 ) ), dim3( kBlock ), 0, st, gf.chunks,
+                          gf.nChunks, m.prims, m.entries, tb, Lf, gf.stride, strideC, flag, (T*) fine, (const T*) coarse,
+                          add ? 1 : 0 );
+}
+
+void launchP2Prolongate( const MeshCtx& m, const LevelGeom& gf, const P2Tables* tb, int Lf, int dtype, void* fine,
+                         const void* coarse, int flag, bool add, hipStream_t st )
+{
+   if ( dtype == 8 )
+      p2ProlongT< double >( m, gf, tb, Lf, fine, coarse, flag, add, st );
+   else
+      p2ProlongT< float >( m, gf, tb, Lf, fine, coarse, flag, add, st );
+}
+
+template < typename T >
+static void p2RestrictT( const MeshCtx& m, const LevelGeom& gc, const P2Tables* tb, int Lc, void* coarse, const void* fine, int flag,
+                         hipStream_t st )
+{
+   const int nC      = m.mesh->nC();
+   const int tA      = ceilDiv( S3i( ( 1 << Lc ) + 1 ), kBlock );
+   const i64 strideF = cellStride( Lc + 1 );
+   hipLaunchKernelGGL( ( kP2RestrictCell< T > ), dim3( nC * tA ), dim3( kBlock ), 0, st, Lc, tA, gc.stride, strideF, tb, (T*) coarse,
+                       (const T*) fine );
+   if ( gc.nChunks > 0 )
+      hipLaunchKernelGGL( ( kP2RestrictLower< T > ), dim3( ceilDiv( gc.nChunks, kBlock / 64 ) ), dim3( kBlock ), 0, st, gc.chunks,
+                          gc.nChunks, m.prims, m.entries, m.cellInfo, tb, Lc, gc.stride, strideF, flag, (T*) coarse,
+                          (const T*) fine );
+}
+
+void launchP2Restrict( const MeshCtx& m, const LevelGeom& gc, const P2Tables* tb, int Lc, int dtype, void* coarse, const void* fine,
+                       int flag, hipStream_t st )
+{
+   if ( dtype == 8 )
+      p2RestrictT< double >( m, gc, tb, Lc, coarse, fine, flag, st );
+   else
+      p2RestrictT< float >( m, gc, tb, Lc, coarse, fine, flag, st );
+}
+
+} // namespace hyg

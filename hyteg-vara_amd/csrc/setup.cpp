@@ -354,6 +354,7 @@ MeshCtx::~MeshCtx()
       (void) hipFree( g.chunks );
    }
    (void) hipFree( prims );
+   (void) hipFree( p2tab );
    (void) hipFree( cellFaces );
    (void) hipFree( entries );
    (void) hipFree( cellInfo );
@@ -502,6 +503,247 @@ void assembleP1DivKGrad( hyg_op_s& op )
    }
 }
 
+// ------------------------------------------------------------------ P2 (quadratic) path
+namespace {
+// local P2 nodes: vertices 0..3, then the edges (2,3) (1,3) (1,2) (0,3) (0,2) (0,1)
+const int kP2Edge[6][2] = { { 2, 3 }, { 1, 3 }, { 1, 2 }, { 0, 3 }, { 0, 2 }, { 0, 1 } };
+
+void p2Basis( const double lam[4], double phi[10] )
+{
+   for ( int v = 0; v < 4; ++v )
+      phi[v] = lam[v] * ( 2.0 * lam[v] - 1.0 );
+   for ( int e = 0; e < 6; ++e )
+      phi[4 + e] = 4.0 * lam[kP2Edge[e][0]] * lam[kP2Edge[e][1]];
+}
+
+// 4 x barycentric coordinates of point r (units of a quarter micro-edge) in the micro-cell of type t whose
+// anchor is at -o; false if outside. The vertex offsets of a type span a unimodular lattice basis.
+bool quarterBary( int t, const int o[3], const int r[3], int l4[4] )
+{
+   int v[4][3];
+   for ( int k = 0; k < 4; ++k )
+      for ( int c = 0; c < 3; ++c )
+         v[k][c] = kMicroTet[t][k][c] - o[c];
+   long J[3][3];
+   for ( int a = 0; a < 3; ++a )
+      for ( int c = 0; c < 3; ++c )
+         J[a][c] = v[c + 1][a] - v[0][a];
+   const long det = J[0][0] * ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) - J[0][1] * ( J[1][0] * J[2][2] - J[1][2] * J[2][0] ) +
+                    J[0][2] * ( J[1][0] * J[2][1] - J[1][1] * J[2][0] );
+   const long A[3][3] = { { J[1][1] * J[2][2] - J[1][2] * J[2][1], J[0][2] * J[2][1] - J[0][1] * J[2][2], J[0][1] * J[1][2] - J[0][2] * J[1][1] },
+                          { J[1][2] * J[2][0] - J[1][0] * J[2][2], J[0][0] * J[2][2] - J[0][2] * J[2][0], J[0][2] * J[1][0] - J[0][0] * J[1][2] },
+                          { J[1][0] * J[2][1] - J[1][1] * J[2][0], J[0][1] * J[2][0] - J[0][0] * J[2][1], J[0][0] * J[1][1] - J[0][1] * J[1][0] } };
+   const long d[3] = { r[0] - 4 * v[0][0], r[1] - 4 * v[0][1], r[2] - 4 * v[0][2] };
+   l4[0]           = 4;
+   for ( int k = 0; k < 3; ++k )
+   {
+      const long num = A[k][0] * d[0] + A[k][1] * d[1] + A[k][2] * d[2];
+      l4[k + 1]      = int( num / det );
+      l4[0] -= l4[k + 1];
+   }
+   for ( int k = 0; k < 4; ++k )
+      if ( l4[k] < 0 )
+         return false;
+   return true;
+}
+} // namespace
+
+void buildP2Tables( P2Tables& tb )
+{
+   tb = P2Tables{};
+   for ( int t = 0; t < 6; ++t )
+   {
+      for ( int v = 0; v < 4; ++v )
+         for ( int c = 0; c < 3; ++c )
+            tb.nd[t][v][c] = 2 * kMicroTet[t][v][c];
+      for ( int e = 0; e < 6; ++e )
+         for ( int c = 0; c < 3; ++c )
+            tb.nd[t][4 + e][c] = kMicroTet[t][kP2Edge[e][0]][c] + kMicroTet[t][kP2Edge[e][1]][c];
+      for ( int l = 0; l < 10; ++l )
+      {
+         const int cls = ( tb.nd[t][l][0] & 1 ) | ( ( tb.nd[t][l][1] & 1 ) << 1 ) | ( ( tb.nd[t][l][2] & 1 ) << 2 );
+         tb.ent[cls][tb.nEnt[cls]++] = 16 * t + l;
+      }
+   }
+   // prolongation: micro-cells containing each residue of the fine lattice modulo 4
+   for ( int rz = 0; rz < 4; ++rz )
+      for ( int ry = 0; ry < 4; ++ry )
+         for ( int rx = 0; rx < 4; ++rx )
+         {
+            const int r[3] = { rx, ry, rz }, ri = rx + 4 * ry + 16 * rz;
+            if ( ( ( rx | ry | rz ) & 1 ) == 0 )
+               continue; // points of the coarse lattice take the coarse value itself
+            for ( int t = 0; t < 6; ++t )
+               for ( int o = 0; o < 8; ++o )
+               {
+                  const int oo[3] = { o & 1, ( o >> 1 ) & 1, ( o >> 2 ) & 1 };
+                  int       l4[4];
+                  if ( !quarterBary( t, oo, r, l4 ) )
+                     continue;
+                  if ( tb.nCand[ri] == 6 )
+                     throw std::logic_error( "more than six micro-cells contain a lattice point" );
+                  P2Tables::Cand& cd = tb.cand[ri][tb.nCand[ri]++];
+                  cd.t               = t;
+                  cd.o               = o;
+                  const double lam[4] = { l4[0] / 4.0, l4[1] / 4.0, l4[2] / 4.0, l4[3] / 4.0 };
+                  p2Basis( lam, cd.phi );
+               }
+         }
+   // restriction: support of the basis function of a coarse point of each parity class (point n = 8 + cls,
+   // away from any boundary), on the fine lattice: micro-cell (t, a) with n as node l, its 35 quarter points
+   for ( int cls = 0; cls < 8; ++cls )
+   {
+      const int n[3] = { 8 + ( cls & 1 ), 8 + ( ( cls >> 1 ) & 1 ), 8 + ( ( cls >> 2 ) & 1 ) };
+      for ( int k = 0; k < tb.nEnt[cls]; ++k )
+      {
+         const int t = tb.ent[cls][k] >> 4, l = tb.ent[cls][k] & 15;
+         int       a[3];
+         for ( int c = 0; c < 3; ++c )
+            a[c] = ( n[c] - tb.nd[t][l][c] ) / 2;
+         for ( int i = 0; i <= 4; ++i )
+            for ( int j = 0; i + j <= 4; ++j )
+               for ( int q = 0; i + j + q <= 4; ++q )
+               {
+                  const int    l4[4]  = { 4 - i - j - q, i, j, q };
+                  const double lam[4] = { l4[0] / 4.0, l4[1] / 4.0, l4[2] / 4.0, l4[3] / 4.0 };
+                  double       phi[10];
+                  p2Basis( lam, phi );
+                  if ( phi[l] == 0.0 )
+                     continue;
+                  int d[3];
+                  for ( int c = 0; c < 3; ++c )
+                  {
+                     int p = 4 * a[c];
+                     for ( int v = 0; v < 4; ++v )
+                        p += l4[v] * kMicroTet[t][v][c];
+                     d[c] = p - 2 * n[c];
+                  }
+                  int m = 0;
+                  while ( m < tb.nSup[cls] && !( tb.sup[cls][m].d[0] == d[0] && tb.sup[cls][m].d[1] == d[1] && tb.sup[cls][m].d[2] == d[2] ) )
+                     ++m;
+                  if ( m < tb.nSup[cls] )
+                  {
+                     if ( tb.sup[cls][m].w != phi[l] )
+                        throw std::logic_error( "inconsistent P2 interpolation weight" );
+                     continue;
+                  }
+                  if ( tb.nSup[cls] == 128 )
+                     throw std::logic_error( "P2 restriction support exceeds its table" );
+                  P2Tables::Sup& sp = tb.sup[cls][tb.nSup[cls]++];
+                  sp.d[0] = d[0];
+                  sp.d[1] = d[1];
+                  sp.d[2] = d[2];
+                  sp.w    = phi[l];
+               }
+      }
+   }
+}
+
+const P2Tables* MeshCtx::p2Tables()
+{
+   if ( !p2tab )
+   {
+      std::vector< P2Tables > tb( 1 );
+      buildP2Tables( tb[0] );
+      p2tab = upload( tb );
+   }
+   return p2tab;
+}
+
+// P2ConstantOperator< p2_tet_diffusion >: per lattice level the element matrices of the six micro-cell types
+// of every cell (micro-cells of level lattice-1), with the exact integrals of the products of the P2 basis
+// gradients (grad phi_i = sum_k lambda_k a_ik + b_i, int lambda_k lambda_l = vol (1 + delta_kl) / 20,
+// int lambda_k = vol / 4), the partial-sum buffers and the inverse diagonal (computed on the device).
+void assembleP2Const( hyg_op_s& op )
+{
+   const MeshCtx& mc = op.mesh->ctx;
+   const Mesh&    m  = *mc.mesh;
+   op.levels.resize( op.maxL + 1 );
+   auto alloc = []( size_t bytes ) {
+      void* d = nullptr;
+      if ( bytes > 0 && hipMalloc( &d, bytes ) != hipSuccess )
+         throw std::runtime_error( "hipMalloc failed for the P2 operator" );
+      return d;
+   };
+   for ( int L = op.minL; L <= op.maxL; ++L )
+   {
+      hyg_op_s::Level& ol = op.levels[L];
+      const int        N  = 1 << L;
+      const double     Ne = double( N / 2 ); // micro-cells of the P2 elements: level L-1
+      std::vector< double > el( size_t( m.nC() ) * 600 );
+      for ( int c = 0; c < m.nC(); ++c )
+      {
+         const auto& cv = m.Cv[c];
+         const V3&   P0 = m.X[cv[0]];
+         const V3    a1 = sub( m.X[cv[1]], P0 ), a2 = sub( m.X[cv[2]], P0 ), a3 = sub( m.X[cv[3]], P0 );
+         for ( int t = 0; t < 6; ++t )
+         {
+            V3 P[4];
+            for ( int k = 0; k < 4; ++k )
+               for ( int r = 0; r < 3; ++r )
+                  P[k][r] = ( kMicroTet[t][k][0] * a1[r] + kMicroTet[t][k][1] * a2[r] + kMicroTet[t][k][2] * a3[r] ) / Ne;
+            const double vol = std::fabs( dot3( sub( P[1], P[0] ), cross( sub( P[2], P[0] ), sub( P[3], P[0] ) ) ) ) / 6.0;
+            V3           g[4];
+            for ( int i = 0; i < 4; ++i )
+            {
+               const int    a = ( i + 1 ) % 4, b = ( i + 2 ) % 4, d = ( i + 3 ) % 4;
+               const V3     nrm = cross( sub( P[b], P[a] ), sub( P[d], P[a] ) );
+               const double sc  = dot3( nrm, sub( P[i], P[a] ) );
+               for ( int r = 0; r < 3; ++r )
+                  g[i][r] = nrm[r] / sc;
+            }
+            // grad phi_i = sum_k lambda_k A[i][k] + B[i]
+            V3 A[10][4], B[10];
+            for ( int i = 0; i < 10; ++i )
+            {
+               B[i] = { 0, 0, 0 };
+               for ( int k = 0; k < 4; ++k )
+                  A[i][k] = { 0, 0, 0 };
+            }
+            for ( int v = 0; v < 4; ++v )
+               for ( int r = 0; r < 3; ++r )
+               {
+                  A[v][v][r] = 4.0 * g[v][r];
+                  B[v][r]    = -g[v][r];
+               }
+            for ( int e = 0; e < 6; ++e )
+            {
+               const int pa = kP2Edge[e][0], pb = kP2Edge[e][1];
+               for ( int r = 0; r < 3; ++r )
+               {
+                  A[4 + e][pa][r] = 4.0 * g[pb][r];
+                  A[4 + e][pb][r] = 4.0 * g[pa][r];
+               }
+            }
+            for ( int i = 0; i < 10; ++i )
+               for ( int j = 0; j < 10; ++j )
+               {
+                  double v = dot3( B[i], B[j] );
+                  for ( int k = 0; k < 4; ++k )
+                  {
+                     v += 0.25 * ( dot3( A[i][k], B[j] ) + dot3( B[i], A[j][k] ) );
+                     for ( int l = 0; l < 4; ++l )
+                        v += dot3( A[i][k], A[j][l] ) * ( k == l ? 2.0 : 1.0 ) / 20.0;
+                  }
+                  el[size_t( c ) * 600 + 100 * t + 10 * i + j] = vol * v;
+               }
+         }
+      }
+      std::vector< float > elF( el.begin(), el.end() );
+      ol.p2el              = upload( el );
+      ol.p2elF             = upload( elF );
+      const size_t nLow    = mc.entriesHost.size() * size_t( N + 1 );
+      const size_t nFace   = size_t( m.nF() ) * 2 * size_t( N + 1 ) * size_t( N + 1 );
+      const size_t nFunc   = size_t( m.nC() ) * size_t( cellStride( L ) ) + size_t( kTailSlack );
+      ol.lowPart   = static_cast< double* >( alloc( nLow * sizeof( double ) ) );
+      ol.lowPartF  = static_cast< float* >( alloc( nLow * sizeof( float ) ) );
+      ol.facePart  = static_cast< double* >( alloc( nFace * sizeof( double ) ) );
+      ol.facePartF = static_cast< float* >( alloc( nFace * sizeof( float ) ) );
+      ol.dinv      = static_cast< double* >( alloc( nFunc * sizeof( double ) ) );
+      ol.dinvF     = static_cast< float* >( alloc( nFunc * sizeof( float ) ) );
+   }
+}
+
 } // namespace hyg
 
 hyg_op_s::~hyg_op_s()
@@ -523,6 +765,8 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.lowPartF );
       (void) hipFree( l.dinv );
       (void) hipFree( l.dinvF );
+      (void) hipFree( l.p2el );
+      (void) hipFree( l.p2elF );
    }
    (void) hipFree( modes );
 }

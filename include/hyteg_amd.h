@@ -73,6 +73,13 @@ int hyg_mesh_dof_coords( hyg_mesh mesh, int level, double* xyz, uint8_t* boundar
  * Replaces P1Function<ValueType>(name, storage, minLevel, maxLevel)
  * (p1functionspace/VertexDoFFunction.cpp:84-169; memory/FunctionMemory.hpp:46-240). */
 int hyg_func_create( hyg_mesh mesh, int min_level, int max_level, int dtype, hyg_func* out );
+/* Replaces P2Function<ValueType>(name, storage, minLevel, maxLevel) (p2functionspace/P2Function.hpp;
+ * its VertexDoFFunction and EdgeDoFFunction, edgedofspace/EdgeDoFIndexing.hpp:919-1000). The vertex DoFs of
+ * level L and the DoFs at the midpoints of the level-L micro-edges are exactly the vertices of level L+1,
+ * so a P2 function of level L exchanges its values in the canonical order of level L+1
+ * (hyg_mesh_num_dofs / hyg_mesh_dof_coords with level + 1). Every call below takes P2 levels for a P2
+ * function; P1 and P2 handles cannot be mixed in one call. max_level <= 9. */
+int hyg_func_create_p2( hyg_mesh mesh, int min_level, int max_level, int dtype, hyg_func* out );
 int hyg_func_destroy( hyg_func f );
 /* Host <-> device copy of one level in canonical order (double on the host side).
  * Upload also writes every copy of a value that is shared between macro-cells. */
@@ -109,6 +116,13 @@ int hyg_op_destroy( hyg_op op );
  * in FP64 and FP32. */
 int hyg_op_create_p1_divkgrad( hyg_mesh mesh, int min_level, int max_level, double k0, const double* modes, int n_modes,
                                hyg_op* out );
+/* Replaces P2ConstantLaplaceOperator(storage, minLevel, maxLevel) (P2ConstantOperator< p2_tet_diffusion >,
+ * p2functionspace/P2ConstantOperator.cpp:99-113, with its vertex/edge sub-operators and
+ * computeInverseDiagonalOperatorValues), applied element by element with the element matrices of the six
+ * micro-cell types of each macro-cell (the sums the reference's stencils hold). Works on P2 functions; with
+ * hyg_restrict / hyg_prolongate on P2 functions (quadratic transfers) it supports hyg_apply, hyg_smooth_jac,
+ * hyg_residual, hyg_inverse_diagonal, hyg_power_iteration, GMG and defect correction in FP64 and FP32. */
+int hyg_op_create_p2_const( hyg_mesh mesh, int min_level, int max_level, hyg_op* out );
 /* Cell stencil (15 weights in the order C W E N S NW SE TC TW TS TSE BC BN BE BNW) of cell `cell`
  * (constant-coefficient operators only). */
 int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 );
@@ -136,9 +150,11 @@ int hyg_op_set_config( hyg_op op, int level, int dtype, int mode, int config );
 int hyg_op_get_config( hyg_op op, int level, int dtype, int mode, int* config, int* rows, int* points );
 
 /* ---------------------------------------------------------------- grid transfer
- * P1toP1LinearRestriction::restrict(function, sourceLevel, flag) (gridtransferoperators/P1toP1LinearRestriction.cpp:168-345) */
+ * P1toP1LinearRestriction::restrict(function, sourceLevel, flag) (gridtransferoperators/P1toP1LinearRestriction.cpp:168-345);
+ * on a P2 function P2toP2QuadraticRestriction::restrict (P2toP2QuadraticRestriction.cpp:131-286) */
 int hyg_restrict( hyg_func f, int source_level, int flag, void* stream );
-/* P1toP1LinearProlongation::prolongate / prolongateAndAdd (gridtransferoperators/P1toP1LinearProlongation.cpp:194-398) */
+/* P1toP1LinearProlongation::prolongate / prolongateAndAdd (gridtransferoperators/P1toP1LinearProlongation.cpp:194-398);
+ * on a P2 function P2toP2QuadraticProlongation::prolongate / prolongateAndAdd (P2toP2QuadraticProlongation.cpp:217-424) */
 int hyg_prolongate( hyg_func f, int source_level, int flag, int update, void* stream );
 
 /* ---------------------------------------------------------------- solvers
