@@ -182,6 +182,66 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
     return out
 
 
+def bench_p2(hyg, hip, st, steps, level, seed):
+    """P2 constant-coefficient -Laplace on the 24-tet cube (P2 level 7 has the DoFs of P1 level 8): apply
+    throughput in FP64 and FP32 (element by element: per DoF 75 multiply-adds on average, 24 micro-cells x 10
+    at vertex DoFs, 4-6 x 10 at edge DoFs), and V(3,3) Chebyshev cycles to 1e-10 in FP64 and as FP32 cycles
+    inside FP64 defect correction."""
+    mesh = hyg.Mesh("cube_24el.msh")
+    L = level
+    n = mesh.ndofs(L + 1)
+    bnd = mesh.boundary(L + 1)
+    inner = int((~bnd).sum())
+    lib = hyg.load()
+    op = hyg.P2ConstantOperator(mesh, 0, L)
+    out = {"workload": f"P2 -Laplace, cube_24el.msh P2 level {L} ({n} DoFs)", "dofs_inner": inner,
+           "algorithmic_bytes_per_dof_f64": 16, "flops_per_dof": 150}
+    rng = np.random.default_rng(seed)
+    for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):
+        x, y = hyg.Function(mesh, L, L, dt, space="p2"), hyg.Function(mesh, L, L, dt, space="p2")
+        x.upload(L, rng.uniform(-1, 1, n))
+
+        def ap():
+            hyg.check(lib.hyg_apply(op.h, x.h, y.h, L, hyg.INNER, hyg.REPLACE, st))
+
+        for _ in range(2):
+            ap()
+        ms = timed(hip, st, ap, steps) / steps
+        out[f"apply_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3),
+                                "GB/s": round(inner * (16 if dt == hyg.F64 else 8) / (ms / 1e3) / 1e9, 1)}
+        del x, y
+    b = hyg.Function(mesh, 0, L, space="p2")
+    bv = rng.uniform(-1.0, 1.0, n)
+    bv[bnd] = 0.0
+    b.upload(L, bv)
+    Lr = min(L, 4)
+    pv, pt = hyg.Function(mesh, Lr, Lr, space="p2"), hyg.Function(mesh, Lr, Lr, space="p2")
+    pv.upload(Lr, np.random.default_rng(5).uniform(-1, 1, mesh.ndofs(Lr + 1)))
+    rho = op.power_iteration(pv, pt, 40, Lr)
+    kw = dict(pre=3, post=3, smoother="cheb", cheb_order=2, cheb_bounds=(0.3 * rho, 1.2 * rho))
+    g64 = hyg.GMG(op, 0, L, hyg.F64, **kw)
+    g32 = hyg.GMG(op, 0, L, hyg.F32, **kw)
+    x64, xmx = hyg.Function(mesh, 0, L, space="p2"), hyg.Function(mesh, 0, L, space="p2")
+    g64.solve_tol(x64, b, L, 1, 1e-10, st)
+    hyg.ir_solve(op, g32, xmx, b, L, inner=1, max_outer=1, rel_tol=1e-10, stream=st)
+    for f in (x64, xmx):
+        f.interpolate(0.0, L)
+    hip.sync(st)
+    t0 = time.perf_counter()
+    r64 = g64.solve_tol(x64, b, L, 60, 1e-10, st)
+    t64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rmx = hyg.ir_solve(op, g32, xmx, b, L, inner=1, max_outer=60, rel_tol=1e-10, stream=st)
+    tmx = time.perf_counter() - t0
+    u64, umx = x64.download(L), xmx.download(L)
+    out["solve"] = {"cycle": f"V(3,3) Chebyshev order 2, rho(D^-1 A) = {rho:.5f}, coarse P2 level 0 CG",
+                    "fp64": {"seconds": round(t64, 4), "cycles": len(r64) - 1, "rel_residual": float(r64[-1] / r64[0])},
+                    "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "rel_residual": float(rmx[-1] / rmx[0]),
+                              "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
+                    "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +256,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default cell-kernel configuration")
     ap.add_argument("--no-divkgrad", action="store_true", help="skip the variable-coefficient workload")
     ap.add_argument("--divkgrad-level", type=int, default=7)
+    ap.add_argument("--no-p2", action="store_true", help="skip the P2 workload")
+    ap.add_argument("--p2-level", type=int, default=7)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -319,6 +381,10 @@ def main():
     # ---------------- variable-coefficient workload
     if not args.no_divkgrad:
         out["divkgrad"] = bench_divkgrad(hyg, hip, st, max(3, args.steps // 10), args.divkgrad_level, 2024)
+
+    # ---------------- P2 workload
+    if not args.no_p2:
+        out["p2"] = bench_p2(hyg, hip, st, max(3, args.steps // 10), args.p2_level, 2025)
 
     # ---------------- CPU baseline (oracle), rank 0, N = 1
     if not args.no_cpu:
