@@ -36,6 +36,7 @@ def _sig(lib):
         "hyg_last_error": (C.c_char_p, []),
         "hyg_init": (i, [i]),
         "hyg_mesh_create_from_msh": (i, [C.c_char_p, pvp]),
+        "hyg_mesh_create_spherical_shell": (i, [i, i, d, d, pvp]),
         "hyg_mesh_destroy": (i, [vp]),
         "hyg_mesh_info": (i, [vp, P(np.int64)]),
         "hyg_mesh_num_dofs": (i, [vp, i, C.POINTER(C.c_int64)]),
@@ -121,11 +122,18 @@ def _out():
 
 
 class Mesh:
-    def __init__(self, name):
+    """A Gmsh v2.2 file (name or path), or the spherical shell ("shell", ntan, nrad, rmin, rmax)."""
+
+    def __init__(self, name, *shell):
         init()
-        self.path = name if os.path.isabs(name) else os.path.join(MESHES, name)
         h = _out()
-        check(load().hyg_mesh_create_from_msh(self.path.encode(), C.byref(h)))
+        if name == "shell":
+            self.path = None
+            ntan, nrad, rmin, rmax = shell
+            check(load().hyg_mesh_create_spherical_shell(ntan, nrad, rmin, rmax, C.byref(h)))
+        else:
+            self.path = name if os.path.isabs(name) else os.path.join(MESHES, name)
+            check(load().hyg_mesh_create_from_msh(self.path.encode(), C.byref(h)))
         self.h = h
 
     def info(self):

@@ -520,6 +520,30 @@ int hyg_mesh_create_from_msh( const char* path, hyg_mesh* out )
    return HYG_OK;
 }
 
+int hyg_mesh_create_spherical_shell( int ntan, int nrad, double rmin, double rmax, hyg_mesh* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !out )
+      return fail( HYG_ERR_ARG, "null argument" );
+   std::string err;
+   auto        m = meshSphericalShell( ntan, nrad, rmin, rmax, err );
+   if ( !m )
+      return fail( HYG_ERR_ARG, err );
+   auto* h = new hyg_mesh_s;
+   h->ctx.mesh   = std::move( m );
+   h->ctx.device = g_device;
+   try
+   {
+      h->ctx.buildTopology();
+   } catch ( const std::exception& e )
+   {
+      delete h;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   *out = h;
+   return HYG_OK;
+}
+
 int hyg_mesh_destroy( hyg_mesh mesh )
 {
    delete mesh;

@@ -12,6 +12,7 @@
 #include "hyg_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <fstream>
 #include <map>
 #include <sstream>
@@ -20,6 +21,9 @@ namespace hyg {
 
 static const int kCellFaceVerts[4][3] = { { 0, 1, 2 }, { 0, 1, 3 }, { 0, 2, 3 }, { 1, 2, 3 } };
 static const int kCellEdgeVerts[6][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 }, { 0, 3 }, { 1, 3 }, { 2, 3 } };
+
+std::unique_ptr< Mesh > buildMesh( const std::map< long, std::array< double, 3 > >& nodes,
+                                   const std::map< std::vector< long >, int >& cellKeys, const std::string& what, std::string& err );
 
 std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err )
 {
@@ -97,6 +101,17 @@ std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err )
          cellKeys[vs] = 0;
       }
    }
+   return buildMesh( nodes, cellKeys, path, err );
+}
+
+// Primitives of a tetrahedral mesh given by vertex IDs -> coordinates and cells (vertex ID lists in the
+// cell's local order), numbered as SetupPrimitiveStorage numbers the containers of a MeshInfo
+// (vertices by ID, cells by their vertex list, edges and faces by their sorted vertex lists); boundary
+// flags by setMeshBoundaryFlagsOnBoundary (faces with one cell, and their edges and vertices).
+std::unique_ptr< Mesh > buildMesh( const std::map< long, std::array< double, 3 > >& nodes,
+                                   const std::map< std::vector< long >, int >& cellKeys, const std::string& what, std::string& err )
+{
+   const std::string path = what;
    if ( cellKeys.empty() )
    {
       err = "mesh has no tetrahedra: " + path;
@@ -186,6 +201,136 @@ std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err )
          m->vB[v] = 1;
    }
    return m;
+}
+
+// MeshInfo::meshSphericalShell( ntan, nrad, rmin, rmax ) with the default SHELLMESH_CLASSIC meshing
+// (mesh/MeshGenSphericalShell.cpp): the icosahedron's ten diamonds on the unit sphere, refined by arc
+// midpoints (ntan - 1 a power of two), scaled to nrad equidistant radii; each diamond cell between two
+// layers is split into six tetrahedra (northern and southern hemispheres use mirrored splittings).
+std::unique_ptr< Mesh > meshSphericalShell( int ntan, int nrad, double rmin, double rmax, std::string& err )
+{
+   const int nt = ntan - 1;
+   if ( ntan < 2 || ( nt & ( nt - 1 ) ) != 0 || nrad < 2 || !( rmin > 0.0 ) || !( rmin < rmax ) )
+   {
+      err = "spherical shell needs ntan - 1 a power of two, nrad >= 2 and 0 < rmin < rmax";
+      return nullptr;
+   }
+   using P = std::array< double, 3 >;
+   // the twelve icosahedral nodes: poles, upper ring, lower ring (anti-clockwise)
+   const double fifthpi = 0.4 * std::asin( 1.0 );
+   const double w       = 2.0 * std::acos( 1.0 / ( 2.0 * std::sin( fifthpi ) ) );
+   const double cosw = std::cos( w ), sinw = std::sin( w );
+   P            ico[12];
+   ico[0]  = { 0.0, 0.0, 1.0 };
+   ico[11] = { 0.0, 0.0, -1.0 };
+   for ( int k = 1; k <= 5; ++k )
+   {
+      const double pu = 2.0 * ( double( k ) - 0.5 ) * fifthpi, pl = 2.0 * ( double( k ) - 1.0 ) * fifthpi;
+      ico[k]     = { sinw * std::cos( pu ), sinw * std::sin( pu ), cosw };
+      ico[k + 5] = { sinw * std::cos( pl ), sinw * std::sin( pl ), -cosw };
+   }
+   // diamonds: pole, then the other three nodes anti-clockwise
+   static const int dia[10][4] = { { 0, 5, 6, 1 },   { 0, 1, 7, 2 },  { 0, 2, 8, 3 },  { 0, 3, 9, 4 },  { 0, 4, 10, 5 },
+                                   { 11, 7, 1, 6 }, { 11, 8, 2, 7 }, { 11, 9, 3, 8 }, { 11, 10, 4, 9 }, { 11, 6, 5, 10 } };
+   // unit-sphere nodes of every diamond, [is1][is2][id], by repeated arc-midpoint refinement
+   std::vector< P > uc( size_t( ntan ) * ntan * 10 );
+   auto             U = [&]( int a, int b, int d ) -> P& { return uc[( size_t( a ) * ntan + b ) * 10 + d]; };
+   auto             mid = []( const P& a, const P& b ) {
+      const double x = a[0] + b[0], y = a[1] + b[1], z = a[2] + b[2], s = std::sqrt( x * x + y * y + z * z );
+      return P{ x / s, y / s, z / s };
+   };
+   for ( int d = 0; d < 10; ++d )
+   {
+      const int l = d < 5 ? 1 : 3, r = d < 5 ? 3 : 1;
+      U( 0, 0, d )     = ico[dia[d][0]];
+      U( nt, 0, d )    = ico[dia[d][l]];
+      U( nt, nt, d )   = ico[dia[d][2]];
+      U( 0, nt, d )    = ico[dia[d][r]];
+      for ( int m = 1; m < nt; m *= 2 )
+      {
+         const int len = nt / m, h = len / 2;
+         for ( int j1 = 0; j1 <= m; ++j1 ) // rows
+            for ( int j2 = 0; j2 < m; ++j2 )
+               U( j1 * len, j2 * len + h, d ) = mid( U( j1 * len, j2 * len, d ), U( j1 * len, j2 * len + len, d ) );
+         for ( int j1 = 0; j1 <= m; ++j1 ) // columns
+            for ( int j2 = 0; j2 < m; ++j2 )
+               U( j2 * len + h, j1 * len, d ) = mid( U( j2 * len, j1 * len, d ), U( j2 * len + len, j1 * len, d ) );
+         for ( int j1 = 0; j1 < m; ++j1 ) // diagonals
+            for ( int j2 = 0; j2 < m; ++j2 )
+               U( j1 * len + h, j2 * len + h, d ) = mid( U( j1 * len, j2 * len + len, d ), U( j1 * len + len, j2 * len, d ) );
+      }
+   }
+   std::vector< double > radius( nrad );
+   for ( int k = 0; k < nrad; ++k )
+      radius[k] = rmin + ( ( rmax - rmin ) / double( nrad - 1 ) ) * double( k );
+   // vertex IDs: north poles (one per layer), south poles, then per layer, diamond, row and column
+   auto vertexId = [&]( int s1, int s2, int d, int r ) -> long {
+      if ( s1 == 0 && s2 == 0 )
+         return d < 5 ? r : nrad + r;
+      if ( s2 == 0 ) // upper left edge belongs to the neighbouring diamond
+      {
+         s2 = s1;
+         s1 = 0;
+         d  = d < 5 ? ( d + 4 ) % 5 : ( d + 4 ) % 5 + 5;
+      }
+      else if ( s1 == nt ) // lower left edge
+      {
+         s1 = nt - s2;
+         s2 = nt;
+         d  = d < 5 ? ( d + 4 ) % 5 + 5 : d % 5;
+      }
+      return 2L * nrad + long( r ) * nt * nt * 10 + long( d ) * nt * nt + long( s2 - 1 ) * nt + s1;
+   };
+   std::map< long, P > nodes;
+   const long          nVerts = ( 10L * nt * nt + 2 ) * nrad;
+   for ( long v = 0; v < nVerts; ++v )
+   {
+      int s1 = 0, s2 = 0, d = 0, r = 0;
+      if ( v >= 2L * nrad )
+      {
+         long i = v - 2L * nrad;
+         r      = int( i / ( 10L * nt * nt ) );
+         i -= long( r ) * 10 * nt * nt;
+         d = int( i / ( long( nt ) * nt ) );
+         i -= long( d ) * nt * nt;
+         s2 = int( i / nt ) + 1;
+         s1 = int( i - long( s2 - 1 ) * nt );
+      }
+      else if ( v < nrad )
+         r = int( v );
+      else
+      {
+         d = 5;
+         r = int( v - nrad );
+      }
+      const P& u = U( s1, s2, d );
+      nodes[v]   = { u[0] * radius[r], u[1] * radius[r], u[2] * radius[r] };
+   }
+   // the six tetrahedra of a diamond cell (local vertices 0-3 on the outer layer, 4-7 on the inner)
+   static const int tN[6][4]  = { { 0, 1, 3, 7 }, { 4, 7, 5, 0 }, { 0, 5, 1, 7 }, { 1, 2, 3, 6 }, { 5, 7, 6, 1 }, { 3, 6, 7, 1 } };
+   static const int tS[6][4]  = { { 4, 7, 5, 1 }, { 0, 1, 3, 4 }, { 4, 3, 7, 1 }, { 7, 6, 5, 2 }, { 3, 1, 2, 7 }, { 5, 2, 1, 7 } };
+   static const int off[8][3] = { { 0, 0, 1 }, { 1, 0, 1 }, { 1, 1, 1 }, { 0, 1, 1 }, { 0, 0, 0 }, { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 } };
+   std::map< std::vector< long >, int > cellKeys;
+   const long                           nElems = long( nt ) * nt * ( nrad - 1 ) * 10 * 6;
+   for ( long e = 0; e < nElems; ++e )
+   {
+      long      i  = e;
+      const int r  = int( i / ( 60L * nt * nt ) );
+      i -= long( r ) * 60 * nt * nt;
+      const int d = int( i / ( 6L * nt * nt ) );
+      i -= long( d ) * 6 * nt * nt;
+      const int s2 = int( i / ( 6L * nt ) );
+      i -= long( s2 ) * 6 * nt;
+      const int s1 = int( i / 6 );
+      const int t  = int( i - 6L * s1 );
+      long      vi[8];
+      for ( int k = 0; k < 8; ++k )
+         vi[k] = d <= 4 ? vertexId( s1 + off[k][0], s2 + off[k][1], d, r + off[k][2] )
+                        : vertexId( s1 + off[k][1], s2 + off[k][0], d, r + off[k][2] );
+      const int( &tl )[4] = d <= 4 ? tN[t] : tS[t];
+      cellKeys[{ vi[tl[0]], vi[tl[1]], vi[tl[2]], vi[tl[3]] }] = 0;
+   }
+   return buildMesh( nodes, cellKeys, "spherical shell", err );
 }
 
 i64 numDofs( const Mesh& m, int L )

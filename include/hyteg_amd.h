@@ -59,6 +59,11 @@ int hyg_init( int device );
  * (primitivestorage/SetupPrimitiveStorage.cpp:40-365, 781-792) and PrimitiveStorage.
  * Boundary primitives get mesh flag 1 = Dirichlet (boundary/BoundaryConditions.hpp create0123BC). */
 int hyg_mesh_create_from_msh( const char* path, hyg_mesh* out );
+/* Replaces MeshInfo::meshSphericalShell( ntan, nrad, rmin, rmax ) (mesh/MeshGenSphericalShell.cpp, default
+ * SHELLMESH_CLASSIC: arc-midpoint refinement of the icosahedron's diamonds, ntan - 1 a power of two,
+ * nrad >= 2 equidistant radii) followed by the same storage setup and boundary flags as above: the inner
+ * and outer spheres are Dirichlet. (ntan, nrad) = (5, 2) gives 960 macro-cells. */
+int hyg_mesh_create_spherical_shell( int ntan, int nrad, double rmin, double rmax, hyg_mesh* out );
 int hyg_mesh_destroy( hyg_mesh mesh );
 /* out[0..6] = #vertices, #edges, #faces, #cells, #boundary vertices, #boundary edges, #boundary faces */
 int hyg_mesh_info( hyg_mesh mesh, int64_t* out );

@@ -140,6 +140,8 @@ struct Mesh
 static const int CELL_FACE_VERTS[4][3] = { { 0, 1, 2 }, { 0, 1, 3 }, { 0, 2, 3 }, { 1, 2, 3 } };
 static const int CELL_EDGE_VERTS[6][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 }, { 0, 3 }, { 1, 3 }, { 2, 3 } };
 
+static std::unique_ptr< Mesh > buildMesh( const std::map< idx, P3 >& nodes, const std::set< std::vector< idx > >& cellsGmsh );
+
 static std::unique_ptr< Mesh > loadGmsh( const std::string& path )
 {
    std::ifstream in( path );
@@ -180,6 +182,12 @@ static std::unique_ptr< Mesh > loadGmsh( const std::string& path )
       if ( type == 4 )
          cellsGmsh.insert( vs );
    }
+   return buildMesh( nodes, cellsGmsh );
+}
+
+// primitives of a MeshInfo given by its vertex container (ID -> coordinates) and cell container (vertex lists)
+static std::unique_ptr< Mesh > buildMesh( const std::map< idx, P3 >& nodes, const std::set< std::vector< idx > >& cellsGmsh )
+{
    auto m = std::make_unique< Mesh >();
    std::map< idx, int > vid;
    for ( auto& [id, x] : nodes )
@@ -1552,6 +1560,161 @@ void P2ConstantLaplace< T >::prolongateOp( Function< T >& f, int srcL, int flag,
    prolongateP2< T >( f, srcL, flag, add );
 }
 
+// ---------------------------------------------------------------- P2 variable coefficient
+// Coefficient of -div(k grad u) for the P2 elementwise operator. kind 0: k(x) = k0 exp(alpha (1 - |x|)) (1 + beta xi_e)
+// with xi_e in [-1, 1) drawn per micro-element by SplitMix64 from (seed, element id); kind 1: the polynomial
+// k(x) = 1 + x/2 + y^2/4 (for exactness checks of the quadrature).
+struct P2KField
+{
+   int      kind  = 0;
+   double   k0    = 1.0, alpha = 0.0, beta = 0.0;
+   uint64_t seed  = 0;
+   double   operator()( const P3& x, uint64_t eid ) const
+   {
+      if ( kind == 1 )
+         return 1.0 + 0.5 * x[0] + 0.25 * x[1] * x[1];
+      uint64_t z = seed + ( eid + 1 ) * 0x9E3779B97F4A7C15ull;
+      z          = ( z ^ ( z >> 30 ) ) * 0xBF58476D1CE4E5B9ull;
+      z          = ( z ^ ( z >> 27 ) ) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      const double xi = 2.0 * ( double( z >> 11 ) * 0x1.0p-53 ) - 1.0;
+      return k0 * std::exp( alpha * ( 1.0 - std::sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] ) ) ) * ( 1.0 + beta * xi );
+   }
+};
+
+// Xiao-Gimbutas degree-4 rule with 11 points (barycentric coordinates w.r.t. vertices 0..3, weights summing to 1/6),
+// the rule of forms/form_hyteg_generated/p2/p2_div_k_grad_affine_q4.cpp (integrateAll, quadrature points and the
+// weight factors of |det J|).
+static const double XG4_LAMBDA[11][4] = {
+    { 0.17469405869723054, 0.040490506727590428, 0.01356070187980288, 0.77125473269537614 },
+    { 0.081404918402859225, 0.75250850700965499, 0.068099370938206658, 0.097987203649279112 },
+    { 0.30444840243449711, 0.24027666492807259, 0.12680172591539199, 0.32847320672203839 },
+    { 0.74122888209362259, 0.067223294893383398, 0.035183929773598722, 0.15636389323939531 },
+    { 0.053341239535745266, 0.41926631387951302, 0.047781435559086663, 0.47961101102565512 },
+    { 0.43295349048135578, 0.45076587609127677, 0.059456616299433829, 0.056824017127933668 },
+    { 0.53800720391618573, 0.12941137378891041, 0.33019041483746447, 0.0023910074574393651 },
+    { 0.008991260093335729, 0.1215419913339278, 0.30649398842969028, 0.56297276014304609 },
+    { 0.10660417256199362, 0.097204644587583267, 0.68439041545304002, 0.11180076739738309 },
+    { 0.32923295974264682, 0.029569495206479609, 0.31790356021339461, 0.32329398483747901 },
+    { 0.10384411641099309, 0.43271023904776862, 0.35382323920929709, 0.1096224053319412 } };
+static const double XG4_W[11] = { 0.006541848487473325,  0.0092122281926561474, 0.032235135341605742, 0.0092322998119293929,
+                                  0.0099888641910932524, 0.011578327656272558,  0.012693785874259723, 0.013237780011337548,
+                                  0.017744672359248346,  0.018372372071416277,  0.025829352669374347 };
+
+// p2_div_k_grad_affine_q4::integrateAll: a_ij = sum_q w_q |det J| k(x_q) grad phi_i(x_q) . grad phi_j(x_q)
+inline void p2DivKGradElement( const P3* x, const P2KField& k, uint64_t eid, double a[10][10] )
+{
+   double       G[4][3];
+   const double adet = std::fabs( baryGrads( x, G ) );
+   for ( int i = 0; i < 10; ++i )
+      for ( int j = 0; j < 10; ++j )
+         a[i][j] = 0.0;
+   for ( int q = 0; q < 11; ++q )
+   {
+      P3 xq{ 0, 0, 0 };
+      for ( int v = 0; v < 4; ++v )
+         for ( int c = 0; c < 3; ++c )
+            xq[c] += XG4_LAMBDA[q][v] * x[v][c];
+      const double s = XG4_W[q] * adet * k( xq, eid );
+      double       d[10][3];
+      p2Grads( XG4_LAMBDA[q], G, d );
+      for ( int i = 0; i < 10; ++i )
+         for ( int j = 0; j < 10; ++j )
+            a[i][j] += s * ( d[i][0] * d[j][0] + d[i][1] * d[j][1] + d[i][2] * d[j][2] );
+   }
+}
+
+// P2ElementwiseOperator< p2_div_k_grad_affine_q4 > (elementwiseoperators/P2ElementwiseOperator.cpp:100-310): the
+// element matrix of every micro-cell of level lat-1 from the 11-point rule, y_T += A_T x_T, additive reduction to
+// the lower-dimensional primitives. Element id of micro-cell (t, anchor a) of cell c: (6 c + t) S3(Nc + 1) +
+// index(Nc + 1, a), Nc = 2^(lat-1). Levels are lattice levels.
+template < typename T >
+struct P2ElementwiseDivKGrad : P2ConstantLaplace< T >
+{
+   using OperatorBase< T >::m;
+   using OperatorBase< T >::invDiag;
+   P2KField k;
+
+   P2ElementwiseDivKGrad( const Mesh* mesh, int lo, int hi, const P2KField& kf )
+   : P2ConstantLaplace< T >( mesh, lo, hi )
+   , k( kf )
+   {
+      for ( int lat = this->minL; lat <= this->maxL; ++lat )
+         computeInverseDiagonalK( lat );
+   }
+
+   template < typename Fn >
+   void forElements( idx c, int lat, Fn&& fn ) const
+   {
+      const idx Nc = idx( 1 ) << ( lat - 1 );
+      forP2Cells( lat, [&]( int t, const idx( *nd )[3] ) {
+         // anchor = (vertex 0 lattice coords) / 2 - type offset of vertex 0
+         const idx ax = nd[0][0] / 2 - MICRO_CELL[t][0][0], ay = nd[0][1] / 2 - MICRO_CELL[t][0][1], az = nd[0][2] / 2 - MICRO_CELL[t][0][2];
+         const uint64_t eid = uint64_t( ( 6 * c + t ) * S3( Nc + 1 ) + cidx( Nc + 1, ax, ay, az ) );
+         P3             xc[4];
+         for ( int v = 0; v < 4; ++v )
+            xc[v] = microCoord( *m, (int) c, lat - 1, ax + MICRO_CELL[t][v][0], ay + MICRO_CELL[t][v][1], az + MICRO_CELL[t][v][2] );
+         double a[10][10];
+         p2DivKGradElement( xc, k, eid, a );
+         fn( nd, a );
+      } );
+   }
+
+   void apply( const Function< T >& src, Function< T >& dst, int lat, int flag, bool add ) const override
+   {
+      src.commDown( lat );
+      if ( !add )
+         interpolate< T >( dst, 0.0, lat, flag );
+      const idx N = idx( 1 ) << lat;
+      const idx w = N + 1;
+#pragma omp parallel for schedule( dynamic )
+      for ( idx c = 0; c < (idx) m->Cl.size(); ++c )
+      {
+         auto&       dc = dst.c[lat][c];
+         const auto& sc = src.c[lat][c];
+         for ( idx z = 0; z <= N; ++z )
+            for ( idx y = 0; y + z <= N; ++y )
+               for ( idx x = 0; x + y + z <= N; ++x )
+                  if ( x == 0 || y == 0 || z == 0 || x + y + z == N )
+                     dc[cidx( w, x, y, z )] = T( 0 );
+         forElements( c, lat, [&]( const idx( *nd )[3], const double( *a )[10] ) {
+            idx vi[10];
+            for ( int i = 0; i < 10; ++i )
+               vi[i] = cidx( w, nd[i][0], nd[i][1], nd[i][2] );
+            for ( int i = 0; i < 10; ++i )
+            {
+               double s = 0.0;
+               for ( int j = 0; j < 10; ++j )
+                  s += a[i][j] * double( sc[vi[j]] );
+               dc[vi[i]] += T( s );
+            }
+         } );
+      }
+      commAdditiveCellToLower( dst, lat, All ^ flag, !add );
+   }
+
+   void computeInverseDiagonalK( int lat )
+   {
+      auto&     D = *invDiag;
+      const idx w = ( idx( 1 ) << lat ) + 1;
+      interpolate< T >( D, 0.0, lat, All );
+      for ( idx c = 0; c < (idx) m->Cl.size(); ++c )
+      {
+         auto& dc = D.c[lat][c];
+         std::fill( dc.begin(), dc.end(), T( 0 ) );
+         forElements( c, lat, [&]( const idx( *nd )[3], const double( *a )[10] ) {
+            for ( int i = 0; i < 10; ++i )
+               dc[cidx( w, nd[i][0], nd[i][1], nd[i][2] )] += T( a[i][i] );
+         } );
+      }
+      commAdditiveCellToLower( D, lat, None, true );
+      forOwned< T >( *m, lat, All, [&]( int kind, int p, idx i ) {
+         T& v = at( D, lat, kind, p, i );
+         v    = T( 1.0 ) / v;
+      } );
+   }
+};
+
 // ---------------------------------------------------------------- smoothers and solvers
 template < typename T >
 struct Solvers
@@ -1848,9 +2011,183 @@ static void gmgRun( OrOp* o, OperatorBase< T >* A, Function< T >& x, Function< T
    }
 }
 
+// MeshInfo::meshSphericalShell( ntan, nrad, rmin, rmax ) with SHELLMESH_CLASSIC (mesh/MeshGenSphericalShell.cpp):
+// elemIdx2Tuple, vertIdx2Tuple, tuple2VertIdx, setupCoordsClassic, computeSphericalShellVertices and the
+// six-tetrahedra splittings tNode / sNode of a diamond cell, followed by the usual storage setup.
+namespace shell {
+struct Gen
+{
+   idx                     nt, nr;
+   std::vector< double >   layers;
+   double                  ico[12][3];
+   idx                     dn[10][4] = { { 0, 5, 6, 1 },   { 0, 1, 7, 2 },  { 0, 2, 8, 3 },  { 0, 3, 9, 4 },  { 0, 4, 10, 5 },
+                                         { 11, 7, 1, 6 }, { 11, 8, 2, 7 }, { 11, 9, 3, 8 }, { 11, 10, 4, 9 }, { 11, 6, 5, 10 } };
+   std::vector< P3 >       nc; // [is1][is2][id] on the unit sphere
+   P3&                     node( idx a, idx b, idx d ) { return nc[size_t( ( a * nt + b ) * 10 + d )]; }
+
+   void setup()
+   {
+      const double fifthpi = 0.4 * std::asin( 1.0 );
+      const double w       = 2.0 * std::acos( 1.0 / ( 2.0 * std::sin( fifthpi ) ) );
+      const double cw = std::cos( w ), sw = std::sin( w );
+      ico[0][0] = 0.0, ico[0][1] = 0.0, ico[0][2] = 1.0;
+      ico[11][0] = 0.0, ico[11][1] = 0.0, ico[11][2] = -1.0;
+      for ( int k = 1; k <= 5; ++k )
+      {
+         double phi = 2.0 * ( double( k ) - 0.5 ) * fifthpi;
+         ico[k][0] = sw * std::cos( phi ), ico[k][1] = sw * std::sin( phi ), ico[k][2] = cw;
+         phi = 2.0 * ( double( k ) - 1.0 ) * fifthpi;
+         ico[k + 5][0] = sw * std::cos( phi ), ico[k + 5][1] = sw * std::sin( phi ), ico[k + 5][2] = -cw;
+      }
+      nc.assign( size_t( nt * nt * 10 ), P3{ 0, 0, 0 } );
+      const idx n1 = nt - 1;
+      for ( idx id = 0; id < 10; ++id )
+      {
+         const idx L = id < 5 ? 1 : 3, R = id < 5 ? 3 : 1;
+         for ( int c = 0; c < 3; ++c )
+         {
+            node( 0, 0, id )[c]   = ico[dn[id][0]][c];
+            node( n1, 0, id )[c]  = ico[dn[id][L]][c];
+            node( n1, n1, id )[c] = ico[dn[id][2]][c];
+            node( 0, n1, id )[c]  = ico[dn[id][R]][c];
+         }
+         auto arcMid = [&]( idx a1, idx a2, idx b1, idx b2, idx t1, idx t2 ) {
+            P3     v{ node( a1, a2, id )[0] + node( b1, b2, id )[0], node( a1, a2, id )[1] + node( b1, b2, id )[1],
+                  node( a1, a2, id )[2] + node( b1, b2, id )[2] };
+            double sc = std::sqrt( v[0] * v[0] + v[1] * v[1] + v[2] * v[2] );
+            node( t1, t2, id ) = { v[0] / sc, v[1] / sc, v[2] / sc };
+         };
+         for ( idx m = 1; m < n1; m *= 2 )
+         {
+            const idx l = n1 / m, l2 = l / 2;
+            for ( idx j1 = 0; j1 <= m; ++j1 )
+               for ( idx j2 = 0; j2 < m; ++j2 )
+               {
+                  idx i1 = j1 * l, i2 = j2 * l + l2;
+                  arcMid( i1, i2 - l2, i1, i2 + l2, i1, i2 );
+               }
+            for ( idx j1 = 0; j1 <= m; ++j1 )
+               for ( idx j2 = 0; j2 < m; ++j2 )
+               {
+                  idx i1 = j2 * l + l2, i2 = j1 * l;
+                  arcMid( i1 - l2, i2, i1 + l2, i2, i1, i2 );
+               }
+            for ( idx j1 = 0; j1 < m; ++j1 )
+               for ( idx j2 = 0; j2 < m; ++j2 )
+               {
+                  idx i1 = j1 * l + l2, i2 = j2 * l + l2;
+                  arcMid( i1 - l2, i2 + l2, i1 + l2, i2 - l2, i1, i2 );
+               }
+         }
+      }
+   }
+   void vertIdx2Tuple( idx jv, idx& is1, idx& is2, idx& id, idx& ir ) const
+   {
+      if ( jv >= 2 * nr )
+      {
+         idx i = jv - 2 * nr;
+         ir    = i / ( 10 * ( nt - 1 ) * ( nt - 1 ) );
+         i -= ir * 10 * ( nt - 1 ) * ( nt - 1 );
+         id = i / ( ( nt - 1 ) * ( nt - 1 ) );
+         i -= id * ( nt - 1 ) * ( nt - 1 );
+         is2 = i / ( nt - 1 );
+         i -= is2 * ( nt - 1 );
+         ++is2;
+         is1 = i;
+      }
+      else if ( jv < nr )
+         is1 = 0, is2 = 0, id = 0, ir = jv;
+      else
+         is1 = 0, is2 = 0, id = 5, ir = jv - nr;
+   }
+   idx tuple2VertIdx( idx is1, idx is2, idx id, idx ir ) const
+   {
+      if ( is1 == 0 && is2 == 0 )
+         return id < 5 ? ir : nr + ir;
+      if ( is2 == 0 )
+      {
+         is2 = is1;
+         is1 = 0;
+         id  = id < 5 ? ( id + 4 ) % 5 : ( id + 4 ) % 5 + 5;
+      }
+      else if ( is1 == nt - 1 )
+      {
+         is1 = ( nt - 1 ) - is2;
+         is2 = nt - 1;
+         id  = id < 5 ? ( id + 4 ) % 5 + 5 : id % 5;
+      }
+      return ir * ( nt - 1 ) * ( nt - 1 ) * 10 + id * ( nt - 1 ) * ( nt - 1 ) + ( is2 - 1 ) * ( nt - 1 ) + is1 + 2 * nr;
+   }
+};
+} // namespace shell
+
+static std::unique_ptr< Mesh > makeShell( idx ntan, idx nrad, double rmin, double rmax )
+{
+   if ( ntan < 2 || ( ( ntan - 1 ) & ( ntan - 2 ) ) != 0 || nrad < 2 || !( rmin > 0 ) || !( rmin < rmax ) )
+      return nullptr;
+   shell::Gen g;
+   g.nt = ntan;
+   g.nr = nrad;
+   for ( idx k = 0; k < nrad; ++k )
+      g.layers.push_back( rmin + ( ( rmax - rmin ) / double( nrad - 1 ) ) * double( k ) );
+   g.setup();
+   std::map< idx, P3 > nodes;
+   const idx           nVerts = ( 10 * ( ntan - 1 ) * ( ntan - 1 ) + 2 ) * nrad;
+   for ( idx v = 0; v < nVerts; ++v )
+   {
+      idx is1, is2, id, ir;
+      g.vertIdx2Tuple( v, is1, is2, id, ir );
+      const P3& u = g.node( is1, is2, id );
+      nodes[v]    = { u[0] * g.layers[ir], u[1] * g.layers[ir], u[2] * g.layers[ir] };
+   }
+   static const idx tNode[6][4] = { { 0, 1, 3, 7 }, { 4, 7, 5, 0 }, { 0, 5, 1, 7 }, { 1, 2, 3, 6 }, { 5, 7, 6, 1 }, { 3, 6, 7, 1 } };
+   static const idx sNode[6][4] = { { 4, 7, 5, 1 }, { 0, 1, 3, 4 }, { 4, 3, 7, 1 }, { 7, 6, 5, 2 }, { 3, 1, 2, 7 }, { 5, 2, 1, 7 } };
+   static const idx offset[8][3] = { { 0, 0, 1 }, { 1, 0, 1 }, { 1, 1, 1 }, { 0, 1, 1 },
+                                     { 0, 0, 0 }, { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 } };
+   std::set< std::vector< idx > > cells;
+   const idx                      nElems = ( ntan - 1 ) * ( ntan - 1 ) * ( nrad - 1 ) * 10 * 6;
+   for ( idx jelem = 0; jelem < nElems; ++jelem )
+   {
+      idx x = jelem;
+      idx ir = x / ( 60 * ( ntan - 1 ) * ( ntan - 1 ) );
+      x -= ir * 60 * ( ntan - 1 ) * ( ntan - 1 );
+      idx id = x / ( 6 * ( ntan - 1 ) * ( ntan - 1 ) );
+      x -= id * 6 * ( ntan - 1 ) * ( ntan - 1 );
+      idx is2 = x / ( 6 * ( ntan - 1 ) );
+      x -= is2 * 6 * ( ntan - 1 );
+      idx is1 = x / 6;
+      idx it  = x - 6 * is1;
+      idx vIdx[8];
+      for ( int k = 0; k < 8; ++k )
+      {
+         const idx ks1 = is1 + ( id <= 4 ? offset[k][0] : offset[k][1] );
+         const idx ks2 = is2 + ( id <= 4 ? offset[k][1] : offset[k][0] );
+         vIdx[k]       = g.tuple2VertIdx( ks1, ks2, id, ir + offset[k][2] );
+      }
+      const idx* tn = id <= 4 ? tNode[it] : sNode[it];
+      cells.insert( { vIdx[tn[0]], vIdx[tn[1]], vIdx[tn[2]], vIdx[tn[3]] } );
+   }
+   return buildMesh( nodes, cells );
+}
+
 extern "C" {
 
 void* or_mesh_load( const char* path ) { return loadGmsh( path ).release(); }
+void* or_mesh_shell( int ntan, int nrad, double rmin, double rmax ) { return makeShell( ntan, nrad, rmin, rmax ).release(); }
+
+// sum of the macro-cell volumes
+double or_mesh_volume( void* mh )
+{
+   const auto* m = static_cast< Mesh* >( mh );
+   double      v = 0.0;
+   for ( const auto& cv : m->Cl )
+   {
+      P3 x[4] = { m->X[cv[0]], m->X[cv[1]], m->X[cv[2]], m->X[cv[3]] };
+      double G[4][3];
+      v += std::fabs( baryGrads( x, G ) ) / 6.0;
+   }
+   return v;
+}
 void  or_mesh_free( void* m ) { delete static_cast< Mesh* >( m ); }
 
 // V, E, F, C, boundary V, boundary E, boundary F
@@ -1983,6 +2320,54 @@ void* or_op_new_p2( void* mh, int minL, int maxL, int prec )
    else
       o->d = std::make_unique< P2ConstantLaplace< double > >( o->m, minL, maxL );
    return o;
+}
+
+// P2ElementwiseOperator< p2_div_k_grad_affine_q4 > with P2KField (kind, k0, alpha, beta, seed)
+void* or_op_new_p2_divkgrad( void* mh, int minL, int maxL, int prec, int kind, double k0, double alpha, double beta,
+                             uint64_t seed )
+{
+   P2KField kf;
+   kf.kind  = kind;
+   kf.k0    = k0;
+   kf.alpha = alpha;
+   kf.beta  = beta;
+   kf.seed  = seed;
+   auto* o  = new OrOp;
+   o->prec  = prec;
+   o->m     = static_cast< Mesh* >( mh );
+   if ( prec == 4 )
+      o->f = std::make_unique< P2ElementwiseDivKGrad< float > >( o->m, minL, maxL, kf );
+   else
+      o->d = std::make_unique< P2ElementwiseDivKGrad< double > >( o->m, minL, maxL, kf );
+   return o;
+}
+
+// value of the P2 coefficient at x for element id eid
+double or_p2k_value( int kind, double k0, double alpha, double beta, uint64_t seed, const double* x, uint64_t eid )
+{
+   P2KField kf;
+   kf.kind  = kind;
+   kf.k0    = k0;
+   kf.alpha = alpha;
+   kf.beta  = beta;
+   kf.seed  = seed;
+   return kf( { x[0], x[1], x[2] }, eid );
+}
+
+// P2 div-k-grad element matrix (11-point rule) of the tetrahedron x (12 coordinates), coefficient P2KField kind 1
+// (polynomial) -> 100 values, row major
+void or_p2_divkgrad_element_poly( const double* x, double* out )
+{
+   P3 xc[4];
+   for ( int v = 0; v < 4; ++v )
+      xc[v] = { x[3 * v], x[3 * v + 1], x[3 * v + 2] };
+   P2KField kf;
+   kf.kind = 1;
+   double a[10][10];
+   p2DivKGradElement( xc, kf, 0, a );
+   for ( int i = 0; i < 10; ++i )
+      for ( int j = 0; j < 10; ++j )
+         out[10 * i + j] = a[i][j];
 }
 
 // P2 element matrix of -Laplace on a tetrahedron (12 coordinates) -> 100 values, row major

@@ -24,6 +24,8 @@ def lib():
         P = np.ctypeslib.ndpointer
         sig = {
             "or_mesh_load": (vp, [C.c_char_p]),
+            "or_mesh_shell": (vp, [i, i, d, d]),
+            "or_mesh_volume": (d, [vp]),
             "or_mesh_free": (None, [vp]),
             "or_mesh_info": (None, [vp, P(np.int64)]),
             "or_ndofs": (i64, [vp, i]),
@@ -47,6 +49,9 @@ def lib():
             "or_gmg": (None, [vp, vp, vp, P(np.int32), P(np.float64), i, P(np.float64)]),
             "or_ir": (None, [vp, vp, vp, vp, P(np.int32), P(np.float64), i, i, P(np.float64)]),
             "or_op_new_p2": (vp, [vp, i, i, i]),
+            "or_op_new_p2_divkgrad": (vp, [vp, i, i, i, i, d, d, d, C.c_uint64]),
+            "or_p2_divkgrad_element_poly": (None, [P(np.float64), P(np.float64)]),
+            "or_p2k_value": (d, [i, d, d, d, C.c_uint64, P(np.float64), C.c_uint64]),
             "or_p2_element": (None, [P(np.float64), P(np.float64)]),
             "or_p2_restrict": (None, [vp, i, i]),
             "or_p2_prolongate": (None, [vp, i, i, i]),
@@ -60,7 +65,15 @@ def lib():
 
 
 class Mesh:
-    def __init__(self, name):
+    """A Gmsh v2.2 file (name or path), or the spherical shell ("shell", ntan, nrad, rmin, rmax)."""
+
+    def __init__(self, name, *shell):
+        if name == "shell":
+            self.path = None
+            self.h = lib().or_mesh_shell(*shell)
+            if not self.h:
+                raise RuntimeError("bad spherical shell parameters")
+            return
         path = name if os.path.isabs(name) else os.path.join(MESHES, name)
         self.path = path
         self.h = lib().or_mesh_load(path.encode())
@@ -74,6 +87,9 @@ class Mesh:
 
     def ndofs(self, L):
         return lib().or_ndofs(self.h, L)
+
+    def volume(self):
+        return lib().or_mesh_volume(self.h)
 
     def coords(self, L):
         n = self.ndofs(L)
@@ -139,6 +155,22 @@ class P2Operator(Operator):
     def __init__(self, mesh, minL, maxL, prec=8):
         self.mesh = mesh
         self.h = lib().or_op_new_p2(mesh.h, minL, maxL, prec)
+
+
+class P2DivKGradOperator(Operator):
+    """P2 elementwise -div(k grad) on P2 levels minL..maxL (11-point rule); kind 0: k = k0 exp(alpha (1 - |x|))
+    (1 + beta xi_e) with xi_e per micro-element from SplitMix64(seed, element id); kind 1: 1 + x/2 + y^2/4."""
+
+    def __init__(self, mesh, minL, maxL, kind=0, k0=1.0, alpha=0.0, beta=0.0, seed=0, prec=8):
+        self.mesh = mesh
+        self.h = lib().or_op_new_p2_divkgrad(mesh.h, minL, maxL, prec, kind, k0, alpha, beta, seed)
+
+
+def p2_divkgrad_element_poly(x):
+    """P2 element matrix of -div(k grad) with k = 1 + x/2 + y^2/4 (11-point rule) on tetrahedron x (4 x 3)."""
+    out = np.zeros(100)
+    lib().or_p2_divkgrad_element_poly(np.ascontiguousarray(x, dtype=np.float64).ravel(), out)
+    return out.reshape(10, 10)
 
 
 def p2_element(x):
