@@ -55,6 +55,7 @@ def _sig(lib):
         "hyg_op_create_p1_const": (i, [vp, i, i, pvp]),
         "hyg_op_create_p1_divkgrad": (i, [vp, i, i, d, P(np.float64), i, pvp]),
         "hyg_op_create_p2_const": (i, [vp, i, i, pvp]),
+        "hyg_op_create_p2_divkgrad": (i, [vp, i, i, i, d, d, d, C.c_uint64, pvp]),
         "hyg_op_destroy": (i, [vp]),
         "hyg_op_cell_stencil": (i, [vp, i, i, P(np.float64)]),
         "hyg_apply": (i, [vp, vp, vp, i, i, i, vp]),
@@ -283,6 +284,17 @@ class P2ConstantOperator(P1ConstantOperator):
         self.mesh = mesh
         h = _out()
         check(load().hyg_op_create_p2_const(mesh.h, minL, maxL, C.byref(h)))
+        self.h = h
+
+
+class P2DivKGradOperator(P1ConstantOperator):
+    """P2 -div(k grad) on P2 levels minL..maxL; kind 0: k = k0 exp(alpha (1 - |x|)) (1 + beta xi_e),
+    kind 1: k = 1 + x/2 + y^2/4."""
+
+    def __init__(self, mesh, minL, maxL, kind=0, k0=1.0, alpha=0.0, beta=0.0, seed=0):
+        self.mesh = mesh
+        h = _out()
+        check(load().hyg_op_create_p2_divkgrad(mesh.h, minL, maxL, kind, k0, alpha, beta, seed, C.byref(h)))
         self.h = h
 
 

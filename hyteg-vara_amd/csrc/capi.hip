@@ -13,6 +13,7 @@ namespace hyg {
 void assembleP1Const( hyg_op_s& op );
 void assembleP1DivKGrad( hyg_op_s& op );
 void assembleP2Const( hyg_op_s& op );
+void assembleP2DivKGrad( hyg_op_s& op );
 void primEntriesPos( const LowerEntry& e, int N, int i, int j, int& qx, int& qy, int& qz );
 } // namespace hyg
 
@@ -233,7 +234,7 @@ int stencilImpl( hyg_op_s* op, int L, int dtype, const StencilCall& c, hipStream
    MeshCtx& mc = op->mesh->ctx;
    if ( op->kind == 1 )
       launchElementwise( mc, mc.geom( L ), *op, L, dtype, c, st );
-   else if ( op->kind == 2 )
+   else if ( op->kind == 2 || op->kind == 3 )
       launchP2( mc, mc.geom( L ), *op, L, dtype, c, st );
    else
       launchStencil( mc, mc.geom( L ), op->levels[L], L, dtype, c, st );
@@ -898,6 +899,54 @@ int hyg_op_create_p2_const( hyg_mesh mesh, int min_level, int max_level, hyg_op*
    return HYG_OK;
 }
 
+int hyg_op_create_p2_divkgrad( hyg_mesh mesh, int min_level, int max_level, int kind, double k0, double alpha, double beta,
+                               uint64_t seed, hyg_op* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh || !out || min_level < 0 || max_level < min_level || max_level > 9 || ( kind != 0 && kind != 1 ) ||
+        !std::isfinite( k0 ) || !std::isfinite( alpha ) || !std::isfinite( beta ) )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   auto* op = new hyg_op_s;
+   op->mesh     = mesh;
+   op->lat      = 1;
+   op->minL     = min_level + 1;
+   op->maxL     = max_level + 1;
+   op->kind     = 3;
+   op->p2k.kind = kind;
+   op->p2k.k0   = k0;
+   op->p2k.alpha = alpha;
+   op->p2k.beta = beta;
+   op->p2k.seed = seed;
+   try
+   {
+      mesh->ctx.p2Tables();
+      for ( int L = op->minL; L <= op->maxL; ++L )
+         mesh->ctx.geom( L );
+      assembleP2DivKGrad( *op );
+      for ( int L = op->minL; L <= op->maxL; ++L )
+      {
+         launchP2Coefficients( mesh->ctx, *op, L, nullptr );
+         launchP2InverseDiagonal( mesh->ctx, mesh->ctx.geom( L ), *op, L, nullptr );
+      }
+   } catch ( const std::exception& e )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   if ( int rc = launched( "P2 variable-coefficient operator setup" ) )
+   {
+      delete op;
+      return rc;
+   }
+   if ( hipDeviceSynchronize() != hipSuccess )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, "P2 variable-coefficient operator setup failed" );
+   }
+   *out = op;
+   return HYG_OK;
+}
+
 int hyg_op_destroy( hyg_op op )
 {
    delete op;
@@ -1075,7 +1124,7 @@ int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream )
    HYG_TRY( checkLevel( dst, level ) );
    HYG_TRY( checkSpace( op, dst ) );
    MeshCtx& mc = op->mesh->ctx;
-   if ( op->kind == 1 || op->kind == 2 )
+   if ( op->kind >= 1 )
    {
       // the elementwise and P2 operators keep their inverse diagonal as a function array
       const auto& ol = op->levels[level];

@@ -21,14 +21,6 @@
 
 namespace hyg {
 
-// micro-cell types WHITE_UP, BLUE_UP, GREEN_UP, WHITE_DOWN, BLUE_DOWN, GREEN_DOWN: vertex offsets
-// from the micro-cell's anchor (volumedofspace/CellDoFIndexing.hpp:155-198)
-__constant__ int kMC[6][4][3] = { { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } },
-                                  { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
-                                  { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } },
-                                  { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
-                                  { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } },
-                                  { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } };
 // stencil direction (C W E N S NW SE TC TW TS TSE BC BN BE BNW) from vertex k to vertex j of type t
 constexpr int kMCDir[6][4][4] = { { { 0, 2, 3, 7 }, { 1, 0, 5, 8 }, { 4, 6, 0, 9 }, { 11, 13, 12, 0 } },
                                   { { 0, 3, 5, 7 }, { 4, 0, 1, 9 }, { 6, 2, 0, 10 }, { 11, 12, 14, 0 } },

@@ -54,6 +54,14 @@ __global__ __launch_bounds__( kBlock ) void kFaceCombine( const LowerChunk* __re
    }
 }
 
+// micro-cell types WHITE_UP, BLUE_UP, GREEN_UP, WHITE_DOWN, BLUE_DOWN, GREEN_DOWN: vertex offsets
+// from the micro-cell's anchor (volumedofspace/CellDoFIndexing.hpp:155-198)
+constexpr int kMC[6][4][3] = { { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } },
+                                  { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
+                                  { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } },
+                                  { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
+                                  { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } },
+                                  { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } };
 // largest coordinate sum of a type's vertex offsets: the element anchored at a lies in the cell iff
 // a >= 0 and |a| + kMCMax[t] <= N
 constexpr int kMCMax[6] = { 1, 2, 2, 3, 2, 2 };

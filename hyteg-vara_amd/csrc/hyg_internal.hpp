@@ -212,6 +212,24 @@ struct P2Tables
 };
 void buildP2Tables( P2Tables& t );
 
+// Xiao-Gimbutas degree-4 rule with 11 points of p2_div_k_grad_affine_q4: barycentric coordinates (vertices 0..3)
+// and weights (summing to 1/6, the factors of |det J|)
+constexpr double kXG4Lambda[11][4] = {
+    { 0.17469405869723054, 0.040490506727590428, 0.01356070187980288, 0.77125473269537614 },
+    { 0.081404918402859225, 0.75250850700965499, 0.068099370938206658, 0.097987203649279112 },
+    { 0.30444840243449711, 0.24027666492807259, 0.12680172591539199, 0.32847320672203839 },
+    { 0.74122888209362259, 0.067223294893383398, 0.035183929773598722, 0.15636389323939531 },
+    { 0.053341239535745266, 0.41926631387951302, 0.047781435559086663, 0.47961101102565512 },
+    { 0.43295349048135578, 0.45076587609127677, 0.059456616299433829, 0.056824017127933668 },
+    { 0.53800720391618573, 0.12941137378891041, 0.33019041483746447, 0.0023910074574393651 },
+    { 0.008991260093335729, 0.1215419913339278, 0.30649398842969028, 0.56297276014304609 },
+    { 0.10660417256199362, 0.097204644587583267, 0.68439041545304002, 0.11180076739738309 },
+    { 0.32923295974264682, 0.029569495206479609, 0.31790356021339461, 0.32329398483747901 },
+    { 0.10384411641099309, 0.43271023904776862, 0.35382323920929709, 0.1096224053319412 } };
+constexpr double kXG4Weight[11] = { 0.006541848487473325,  0.0092122281926561474, 0.032235135341605742, 0.0092322998119293929,
+                                    0.0099888641910932524, 0.011578327656272558,  0.012693785874259723, 0.013237780011337548,
+                                    0.017744672359248346,  0.018372372071416277,  0.025829352669374347 };
+
 // ------------------------------------------------------------------ host objects
 struct Mesh
 {
@@ -314,7 +332,14 @@ struct hyg_op_s
    // 0: constant-coefficient stencils (P1ConstantLaplaceOperator)
    // 1: elementwise -div(k grad) with k = k0 exp( sum_m a_m cos(2 pi kappa_m . x + phi_m) )
    // 2: P2 constant-coefficient -Laplace (P2ConstantLaplaceOperator), element by element
+   // 3: P2 elementwise -div(k grad) (p2_div_k_grad_affine_q4, 11-point rule), k of P2Coefficient
    int                   kind = 0;
+   struct P2Coefficient
+   {
+      int      kind  = 0; // 0: k0 exp(alpha (1 - |x|)) (1 + beta xi_e), 1: 1 + x/2 + y^2/4
+      double   k0    = 1.0, alpha = 0.0, beta = 0.0;
+      uint64_t seed  = 0;
+   } p2k;
    double                k0   = 1.0;
    std::vector< double > modesHost; // nModes x 5: kappa_x, kappa_y, kappa_z, phi, a
    double*               modes  = nullptr;
@@ -343,9 +368,13 @@ struct hyg_op_s
       float*                lowPartF = nullptr;
       double*               dinv    = nullptr;
       float*                dinvF   = nullptr;
-      // P2: element matrices (6 types x 10 x 10 per cell) of the micro-cells of level lattice-1
+      // P2: element matrices (6 types x 10 x 10 per cell) of the micro-cells of level lattice-1;
+      // variable coefficient: per cell and type the 11 quadrature-point matrices w_q |det J| grad phi grad phi^T
+      // (11 x 10 x 10) and per micro-cell the coefficient at its 11 quadrature points
       double*               p2el    = nullptr;
       float*                p2elF   = nullptr;
+      double*               p2kq    = nullptr;
+      float*                p2kqF   = nullptr;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

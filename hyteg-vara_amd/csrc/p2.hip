@@ -13,13 +13,83 @@
 // macro-cell that have it as a node, the row of the cell-type's 10x10 element matrix times the ten node
 // values. Cell-interior points get the mode's epilogue; points on faces, edges and vertices store the
 // cell's one-sided sum for kFaceCombine / kLowCombine, exactly as the P1 elementwise operator does.
+//
+// Variable coefficient (P2ElementwiseOperator< p2_div_k_grad_affine_q4 >, P2ElementwiseOperator.cpp:100-310):
+// the element matrix of micro-cell e of type t is sum_q k_e,q M_t,q with the 11 quadrature-point matrices
+// M_t,q = w_q |det J| grad phi(l_q) grad phi(l_q)^T of the cell's type t (affine map: the same for every
+// micro-cell of that type) and the coefficient k_e,q at the quadrature points, evaluated once at operator
+// setup (kP2KQ). A point assembles the rows it needs on the fly.
 #include "face_combine.hpp"
 
 namespace hyg {
 
+// ------------------------------------------------------------------ variable coefficient at the quadrature points
+// k(x) = k0 exp(alpha (1 - |x|)) (1 + beta xi_e), xi_e uniform in [-1, 1) from SplitMix64(seed, element id) (kind 0),
+// or 1 + x/2 + y^2/4 (kind 1)
+struct P2Coef
+{
+   int      kind;
+   double   k0, alpha, beta;
+   uint64_t seed;
+};
+
+__device__ __forceinline__ double p2CoefAt( const P2Coef& k, double x, double y, double z, uint64_t eid )
+{
+   if ( k.kind == 1 )
+      return 1.0 + 0.5 * x + 0.25 * y * y;
+   uint64_t h = k.seed + ( eid + 1 ) * 0x9E3779B97F4A7C15ull;
+   h          = ( h ^ ( h >> 30 ) ) * 0xBF58476D1CE4E5B9ull;
+   h          = ( h ^ ( h >> 27 ) ) * 0x94D049BB133111EBull;
+   h ^= h >> 31;
+   const double xi = 2.0 * ( double( h >> 11 ) * 0x1.0p-53 ) - 1.0;
+   return k.k0 * exp( k.alpha * ( 1.0 - sqrt( x * x + y * y + z * z ) ) ) * ( 1.0 + k.beta * xi );
+}
+
+// One thread per anchor of the micro-cell lattice (level L-1) of a macro-cell; the valid types of that anchor.
+// Element id (6 c + t) S3(Nc + 1) + index(Nc + 1, anchor); kq[11 id + q].
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kP2KQ( int L, int tilesPerCell, const ElemCell* __restrict__ cells, P2Coef coef,
+                                                   T* __restrict__ kq )
+{
+   const int c    = int( blockIdx.x ) / tilesPerCell;
+   const int tile = int( blockIdx.x ) - c * tilesPerCell;
+   const int Nc = 1 << ( L - 1 ), SA = S3i( Nc + 1 );
+   const int s = tile * kBlock + int( threadIdx.x );
+   if ( s >= SA )
+      return;
+   int ax, ay, az;
+   tetXYZ( Nc + 1, s, ax, ay, az );
+   const ElemCell& ec = cells[c];
+   for ( int t = 0; t < 6; ++t )
+   {
+      if ( ax + ay + az + kMCMax[t] > Nc )
+         continue;
+      double X[4][3];
+#pragma unroll
+      for ( int v = 0; v < 4; ++v )
+      {
+         const double i = ax + kMC[t][v][0], j = ay + kMC[t][v][1], k = az + kMC[t][v][2];
+#pragma unroll
+         for ( int r = 0; r < 3; ++r )
+            X[v][r] = ec.v0[r] + ec.a[r][0] * i + ec.a[r][1] * j + ec.a[r][2] * k;
+      }
+      const uint64_t eid = uint64_t( c * 6 + t ) * uint64_t( SA ) + uint64_t( s );
+      for ( int q = 0; q < 11; ++q )
+      {
+         double x[3];
+#pragma unroll
+         for ( int r = 0; r < 3; ++r )
+            x[r] = kXG4Lambda[q][0] * X[0][r] + kXG4Lambda[q][1] * X[1][r] + kXG4Lambda[q][2] * X[2][r] + kXG4Lambda[q][3] * X[3][r];
+         kq[11 * eid + q] = T( p2CoefAt( coef, x[0], x[1], x[2], eid ) );
+      }
+   }
+}
+
 // ------------------------------------------------------------------ operator application
-template < typename T, int MODE, bool DIAG >
+// KQ: variable coefficient (el = 11 quadrature-point matrices per cell and type, kq per micro-cell)
+template < typename T, int MODE, bool DIAG, bool KQ >
 __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i64 stride, const T* __restrict__ el,
+                                                      const T* __restrict__ kq,
                                                       const P2Tables* __restrict__ tb, const CellInfo* __restrict__ cinfo,
                                                       const CellFaceSlot* __restrict__ cfs, const LowerPrim* __restrict__ prims,
                                                       const LowerEntry* __restrict__ entries, T* __restrict__ part,
@@ -39,7 +109,8 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
    if ( slot == 0 && !( flag & 1 ) )
       return;
    const int cls = ( px & 1 ) | ( ( py & 1 ) << 1 ) | ( ( pz & 1 ) << 2 );
-   const T*  ec  = el + i64( c ) * 600;
+   const T*  ec  = el + i64( c ) * ( KQ ? 6600 : 600 );
+   const int SA  = S3i( Nc + 1 );
    const T*  xc  = x + i64( c ) * stride;
    T         val = T( 0 );
    const int ne  = tb->nEnt[cls];
@@ -51,14 +122,44 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
       const int ax = ( px - tb->nd[t][l][0] ) >> 1, ay = ( py - tb->nd[t][l][1] ) >> 1, az = ( pz - tb->nd[t][l][2] ) >> 1;
       if ( ax < 0 || ay < 0 || az < 0 || ax + ay + az + kMCMax[t] > Nc )
          continue;
-      const T* row = ec + 100 * t + 10 * l;
-      if constexpr ( DIAG )
-         val += row[l];
+      if constexpr ( KQ )
+      {
+         const T* M  = ec + 1100 * t + 10 * l; // M[100 q + j]: row l of quadrature-point matrix q
+         const T* kv = kq + 11 * ( i64( c * 6 + t ) * SA + cidx( Nc + 1, ax, ay, az ) );
+         T        kk[11];
+#pragma unroll
+         for ( int q = 0; q < 11; ++q )
+            kk[q] = kv[q];
+         if constexpr ( DIAG )
+         {
+#pragma unroll
+            for ( int q = 0; q < 11; ++q )
+               val += kk[q] * M[100 * q + l];
+         }
+         else
+         {
+#pragma unroll
+            for ( int j = 0; j < 10; ++j )
+            {
+               T r = T( 0 );
+#pragma unroll
+               for ( int q = 0; q < 11; ++q )
+                  r += kk[q] * M[100 * q + j];
+               val += r * xc[cidx( w, 2 * ax + tb->nd[t][j][0], 2 * ay + tb->nd[t][j][1], 2 * az + tb->nd[t][j][2] )];
+            }
+         }
+      }
       else
       {
+         const T* row = ec + 100 * t + 10 * l;
+         if constexpr ( DIAG )
+            val += row[l];
+         else
+         {
 #pragma unroll
-         for ( int j = 0; j < 10; ++j )
-            val += row[j] * xc[cidx( w, 2 * ax + tb->nd[t][j][0], 2 * ay + tb->nd[t][j][1], 2 * az + tb->nd[t][j][2] )];
+            for ( int j = 0; j < 10; ++j )
+               val += row[j] * xc[cidx( w, 2 * ax + tb->nd[t][j][0], 2 * ay + tb->nd[t][j][1], 2 * az + tb->nd[t][j][2] )];
+         }
       }
    }
    const i64 q = i64( c ) * stride + cidx( w, px, py, pz );
@@ -263,19 +364,20 @@ __global__ __launch_bounds__( kBlock ) void kP2RestrictLower( const LowerChunk* 
 }
 
 // ------------------------------------------------------------------ launch
-template < typename T, int MODE, bool DIAG >
-static void p2T( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s )
+template < typename T, int MODE, bool DIAG, bool KQ >
+static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s )
 {
    constexpr bool         dbl  = std::is_same< T, double >::value;
    const hyg_op_s::Level& ol   = op.levels[L];
    const int              nC   = m.mesh->nC();
    const int              w    = ( 1 << L ) + 1;
    const T*               el   = dbl ? (const T*) ol.p2el : (const T*) ol.p2elF;
+   const T*               kq   = dbl ? (const T*) ol.p2kq : (const T*) ol.p2kqF;
    T*                     part = dbl ? (T*) ol.facePart : (T*) ol.facePartF;
    T*                     low  = dbl ? (T*) ol.lowPart : (T*) ol.lowPartF;
    const T*               dv   = DIAG ? nullptr : ( dbl ? (const T*) ol.dinv : (const T*) ol.dinvF );
    const int              tA   = ceilDiv( S3i( w ), kBlock );
-   hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, el, m.p2tab,
+   hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, el, kq, m.p2tab,
                        m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x, (const T*) c.in1,
                        (T*) c.out0, (T*) c.out1, T( c.a ) );
    const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
@@ -288,6 +390,15 @@ static void p2T( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, cons
       hipLaunchKernelGGL( ( kLowCombine< T, MODE > ), dim3( ceilDiv( nLow, kBlock / 64 ) ), dim3( kBlock ), 0, s, g.chunks, nLow,
                           m.prims, m.entries, low, dv, L, g.stride, c.flag, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
                           (T*) c.out1, T( c.a ) );
+}
+
+template < typename T, int MODE, bool DIAG >
+static void p2T( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s )
+{
+   if ( op.kind == 3 )
+      p2TK< T, MODE, DIAG, true >( m, g, op, L, c, s );
+   else
+      p2TK< T, MODE, DIAG, false >( m, g, op, L, c, s );
 }
 
 template < typename T >
@@ -322,6 +433,17 @@ void launchP2( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dt
       p2Mode< double >( m, g, op, L, c, s );
    else
       p2Mode< float >( m, g, op, L, c, s );
+}
+
+// coefficient values of every micro-cell at its 11 quadrature points (variable-coefficient P2 operator)
+void launchP2Coefficients( const MeshCtx& m, const hyg_op_s& op, int L, hipStream_t s )
+{
+   const hyg_op_s::Level& ol = op.levels[L];
+   const int              nC = m.mesh->nC();
+   const int              tA = ceilDiv( S3i( ( 1 << ( L - 1 ) ) + 1 ), kBlock );
+   const P2Coef           k{ op.p2k.kind, op.p2k.k0, op.p2k.alpha, op.p2k.beta, op.p2k.seed };
+   hipLaunchKernelGGL( ( kP2KQ< double > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, ol.elem, k, ol.p2kq );
+   hipLaunchKernelGGL( ( kP2KQ< float > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, ol.elem, k, ol.p2kqF );
 }
 
 // inverse diagonal of a P2 operator at lattice level L into its per-point arrays (FP64 and FP32 copy)

@@ -744,6 +744,94 @@ void assembleP2Const( hyg_op_s& op )
    }
 }
 
+// P2ElementwiseOperator< p2_div_k_grad_affine_q4 >: per lattice level and per cell and type the 11 quadrature-point
+// matrices w_q |det J| grad phi(l_q) grad phi(l_q)^T of the micro-cells of level lattice-1, the cell geometry
+// for evaluating the coefficient, and the buffers (coefficient values, partial sums, inverse diagonal) filled
+// on the device.
+void assembleP2DivKGrad( hyg_op_s& op )
+{
+   const MeshCtx& mc = op.mesh->ctx;
+   const Mesh&    m  = *mc.mesh;
+   op.levels.resize( op.maxL + 1 );
+   auto alloc = []( size_t bytes ) {
+      void* d = nullptr;
+      if ( bytes > 0 && hipMalloc( &d, bytes ) != hipSuccess )
+         throw std::runtime_error( "hipMalloc failed for the P2 variable-coefficient operator" );
+      return d;
+   };
+   for ( int L = op.minL; L <= op.maxL; ++L )
+   {
+      hyg_op_s::Level& ol = op.levels[L];
+      const int        N  = 1 << L;
+      const int        Ne = N / 2;
+      std::vector< double >   el( size_t( m.nC() ) * 6600 );
+      std::vector< ElemCell > ec( m.nC() );
+      for ( int c = 0; c < m.nC(); ++c )
+      {
+         const auto& cv = m.Cv[c];
+         const V3&   P0 = m.X[cv[0]];
+         const V3    a1 = sub( m.X[cv[1]], P0 ), a2 = sub( m.X[cv[2]], P0 ), a3 = sub( m.X[cv[3]], P0 );
+         for ( int r = 0; r < 3; ++r )
+         {
+            ec[c].v0[r]   = P0[r];
+            ec[c].a[r][0] = a1[r] / double( Ne );
+            ec[c].a[r][1] = a2[r] / double( Ne );
+            ec[c].a[r][2] = a3[r] / double( Ne );
+         }
+         for ( int t = 0; t < 6; ++t )
+         {
+            V3 P[4];
+            for ( int k = 0; k < 4; ++k )
+               for ( int r = 0; r < 3; ++r )
+                  P[k][r] = ( kMicroTet[t][k][0] * a1[r] + kMicroTet[t][k][1] * a2[r] + kMicroTet[t][k][2] * a3[r] ) / double( Ne );
+            const double adet = std::fabs( dot3( sub( P[1], P[0] ), cross( sub( P[2], P[0] ), sub( P[3], P[0] ) ) ) );
+            V3           g[4];
+            for ( int i = 0; i < 4; ++i )
+            {
+               const int    a = ( i + 1 ) % 4, b = ( i + 2 ) % 4, d = ( i + 3 ) % 4;
+               const V3     nrm = cross( sub( P[b], P[a] ), sub( P[d], P[a] ) );
+               const double sc  = dot3( nrm, sub( P[i], P[a] ) );
+               for ( int r = 0; r < 3; ++r )
+                  g[i][r] = nrm[r] / sc;
+            }
+            for ( int q = 0; q < 11; ++q )
+            {
+               const double* lam = kXG4Lambda[q];
+               V3            d[10];
+               for ( int v = 0; v < 4; ++v )
+                  for ( int r = 0; r < 3; ++r )
+                     d[v][r] = ( 4.0 * lam[v] - 1.0 ) * g[v][r];
+               for ( int e = 0; e < 6; ++e )
+               {
+                  const int pa = kP2Edge[e][0], pb = kP2Edge[e][1];
+                  for ( int r = 0; r < 3; ++r )
+                     d[4 + e][r] = 4.0 * ( lam[pa] * g[pb][r] + lam[pb] * g[pa][r] );
+               }
+               for ( int i = 0; i < 10; ++i )
+                  for ( int j = 0; j < 10; ++j )
+                     el[size_t( c ) * 6600 + 1100 * t + 100 * q + 10 * i + j] = kXG4Weight[q] * adet * dot3( d[i], d[j] );
+            }
+         }
+      }
+      std::vector< float > elF( el.begin(), el.end() );
+      ol.p2el              = upload( el );
+      ol.p2elF             = upload( elF );
+      ol.elem              = upload( ec );
+      const size_t nKq     = size_t( m.nC() ) * 6 * size_t( S3( Ne + 1 ) ) * 11;
+      const size_t nLow    = mc.entriesHost.size() * size_t( N + 1 );
+      const size_t nFace   = size_t( m.nF() ) * 2 * size_t( N + 1 ) * size_t( N + 1 );
+      const size_t nFunc   = size_t( m.nC() ) * size_t( cellStride( L ) ) + size_t( kTailSlack );
+      ol.p2kq      = static_cast< double* >( alloc( nKq * sizeof( double ) ) );
+      ol.p2kqF     = static_cast< float* >( alloc( nKq * sizeof( float ) ) );
+      ol.lowPart   = static_cast< double* >( alloc( nLow * sizeof( double ) ) );
+      ol.lowPartF  = static_cast< float* >( alloc( nLow * sizeof( float ) ) );
+      ol.facePart  = static_cast< double* >( alloc( nFace * sizeof( double ) ) );
+      ol.facePartF = static_cast< float* >( alloc( nFace * sizeof( float ) ) );
+      ol.dinv      = static_cast< double* >( alloc( nFunc * sizeof( double ) ) );
+      ol.dinvF     = static_cast< float* >( alloc( nFunc * sizeof( float ) ) );
+   }
+}
+
 } // namespace hyg
 
 hyg_op_s::~hyg_op_s()
@@ -767,6 +855,8 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.dinvF );
       (void) hipFree( l.p2el );
       (void) hipFree( l.p2elF );
+      (void) hipFree( l.p2kq );
+      (void) hipFree( l.p2kqF );
    }
    (void) hipFree( modes );
 }

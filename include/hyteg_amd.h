@@ -128,6 +128,16 @@ int hyg_op_create_p1_divkgrad( hyg_mesh mesh, int min_level, int max_level, doub
  * hyg_restrict / hyg_prolongate on P2 functions (quadratic transfers) it supports hyg_apply, hyg_smooth_jac,
  * hyg_residual, hyg_inverse_diagonal, hyg_power_iteration, GMG and defect correction in FP64 and FP32. */
 int hyg_op_create_p2_const( hyg_mesh mesh, int min_level, int max_level, hyg_op* out );
+/* Replaces P2ElementwiseOperator< forms::p2_div_k_grad_affine_q4 >(storage, minLevel, maxLevel, form)
+ * (elementwiseoperators/P2ElementwiseOperator.cpp:100-310; forms/form_hyteg_generated/p2/p2_div_k_grad_affine_q4.cpp,
+ * Xiao-Gimbutas 11-point rule) for -div(k grad u) on P2 functions, with the coefficient
+ *   kind 0: k(x) = k0 exp(alpha (1 - |x|)) (1 + beta xi_e), xi_e uniform in [-1, 1) per micro-element from
+ *           SplitMix64(seed + (id + 1) 0x9E3779B97F4A7C15), id = (6 cell + type) S3(Nc + 1) + anchor index;
+ *   kind 1: k(x) = 1 + x/2 + y^2/4.
+ * k is evaluated at the 11 quadrature points of every element once, at creation; each apply assembles the
+ * element rows on the fly. Supports what hyg_op_create_p2_const supports. */
+int hyg_op_create_p2_divkgrad( hyg_mesh mesh, int min_level, int max_level, int kind, double k0, double alpha, double beta,
+                               uint64_t seed, hyg_op* out );
 /* Cell stencil (15 weights in the order C W E N S NW SE TC TW TS TSE BC BN BE BNW) of cell `cell`
  * (constant-coefficient operators only). */
 int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 );
