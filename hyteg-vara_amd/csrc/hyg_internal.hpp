@@ -212,6 +212,62 @@ struct P2Tables
 };
 void buildP2Tables( P2Tables& t );
 
+// micro-cell types WHITE_UP, BLUE_UP, GREEN_UP, WHITE_DOWN, BLUE_DOWN, GREEN_DOWN: vertex offsets from the anchor
+// (volumedofspace/CellDoFIndexing.hpp:155-198)
+constexpr int kMicroCell[6][4][3] = { { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } },
+                                      { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 }, { 1, 0, 1 } },
+                                      { { 1, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 0, 0, 1 } },
+                                      { { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 }, { 1, 0, 1 } },
+                                      { { 1, 0, 1 }, { 0, 1, 1 }, { 0, 0, 1 }, { 0, 1, 0 } },
+                                      { { 0, 1, 0 }, { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 } } };
+constexpr int kP2EdgeNodes[6][2] = { { 2, 3 }, { 1, 3 }, { 1, 2 }, { 0, 3 }, { 0, 2 }, { 0, 1 } };
+
+// lattice offset of P2 node l of a micro-cell of type t from 2 * anchor
+constexpr int p2NodeOffset( int t, int l, int c )
+{
+   return l < 4 ? 2 * kMicroCell[t][l][c] : kMicroCell[t][kP2EdgeNodes[l - 4][0]][c] + kMicroCell[t][kP2EdgeNodes[l - 4][1]][c];
+}
+constexpr int p2NodeClass( int t, int l )
+{
+   return ( p2NodeOffset( t, l, 0 ) & 1 ) | ( ( p2NodeOffset( t, l, 1 ) & 1 ) << 1 ) | ( ( p2NodeOffset( t, l, 2 ) & 1 ) << 2 );
+}
+
+// Merged P2 stencil of a cell-interior point of parity class cls: the distinct offsets (within [-2, 2]^3) to
+// the nodes of all micro-cells that have the point as a node, in order of first appearance over (t, l, j).
+// Sizes 65 (vertex DoFs), 27 or 19 (edge DoFs).
+struct P2StencilOffsets
+{
+   int n = 0;
+   int d[65][3] = {};
+};
+constexpr P2StencilOffsets p2StencilOffsets( int cls )
+{
+   P2StencilOffsets s;
+   for ( int t = 0; t < 6; ++t )
+      for ( int l = 0; l < 10; ++l )
+      {
+         if ( p2NodeClass( t, l ) != cls )
+            continue;
+         for ( int j = 0; j < 10; ++j )
+         {
+            const int dx = p2NodeOffset( t, j, 0 ) - p2NodeOffset( t, l, 0 ), dy = p2NodeOffset( t, j, 1 ) - p2NodeOffset( t, l, 1 ),
+                      dz = p2NodeOffset( t, j, 2 ) - p2NodeOffset( t, l, 2 );
+            bool seen = false;
+            for ( int e = 0; e < s.n; ++e )
+               seen = seen || ( s.d[e][0] == dx && s.d[e][1] == dy && s.d[e][2] == dz );
+            if ( !seen )
+            {
+               s.d[s.n][0] = dx;
+               s.d[s.n][1] = dy;
+               s.d[s.n][2] = dz;
+               ++s.n;
+            }
+         }
+      }
+   return s;
+}
+constexpr int kP2StencilMax = 65;
+
 // Xiao-Gimbutas degree-4 rule with 11 points of p2_div_k_grad_affine_q4: barycentric coordinates (vertices 0..3)
 // and weights (summing to 1/6, the factors of |det J|)
 constexpr double kXG4Lambda[11][4] = {
@@ -375,6 +431,10 @@ struct hyg_op_s
       float*                p2elF   = nullptr;
       double*               p2kq    = nullptr;
       float*                p2kqF   = nullptr;
+      // constant P2 operator: merged stencil weights of cell-interior points, per cell and parity class
+      // (8 x kP2StencilMax, offsets of p2StencilOffsets)
+      double*               p2st    = nullptr;
+      float*                p2stF   = nullptr;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

@@ -87,7 +87,7 @@ __global__ __launch_bounds__( kBlock ) void kP2KQ( int L, int tilesPerCell, cons
 
 // ------------------------------------------------------------------ operator application
 // KQ: variable coefficient (el = 11 quadrature-point matrices per cell and type, kq per micro-cell)
-template < typename T, int MODE, bool DIAG, bool KQ >
+template < typename T, int MODE, bool DIAG, bool KQ, bool SKIPIN = false >
 __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i64 stride, const T* __restrict__ el,
                                                       const T* __restrict__ kq,
                                                       const P2Tables* __restrict__ tb, const CellInfo* __restrict__ cinfo,
@@ -106,7 +106,7 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
    int px, py, pz;
    tetXYZ( w, s, px, py, pz );
    const int slot = classifySlot( N, px, py, pz );
-   if ( slot == 0 && !( flag & 1 ) )
+   if ( slot == 0 && ( SKIPIN || !( flag & 1 ) ) )
       return;
    const int cls = ( px & 1 ) | ( ( py & 1 ) << 1 ) | ( ( pz & 1 ) << 2 );
    const T*  ec  = el + i64( c ) * ( KQ ? 6600 : 600 );
@@ -200,6 +200,95 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
          }
          lowPart[i64( e ) * NP + i] = val;
       }
+   }
+}
+
+// ------------------------------------------------------------------ constant operator: cell-interior points
+// The cell-interior points of one parity class CLS, p = 2 q + CLS, enumerated as the tetrahedral lattice of
+// q' = q - (1 - CLS) with |q'| <= M = (N - 1 - |CLS|) / 2 - (3 - |CLS|). Every micro-cell at such a point lies in
+// the cell, so the merged stencil (offsets known at compile time, weights per cell) applies unchanged and all
+// lanes of a block run the same instruction stream.
+__host__ __device__ constexpr int p2InteriorExtent( int N, int cls )
+{
+   const int k = ( cls & 1 ) + ( ( cls >> 1 ) & 1 ) + ( ( cls >> 2 ) & 1 );
+   return ( N - 1 - k ) / 2 - ( 3 - k );
+}
+
+template < typename T, int MODE, int CLS >
+__device__ __forceinline__ void p2InteriorClass( int c, int s, int L, i64 stride, const T* __restrict__ st, int flag,
+                                                 const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
+                                                 T* __restrict__ out1, const T* __restrict__ dinv, T a )
+{
+   constexpr int               kx = CLS & 1, ky = ( CLS >> 1 ) & 1, kz = ( CLS >> 2 ) & 1;
+   constexpr P2StencilOffsets  off = p2StencilOffsets( CLS );
+   const int                   N = 1 << L, w = N + 1;
+   const int                   M = p2InteriorExtent( N, CLS );
+   if ( M < 0 || s >= S3i( M + 1 ) || !( flag & 1 ) )
+      return;
+   int qx, qy, qz;
+   tetXYZ( M + 1, s, qx, qy, qz );
+   const int px = 2 * ( qx + 1 - kx ) + kx, py = 2 * ( qy + 1 - ky ) + ky, pz = 2 * ( qz + 1 - kz ) + kz;
+   const T*  xc = x + i64( c ) * stride;
+   const T*  wc = st + i64( c * 8 + CLS ) * kP2StencilMax;
+   // start of row py+dy in plane pz+dz
+   int rb[5][5];
+#pragma unroll
+   for ( int dz = -2; dz <= 2; ++dz )
+   {
+      const int W  = w - ( pz + dz );
+      const int pb = S3i( w ) - S3i( W );
+#pragma unroll
+      for ( int dy = -2; dy <= 2; ++dy )
+         rb[dz + 2][dy + 2] = pb + rowStart( W, py + dy );
+   }
+   T val = T( 0 );
+#pragma unroll
+   for ( int e = 0; e < off.n; ++e )
+      val += wc[e] * xc[rb[off.d[e][2] + 2][off.d[e][1] + 2] + px + off.d[e][0]];
+   const i64 q = i64( c ) * stride + rb[2][2] + px;
+   T         i1 = T( 0 ), r0, r1;
+   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      i1 = in1[q];
+   if constexpr ( MODE == MODE_APPLY_ADD )
+      i1 = out0[q];
+   T xv = T( 0 ), dv = T( 1 );
+   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
+      xv = x[q];
+   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      dv = dinv[q];
+   epilogue< T, MODE >( val, xv, i1, i1, dv, a, r0, r1 );
+   out0[q] = r0;
+   if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      out1[q] = r1;
+}
+
+struct P2ClassTiles
+{
+   int start[9]; // first tile of each class within a cell; start[8] = tiles per cell
+};
+
+template < typename T, int MODE >
+__global__ __launch_bounds__( kBlock ) void kP2Interior( int L, i64 stride, P2ClassTiles ct, const T* __restrict__ st, int flag,
+                                                         const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
+                                                         T* __restrict__ out1, const T* __restrict__ dinv, T a )
+{
+   const int per = ct.start[8];
+   const int c   = int( blockIdx.x ) / per;
+   const int r   = int( blockIdx.x ) - c * per;
+   int       cls = 0;
+   while ( cls < 7 && r >= ct.start[cls + 1] )
+      ++cls;
+   const int s = ( r - ct.start[cls] ) * kBlock + int( threadIdx.x );
+   switch ( cls )
+   {
+   case 0: p2InteriorClass< T, MODE, 0 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
+   case 1: p2InteriorClass< T, MODE, 1 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
+   case 2: p2InteriorClass< T, MODE, 2 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
+   case 3: p2InteriorClass< T, MODE, 3 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
+   case 4: p2InteriorClass< T, MODE, 4 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
+   case 5: p2InteriorClass< T, MODE, 5 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
+   case 6: p2InteriorClass< T, MODE, 6 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
+   default: p2InteriorClass< T, MODE, 7 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
    }
 }
 
@@ -377,6 +466,25 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
    T*                     low  = dbl ? (T*) ol.lowPart : (T*) ol.lowPartF;
    const T*               dv   = DIAG ? nullptr : ( dbl ? (const T*) ol.dinv : (const T*) ol.dinvF );
    const int              tA   = ceilDiv( S3i( w ), kBlock );
+   if constexpr ( !KQ && !DIAG )
+   {
+      // constant coefficient: cell interiors by merged stencils, the rest element by element
+      P2ClassTiles ct;
+      ct.start[0] = 0;
+      for ( int k = 0; k < 8; ++k )
+      {
+         const int M     = p2InteriorExtent( 1 << L, k );
+         ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
+      }
+      const T* stw = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
+      if ( ct.start[8] > 0 )
+         hipLaunchKernelGGL( ( kP2Interior< T, MODE > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, stw, c.flag,
+                             (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
+      hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, el, kq,
+                          m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
+                          (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
+   }
+   else
    hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, el, kq, m.p2tab,
                        m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x, (const T*) c.in1,
                        (T*) c.out0, (T*) c.out1, T( c.a ) );
