@@ -87,6 +87,46 @@ __global__ __launch_bounds__( kBlock ) void kP2KQ( int L, int tilesPerCell, cons
 
 // ------------------------------------------------------------------ operator application
 // KQ: variable coefficient (el = 11 quadrature-point matrices per cell and type, kq per micro-cell)
+// points of the four faces of a cell lattice with N intervals per edge: z = 0; y = 0 and z >= 1; x = 0 and
+// y, z >= 1; x + y + z = N and x, y, z >= 1 (triangles of N + 1, N, N - 1 and N - 2 points per edge)
+__host__ __device__ constexpr int triCount( int n )
+{
+   return n > 0 ? n * ( n + 1 ) / 2 : 0;
+}
+__host__ __device__ constexpr int p2BoundaryCount( int N )
+{
+   return triCount( N + 1 ) + triCount( N ) + triCount( N - 1 ) + triCount( N - 2 );
+}
+__device__ __forceinline__ bool boundaryXYZ( int N, int s, int& x, int& y, int& z )
+{
+   int a, b;
+   const int n0 = triCount( N + 1 ), n1 = triCount( N ), n2 = triCount( N - 1 ), n3 = triCount( N - 2 );
+   if ( s < n0 )
+   {
+      planeXY( N + 1, s, a, b );
+      x = a, y = b, z = 0;
+   }
+   else if ( ( s -= n0 ) < n1 )
+   {
+      planeXY( N, s, a, b );
+      x = a, y = 0, z = b + 1;
+   }
+   else if ( ( s -= n1 ) < n2 )
+   {
+      planeXY( N - 1, s, a, b );
+      x = 0, y = a + 1, z = b + 1;
+   }
+   else if ( ( s -= n2 ) < n3 )
+   {
+      planeXY( N - 2, s, a, b );
+      x = a + 1, y = b + 1, z = N - x - y;
+   }
+   else
+      return false;
+   return true;
+}
+
+// SKIPIN: the cell-interior points are done by another kernel; the grid then covers only the face points
 template < typename T, int MODE, bool DIAG, bool KQ, bool SKIPIN = false >
 __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i64 stride, const T* __restrict__ el,
                                                       const T* __restrict__ kq,
@@ -101,10 +141,18 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
    const int tile = int( blockIdx.x ) - c * tilesPerCell;
    const int N = 1 << L, w = N + 1, NP = N + 1, Nc = N >> 1;
    const int s = tile * kBlock + int( threadIdx.x );
-   if ( s >= S3i( w ) )
-      return;
-   int px, py, pz;
-   tetXYZ( w, s, px, py, pz );
+   int       px, py, pz;
+   if constexpr ( SKIPIN )
+   {
+      if ( !boundaryXYZ( N, s, px, py, pz ) )
+         return;
+   }
+   else
+   {
+      if ( s >= S3i( w ) )
+         return;
+      tetXYZ( w, s, px, py, pz );
+   }
    const int slot = classifySlot( N, px, py, pz );
    if ( slot == 0 && ( SKIPIN || !( flag & 1 ) ) )
       return;
@@ -480,7 +528,8 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
       if ( ct.start[8] > 0 )
          hipLaunchKernelGGL( ( kP2Interior< T, MODE > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, stw, c.flag,
                              (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
-      hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, el, kq,
+      const int tB = ceilDiv( p2BoundaryCount( 1 << L ), kBlock );
+      hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq,
                           m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
                           (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
    }
