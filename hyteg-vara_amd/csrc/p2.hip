@@ -252,92 +252,176 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
 }
 
 // ------------------------------------------------------------------ constant operator: cell-interior points
-// The cell-interior points of one parity class CLS, p = 2 q + CLS, enumerated as the tetrahedral lattice of
-// q' = q - (1 - CLS) with |q'| <= M = (N - 1 - |CLS|) / 2 - (3 - |CLS|). Every micro-cell at such a point lies in
-// the cell, so the merged stencil (offsets known at compile time, weights per cell) applies unchanged and all
-// lanes of a block run the same instruction stream.
-__host__ __device__ constexpr int p2InteriorExtent( int N, int cls )
-{
-   const int k = ( cls & 1 ) + ( ( cls >> 1 ) & 1 ) + ( ( cls >> 2 ) & 1 );
-   return ( N - 1 - k ) / 2 - ( 3 - k );
-}
+// One wave computes the interior points of one lattice row (y, z) in a segment of 124 x values. Lane i holds the
+// pair x0 + 2i (even x) and x0 + 2i + 1 (odd x), x0 = 124 k - 2; the even points share one parity class cE and the
+// odd points the class cE | 1, so both merged stencils have offsets known at compile time and weights that are
+// the same for every lane of the wave (scalar loads). Each stencil row (dy, dz) is read once, as one pair load per
+// lane; a value at x + dx reaches the lane that needs it through partial sums: the terms a lane's values add to
+// its left and right neighbours are summed in place and shifted once per wave (lanes 0 and 63 are halo only).
+constexpr int kP2RowSeg = 124;
 
-template < typename T, int MODE, int CLS >
-__device__ __forceinline__ void p2InteriorClass( int c, int s, int L, i64 stride, const T* __restrict__ st, int flag,
-                                                 const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
-                                                 T* __restrict__ out1, const T* __restrict__ dinv, T a )
+struct P2RowSet
 {
-   constexpr int               kx = CLS & 1, ky = ( CLS >> 1 ) & 1, kz = ( CLS >> 2 ) & 1;
-   constexpr P2StencilOffsets  off = p2StencilOffsets( CLS );
-   const int                   N = 1 << L, w = N + 1;
-   const int                   M = p2InteriorExtent( N, CLS );
-   if ( M < 0 || s >= S3i( M + 1 ) || !( flag & 1 ) )
-      return;
-   int qx, qy, qz;
-   tetXYZ( M + 1, s, qx, qy, qz );
-   const int px = 2 * ( qx + 1 - kx ) + kx, py = 2 * ( qy + 1 - ky ) + ky, pz = 2 * ( qz + 1 - kz ) + kz;
-   const T*  xc = x + i64( c ) * stride;
-   const T*  wc = st + i64( c * 8 + CLS ) * kP2StencilMax;
-   // start of row py+dy in plane pz+dz
-   int rb[5][5];
-#pragma unroll
-   for ( int dz = -2; dz <= 2; ++dz )
-   {
-      const int W  = w - ( pz + dz );
-      const int pb = S3i( w ) - S3i( W );
-#pragma unroll
-      for ( int dy = -2; dy <= 2; ++dy )
-         rb[dz + 2][dy + 2] = pb + rowStart( W, py + dy );
-   }
-   T val = T( 0 );
-#pragma unroll
-   for ( int e = 0; e < off.n; ++e )
-      val += wc[e] * xc[rb[off.d[e][2] + 2][off.d[e][1] + 2] + px + off.d[e][0]];
-   const i64 q = i64( c ) * stride + rb[2][2] + px;
-   T         i1 = T( 0 ), r0, r1;
-   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-      i1 = in1[q];
-   if constexpr ( MODE == MODE_APPLY_ADD )
-      i1 = out0[q];
-   T xv = T( 0 ), dv = T( 1 );
-   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
-      xv = x[q];
-   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-      dv = dinv[q];
-   epilogue< T, MODE >( val, xv, i1, i1, dv, a, r0, r1 );
-   out0[q] = r0;
-   if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-      out1[q] = r1;
-}
-
-struct P2ClassTiles
-{
-   int start[9]; // first tile of each class within a cell; start[8] = tiles per cell
+   int n;
+   int d[25][2]; // (dy, dz)
 };
+constexpr P2RowSet p2RowSet( int ky, int kz )
+{
+   P2RowSet               r{};
+   const P2StencilOffsets a = p2StencilOffsets( ( ky << 1 ) | ( kz << 2 ) ), b = p2StencilOffsets( ( ky << 1 ) | ( kz << 2 ) | 1 );
+   for ( int dz = -2; dz <= 2; ++dz )
+      for ( int dy = -2; dy <= 2; ++dy )
+      {
+         bool used = false;
+         for ( int e = 0; e < a.n; ++e )
+            used = used || ( a.d[e][1] == dy && a.d[e][2] == dz );
+         for ( int e = 0; e < b.n; ++e )
+            used = used || ( b.d[e][1] == dy && b.d[e][2] == dz );
+         if ( used )
+         {
+            r.d[r.n][0] = dy;
+            r.d[r.n][1] = dz;
+            ++r.n;
+         }
+      }
+   return r;
+}
+
+// first x of segment k, and the number of rows (y, z) whose interior reaches it
+__host__ __device__ constexpr int p2SegStart( int k )
+{
+   return k == 0 ? 1 : kP2RowSeg * k;
+}
+__host__ __device__ constexpr int p2SegRows( int N, int k )
+{
+   return triCount( N - 2 - p2SegStart( k ) );
+}
+
+template < typename T, int MODE, int KY, int KZ >
+__device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_buffer_rsrc_t rx, const T* __restrict__ wE,
+                                           const T* __restrict__ wO, i64 cellOff, const T* __restrict__ x, const T* __restrict__ in1,
+                                           T* __restrict__ out0, T* __restrict__ out1, const T* __restrict__ dinv, T a )
+{
+   constexpr int              cE = ( KY << 1 ) | ( KZ << 2 ), cO = cE | 1;
+   constexpr P2StencilOffsets oE = p2StencilOffsets( cE ), oO = p2StencilOffsets( cO );
+   constexpr P2RowSet         rs = p2RowSet( KY, KZ );
+   const int                  w = N + 1, lane = int( threadIdx.x ) & 63;
+   const int                  xe = kP2RowSeg * k - 2 + 2 * lane;
+   T                          aE = T( 0 ), aO = T( 0 ), rE = T( 0 ), rO = T( 0 ), lE = T( 0 ), lO = T( 0 );
+   T                          vEr[rs.n], vOr[rs.n];
+#pragma unroll
+   for ( int r = 0; r < rs.n; ++r )
+   {
+      const int      dy = rs.d[r][0], dz = rs.d[r][1];
+      const int      W = w - ( z + dz ), yy = y + dy;
+      const int      base = S3i( w ) - S3i( W ) + rowStart( W, yy );
+      const unsigned vo   = ( xe >= 0 && xe < W - yy ) ? unsigned( base + xe ) * unsigned( sizeof( T ) ) : kDropOffset;
+      bufLoad2( rx, vo, 0, vEr[r], vOr[r] );
+   }
+   // the weights are the same for every lane: read through the constant address space they become scalar loads
+   typedef const T __attribute__( ( address_space( 4 ) ) ) CT;
+   const CT* wEr = (const CT*) wE;
+   const CT* wOr = (const CT*) wO;
+#pragma unroll
+   for ( int r = 0; r < rs.n; ++r )
+   {
+      const int dy = rs.d[r][0], dz = rs.d[r][1];
+      const T   vE = vEr[r], vO = vOr[r];
+#pragma unroll
+      for ( int e = 0; e < oE.n; ++e )
+         if ( oE.d[e][1] == dy && oE.d[e][2] == dz )
+         {
+            const T wt = wEr[e];
+            switch ( oE.d[e][0] )
+            {
+            case -2: rE += wt * vE; break;
+            case -1: rE += wt * vO; break;
+            case 0: aE += wt * vE; break;
+            case 1: aE += wt * vO; break;
+            default: lE += wt * vE; break;
+            }
+         }
+#pragma unroll
+      for ( int e = 0; e < oO.n; ++e )
+         if ( oO.d[e][1] == dy && oO.d[e][2] == dz )
+         {
+            const T wt = wOr[e];
+            switch ( oO.d[e][0] )
+            {
+            case -2: rO += wt * vO; break;
+            case -1: aO += wt * vE; break;
+            case 0: aO += wt * vO; break;
+            case 1: lO += wt * vE; break;
+            default: lO += wt * vO; break;
+            }
+         }
+   }
+   const T   vals[2] = { aE + laneShift< kDppWaveShr1 >( rE ) + laneShift< kDppWaveShl1 >( lE ),
+                         aO + laneShift< kDppWaveShr1 >( rO ) + laneShift< kDppWaveShl1 >( lO ) };
+   const int xmax    = N - 1 - y - z;
+   const i64 row     = cellOff + S3i( w ) - S3i( w - z ) + rowStart( w - z, y );
+   if ( lane == 0 || lane == 63 )
+      return;
+#pragma unroll
+   for ( int h = 0; h < 2; ++h )
+   {
+      const int px = xe + h;
+      if ( px < 1 || px > xmax )
+         continue;
+      const i64 q  = row + px;
+      T         i1 = T( 0 ), r0, r1;
+      if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+         i1 = in1[q];
+      if constexpr ( MODE == MODE_APPLY_ADD )
+         i1 = out0[q];
+      T xv = T( 0 ), dv = T( 1 );
+      if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
+         xv = x[q];
+      if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+         dv = dinv[q];
+      epilogue< T, MODE >( vals[h], xv, i1, i1, dv, a, r0, r1 );
+      out0[q] = r0;
+      if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+         out1[q] = r1;
+   }
+}
 
 template < typename T, int MODE >
-__global__ __launch_bounds__( kBlock ) void kP2Interior( int L, i64 stride, P2ClassTiles ct, const T* __restrict__ st, int flag,
-                                                         const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
-                                                         T* __restrict__ out1, const T* __restrict__ dinv, T a )
+__global__ __launch_bounds__( kBlock ) void kP2Rows( int L, int nC, i64 stride, int wavesPerCell, const T* __restrict__ st,
+                                                     const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
+                                                     T* __restrict__ out1, const T* __restrict__ dinv, T a )
 {
-   const int per = ct.start[8];
-   const int c   = int( blockIdx.x ) / per;
-   const int r   = int( blockIdx.x ) - c * per;
-   int       cls = 0;
-   while ( cls < 7 && r >= ct.start[cls + 1] )
-      ++cls;
-   const int s = ( r - ct.start[cls] ) * kBlock + int( threadIdx.x );
-   switch ( cls )
+   const int N  = 1 << L;
+   const int b  = remapXcd( int( blockIdx.x ), int( gridDim.x ) );
+   const int g  = b * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
+   const int c  = g / wavesPerCell;
+   int       r  = g - c * wavesPerCell;
+   int       k  = 0;
+   if ( c >= nC )
+      return;
+   while ( r >= p2SegRows( N, k ) )
+      r -= p2SegRows( N, k++ );
+   int yp, zp;
+   planeXY( N - 2 - p2SegStart( k ), r, yp, zp );
+   const int y = yp + 1, z = zp + 1;
+   const i64 cellOff = i64( c ) * stride;
+   const auto rx      = makeRsrc( x + cellOff, unsigned( S3i( N + 1 ) ) * unsigned( sizeof( T ) ) );
+   const T*   wc      = st + i64( c ) * 8 * kP2StencilMax;
+   switch ( ( y & 1 ) | ( ( z & 1 ) << 1 ) )
    {
-   case 0: p2InteriorClass< T, MODE, 0 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
-   case 1: p2InteriorClass< T, MODE, 1 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
-   case 2: p2InteriorClass< T, MODE, 2 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
-   case 3: p2InteriorClass< T, MODE, 3 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
-   case 4: p2InteriorClass< T, MODE, 4 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
-   case 5: p2InteriorClass< T, MODE, 5 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
-   case 6: p2InteriorClass< T, MODE, 6 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
-   default: p2InteriorClass< T, MODE, 7 >( c, s, L, stride, st, flag, x, in1, out0, out1, dinv, a ); break;
+   case 0: p2RowPair< T, MODE, 0, 0 >( N, y, z, k, rx, wc, wc + kP2StencilMax, cellOff, x, in1, out0, out1, dinv, a ); break;
+   case 1: p2RowPair< T, MODE, 1, 0 >( N, y, z, k, rx, wc + 2 * kP2StencilMax, wc + 3 * kP2StencilMax, cellOff, x, in1, out0, out1, dinv, a ); break;
+   case 2: p2RowPair< T, MODE, 0, 1 >( N, y, z, k, rx, wc + 4 * kP2StencilMax, wc + 5 * kP2StencilMax, cellOff, x, in1, out0, out1, dinv, a ); break;
+   default: p2RowPair< T, MODE, 1, 1 >( N, y, z, k, rx, wc + 6 * kP2StencilMax, wc + 7 * kP2StencilMax, cellOff, x, in1, out0, out1, dinv, a ); break;
    }
+}
+
+inline int p2RowWavesPerCell( int N )
+{
+   int n = 0;
+   for ( int k = 0; p2SegRows( N, k ) > 0; ++k )
+      n += p2SegRows( N, k );
+   return n;
 }
 
 // ------------------------------------------------------------------ prolongation
@@ -516,18 +600,12 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
    const int              tA   = ceilDiv( S3i( w ), kBlock );
    if constexpr ( !KQ && !DIAG )
    {
-      // constant coefficient: cell interiors by merged stencils, the rest element by element
-      P2ClassTiles ct;
-      ct.start[0] = 0;
-      for ( int k = 0; k < 8; ++k )
-      {
-         const int M     = p2InteriorExtent( 1 << L, k );
-         ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
-      }
-      const T* stw = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
-      if ( ct.start[8] > 0 )
-         hipLaunchKernelGGL( ( kP2Interior< T, MODE > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, stw, c.flag,
-                             (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
+      // constant coefficient: cell interiors by merged stencils, the face points element by element
+      const int wpc = p2RowWavesPerCell( 1 << L );
+      const T*  stw = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
+      if ( wpc > 0 && ( c.flag & 1 ) )
+         hipLaunchKernelGGL( ( kP2Rows< T, MODE > ), dim3( xcdGrid( ceilDiv( i64( nC ) * wpc, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, L,
+                             nC, g.stride, wpc, stw, (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
       const int tB = ceilDiv( p2BoundaryCount( 1 << L ), kBlock );
       hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq,
                           m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
