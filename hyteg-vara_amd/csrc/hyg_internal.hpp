@@ -232,21 +232,41 @@ constexpr int p2NodeClass( int t, int l )
    return ( p2NodeOffset( t, l, 0 ) & 1 ) | ( ( p2NodeOffset( t, l, 1 ) & 1 ) << 1 ) | ( ( p2NodeOffset( t, l, 2 ) & 1 ) << 2 );
 }
 
+// largest coordinate sum of the vertices of a micro-cell type (in level-L units)
+constexpr int p2CellMaxSum( int t )
+{
+   int m = 0;
+   for ( int v = 0; v < 4; ++v )
+   {
+      const int sum = kMicroCell[t][v][0] + kMicroCell[t][v][1] + kMicroCell[t][v][2];
+      m             = sum > m ? sum : m;
+   }
+   return m;
+}
+// Does the micro-cell of type t that has a point of cell face f as its node l lie on the cell's side of the
+// face? Faces 0..3 are z = 0, y = 0, x = 0 and x + y + z = N; f < 0 means no face (every micro-cell).
+constexpr bool p2OnCellSide( int f, int t, int l )
+{
+   return f < 0 || ( f < 3 ? p2NodeOffset( t, l, 2 - f ) == 0
+                           : p2NodeOffset( t, l, 0 ) + p2NodeOffset( t, l, 1 ) + p2NodeOffset( t, l, 2 ) >= 2 * p2CellMaxSum( t ) );
+}
+
 // Merged P2 stencil of a cell-interior point of parity class cls: the distinct offsets (within [-2, 2]^3) to
 // the nodes of all micro-cells that have the point as a node, in order of first appearance over (t, l, j).
-// Sizes 65 (vertex DoFs), 27 or 19 (edge DoFs).
+// Sizes 65 (vertex DoFs), 27 or 19 (edge DoFs). With a face f >= 0, the one-sided stencil of a point inside
+// that cell face (not on a cell edge): only the micro-cells on the cell's side of the face.
 struct P2StencilOffsets
 {
    int n = 0;
    int d[65][3] = {};
 };
-constexpr P2StencilOffsets p2StencilOffsets( int cls )
+constexpr P2StencilOffsets p2StencilOffsets( int cls, int f = -1 )
 {
    P2StencilOffsets s;
    for ( int t = 0; t < 6; ++t )
       for ( int l = 0; l < 10; ++l )
       {
-         if ( p2NodeClass( t, l ) != cls )
+         if ( p2NodeClass( t, l ) != cls || !p2OnCellSide( f, t, l ) )
             continue;
          for ( int j = 0; j < 10; ++j )
          {
@@ -435,6 +455,10 @@ struct hyg_op_s
       // (8 x kP2StencilMax, offsets of p2StencilOffsets)
       double*               p2st    = nullptr;
       float*                p2stF   = nullptr;
+      // one-sided stencils of the cell-face points, per cell, face and class (4 x 8 x kP2StencilMax, offsets
+      // of p2StencilOffsets( cls, face ))
+      double*               p2fst   = nullptr;
+      float*                p2fstF  = nullptr;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

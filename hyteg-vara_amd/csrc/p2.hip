@@ -87,46 +87,43 @@ __global__ __launch_bounds__( kBlock ) void kP2KQ( int L, int tilesPerCell, cons
 
 // ------------------------------------------------------------------ operator application
 // KQ: variable coefficient (el = 11 quadrature-point matrices per cell and type, kq per micro-cell)
-// points of the four faces of a cell lattice with N intervals per edge: z = 0; y = 0 and z >= 1; x = 0 and
-// y, z >= 1; x + y + z = N and x, y, z >= 1 (triangles of N + 1, N, N - 1 and N - 2 points per edge)
 __host__ __device__ constexpr int triCount( int n )
 {
    return n > 0 ? n * ( n + 1 ) / 2 : 0;
 }
-__host__ __device__ constexpr int p2BoundaryCount( int N )
+// points on the cell edges and vertices of a cell lattice with N intervals per edge: the four vertices, then
+// N - 1 points of each edge e0 = (v0, v1) ... e5 = (v2, v3)
+__host__ __device__ constexpr int p2EdgeVertexCount( int N )
 {
-   return triCount( N + 1 ) + triCount( N ) + triCount( N - 1 ) + triCount( N - 2 );
+   return 4 + 6 * ( N - 1 );
 }
-__device__ __forceinline__ bool boundaryXYZ( int N, int s, int& x, int& y, int& z )
+__device__ __forceinline__ bool edgeVertexXYZ( int N, int s, int& x, int& y, int& z )
 {
-   int a, b;
-   const int n0 = triCount( N + 1 ), n1 = triCount( N ), n2 = triCount( N - 1 ), n3 = triCount( N - 2 );
-   if ( s < n0 )
+   x = y = z = 0;
+   if ( s < 4 )
    {
-      planeXY( N + 1, s, a, b );
-      x = a, y = b, z = 0;
+      x = s == 1 ? N : 0;
+      y = s == 2 ? N : 0;
+      z = s == 3 ? N : 0;
+      return true;
    }
-   else if ( ( s -= n0 ) < n1 )
+   s -= 4;
+   const int e = s / ( N - 1 ), i = s - e * ( N - 1 ) + 1;
+   switch ( e )
    {
-      planeXY( N, s, a, b );
-      x = a, y = 0, z = b + 1;
+   case 0: x = i; break;
+   case 1: y = i; break;
+   case 2: x = N - i, y = i; break;
+   case 3: z = i; break;
+   case 4: x = N - i, z = i; break;
+   case 5: y = N - i, z = i; break;
+   default: return false;
    }
-   else if ( ( s -= n1 ) < n2 )
-   {
-      planeXY( N - 1, s, a, b );
-      x = 0, y = a + 1, z = b + 1;
-   }
-   else if ( ( s -= n2 ) < n3 )
-   {
-      planeXY( N - 2, s, a, b );
-      x = a + 1, y = b + 1, z = N - x - y;
-   }
-   else
-      return false;
    return true;
 }
 
-// SKIPIN: the cell-interior points are done by another kernel; the grid then covers only the face points
+// SKIPIN: the cell-interior and cell-face points are done by the stencil kernels; the grid then covers only the
+// points on cell edges and vertices
 template < typename T, int MODE, bool DIAG, bool KQ, bool SKIPIN = false >
 __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i64 stride, const T* __restrict__ el,
                                                       const T* __restrict__ kq,
@@ -144,7 +141,7 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
    int       px, py, pz;
    if constexpr ( SKIPIN )
    {
-      if ( !boundaryXYZ( N, s, px, py, pz ) )
+      if ( s >= p2EdgeVertexCount( N ) || !edgeVertexXYZ( N, s, px, py, pz ) )
          return;
    }
    else
@@ -249,6 +246,99 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
          lowPart[i64( e ) * NP + i] = val;
       }
    }
+}
+
+// ------------------------------------------------------------------ constant operator: cell-face points
+// Points inside a cell face (not on a cell edge) take the one-sided merged stencil of their face and parity
+// class; the sum goes to the face's partial-sum block like the element kernel's. On each face the points split
+// into four sub-lattices by the parities of their face coordinates (u, v) = (2 i + 2 - ku, 2 j + 2 - kv), u, v >= 1,
+// u + v <= N - 1, and each sub-lattice has one parity class, so the offsets are known at compile time.
+__host__ __device__ constexpr int p2FaceSubCount( int N, int ku, int kv )
+{
+   return N - 5 + ku + kv < 0 ? 0 : triCount( ( N - 5 + ku + kv ) / 2 + 1 );
+}
+struct P2FaceTiles
+{
+   int start[5]; // first tile of each (ku, kv) = (s & 1, s >> 1) sub-lattice within a face; start[4] = per face
+};
+
+template < typename T, int F, int KU, int KV >
+__device__ __forceinline__ void p2FacePoint( int c, int s, int N, i64 stride, const T* __restrict__ fst, const CellFaceSlot& fs,
+                                             T* __restrict__ part, const T* __restrict__ x )
+{
+   // face coordinates (u, v) -> cell coordinates; parity class of the points
+   constexpr int              bx = F == 2 ? 0 : KU, by = F == 0 || F == 3 ? KV : ( F == 2 ? KU : 0 );
+   constexpr int              bz = F == 0 ? 0 : ( F == 3 ? ( KU + KV ) & 1 : KV );
+   constexpr int              CLS = bx | ( by << 1 ) | ( bz << 2 );
+   constexpr P2StencilOffsets off = p2StencilOffsets( CLS, F );
+   const int                  M   = ( N - 5 + KU + KV ) / 2;
+   if ( s >= triCount( M + 1 ) )
+      return;
+   int i, j;
+   planeXY( M + 1, s, i, j );
+   const int u = 2 * i + 2 - KU, v = 2 * j + 2 - KV;
+   const int px = F == 2 ? 0 : u, py = F == 0 || F == 3 ? v : ( F == 2 ? u : 0 ), pz = F == 0 ? 0 : ( F == 3 ? N - u - v : v );
+   const int w = N + 1;
+   typedef const T __attribute__( ( address_space( 4 ) ) ) CT;
+   const CT* wc = (const CT*) ( fst + ( i64( c * 4 + F ) * 8 + CLS ) * kP2StencilMax );
+   const T*  xc = x + i64( c ) * stride;
+   int       rb[5][5];
+#pragma unroll
+   for ( int dz = -2; dz <= 2; ++dz )
+   {
+      const int W  = w - ( pz + dz );
+      const int pb = S3i( w ) - S3i( W );
+#pragma unroll
+      for ( int dy = -2; dy <= 2; ++dy )
+         rb[dz + 2][dy + 2] = pb + rowStart( W, py + dy );
+   }
+   T val = T( 0 );
+#pragma unroll
+   for ( int e = 0; e < off.n; ++e )
+      val += wc[e] * xc[rb[off.d[e][2] + 2][off.d[e][1] + 2] + px + off.d[e][0]];
+   part[i64( fs.block ) * w * w + facePartIndex( F, N, px, py, pz )] = val;
+}
+
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kP2Face( int L, i64 stride, P2FaceTiles ft, const T* __restrict__ fst,
+                                                     const CellFaceSlot* __restrict__ cfs, int flag, T* __restrict__ part,
+                                                     const T* __restrict__ x )
+{
+   const int per = ft.start[4];
+   const int c   = int( blockIdx.x ) / ( 4 * per );
+   int       r   = int( blockIdx.x ) - c * 4 * per;
+   const int f   = r / per;
+   r -= f * per;
+   int sub = 0;
+   while ( sub < 3 && r >= ft.start[sub + 1] )
+      ++sub;
+   const int          s  = ( r - ft.start[sub] ) * kBlock + int( threadIdx.x );
+   const CellFaceSlot fs = cfs[4 * c + f];
+   if ( !( fs.bc & flag ) )
+      return;
+   const int N = 1 << L;
+#define HYG_P2_FACE( F, SUB )                                                                                         \
+   case 4 * F + SUB: p2FacePoint< T, F, ( SUB & 1 ), ( SUB >> 1 ) >( c, s, N, stride, fst, fs, part, x ); break;
+   switch ( 4 * f + sub )
+   {
+      HYG_P2_FACE( 0, 0 )
+      HYG_P2_FACE( 0, 1 )
+      HYG_P2_FACE( 0, 2 )
+      HYG_P2_FACE( 0, 3 )
+      HYG_P2_FACE( 1, 0 )
+      HYG_P2_FACE( 1, 1 )
+      HYG_P2_FACE( 1, 2 )
+      HYG_P2_FACE( 1, 3 )
+      HYG_P2_FACE( 2, 0 )
+      HYG_P2_FACE( 2, 1 )
+      HYG_P2_FACE( 2, 2 )
+      HYG_P2_FACE( 2, 3 )
+      HYG_P2_FACE( 3, 0 )
+      HYG_P2_FACE( 3, 1 )
+      HYG_P2_FACE( 3, 2 )
+   default: p2FacePoint< T, 3, 1, 1 >( c, s, N, stride, fst, fs, part, x ); break;
+   }
+#undef HYG_P2_FACE
 }
 
 // ------------------------------------------------------------------ constant operator: cell-interior points
@@ -600,13 +690,22 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
    const int              tA   = ceilDiv( S3i( w ), kBlock );
    if constexpr ( !KQ && !DIAG )
    {
-      // constant coefficient: cell interiors by merged stencils, the face points element by element
+      // constant coefficient: cell interiors by row pairs, cell faces by one-sided stencils, the points on cell
+      // edges and vertices element by element
       const int wpc = p2RowWavesPerCell( 1 << L );
       const T*  stw = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
       if ( wpc > 0 && ( c.flag & 1 ) )
          hipLaunchKernelGGL( ( kP2Rows< T, MODE > ), dim3( xcdGrid( ceilDiv( i64( nC ) * wpc, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, L,
                              nC, g.stride, wpc, stw, (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
-      const int tB = ceilDiv( p2BoundaryCount( 1 << L ), kBlock );
+      P2FaceTiles ft;
+      ft.start[0] = 0;
+      for ( int k = 0; k < 4; ++k )
+         ft.start[k + 1] = ft.start[k] + ceilDiv( p2FaceSubCount( 1 << L, k & 1, k >> 1 ), kBlock );
+      const T* fsw = dbl ? (const T*) ol.p2fst : (const T*) ol.p2fstF;
+      if ( ft.start[4] > 0 )
+         hipLaunchKernelGGL( ( kP2Face< T > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, fsw, m.cellFaces,
+                             c.flag, part, (const T*) c.x );
+      const int tB = ceilDiv( p2EdgeVertexCount( 1 << L ), kBlock );
       hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq,
                           m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
                           (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );

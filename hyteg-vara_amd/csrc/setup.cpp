@@ -732,32 +732,39 @@ void assembleP2Const( hyg_op_s& op )
       std::vector< float > elF( el.begin(), el.end() );
       ol.p2el              = upload( el );
       ol.p2elF             = upload( elF );
-      // merged stencils of the cell-interior points: sum over the (type, node) pairs of a class of the element
-      // rows, keyed by the offset to the other node
-      std::vector< double > st( size_t( m.nC() ) * 8 * kP2StencilMax, 0.0 );
-      for ( int cls = 0; cls < 8; ++cls )
-      {
-         const P2StencilOffsets off = p2StencilOffsets( cls );
-         for ( int c = 0; c < m.nC(); ++c )
-            for ( int t = 0; t < 6; ++t )
-               for ( int l = 0; l < 10; ++l )
-               {
-                  if ( p2NodeClass( t, l ) != cls )
-                     continue;
-                  for ( int j = 0; j < 10; ++j )
+      // merged stencils of the cell-interior points (f = -1) and the one-sided stencils of the cell-face points
+      // (f = 0..3): sum over the (type, node) pairs of a class of the element rows, keyed by the offset to the
+      // other node; the face stencils take only the micro-cells on the cell's side of the face
+      std::vector< double > st( size_t( m.nC() ) * 8 * kP2StencilMax, 0.0 ), fst( size_t( m.nC() ) * 32 * kP2StencilMax, 0.0 );
+      for ( int f = -1; f < 4; ++f )
+         for ( int cls = 0; cls < 8; ++cls )
+         {
+            const P2StencilOffsets off = p2StencilOffsets( cls, f );
+            for ( int c = 0; c < m.nC(); ++c )
+            {
+               double* dst = f < 0 ? &st[( size_t( c ) * 8 + cls ) * kP2StencilMax] : &fst[( ( size_t( c ) * 4 + f ) * 8 + cls ) * kP2StencilMax];
+               for ( int t = 0; t < 6; ++t )
+                  for ( int l = 0; l < 10; ++l )
                   {
-                     int e = 0;
-                     while ( !( off.d[e][0] == p2NodeOffset( t, j, 0 ) - p2NodeOffset( t, l, 0 ) &&
-                                off.d[e][1] == p2NodeOffset( t, j, 1 ) - p2NodeOffset( t, l, 1 ) &&
-                                off.d[e][2] == p2NodeOffset( t, j, 2 ) - p2NodeOffset( t, l, 2 ) ) )
-                        ++e;
-                     st[( size_t( c ) * 8 + cls ) * kP2StencilMax + e] += el[size_t( c ) * 600 + 100 * t + 10 * l + j];
+                     if ( p2NodeClass( t, l ) != cls || !p2OnCellSide( f, t, l ) )
+                        continue;
+                     for ( int j = 0; j < 10; ++j )
+                     {
+                        int e = 0;
+                        while ( !( off.d[e][0] == p2NodeOffset( t, j, 0 ) - p2NodeOffset( t, l, 0 ) &&
+                                   off.d[e][1] == p2NodeOffset( t, j, 1 ) - p2NodeOffset( t, l, 1 ) &&
+                                   off.d[e][2] == p2NodeOffset( t, j, 2 ) - p2NodeOffset( t, l, 2 ) ) )
+                           ++e;
+                        dst[e] += el[size_t( c ) * 600 + 100 * t + 10 * l + j];
+                     }
                   }
-               }
-      }
-      std::vector< float > stF( st.begin(), st.end() );
+            }
+         }
+      std::vector< float > stF( st.begin(), st.end() ), fstF( fst.begin(), fst.end() );
       ol.p2st              = upload( st );
       ol.p2stF             = upload( stF );
+      ol.p2fst             = upload( fst );
+      ol.p2fstF            = upload( fstF );
       const size_t nLow    = mc.entriesHost.size() * size_t( N + 1 );
       const size_t nFace   = size_t( m.nF() ) * 2 * size_t( N + 1 ) * size_t( N + 1 );
       const size_t nFunc   = size_t( m.nC() ) * size_t( cellStride( L ) ) + size_t( kTailSlack );
@@ -885,6 +892,8 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.p2kqF );
       (void) hipFree( l.p2st );
       (void) hipFree( l.p2stF );
+      (void) hipFree( l.p2fst );
+      (void) hipFree( l.p2fstF );
    }
    (void) hipFree( modes );
 }
