@@ -288,6 +288,145 @@ constexpr P2StencilOffsets p2StencilOffsets( int cls, int f = -1 )
 }
 constexpr int kP2StencilMax = 65;
 
+// ------------------------------------------------------------------ P2 transfer stencils (compile-time tables)
+// P2 basis (vertices 0..3, then the edges of kP2EdgeNodes) at barycentric coordinates lam
+constexpr void p2BasisAt( const double lam[4], double phi[10] )
+{
+   for ( int v = 0; v < 4; ++v )
+      phi[v] = lam[v] * ( 2.0 * lam[v] - 1.0 );
+   for ( int e = 0; e < 6; ++e )
+      phi[4 + e] = 4.0 * lam[kP2EdgeNodes[e][0]] * lam[kP2EdgeNodes[e][1]];
+}
+// 4 x barycentric coordinates of point r (units of a quarter micro-edge) in the micro-cell of type t whose
+// anchor is at -o; false if outside. The vertex offsets of a type span a unimodular lattice basis.
+constexpr bool p2QuarterBary( int t, const int o[3], const int r[3], int l4[4] )
+{
+   int v[4][3] = {};
+   for ( int k = 0; k < 4; ++k )
+      for ( int c = 0; c < 3; ++c )
+         v[k][c] = kMicroCell[t][k][c] - o[c];
+   long J[3][3] = {};
+   for ( int a = 0; a < 3; ++a )
+      for ( int c = 0; c < 3; ++c )
+         J[a][c] = v[c + 1][a] - v[0][a];
+   const long det = J[0][0] * ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) - J[0][1] * ( J[1][0] * J[2][2] - J[1][2] * J[2][0] ) +
+                    J[0][2] * ( J[1][0] * J[2][1] - J[1][1] * J[2][0] );
+   const long A[3][3] = { { J[1][1] * J[2][2] - J[1][2] * J[2][1], J[0][2] * J[2][1] - J[0][1] * J[2][2], J[0][1] * J[1][2] - J[0][2] * J[1][1] },
+                          { J[1][2] * J[2][0] - J[1][0] * J[2][2], J[0][0] * J[2][2] - J[0][2] * J[2][0], J[0][2] * J[1][0] - J[0][0] * J[1][2] },
+                          { J[1][0] * J[2][1] - J[1][1] * J[2][0], J[0][1] * J[2][0] - J[0][0] * J[2][1], J[0][0] * J[1][1] - J[0][1] * J[1][0] } };
+   const long d[3]    = { r[0] - 4 * v[0][0], r[1] - 4 * v[0][1], r[2] - 4 * v[0][2] };
+   l4[0]              = 4;
+   for ( int k = 0; k < 3; ++k )
+   {
+      const long num = A[k][0] * d[0] + A[k][1] * d[1] + A[k][2] * d[2];
+      l4[k + 1]      = int( num / det );
+      l4[0] -= l4[k + 1];
+   }
+   for ( int k = 0; k < 4; ++k )
+      if ( l4[k] < 0 )
+         return false;
+   return true;
+}
+
+// Quadratic interpolation at a fine lattice point P = 4 Q + r (fine lattice = 2 x coarse lattice): the coarse
+// lattice points 2 Q + d and weights of the first micro-cell (type-major, then anchor offset o) of the coarse
+// P2 level that contains it; points of the coarse lattice (r even) take the coarse value itself. Every
+// micro-cell containing a cell-interior point lies in the cell, so any of them gives the interpolant.
+struct P2ProlongStencil
+{
+   int    n = 0;
+   int    d[10][3] = {};
+   double w[10] = {};
+};
+constexpr P2ProlongStencil p2ProlongStencil( int rx, int ry, int rz )
+{
+   P2ProlongStencil ps;
+   if ( ( ( rx | ry | rz ) & 1 ) == 0 )
+   {
+      ps.n       = 1;
+      ps.d[0][0] = rx / 2;
+      ps.d[0][1] = ry / 2;
+      ps.d[0][2] = rz / 2;
+      ps.w[0]    = 1.0;
+      return ps;
+   }
+   const int r[3] = { rx, ry, rz };
+   for ( int t = 0; t < 6; ++t )
+      for ( int o = 0; o < 8; ++o )
+      {
+         const int oo[3] = { o & 1, ( o >> 1 ) & 1, ( o >> 2 ) & 1 };
+         int       l4[4] = {};
+         if ( !p2QuarterBary( t, oo, r, l4 ) )
+            continue;
+         const double lam[4]  = { l4[0] / 4.0, l4[1] / 4.0, l4[2] / 4.0, l4[3] / 4.0 };
+         double       phi[10] = {};
+         p2BasisAt( lam, phi );
+         for ( int j = 0; j < 10; ++j )
+            if ( phi[j] != 0.0 )
+            {
+               for ( int c = 0; c < 3; ++c )
+                  ps.d[ps.n][c] = p2NodeOffset( t, j, c ) - 2 * oo[c];
+               ps.w[ps.n++] = phi[j];
+            }
+         return ps;
+      }
+   return ps;
+}
+
+// Restriction (transpose of the interpolation) at a coarse point n of parity class cls away from the cell
+// boundary: the fine lattice points 2 n + d where its basis function is nonzero, with those values. The support
+// is the union of the coarse micro-cells that have n as a node; each contributes its 35 quarter points.
+constexpr int kP2RestrictMax = 128;
+struct P2RestrictStencil
+{
+   int    n = 0;
+   int    d[kP2RestrictMax][3] = {};
+   double w[kP2RestrictMax] = {};
+};
+constexpr P2RestrictStencil p2RestrictStencil( int cls )
+{
+   P2RestrictStencil rs;
+   const int         n[3] = { 8 + ( cls & 1 ), 8 + ( ( cls >> 1 ) & 1 ), 8 + ( ( cls >> 2 ) & 1 ) };
+   for ( int t = 0; t < 6; ++t )
+      for ( int l = 0; l < 10; ++l )
+      {
+         if ( p2NodeClass( t, l ) != cls )
+            continue;
+         int a[3] = {};
+         for ( int c = 0; c < 3; ++c )
+            a[c] = ( n[c] - p2NodeOffset( t, l, c ) ) / 2;
+         for ( int i = 0; i <= 4; ++i )
+            for ( int j = 0; i + j <= 4; ++j )
+               for ( int q = 0; i + j + q <= 4; ++q )
+               {
+                  const int    l4[4]   = { 4 - i - j - q, i, j, q };
+                  const double lam[4]  = { l4[0] / 4.0, l4[1] / 4.0, l4[2] / 4.0, l4[3] / 4.0 };
+                  double       phi[10] = {};
+                  p2BasisAt( lam, phi );
+                  if ( phi[l] == 0.0 )
+                     continue;
+                  int d[3] = {};
+                  for ( int c = 0; c < 3; ++c )
+                  {
+                     int p = 4 * a[c];
+                     for ( int v = 0; v < 4; ++v )
+                        p += l4[v] * kMicroCell[t][v][c];
+                     d[c] = p - 2 * n[c];
+                  }
+                  bool seen = false;
+                  for ( int m = 0; m < rs.n; ++m )
+                     seen = seen || ( rs.d[m][0] == d[0] && rs.d[m][1] == d[1] && rs.d[m][2] == d[2] );
+                  if ( seen )
+                     continue;
+                  for ( int c = 0; c < 3; ++c )
+                     rs.d[rs.n][c] = d[c];
+                  rs.w[rs.n++] = phi[l];
+               }
+      }
+   return rs;
+}
+
+
 // Xiao-Gimbutas degree-4 rule with 11 points of p2_div_k_grad_affine_q4: barycentric coordinates (vertices 0..3)
 // and weights (summing to 1/6, the factors of |det J|)
 constexpr double kXG4Lambda[11][4] = {

@@ -514,6 +514,122 @@ inline int p2RowWavesPerCell( int N )
    return n;
 }
 
+// ------------------------------------------------------------------ transfers: cell-interior points
+// Interior fine points of one residue class r = P mod 4 share one interpolation stencil (coarse offsets and
+// weights known at compile time); interior coarse points of one parity class share one restriction stencil.
+// Fine points P = 4 (Q' + lo) + r with lo_c = 1 where r_c = 0 and |Q'| <= M enumerate the residue class.
+__host__ __device__ constexpr int p2ProlongExtent( int Nf, int rx, int ry, int rz )
+{
+   return Nf - 1 - rx - ry - rz < 0 ? -1 : ( Nf - 1 - rx - ry - rz ) / 4 - ( rx == 0 ) - ( ry == 0 ) - ( rz == 0 );
+}
+// interior points of a lattice with N intervals of parity class cls: p = 2 q + cls, q' = q - (1 - cls), |q'| <= M
+__host__ __device__ constexpr int p2InteriorExtent( int N, int cls )
+{
+   const int k = ( cls & 1 ) + ( ( cls >> 1 ) & 1 ) + ( ( cls >> 2 ) & 1 );
+   return ( N - 1 - k ) / 2 - ( 3 - k );
+}
+struct P2ClassTiles
+{
+   int start[65]; // first tile of each class within a cell; start[nClasses] = tiles per cell
+};
+
+template < typename T, int RX, int RY, int RZ >
+__device__ __forceinline__ void p2ProlongPt( int c, int s, int Nf, i64 strideF, i64 strideC, T* __restrict__ fine,
+                                             const T* __restrict__ coarse, int add )
+{
+   constexpr P2ProlongStencil ps = p2ProlongStencil( RX, RY, RZ );
+   const int                  M  = p2ProlongExtent( Nf, RX, RY, RZ );
+   if ( M < 0 || s >= S3i( M + 1 ) )
+      return;
+   int a, b, d;
+   tetXYZ( M + 1, s, a, b, d );
+   const int qx = a + ( RX == 0 ), qy = b + ( RY == 0 ), qz = d + ( RZ == 0 );
+   const int wc = ( Nf >> 1 ) + 1, wf = Nf + 1;
+   const T*  cs = coarse + i64( c ) * strideC;
+   T         v  = T( 0 );
+#pragma unroll
+   for ( int k = 0; k < ps.n; ++k )
+      v += T( ps.w[k] ) * cs[cidx( wc, 2 * qx + ps.d[k][0], 2 * qy + ps.d[k][1], 2 * qz + ps.d[k][2] )];
+   const i64 o = i64( c ) * strideF + cidx( wf, 4 * qx + RX, 4 * qy + RY, 4 * qz + RZ );
+   fine[o]     = add ? fine[o] + v : v;
+}
+template < typename T, int R >
+__device__ __forceinline__ void p2ProlongDispatch( int r, int c, int s, int Nf, i64 strideF, i64 strideC, T* __restrict__ fine,
+                                                   const T* __restrict__ coarse, int add )
+{
+   if ( r == R )
+      p2ProlongPt< T, ( R & 3 ), ( ( R >> 2 ) & 3 ), ( R >> 4 ) >( c, s, Nf, strideF, strideC, fine, coarse, add );
+   else if constexpr ( R < 63 )
+      p2ProlongDispatch< T, R + 1 >( r, c, s, Nf, strideF, strideC, fine, coarse, add );
+}
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kP2ProlongInt( int Lf, i64 strideF, i64 strideC, P2ClassTiles ct, int flag,
+                                                           T* __restrict__ fine, const T* __restrict__ coarse, int add )
+{
+   if ( !( flag & 1 ) )
+      return;
+   const int per = ct.start[64];
+   const int c   = int( blockIdx.x ) / per;
+   const int t   = int( blockIdx.x ) - c * per;
+   int       r   = 0;
+   while ( r < 63 && t >= ct.start[r + 1] )
+      ++r;
+   p2ProlongDispatch< T, 0 >( r, c, ( t - ct.start[r] ) * kBlock + int( threadIdx.x ), 1 << Lf, strideF, strideC, fine, coarse, add );
+}
+
+template < typename T, int CLS >
+__device__ __forceinline__ void p2RestrictPt( int c, int s, int Nc, i64 strideC, i64 strideF, T* __restrict__ coarse,
+                                              const T* __restrict__ fine )
+{
+   constexpr P2RestrictStencil rs = p2RestrictStencil( CLS );
+   constexpr int               kx = CLS & 1, ky = ( CLS >> 1 ) & 1, kz = ( CLS >> 2 ) & 1;
+   const int                   M  = p2InteriorExtent( Nc, CLS );
+   if ( M < 0 || s >= S3i( M + 1 ) )
+      return;
+   int qx, qy, qz;
+   tetXYZ( M + 1, s, qx, qy, qz );
+   const int nx = 2 * ( qx + 1 - kx ) + kx, ny = 2 * ( qy + 1 - ky ) + ky, nz = 2 * ( qz + 1 - kz ) + kz;
+   const int wc = Nc + 1, wf = 2 * Nc + 1;
+   // start of fine row 2 ny + dy in plane 2 nz + dz, |dy|, |dz| <= 3
+   int rb[7][7];
+#pragma unroll
+   for ( int dz = -3; dz <= 3; ++dz )
+   {
+      const int W  = wf - ( 2 * nz + dz );
+      const int pb = S3i( wf ) - S3i( W );
+#pragma unroll
+      for ( int dy = -3; dy <= 3; ++dy )
+         rb[dz + 3][dy + 3] = pb + rowStart( W, 2 * ny + dy );
+   }
+   const T* fs = fine + i64( c ) * strideF;
+   T        v  = T( 0 );
+#pragma unroll
+   for ( int k = 0; k < rs.n; ++k )
+      v += T( rs.w[k] ) * fs[rb[rs.d[k][2] + 3][rs.d[k][1] + 3] + 2 * nx + rs.d[k][0]];
+   coarse[i64( c ) * strideC + cidx( wc, nx, ny, nz )] = v;
+}
+template < typename T, int CLS >
+__device__ __forceinline__ void p2RestrictDispatch( int cls, int c, int s, int Nc, i64 strideC, i64 strideF, T* __restrict__ coarse,
+                                                    const T* __restrict__ fine )
+{
+   if ( cls == CLS )
+      p2RestrictPt< T, CLS >( c, s, Nc, strideC, strideF, coarse, fine );
+   else if constexpr ( CLS < 7 )
+      p2RestrictDispatch< T, CLS + 1 >( cls, c, s, Nc, strideC, strideF, coarse, fine );
+}
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kP2RestrictInt( int Lc, i64 strideC, i64 strideF, P2ClassTiles ct, T* __restrict__ coarse,
+                                                            const T* __restrict__ fine )
+{
+   const int per = ct.start[8];
+   const int c   = int( blockIdx.x ) / per;
+   const int t   = int( blockIdx.x ) - c * per;
+   int       cls = 0;
+   while ( cls < 7 && t >= ct.start[cls + 1] )
+      ++cls;
+   p2RestrictDispatch< T, 0 >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << Lc, strideC, strideF, coarse, fine );
+}
+
 // ------------------------------------------------------------------ prolongation
 // Value at point p (fine lattice, N = 2^Lf) of the quadratic interpolant of the coarse function cs
 // (lattice Lf-1, w = Nf/2 + 1) on a micro-cell of level Lf-2 of the macro-cell containing p.
@@ -541,29 +657,6 @@ __device__ __forceinline__ T p2Interp( const P2Tables* __restrict__ tb, int Nf, 
       return v;
    }
    return T( 0 );
-}
-
-// cell-interior points of the fine lattice of every cell
-template < typename T >
-__global__ __launch_bounds__( kBlock ) void kP2ProlongCell( int Lf, int tilesPerCell, i64 strideF, i64 strideC,
-                                                            const P2Tables* __restrict__ tb, int flag, T* __restrict__ fine,
-                                                            const T* __restrict__ coarse, int add )
-{
-   if ( !( flag & 1 ) )
-      return;
-   const int c    = int( blockIdx.x ) / tilesPerCell;
-   const int tile = int( blockIdx.x ) - c * tilesPerCell;
-   const int N = 1 << Lf, w = N + 1;
-   const int s = tile * kBlock + int( threadIdx.x );
-   if ( s >= S3i( w ) )
-      return;
-   int px, py, pz;
-   tetXYZ( w, s, px, py, pz );
-   if ( classifySlot( N, px, py, pz ) != 0 )
-      return;
-   const T   v = p2Interp< T >( tb, N, coarse + i64( c ) * strideC, px, py, pz );
-   const i64 o = i64( c ) * strideF + cidx( w, px, py, pz );
-   fine[o]     = add ? fine[o] + v : v;
 }
 
 // points of macro-faces, -edges and -vertices: interpolated in the first incident cell, written to every replica
@@ -599,34 +692,6 @@ __global__ __launch_bounds__( kBlock ) void kP2ProlongLower( const LowerChunk* _
 }
 
 // ------------------------------------------------------------------ restriction (transpose of the prolongation)
-// cell-interior coarse points: the weighted sum over the support of their basis function, which lies in the cell
-template < typename T >
-__global__ __launch_bounds__( kBlock ) void kP2RestrictCell( int Lc, int tilesPerCell, i64 strideC, i64 strideF,
-                                                             const P2Tables* __restrict__ tb, T* __restrict__ coarse,
-                                                             const T* __restrict__ fine )
-{
-   const int c    = int( blockIdx.x ) / tilesPerCell;
-   const int tile = int( blockIdx.x ) - c * tilesPerCell;
-   const int Nc = 1 << Lc, wc = Nc + 1, wf = 2 * Nc + 1;
-   const int s = tile * kBlock + int( threadIdx.x );
-   if ( s >= S3i( wc ) )
-      return;
-   int nx, ny, nz;
-   tetXYZ( wc, s, nx, ny, nz );
-   if ( classifySlot( Nc, nx, ny, nz ) != 0 )
-      return;
-   const int cls = ( nx & 1 ) | ( ( ny & 1 ) << 1 ) | ( ( nz & 1 ) << 2 );
-   const T*  fs  = fine + i64( c ) * strideF;
-   T         v   = T( 0 );
-   const int ns  = tb->nSup[cls];
-   for ( int k = 0; k < ns; ++k )
-   {
-      const P2Tables::Sup& sp = tb->sup[cls][k];
-      v += T( sp.w ) * fs[cidx( wf, 2 * nx + sp.d[0], 2 * ny + sp.d[1], 2 * nz + sp.d[2] )];
-   }
-   coarse[i64( c ) * strideC + cidx( wc, nx, ny, nz )] = v;
-}
-
 // coarse points of macro-faces, -edges and -vertices: sum over the incident cells of the in-cell support,
 // fine points on macro-primitives scaled by 1 / (#cells sharing them); written to every replica
 template < typename T >
@@ -799,11 +864,18 @@ template < typename T >
 static void p2ProlongT( const MeshCtx& m, const LevelGeom& gf, const P2Tables* tb, int Lf, void* fine, const void* coarse, int flag,
                         bool add, hipStream_t st )
 {
-   const int nC      = m.mesh->nC();
-   const int tA      = ceilDiv( S3i( ( 1 << Lf ) + 1 ), kBlock );
-   const i64 strideC = cellStride( Lf - 1 );
-   hipLaunchKernelGGL( ( kP2ProlongCell< T > ), dim3( nC * tA ), dim3( kBlock ), 0, st, Lf, tA, gf.stride, strideC, tb, flag, (T*) fine,
-                       (const T*) coarse, add ? 1 : 0 );
+   const int    nC      = m.mesh->nC();
+   const i64    strideC = cellStride( Lf - 1 );
+   P2ClassTiles ct;
+   ct.start[0] = 0;
+   for ( int r = 0; r < 64; ++r )
+   {
+      const int M     = p2ProlongExtent( 1 << Lf, r & 3, ( r >> 2 ) & 3, r >> 4 );
+      ct.start[r + 1] = ct.start[r] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
+   }
+   if ( ct.start[64] > 0 )
+      hipLaunchKernelGGL( ( kP2ProlongInt< T > ), dim3( nC * ct.start[64] ), dim3( kBlock ), 0, st, Lf, gf.stride, strideC, ct, flag,
+                          (T*) fine, (const T*) coarse, add ? 1 : 0 );
    if ( gf.nChunks > 0 )
       hipLaunchKernelGGL( ( kP2ProlongLower< T > ), dim3( ceilDiv( gf.nChunks, kBlock / 64 ) ), dim3( kBlock ), 0, st, gf.chunks,
                           gf.nChunks, m.prims, m.entries, tb, Lf, gf.stride, strideC, flag, (T*) fine, (const T*) coarse,
@@ -823,11 +895,18 @@ template < typename T >
 static void p2RestrictT( const MeshCtx& m, const LevelGeom& gc, const P2Tables* tb, int Lc, void* coarse, const void* fine, int flag,
                          hipStream_t st )
 {
-   const int nC      = m.mesh->nC();
-   const int tA      = ceilDiv( S3i( ( 1 << Lc ) + 1 ), kBlock );
-   const i64 strideF = cellStride( Lc + 1 );
-   hipLaunchKernelGGL( ( kP2RestrictCell< T > ), dim3( nC * tA ), dim3( kBlock ), 0, st, Lc, tA, gc.stride, strideF, tb, (T*) coarse,
-                       (const T*) fine );
+   const int    nC      = m.mesh->nC();
+   const i64    strideF = cellStride( Lc + 1 );
+   P2ClassTiles ct;
+   ct.start[0] = 0;
+   for ( int k = 0; k < 8; ++k )
+   {
+      const int M     = p2InteriorExtent( 1 << Lc, k );
+      ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
+   }
+   if ( ct.start[8] > 0 )
+      hipLaunchKernelGGL( ( kP2RestrictInt< T > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, st, Lc, gc.stride, strideF, ct, (T*) coarse,
+                          (const T*) fine );
    if ( gc.nChunks > 0 )
       hipLaunchKernelGGL( ( kP2RestrictLower< T > ), dim3( ceilDiv( gc.nChunks, kBlock / 64 ) ), dim3( kBlock ), 0, st, gc.chunks,
                           gc.nChunks, m.prims, m.entries, m.cellInfo, tb, Lc, gc.stride, strideF, flag, (T*) coarse,

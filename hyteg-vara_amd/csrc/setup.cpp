@@ -508,43 +508,14 @@ namespace {
 // local P2 nodes: vertices 0..3, then the edges (2,3) (1,3) (1,2) (0,3) (0,2) (0,1)
 const int kP2Edge[6][2] = { { 2, 3 }, { 1, 3 }, { 1, 2 }, { 0, 3 }, { 0, 2 }, { 0, 1 } };
 
+// basis and quarter-point barycentric coordinates: the compile-time versions the transfer kernels use
 void p2Basis( const double lam[4], double phi[10] )
 {
-   for ( int v = 0; v < 4; ++v )
-      phi[v] = lam[v] * ( 2.0 * lam[v] - 1.0 );
-   for ( int e = 0; e < 6; ++e )
-      phi[4 + e] = 4.0 * lam[kP2Edge[e][0]] * lam[kP2Edge[e][1]];
+   p2BasisAt( lam, phi );
 }
-
-// 4 x barycentric coordinates of point r (units of a quarter micro-edge) in the micro-cell of type t whose
-// anchor is at -o; false if outside. The vertex offsets of a type span a unimodular lattice basis.
 bool quarterBary( int t, const int o[3], const int r[3], int l4[4] )
 {
-   int v[4][3];
-   for ( int k = 0; k < 4; ++k )
-      for ( int c = 0; c < 3; ++c )
-         v[k][c] = kMicroTet[t][k][c] - o[c];
-   long J[3][3];
-   for ( int a = 0; a < 3; ++a )
-      for ( int c = 0; c < 3; ++c )
-         J[a][c] = v[c + 1][a] - v[0][a];
-   const long det = J[0][0] * ( J[1][1] * J[2][2] - J[1][2] * J[2][1] ) - J[0][1] * ( J[1][0] * J[2][2] - J[1][2] * J[2][0] ) +
-                    J[0][2] * ( J[1][0] * J[2][1] - J[1][1] * J[2][0] );
-   const long A[3][3] = { { J[1][1] * J[2][2] - J[1][2] * J[2][1], J[0][2] * J[2][1] - J[0][1] * J[2][2], J[0][1] * J[1][2] - J[0][2] * J[1][1] },
-                          { J[1][2] * J[2][0] - J[1][0] * J[2][2], J[0][0] * J[2][2] - J[0][2] * J[2][0], J[0][2] * J[1][0] - J[0][0] * J[1][2] },
-                          { J[1][0] * J[2][1] - J[1][1] * J[2][0], J[0][1] * J[2][0] - J[0][0] * J[2][1], J[0][0] * J[1][1] - J[0][1] * J[1][0] } };
-   const long d[3] = { r[0] - 4 * v[0][0], r[1] - 4 * v[0][1], r[2] - 4 * v[0][2] };
-   l4[0]           = 4;
-   for ( int k = 0; k < 3; ++k )
-   {
-      const long num = A[k][0] * d[0] + A[k][1] * d[1] + A[k][2] * d[2];
-      l4[k + 1]      = int( num / det );
-      l4[0] -= l4[k + 1];
-   }
-   for ( int k = 0; k < 4; ++k )
-      if ( l4[k] < 0 )
-         return false;
-   return true;
+   return p2QuarterBary( t, o, r, l4 );
 }
 } // namespace
 
