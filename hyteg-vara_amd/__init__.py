@@ -37,6 +37,7 @@ def _sig(lib):
         "hyg_init": (i, [i]),
         "hyg_mesh_create_from_msh": (i, [C.c_char_p, pvp]),
         "hyg_mesh_create_spherical_shell": (i, [i, i, d, d, pvp]),
+        "hyg_mesh_create_symmetric_cuboid": (i, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i, pvp]),
         "hyg_mesh_destroy": (i, [vp]),
         "hyg_mesh_info": (i, [vp, P(np.int64)]),
         "hyg_mesh_num_dofs": (i, [vp, i, C.POINTER(C.c_int64)]),
@@ -123,15 +124,21 @@ def _out():
 
 
 class Mesh:
-    """A Gmsh v2.2 file (name or path), or the spherical shell ("shell", ntan, nrad, rmin, rmax)."""
+    """A Gmsh v2.2 file (name or path), the spherical shell ("shell", ntan, nrad, rmin, rmax) or the symmetric
+    cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz)."""
 
-    def __init__(self, name, *shell):
+    def __init__(self, name, *args):
         init()
         h = _out()
         if name == "shell":
             self.path = None
-            ntan, nrad, rmin, rmax = shell
+            ntan, nrad, rmin, rmax = args
             check(load().hyg_mesh_create_spherical_shell(ntan, nrad, rmin, rmax, C.byref(h)))
+        elif name == "symmetric_cuboid":
+            self.path = None
+            lo, hi, nx, ny, nz = args
+            lo3, hi3 = (C.c_double * 3)(*lo), (C.c_double * 3)(*hi)
+            check(load().hyg_mesh_create_symmetric_cuboid(lo3, hi3, nx, ny, nz, C.byref(h)))
         else:
             self.path = name if os.path.isabs(name) else os.path.join(MESHES, name)
             check(load().hyg_mesh_create_from_msh(self.path.encode(), C.byref(h)))

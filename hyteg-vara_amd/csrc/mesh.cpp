@@ -333,6 +333,66 @@ std::unique_ptr< Mesh > meshSphericalShell( int ntan, int nrad, double rmin, dou
    return buildMesh( nodes, cellKeys, "spherical shell", err );
 }
 
+// MeshInfo::meshSymmetricCuboid( lowerLeftFront, upperRightBack, nx, ny, nz ) (mesh/MeshGenSymmetricCuboid.cpp:35-253):
+// nx x ny x nz sub-cubes of 24 tetrahedra around the centres of their six faces. Sub-cubes are visited z-major,
+// then y, then x; a sub-cube's 14 vertices (8 corners, 6 face centres) take the ID of the same point in an
+// earlier sub-cube, and the points seen for the first time get new IDs in the order of their local index.
+// Every sub-cube writes the coordinates of all its vertices, so a shared vertex keeps the value computed in
+// the last sub-cube that contains it.
+std::unique_ptr< Mesh > meshSymmetricCuboid( const double lo[3], const double hi[3], int nx, int ny, int nz, std::string& err )
+{
+   if ( nx < 1 || ny < 1 || nz < 1 )
+   {
+      err = "symmetric cuboid needs at least one sub-cube in every direction";
+      return nullptr;
+   }
+   const int    n[3] = { nx, ny, nz };
+   double       w[3];
+   for ( int c = 0; c < 3; ++c )
+      w[c] = ( hi[c] - lo[c] ) / double( n[c] );
+   // local vertices in units of half a sub-cube: corners of the bottom face, its centre, the four side-face
+   // centres, then the top face
+   static const int lp[14][3] = { { 0, 0, 0 }, { 2, 0, 0 }, { 1, 1, 0 }, { 0, 2, 0 }, { 2, 2, 0 }, { 1, 0, 1 }, { 0, 1, 1 },
+                                  { 2, 1, 1 }, { 1, 2, 1 }, { 0, 0, 2 }, { 2, 0, 2 }, { 1, 1, 2 }, { 0, 2, 2 }, { 2, 2, 2 } };
+   static const int tets[24][4] = { { 0, 1, 2, 5 },    { 0, 3, 2, 6 },    { 1, 2, 4, 7 },    { 2, 4, 3, 8 },   { 2, 0, 5, 6 },
+                                    { 2, 1, 5, 7 },    { 2, 3, 6, 8 },    { 2, 4, 7, 8 },    { 0, 9, 5, 6 },   { 1, 10, 5, 7 },
+                                    { 3, 12, 6, 8 },   { 4, 13, 7, 8 },   { 2, 5, 6, 8 },    { 2, 5, 7, 8 },   { 11, 5, 6, 8 },
+                                    { 11, 5, 7, 8 },   { 11, 9, 5, 6 },   { 11, 10, 5, 7 },  { 11, 12, 6, 8 }, { 11, 13, 7, 8 },
+                                    { 11, 9, 10, 5 },  { 11, 9, 12, 6 },  { 11, 10, 13, 7 }, { 11, 12, 13, 8 } };
+   std::map< std::array< int, 3 >, long > idOf;
+   std::map< long, std::array< double, 3 > > nodes;
+   std::map< std::vector< long >, int >      cellKeys;
+   for ( int z = 0; z < nz; ++z )
+      for ( int y = 0; y < ny; ++y )
+         for ( int x = 0; x < nx; ++x )
+         {
+            const int q[3] = { x, y, z };
+            long      id[14];
+            for ( int i = 0; i < 14; ++i )
+            {
+               const std::array< int, 3 > key{ 2 * x + lp[i][0], 2 * y + lp[i][1], 2 * z + lp[i][2] };
+               auto                       it = idOf.find( key );
+               if ( it == idOf.end() )
+               {
+                  const long fresh = long( idOf.size() );
+                  it               = idOf.emplace( key, fresh ).first;
+               }
+               id[i] = it->second;
+               std::array< double, 3 > X;
+               for ( int c = 0; c < 3; ++c )
+               {
+                  // offset of the sub-cube plus the local coordinate (0, w/2 or w), in that order
+                  const double loc = lp[i][c] == 0 ? 0.0 : ( lp[i][c] == 1 ? w[c] / 2 : w[c] );
+                  X[c]             = ( lo[c] + w[c] * double( q[c] ) ) + loc;
+               }
+               nodes[id[i]] = X;
+            }
+            for ( auto& t : tets )
+               cellKeys[{ id[t[0]], id[t[1]], id[t[2]], id[t[3]] }] = 0;
+         }
+   return buildMesh( nodes, cellKeys, "symmetric cuboid", err );
+}
+
 i64 numDofs( const Mesh& m, int L )
 {
    const i64 N = i64( 1 ) << L;

@@ -64,6 +64,11 @@ int hyg_mesh_create_from_msh( const char* path, hyg_mesh* out );
  * nrad >= 2 equidistant radii) followed by the same storage setup and boundary flags as above: the inner
  * and outer spheres are Dirichlet. (ntan, nrad) = (5, 2) gives 960 macro-cells. */
 int hyg_mesh_create_spherical_shell( int ntan, int nrad, double rmin, double rmax, hyg_mesh* out );
+/* Replaces MeshInfo::meshSymmetricCuboid( lowerLeftFront, upperRightBack, nx, ny, nz )
+ * (mesh/MeshGenSymmetricCuboid.cpp:35-253): nx x ny x nz sub-cubes of 24 tetrahedra each (corners plus
+ * face centres), point-symmetric about the centre, followed by the same storage setup and boundary flags
+ * as above (the whole surface is Dirichlet). */
+int hyg_mesh_create_symmetric_cuboid( const double lower[3], const double upper[3], int nx, int ny, int nz, hyg_mesh* out );
 int hyg_mesh_destroy( hyg_mesh mesh );
 /* out[0..6] = #vertices, #edges, #faces, #cells, #boundary vertices, #boundary edges, #boundary faces */
 int hyg_mesh_info( hyg_mesh mesh, int64_t* out );

@@ -2121,6 +2121,60 @@ struct Gen
 };
 } // namespace shell
 
+// MeshInfo::meshSymmetricCuboid (mesh/MeshGenSymmetricCuboid.cpp:35-253). Restated with the reference's
+// per-sub-cube vertex map: a local vertex shared with the sub-cube below (z - 1: local 0-4 are its 9-13), in
+// front (y - 1: 0, 1, 5, 9, 10 are its 3, 4, 8, 12, 13) or to the left (x - 1: 0, 3, 6, 9, 12 are its 1, 4, 7,
+// 10, 13) takes that vertex's ID (:110-220); the rest get new IDs in local order. Coordinates are
+// offset + local position per sub-cube, the last sub-cube writing a shared vertex winning (:229-239).
+static std::unique_ptr< Mesh > makeSymmetricCuboid( const double* lo, const double* hi, int nx, int ny, int nz )
+{
+   const double wx = ( hi[0] - lo[0] ) / nx, wy = ( hi[1] - lo[1] ) / ny, wz = ( hi[2] - lo[2] ) / nz;
+   const P3     local[14] = { { 0, 0, 0 },           { wx, 0, 0 },           { wx / 2, wy / 2, 0 },  { 0, wy, 0 },
+                              { wx, wy, 0 },         { wx / 2, 0, wz / 2 },  { 0, wy / 2, wz / 2 },  { wx, wy / 2, wz / 2 },
+                              { wx / 2, wy, wz / 2 }, { 0, 0, wz },           { wx, 0, wz },          { wx / 2, wy / 2, wz },
+                              { 0, wy, wz },         { wx, wy, wz } };
+   const int    cellV[24][4] = { { 0, 1, 2, 5 },   { 0, 3, 2, 6 },   { 1, 2, 4, 7 },    { 2, 4, 3, 8 },    { 2, 0, 5, 6 },   { 2, 1, 5, 7 },
+                                 { 2, 3, 6, 8 },   { 2, 4, 7, 8 },   { 0, 9, 5, 6 },    { 1, 10, 5, 7 },   { 3, 12, 6, 8 },  { 4, 13, 7, 8 },
+                                 { 2, 5, 6, 8 },   { 2, 5, 7, 8 },   { 11, 5, 6, 8 },   { 11, 5, 7, 8 },   { 11, 9, 5, 6 },  { 11, 10, 5, 7 },
+                                 { 11, 12, 6, 8 }, { 11, 13, 7, 8 }, { 11, 9, 10, 5 },  { 11, 9, 12, 6 },  { 11, 10, 13, 7 }, { 11, 12, 13, 8 } };
+   const int    fromBelow[14] = { 9, 10, 11, 12, 13, -1, -1, -1, -1, -1, -1, -1, -1, -1 };
+   const int    fromFront[14] = { 3, 4, -1, -1, -1, 8, -1, -1, -1, 12, 13, -1, -1, -1 };
+   const int    fromLeft[14]  = { 1, -1, -1, 4, -1, -1, 7, -1, -1, 10, -1, -1, 13, -1 };
+   std::vector< std::array< idx, 14 > > vmap( size_t( nx ) * ny * nz );
+   auto at = [&]( int x, int y, int z ) -> std::array< idx, 14 >& { return vmap[( size_t( z ) * ny + y ) * nx + x]; };
+   idx  next = 0;
+   std::map< idx, P3 >           nodes;
+   std::set< std::vector< idx > > cells;
+   for ( int z = 0; z < nz; ++z )
+      for ( int y = 0; y < ny; ++y )
+         for ( int x = 0; x < nx; ++x )
+         {
+            auto& m = at( x, y, z );
+            for ( int i = 0; i < 14; ++i )
+            {
+               if ( z > 0 && fromBelow[i] >= 0 )
+                  m[i] = at( x, y, z - 1 )[fromBelow[i]];
+               else if ( y > 0 && fromFront[i] >= 0 )
+                  m[i] = at( x, y - 1, z )[fromFront[i]];
+               else if ( x > 0 && fromLeft[i] >= 0 )
+                  m[i] = at( x - 1, y, z )[fromLeft[i]];
+               else
+                  m[i] = next++;
+            }
+         }
+   for ( int z = 0; z < nz; ++z )
+      for ( int y = 0; y < ny; ++y )
+         for ( int x = 0; x < nx; ++x )
+         {
+            const P3 off{ lo[0] + wx * x, lo[1] + wy * y, lo[2] + wz * z };
+            for ( int i = 0; i < 14; ++i )
+               nodes[at( x, y, z )[i]] = P3{ off[0] + local[i][0], off[1] + local[i][1], off[2] + local[i][2] };
+            for ( auto& c : cellV )
+               cells.insert( { at( x, y, z )[c[0]], at( x, y, z )[c[1]], at( x, y, z )[c[2]], at( x, y, z )[c[3]] } );
+         }
+   return buildMesh( nodes, cells );
+}
+
 static std::unique_ptr< Mesh > makeShell( idx ntan, idx nrad, double rmin, double rmax )
 {
    if ( ntan < 2 || ( ( ntan - 1 ) & ( ntan - 2 ) ) != 0 || nrad < 2 || !( rmin > 0 ) || !( rmin < rmax ) )
@@ -2174,6 +2228,10 @@ extern "C" {
 
 void* or_mesh_load( const char* path ) { return loadGmsh( path ).release(); }
 void* or_mesh_shell( int ntan, int nrad, double rmin, double rmax ) { return makeShell( ntan, nrad, rmin, rmax ).release(); }
+void* or_mesh_symmetric_cuboid( const double* lo, const double* hi, int nx, int ny, int nz )
+{
+   return ( nx < 1 || ny < 1 || nz < 1 ) ? nullptr : makeSymmetricCuboid( lo, hi, nx, ny, nz ).release();
+}
 
 // sum of the macro-cell volumes
 double or_mesh_volume( void* mh )

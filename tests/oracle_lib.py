@@ -25,6 +25,7 @@ def lib():
         sig = {
             "or_mesh_load": (vp, [C.c_char_p]),
             "or_mesh_shell": (vp, [i, i, d, d]),
+            "or_mesh_symmetric_cuboid": (vp, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i]),
             "or_mesh_volume": (d, [vp]),
             "or_mesh_free": (None, [vp]),
             "or_mesh_info": (None, [vp, P(np.int64)]),
@@ -65,14 +66,22 @@ def lib():
 
 
 class Mesh:
-    """A Gmsh v2.2 file (name or path), or the spherical shell ("shell", ntan, nrad, rmin, rmax)."""
+    """A Gmsh v2.2 file (name or path), the spherical shell ("shell", ntan, nrad, rmin, rmax) or the symmetric
+    cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz)."""
 
-    def __init__(self, name, *shell):
+    def __init__(self, name, *args):
         if name == "shell":
             self.path = None
-            self.h = lib().or_mesh_shell(*shell)
+            self.h = lib().or_mesh_shell(*args)
             if not self.h:
                 raise RuntimeError("bad spherical shell parameters")
+            return
+        if name == "symmetric_cuboid":
+            self.path = None
+            lo, hi, nx, ny, nz = args
+            self.h = lib().or_mesh_symmetric_cuboid((C.c_double * 3)(*lo), (C.c_double * 3)(*hi), nx, ny, nz)
+            if not self.h:
+                raise RuntimeError("bad symmetric cuboid parameters")
             return
         path = name if os.path.isabs(name) else os.path.join(MESHES, name)
         self.path = path

@@ -1,0 +1,53 @@
+"""Oracle restatement of MeshInfo::meshSymmetricCuboid (mesh/MeshGenSymmetricCuboid.cpp:35-253), pinned by what the
+construction fixes: 24 tetrahedra per sub-cube around its 8 corners and 6 face centres, so the vertex count is the
+corner lattice plus one centre per sub-cube face; the closed surface splits each sub-cube face into 4 triangles;
+the complex is a ball (Euler characteristic 1); the cells fill the box exactly; every macro-cell has positive volume
+and the boundary DoFs at level L lie on the box surface."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+CASES = [((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1, 1, 1),
+         ((-1.0, 0.5, 2.0), (2.0, 1.5, 2.75), 3, 1, 2),
+         ((0.0, 0.0, 0.0), (1.0, 2.0, 3.0), 2, 2, 2)]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_counts_euler_volume(case):
+    lo, hi, nx, ny, nz = case
+    m = ol.Mesh("symmetric_cuboid", lo, hi, nx, ny, nz)
+    nv, ne, nf, nc, bv, be, bf = [int(v) for v in m.info()]
+    corners = (nx + 1) * (ny + 1) * (nz + 1)
+    centres = nx * ny * (nz + 1) + ny * nz * (nx + 1) + nx * nz * (ny + 1)
+    assert nv == corners + centres
+    assert nc == 24 * nx * ny * nz
+    assert nv - ne + nf - nc == 1
+    assert bf == 4 * 2 * (nx * ny + ny * nz + nx * nz)
+    # the surface is a sphere: V - E + F = 2
+    assert bv - be + bf == 2
+    vol = np.prod(np.array(hi) - np.array(lo))
+    assert abs(m.volume() - vol) <= 1e-13 * vol
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_boundary_dofs_on_the_box_surface(case):
+    lo, hi, nx, ny, nz = case
+    m = ol.Mesh("symmetric_cuboid", lo, hi, nx, ny, nz)
+    lo, hi = np.array(lo), np.array(hi)
+    for L in (0, 2):
+        xyz, bnd = m.coords(L)
+        tol = 1e-12 * np.abs(hi - lo).max()
+        on_surface = (np.abs(xyz - lo) <= tol).any(axis=1) | (np.abs(xyz - hi) <= tol).any(axis=1)
+        assert np.array_equal(bnd, on_surface)
+        assert (xyz >= lo - tol).all() and (xyz <= hi + tol).all()
+
+
+def test_single_cube_vertex_numbering():
+    """The first sub-cube numbers its 14 vertices in local order (:103-107): corners of the bottom face, its
+    centre, the four side centres, then the top face."""
+    m = ol.Mesh("symmetric_cuboid", (0.0, 0.0, 0.0), (2.0, 2.0, 2.0), 1, 1, 1)
+    xyz, _ = m.coords(0)
+    want = [(0, 0, 0), (2, 0, 0), (1, 1, 0), (0, 2, 0), (2, 2, 0), (1, 0, 1), (0, 1, 1), (2, 1, 1), (1, 2, 1),
+            (0, 0, 2), (2, 0, 2), (1, 1, 2), (0, 2, 2), (2, 2, 2)]
+    assert np.array_equal(xyz[:14], np.array(want, dtype=float))
