@@ -112,10 +112,10 @@ struct ZSeg
 };
 constexpr int kZSegChunk = 24; // planes per z-marching segment (short segments balance the tail of the launch)
 // cell-interior kernel configurations: rows per wavefront, points per lane, planes in flight
-constexpr int kCellRows[]   = { 1, 2, 4, 1, 2, 4 };
-constexpr int kCellPoints[] = { 1, 1, 1, 1, 1, 1 };
-constexpr int kCellDepth[]  = { 2, 2, 2, 4, 4, 4 };
-constexpr int kNumCellCfgs  = 6;
+constexpr int kCellRows[]   = { 1, 2, 4, 1, 2, 4, 8 };
+constexpr int kCellPoints[] = { 1, 1, 1, 1, 1, 1, 1 };
+constexpr int kCellDepth[]  = { 2, 2, 2, 4, 4, 4, 2 };
+constexpr int kNumCellCfgs  = 7;
 
 // Lower-dimensional primitive (macro-vertex, -edge or -face); order: vertices, edges, faces.
 struct LowerPrim

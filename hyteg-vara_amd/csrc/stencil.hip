@@ -568,7 +568,8 @@ static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol,
       case 2: kf = kSweep< T, MODE, 4, 1, 2 >; break;
       case 3: kf = kSweep< T, MODE, 1, 1, 4 >; break;
       case 4: kf = kSweep< T, MODE, 2, 1, 4 >; break;
-      default: kf = kSweep< T, MODE, 4, 1, 4 >; break;
+      case 5: kf = kSweep< T, MODE, 4, 1, 4 >; break;
+      default: kf = kSweep< T, MODE, 8, 1, 2 >; break;
       }
       hipLaunchKernelGGL( kf, dim3( p.gC + p.gF + gL ), dim3( kBlock ), 0, s, p );
    }

@@ -222,7 +222,7 @@ def test_apply_parity_large(hyg, meshes, L):
     assert np.abs(r).max() <= 1e-13 * np.abs(lin).max() * (1 << L)
 
 
-@pytest.mark.parametrize("cfg", range(6))
+@pytest.mark.parametrize("cfg", range(7))
 @pytest.mark.parametrize("L", [3, 5, 6])
 def test_every_cell_kernel_configuration(hyg, meshes, cfg, L):
     """Apply, residual and Jacobi sweep under each cell-kernel configuration (rows x points per lane),
