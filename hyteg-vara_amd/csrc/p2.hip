@@ -377,6 +377,21 @@ constexpr P2RowSet p2RowSet( int ky, int kz )
    return r;
 }
 
+constexpr int p2RowIndex( const P2RowSet& rs, int dy, int dz )
+{
+   for ( int r = 0; r < rs.n; ++r )
+      if ( rs.d[r][0] == dy && rs.d[r][1] == dz )
+         return r;
+   return -1;
+}
+constexpr int p2CentreIndex( const P2StencilOffsets& o )
+{
+   for ( int e = 0; e < o.n; ++e )
+      if ( o.d[e][0] == 0 && o.d[e][1] == 0 && o.d[e][2] == 0 )
+         return e;
+   return -1;
+}
+
 // first x of segment k, and the number of rows (y, z) whose interior reaches it
 __host__ __device__ constexpr int p2SegStart( int k )
 {
@@ -449,27 +464,39 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
    const T   vals[2] = { aE + laneShift< kDppWaveShr1 >( rE ) + laneShift< kDppWaveShl1 >( lE ),
                          aO + laneShift< kDppWaveShr1 >( rO ) + laneShift< kDppWaveShl1 >( lO ) };
    const int xmax    = N - 1 - y - z;
-   const i64 row     = cellOff + S3i( w ) - S3i( w - z ) + rowStart( w - z, y );
+   const int row     = S3i( w ) - S3i( w - z ) + rowStart( w - z, y ); // cell-local start of row (y, z)
    if ( lane == 0 || lane == 63 )
       return;
+   // epilogue inputs of the lane's pair: the second right-hand side (or the old output) as one pair load
+   // (x + 1 <= xmax + 1 stays in the row); the point values themselves are the stencil's centre row; the
+   // inverse diagonal of a cell-interior point is the inverse of its merged stencil's centre weight
+   constexpr bool kIn = MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK ||
+                        MODE == MODE_APPLY_ADD;
+   T i1[2] = { T( 0 ), T( 0 ) };
+   if constexpr ( kIn )
+   {
+      const auto     re = makeRsrc( ( MODE == MODE_APPLY_ADD ? out0 : in1 ) + cellOff, unsigned( S3i( w ) ) * unsigned( sizeof( T ) ) );
+      const unsigned vq = xe <= xmax ? unsigned( row + xe ) * unsigned( sizeof( T ) ) : kDropOffset;
+      bufLoad2( re, vq, 0, i1[0], i1[1] );
+   }
+   constexpr int rc = p2RowIndex( rs, 0, 0 );
+   const T       xv[2] = { vEr[rc], vOr[rc] };
+   T             dv[2] = { T( 1 ), T( 1 ) };
+   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+   {
+      dv[0] = T( 1 ) / wEr[p2CentreIndex( oE )];
+      dv[1] = T( 1 ) / wOr[p2CentreIndex( oO )];
+   }
+   (void) dinv;
 #pragma unroll
    for ( int h = 0; h < 2; ++h )
    {
       const int px = xe + h;
       if ( px < 1 || px > xmax )
          continue;
-      const i64 q  = row + px;
-      T         i1 = T( 0 ), r0, r1;
-      if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-         i1 = in1[q];
-      if constexpr ( MODE == MODE_APPLY_ADD )
-         i1 = out0[q];
-      T xv = T( 0 ), dv = T( 1 );
-      if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
-         xv = x[q];
-      if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-         dv = dinv[q];
-      epilogue< T, MODE >( vals[h], xv, i1, i1, dv, a, r0, r1 );
+      const i64 q = cellOff + row + px;
+      T         r0, r1;
+      epilogue< T, MODE >( vals[h], xv[h], i1[h], i1[h], dv[h], a, r0, r1 );
       out0[q] = r0;
       if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
          out1[q] = r1;
