@@ -68,3 +68,31 @@ def test_vcycle_history(hyg, meshes):
     ref = ol.gmg(oop, ox, ob, minL, maxL, cycles, **kw)
     assert ref[-1] < 0.2 * ref[0]
     assert np.all(np.abs(got - ref) <= 1e-10 * ref + 1e-14 * ref[0]), (got, ref)
+
+
+def test_p2_apply_and_transfers(hyg, meshes):
+    gm, om = meshes[1]
+    Lc = 2
+    nc, nf = gm.ndofs(Lc + 1), gm.ndofs(Lc + 2)
+    rng = np.random.default_rng(9)
+    u, v = rng.uniform(-1, 1, nc), rng.uniform(-1, 1, nf)
+    op, oop = hyg.P2ConstantOperator(gm, Lc + 1, Lc + 1), ol.P2Operator(om, Lc + 1, Lc + 1)
+    s, d = hyg.Function(gm, Lc + 1, Lc + 1, space="p2"), hyg.Function(gm, Lc + 1, Lc + 1, space="p2")
+    os_, od = ol.Function(om, Lc + 2, Lc + 2), ol.Function(om, Lc + 2, Lc + 2)
+    s.upload(Lc + 1, v)
+    os_.set(Lc + 2, v)
+    op.apply(s, d, Lc + 1, hyg.ALL)
+    oop.apply(os_, od, Lc + 2, ol.ALL, False)
+    assert rel_inf(d.download(Lc + 1), od.get(Lc + 2)) <= 1e-13
+    f = hyg.Function(gm, Lc, Lc + 1, space="p2")
+    of = ol.Function(om, Lc + 1, Lc + 2)
+    f.upload(Lc, u)
+    of.set(Lc + 1, u)
+    f.prolongate(Lc, hyg.ALL, hyg.REPLACE)
+    ol.p2_prolongate(of, Lc + 1, hyg.ALL, False)
+    assert rel_inf(f.download(Lc + 1), of.get(Lc + 2)) <= 1e-13
+    f.upload(Lc + 1, v)
+    of.set(Lc + 2, v)
+    f.restrict(Lc + 1, hyg.ALL)
+    ol.p2_restrict(of, Lc + 2, hyg.ALL)
+    assert rel_inf(f.download(Lc), of.get(Lc + 1)) <= 1e-13

@@ -115,3 +115,33 @@ def test_vcycle_history_on_the_shell(hyg, meshes):
     assert ref[-1] < 0.2 * ref[0]
     assert np.all(np.abs(got - ref) <= 1e-10 * ref + 1e-14 * ref[0]), (got, ref)
     assert rel_inf(x.download(maxL), ox.get(maxL + 1)) <= 1e-10
+
+
+@pytest.mark.parametrize("name,L", [("shell", 2), ("shell5", 2)])
+def test_p2_constant_apply_and_smoother_on_the_shell(hyg, meshes, name, L):
+    """The constant P2 operator's stencil kernels (cell interiors, one-sided face stencils, edges and vertices)
+    on macro-cells of every orientation the shell produces."""
+    gm, om = meshes[name]
+    n = gm.ndofs(L + 1)
+    x0, b0, d0 = rnd(n, 12), rnd(n, 13), rnd(n, 14)
+    op, oop = hyg.P2ConstantOperator(gm, L, L), ol.P2Operator(om, L, L)
+    s, d = hyg.Function(gm, L, L, space="p2"), hyg.Function(gm, L, L, space="p2")
+    os_, od = ol.Function(om, L + 1, L + 1), ol.Function(om, L + 1, L + 1)
+    s.upload(L, x0)
+    os_.set(L + 1, x0)
+    for flag in (hyg.INNER, hyg.ALL):
+        for upd in (hyg.REPLACE, hyg.ADD):
+            d.upload(L, d0)
+            od.set(L + 1, d0)
+            op.apply(s, d, L, flag, upd)
+            oop.apply(os_, od, L + 1, flag, upd == hyg.ADD)
+            assert rel_inf(d.download(L), od.get(L + 1)) <= TOL64, (flag, upd)
+    b = hyg.Function(gm, L, L, space="p2")
+    b.upload(L, b0)
+    d.upload(L, d0)
+    op.smooth_jac(d, b, s, 0.6, L, hyg.INNER)
+    ob = ol.Function(om, L + 1, L + 1)
+    ob.set(L + 1, b0)
+    od.set(L + 1, d0)
+    oop.smooth_jac(od, ob, os_, 0.6, L + 1, ol.INNER)
+    assert rel_inf(d.download(L), od.get(L + 1)) <= TOL64
