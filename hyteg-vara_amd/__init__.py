@@ -57,6 +57,7 @@ def _sig(lib):
         "hyg_op_create_p1_divkgrad": (i, [vp, i, i, d, P(np.float64), i, pvp]),
         "hyg_op_create_p2_const": (i, [vp, i, i, pvp]),
         "hyg_op_create_p2_divkgrad": (i, [vp, i, i, i, d, d, d, C.c_uint64, pvp]),
+        "hyg_op_store_local_matrices": (i, [vp]),
         "hyg_op_destroy": (i, [vp]),
         "hyg_op_cell_stencil": (i, [vp, i, i, P(np.float64)]),
         "hyg_apply": (i, [vp, vp, vp, i, i, i, vp]),
@@ -303,6 +304,10 @@ class P2DivKGradOperator(P1ConstantOperator):
         h = _out()
         check(load().hyg_op_create_p2_divkgrad(mesh.h, minL, maxL, kind, k0, alpha, beta, seed, C.byref(h)))
         self.h = h
+
+    def store_local_matrices(self):
+        """Assemble and keep the per-point stencils (computeAndStoreLocalElementMatrices)."""
+        check(load().hyg_op_store_local_matrices(self.h))
 
 
 def cheb_coefficients(order, lower, upper):

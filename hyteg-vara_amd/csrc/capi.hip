@@ -971,6 +971,28 @@ int hyg_op_create_p2_divkgrad( hyg_mesh mesh, int min_level, int max_level, int 
    return HYG_OK;
 }
 
+int hyg_op_store_local_matrices( hyg_op op )
+{
+   HYG_TRY( checkInit() );
+   if ( !op )
+      return fail( HYG_ERR_ARG, "null operator" );
+   if ( op->kind != 3 )
+      return fail( HYG_ERR_ARG, "stored local matrices are for the P2 variable-coefficient operator" );
+   try
+   {
+      for ( int L = op->minL; L <= op->maxL; ++L )
+         launchP2StoreStencils( op->mesh->ctx, *op, L, nullptr );
+   } catch ( const std::exception& e )
+   {
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   if ( int rc = launched( "stored P2 stencils" ) )
+      return rc;
+   if ( hipDeviceSynchronize() != hipSuccess )
+      return fail( HYG_ERR_HIP, "stored P2 stencil setup failed" );
+   return HYG_OK;
+}
+
 int hyg_op_destroy( hyg_op op )
 {
    delete op;

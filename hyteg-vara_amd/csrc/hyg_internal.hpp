@@ -599,6 +599,10 @@ struct hyg_op_s
       // of p2StencilOffsets( cls, face ))
       double*               p2fst   = nullptr;
       float*                p2fstF  = nullptr;
+      // variable-coefficient P2 operator after hyg_op_store_local_matrices: merged stencils of every cell-interior
+      // point and one-sided stencils of every cell-face point (per cell p2StoredCellSize(N) values)
+      double*               p2s     = nullptr;
+      float*                p2sF    = nullptr;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

@@ -41,6 +41,7 @@ void launchElementwiseInverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg
 void launchP2( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c, hipStream_t s );
 void launchP2InverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );
 void launchP2Coefficients( const MeshCtx& m, const hyg_op_s& op, int L, hipStream_t s );
+void launchP2StoreStencils( const MeshCtx& m, hyg_op_s& op, int L, hipStream_t s );
 void launchP2Prolongate( const MeshCtx& m, const LevelGeom& gf, const P2Tables* tb, int Lf, int dtype, void* fine,
                          const void* coarse, int flag, bool add, hipStream_t st );
 void launchP2Restrict( const MeshCtx& m, const LevelGeom& gc, const P2Tables* tb, int Lc, int dtype, void* coarse, const void* fine,

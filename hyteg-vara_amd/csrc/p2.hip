@@ -262,14 +262,55 @@ struct P2FaceTiles
    int start[5]; // first tile of each (ku, kv) = (s & 1, s >> 1) sub-lattice within a face; start[4] = per face
 };
 
-template < typename T, int F, int KU, int KV >
+// parity class of the points of face F whose face coordinates have parities (KU, KV)
+__host__ __device__ constexpr int p2FaceClass( int F, int KU, int KV )
+{
+   return ( F == 2 ? 0 : KU ) | ( ( F == 0 || F == 3 ? KV : ( F == 2 ? KU : 0 ) ) << 1 ) | ( ( F == 0 ? 0 : ( F == 3 ? ( KU + KV ) & 1 : KV ) ) << 2 );
+}
+
+// Stored stencils of the variable-coefficient operator: per cell, the merged stencils of the cell-interior
+// points class by class ([e][point of the class sub-lattice], the class-kappa points p = 2 q + kappa of the cell
+// form a tetrahedral lattice of width (N - |kappa|) / 2 + 1), then the one-sided stencils of the cell-face points
+// face by face and sub-lattice by sub-lattice ([e][face sub-lattice index]).
+__host__ __device__ constexpr int p2SubWidth( int N, int cls )
+{
+   return ( N - ( cls & 1 ) - ( ( cls >> 1 ) & 1 ) - ( ( cls >> 2 ) & 1 ) ) / 2 + 1;
+}
+__host__ __device__ inline i64 p2StoredIntBase( int N, int cls )
+{
+   constexpr int n[8] = { p2StencilOffsets( 0 ).n, p2StencilOffsets( 1 ).n, p2StencilOffsets( 2 ).n, p2StencilOffsets( 3 ).n,
+                          p2StencilOffsets( 4 ).n, p2StencilOffsets( 5 ).n, p2StencilOffsets( 6 ).n, p2StencilOffsets( 7 ).n };
+   i64           b    = 0;
+   for ( int k = 0; k < cls; ++k )
+      b += i64( n[k] ) * S3i( p2SubWidth( N, k ) );
+   return b;
+}
+__host__ __device__ inline i64 p2StoredFaceBase( int N, int F, int sub )
+{
+   constexpr int nf[4][4] = { { p2StencilOffsets( p2FaceClass( 0, 0, 0 ), 0 ).n, p2StencilOffsets( p2FaceClass( 0, 1, 0 ), 0 ).n,
+                                p2StencilOffsets( p2FaceClass( 0, 0, 1 ), 0 ).n, p2StencilOffsets( p2FaceClass( 0, 1, 1 ), 0 ).n },
+                              { p2StencilOffsets( p2FaceClass( 1, 0, 0 ), 1 ).n, p2StencilOffsets( p2FaceClass( 1, 1, 0 ), 1 ).n,
+                                p2StencilOffsets( p2FaceClass( 1, 0, 1 ), 1 ).n, p2StencilOffsets( p2FaceClass( 1, 1, 1 ), 1 ).n },
+                              { p2StencilOffsets( p2FaceClass( 2, 0, 0 ), 2 ).n, p2StencilOffsets( p2FaceClass( 2, 1, 0 ), 2 ).n,
+                                p2StencilOffsets( p2FaceClass( 2, 0, 1 ), 2 ).n, p2StencilOffsets( p2FaceClass( 2, 1, 1 ), 2 ).n },
+                              { p2StencilOffsets( p2FaceClass( 3, 0, 0 ), 3 ).n, p2StencilOffsets( p2FaceClass( 3, 1, 0 ), 3 ).n,
+                                p2StencilOffsets( p2FaceClass( 3, 0, 1 ), 3 ).n, p2StencilOffsets( p2FaceClass( 3, 1, 1 ), 3 ).n } };
+   i64 b = p2StoredIntBase( N, 8 );
+   for ( int k = 0; k < 4 * F + sub; ++k )
+      b += i64( nf[k >> 2][k & 3] ) * p2FaceSubCount( N, k & 1, ( k >> 1 ) & 1 );
+   return b;
+}
+__host__ __device__ inline i64 p2StoredCellSize( int N )
+{
+   return ( p2StoredFaceBase( N, 4, 0 ) + 63 ) / 64 * 64;
+}
+
+template < typename T, int F, int KU, int KV, bool STORED >
 __device__ __forceinline__ void p2FacePoint( int c, int s, int N, i64 stride, const T* __restrict__ fst, const CellFaceSlot& fs,
                                              T* __restrict__ part, const T* __restrict__ x )
 {
    // face coordinates (u, v) -> cell coordinates; parity class of the points
-   constexpr int              bx = F == 2 ? 0 : KU, by = F == 0 || F == 3 ? KV : ( F == 2 ? KU : 0 );
-   constexpr int              bz = F == 0 ? 0 : ( F == 3 ? ( KU + KV ) & 1 : KV );
-   constexpr int              CLS = bx | ( by << 1 ) | ( bz << 2 );
+   constexpr int              CLS = p2FaceClass( F, KU, KV );
    constexpr P2StencilOffsets off = p2StencilOffsets( CLS, F );
    const int                  M   = ( N - 5 + KU + KV ) / 2;
    if ( s >= triCount( M + 1 ) )
@@ -280,8 +321,11 @@ __device__ __forceinline__ void p2FacePoint( int c, int s, int N, i64 stride, co
    const int px = F == 2 ? 0 : u, py = F == 0 || F == 3 ? v : ( F == 2 ? u : 0 ), pz = F == 0 ? 0 : ( F == 3 ? N - u - v : v );
    const int w = N + 1;
    typedef const T __attribute__( ( address_space( 4 ) ) ) CT;
+   // uniform weights of the cell (constant coefficient), or this point's stored stencil
    const CT* wc = (const CT*) ( fst + ( i64( c * 4 + F ) * 8 + CLS ) * kP2StencilMax );
-   const T*  xc = x + i64( c ) * stride;
+   const T*  ws = fst + i64( c ) * p2StoredCellSize( N ) + p2StoredFaceBase( N, F, KU | ( KV << 1 ) ) + s;
+   const int cnt = triCount( M + 1 );
+   const T*  xc  = x + i64( c ) * stride;
    int       rb[5][5];
 #pragma unroll
    for ( int dz = -2; dz <= 2; ++dz )
@@ -295,11 +339,11 @@ __device__ __forceinline__ void p2FacePoint( int c, int s, int N, i64 stride, co
    T val = T( 0 );
 #pragma unroll
    for ( int e = 0; e < off.n; ++e )
-      val += wc[e] * xc[rb[off.d[e][2] + 2][off.d[e][1] + 2] + px + off.d[e][0]];
+      val += ( STORED ? ws[i64( e ) * cnt] : wc[e] ) * xc[rb[off.d[e][2] + 2][off.d[e][1] + 2] + px + off.d[e][0]];
    part[i64( fs.block ) * w * w + facePartIndex( F, N, px, py, pz )] = val;
 }
 
-template < typename T >
+template < typename T, bool STORED >
 __global__ __launch_bounds__( kBlock ) void kP2Face( int L, i64 stride, P2FaceTiles ft, const T* __restrict__ fst,
                                                      const CellFaceSlot* __restrict__ cfs, int flag, T* __restrict__ part,
                                                      const T* __restrict__ x )
@@ -318,7 +362,7 @@ __global__ __launch_bounds__( kBlock ) void kP2Face( int L, i64 stride, P2FaceTi
       return;
    const int N = 1 << L;
 #define HYG_P2_FACE( F, SUB )                                                                                         \
-   case 4 * F + SUB: p2FacePoint< T, F, ( SUB & 1 ), ( SUB >> 1 ) >( c, s, N, stride, fst, fs, part, x ); break;
+   case 4 * F + SUB: p2FacePoint< T, F, ( SUB & 1 ), ( SUB >> 1 ), STORED >( c, s, N, stride, fst, fs, part, x ); break;
    switch ( 4 * f + sub )
    {
       HYG_P2_FACE( 0, 0 )
@@ -336,7 +380,7 @@ __global__ __launch_bounds__( kBlock ) void kP2Face( int L, i64 stride, P2FaceTi
       HYG_P2_FACE( 3, 0 )
       HYG_P2_FACE( 3, 1 )
       HYG_P2_FACE( 3, 2 )
-   default: p2FacePoint< T, 3, 1, 1 >( c, s, N, stride, fst, fs, part, x ); break;
+   default: p2FacePoint< T, 3, 1, 1, STORED >( c, s, N, stride, fst, fs, part, x ); break;
    }
 #undef HYG_P2_FACE
 }
@@ -402,7 +446,7 @@ __host__ __device__ constexpr int p2SegRows( int N, int k )
    return triCount( N - 2 - p2SegStart( k ) );
 }
 
-template < typename T, int MODE, int KY, int KZ >
+template < typename T, int MODE, int KY, int KZ, bool STORED >
 __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_buffer_rsrc_t rx, const T* __restrict__ wE,
                                            const T* __restrict__ wO, i64 cellOff, const T* __restrict__ x, const T* __restrict__ in1,
                                            T* __restrict__ out0, T* __restrict__ out1, const T* __restrict__ dinv, T a )
@@ -423,20 +467,52 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
       const unsigned vo   = ( xe >= 0 && xe < W - yy ) ? unsigned( base + xe ) * unsigned( sizeof( T ) ) : kDropOffset;
       bufLoad2( rx, vo, 0, vEr[r], vOr[r] );
    }
-   // the weights are the same for every lane: read through the constant address space they become scalar loads
+   // constant coefficient: the weights are the same for every lane, read through the constant address space
+   // they become scalar loads. Stored stencils: wE / wO point at the cell's class arrays ([e][sub-lattice
+   // point]); the lane's two points are (xe / 2, (y - KY) / 2, (z - KZ) / 2) of their sub-lattices (lanes
+   // without a point in the row read point 0, whose weights are finite and unused).
    typedef const T __attribute__( ( address_space( 4 ) ) ) CT;
    const CT* wEr = (const CT*) wE;
    const CT* wOr = (const CT*) wO;
+   const int SWE = p2SubWidth( N, cE ), SWO = p2SubWidth( N, cO );
+   const int okE = xe >= 0 && xe < w - y - z, okO = xe >= 0 && xe + 1 < w - y - z;
+   const T*  pE  = wE + ( okE ? cidx( SWE, xe >> 1, ( y - KY ) >> 1, ( z - KZ ) >> 1 ) : 0 );
+   const T*  pO  = wO + ( okO ? cidx( SWO, xe >> 1, ( y - KY ) >> 1, ( z - KZ ) >> 1 ) : 0 );
+   const int nE  = S3i( SWE ), nO = S3i( SWO );
+   auto      wtE = [&]( int e ) -> T { return STORED ? pE[i64( e ) * nE] : wEr[e]; };
+   auto      wtO = [&]( int e ) -> T { return STORED ? pO[i64( e ) * nO] : wOr[e]; };
 #pragma unroll
    for ( int r = 0; r < rs.n; ++r )
    {
       const int dy = rs.d[r][0], dz = rs.d[r][1];
       const T   vE = vEr[r], vO = vOr[r];
+      if constexpr ( STORED )
+      {
+         // per-point weights: a term belongs to the lane whose point it is, so the neighbours' values move
+         // instead of the partial sums (x - 2, x - 1 from lane i - 1; x + 1, x + 2 from lane i + 1)
+         const T vEm = laneShift< kDppWaveShr1 >( vE ), vOm = laneShift< kDppWaveShr1 >( vO );
+         const T vEp = laneShift< kDppWaveShl1 >( vE ), vOp = laneShift< kDppWaveShl1 >( vO );
+#pragma unroll
+         for ( int e = 0; e < oE.n; ++e )
+            if ( oE.d[e][1] == dy && oE.d[e][2] == dz )
+            {
+               const int dx = oE.d[e][0];
+               aE += wtE( e ) * ( dx == -2 ? vEm : dx == -1 ? vOm : dx == 0 ? vE : dx == 1 ? vO : vEp );
+            }
+#pragma unroll
+         for ( int e = 0; e < oO.n; ++e )
+            if ( oO.d[e][1] == dy && oO.d[e][2] == dz )
+            {
+               const int dx = oO.d[e][0];
+               aO += wtO( e ) * ( dx == -2 ? vOm : dx == -1 ? vE : dx == 0 ? vO : dx == 1 ? vEp : vOp );
+            }
+         continue;
+      }
 #pragma unroll
       for ( int e = 0; e < oE.n; ++e )
          if ( oE.d[e][1] == dy && oE.d[e][2] == dz )
          {
-            const T wt = wEr[e];
+            const T wt = wtE( e );
             switch ( oE.d[e][0] )
             {
             case -2: rE += wt * vE; break;
@@ -450,7 +526,7 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
       for ( int e = 0; e < oO.n; ++e )
          if ( oO.d[e][1] == dy && oO.d[e][2] == dz )
          {
-            const T wt = wOr[e];
+            const T wt = wtO( e );
             switch ( oO.d[e][0] )
             {
             case -2: rO += wt * vO; break;
@@ -484,8 +560,8 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
    T             dv[2] = { T( 1 ), T( 1 ) };
    if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
    {
-      dv[0] = T( 1 ) / wEr[p2CentreIndex( oE )];
-      dv[1] = T( 1 ) / wOr[p2CentreIndex( oO )];
+      dv[0] = T( 1 ) / wtE( p2CentreIndex( oE ) );
+      dv[1] = T( 1 ) / wtO( p2CentreIndex( oO ) );
    }
    (void) dinv;
 #pragma unroll
@@ -503,7 +579,7 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
    }
 }
 
-template < typename T, int MODE >
+template < typename T, int MODE, bool STORED >
 __global__ __launch_bounds__( kBlock ) void kP2Rows( int L, int nC, i64 stride, int wavesPerCell, const T* __restrict__ st,
                                                      const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
                                                      T* __restrict__ out1, const T* __restrict__ dinv, T a )
@@ -520,16 +596,19 @@ __global__ __launch_bounds__( kBlock ) void kP2Rows( int L, int nC, i64 stride, 
       r -= p2SegRows( N, k++ );
    int yp, zp;
    planeXY( N - 2 - p2SegStart( k ), r, yp, zp );
-   const int y = yp + 1, z = zp + 1;
-   const i64 cellOff = i64( c ) * stride;
+   const int  y = yp + 1, z = zp + 1;
+   const i64  cellOff = i64( c ) * stride;
    const auto rx      = makeRsrc( x + cellOff, unsigned( S3i( N + 1 ) ) * unsigned( sizeof( T ) ) );
-   const T*   wc      = st + i64( c ) * 8 * kP2StencilMax;
+   // weights of class cls: per cell (constant coefficient) or the cell's stored class array
+   auto wcls = [&]( int cls ) -> const T* {
+      return STORED ? st + i64( c ) * p2StoredCellSize( N ) + p2StoredIntBase( N, cls ) : st + ( i64( c ) * 8 + cls ) * kP2StencilMax;
+   };
    switch ( ( y & 1 ) | ( ( z & 1 ) << 1 ) )
    {
-   case 0: p2RowPair< T, MODE, 0, 0 >( N, y, z, k, rx, wc, wc + kP2StencilMax, cellOff, x, in1, out0, out1, dinv, a ); break;
-   case 1: p2RowPair< T, MODE, 1, 0 >( N, y, z, k, rx, wc + 2 * kP2StencilMax, wc + 3 * kP2StencilMax, cellOff, x, in1, out0, out1, dinv, a ); break;
-   case 2: p2RowPair< T, MODE, 0, 1 >( N, y, z, k, rx, wc + 4 * kP2StencilMax, wc + 5 * kP2StencilMax, cellOff, x, in1, out0, out1, dinv, a ); break;
-   default: p2RowPair< T, MODE, 1, 1 >( N, y, z, k, rx, wc + 6 * kP2StencilMax, wc + 7 * kP2StencilMax, cellOff, x, in1, out0, out1, dinv, a ); break;
+   case 0: p2RowPair< T, MODE, 0, 0, STORED >( N, y, z, k, rx, wcls( 0 ), wcls( 1 ), cellOff, x, in1, out0, out1, dinv, a ); break;
+   case 1: p2RowPair< T, MODE, 1, 0, STORED >( N, y, z, k, rx, wcls( 2 ), wcls( 3 ), cellOff, x, in1, out0, out1, dinv, a ); break;
+   case 2: p2RowPair< T, MODE, 0, 1, STORED >( N, y, z, k, rx, wcls( 4 ), wcls( 5 ), cellOff, x, in1, out0, out1, dinv, a ); break;
+   default: p2RowPair< T, MODE, 1, 1, STORED >( N, y, z, k, rx, wcls( 6 ), wcls( 7 ), cellOff, x, in1, out0, out1, dinv, a ); break;
    }
 }
 
@@ -655,6 +734,159 @@ __global__ __launch_bounds__( kBlock ) void kP2RestrictInt( int Lc, i64 strideC,
    while ( cls < 7 && t >= ct.start[cls + 1] )
       ++cls;
    p2RestrictDispatch< T, 0 >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << Lc, strideC, strideF, coarse, fine );
+}
+
+// ------------------------------------------------------------------ stored stencils (variable coefficient)
+// Setup: the stencil of a point is the sum, over the micro-cells that have it as node l (on the cell's side of
+// face F for a face point), of row l of sum_q k_q M_{t,q}, keyed by the offset to the other node.
+constexpr int p2OffsetIndex( const P2StencilOffsets& o, int dx, int dy, int dz )
+{
+   for ( int e = 0; e < o.n; ++e )
+      if ( o.d[e][0] == dx && o.d[e][1] == dy && o.d[e][2] == dz )
+         return e;
+   return -1;
+}
+// offset index of node j seen from node l of micro-cell type t, for points of class CLS (face F)
+struct P2OffsetTable
+{
+   int e[6][10][10] = {};
+};
+constexpr P2OffsetTable p2OffsetTable( int cls, int f )
+{
+   P2OffsetTable          tb;
+   const P2StencilOffsets off = p2StencilOffsets( cls, f );
+   for ( int t = 0; t < 6; ++t )
+      for ( int l = 0; l < 10; ++l )
+         for ( int j = 0; j < 10; ++j )
+            tb.e[t][l][j] = p2OffsetIndex( off, p2NodeOffset( t, j, 0 ) - p2NodeOffset( t, l, 0 ), p2NodeOffset( t, j, 1 ) - p2NodeOffset( t, l, 1 ),
+                                           p2NodeOffset( t, j, 2 ) - p2NodeOffset( t, l, 2 ) );
+   return tb;
+}
+template < int CLS, int F >
+__device__ __forceinline__ void p2StorePoint( int c, int N, int px, int py, int pz, const double* __restrict__ el,
+                                              const double* __restrict__ kq, double* __restrict__ out, i64 estride )
+{
+   constexpr P2StencilOffsets off = p2StencilOffsets( CLS, F );
+   constexpr P2OffsetTable    tab = p2OffsetTable( CLS, F );
+   typedef const double __attribute__( ( address_space( 4 ) ) ) CD;
+   const CD* ec = (const CD*) ( el + i64( c ) * 6600 );
+   const int Nc = N >> 1, SA = S3i( Nc + 1 );
+   double    W[off.n];
+#pragma unroll
+   for ( int e = 0; e < off.n; ++e )
+      W[e] = 0.0;
+#pragma unroll
+   for ( int t = 0; t < 6; ++t )
+#pragma unroll
+      for ( int l = 0; l < 10; ++l )
+      {
+         if ( p2NodeClass( t, l ) != CLS || !p2OnCellSide( F, t, l ) )
+            continue;
+         const int     ax = ( px - p2NodeOffset( t, l, 0 ) ) >> 1, ay = ( py - p2NodeOffset( t, l, 1 ) ) >> 1,
+                       az = ( pz - p2NodeOffset( t, l, 2 ) ) >> 1;
+         const double* kv = kq + 11 * ( i64( c * 6 + t ) * SA + cidx( Nc + 1, ax, ay, az ) );
+         double        kk[11];
+#pragma unroll
+         for ( int q = 0; q < 11; ++q )
+            kk[q] = kv[q];
+#pragma unroll
+         for ( int j = 0; j < 10; ++j )
+         {
+            double r = 0.0;
+#pragma unroll
+            for ( int q = 0; q < 11; ++q )
+               r += kk[q] * ec[1100 * t + 100 * q + 10 * l + j];
+            W[tab.e[t][l][j]] += r;
+         }
+      }
+#pragma unroll
+   for ( int e = 0; e < off.n; ++e )
+      out[i64( e ) * estride] = W[e];
+}
+template < int CLS >
+__device__ __forceinline__ void p2StoreIntPt( int c, int s, int N, const double* __restrict__ el, const double* __restrict__ kq,
+                                              double* __restrict__ sw )
+{
+   constexpr int kx = CLS & 1, ky = ( CLS >> 1 ) & 1, kz = ( CLS >> 2 ) & 1;
+   const int     M  = p2InteriorExtent( N, CLS );
+   if ( M < 0 || s >= S3i( M + 1 ) )
+      return;
+   int qx, qy, qz;
+   tetXYZ( M + 1, s, qx, qy, qz );
+   // lattice point and its position (p - CLS) / 2 in the class sub-lattice
+   const int px = 2 * ( qx + 1 - kx ) + kx, py = 2 * ( qy + 1 - ky ) + ky, pz = 2 * ( qz + 1 - kz ) + kz;
+   const int SW = p2SubWidth( N, CLS );
+   double*   o  = sw + i64( c ) * p2StoredCellSize( N ) + p2StoredIntBase( N, CLS ) + cidx( SW, px >> 1, py >> 1, pz >> 1 );
+   p2StorePoint< CLS, -1 >( c, N, px, py, pz, el, kq, o, S3i( SW ) );
+}
+template < int CLS >
+__device__ __forceinline__ void p2StoreIntDispatch( int cls, int c, int s, int N, const double* el, const double* kq, double* sw )
+{
+   if ( cls == CLS )
+      p2StoreIntPt< CLS >( c, s, N, el, kq, sw );
+   else if constexpr ( CLS < 7 )
+      p2StoreIntDispatch< CLS + 1 >( cls, c, s, N, el, kq, sw );
+}
+__global__ __launch_bounds__( kBlock ) void kP2StoreInt( int L, P2ClassTiles ct, const double* __restrict__ el,
+                                                         const double* __restrict__ kq, double* __restrict__ sw )
+{
+   const int per = ct.start[8];
+   const int c   = int( blockIdx.x ) / per;
+   const int t   = int( blockIdx.x ) - c * per;
+   int       cls = 0;
+   while ( cls < 7 && t >= ct.start[cls + 1] )
+      ++cls;
+   p2StoreIntDispatch< 0 >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << L, el, kq, sw );
+}
+template < int F, int KU, int KV >
+__device__ __forceinline__ void p2StoreFacePt( int c, int s, int N, const double* __restrict__ el, const double* __restrict__ kq,
+                                               double* __restrict__ sw )
+{
+   const int M = ( N - 5 + KU + KV ) / 2;
+   if ( s >= triCount( M + 1 ) )
+      return;
+   int i, j;
+   planeXY( M + 1, s, i, j );
+   const int u = 2 * i + 2 - KU, v = 2 * j + 2 - KV;
+   const int px = F == 2 ? 0 : u, py = F == 0 || F == 3 ? v : ( F == 2 ? u : 0 ), pz = F == 0 ? 0 : ( F == 3 ? N - u - v : v );
+   double*   o  = sw + i64( c ) * p2StoredCellSize( N ) + p2StoredFaceBase( N, F, KU | ( KV << 1 ) ) + s;
+   p2StorePoint< p2FaceClass( F, KU, KV ), F >( c, N, px, py, pz, el, kq, o, triCount( M + 1 ) );
+}
+__global__ __launch_bounds__( kBlock ) void kP2StoreFace( int L, P2FaceTiles ft, const double* __restrict__ el,
+                                                          const double* __restrict__ kq, double* __restrict__ sw )
+{
+   const int per = ft.start[4];
+   const int c   = int( blockIdx.x ) / ( 4 * per );
+   int       r   = int( blockIdx.x ) - c * 4 * per;
+   const int f   = r / per;
+   r -= f * per;
+   int sub = 0;
+   while ( sub < 3 && r >= ft.start[sub + 1] )
+      ++sub;
+   const int s = ( r - ft.start[sub] ) * kBlock + int( threadIdx.x );
+   const int N = 1 << L;
+#define HYG_P2_STORE_FACE( F, SUB ) \
+   case 4 * F + SUB: p2StoreFacePt< F, ( SUB & 1 ), ( SUB >> 1 ) >( c, s, N, el, kq, sw ); break;
+   switch ( 4 * f + sub )
+   {
+      HYG_P2_STORE_FACE( 0, 0 )
+      HYG_P2_STORE_FACE( 0, 1 )
+      HYG_P2_STORE_FACE( 0, 2 )
+      HYG_P2_STORE_FACE( 0, 3 )
+      HYG_P2_STORE_FACE( 1, 0 )
+      HYG_P2_STORE_FACE( 1, 1 )
+      HYG_P2_STORE_FACE( 1, 2 )
+      HYG_P2_STORE_FACE( 1, 3 )
+      HYG_P2_STORE_FACE( 2, 0 )
+      HYG_P2_STORE_FACE( 2, 1 )
+      HYG_P2_STORE_FACE( 2, 2 )
+      HYG_P2_STORE_FACE( 2, 3 )
+      HYG_P2_STORE_FACE( 3, 0 )
+      HYG_P2_STORE_FACE( 3, 1 )
+      HYG_P2_STORE_FACE( 3, 2 )
+   default: p2StoreFacePt< 3, 1, 1 >( c, s, N, el, kq, sw ); break;
+   }
+#undef HYG_P2_STORE_FACE
 }
 
 // ------------------------------------------------------------------ prolongation
@@ -787,7 +1019,7 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
       const int wpc = p2RowWavesPerCell( 1 << L );
       const T*  stw = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
       if ( wpc > 0 && ( c.flag & 1 ) )
-         hipLaunchKernelGGL( ( kP2Rows< T, MODE > ), dim3( xcdGrid( ceilDiv( i64( nC ) * wpc, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, L,
+         hipLaunchKernelGGL( ( kP2Rows< T, MODE, false > ), dim3( xcdGrid( ceilDiv( i64( nC ) * wpc, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, L,
                              nC, g.stride, wpc, stw, (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
       P2FaceTiles ft;
       ft.start[0] = 0;
@@ -795,7 +1027,27 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
          ft.start[k + 1] = ft.start[k] + ceilDiv( p2FaceSubCount( 1 << L, k & 1, k >> 1 ), kBlock );
       const T* fsw = dbl ? (const T*) ol.p2fst : (const T*) ol.p2fstF;
       if ( ft.start[4] > 0 )
-         hipLaunchKernelGGL( ( kP2Face< T > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, fsw, m.cellFaces,
+         hipLaunchKernelGGL( ( kP2Face< T, false > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, fsw, m.cellFaces,
+                             c.flag, part, (const T*) c.x );
+      const int tB = ceilDiv( p2EdgeVertexCount( 1 << L ), kBlock );
+      hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq,
+                          m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
+                          (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
+   }
+   else if ( KQ && !DIAG && ol.p2s )
+   {
+      // variable coefficient with stored stencils: the same row and face kernels with per-point weights
+      const T*  sw  = dbl ? (const T*) ol.p2s : (const T*) ol.p2sF;
+      const int wpc = p2RowWavesPerCell( 1 << L );
+      if ( wpc > 0 && ( c.flag & 1 ) )
+         hipLaunchKernelGGL( ( kP2Rows< T, MODE, true > ), dim3( xcdGrid( ceilDiv( i64( nC ) * wpc, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, L,
+                             nC, g.stride, wpc, sw, (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
+      P2FaceTiles ft;
+      ft.start[0] = 0;
+      for ( int k = 0; k < 4; ++k )
+         ft.start[k + 1] = ft.start[k] + ceilDiv( p2FaceSubCount( 1 << L, k & 1, k >> 1 ), kBlock );
+      if ( ft.start[4] > 0 )
+         hipLaunchKernelGGL( ( kP2Face< T, true > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, sw, m.cellFaces,
                              c.flag, part, (const T*) c.x );
       const int tB = ceilDiv( p2EdgeVertexCount( 1 << L ), kBlock );
       hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq,
@@ -870,6 +1122,38 @@ void launchP2Coefficients( const MeshCtx& m, const hyg_op_s& op, int L, hipStrea
    const P2Coef           k{ op.p2k.kind, op.p2k.k0, op.p2k.alpha, op.p2k.beta, op.p2k.seed };
    hipLaunchKernelGGL( ( kP2KQ< double > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, ol.elem, k, ol.p2kq );
    hipLaunchKernelGGL( ( kP2KQ< float > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, ol.elem, k, ol.p2kqF );
+}
+
+// stored stencils of the variable-coefficient P2 operator at lattice level L (FP64, and an FP32 copy)
+void launchP2StoreStencils( const MeshCtx& m, hyg_op_s& op, int L, hipStream_t s )
+{
+   hyg_op_s::Level& ol = op.levels[L];
+   const int        nC = m.mesh->nC(), N = 1 << L;
+   const i64        n  = i64( nC ) * p2StoredCellSize( N );
+   if ( !ol.p2s )
+   {
+      if ( hipMalloc( &ol.p2s, size_t( n ) * sizeof( double ) ) != hipSuccess ||
+           hipMalloc( &ol.p2sF, size_t( n ) * sizeof( float ) ) != hipSuccess )
+         throw std::runtime_error( "hipMalloc failed for the stored P2 stencils" );
+   }
+   // positions of a class sub-lattice that are no cell-interior points keep weight 0
+   (void) hipMemsetAsync( ol.p2s, 0, size_t( n ) * sizeof( double ), s );
+   P2ClassTiles ct;
+   ct.start[0] = 0;
+   for ( int k = 0; k < 8; ++k )
+   {
+      const int M     = p2InteriorExtent( N, k );
+      ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
+   }
+   if ( ct.start[8] > 0 )
+      hipLaunchKernelGGL( kP2StoreInt, dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, ct, ol.p2el, ol.p2kq, ol.p2s );
+   P2FaceTiles ft;
+   ft.start[0] = 0;
+   for ( int k = 0; k < 4; ++k )
+      ft.start[k + 1] = ft.start[k] + ceilDiv( p2FaceSubCount( N, k & 1, k >> 1 ), kBlock );
+   if ( ft.start[4] > 0 )
+      hipLaunchKernelGGL( kP2StoreFace, dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, ft, ol.p2el, ol.p2kq, ol.p2s );
+   launchConvert( n, 4, ol.p2sF, 8, ol.p2s, s );
 }
 
 // inverse diagonal of a P2 operator at lattice level L into its per-point arrays (FP64 and FP32 copy)

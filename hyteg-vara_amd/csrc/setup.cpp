@@ -865,6 +865,8 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.p2stF );
       (void) hipFree( l.p2fst );
       (void) hipFree( l.p2fstF );
+      (void) hipFree( l.p2s );
+      (void) hipFree( l.p2sF );
    }
    (void) hipFree( modes );
 }

@@ -1,7 +1,7 @@
 """GPU parity of the spherical shell mesh and of the P2 variable-coefficient operator (P2ElementwiseOperator with
 p2_div_k_grad_affine_q4) against the oracle, through the C ABI: the shell's DoF coordinates and boundary flags
 agree exactly; FP64 apply, inverse diagonal and Jacobi sweep within 1e-13 relative l-inf; FP32 apply within 1e-5;
-V-cycle residual history on the shell within 1e-10 per cycle."""
+V-cycle residual history on the shell within 1e-10 per cycle; the same with stored per-point stencils."""
 import numpy as np
 import pytest
 
@@ -58,12 +58,16 @@ def test_p1_apply_on_the_shell(hyg, meshes):
 
 @pytest.mark.parametrize("name,L", [("cube_24el.msh", 1), ("cube_24el.msh", 3), ("shell", 1), ("shell", 2)])
 @pytest.mark.parametrize("coef", ["exp", "poly"])
-def test_apply_diagonal_smoother_parity(hyg, meshes, name, L, coef):
+@pytest.mark.parametrize("stored", [False, True])
+def test_apply_diagonal_smoother_parity(hyg, meshes, name, L, coef, stored):
+    """On the fly, and with the per-point stencils stored (computeAndStoreLocalElementMatrices)."""
     gm, om = meshes[name]
     kw = COEF if coef == "exp" else dict(kind=1)
     n = gm.ndofs(L + 1)
     x0, b0, d0 = rnd(n, 2), rnd(n, 3), rnd(n, 4)
     op = hyg.P2DivKGradOperator(gm, L, L, **kw)
+    if stored:
+        op.store_local_matrices()
     oop = ol.P2DivKGradOperator(om, L, L, **kw)
     s, d = hyg.Function(gm, L, L, space="p2"), hyg.Function(gm, L, L, space="p2")
     os_, od = ol.Function(om, L + 1, L + 1), ol.Function(om, L + 1, L + 1)
@@ -96,7 +100,8 @@ def test_apply_diagonal_smoother_parity(hyg, meshes, name, L, coef):
     assert rel_inf(y32.download(L), d.download(L)) <= 1e-5
 
 
-def test_vcycle_history_on_the_shell(hyg, meshes):
+@pytest.mark.parametrize("stored", [False, True])
+def test_vcycle_history_on_the_shell(hyg, meshes, stored):
     gm, om = meshes["shell"]
     minL, maxL, cycles = 0, 2, 5
     n = gm.ndofs(maxL + 1)
@@ -105,6 +110,8 @@ def test_vcycle_history_on_the_shell(hyg, meshes):
     b0[bnd] = 0.0
     kw = dict(pre=2, post=2, smoother="jacobi")
     op = hyg.P2DivKGradOperator(gm, minL, maxL, **COEF)
+    if stored:
+        op.store_local_matrices()
     x, b = hyg.Function(gm, minL, maxL, space="p2"), hyg.Function(gm, minL, maxL, space="p2")
     b.upload(maxL, b0)
     got = hyg.GMG(op, minL, maxL, **kw).solve(x, b, maxL, cycles)
