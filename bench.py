@@ -246,8 +246,9 @@ def bench_shell(hyg, hip, st, steps, level, seed, solve_cycles):
     """Spherical shell ntan = 5, nrad = 2, r in [0.55, 1] (960 macro-cells), P2 level 5: the vector Laplacian
     -div(eta grad u_i), i = 1..3, with eta(x) = exp(2.99 (1 - |x|)) (1 + 0.1 xi_e), xi_e uniform in [-1, 1) per
     micro-element (P2ElementwiseOperator with p2_div_k_grad_affine_q4; the same scalar operator on each
-    component). Apply throughput of the three components in FP64 and FP32; one component solved to 1e-10 with
-    V(3,3) Chebyshev cycles in FP64 and as FP32 cycles inside FP64 defect correction."""
+    component). Apply throughput of the three components in FP64 and FP32, with the element rows assembled per
+    apply and with stored per-point stencils; one component solved to 1e-10 with V(3,3) Chebyshev cycles in FP64
+    and as FP32 cycles inside FP64 defect correction (stored stencils)."""
     mesh = hyg.Mesh("shell", 5, 2, 0.55, 1.0)
     L = level
     n = mesh.ndofs(L + 1)
@@ -272,6 +273,25 @@ def bench_shell(hyg, hip, st, steps, level, seed, solve_cycles):
         ap()
         ms = timed(hip, st, ap, steps) / steps
         out[f"vector_apply_{name}"] = {"ms": round(ms, 3), "GDoF/s": round(3 * inner / (ms / 1e3) / 1e9, 3)}
+        del xs, ys
+    # the same operator with its per-point stencils stored (computeAndStoreLocalElementMatrices): the solves use it
+    hip.sync(st)
+    t0 = time.perf_counter()
+    op.store_local_matrices()
+    out["store_local_matrices_seconds"] = round(time.perf_counter() - t0, 3)
+    for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):
+        xs = [hyg.Function(mesh, L, L, dt, space="p2") for _ in range(3)]
+        ys = [hyg.Function(mesh, L, L, dt, space="p2") for _ in range(3)]
+        for x in xs:
+            x.upload(L, rng.uniform(-1, 1, n))
+
+        def ap():
+            for x, y in zip(xs, ys):
+                hyg.check(lib.hyg_apply(op.h, x.h, y.h, L, hyg.INNER, hyg.REPLACE, st))
+
+        ap()
+        ms = timed(hip, st, ap, steps) / steps
+        out[f"vector_apply_stored_{name}"] = {"ms": round(ms, 3), "GDoF/s": round(3 * inner / (ms / 1e3) / 1e9, 3)}
         del xs, ys
     b = hyg.Function(mesh, 0, L, space="p2")
     bv = rng.uniform(-1.0, 1.0, n)

@@ -446,7 +446,7 @@ __host__ __device__ constexpr int p2SegRows( int N, int k )
    return triCount( N - 2 - p2SegStart( k ) );
 }
 
-template < typename T, int MODE, int KY, int KZ, bool STORED >
+template < typename T, int MODE, int KY, int KZ >
 __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_buffer_rsrc_t rx, const T* __restrict__ wE,
                                            const T* __restrict__ wO, i64 cellOff, const T* __restrict__ x, const T* __restrict__ in1,
                                            T* __restrict__ out0, T* __restrict__ out1, const T* __restrict__ dinv, T a )
@@ -467,52 +467,20 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
       const unsigned vo   = ( xe >= 0 && xe < W - yy ) ? unsigned( base + xe ) * unsigned( sizeof( T ) ) : kDropOffset;
       bufLoad2( rx, vo, 0, vEr[r], vOr[r] );
    }
-   // constant coefficient: the weights are the same for every lane, read through the constant address space
-   // they become scalar loads. Stored stencils: wE / wO point at the cell's class arrays ([e][sub-lattice
-   // point]); the lane's two points are (xe / 2, (y - KY) / 2, (z - KZ) / 2) of their sub-lattices (lanes
-   // without a point in the row read point 0, whose weights are finite and unused).
+   // the weights are the same for every lane: read through the constant address space they become scalar loads
    typedef const T __attribute__( ( address_space( 4 ) ) ) CT;
    const CT* wEr = (const CT*) wE;
    const CT* wOr = (const CT*) wO;
-   const int SWE = p2SubWidth( N, cE ), SWO = p2SubWidth( N, cO );
-   const int okE = xe >= 0 && xe < w - y - z, okO = xe >= 0 && xe + 1 < w - y - z;
-   const T*  pE  = wE + ( okE ? cidx( SWE, xe >> 1, ( y - KY ) >> 1, ( z - KZ ) >> 1 ) : 0 );
-   const T*  pO  = wO + ( okO ? cidx( SWO, xe >> 1, ( y - KY ) >> 1, ( z - KZ ) >> 1 ) : 0 );
-   const int nE  = S3i( SWE ), nO = S3i( SWO );
-   auto      wtE = [&]( int e ) -> T { return STORED ? pE[i64( e ) * nE] : wEr[e]; };
-   auto      wtO = [&]( int e ) -> T { return STORED ? pO[i64( e ) * nO] : wOr[e]; };
 #pragma unroll
    for ( int r = 0; r < rs.n; ++r )
    {
       const int dy = rs.d[r][0], dz = rs.d[r][1];
       const T   vE = vEr[r], vO = vOr[r];
-      if constexpr ( STORED )
-      {
-         // per-point weights: a term belongs to the lane whose point it is, so the neighbours' values move
-         // instead of the partial sums (x - 2, x - 1 from lane i - 1; x + 1, x + 2 from lane i + 1)
-         const T vEm = laneShift< kDppWaveShr1 >( vE ), vOm = laneShift< kDppWaveShr1 >( vO );
-         const T vEp = laneShift< kDppWaveShl1 >( vE ), vOp = laneShift< kDppWaveShl1 >( vO );
-#pragma unroll
-         for ( int e = 0; e < oE.n; ++e )
-            if ( oE.d[e][1] == dy && oE.d[e][2] == dz )
-            {
-               const int dx = oE.d[e][0];
-               aE += wtE( e ) * ( dx == -2 ? vEm : dx == -1 ? vOm : dx == 0 ? vE : dx == 1 ? vO : vEp );
-            }
-#pragma unroll
-         for ( int e = 0; e < oO.n; ++e )
-            if ( oO.d[e][1] == dy && oO.d[e][2] == dz )
-            {
-               const int dx = oO.d[e][0];
-               aO += wtO( e ) * ( dx == -2 ? vOm : dx == -1 ? vE : dx == 0 ? vO : dx == 1 ? vEp : vOp );
-            }
-         continue;
-      }
 #pragma unroll
       for ( int e = 0; e < oE.n; ++e )
          if ( oE.d[e][1] == dy && oE.d[e][2] == dz )
          {
-            const T wt = wtE( e );
+            const T wt = wEr[e];
             switch ( oE.d[e][0] )
             {
             case -2: rE += wt * vE; break;
@@ -526,7 +494,7 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
       for ( int e = 0; e < oO.n; ++e )
          if ( oO.d[e][1] == dy && oO.d[e][2] == dz )
          {
-            const T wt = wtO( e );
+            const T wt = wOr[e];
             switch ( oO.d[e][0] )
             {
             case -2: rO += wt * vO; break;
@@ -560,8 +528,8 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
    T             dv[2] = { T( 1 ), T( 1 ) };
    if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
    {
-      dv[0] = T( 1 ) / wtE( p2CentreIndex( oE ) );
-      dv[1] = T( 1 ) / wtO( p2CentreIndex( oO ) );
+      dv[0] = T( 1 ) / wEr[p2CentreIndex( oE )];
+      dv[1] = T( 1 ) / wOr[p2CentreIndex( oO )];
    }
    (void) dinv;
 #pragma unroll
@@ -579,7 +547,7 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
    }
 }
 
-template < typename T, int MODE, bool STORED >
+template < typename T, int MODE >
 __global__ __launch_bounds__( kBlock ) void kP2Rows( int L, int nC, i64 stride, int wavesPerCell, const T* __restrict__ st,
                                                      const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
                                                      T* __restrict__ out1, const T* __restrict__ dinv, T a )
@@ -599,16 +567,13 @@ __global__ __launch_bounds__( kBlock ) void kP2Rows( int L, int nC, i64 stride, 
    const int  y = yp + 1, z = zp + 1;
    const i64  cellOff = i64( c ) * stride;
    const auto rx      = makeRsrc( x + cellOff, unsigned( S3i( N + 1 ) ) * unsigned( sizeof( T ) ) );
-   // weights of class cls: per cell (constant coefficient) or the cell's stored class array
-   auto wcls = [&]( int cls ) -> const T* {
-      return STORED ? st + i64( c ) * p2StoredCellSize( N ) + p2StoredIntBase( N, cls ) : st + ( i64( c ) * 8 + cls ) * kP2StencilMax;
-   };
+   auto       wcls    = [&]( int cls ) -> const T* { return st + ( i64( c ) * 8 + cls ) * kP2StencilMax; };
    switch ( ( y & 1 ) | ( ( z & 1 ) << 1 ) )
    {
-   case 0: p2RowPair< T, MODE, 0, 0, STORED >( N, y, z, k, rx, wcls( 0 ), wcls( 1 ), cellOff, x, in1, out0, out1, dinv, a ); break;
-   case 1: p2RowPair< T, MODE, 1, 0, STORED >( N, y, z, k, rx, wcls( 2 ), wcls( 3 ), cellOff, x, in1, out0, out1, dinv, a ); break;
-   case 2: p2RowPair< T, MODE, 0, 1, STORED >( N, y, z, k, rx, wcls( 4 ), wcls( 5 ), cellOff, x, in1, out0, out1, dinv, a ); break;
-   default: p2RowPair< T, MODE, 1, 1, STORED >( N, y, z, k, rx, wcls( 6 ), wcls( 7 ), cellOff, x, in1, out0, out1, dinv, a ); break;
+   case 0: p2RowPair< T, MODE, 0, 0 >( N, y, z, k, rx, wcls( 0 ), wcls( 1 ), cellOff, x, in1, out0, out1, dinv, a ); break;
+   case 1: p2RowPair< T, MODE, 1, 0 >( N, y, z, k, rx, wcls( 2 ), wcls( 3 ), cellOff, x, in1, out0, out1, dinv, a ); break;
+   case 2: p2RowPair< T, MODE, 0, 1 >( N, y, z, k, rx, wcls( 4 ), wcls( 5 ), cellOff, x, in1, out0, out1, dinv, a ); break;
+   default: p2RowPair< T, MODE, 1, 1 >( N, y, z, k, rx, wcls( 6 ), wcls( 7 ), cellOff, x, in1, out0, out1, dinv, a ); break;
    }
 }
 
@@ -734,6 +699,77 @@ __global__ __launch_bounds__( kBlock ) void kP2RestrictInt( int Lc, i64 strideC,
    while ( cls < 7 && t >= ct.start[cls + 1] )
       ++cls;
    p2RestrictDispatch< T, 0 >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << Lc, strideC, strideF, coarse, fine );
+}
+
+// Stored stencils, cell-interior points one class per wave (the class sub-lattice, p2InteriorExtent): each
+// thread gathers its point's neighbours and reads its own weights ([e][sub-lattice point], consecutive threads
+// read consecutive weights), then the mode's epilogue; the inverse diagonal is the inverse centre weight.
+template < typename T, int MODE, int CLS >
+__device__ __forceinline__ void p2StoredIntPt( int c, int s, int N, i64 stride, const T* __restrict__ sw, const T* __restrict__ x,
+                                               const T* __restrict__ in1, T* __restrict__ out0, T* __restrict__ out1, T a )
+{
+   constexpr int              kx = CLS & 1, ky = ( CLS >> 1 ) & 1, kz = ( CLS >> 2 ) & 1;
+   constexpr P2StencilOffsets off = p2StencilOffsets( CLS );
+   const int                  M   = p2InteriorExtent( N, CLS );
+   if ( M < 0 || s >= S3i( M + 1 ) )
+      return;
+   int qx, qy, qz;
+   tetXYZ( M + 1, s, qx, qy, qz );
+   const int px = 2 * ( qx + 1 - kx ) + kx, py = 2 * ( qy + 1 - ky ) + ky, pz = 2 * ( qz + 1 - kz ) + kz;
+   const int w = N + 1, SW = p2SubWidth( N, CLS ), nS = S3i( SW );
+   const T*  ws = sw + i64( c ) * p2StoredCellSize( N ) + p2StoredIntBase( N, CLS ) + cidx( SW, px >> 1, py >> 1, pz >> 1 );
+   const T*  xc = x + i64( c ) * stride;
+   int       rb[5][5];
+#pragma unroll
+   for ( int dz = -2; dz <= 2; ++dz )
+   {
+      const int W  = w - ( pz + dz );
+      const int pb = S3i( w ) - S3i( W );
+#pragma unroll
+      for ( int dy = -2; dy <= 2; ++dy )
+         rb[dz + 2][dy + 2] = pb + rowStart( W, py + dy );
+   }
+   T val = T( 0 );
+#pragma unroll
+   for ( int e = 0; e < off.n; ++e )
+      val += ws[i64( e ) * nS] * xc[rb[off.d[e][2] + 2][off.d[e][1] + 2] + px + off.d[e][0]];
+   const i64 q = i64( c ) * stride + rb[2][2] + px;
+   T         i1 = T( 0 ), r0, r1, xv = T( 0 ), dv = T( 1 );
+   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      i1 = in1[q];
+   if constexpr ( MODE == MODE_APPLY_ADD )
+      i1 = out0[q];
+   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
+      xv = x[q];
+   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      dv = T( 1 ) / ws[i64( p2CentreIndex( off ) ) * nS];
+   epilogue< T, MODE >( val, xv, i1, i1, dv, a, r0, r1 );
+   out0[q] = r0;
+   if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+      out1[q] = r1;
+}
+template < typename T, int MODE, int CLS >
+__device__ __forceinline__ void p2StoredIntDispatch( int cls, int c, int s, int N, i64 stride, const T* sw, const T* x, const T* in1,
+                                                     T* out0, T* out1, T a )
+{
+   if ( cls == CLS )
+      p2StoredIntPt< T, MODE, CLS >( c, s, N, stride, sw, x, in1, out0, out1, a );
+   else if constexpr ( CLS < 7 )
+      p2StoredIntDispatch< T, MODE, CLS + 1 >( cls, c, s, N, stride, sw, x, in1, out0, out1, a );
+}
+template < typename T, int MODE >
+__global__ __launch_bounds__( kBlock ) void kP2StoredInt( int L, i64 stride, P2ClassTiles ct, const T* __restrict__ sw,
+                                                          const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
+                                                          T* __restrict__ out1, T a )
+{
+   const int per = ct.start[8];
+   const int c   = int( blockIdx.x ) / per;
+   const int t   = int( blockIdx.x ) - c * per;
+   int       cls = 0;
+   while ( cls < 7 && t >= ct.start[cls + 1] )
+      ++cls;
+   p2StoredIntDispatch< T, MODE, 0 >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << L, stride, sw, x, in1, out0, out1,
+                                      a );
 }
 
 // ------------------------------------------------------------------ stored stencils (variable coefficient)
@@ -1019,7 +1055,7 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
       const int wpc = p2RowWavesPerCell( 1 << L );
       const T*  stw = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
       if ( wpc > 0 && ( c.flag & 1 ) )
-         hipLaunchKernelGGL( ( kP2Rows< T, MODE, false > ), dim3( xcdGrid( ceilDiv( i64( nC ) * wpc, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, L,
+         hipLaunchKernelGGL( ( kP2Rows< T, MODE > ), dim3( xcdGrid( ceilDiv( i64( nC ) * wpc, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, L,
                              nC, g.stride, wpc, stw, (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
       P2FaceTiles ft;
       ft.start[0] = 0;
@@ -1037,11 +1073,17 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
    else if ( KQ && !DIAG && ol.p2s )
    {
       // variable coefficient with stored stencils: the same row and face kernels with per-point weights
-      const T*  sw  = dbl ? (const T*) ol.p2s : (const T*) ol.p2sF;
-      const int wpc = p2RowWavesPerCell( 1 << L );
-      if ( wpc > 0 && ( c.flag & 1 ) )
-         hipLaunchKernelGGL( ( kP2Rows< T, MODE, true > ), dim3( xcdGrid( ceilDiv( i64( nC ) * wpc, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, L,
-                             nC, g.stride, wpc, sw, (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
+      const T*     sw = dbl ? (const T*) ol.p2s : (const T*) ol.p2sF;
+      P2ClassTiles ct;
+      ct.start[0] = 0;
+      for ( int k = 0; k < 8; ++k )
+      {
+         const int M     = p2InteriorExtent( 1 << L, k );
+         ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
+      }
+      if ( ct.start[8] > 0 && ( c.flag & 1 ) )
+         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw,
+                             (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
       P2FaceTiles ft;
       ft.start[0] = 0;
       for ( int k = 0; k < 4; ++k )
