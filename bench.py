@@ -150,6 +150,20 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
     # coefficient evaluation: per lattice anchor 32 modes x (3 FMA phase + 30 x 2 FMA) + 30 exp + 6 x 5 FMA
     anchors = int(mesh.info()[3]) * ((1 << L) * ((1 << L) + 1) * ((1 << L) + 2) // 6)
     out["kbar_flops_per_apply"] = anchors * (32 * (6 + 120) + 60)
+    # a second operator with the coefficient integrals kept (computeAndStoreLocalElementMatrices)
+    ops = hyg.P1DivKGradOperator(mesh, 0, L, 1.0, modes)
+    ops.store_local_matrices()
+    for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):
+        x, y = hyg.Function(mesh, L, L, dt), hyg.Function(mesh, L, L, dt)
+        x.upload(L, rng.uniform(-1, 1, n))
+
+        def aps():
+            hyg.check(lib.hyg_apply(ops.h, x.h, y.h, L, hyg.INNER, hyg.REPLACE, st))
+
+        for _ in range(2):
+            aps()
+        ms = timed(hip, st, aps, steps) / steps
+        out[f"apply_stored_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3)}
     b = hyg.Function(mesh, 0, L)
     bv = rng.uniform(-1.0, 1.0, n)
     bv[bnd] = 0.0
@@ -179,14 +193,29 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
                     "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "rel_residual": float(rmx[-1] / rmx[0]),
                               "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
                     "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
+    # the same solves with the coefficient integrals kept
+    g64s = hyg.GMG(ops, 0, L, hyg.F64, **kw)
+    g32s = hyg.GMG(ops, 0, L, hyg.F32, **kw)
+    for f in (x64, xmx):
+        f.interpolate(0.0, L)
+    hip.sync(st)
+    t0 = time.perf_counter()
+    r64 = g64s.solve_tol(x64, b, L, 100, 1e-10, st)
+    t64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rmx = hyg.ir_solve(ops, g32s, xmx, b, L, inner=1, max_outer=100, rel_tol=1e-10, stream=st)
+    tmx = time.perf_counter() - t0
+    out["solve_stored"] = {"fp64": {"seconds": round(t64, 4), "cycles": len(r64) - 1, "rel_residual": float(r64[-1] / r64[0])},
+                           "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1,
+                                     "rel_residual": float(rmx[-1] / rmx[0])},
+                           "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
     return out
 
 
 def bench_p2(hyg, hip, st, steps, level, seed):
     """P2 constant-coefficient -Laplace on the 24-tet cube (P2 level 7 has the DoFs of P1 level 8): apply
-    throughput in FP64 and FP32 (element by element: per DoF 75 multiply-adds on average, 24 micro-cells x 10
-    at vertex DoFs, 4-6 x 10 at edge DoFs), and V(3,3) Chebyshev cycles to 1e-10 in FP64 and as FP32 cycles
-    inside FP64 defect correction."""
+    throughput in FP64 and FP32 (merged per-class stencils: 65 terms at vertex DoFs, 19-27 at edge DoFs), and
+    V(3,3) Chebyshev cycles to 1e-10 in FP64 and as FP32 cycles inside FP64 defect correction."""
     mesh = hyg.Mesh("cube_24el.msh")
     L = level
     n = mesh.ndofs(L + 1)

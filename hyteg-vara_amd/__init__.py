@@ -284,6 +284,10 @@ class P1DivKGradOperator(P1ConstantOperator):
         check(load().hyg_op_create_p1_divkgrad(mesh.h, minL, maxL, float(k0), md, nm, C.byref(h)))
         self.h = h
 
+    def store_local_matrices(self):
+        """Keep the coefficient integrals of every micro-element (computeAndStoreLocalElementMatrices)."""
+        check(load().hyg_op_store_local_matrices(self.h))
+
 
 class P2ConstantOperator(P1ConstantOperator):
     """P2 -Laplace on P2 levels minL..maxL (functions created with space="p2")."""

@@ -976,12 +976,18 @@ int hyg_op_store_local_matrices( hyg_op op )
    HYG_TRY( checkInit() );
    if ( !op )
       return fail( HYG_ERR_ARG, "null operator" );
-   if ( op->kind != 3 )
-      return fail( HYG_ERR_ARG, "stored local matrices are for the P2 variable-coefficient operator" );
+   if ( op->kind != 1 && op->kind != 3 )
+      return fail( HYG_ERR_ARG, "stored local matrices are for the variable-coefficient operators" );
    try
    {
-      for ( int L = op->minL; L <= op->maxL; ++L )
-         launchP2StoreStencils( op->mesh->ctx, *op, L, nullptr );
+      if ( op->kind == 1 )
+         launchElementwiseStore( op->mesh->ctx, *op, nullptr );
+      else
+      {
+         for ( int L = op->minL; L <= op->maxL; ++L )
+            launchP2StoreStencils( op->mesh->ctx, *op, L, nullptr );
+         op->stored = true;
+      }
    } catch ( const std::exception& e )
    {
       return fail( HYG_ERR_HIP, e.what() );

@@ -225,6 +225,30 @@ __global__ __launch_bounds__( kBlock ) void kElemApply( int L, int tilesPerCell,
 }
 
 // ------------------------------------------------------------------ launch
+template < typename T >
+static void launchKBar( const hyg_op_s& op, int L, int nC, T* kb, hipStream_t s )
+{
+   const int N  = 1 << L;
+   const int tK = ceilDiv( S3i( N ), kBlock );
+   if ( tK > 0 && S3i( N ) > 0 )
+   {
+      const size_t lds = size_t( op.nModes ) * ( 5 * sizeof( double ) + 60 * sizeof( T ) );
+      hipLaunchKernelGGL( ( kKBar< T > ), dim3( nC * tK ), dim3( kBlock ), lds, s, L, tK, op.levels[L].elem, op.modes, op.nModes, op.k0,
+                          kb );
+   }
+}
+
+// coefficient integrals of every level in both value types, kept for the applies that follow
+void launchElementwiseStore( const MeshCtx& m, hyg_op_s& op, hipStream_t s )
+{
+   for ( int L = op.minL; L <= op.maxL; ++L )
+   {
+      launchKBar< double >( op, L, m.mesh->nC(), op.levels[L].kb, s );
+      launchKBar< float >( op, L, m.mesh->nC(), op.levels[L].kbF, s );
+   }
+   op.stored = true;
+}
+
 template < typename T, int MODE, bool DIAG >
 static void elemT( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s )
 {
@@ -240,13 +264,9 @@ static void elemT( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, co
    const T*                   in1  = (const T*) c.in1;
    T*                         out0 = (T*) c.out0;
    T*                         out1 = (T*) c.out1;
-   // 1. coefficient integrals of all micro-cells
-   const int tK = ceilDiv( S3i( N ), kBlock );
-   if ( tK > 0 && S3i( N ) > 0 )
-   {
-      const size_t lds = size_t( op.nModes ) * ( 5 * sizeof( double ) + 60 * sizeof( T ) );
-      hipLaunchKernelGGL( ( kKBar< T > ), dim3( nC * tK ), dim3( kBlock ), lds, s, L, tK, ol.elem, op.modes, op.nModes, op.k0, kb );
-   }
+   // 1. coefficient integrals of all micro-cells (kept from hyg_op_store_local_matrices when stored)
+   if ( !op.stored )
+      launchKBar< T >( op, L, nC, kb, s );
    // 2. per-cell stencils (interior results, one-sided sums of face, edge and vertex points)
    const int tA = ceilDiv( S3i( w ), kBlock );
    hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem, kb,

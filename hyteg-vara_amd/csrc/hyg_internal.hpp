@@ -547,9 +547,12 @@ struct hyg_op_s
    int         lat = 0;    // 0: P1 operator, 1: P2 operator (its functions hold lattice level L+1)
    // 0: constant-coefficient stencils (P1ConstantLaplaceOperator)
    // 1: elementwise -div(k grad) with k = k0 exp( sum_m a_m cos(2 pi kappa_m . x + phi_m) )
-   // 2: P2 constant-coefficient -Laplace (P2ConstantLaplaceOperator), element by element
+   // 2: P2 constant-coefficient -Laplace (P2ConstantLaplaceOperator), merged stencils
    // 3: P2 elementwise -div(k grad) (p2_div_k_grad_affine_q4, 11-point rule), k of P2Coefficient
    int                   kind = 0;
+   // kind 1 after hyg_op_store_local_matrices: the coefficient integrals kb of every level are kept (not
+   // re-evaluated per apply); kind 3 keeps per-point stencils in Level::p2s instead
+   bool                  stored = false;
    struct P2Coefficient
    {
       int      kind  = 0; // 0: k0 exp(alpha (1 - |x|)) (1 + beta xi_e), 1: 1 + x/2 + y^2/4

@@ -36,6 +36,7 @@ void launchStencil( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, i
 void launchElementwise( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c,
                         hipStream_t s );
 void launchElementwiseInverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );
+void launchElementwiseStore( const MeshCtx& m, hyg_op_s& op, hipStream_t s );
 // P2 operator (kind 2) at lattice level L: the stencil modes, the inverse diagonal, and the quadratic transfers
 // (fine lattice Lf from Lf-1; coarse lattice Lc from Lc+1)
 void launchP2( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c, hipStream_t s );

@@ -143,11 +143,14 @@ int hyg_op_create_p2_const( hyg_mesh mesh, int min_level, int max_level, hyg_op*
  * element rows on the fly. Supports what hyg_op_create_p2_const supports. */
 int hyg_op_create_p2_divkgrad( hyg_mesh mesh, int min_level, int max_level, int kind, double k0, double alpha, double beta,
                                uint64_t seed, hyg_op* out );
-/* Replaces P2ElementwiseOperator::computeAndStoreLocalElementMatrices (elementwiseoperators/
- * P2ElementwiseOperator.cpp:503-580) for the variable-coefficient P2 operator: assembles, on every level, the
- * stencil of every cell-interior point and the one-sided stencil of every cell-face point from the element
- * matrices (about 29 values per point on average, FP64 plus an FP32 copy) and applies those from then on; points
- * on cell edges and vertices stay element by element. Trades HBM for the per-apply assembly. */
+/* Replaces computeAndStoreLocalElementMatrices of the elementwise operators (elementwiseoperators/
+ * P2ElementwiseOperator.cpp:503-580, P1ElementwiseOperator.cpp:418). For the variable-coefficient P2
+ * operator: assembles, on every level, the stencil of every cell-interior point and the one-sided stencil of
+ * every cell-face point from the element matrices (about 29 values per point on average, FP64 plus an FP32 copy)
+ * and applies those from then on; points on cell edges and vertices stay element by element. For the
+ * variable-coefficient P1 operator: keeps the coefficient integral of every micro-element (the element matrix is
+ * that integral times the cell's matrix of its type) instead of evaluating k in every apply. Trades HBM for the
+ * per-apply work; results are unchanged up to rounding (P2) or bit for bit (P1). */
 int hyg_op_store_local_matrices( hyg_op op );
 /* Cell stencil (15 weights in the order C W E N S NW SE TC TW TS TSE BC BN BE BNW) of cell `cell`
  * (constant-coefficient operators only). */

@@ -57,11 +57,15 @@ def test_apply_and_diagonal_parity(hyg, meshes, name, L, field):
 
 
 @pytest.mark.parametrize("L", [3, 5])
-def test_smoother_residual_and_fp32(hyg, meshes, L):
+@pytest.mark.parametrize("stored", [False, True])
+def test_smoother_residual_and_fp32(hyg, meshes, L, stored):
+    """On the fly, and with the coefficient integrals kept (computeAndStoreLocalElementMatrices)."""
     gm, om = meshes["cube_24el.msh"]
     n = gm.ndofs(L)
     x0, b0, d0 = rnd(n, 1), rnd(n, 2), rnd(n, 3)
     op = hyg.P1DivKGradOperator(gm, L, L, 1.3, MODES)
+    if stored:
+        op.store_local_matrices()
     oop = ol.DivKGradOperator(om, L, L, ("fourier", 1.3, MODES))
     x, b, d = (hyg.Function(gm, L, L) for _ in range(3))
     x.upload(L, x0)
@@ -91,8 +95,8 @@ def test_smoother_residual_and_fp32(hyg, meshes, L):
     assert rel_inf(y32.download(L), y64.download(L)) <= 1e-5
 
 
-@pytest.mark.parametrize("name", ["cube_24el.msh", "cube_kuhn_6el.msh"])
-def test_vcycle_history_parity(hyg, meshes, name):
+@pytest.mark.parametrize("name,stored", [("cube_24el.msh", False), ("cube_kuhn_6el.msh", False), ("cube_24el.msh", True)])
+def test_vcycle_history_parity(hyg, meshes, name, stored):
     gm, om = meshes[name]
     minL, maxL, cycles = 0, 4, 5
     n = gm.ndofs(maxL)
@@ -101,6 +105,8 @@ def test_vcycle_history_parity(hyg, meshes, name):
     b0[bnd] = 0.0
     kw = dict(pre=2, post=2, smoother="jacobi")
     op = hyg.P1DivKGradOperator(gm, minL, maxL, 1.3, MODES)
+    if stored:
+        op.store_local_matrices()
     x, b = hyg.Function(gm, minL, maxL), hyg.Function(gm, minL, maxL)
     b.upload(maxL, b0)
     got = hyg.GMG(op, minL, maxL, **kw).solve(x, b, maxL, cycles)
