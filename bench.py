@@ -26,6 +26,10 @@ import time
 
 import numpy as np
 
+# FP32 V-cycles per FP64 residual in the headline mixed-precision solve (--ir-inner); the other workloads
+# take one, so their mixed solves run the same number of V-cycles as the FP64 ones
+IR_INNER = 3
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK_HBM_GBS = 8000.0
 
@@ -190,7 +194,7 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
     u64, umx = x64.download(L), xmx.download(L)
     out["solve"] = {"cycle": f"V(3,3) Chebyshev order 2, rho(D^-1 A) = {rho:.5f}",
                     "fp64": {"seconds": round(t64, 4), "cycles": len(r64) - 1, "rel_residual": float(r64[-1] / r64[0])},
-                    "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "rel_residual": float(rmx[-1] / rmx[0]),
+                    "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "inner_cycles": 1, "rel_residual": float(rmx[-1] / rmx[0]),
                               "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
                     "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
     # the same solves with the coefficient integrals kept
@@ -206,7 +210,7 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
     rmx = hyg.ir_solve(ops, g32s, xmx, b, L, inner=1, max_outer=100, rel_tol=1e-10, stream=st)
     tmx = time.perf_counter() - t0
     out["solve_stored"] = {"fp64": {"seconds": round(t64, 4), "cycles": len(r64) - 1, "rel_residual": float(r64[-1] / r64[0])},
-                           "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1,
+                           "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "inner_cycles": 1,
                                      "rel_residual": float(rmx[-1] / rmx[0])},
                            "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
     return out
@@ -265,7 +269,7 @@ def bench_p2(hyg, hip, st, steps, level, seed):
     u64, umx = x64.download(L), xmx.download(L)
     out["solve"] = {"cycle": f"V(3,3) Chebyshev order 2, rho(D^-1 A) = {rho:.5f}, coarse P2 level 0 CG",
                     "fp64": {"seconds": round(t64, 4), "cycles": len(r64) - 1, "rel_residual": float(r64[-1] / r64[0])},
-                    "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "rel_residual": float(rmx[-1] / rmx[0]),
+                    "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "inner_cycles": 1, "rel_residual": float(rmx[-1] / rmx[0]),
                               "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
                     "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
     return out
@@ -346,7 +350,7 @@ def bench_shell(hyg, hip, st, steps, level, seed, solve_cycles):
         "cycle": f"V(3,3) Chebyshev order 2, rho(D^-1 A) = {rho:.5f}, coarse P2 level 0 CG",
         "fp64": {"seconds": round(t64, 3), "cycles": len(r64) - 1, "rel_residual": float(r64[-1] / r64[0]),
                  "rate_per_cycle": float((r64[-1] / r64[0]) ** (1.0 / max(1, len(r64) - 1)))},
-        "mixed": {"seconds": round(tmx, 3), "outer_steps": len(rmx) - 1, "rel_residual": float(rmx[-1] / rmx[0]),
+        "mixed": {"seconds": round(tmx, 3), "outer_steps": len(rmx) - 1, "inner_cycles": 1, "rel_residual": float(rmx[-1] / rmx[0]),
                   "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
         "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
     return out
@@ -371,7 +375,10 @@ def main():
     ap.add_argument("--shell-level", type=int, default=5)
     ap.add_argument("--shell-cycles", type=int, default=8, help="cycle cap of the shell solves (1e-10 needs ~60)")
     ap.add_argument("--p2-level", type=int, default=7)
+    ap.add_argument("--ir-inner", type=int, default=3, help="FP32 V-cycles per FP64 residual in the mixed solves")
     args = ap.parse_args()
+    global IR_INNER
+    IR_INNER = max(1, args.ir_inner)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
@@ -465,7 +472,7 @@ def main():
         xmx = hyg.Function(mesh, 0, L)
         # warm-up solves (first-touch of all levels), then timed solves from x = 0
         g64.solve_tol(x64, b, L, 1, 1e-10, st)
-        hyg.ir_solve(op, g32, xmx, b, L, inner=1, max_outer=1, rel_tol=1e-10, stream=st)
+        hyg.ir_solve(op, g32, xmx, b, L, inner=IR_INNER, max_outer=1, rel_tol=1e-10, stream=st)
         for f in (x64, xmx):
             f.interpolate(0.0, L)
         b.upload(L, bv)
@@ -476,7 +483,7 @@ def main():
         b.upload(L, bv)
         hip.sync(st)
         t0 = time.perf_counter()
-        resmx = hyg.ir_solve(op, g32, xmx, b, L, inner=1, max_outer=60, rel_tol=1e-10, stream=st)
+        resmx = hyg.ir_solve(op, g32, xmx, b, L, inner=IR_INNER, max_outer=60, rel_tol=1e-10, stream=st)
         tmx = time.perf_counter() - t0
         u64, umx = x64.download(L), xmx.download(L)
         out["solve"] = {
@@ -485,7 +492,7 @@ def main():
                      f"coarse level 0 CG",
             "fp64": {"seconds": round(t64, 5), "cycles": len(res64) - 1,
                      "rel_residual": float(res64[-1] / res64[0])},
-            "mixed": {"seconds": round(tmx, 5), "outer_steps": len(resmx) - 1,
+            "mixed": {"seconds": round(tmx, 5), "outer_steps": len(resmx) - 1, "inner_cycles": IR_INNER,
                       "rel_residual": float(resmx[-1] / resmx[0]),
                       "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
             "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
