@@ -505,10 +505,10 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
             }
          }
    }
-   const T   vals[2] = { aE + laneShift< kDppWaveShr1 >( rE ) + laneShift< kDppWaveShl1 >( lE ),
-                         aO + laneShift< kDppWaveShr1 >( rO ) + laneShift< kDppWaveShl1 >( lO ) };
-   const int xmax    = N - 1 - y - z;
-   const int row     = S3i( w ) - S3i( w - z ) + rowStart( w - z, y ); // cell-local start of row (y, z)
+   const T   valE = aE + laneShift< kDppWaveShr1 >( rE ) + laneShift< kDppWaveShl1 >( lE );
+   const T   valO = aO + laneShift< kDppWaveShr1 >( rO ) + laneShift< kDppWaveShl1 >( lO );
+   const int xmax = N - 1 - y - z;
+   const int row  = S3i( w ) - S3i( w - z ) + rowStart( w - z, y ); // cell-local start of row (y, z)
    if ( lane == 0 || lane == 63 )
       return;
    // epilogue inputs of the lane's pair: the second right-hand side (or the old output) as one pair load
@@ -516,35 +516,33 @@ __device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_
    // inverse diagonal of a cell-interior point is the inverse of its merged stencil's centre weight
    constexpr bool kIn = MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK ||
                         MODE == MODE_APPLY_ADD;
-   T i1[2] = { T( 0 ), T( 0 ) };
+   T i1E = T( 0 ), i1O = T( 0 );
    if constexpr ( kIn )
    {
       const auto     re = makeRsrc( ( MODE == MODE_APPLY_ADD ? out0 : in1 ) + cellOff, unsigned( S3i( w ) ) * unsigned( sizeof( T ) ) );
       const unsigned vq = xe <= xmax ? unsigned( row + xe ) * unsigned( sizeof( T ) ) : kDropOffset;
-      bufLoad2( re, vq, 0, i1[0], i1[1] );
+      bufLoad2( re, vq, 0, i1E, i1O );
    }
    constexpr int rc = p2RowIndex( rs, 0, 0 );
-   const T       xv[2] = { vEr[rc], vOr[rc] };
-   T             dv[2] = { T( 1 ), T( 1 ) };
+   T             dvE = T( 1 ), dvO = T( 1 );
    if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
    {
-      dv[0] = T( 1 ) / wEr[p2CentreIndex( oE )];
-      dv[1] = T( 1 ) / wOr[p2CentreIndex( oO )];
+      dvE = T( 1 ) / wEr[p2CentreIndex( oE )];
+      dvO = T( 1 ) / wOr[p2CentreIndex( oO )];
    }
    (void) dinv;
-#pragma unroll
-   for ( int h = 0; h < 2; ++h )
-   {
-      const int px = xe + h;
+   auto finish = [&]( int px, T val, T xv, T i1, T dv ) {
       if ( px < 1 || px > xmax )
-         continue;
+         return;
       const i64 q = cellOff + row + px;
       T         r0, r1;
-      epilogue< T, MODE >( vals[h], xv[h], i1[h], i1[h], dv[h], a, r0, r1 );
+      epilogue< T, MODE >( val, xv, i1, i1, dv, a, r0, r1 );
       out0[q] = r0;
       if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
          out1[q] = r1;
-   }
+   };
+   finish( xe, valE, vEr[rc], i1E, dvE );
+   finish( xe + 1, valO, vOr[rc], i1O, dvO );
 }
 
 template < typename T, int MODE >
