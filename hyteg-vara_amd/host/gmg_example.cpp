@@ -55,7 +55,17 @@ int main( int argc, char** argv )
          r.assign( { 1.0, -1.0 }, { &f, &r }, level, Inner );
          std::printf( "%s%.17g", c ? ", " : "", std::sqrt( r.dotGlobal( r, level, Inner ) ) );
       }
-      std::printf( "]}\n" );
+      std::printf( "]" );
+      // the P2 operator on the same mesh: P2 level level - 1 holds the DoFs of P1 level `level`
+      if ( level >= 1 )
+      {
+         P2Function< double >                p2u( "p2u", storage, level - 1, level - 1 ), p2Au( "p2Au", storage, level - 1, level - 1 );
+         P2ConstantLaplaceOperator< double > A2( storage, level - 1, level - 1 );
+         p2u.setValues( level - 1, v );
+         A2.apply( p2u, p2Au, level - 1, Inner, Replace );
+         std::printf( ", \"p2_apply_norm\": %.17g", std::sqrt( p2Au.dotGlobal( p2Au, level - 1, Inner ) ) );
+      }
+      std::printf( "}\n" );
    } catch ( const std::exception& e )
    {
       std::fprintf( stderr, "error: %s\n", e.what() );

@@ -13,6 +13,8 @@
 
 #include "../../include/hyteg_amd.h"
 
+#include <array>
+#include <cstdint>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -45,7 +47,8 @@ inline void check( int rc )
 // Device selection; call once per process before anything else.
 inline void initialise( int device = 0 ) { check( hyg_init( device ) ); }
 
-// Macro mesh + boundary flags (MeshInfo::fromGmshFile + SetupPrimitiveStorage + PrimitiveStorage).
+// Macro mesh + boundary flags (MeshInfo::fromGmshFile / meshSphericalShell / meshSymmetricCuboid +
+// SetupPrimitiveStorage + PrimitiveStorage).
 class PrimitiveStorage
 {
  public:
@@ -55,8 +58,23 @@ class PrimitiveStorage
       check( hyg_mesh_create_from_msh( gmshFile.c_str(), &m ) );
       h_.reset( m, []( hyg_mesh p ) { hyg_mesh_destroy( p ); } );
    }
+   // MeshInfo::meshSphericalShell( ntan, nrad, rmin, rmax )
+   static std::shared_ptr< PrimitiveStorage > sphericalShell( int ntan, int nrad, double rmin, double rmax )
+   {
+      hyg_mesh m = nullptr;
+      check( hyg_mesh_create_spherical_shell( ntan, nrad, rmin, rmax, &m ) );
+      return std::shared_ptr< PrimitiveStorage >( new PrimitiveStorage( m ) );
+   }
+   // MeshInfo::meshSymmetricCuboid( lowerLeftFront, upperRightBack, nx, ny, nz )
+   static std::shared_ptr< PrimitiveStorage > symmetricCuboid( const double lower[3], const double upper[3], int nx, int ny, int nz )
+   {
+      hyg_mesh m = nullptr;
+      check( hyg_mesh_create_symmetric_cuboid( lower, upper, nx, ny, nz, &m ) );
+      return std::shared_ptr< PrimitiveStorage >( new PrimitiveStorage( m ) );
+   }
    hyg_mesh handle() const { return h_.get(); }
-   int64_t  numberOfDoFs( int level ) const
+   // vertex DoFs of a P1 level in canonical order (a P2 level L has the DoFs of P1 level L + 1)
+   int64_t numberOfDoFs( int level ) const
    {
       int64_t n = 0;
       check( hyg_mesh_num_dofs( handle(), level, &n ) );
@@ -64,23 +82,27 @@ class PrimitiveStorage
    }
 
  private:
+   explicit PrimitiveStorage( hyg_mesh m ) { h_.reset( m, []( hyg_mesh p ) { hyg_mesh_destroy( p ); } ); }
    std::shared_ptr< hyg_mesh_s > h_;
 };
 
-// P1Function<ValueType> (double or float): device arrays for levels minLevel..maxLevel.
-template < typename ValueType >
-class P1Function
+// Device arrays of a P1Function<ValueType> or P2Function<ValueType> (double or float) for levels
+// minLevel..maxLevel. A P2 function of level L exchanges its values in the canonical order of P1 level L + 1.
+template < typename ValueType, bool IsP2 >
+class LatticeFunction
 {
    static_assert( std::is_same< ValueType, double >::value || std::is_same< ValueType, float >::value, "double or float" );
 
  public:
    using valueType = ValueType;
-   P1Function( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
+   LatticeFunction( const std::string& name, const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
    : name_( name )
    , storage_( storage )
    {
-      hyg_func f = nullptr;
-      check( hyg_func_create( storage->handle(), minLevel, maxLevel, sizeof( ValueType ) == 8 ? HYG_F64 : HYG_F32, &f ) );
+      hyg_func  f  = nullptr;
+      const int dt = sizeof( ValueType ) == 8 ? HYG_F64 : HYG_F32;
+      check( IsP2 ? hyg_func_create_p2( storage->handle(), minLevel, maxLevel, dt, &f )
+                  : hyg_func_create( storage->handle(), minLevel, maxLevel, dt, &f ) );
       h_.reset( f, []( hyg_func p ) { hyg_func_destroy( p ); } );
    }
    hyg_func           handle() const { return h_.get(); }
@@ -90,7 +112,7 @@ class P1Function
    {
       check( hyg_interpolate_const( handle(), constant, level, flag, stream_ ) );
    }
-   void assign( const std::vector< double >& scalars, const std::vector< P1Function* >& functions, int level,
+   void assign( const std::vector< double >& scalars, const std::vector< LatticeFunction* >& functions, int level,
                 DoFType flag = All ) const
    {
       std::vector< hyg_func > f;
@@ -98,7 +120,7 @@ class P1Function
          f.push_back( p->handle() );
       check( hyg_assign( handle(), int( f.size() ), scalars.data(), f.data(), level, flag, stream_ ) );
    }
-   void add( const std::vector< double >& scalars, const std::vector< P1Function* >& functions, int level,
+   void add( const std::vector< double >& scalars, const std::vector< LatticeFunction* >& functions, int level,
              DoFType flag = All ) const
    {
       std::vector< hyg_func > f;
@@ -106,18 +128,18 @@ class P1Function
          f.push_back( p->handle() );
       check( hyg_add( handle(), int( f.size() ), scalars.data(), f.data(), level, flag, stream_ ) );
    }
-   void multElementwise( const P1Function& a, const P1Function& b, int level, DoFType flag = All ) const
+   void multElementwise( const LatticeFunction& a, const LatticeFunction& b, int level, DoFType flag = All ) const
    {
       check( hyg_mult_elementwise( handle(), a.handle(), b.handle(), level, flag, stream_ ) );
    }
-   double dotGlobal( const P1Function& other, int level, DoFType flag = All ) const
+   double dotGlobal( const LatticeFunction& other, int level, DoFType flag = All ) const
    {
       double r = 0.0;
       check( hyg_dot( handle(), other.handle(), level, flag, &r, stream_ ) );
       return r;
    }
    template < typename Other >
-   void copyFrom( const P1Function< Other >& other, int level ) const
+   void copyFrom( const LatticeFunction< Other, IsP2 >& other, int level ) const
    {
       check( hyg_convert( handle(), other.handle(), level, stream_ ) );
    }
@@ -125,7 +147,7 @@ class P1Function
    void setValues( int level, const std::vector< double >& v ) const { check( hyg_func_upload( handle(), level, v.data(), stream_ ) ); }
    std::vector< double > getValues( int level ) const
    {
-      std::vector< double > v( size_t( storage_->numberOfDoFs( level ) ) );
+      std::vector< double > v( size_t( storage_->numberOfDoFs( level + ( IsP2 ? 1 : 0 ) ) ) );
       check( hyg_func_download( handle(), level, v.data(), stream_ ) );
       return v;
    }
@@ -137,23 +159,18 @@ class P1Function
    std::shared_ptr< hyg_func_s >       h_;
    void*                               stream_ = nullptr;
 };
-
-// P1ConstantLaplaceOperator (P1ConstantOperator<P1FenicsForm<p1_diffusion, p1_tet_diffusion>>), one value type.
 template < typename ValueType >
-class P1ConstantLaplaceOperator
+using P1Function = LatticeFunction< ValueType, false >;
+template < typename ValueType >
+using P2Function = LatticeFunction< ValueType, true >;
+
+// The Operator<Src, Dst> interface shared by every operator handle: apply, smooth_jac, inverse diagonal.
+template < typename Function >
+class MatrixFreeOperator
 {
  public:
-   using srcType = P1Function< ValueType >;
-   using dstType = P1Function< ValueType >;
-   P1ConstantLaplaceOperator( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
-   : storage_( storage )
-   , minLevel_( minLevel )
-   , maxLevel_( maxLevel )
-   {
-      hyg_op op = nullptr;
-      check( hyg_op_create_p1_const( storage->handle(), minLevel, maxLevel, &op ) );
-      h_.reset( op, []( hyg_op p ) { hyg_op_destroy( p ); } );
-   }
+   using srcType = Function;
+   using dstType = Function;
    hyg_op handle() const { return h_.get(); }
    int    minLevel() const { return minLevel_; }
    int    maxLevel() const { return maxLevel_; }
@@ -173,11 +190,103 @@ class P1ConstantLaplaceOperator
    }
    void setStream( void* s ) { stream_ = s; }
 
+ protected:
+   MatrixFreeOperator( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel, hyg_op op )
+   : storage_( storage )
+   , minLevel_( minLevel )
+   , maxLevel_( maxLevel )
+   {
+      h_.reset( op, []( hyg_op p ) { hyg_op_destroy( p ); } );
+   }
+
+
  private:
    std::shared_ptr< PrimitiveStorage > storage_;
    std::shared_ptr< hyg_op_s >         h_;
    int                                 minLevel_, maxLevel_;
    void*                               stream_ = nullptr;
+};
+
+// P1ConstantLaplaceOperator (P1ConstantOperator<P1FenicsForm<p1_diffusion, p1_tet_diffusion>>), one value type.
+template < typename ValueType >
+class P1ConstantLaplaceOperator : public MatrixFreeOperator< P1Function< ValueType > >
+{
+ public:
+   P1ConstantLaplaceOperator( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
+   : MatrixFreeOperator< P1Function< ValueType > >( storage, minLevel, maxLevel, make( storage, minLevel, maxLevel ) )
+   {}
+
+ private:
+   static hyg_op make( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
+   {
+      hyg_op op = nullptr;
+      check( hyg_op_create_p1_const( storage->handle(), minLevel, maxLevel, &op ) );
+      return op;
+   }
+};
+
+// P1ElementwiseOperator<p1_div_k_grad_affine_q3> with k = k0 exp( sum_m a_m cos(2 pi kappa_m . x + phi_m) ),
+// modes: M rows (kappa_x, kappa_y, kappa_z, phi, a).
+template < typename ValueType >
+class P1ElementwiseDivKGradOperator : public MatrixFreeOperator< P1Function< ValueType > >
+{
+ public:
+   P1ElementwiseDivKGradOperator( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel, double k0,
+                                  const std::vector< std::array< double, 5 > >& modes )
+   : MatrixFreeOperator< P1Function< ValueType > >( storage, minLevel, maxLevel, make( storage, minLevel, maxLevel, k0, modes ) )
+   {}
+   void computeAndStoreLocalElementMatrices() const { check( hyg_op_store_local_matrices( this->handle() ) ); }
+
+ private:
+   static hyg_op make( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel, double k0,
+                       const std::vector< std::array< double, 5 > >& modes )
+   {
+      hyg_op op = nullptr;
+      check( hyg_op_create_p1_divkgrad( storage->handle(), minLevel, maxLevel, k0, modes.empty() ? nullptr : modes[0].data(),
+                                        int( modes.size() ), &op ) );
+      return op;
+   }
+};
+
+// P2ConstantLaplaceOperator, one value type.
+template < typename ValueType >
+class P2ConstantLaplaceOperator : public MatrixFreeOperator< P2Function< ValueType > >
+{
+ public:
+   P2ConstantLaplaceOperator( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
+   : MatrixFreeOperator< P2Function< ValueType > >( storage, minLevel, maxLevel, make( storage, minLevel, maxLevel ) )
+   {}
+
+ private:
+   static hyg_op make( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
+   {
+      hyg_op op = nullptr;
+      check( hyg_op_create_p2_const( storage->handle(), minLevel, maxLevel, &op ) );
+      return op;
+   }
+};
+
+// P2ElementwiseOperator<p2_div_k_grad_affine_q4>; kind 0: k = k0 exp(alpha (1 - |x|)) (1 + beta xi_e) with xi_e
+// seeded per micro-element, kind 1: k = 1 + x/2 + y^2/4.
+template < typename ValueType >
+class P2ElementwiseDivKGradOperator : public MatrixFreeOperator< P2Function< ValueType > >
+{
+ public:
+   P2ElementwiseDivKGradOperator( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel, int kind, double k0,
+                                  double alpha, double beta, uint64_t seed )
+   : MatrixFreeOperator< P2Function< ValueType > >( storage, minLevel, maxLevel,
+                                                    make( storage, minLevel, maxLevel, kind, k0, alpha, beta, seed ) )
+   {}
+   void computeAndStoreLocalElementMatrices() const { check( hyg_op_store_local_matrices( this->handle() ) ); }
+
+ private:
+   static hyg_op make( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel, int kind, double k0, double alpha,
+                       double beta, uint64_t seed )
+   {
+      hyg_op op = nullptr;
+      check( hyg_op_create_p2_divkgrad( storage->handle(), minLevel, maxLevel, kind, k0, alpha, beta, seed, &op ) );
+      return op;
+   }
 };
 
 class P1toP1LinearRestriction
@@ -204,6 +313,11 @@ class P1toP1LinearProlongation
       check( hyg_prolongate( function.handle(), sourceLevel, flag, HYG_ADD, nullptr ) );
    }
 };
+
+// P2toP2QuadraticRestriction / P2toP2QuadraticProlongation: the same calls on a P2 function (the library
+// applies the quadratic transfers to P2 handles)
+using P2toP2QuadraticRestriction  = P1toP1LinearRestriction;
+using P2toP2QuadraticProlongation = P1toP1LinearProlongation;
 
 // Smoother / coarse-solver descriptions consumed by GeometricMultigridSolver (the library runs them).
 struct WeightedJacobiSmoother
@@ -286,11 +400,11 @@ class GeometricMultigridSolver
    std::shared_ptr< hyg_gmg_s > h_;
 };
 
-// Mixed-precision defect correction: FP64 residual and update, `inner` FP32 V-cycles per step.
-inline std::vector< double > mixedPrecisionSolve( const P1ConstantLaplaceOperator< double >&       A64,
-                                                  const GeometricMultigridSolver< P1ConstantLaplaceOperator< float > >& gmg32,
-                                                  const P1Function< double >& x, const P1Function< double >& b, int level,
-                                                  int inner, int maxOuter, double relTol )
+// Mixed-precision defect correction: FP64 residual and update, `inner` FP32 V-cycles per step (the FP32
+// multigrid of the same operator handle, built on an FP32 function type).
+template < typename Op64, typename Op32 >
+std::vector< double > mixedPrecisionSolve( const Op64& A64, const GeometricMultigridSolver< Op32 >& gmg32, const typename Op64::srcType& x,
+                                           const typename Op64::srcType& b, int level, int inner, int maxOuter, double relTol )
 {
    std::vector< double > res( size_t( maxOuter ) + 1 );
    int                   n = 0;

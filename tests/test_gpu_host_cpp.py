@@ -1,4 +1,5 @@
-"""The C++ host classes (hyteg-vara_amd/host/hyteg_amd.hpp) driven by a plain C++ program, against the oracle."""
+"""The C++ host classes (hyteg-vara_amd/host/hyteg_amd.hpp) driven by a plain C++ program, against the oracle: P1
+apply and V-cycle residual history, and the P2 apply."""
 import json
 import os
 import subprocess
@@ -42,3 +43,10 @@ def test_cpp_host_program_matches_oracle(tmp_path):
     b.set(L, b0)
     res = ol.gmg(op2, x, b, 0, L, cycles, pre=2, post=2, smoother="jacobi")
     assert np.all(np.abs(np.array(got["residuals"]) - res) <= 1e-10 * res), (got["residuals"], res)
+    # P2ConstantLaplaceOperator through the C++ classes: P2 level L - 1 on the same values
+    op3 = ol.P2Operator(om, L - 1, L - 1)
+    s3, d3 = ol.Function(om, L, L), ol.Function(om, L, L)
+    s3.set(L, v)
+    op3.apply(s3, d3, L, ol.INNER, False)
+    ref3 = np.sqrt(np.sum(d3.get(L)[~bnd] ** 2))
+    assert abs(got["p2_apply_norm"] - ref3) <= 1e-12 * ref3
