@@ -222,6 +222,37 @@ def test_apply_parity_large(hyg, meshes, L):
     assert np.abs(r).max() <= 1e-13 * np.abs(lin).max() * (1 << L)
 
 
+def test_apply_at_level_9(hyg, meshes):
+    """The largest level the library takes (543M DoFs on the 24-tet cube, 4.3 GB per FP64 function), checked
+    through size-independent properties of the assembled Laplacian: A 1 = 0 on inner DoFs, zero column sums
+    (sum_i (A x)_i = 0 over all DoFs for the operator without boundary conditions), and symmetry
+    (A x, z) = (x, A z); the Jacobi sweep with b = A x reproduces x on inner DoFs."""
+    gm, _ = meshes["cube_24el.msh"]
+    L = 9
+    n = gm.ndofs(L)
+    op = hyg.P1ConstantOperator(gm, L, L)
+    x, y, z, w = (hyg.Function(gm, L, L) for _ in range(4))
+    x.interpolate(1.0, L)
+    op.apply(x, y, L, hyg.INNER)
+    assert y.dot(y, L, hyg.INNER) <= 1e-24 * n
+    rng = np.random.default_rng(909)
+    x.upload(L, rng.uniform(-1.0, 1.0, n))
+    z.upload(L, rng.uniform(-1.0, 1.0, n))
+    op.apply(x, y, L, hyg.ALL)
+    op.apply(z, w, L, hyg.ALL)
+    ny = np.sqrt(y.dot(y, L))
+    ones = hyg.Function(gm, L, L)
+    ones.interpolate(1.0, L)
+    assert abs(y.dot(ones, L)) <= 1e-12 * ny * np.sqrt(n)
+    assert abs(y.dot(z, L) - x.dot(w, L)) <= 1e-12 * ny * np.sqrt(n)
+    # one Jacobi sweep with b = A x started from x leaves x unchanged on inner DoFs (relative 1e-12)
+    op.apply(x, y, L, hyg.INNER)
+    w.assign([1.0], [x], L)
+    op.smooth_jac(w, y, x, 0.8, L, hyg.INNER)
+    w.add([-1.0], [x], L)
+    assert np.sqrt(w.dot(w, L, hyg.INNER)) <= 1e-12 * np.sqrt(x.dot(x, L, hyg.INNER))
+
+
 @pytest.mark.parametrize("cfg", range(7))
 @pytest.mark.parametrize("L", [3, 5, 6])
 def test_every_cell_kernel_configuration(hyg, meshes, cfg, L):
