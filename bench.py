@@ -280,7 +280,8 @@ def bench_shell(hyg, hip, st, steps, level, seed, solve_cycles):
     -div(eta grad u_i), i = 1..3, with eta(x) = exp(2.99 (1 - |x|)) (1 + 0.1 xi_e), xi_e uniform in [-1, 1) per
     micro-element (P2ElementwiseOperator with p2_div_k_grad_affine_q4; the same scalar operator on each
     component). Apply throughput of the three components in FP64 and FP32, with the element rows assembled per
-    apply and with stored per-point stencils; one component solved to 1e-10 with V(3,3) Chebyshev cycles in FP64
+    apply and with stored per-point stencils (hyg_apply_vector: one pass over the stored weights for the three
+    components' cell interiors); one component solved to 1e-10 with V(3,3) Chebyshev cycles in FP64
     and as FP32 cycles inside FP64 defect correction (stored stencils)."""
     mesh = hyg.Mesh("shell", 5, 2, 0.55, 1.0)
     L = level
@@ -319,8 +320,7 @@ def bench_shell(hyg, hip, st, steps, level, seed, solve_cycles):
             x.upload(L, rng.uniform(-1, 1, n))
 
         def ap():
-            for x, y in zip(xs, ys):
-                hyg.check(lib.hyg_apply(op.h, x.h, y.h, L, hyg.INNER, hyg.REPLACE, st))
+            hyg.apply_vector(op, xs, ys, L, hyg.INNER, hyg.REPLACE, st)
 
         ap()
         ms = timed(hip, st, ap, steps) / steps

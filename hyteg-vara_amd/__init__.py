@@ -58,6 +58,7 @@ def _sig(lib):
         "hyg_op_create_p2_const": (i, [vp, i, i, pvp]),
         "hyg_op_create_p2_divkgrad": (i, [vp, i, i, i, d, d, d, C.c_uint64, pvp]),
         "hyg_op_store_local_matrices": (i, [vp]),
+        "hyg_apply_vector": (i, [vp, i, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), i, i, i, vp]),
         "hyg_op_destroy": (i, [vp]),
         "hyg_op_cell_stencil": (i, [vp, i, i, P(np.float64)]),
         "hyg_apply": (i, [vp, vp, vp, i, i, i, vp]),
@@ -312,6 +313,14 @@ class P2DivKGradOperator(P1ConstantOperator):
     def store_local_matrices(self):
         """Assemble and keep the per-point stencils (computeAndStoreLocalElementMatrices)."""
         check(load().hyg_op_store_local_matrices(self.h))
+
+
+def apply_vector(op, srcs, dsts, L, flag=INNER, update=REPLACE, stream=None):
+    """The scalar operator applied to each component (VectorLaplaceOperator::apply), up to three at once."""
+    n = len(srcs)
+    sa = (C.c_void_p * n)(*[f.h.value for f in srcs])
+    da = (C.c_void_p * n)(*[f.h.value for f in dsts])
+    check(load().hyg_apply_vector(op.h, n, sa, da, L, flag, update, stream))
 
 
 def cheb_coefficients(order, lower, upper):

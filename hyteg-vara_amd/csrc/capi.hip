@@ -1150,6 +1150,43 @@ int hyg_apply( hyg_op op, hyg_func src, hyg_func dst, int level, int flag, int u
    return stencilImpl( op, level, src->dtype, c, S( stream ) );
 }
 
+int hyg_apply_vector( hyg_op op, int n, const hyg_func* srcs, const hyg_func* dsts, int level, int flag, int update, void* stream )
+{
+   if ( n < 1 || n > 3 || !srcs || !dsts )
+      return fail( HYG_ERR_ARG, "hyg_apply_vector takes 1 to 3 components" );
+   const int L = lat( op, level );
+   HYG_TRY( checkOpLevel( op, L ) );
+   if ( update != HYG_REPLACE && update != HYG_ADD )
+      return fail( HYG_ERR_ARG, "update must be HYG_REPLACE or HYG_ADD" );
+   StencilCall cs[3];
+   for ( int k = 0; k < n; ++k )
+   {
+      HYG_TRY( checkPair( srcs[k], dsts[k], L ) );
+      HYG_TRY( checkSpace( op, srcs[k] ) );
+      if ( srcs[k]->dtype != srcs[0]->dtype )
+         return fail( HYG_ERR_ARG, "components of different value types" );
+      if ( srcs[k]->mesh != op->mesh )
+         return fail( HYG_ERR_ARG, "function and operator live on different meshes" );
+      for ( int j = 0; j < n; ++j )
+         if ( srcs[k] == dsts[j] || ( j != k && dsts[k] == dsts[j] ) )
+            return fail( HYG_ERR_ARG, "apply requires distinct sources and destinations" );
+      cs[k].mode = update == HYG_ADD ? MODE_APPLY_ADD : MODE_APPLY;
+      cs[k].flag = flag;
+      cs[k].x    = srcs[k]->data[L];
+      cs[k].out0 = dsts[k]->data[L];
+   }
+   MeshCtx& mc = op->mesh->ctx;
+   if ( op->kind == 2 || op->kind == 3 )
+   {
+      launchP2Vector( mc, mc.geom( L ), *op, L, srcs[0]->dtype, cs, n, S( stream ) );
+      return launched( "vector apply" );
+   }
+   for ( int k = 0; k < n; ++k )
+      if ( int rc = stencilImpl( op, L, srcs[0]->dtype, cs[k], S( stream ) ) )
+         return rc;
+   return HYG_OK;
+}
+
 int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double relax, int level, int flag, void* stream )
 {
    level = lat( op, level );

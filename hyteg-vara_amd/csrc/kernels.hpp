@@ -43,6 +43,7 @@ void launchP2( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dt
 void launchP2InverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );
 void launchP2Coefficients( const MeshCtx& m, const hyg_op_s& op, int L, hipStream_t s );
 void launchP2StoreStencils( const MeshCtx& m, hyg_op_s& op, int L, hipStream_t s );
+void launchP2Vector( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall* cs, int n, hipStream_t s );
 void launchP2Prolongate( const MeshCtx& m, const LevelGeom& gf, const P2Tables* tb, int Lf, int dtype, void* fine,
                          const void* coarse, int flag, bool add, hipStream_t st );
 void launchP2Restrict( const MeshCtx& m, const LevelGeom& gc, const P2Tables* tb, int Lc, int dtype, void* coarse, const void* fine,

@@ -702,9 +702,18 @@ __global__ __launch_bounds__( kBlock ) void kP2RestrictInt( int Lc, i64 strideC,
 // Stored stencils, cell-interior points one class per wave (the class sub-lattice, p2InteriorExtent): each
 // thread gathers its point's neighbours and reads its own weights ([e][sub-lattice point], consecutive threads
 // read consecutive weights), then the mode's epilogue; the inverse diagonal is the inverse centre weight.
-template < typename T, int MODE, int CLS >
-__device__ __forceinline__ void p2StoredIntPt( int c, int s, int N, i64 stride, const T* __restrict__ sw, const T* __restrict__ x,
-                                               const T* __restrict__ in1, T* __restrict__ out0, T* __restrict__ out1, T a )
+// up to three functions handled by one stored-stencil pass (the components of a vector function)
+template < typename T >
+struct P2Comps
+{
+   const T* x[3];
+   const T* in1[3];
+   T*       out0[3];
+   T*       out1[3];
+};
+
+template < typename T, int MODE, int CLS, int NC >
+__device__ __forceinline__ void p2StoredIntPt( int c, int s, int N, i64 stride, const T* __restrict__ sw, const P2Comps< T >& cp, T a )
 {
    constexpr int              kx = CLS & 1, ky = ( CLS >> 1 ) & 1, kz = ( CLS >> 2 ) & 1;
    constexpr P2StencilOffsets off = p2StencilOffsets( CLS );
@@ -716,7 +725,6 @@ __device__ __forceinline__ void p2StoredIntPt( int c, int s, int N, i64 stride, 
    const int px = 2 * ( qx + 1 - kx ) + kx, py = 2 * ( qy + 1 - ky ) + ky, pz = 2 * ( qz + 1 - kz ) + kz;
    const int w = N + 1, SW = p2SubWidth( N, CLS ), nS = S3i( SW );
    const T*  ws = sw + i64( c ) * p2StoredCellSize( N ) + p2StoredIntBase( N, CLS ) + cidx( SW, px >> 1, py >> 1, pz >> 1 );
-   const T*  xc = x + i64( c ) * stride;
    int       rb[5][5];
 #pragma unroll
    for ( int dz = -2; dz <= 2; ++dz )
@@ -727,38 +735,51 @@ __device__ __forceinline__ void p2StoredIntPt( int c, int s, int N, i64 stride, 
       for ( int dy = -2; dy <= 2; ++dy )
          rb[dz + 2][dy + 2] = pb + rowStart( W, py + dy );
    }
-   T val = T( 0 );
+   // each weight is read once and applied to every component
+   T val[NC];
+#pragma unroll
+   for ( int k = 0; k < NC; ++k )
+      val[k] = T( 0 );
 #pragma unroll
    for ( int e = 0; e < off.n; ++e )
-      val += ws[i64( e ) * nS] * xc[rb[off.d[e][2] + 2][off.d[e][1] + 2] + px + off.d[e][0]];
-   const i64 q = i64( c ) * stride + rb[2][2] + px;
-   T         i1 = T( 0 ), r0, r1, xv = T( 0 ), dv = T( 1 );
-   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-      i1 = in1[q];
-   if constexpr ( MODE == MODE_APPLY_ADD )
-      i1 = out0[q];
-   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
-      xv = x[q];
+   {
+      const T   wt = ws[i64( e ) * nS];
+      const i64 o  = i64( c ) * stride + rb[off.d[e][2] + 2][off.d[e][1] + 2] + px + off.d[e][0];
+#pragma unroll
+      for ( int k = 0; k < NC; ++k )
+         val[k] += wt * cp.x[k][o];
+   }
+   const i64 q  = i64( c ) * stride + rb[2][2] + px;
+   T         dv = T( 1 );
    if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
       dv = T( 1 ) / ws[i64( p2CentreIndex( off ) ) * nS];
-   epilogue< T, MODE >( val, xv, i1, i1, dv, a, r0, r1 );
-   out0[q] = r0;
-   if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-      out1[q] = r1;
+#pragma unroll
+   for ( int k = 0; k < NC; ++k )
+   {
+      T i1 = T( 0 ), r0, r1, xv = T( 0 );
+      if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+         i1 = cp.in1[k][q];
+      if constexpr ( MODE == MODE_APPLY_ADD )
+         i1 = cp.out0[k][q];
+      if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
+         xv = cp.x[k][q];
+      epilogue< T, MODE >( val[k], xv, i1, i1, dv, a, r0, r1 );
+      cp.out0[k][q] = r0;
+      if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+         cp.out1[k][q] = r1;
+   }
 }
-template < typename T, int MODE, int CLS >
-__device__ __forceinline__ void p2StoredIntDispatch( int cls, int c, int s, int N, i64 stride, const T* sw, const T* x, const T* in1,
-                                                     T* out0, T* out1, T a )
+template < typename T, int MODE, int CLS, int NC >
+__device__ __forceinline__ void p2StoredIntDispatch( int cls, int c, int s, int N, i64 stride, const T* sw, const P2Comps< T >& cp, T a )
 {
    if ( cls == CLS )
-      p2StoredIntPt< T, MODE, CLS >( c, s, N, stride, sw, x, in1, out0, out1, a );
+      p2StoredIntPt< T, MODE, CLS, NC >( c, s, N, stride, sw, cp, a );
    else if constexpr ( CLS < 7 )
-      p2StoredIntDispatch< T, MODE, CLS + 1 >( cls, c, s, N, stride, sw, x, in1, out0, out1, a );
+      p2StoredIntDispatch< T, MODE, CLS + 1, NC >( cls, c, s, N, stride, sw, cp, a );
 }
-template < typename T, int MODE >
-__global__ __launch_bounds__( kBlock ) void kP2StoredInt( int L, i64 stride, P2ClassTiles ct, const T* __restrict__ sw,
-                                                          const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
-                                                          T* __restrict__ out1, T a )
+template < typename T, int MODE, int NC >
+__global__ __launch_bounds__( kBlock ) void kP2StoredInt( int L, i64 stride, P2ClassTiles ct, const T* __restrict__ sw, P2Comps< T > cp,
+                                                          T a )
 {
    const int per = ct.start[8];
    const int c   = int( blockIdx.x ) / per;
@@ -766,8 +787,18 @@ __global__ __launch_bounds__( kBlock ) void kP2StoredInt( int L, i64 stride, P2C
    int       cls = 0;
    while ( cls < 7 && t >= ct.start[cls + 1] )
       ++cls;
-   p2StoredIntDispatch< T, MODE, 0 >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << L, stride, sw, x, in1, out0, out1,
-                                      a );
+   p2StoredIntDispatch< T, MODE, 0, NC >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << L, stride, sw, cp, a );
+}
+inline P2ClassTiles p2InteriorTiles( int N )
+{
+   P2ClassTiles ct;
+   ct.start[0] = 0;
+   for ( int k = 0; k < 8; ++k )
+   {
+      const int M     = p2InteriorExtent( N, k );
+      ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
+   }
+   return ct;
 }
 
 // ------------------------------------------------------------------ stored stencils (variable coefficient)
@@ -1034,7 +1065,8 @@ __global__ __launch_bounds__( kBlock ) void kP2RestrictLower( const LowerChunk* 
 
 // ------------------------------------------------------------------ launch
 template < typename T, int MODE, bool DIAG, bool KQ >
-static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s )
+static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s,
+                  bool skipInterior = false )
 {
    constexpr bool         dbl  = std::is_same< T, double >::value;
    const hyg_op_s::Level& ol   = op.levels[L];
@@ -1071,17 +1103,14 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
    else if ( KQ && !DIAG && ol.p2s )
    {
       // variable coefficient with stored stencils: the same row and face kernels with per-point weights
-      const T*     sw = dbl ? (const T*) ol.p2s : (const T*) ol.p2sF;
-      P2ClassTiles ct;
-      ct.start[0] = 0;
-      for ( int k = 0; k < 8; ++k )
+      const T*           sw = dbl ? (const T*) ol.p2s : (const T*) ol.p2sF;
+      const P2ClassTiles ct = p2InteriorTiles( 1 << L );
+      if ( ct.start[8] > 0 && ( c.flag & 1 ) && !skipInterior )
       {
-         const int M     = p2InteriorExtent( 1 << L, k );
-         ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
+         const P2Comps< T > cp{ { (const T*) c.x }, { (const T*) c.in1 }, { (T*) c.out0 }, { (T*) c.out1 } };
+         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 1 > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp,
+                             T( c.a ) );
       }
-      if ( ct.start[8] > 0 && ( c.flag & 1 ) )
-         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw,
-                             (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
       P2FaceTiles ft;
       ft.start[0] = 0;
       for ( int k = 0; k < 4; ++k )
@@ -1151,6 +1180,52 @@ void launchP2( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dt
       p2Mode< double >( m, g, op, L, c, s );
    else
       p2Mode< float >( m, g, op, L, c, s );
+}
+
+// Apply (replace or add) to up to three functions at once. With stored stencils the cell interiors of all
+// components go through one pass that reads every weight once; the face, edge and vertex points and the
+// combine run per component. Other operators apply component by component.
+template < typename T, int MODE >
+static void p2VectorT( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall* cs, int n, hipStream_t s )
+{
+   constexpr bool         dbl = std::is_same< T, double >::value;
+   const hyg_op_s::Level& ol  = op.levels[L];
+   const P2ClassTiles     ct  = p2InteriorTiles( 1 << L );
+   if ( ct.start[8] > 0 && ( cs[0].flag & 1 ) )
+   {
+      P2Comps< T > cp{};
+      for ( int k = 0; k < n; ++k )
+      {
+         cp.x[k]    = (const T*) cs[k].x;
+         cp.out0[k] = (T*) cs[k].out0;
+      }
+      const T*  sw = dbl ? (const T*) ol.p2s : (const T*) ol.p2sF;
+      const int nC = m.mesh->nC();
+      if ( n == 3 )
+         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 3 > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp, T( 0 ) );
+      else if ( n == 2 )
+         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 2 > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp, T( 0 ) );
+      else
+         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 1 > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp, T( 0 ) );
+   }
+   for ( int k = 0; k < n; ++k )
+      p2TK< T, MODE, false, true >( m, g, op, L, cs[k], s, true );
+}
+
+void launchP2Vector( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall* cs, int n, hipStream_t s )
+{
+   const bool fused = op.kind == 3 && op.levels[L].p2s && n >= 1 && n <= 3 &&
+                      ( cs[0].mode == MODE_APPLY || cs[0].mode == MODE_APPLY_ADD );
+   if ( !fused )
+   {
+      for ( int k = 0; k < n; ++k )
+         launchP2( m, g, op, L, dtype, cs[k], s );
+      return;
+   }
+   if ( dtype == 8 )
+      cs[0].mode == MODE_APPLY ? p2VectorT< double, MODE_APPLY >( m, g, op, L, cs, n, s ) : p2VectorT< double, MODE_APPLY_ADD >( m, g, op, L, cs, n, s );
+   else
+      cs[0].mode == MODE_APPLY ? p2VectorT< float, MODE_APPLY >( m, g, op, L, cs, n, s ) : p2VectorT< float, MODE_APPLY_ADD >( m, g, op, L, cs, n, s );
 }
 
 // coefficient values of every micro-cell at its 11 quadrature points (variable-coefficient P2 operator)

@@ -158,6 +158,11 @@ int hyg_op_cell_stencil( hyg_op op, int level, int cell, double* out15 );
 /* Operator::apply(src, dst, level, flag, updateType) (operators/Operator.hpp:73-80; P1Operator.hpp:196-324).
  * src and dst must differ and have the same value type. */
 int hyg_apply( hyg_op op, hyg_func src, hyg_func dst, int level, int flag, int update, void* stream );
+/* VectorLaplaceOperator::apply (operators/VectorLaplaceOperator.hpp:36-66): the scalar operator applied to
+ * each of n <= 3 component functions (same value type, distinct sources and destinations). For the
+ * variable-coefficient P2 operator with stored stencils the cell interiors of all components are done in one
+ * pass that reads every stored weight once; otherwise the components are applied one after the other. */
+int hyg_apply_vector( hyg_op op, int n, const hyg_func* srcs, const hyg_func* dsts, int level, int flag, int update, void* stream );
 /* WeightedJacobiSmoothable::smooth_jac(dst, rhs, src, relax, level, flag) (solvers/Smoothables.hpp:32-43;
  * P1Operator.hpp:433-451): dst = src + relax * D^-1 (rhs - A src), fused into one pass. */
 int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double relax, int level, int flag,
