@@ -111,6 +111,11 @@ struct ZSeg
    int z1;
 };
 constexpr int kZSegChunk = 24; // planes per z-marching segment (short segments balance the tail of the launch)
+// P2 constant operator, z-marching row pairs: a wavefront computes rows y0 (even) and y0+1 of a segment of
+// kP2ZSeg x values (x0 = kP2ZSeg k - 2 + 2 lane, lanes 0 and 63 halo) over planes [z0, z1), z0 even; the
+// table stores (y0, k, z0, z1) in a ZSeg.
+constexpr int kP2ZSeg   = 124;
+constexpr int kP2ZChunk = 32; // planes per segment where N >= 256 (N = 128: 16, smaller lattices: 8)
 // cell-interior kernel configurations: rows per wavefront, points per lane, planes in flight
 constexpr int kCellRows[]   = { 1, 2, 4, 1, 2, 4, 8 };
 constexpr int kCellPoints[] = { 1, 1, 1, 1, 1, 1, 1 };
@@ -498,6 +503,9 @@ struct LevelGeom
    std::vector< int4 >                      flatItemsHost; // z = 0 / y = 0 face rows: (cell, slot, line, x0)
    int4*                                    flatItems  = nullptr;
    int                                      nFlatItems = 0;
+   std::vector< ZSeg >                      p2ZSegsHost; // P2 constant operator row-pair segments (this lattice level)
+   ZSeg*                                    p2ZSegs  = nullptr;
+   int                                      nP2ZSegs = 0;
    std::vector< LowerChunk >                chunksHost; // macro-vertex, then -edge, then -face work
    LowerChunk*                              chunks  = nullptr;
    int                                      nChunks = 0;

@@ -21,6 +21,8 @@
 // setup (kP2KQ). A point assembles the rows it needs on the fly.
 #include "face_combine.hpp"
 
+#include <utility>
+
 namespace hyg {
 
 // ------------------------------------------------------------------ variable coefficient at the quadrature points
@@ -93,6 +95,7 @@ __host__ __device__ constexpr int triCount( int n )
 }
 // points on the cell edges and vertices of a cell lattice with N intervals per edge: the four vertices, then
 // N - 1 points of each edge e0 = (v0, v1) ... e5 = (v2, v3)
+constexpr int kP2EvLanes = 4; // lanes per point of the edge/vertex kernel
 __host__ __device__ constexpr int p2EdgeVertexCount( int N )
 {
    return 4 + 6 * ( N - 1 );
@@ -123,7 +126,7 @@ __device__ __forceinline__ bool edgeVertexXYZ( int N, int s, int& x, int& y, int
 }
 
 // SKIPIN: the cell-interior and cell-face points are done by the stencil kernels; the grid then covers only the
-// points on cell edges and vertices
+// points on cell edges and vertices, kP2EvLanes lanes per point (each sums every kP2EvLanes-th incident micro-cell)
 template < typename T, int MODE, bool DIAG, bool KQ, bool SKIPIN = false >
 __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i64 stride, const T* __restrict__ el,
                                                       const T* __restrict__ kq,
@@ -137,10 +140,12 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
    const int c    = int( blockIdx.x ) / tilesPerCell;
    const int tile = int( blockIdx.x ) - c * tilesPerCell;
    const int N = 1 << L, w = N + 1, NP = N + 1, Nc = N >> 1;
-   const int s = tile * kBlock + int( threadIdx.x );
+   const int s   = SKIPIN ? ( tile * kBlock + int( threadIdx.x ) ) / kP2EvLanes : tile * kBlock + int( threadIdx.x );
+   const int sub = SKIPIN ? int( threadIdx.x ) % kP2EvLanes : 0;
    int       px, py, pz;
    if constexpr ( SKIPIN )
    {
+      // the lanes of one point leave together
       if ( s >= p2EdgeVertexCount( N ) || !edgeVertexXYZ( N, s, px, py, pz ) )
          return;
    }
@@ -159,7 +164,7 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
    const T*  xc  = x + i64( c ) * stride;
    T         val = T( 0 );
    const int ne  = tb->nEnt[cls];
-   for ( int k = 0; k < ne; ++k )
+   for ( int k = sub; k < ne; k += ( SKIPIN ? kP2EvLanes : 1 ) )
    {
       const int tl = tb->ent[cls][k];
       const int t = tl >> 4, l = tl & 15;
@@ -206,6 +211,14 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
                val += row[j] * xc[cidx( w, 2 * ax + tb->nd[t][j][0], 2 * ay + tb->nd[t][j][1], 2 * az + tb->nd[t][j][2] )];
          }
       }
+   }
+   if constexpr ( SKIPIN )
+   {
+#pragma unroll
+      for ( int m = 1; m < kP2EvLanes; m <<= 1 )
+         val += __shfl_xor( val, m, kP2EvLanes );
+      if ( sub != 0 )
+         return;
    }
    const i64 q = i64( c ) * stride + cidx( w, px, py, pz );
    if ( slot == 0 )
@@ -386,48 +399,6 @@ __global__ __launch_bounds__( kBlock ) void kP2Face( int L, i64 stride, P2FaceTi
 }
 
 // ------------------------------------------------------------------ constant operator: cell-interior points
-// One wave computes the interior points of one lattice row (y, z) in a segment of 124 x values. Lane i holds the
-// pair x0 + 2i (even x) and x0 + 2i + 1 (odd x), x0 = 124 k - 2; the even points share one parity class cE and the
-// odd points the class cE | 1, so both merged stencils have offsets known at compile time and weights that are
-// the same for every lane of the wave (scalar loads). Each stencil row (dy, dz) is read once, as one pair load per
-// lane; a value at x + dx reaches the lane that needs it through partial sums: the terms a lane's values add to
-// its left and right neighbours are summed in place and shifted once per wave (lanes 0 and 63 are halo only).
-constexpr int kP2RowSeg = 124;
-
-struct P2RowSet
-{
-   int n;
-   int d[25][2]; // (dy, dz)
-};
-constexpr P2RowSet p2RowSet( int ky, int kz )
-{
-   P2RowSet               r{};
-   const P2StencilOffsets a = p2StencilOffsets( ( ky << 1 ) | ( kz << 2 ) ), b = p2StencilOffsets( ( ky << 1 ) | ( kz << 2 ) | 1 );
-   for ( int dz = -2; dz <= 2; ++dz )
-      for ( int dy = -2; dy <= 2; ++dy )
-      {
-         bool used = false;
-         for ( int e = 0; e < a.n; ++e )
-            used = used || ( a.d[e][1] == dy && a.d[e][2] == dz );
-         for ( int e = 0; e < b.n; ++e )
-            used = used || ( b.d[e][1] == dy && b.d[e][2] == dz );
-         if ( used )
-         {
-            r.d[r.n][0] = dy;
-            r.d[r.n][1] = dz;
-            ++r.n;
-         }
-      }
-   return r;
-}
-
-constexpr int p2RowIndex( const P2RowSet& rs, int dy, int dz )
-{
-   for ( int r = 0; r < rs.n; ++r )
-      if ( rs.d[r][0] == dy && rs.d[r][1] == dz )
-         return r;
-   return -1;
-}
 constexpr int p2CentreIndex( const P2StencilOffsets& o )
 {
    for ( int e = 0; e < o.n; ++e )
@@ -436,151 +407,209 @@ constexpr int p2CentreIndex( const P2StencilOffsets& o )
    return -1;
 }
 
-// first x of segment k, and the number of rows (y, z) whose interior reaches it
-__host__ __device__ constexpr int p2SegStart( int k )
+// A cell-interior point's merged stencil depends on its parity class (x, y, z) mod 2. One wave computes the rows
+// y0 (even) and y0+1 of one x segment: lane i holds the pair x0 + 2i (even x) and x0 + 2i + 1 (odd x),
+// x0 = 124 k - 2, so each of the wave's four stencils per plane (even / odd x of either row) has compile-time
+// offsets and weights that are the same for every lane (scalar loads). A value at x + dx reaches the lane that
+// needs it through partial sums: the terms a lane's values add to its left and right neighbours are summed in
+// place and shifted once per row (lanes 0 and 63 are halo only). The wave marches the planes z0 .. z1-1 (z0
+// even); a register ring holds rows y0-2 .. y0+2 (every row either row's stencils reach: dy in [-2, 2] for the
+// even row, [-1, 1] for the odd one) of the planes z-2 .. z+2 plus D planes in flight, so a plane step issues
+// five pair loads for two output rows (a row-by-row kernel reads 9 to 19 stencil rows per output row). The
+// parity of z alternates the step's classes between (y, z) = (even, even), (odd, even) and (even, odd),
+// (odd, odd). The mode's epilogue (apply, residual, Jacobi, Chebyshev) is fused.
+template < typename T >
+struct P2ZPlane
 {
-   return k == 0 ? 1 : kP2RowSeg * k;
-}
-__host__ __device__ constexpr int p2SegRows( int N, int k )
-{
-   return triCount( N - 2 - p2SegStart( k ) );
-}
+   T v[5][2]; // rows y0-2 .. y0+2: (even x, odd x) of the lane
+   T e[2][2]; // the epilogue input of rows y0 and y0+1
+};
 
-template < typename T, int MODE, int KY, int KZ >
-__device__ __forceinline__ void p2RowPair( int N, int y, int z, int k, __amdgpu_buffer_rsrc_t rx, const T* __restrict__ wE,
-                                           const T* __restrict__ wO, i64 cellOff, const T* __restrict__ x, const T* __restrict__ in1,
-                                           T* __restrict__ out0, T* __restrict__ out1, const T* __restrict__ dinv, T a )
+// the pair (even x, odd x) of row y0 + KY at plane z (parity KZ) from the planes A .. E = z-2 .. z+2
+template < typename T, int KY, int KZ, int E0 >
+__device__ __forceinline__ void p2ZTerms( const P2ZPlane< T >& A, const P2ZPlane< T >& B, const P2ZPlane< T >& C,
+                                          const P2ZPlane< T >& Dn, const P2ZPlane< T >& E, const T* __restrict__ wc, T& a, T& r,
+                                          T& l )
 {
-   constexpr int              cE = ( KY << 1 ) | ( KZ << 2 ), cO = cE | 1;
-   constexpr P2StencilOffsets oE = p2StencilOffsets( cE ), oO = p2StencilOffsets( cO );
-   constexpr P2RowSet         rs = p2RowSet( KY, KZ );
-   const int                  w = N + 1, lane = int( threadIdx.x ) & 63;
-   const int                  xe = kP2RowSeg * k - 2 + 2 * lane;
-   T                          aE = T( 0 ), aO = T( 0 ), rE = T( 0 ), rO = T( 0 ), lE = T( 0 ), lO = T( 0 );
-   T                          vEr[rs.n], vOr[rs.n];
-#pragma unroll
-   for ( int r = 0; r < rs.n; ++r )
-   {
-      const int      dy = rs.d[r][0], dz = rs.d[r][1];
-      const int      W = w - ( z + dz ), yy = y + dy;
-      const int      base = S3i( w ) - S3i( W ) + rowStart( W, yy );
-      const unsigned vo   = ( xe >= 0 && xe < W - yy ) ? unsigned( base + xe ) * unsigned( sizeof( T ) ) : kDropOffset;
-      bufLoad2( rx, vo, 0, vEr[r], vOr[r] );
-   }
-   // the weights are the same for every lane: read through the constant address space they become scalar loads
+   constexpr int              cls = ( KY << 1 ) | ( KZ << 2 ) | E0;
+   constexpr P2StencilOffsets o   = p2StencilOffsets( cls );
    typedef const T __attribute__( ( address_space( 4 ) ) ) CT;
-   const CT* wEr = (const CT*) wE;
-   const CT* wOr = (const CT*) wO;
+   const CT* w = (const CT*) wc;
 #pragma unroll
-   for ( int r = 0; r < rs.n; ++r )
+   for ( int e = 0; e < o.n; ++e )
    {
-      const int dy = rs.d[r][0], dz = rs.d[r][1];
-      const T   vE = vEr[r], vO = vOr[r];
-#pragma unroll
-      for ( int e = 0; e < oE.n; ++e )
-         if ( oE.d[e][1] == dy && oE.d[e][2] == dz )
+      const P2ZPlane< T >& Q  = o.d[e][2] == -2 ? A : o.d[e][2] == -1 ? B : o.d[e][2] == 0 ? C : o.d[e][2] == 1 ? Dn : E;
+      const int            ry = KY + o.d[e][1] + 2;
+      const T              vE = Q.v[ry][0], vO = Q.v[ry][1], wt = w[e];
+      // even point x: offsets -2 / -1 come from lane-1 (shifted right), 0 / +1 are the lane's own pair, +2 comes
+      // from lane+1 (shifted left); odd point x+1: -2 from lane-1, -1 / 0 own, +1 / +2 from lane+1
+      if constexpr ( E0 == 0 )
+         switch ( o.d[e][0] )
          {
-            const T wt = wEr[e];
-            switch ( oE.d[e][0] )
-            {
-            case -2: rE += wt * vE; break;
-            case -1: rE += wt * vO; break;
-            case 0: aE += wt * vE; break;
-            case 1: aE += wt * vO; break;
-            default: lE += wt * vE; break;
-            }
+         case -2: r += wt * vE; break;
+         case -1: r += wt * vO; break;
+         case 0: a += wt * vE; break;
+         case 1: a += wt * vO; break;
+         default: l += wt * vE; break;
          }
-#pragma unroll
-      for ( int e = 0; e < oO.n; ++e )
-         if ( oO.d[e][1] == dy && oO.d[e][2] == dz )
+      else
+         switch ( o.d[e][0] )
          {
-            const T wt = wOr[e];
-            switch ( oO.d[e][0] )
-            {
-            case -2: rO += wt * vO; break;
-            case -1: aO += wt * vE; break;
-            case 0: aO += wt * vO; break;
-            case 1: lO += wt * vE; break;
-            default: lO += wt * vO; break;
-            }
+         case -2: r += wt * vO; break;
+         case -1: a += wt * vE; break;
+         case 0: a += wt * vO; break;
+         case 1: l += wt * vE; break;
+         default: l += wt * vO; break;
          }
    }
-   const T   valE = aE + laneShift< kDppWaveShr1 >( rE ) + laneShift< kDppWaveShl1 >( lE );
-   const T   valO = aO + laneShift< kDppWaveShr1 >( rO ) + laneShift< kDppWaveShl1 >( lO );
-   const int xmax = N - 1 - y - z;
-   const int row  = S3i( w ) - S3i( w - z ) + rowStart( w - z, y ); // cell-local start of row (y, z)
-   if ( lane == 0 || lane == 63 )
+}
+
+// f(0), f(1), ... in order while they return true
+template < typename F, int... I >
+__device__ __forceinline__ void p2ZSteps( F& f, std::integer_sequence< int, I... > )
+{
+   (void) ( f( std::integral_constant< int, I >{} ) && ... );
+}
+
+template < typename T, int MODE, int D >
+__global__ __launch_bounds__( kBlock ) void kP2Rows( int L, int nC, i64 stride, const ZSeg* __restrict__ segs, int nSegs,
+                                                      const T* __restrict__ st, const T* __restrict__ x, const T* __restrict__ in1,
+                                                      T* __restrict__ out0, T* __restrict__ out1, T a )
+{
+   constexpr bool kIn  = MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK || MODE == MODE_APPLY_ADD;
+   constexpr bool kTwo = MODE == MODE_CHEB0 || MODE == MODE_CHEBK;
+   constexpr bool kDiv = MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK;
+   constexpr int  B    = int( sizeof( T ) );
+   constexpr int  NR   = 5 + D; // ring slots: planes z-2 .. z+2 resident, D in flight after a step's loads
+   const int      nbx  = ( nSegs + 3 ) / 4;
+   const int      nb   = nbx * nC;
+   const int      lb   = remapXcd( int( blockIdx.x ), nb );
+   if ( lb >= nb )
       return;
-   // epilogue inputs of the lane's pair: the second right-hand side (or the old output) as one pair load
-   // (x + 1 <= xmax + 1 stays in the row); the point values themselves are the stencil's centre row; the
-   // inverse diagonal of a cell-interior point is the inverse of its merged stencil's centre weight
-   constexpr bool kIn = MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK ||
-                        MODE == MODE_APPLY_ADD;
-   T i1E = T( 0 ), i1O = T( 0 );
-   if constexpr ( kIn )
+   const int c  = lb / nbx;
+   const int si = ( lb - c * nbx ) * 4 + __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
+   if ( si >= nSegs )
+      return;
+   const ZSeg     sg   = segs[si];
+   const int      N    = 1 << L, w = N + 1, y0 = sg.y, z0 = sg.z0, z1 = sg.z1;
+   const int      lane = int( threadIdx.x ) & 63;
+   const int      xe   = kP2ZSeg * sg.x0 - 2 + 2 * lane;
+   const i64      cb   = i64( c ) * stride;
+   const unsigned nby  = unsigned( S3i( w ) ) * unsigned( B );
+   const auto     rx   = makeRsrc( x + cb, nby );
+   const auto     re   = makeRsrc( ( MODE == MODE_APPLY_ADD ? out0 : in1 ) + cb, nby );
+   const auto     r0   = makeRsrc( out0 + cb, nby );
+   const auto     r1   = makeRsrc( kTwo ? out1 + cb : out0 + cb, nby );
+   const T*       wcl  = st + i64( c ) * 8 * kP2StencilMax;
+
+   // the inverse diagonal of a cell-interior point is the inverse of its merged stencil's centre weight
+   T dvc[8];
+#pragma unroll
+   for ( int k = 0; k < 8; ++k )
+      dvc[k] = T( 1 );
+   if constexpr ( kDiv )
    {
-      const auto     re = makeRsrc( ( MODE == MODE_APPLY_ADD ? out0 : in1 ) + cellOff, unsigned( S3i( w ) ) * unsigned( sizeof( T ) ) );
-      const unsigned vq = xe <= xmax ? unsigned( row + xe ) * unsigned( sizeof( T ) ) : kDropOffset;
-      bufLoad2( re, vq, 0, i1E, i1O );
+      constexpr int ce[8] = { p2CentreIndex( p2StencilOffsets( 0 ) ), p2CentreIndex( p2StencilOffsets( 1 ) ),
+                              p2CentreIndex( p2StencilOffsets( 2 ) ), p2CentreIndex( p2StencilOffsets( 3 ) ),
+                              p2CentreIndex( p2StencilOffsets( 4 ) ), p2CentreIndex( p2StencilOffsets( 5 ) ),
+                              p2CentreIndex( p2StencilOffsets( 6 ) ), p2CentreIndex( p2StencilOffsets( 7 ) ) };
+#pragma unroll
+      for ( int k = 0; k < 8; ++k )
+         dvc[k] = T( 1 ) / wcl[k * kP2StencilMax + ce[k]];
    }
-   constexpr int rc = p2RowIndex( rs, 0, 0 );
-   T             dvE = T( 1 ), dvO = T( 1 );
-   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-   {
-      dvE = T( 1 ) / wEr[p2CentreIndex( oE )];
-      dvO = T( 1 ) / wOr[p2CentreIndex( oO )];
-   }
-   (void) dinv;
-   auto finish = [&]( int px, T val, T xv, T i1, T dv ) {
-      if ( px < 1 || px > xmax )
-         return;
-      const i64 q = cellOff + row + px;
-      T         r0, r1;
-      epilogue< T, MODE >( val, xv, i1, i1, dv, a, r0, r1 );
-      out0[q] = r0;
-      if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-         out1[q] = r1;
+   // plane p: rows y0-2 .. y0+2 (lanes outside a row, rows outside the plane and planes outside the cell read 0)
+   int  pl        = z0 - 2;
+   auto loadPlane = [&]( P2ZPlane< T >& Q ) __attribute__( ( always_inline ) ) {
+      const int  W   = w - pl;
+      const bool okp = pl >= 0 && pl <= N;
+      const int  bz  = S3i( w ) - S3i( okp ? W : w );
+#pragma unroll
+      for ( int j = 0; j < 5; ++j )
+      {
+         const int      yy = y0 - 2 + j;
+         const bool     ok = okp && yy >= 0 && xe >= 0 && xe < W - yy;
+         const unsigned vo = ok ? unsigned( bz + rowStart( W, yy ) + xe ) * unsigned( B ) : kDropOffset;
+         bufLoad2( rx, vo, 0, Q.v[j][0], Q.v[j][1] );
+      }
+      if constexpr ( kIn )
+      {
+#pragma unroll
+         for ( int r = 0; r < 2; ++r )
+         {
+            const int      yy = y0 + r;
+            const bool     ok = okp && yy >= 1 && pl >= 1 && xe >= 0 && xe <= N - 1 - yy - pl;
+            const unsigned vo = ok ? unsigned( bz + rowStart( W, yy ) + xe ) * unsigned( B ) : kDropOffset;
+            bufLoad2( re, vo, 0, Q.e[r][0], Q.e[r][1] );
+         }
+      }
+      ++pl;
    };
-   finish( xe, valE, vEr[rc], i1E, dvE );
-   finish( xe + 1, valO, vOr[rc], i1O, dvO );
-}
-
-template < typename T, int MODE >
-__global__ __launch_bounds__( kBlock ) void kP2Rows( int L, int nC, i64 stride, int wavesPerCell, const T* __restrict__ st,
-                                                     const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
-                                                     T* __restrict__ out1, const T* __restrict__ dinv, T a )
-{
-   const int N  = 1 << L;
-   const int b  = remapXcd( int( blockIdx.x ), int( gridDim.x ) );
-   const int g  = b * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
-   const int c  = g / wavesPerCell;
-   int       r  = g - c * wavesPerCell;
-   int       k  = 0;
-   if ( c >= nC )
-      return;
-   while ( r >= p2SegRows( N, k ) )
-      r -= p2SegRows( N, k++ );
-   int yp, zp;
-   planeXY( N - 2 - p2SegStart( k ), r, yp, zp );
-   const int  y = yp + 1, z = zp + 1;
-   const i64  cellOff = i64( c ) * stride;
-   const auto rx      = makeRsrc( x + cellOff, unsigned( S3i( N + 1 ) ) * unsigned( sizeof( T ) ) );
-   auto       wcls    = [&]( int cls ) -> const T* { return st + ( i64( c ) * 8 + cls ) * kP2StencilMax; };
-   switch ( ( y & 1 ) | ( ( z & 1 ) << 1 ) )
-   {
-   case 0: p2RowPair< T, MODE, 0, 0 >( N, y, z, k, rx, wcls( 0 ), wcls( 1 ), cellOff, x, in1, out0, out1, dinv, a ); break;
-   case 1: p2RowPair< T, MODE, 1, 0 >( N, y, z, k, rx, wcls( 2 ), wcls( 3 ), cellOff, x, in1, out0, out1, dinv, a ); break;
-   case 2: p2RowPair< T, MODE, 0, 1 >( N, y, z, k, rx, wcls( 4 ), wcls( 5 ), cellOff, x, in1, out0, out1, dinv, a ); break;
-   default: p2RowPair< T, MODE, 1, 1 >( N, y, z, k, rx, wcls( 6 ), wcls( 7 ), cellOff, x, in1, out0, out1, dinv, a ); break;
-   }
-}
-
-inline int p2RowWavesPerCell( int N )
-{
-   int n = 0;
-   for ( int k = 0; p2SegRows( N, k ) > 0; ++k )
-      n += p2SegRows( N, k );
-   return n;
+   int  z    = z0;
+   auto step = [&]( auto kz, const P2ZPlane< T >& A, const P2ZPlane< T >& Bp, const P2ZPlane< T >& C, const P2ZPlane< T >& Dn,
+                    const P2ZPlane< T >& E, P2ZPlane< T >& F ) __attribute__( ( always_inline ) ) {
+      constexpr int KZ = decltype( kz )::value;
+      loadPlane( F );
+      const int W  = w - z;
+      const int bz = S3i( w ) - S3i( W );
+      auto      rowOut = [&]( auto ky ) __attribute__( ( always_inline ) ) {
+         constexpr int KY = decltype( ky )::value;
+         constexpr int cE = ( KY << 1 ) | ( KZ << 2 );
+         T             aE = T( 0 ), aO = T( 0 ), rE = T( 0 ), rO = T( 0 ), lE = T( 0 ), lO = T( 0 );
+         // the weights stay in the scalar cache: an opaque copy of the pointer per step keeps the compiler from
+         // holding all four classes of the march in registers
+         const unsigned long long wv = reinterpret_cast< unsigned long long >( wcl );
+         unsigned                 wlo = __builtin_amdgcn_readfirstlane( unsigned( wv ) );
+         unsigned                 whi = __builtin_amdgcn_readfirstlane( unsigned( wv >> 32 ) );
+         asm volatile( "" : "+s"( wlo ), "+s"( whi ) );
+         const T* wb = reinterpret_cast< const T* >( ( ( unsigned long long ) whi << 32 ) | wlo );
+         const T*      wE = wb + cE * kP2StencilMax;
+         const T*      wO = wE + kP2StencilMax;
+         p2ZTerms< T, KY, KZ, 0 >( A, Bp, C, Dn, E, wE, aE, rE, lE );
+         p2ZTerms< T, KY, KZ, 1 >( A, Bp, C, Dn, E, wO, aO, rO, lO );
+         const T    valE = aE + laneShift< kDppWaveShr1 >( rE ) + laneShift< kDppWaveShl1 >( lE );
+         const T    valO = aO + laneShift< kDppWaveShr1 >( rO ) + laneShift< kDppWaveShl1 >( lO );
+         const int  yy   = y0 + KY;
+         const int  xmax = N - 1 - yy - z; // last interior x of the row
+         const int  row  = bz + rowStart( W, yy );
+         const bool okr  = lane >= 1 && lane <= 62 && yy >= 1 && z >= 1;
+         T o0E, o1E, o0O, o1O;
+         epilogue< T, MODE >( valE, C.v[KY + 2][0], C.e[KY][0], C.e[KY][0], dvc[cE], a, o0E, o1E );
+         epilogue< T, MODE >( valO, C.v[KY + 2][1], C.e[KY][1], C.e[KY][1], dvc[cE | 1], a, o0O, o1O );
+         const unsigned sE = okr && xe >= 1 && xe <= xmax ? unsigned( row + xe ) * unsigned( B ) : kDropOffset;
+         const unsigned sO = okr && xe + 1 >= 1 && xe + 1 <= xmax ? unsigned( row + xe + 1 ) * unsigned( B ) : kDropOffset;
+         bufStore( o0E, r0, sE, 0 );
+         bufStore( o0O, r0, sO, 0 );
+         if constexpr ( kTwo )
+         {
+            bufStore( o1E, r1, sE, 0 );
+            bufStore( o1O, r1, sO, 0 );
+         }
+      };
+      rowOut( std::integral_constant< int, 0 >{} );
+      rowOut( std::integral_constant< int, 1 >{} );
+      ++z;
+   };
+   P2ZPlane< T > ring[NR];
+#pragma unroll
+   for ( int u = 0; u < NR; ++u )
+#pragma unroll
+      for ( int r = 0; r < 2; ++r )
+         ring[u].e[r][0] = ring[u].e[r][1] = T( 0 );
+#pragma unroll
+   for ( int u = 0; u < NR - 1; ++u )
+      loadPlane( ring[u] ); // planes z0-2 .. z0+1+D
+   // ring slot u % NR holds plane z0 - 2 + u; U steps return the ring to its start and keep the parity of z
+   // (z0 is even, so step u has parity u & 1). The steps are expanded at compile time, so every ring slot is
+   // a fixed set of registers.
+   constexpr int U     = ( NR & 1 ) ? 2 * NR : NR;
+   auto          stepU = [&]( auto uc ) __attribute__( ( always_inline ) ) {
+      constexpr int u = decltype( uc )::value;
+      step( std::integral_constant< int, ( u & 1 ) >{}, ring[u % NR], ring[( u + 1 ) % NR], ring[( u + 2 ) % NR], ring[( u + 3 ) % NR],
+            ring[( u + 4 ) % NR], ring[( u + NR - 1 ) % NR] );
+      return z < z1;
+   };
+   while ( z < z1 )
+      p2ZSteps( stepU, std::make_integer_sequence< int, U >{} );
 }
 
 // ------------------------------------------------------------------ transfers: cell-interior points
@@ -1082,11 +1111,11 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
    {
       // constant coefficient: cell interiors by row pairs, cell faces by one-sided stencils, the points on cell
       // edges and vertices element by element
-      const int wpc = p2RowWavesPerCell( 1 << L );
-      const T*  stw = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
-      if ( wpc > 0 && ( c.flag & 1 ) )
-         hipLaunchKernelGGL( ( kP2Rows< T, MODE > ), dim3( xcdGrid( ceilDiv( i64( nC ) * wpc, kBlock / 64 ) ) ), dim3( kBlock ), 0, s, L,
-                             nC, g.stride, wpc, stw, (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, dv, T( c.a ) );
+      const T* stw = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
+      if ( ( c.flag & 1 ) && g.nP2ZSegs > 0 )
+         hipLaunchKernelGGL( ( kP2Rows< T, MODE, 1 > ), dim3( xcdGrid( nC * ceilDiv( g.nP2ZSegs, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
+                             L, nC, g.stride, g.p2ZSegs, g.nP2ZSegs, stw, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
+                             (T*) c.out1, T( c.a ) );
       P2FaceTiles ft;
       ft.start[0] = 0;
       for ( int k = 0; k < 4; ++k )
@@ -1095,7 +1124,7 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
       if ( ft.start[4] > 0 )
          hipLaunchKernelGGL( ( kP2Face< T, false > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, fsw, m.cellFaces,
                              c.flag, part, (const T*) c.x );
-      const int tB = ceilDiv( p2EdgeVertexCount( 1 << L ), kBlock );
+      const int tB = ceilDiv( i64( p2EdgeVertexCount( 1 << L ) ) * kP2EvLanes, kBlock );
       hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq,
                           m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
                           (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
@@ -1118,7 +1147,7 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
       if ( ft.start[4] > 0 )
          hipLaunchKernelGGL( ( kP2Face< T, true > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, sw, m.cellFaces,
                              c.flag, part, (const T*) c.x );
-      const int tB = ceilDiv( p2EdgeVertexCount( 1 << L ), kBlock );
+      const int tB = ceilDiv( i64( p2EdgeVertexCount( 1 << L ) ) * kP2EvLanes, kBlock );
       hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq,
                           m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
                           (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );

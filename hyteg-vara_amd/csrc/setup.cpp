@@ -302,6 +302,22 @@ LevelGeom& MeshCtx::geom( int L )
       g.nZSegs[k] = (int) v.size();
       g.zSegs[k]  = upload( v );
    }
+   // P2 row pairs (y0, y0+1), y0 even: segment k holds the x values 124k .. 124k+123 (k = 0 from x = 1); the
+   // pair has interior outputs while z <= N - 1 - x_first - max(y0, 1). Planes are chunked from z = 0 so every
+   // segment starts on an even plane (plane 0 itself has no interior output).
+   // Coarse lattices take shorter segments: more waves, fewer serial plane steps each.
+   g.p2ZSegsHost.clear();
+   const int p2Chunk = N >= 256 ? kP2ZChunk : ( N >= 128 ? 16 : 8 );
+   for ( int z0 = 0; z0 <= N - 3; z0 += p2Chunk )
+      for ( int k = 0; kP2ZSeg * k <= N - 3; ++k )
+         for ( int y0 = 0; y0 <= N - 3; y0 += 2 )
+         {
+            const int xs = k == 0 ? 1 : kP2ZSeg * k, zEnd = N - xs - std::max( y0, 1 );
+            if ( zEnd > std::max( z0, 1 ) )
+               g.p2ZSegsHost.push_back( { y0, k, z0, std::min( z0 + p2Chunk, zEnd ) } );
+         }
+   g.nP2ZSegs = (int) g.p2ZSegsHost.size();
+   g.p2ZSegs  = upload( g.p2ZSegsHost );
    // z = 0 and y = 0 faces of every cell: rows of 62 points
    g.flatItemsHost.clear();
    for ( int c = 0; c < mesh->nC(); ++c )
@@ -351,6 +367,7 @@ MeshCtx::~MeshCtx()
       for ( auto* zs : g.zSegs )
          (void) hipFree( zs );
       (void) hipFree( g.flatItems );
+      (void) hipFree( g.p2ZSegs );
       (void) hipFree( g.chunks );
    }
    (void) hipFree( prims );
