@@ -615,11 +615,6 @@ __global__ __launch_bounds__( kBlock ) void kP2Rows( int L, int nC, i64 stride, 
 // ------------------------------------------------------------------ transfers: cell-interior points
 // Interior fine points of one residue class r = P mod 4 share one interpolation stencil (coarse offsets and
 // weights known at compile time); interior coarse points of one parity class share one restriction stencil.
-// Fine points P = 4 (Q' + lo) + r with lo_c = 1 where r_c = 0 and |Q'| <= M enumerate the residue class.
-__host__ __device__ constexpr int p2ProlongExtent( int Nf, int rx, int ry, int rz )
-{
-   return Nf - 1 - rx - ry - rz < 0 ? -1 : ( Nf - 1 - rx - ry - rz ) / 4 - ( rx == 0 ) - ( ry == 0 ) - ( rz == 0 );
-}
 // interior points of a lattice with N intervals of parity class cls: p = 2 q + cls, q' = q - (1 - cls), |q'| <= M
 __host__ __device__ constexpr int p2InteriorExtent( int N, int cls )
 {
@@ -631,48 +626,90 @@ struct P2ClassTiles
    int start[65]; // first tile of each class within a cell; start[nClasses] = tiles per cell
 };
 
-template < typename T, int RX, int RY, int RZ >
-__device__ __forceinline__ void p2ProlongPt( int c, int s, int Nf, i64 strideF, i64 strideC, T* __restrict__ fine,
-                                             const T* __restrict__ coarse, int add )
+// Interior fine points row by row: one wave per fine row (y, z), y, z >= 1, y + z <= Nf - 2; lane l holds the four
+// consecutive points x = 4 q + r, r = 0..3, q = 64 j + l in pass j. All points of the wave share (y mod 4, z mod 4),
+// so each of a lane's four points has its residue class's compile-time stencil (p2ProlongStencil, coarse offsets
+// from (2 q, 2 (y div 4), 2 (z div 4))), and the wave reads and writes whole fine rows.
+template < typename T, int RY, int RZ >
+__device__ __forceinline__ void p2ProlongRow( int y, int z, int Nf, const T* __restrict__ cs, T* __restrict__ fs, int add )
 {
-   constexpr P2ProlongStencil ps = p2ProlongStencil( RX, RY, RZ );
-   const int                  M  = p2ProlongExtent( Nf, RX, RY, RZ );
-   if ( M < 0 || s >= S3i( M + 1 ) )
-      return;
-   int a, b, d;
-   tetXYZ( M + 1, s, a, b, d );
-   const int qx = a + ( RX == 0 ), qy = b + ( RY == 0 ), qz = d + ( RZ == 0 );
+   const int lane = int( threadIdx.x ) & 63;
    const int wc = ( Nf >> 1 ) + 1, wf = Nf + 1;
-   const T*  cs = coarse + i64( c ) * strideC;
-   T         v  = T( 0 );
+   const int qy = y >> 2, qz = z >> 2;
+   const int xmax = Nf - 1 - y - z; // last interior x of the row
+   const int rowF = S3i( wf ) - S3i( wf - z ) + rowStart( wf - z, y );
+   for ( int x0 = 4 * lane; x0 <= xmax; x0 += 256 )
+   {
+      const int qx = x0 >> 2;
+      T         v[4];
+      auto      point = [&]( auto rx ) __attribute__( ( always_inline ) ) {
+         constexpr int              RX = decltype( rx )::value;
+         constexpr P2ProlongStencil ps = p2ProlongStencil( RX, RY, RZ );
+         T                          u  = T( 0 );
+         // only interior points: their stencils stay inside the coarse cell
+         if ( x0 + RX >= 1 && x0 + RX <= xmax )
+         {
 #pragma unroll
-   for ( int k = 0; k < ps.n; ++k )
-      v += T( ps.w[k] ) * cs[cidx( wc, 2 * qx + ps.d[k][0], 2 * qy + ps.d[k][1], 2 * qz + ps.d[k][2] )];
-   const i64 o = i64( c ) * strideF + cidx( wf, 4 * qx + RX, 4 * qy + RY, 4 * qz + RZ );
-   fine[o]     = add ? fine[o] + v : v;
-}
-template < typename T, int R >
-__device__ __forceinline__ void p2ProlongDispatch( int r, int c, int s, int Nf, i64 strideF, i64 strideC, T* __restrict__ fine,
-                                                   const T* __restrict__ coarse, int add )
-{
-   if ( r == R )
-      p2ProlongPt< T, ( R & 3 ), ( ( R >> 2 ) & 3 ), ( R >> 4 ) >( c, s, Nf, strideF, strideC, fine, coarse, add );
-   else if constexpr ( R < 63 )
-      p2ProlongDispatch< T, R + 1 >( r, c, s, Nf, strideF, strideC, fine, coarse, add );
+            for ( int k = 0; k < ps.n; ++k )
+               u += T( ps.w[k] ) * cs[cidx( wc, 2 * qx + ps.d[k][0], 2 * qy + ps.d[k][1], 2 * qz + ps.d[k][2] )];
+         }
+         v[RX] = u;
+      };
+      point( std::integral_constant< int, 0 >{} );
+      point( std::integral_constant< int, 1 >{} );
+      point( std::integral_constant< int, 2 >{} );
+      point( std::integral_constant< int, 3 >{} );
+#pragma unroll
+      for ( int r = 0; r < 4; ++r )
+      {
+         const int x = x0 + r;
+         if ( x >= 1 && x <= xmax )
+            fs[rowF + x] = add ? fs[rowF + x] + v[r] : v[r];
+      }
+   }
 }
 template < typename T >
-__global__ __launch_bounds__( kBlock ) void kP2ProlongInt( int Lf, i64 strideF, i64 strideC, P2ClassTiles ct, int flag,
-                                                           T* __restrict__ fine, const T* __restrict__ coarse, int add )
+__global__ __launch_bounds__( kBlock ) void kP2ProlongRows( int Lf, int nC, i64 strideF, i64 strideC, int flag, T* __restrict__ fine,
+                                                            const T* __restrict__ coarse, int add )
 {
-   if ( !( flag & 1 ) )
+   const int Nf = 1 << Lf, W = Nf - 3; // fine rows: a triangle with W rows
+   if ( !( flag & 1 ) || W <= 0 )
       return;
-   const int per = ct.start[64];
-   const int c   = int( blockIdx.x ) / per;
-   const int t   = int( blockIdx.x ) - c * per;
-   int       r   = 0;
-   while ( r < 63 && t >= ct.start[r + 1] )
-      ++r;
-   p2ProlongDispatch< T, 0 >( r, c, ( t - ct.start[r] ) * kBlock + int( threadIdx.x ), 1 << Lf, strideF, strideC, fine, coarse, add );
+   const int rows = W * ( W + 1 ) / 2, nbx = ( rows + 3 ) / 4, nb = nbx * nC;
+   const int lb   = remapXcd( int( blockIdx.x ), nb );
+   if ( lb >= nb )
+      return;
+   const int c = lb / nbx;
+   const int r = ( lb - c * nbx ) * 4 + __builtin_amdgcn_readfirstlane( int( threadIdx.x ) >> 6 );
+   if ( r >= rows )
+      return;
+   int a, b;
+   planeXY( W, r, a, b );
+   const int y = a + 1, z = b + 1;
+   const T*  cs = coarse + i64( c ) * strideC;
+   T*        fs = fine + i64( c ) * strideF;
+#define HYG_P2_PROW( RY, RZ )                                                                                          \
+   case ( RY ) | ( ( RZ ) << 2 ): p2ProlongRow< T, RY, RZ >( y, z, Nf, cs, fs, add ); break;
+   switch ( ( y & 3 ) | ( ( z & 3 ) << 2 ) )
+   {
+      HYG_P2_PROW( 0, 0 )
+      HYG_P2_PROW( 1, 0 )
+      HYG_P2_PROW( 2, 0 )
+      HYG_P2_PROW( 3, 0 )
+      HYG_P2_PROW( 0, 1 )
+      HYG_P2_PROW( 1, 1 )
+      HYG_P2_PROW( 2, 1 )
+      HYG_P2_PROW( 3, 1 )
+      HYG_P2_PROW( 0, 2 )
+      HYG_P2_PROW( 1, 2 )
+      HYG_P2_PROW( 2, 2 )
+      HYG_P2_PROW( 3, 2 )
+      HYG_P2_PROW( 0, 3 )
+      HYG_P2_PROW( 1, 3 )
+      HYG_P2_PROW( 2, 3 )
+   default: p2ProlongRow< T, 3, 3 >( y, z, Nf, cs, fs, add ); break;
+   }
+#undef HYG_P2_PROW
 }
 
 template < typename T, int CLS >
@@ -1060,35 +1097,49 @@ __global__ __launch_bounds__( kBlock ) void kP2RestrictLower( const LowerChunk* 
    const int        lane = int( threadIdx.x & 63 );
    const LowerChunk ch   = chunks[ci];
    const LowerPrim  p    = prims[ch.prim];
-   if ( !( p.bc & flag ) || lane >= ch.n )
+   if ( !( p.bc & flag ) )
       return;
    const int Nc = 1 << Lc, wc = Nc + 1, Nf = 2 * Nc, wf = Nf + 1;
-   const int i = ch.i0 + lane;
-   T         v = T( 0 );
-   for ( int k = 0; k < p.nCells; ++k )
+   // G lanes per coarse point (the largest power of two with G n <= 64 for the chunk's n points), each summing every
+   // G-th support point: a macro-vertex point gathers from every incident cell, which one lane would serialise
+   int G = 1;
+   while ( G < 64 && 2 * G * ch.n <= 64 )
+      G *= 2;
+   for ( int base = 0; base < ch.n; base += 64 / G )
    {
-      const LowerEntry e = entries[p.entry0 + k];
-      int              nx, ny, nz;
-      entryPos( e, Nc, i, ch.j, nx, ny, nz );
-      const int       cls = ( nx & 1 ) | ( ( ny & 1 ) << 1 ) | ( ( nz & 1 ) << 2 );
-      const T*        fs  = fine + i64( e.cell ) * strideF;
-      const CellInfo& cf  = cinfo[e.cell];
-      const int       ns  = tb->nSup[cls];
-      for ( int m = 0; m < ns; ++m )
+      const int  pi  = base + lane / G, sub = lane % G;
+      const bool ok  = pi < ch.n;
+      const int  i   = ch.i0 + pi;
+      T          v   = T( 0 );
+      for ( int k = 0; ok && k < p.nCells; ++k )
       {
-         const P2Tables::Sup& sp = tb->sup[cls][m];
-         const int            fx = 2 * nx + sp.d[0], fy = 2 * ny + sp.d[1], fz = 2 * nz + sp.d[2];
-         if ( !inCell( Nf, fx, fy, fz ) )
-            continue;
-         v += T( sp.w * cf.invNC[classifySlot( Nf, fx, fy, fz )] ) * fs[cidx( wf, fx, fy, fz )];
+         const LowerEntry e = entries[p.entry0 + k];
+         int              nx, ny, nz;
+         entryPos( e, Nc, i, ch.j, nx, ny, nz );
+         const int       cls = ( nx & 1 ) | ( ( ny & 1 ) << 1 ) | ( ( nz & 1 ) << 2 );
+         const T*        fs  = fine + i64( e.cell ) * strideF;
+         const CellInfo& cf  = cinfo[e.cell];
+         const int       ns  = tb->nSup[cls];
+         for ( int m = sub; m < ns; m += G )
+         {
+            const P2Tables::Sup& sp = tb->sup[cls][m];
+            const int            fx = 2 * nx + sp.d[0], fy = 2 * ny + sp.d[1], fz = 2 * nz + sp.d[2];
+            if ( !inCell( Nf, fx, fy, fz ) )
+               continue;
+            v += T( sp.w * cf.invNC[classifySlot( Nf, fx, fy, fz )] ) * fs[cidx( wf, fx, fy, fz )];
+         }
       }
-   }
-   for ( int k = 0; k < p.nCells; ++k )
-   {
-      const LowerEntry e = entries[p.entry0 + k];
-      int              nx, ny, nz;
-      entryPos( e, Nc, i, ch.j, nx, ny, nz );
-      coarse[i64( e.cell ) * strideC + cidx( wc, nx, ny, nz )] = v;
+      for ( int o = 1; o < G; o <<= 1 )
+         v += __shfl_xor( v, o );
+      if ( !ok || sub != 0 )
+         continue;
+      for ( int k = 0; k < p.nCells; ++k )
+      {
+         const LowerEntry e = entries[p.entry0 + k];
+         int              nx, ny, nz;
+         entryPos( e, Nc, i, ch.j, nx, ny, nz );
+         coarse[i64( e.cell ) * strideC + cidx( wc, nx, ny, nz )] = v;
+      }
    }
 }
 
@@ -1320,17 +1371,11 @@ static void p2ProlongT( const MeshCtx& m, const LevelGeom& gf, const P2Tables* t
                         bool add, hipStream_t st )
 {
    const int    nC      = m.mesh->nC();
-   const i64    strideC = cellStride( Lf - 1 );
-   P2ClassTiles ct;
-   ct.start[0] = 0;
-   for ( int r = 0; r < 64; ++r )
-   {
-      const int M     = p2ProlongExtent( 1 << Lf, r & 3, ( r >> 2 ) & 3, r >> 4 );
-      ct.start[r + 1] = ct.start[r] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
-   }
-   if ( ct.start[64] > 0 )
-      hipLaunchKernelGGL( ( kP2ProlongInt< T > ), dim3( nC * ct.start[64] ), dim3( kBlock ), 0, st, Lf, gf.stride, strideC, ct, flag,
-                          (T*) fine, (const T*) coarse, add ? 1 : 0 );
+   const i64 strideC = cellStride( Lf - 1 );
+   const int W       = ( 1 << Lf ) - 3;
+   if ( W > 0 )
+      hipLaunchKernelGGL( ( kP2ProlongRows< T > ), dim3( xcdGrid( nC * ceilDiv( i64( W ) * ( W + 1 ) / 2, kBlock / 64 ) ) ), dim3( kBlock ),
+                          0, st, Lf, nC, gf.stride, strideC, flag, (T*) fine, (const T*) coarse, add ? 1 : 0 );
    if ( gf.nChunks > 0 )
       hipLaunchKernelGGL( ( kP2ProlongLower< T > ), dim3( ceilDiv( gf.nChunks, kBlock / 64 ) ), dim3( kBlock ), 0, st, gf.chunks,
                           gf.nChunks, m.prims, m.entries, tb, Lf, gf.stride, strideC, flag, (T*) fine, (const T*) coarse,
