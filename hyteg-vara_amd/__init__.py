@@ -66,6 +66,8 @@ def _sig(lib):
         "hyg_op_set_config": (i, [vp, i, i, i, i]),
         "hyg_op_get_config": (i, [vp, i, i, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "hyg_smooth_jac": (i, [vp, vp, vp, vp, d, i, i, vp]),
+        "hyg_smooth_sor": (i, [vp, vp, vp, d, i, i, i, vp]),
+        "hyg_mesh_sor_colours": (i, [vp, C.POINTER(C.c_int), P(np.int32)]),
         "hyg_inverse_diagonal": (i, [vp, vp, i, vp]),
         "hyg_residual": (i, [vp, vp, vp, vp, i, i, vp]),
         "hyg_restrict": (i, [vp, i, i, vp]),
@@ -163,6 +165,14 @@ class Mesh:
         check(load().hyg_mesh_dof_coords(self.h, L, xyz.ctypes.data, bnd.ctypes.data))
         return xyz.reshape(n, 3), bnd.astype(bool)
 
+    def sor_colours(self):
+        """(number of colours, colour of every macro-vertex) of the multi-colour SOR ordering."""
+        n = C.c_int()
+        nv = int(self.info()[0])
+        cols = np.zeros(max(nv, 1), dtype=np.int32)
+        check(load().hyg_mesh_sor_colours(self.h, C.byref(n), cols))
+        return n.value, cols[:nv]
+
     def boundary(self, L):
         bnd = np.zeros(self.ndofs(L), dtype=np.uint8)
         check(load().hyg_mesh_dof_coords(self.h, L, None, bnd.ctypes.data))
@@ -243,6 +253,9 @@ class P1ConstantOperator:
 
     def smooth_jac(self, dst, rhs, src, relax, L, flag=INNER):
         check(load().hyg_smooth_jac(self.h, dst.h, rhs.h, src.h, relax, L, flag, None))
+
+    def smooth_sor(self, dst, rhs, relax, L, flag=INNER, backwards=False):
+        check(load().hyg_smooth_sor(self.h, dst.h, rhs.h, relax, L, flag, 1 if backwards else 0, None))
 
     def inverse_diagonal(self, dst, L):
         check(load().hyg_inverse_diagonal(self.h, dst.h, L, None))
@@ -333,7 +346,7 @@ class GMG:
     def __init__(self, op, minL, maxL, dtype=F64, pre=2, post=2, inc=0, smoother="jacobi", relax=2.0 / 3.0,
                  cheb_order=2, cheb_bounds=(0.0, 0.0), cg_max_iter=10000, cg_tol=1e-16):
         self.op, self.maxL = op, maxL
-        ip = np.array([minL, maxL, pre, post, inc, 0 if smoother == "jacobi" else 1, cheb_order, cg_max_iter],
+        ip = np.array([minL, maxL, pre, post, inc, {"jacobi": 0, "cheb": 1, "sor": 2}[smoother], cheb_order, cg_max_iter],
                       dtype=np.int32)
         dp = np.array([relax, cheb_bounds[0], cheb_bounds[1], cg_tol])
         h = _out()

@@ -295,6 +295,13 @@ int smoothImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, hipStream_t s
       c.a    = g->relax;
       return stencilImpl( g->op, L, g->dtype, c, st );
    }
+   if ( g->smoother == 2 )
+   {
+      // SORSmoother::solve: smooth_sor(x, b, relax, level, flag), multi-colour order
+      MeshCtx& mc = g->op->mesh->ctx;
+      launchSor( mc, mc.geom( L ), g->op->levels[L], L, g->dtype, x->data[L], b->data[L], g->relax, kGmgFlag, false, st );
+      return launched( "sor" );
+   }
    // ChebyshevSmoother::solve, fused: t0 = D^-1 (b - A x), y = x + c0 t0; t_k = D^-1 A t_{k-1}, y += c_k t_k; x = y
    const int K = g->chebOrder;
    c.mode      = MODE_CHEB0;
@@ -1206,6 +1213,35 @@ int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double 
    return stencilImpl( op, level, src->dtype, c, S( stream ) );
 }
 
+int hyg_smooth_sor( hyg_op op, hyg_func dst, hyg_func rhs, double relax, int level, int flag, int backwards, void* stream )
+{
+   level = lat( op, level );
+   HYG_TRY( checkOpLevel( op, level ) );
+   HYG_TRY( checkPair( dst, rhs, level ) );
+   HYG_TRY( checkSpace( op, dst ) );
+   if ( op->kind != 0 )
+      return fail( HYG_ERR_ARG, "smooth_sor is implemented for the constant P1 operator" );
+   if ( dst == rhs )
+      return fail( HYG_ERR_ARG, "smooth_sor requires dst different from rhs" );
+   MeshCtx& mc = op->mesh->ctx;
+   launchSor( mc, mc.geom( level ), op->levels[level], level, dst->dtype, dst->data[level], rhs->data[level], relax, flag,
+              backwards != 0, S( stream ) );
+   return launched( "sor" );
+}
+
+int hyg_mesh_sor_colours( hyg_mesh mesh, int* ncolours, int* vertex_colours )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh || !ncolours )
+      return fail( HYG_ERR_ARG, "null argument" );
+   const MeshCtx& mc = mesh->ctx;
+   *ncolours         = mc.nColours;
+   if ( vertex_colours )
+      for ( int v = 0; v < mc.mesh->nV(); ++v )
+         vertex_colours[v] = mc.vertexColours[v];
+   return HYG_OK;
+}
+
 int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream )
 {
    level = lat( op, level );
@@ -1375,8 +1411,10 @@ int hyg_gmg_create( hyg_op op, int dtype, const int* ip, const double* dp, hyg_g
       return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
    if ( lat( op, ip[0] ) < op->minL || lat( op, ip[1] ) > op->maxL || ip[0] > ip[1] )
       return fail( HYG_ERR_ARG, "solver levels outside the operator's range" );
-   if ( ip[5] != 0 && ip[5] != 1 )
-      return fail( HYG_ERR_ARG, "smoother must be 0 (weighted Jacobi) or 1 (Chebyshev)" );
+   if ( ip[5] < 0 || ip[5] > 2 )
+      return fail( HYG_ERR_ARG, "smoother must be 0 (weighted Jacobi), 1 (Chebyshev) or 2 (SOR)" );
+   if ( ip[5] == 2 && op->kind != 0 )
+      return fail( HYG_ERR_ARG, "the SOR smoother is implemented for the constant P1 operator" );
    if ( ip[5] == 1 && ( ip[6] < 1 || ip[6] > 5 ) )
       return fail( HYG_ERR_ARG, "Chebyshev order must be 1..5" );
    auto* g      = new hyg_gmg_s;

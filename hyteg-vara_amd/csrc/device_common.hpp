@@ -130,7 +130,7 @@ __device__ __forceinline__ void epilogue( T Ax, T xc, T in1v, T out0v, T dinv, T
       o0 = out0v + Ax;
    else if constexpr ( MODE == MODE_RESIDUAL )
       o0 = in1v - Ax;
-   else if constexpr ( MODE == MODE_JACOBI )
+   else if constexpr ( MODE == MODE_JACOBI || MODE == MODE_SOR )
       o0 = xc + a * dinv * ( in1v - Ax );
    else if constexpr ( MODE == MODE_CHEB0 )
    {

@@ -54,6 +54,54 @@ __global__ __launch_bounds__( kBlock ) void kFaceCombine( const LowerChunk* __re
    }
 }
 
+// colour of point (i, j) of a macro-face, -edge or -vertex in the multi-colour SOR ordering: the sum over the
+// vertices of a cell containing it of (vertex colour) x (barycentric lattice coordinate), mod the number of
+// colours; every containing cell gives the same value, the first entry's is used
+__device__ __forceinline__ int sorColour( const CellInfo* __restrict__ cinfo, const LowerEntry& e, int N, int i, int j, int m )
+{
+   int qx, qy, qz;
+   entryPos( e, N, i, j, qx, qy, qz );
+   const CellInfo& ci = cinfo[e.cell];
+   return ( ci.colour[0] * ( N - qx - qy - qz ) + ci.colour[1] * qx + ci.colour[2] * qy + ci.colour[3] * qz ) % m;
+}
+
+// one colour pass of multi-colour SOR at macro-face points (constant operator): x += a D^-1 (b - A x) at the
+// points of colour sorK, in place, every replica
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kFaceCombineSor( const LowerChunk* __restrict__ chunks, int nChunks,
+                                                             const LowerPrim* __restrict__ prims, const LowerEntry* __restrict__ entries,
+                                                             int firstFacePrim, const T* __restrict__ part, const T* __restrict__ primInv,
+                                                             const CellInfo* __restrict__ cinfo, int sorK, int sorM, int L, i64 stride,
+                                                             int flag, T* __restrict__ x, const T* __restrict__ b, T a )
+{
+   const int nb = ( nChunks + 3 ) / 4;
+   const int ci = remapXcd( int( blockIdx.x ), nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
+   if ( ci >= nChunks )
+      return;
+   const LowerChunk ch   = chunks[ci];
+   const LowerPrim  p    = prims[ch.prim];
+   const int        lane = int( threadIdx.x & 63 );
+   if ( !( p.bc & flag ) || lane >= ch.n )
+      return;
+   const int N = 1 << L, NP = N + 1, w = N + 1;
+   const int i = ch.i0 + lane, j = ch.j, f = ch.prim - firstFacePrim;
+   if ( sorColour( cinfo, entries[p.entry0], N, i, j, sorM ) != sorK )
+      return;
+   T   Ax = T( 0 );
+   i64 o[2];
+   for ( int k = 0; k < p.nCells; ++k )
+   {
+      const LowerEntry e = entries[p.entry0 + k];
+      int              qx, qy, qz;
+      entryPos( e, N, i, j, qx, qy, qz );
+      o[k] = i64( e.cell ) * stride + cidx( w, qx, qy, qz );
+      Ax += part[i64( 2 * f + k ) * NP * NP + facePartIndex( entrySlot( e ), N, qx, qy, qz )];
+   }
+   const T v = x[o[0]] + a * primInv[ch.prim] * ( b[o[0]] - Ax );
+   for ( int k = 0; k < p.nCells; ++k )
+      x[o[k]] = v;
+}
+
 // micro-cell types WHITE_UP, BLUE_UP, GREEN_UP, WHITE_DOWN, BLUE_DOWN, GREEN_DOWN: vertex offsets
 // from the micro-cell's anchor (volumedofspace/CellDoFIndexing.hpp:155-198)
 constexpr int kMC[6][4][3] = { { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } },

@@ -175,6 +175,7 @@ struct CellInfo
    int      prim[15];  // LowerPrim index of local slot k >= 1 (slot 0: -1)
    int      entry[15]; // LowerEntry of the (primitive, this cell) pair of local slot k >= 1 (slot 0: -1)
    double   invNC[16]; // 1 / #cells of the local slot's primitive (slot 0: 1)
+   int      colour[4]; // multi-colour SOR: colour of each local vertex in the macro-vertex colouring
 };
 
 // Geometry of one macro-cell for the variable-coefficient elementwise operator at one level.
@@ -526,6 +527,8 @@ struct MeshCtx
    CellFaceSlot*               cellFaces = nullptr;
    int                         device   = 0;
    P2Tables*                   p2tab    = nullptr; // device copy, built on first use
+   int                         nColours = 0;       // colours of the macro-vertex colouring (multi-colour SOR)
+   std::vector< int >          vertexColours;      // colour of every macro-vertex
    void                        buildTopology(); // prims, entries, cell info (host + device)
    LevelGeom&                  geom( int L );
    const P2Tables*             p2Tables();

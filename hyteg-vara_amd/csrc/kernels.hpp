@@ -14,6 +14,7 @@ enum StencilMode : int
    MODE_JACOBI    = 3, // out0 = x + a * d * (in1 - A x)
    MODE_CHEB0     = 4, // t = d * (in1 - A x); out0 = t; out1 = x + a * t
    MODE_CHEBK     = 5, // t = d * A x;         out0 = t; out1 = in1 + a * t
+   MODE_SOR       = 6, // one colour of multi-colour SOR: out0 = x + a * d * (in1 - A x) at points of that colour (out0 = x)
 };
 
 struct StencilCall
@@ -30,6 +31,9 @@ struct StencilCall
 // Runs the cell-interior kernel and the macro-face/edge/vertex kernel of one stencil operation.
 void launchStencil( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, int dtype, const StencilCall& c,
                     hipStream_t s );
+// One multi-colour SOR sweep of the constant P1 operator on x (in place) with right-hand side b.
+void launchSor( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, int dtype, void* x, const void* b, double relax,
+                int flag, bool backwards, hipStream_t s );
 
 // Variable-coefficient elementwise operator (op.kind == 1): one stencil operation, and the inverse
 // diagonal of level L written into op.levels[L].dinv / dinvF (setup).

@@ -158,10 +158,58 @@ void MeshCtx::buildTopology()
    for ( int f = 0; f < m.nF(); ++f )
       addPrim( 2, m.fB[f], m.fC[f], m.F[f].data(), 3 );
 
+   // Multi-colour SOR. A lattice point with barycentric lattice coordinates b_i (sum N) over the vertices v_i of a
+   // cell containing it gets the colour (sum_i c(v_i) b_i) mod nColours, a value that depends only on the point.
+   // The 15-point stencil couples points that differ by e_i - e_j (i != j) or, along the cell's inner diagonal
+   // (x, y, z) +- (1, -1, 1), by e_1 - e_2 + e_3 - e_0 in the cell's local vertex order, so coupled points never
+   // share a colour if the vertex colours differ along every macro-edge and c_1 + c_3 != c_0 + c_2 (mod
+   // nColours) in every cell. The vertex colours are chosen greedily in index order (smallest admissible colour)
+   // for the smallest nColours >= 4 for which that succeeds.
+   {
+      std::vector< std::vector< int > > nb( m.nV() );
+      for ( const auto& e : m.E )
+      {
+         nb[e[0]].push_back( e[1] );
+         nb[e[1]].push_back( e[0] );
+      }
+      std::vector< int > col;
+      for ( nColours = 4; nColours <= 64; ++nColours )
+      {
+         col.assign( m.nV(), -1 );
+         bool ok = true;
+         for ( int v = 0; v < m.nV() && ok; ++v )
+         {
+            ok = false;
+            for ( int k = 0; k < nColours && !ok; ++k )
+            {
+               bool good = true;
+               for ( int u : nb[v] )
+                  good = good && col[u] != k;
+               col[v] = k;
+               for ( int c : m.vC[v] )
+               {
+                  const auto& cv = m.Cv[c];
+                  if ( good && col[cv[0]] >= 0 && col[cv[1]] >= 0 && col[cv[2]] >= 0 && col[cv[3]] >= 0 )
+                     good = ( ( col[cv[1]] + col[cv[3]] - col[cv[0]] - col[cv[2]] ) % nColours + nColours ) % nColours != 0;
+               }
+               ok = good;
+               if ( !ok )
+                  col[v] = -1;
+            }
+         }
+         if ( ok )
+            break;
+      }
+      if ( nColours > 64 )
+         throw std::runtime_error( "no SOR colouring with at most 64 colours" );
+      vertexColours = col;
+   }
    cellInfoHost.resize( m.nC() );
    for ( int c = 0; c < m.nC(); ++c )
    {
       CellInfo& ci = cellInfoHost[c];
+      for ( int k = 0; k < 4; ++k )
+         ci.colour[k] = vertexColours[m.Cv[c][k]];
       ci.bcword    = 1; // slot 0: cells are always inner
       ci.ownmask   = 1;
       ci.prim[0]   = -1;

@@ -113,9 +113,10 @@ __device__ __forceinline__ void cellZ( int bid, const ZSeg* __restrict__ segs, i
                                                     const T* __restrict__ cellSt, const CellFaceSlot* __restrict__ cfs,
                                                     const T* __restrict__ entryW, T* __restrict__ part, int flag,
                                                     const T* __restrict__ x, const T* __restrict__ in1, T* __restrict__ out0,
-                                                    T* __restrict__ out1, T a )
+                                                    T* __restrict__ out1, T a, const CellInfo* __restrict__ cinfo, int sorK,
+                                                    int sorM )
 {
-   constexpr bool kIn1 = MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK;
+   constexpr bool kIn1 = MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK || MODE == MODE_SOR;
    constexpr bool kE   = kIn1 || MODE == MODE_APPLY_ADD;
    constexpr bool kTwo = MODE == MODE_CHEB0 || MODE == MODE_CHEBK;
    constexpr int  B    = int( sizeof( T ) );
@@ -146,6 +147,16 @@ __device__ __forceinline__ void cellZ( int bid, const ZSeg* __restrict__ segs, i
    for ( int d = 0; d < 16; ++d )
       w[d] = st[d];
    const bool interior = ( flag & 1 ) != 0;
+   // multi-colour SOR: point (x, y, z) has the colour (sb + s1 x + s2 y + s3 z) mod sorM
+   int s1 = 0, s2 = 0, s3 = 0, sb = 0;
+   if constexpr ( MODE == MODE_SOR )
+   {
+      const CellInfo& ci = cinfo[c];
+      s1                 = ( ci.colour[1] - ci.colour[0] + sorM ) % sorM;
+      s2                 = ( ci.colour[2] - ci.colour[0] + sorM ) % sorM;
+      s3                 = ( ci.colour[3] - ci.colour[0] + sorM ) % sorM;
+      sb                 = ( ci.colour[0] * ( N % sorM ) ) % sorM;
+   }
 
    // x = 0 face (slot 2): the first point of lane 0 in the windows starting at a row start, with that
    // face's one-sided weights. Its E-group terms are formed on lane 1 (P = 1, then shifted) or on
@@ -257,7 +268,10 @@ __device__ __forceinline__ void cellZ( int bid, const ZSeg* __restrict__ segs, i
          {
             T o0, o1;
             epilogue< T, MODE >( ax[k], Bp.r[r + 1][k], Bp.e[r][k], Bp.e[r][k], w[15], a, o0, o1 );
-            const unsigned so = sd[k] <= lastD - r - z ? unsigned( ( d0 + k ) * B ) : kDropOffset;
+            bool mine = true;
+            if constexpr ( MODE == MODE_SOR )
+               mine = ( sb + s1 * ( x0 - 1 + d0 + k ) + s2 * ( y0 + r ) + s3 * z ) % sorM == sorK;
+            const unsigned so = mine && sd[k] <= lastD - r - z ? unsigned( ( d0 + k ) * B ) : kDropOffset;
             bufStore( o0, r0, so, o * B );
             if constexpr ( kTwo )
                bufStore( o1, r1, so, o * B );
@@ -396,7 +410,8 @@ __device__ __forceinline__ void lower( int bid, const LowerChunk* __restrict__ c
                                                     const LowerPrim* __restrict__ prims, const LowerEntry* __restrict__ entries,
                                                     const T* __restrict__ entryW, const T* __restrict__ primInv, int L,
                                                     i64 stride, int flag, const T* __restrict__ x, const T* __restrict__ in1,
-                                                    T* __restrict__ out0, T* __restrict__ out1, T a )
+                                                    T* __restrict__ out0, T* __restrict__ out1, T a, const CellInfo* __restrict__ cinfo,
+                                                    int sorK, int sorM )
 {
    const int nb = ( nChunks + 3 ) / 4;
    const int ci = remapXcd( bid, nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
@@ -407,9 +422,11 @@ __device__ __forceinline__ void lower( int bid, const LowerChunk* __restrict__ c
    const LowerPrim  p    = prims[ch.prim];
    if ( !( p.bc & flag ) )
       return;
-   const bool on = lane < ch.n;
-   const int  N = 1 << L, w = N + 1;
-   const int  i = ch.i0 + ( on ? lane : 0 ), j = ch.j;
+   bool      on = lane < ch.n;
+   const int N = 1 << L, w = N + 1;
+   const int i = ch.i0 + ( on ? lane : 0 ), j = ch.j;
+   if constexpr ( MODE == MODE_SOR )
+      on = on && sorColour( cinfo, entries[p.entry0], N, i, j, sorM ) == sorK;
    T          Ax = T( 0 );
    for ( int k = 0; k < p.nCells; ++k )
    {
@@ -438,9 +455,9 @@ __device__ __forceinline__ void lower( int bid, const LowerChunk* __restrict__ c
    entryPos( e0, N, i, j, qx, qy, qz );
    const i64 o0i = i64( e0.cell ) * stride + cidx( w, qx, qy, qz );
    T         xc = T( 0 ), i1 = T( 0 ), ov = T( 0 ), r0, r1;
-   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 )
+   if constexpr ( MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_SOR )
       xc = x[o0i];
-   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+   if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK || MODE == MODE_SOR )
       i1 = in1[o0i];
    if constexpr ( MODE == MODE_APPLY_ADD )
       ov = out0[o0i];
@@ -484,6 +501,8 @@ struct SweepArgs
    const LowerPrim*    prims;
    const LowerEntry*   entries;
    int                 gC, gF;
+   const CellInfo*     cinfo = nullptr; // multi-colour SOR: cell colours, the colour of this pass and their number
+   int                 sorK = 0, sorM = 1;
 };
 
 template < typename T, int MODE, int R, int P, int D >
@@ -492,12 +511,12 @@ __global__ __launch_bounds__( kBlock ) void kSweep( SweepArgs< T > p )
    const int b = int( blockIdx.x );
    if ( b < p.gC )
       cellZ< T, MODE, R, P, D >( b, p.segs, p.nSegs, p.nC, p.L, p.stride, p.cellSt, p.cfs, p.entryW, p.part, p.flag, p.x, p.in1,
-                              p.out0, p.out1, p.a );
+                              p.out0, p.out1, p.a, p.cinfo, p.sorK, p.sorM );
    else if ( b < p.gC + p.gF )
       faceFlat< T >( b - p.gC, p.items, p.nItems, p.L, p.stride, p.cfs, p.entryW, p.part, p.flag, p.x );
    else
       lower< T, MODE >( b - p.gC - p.gF, p.chunks, p.nChunks, p.prims, p.entries, p.entryW, p.primInv, p.L, p.stride, p.flag,
-                        p.x, p.in1, p.out0, p.out1, p.a );
+                        p.x, p.in1, p.out0, p.out1, p.a, p.cinfo, p.sorK, p.sorM );
 }
 
 // ------------------------------------------------------------------ launch
@@ -578,6 +597,69 @@ static void stencilT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol,
       hipLaunchKernelGGL( ( kFaceCombine< T, MODE > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
                           g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, m.mesh->nV() + m.mesh->nE(), part, pi,
                           (const T*) nullptr, L, g.stride, c.flag, x, in1, out0, out1, T( c.a ) );
+}
+
+// Multi-colour SOR sweep (forward: colours 0 .. nColours-1, backward: the reverse order). Each colour pass is the
+// Jacobi update restricted to the points of that colour, written in place: the sweep launch (cell interiors,
+// one-sided face sums, macro-edges and -vertices) and the face combine. Points of one colour are not coupled, so
+// a pass reads only values it does not write (the centre value aside, which each point reads before its own
+// store); outputs of other colours are computed and dropped.
+template < typename T >
+static void sorT( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, void* xv, const void* bv, double relax, int flag,
+                  bool backwards, hipStream_t s )
+{
+   constexpr bool dbl = std::is_same< T, double >::value;
+   const int      cfg = 1; // 2 rows per wave
+   const int      nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
+   T*             part = dbl ? (T*) ol.facePart : (T*) ol.facePartF;
+   const T*       pi   = dbl ? (const T*) ol.primInv : (const T*) ol.primInvF;
+   SweepArgs< T > p;
+   p.segs    = g.zSegs[cfg];
+   p.nSegs   = g.nZSegs[cfg];
+   p.nC      = m.mesh->nC();
+   p.L       = L;
+   p.flag    = flag;
+   p.stride  = g.stride;
+   p.cellSt  = dbl ? (const T*) ol.cellSt : (const T*) ol.cellStF;
+   p.cfs     = m.cellFaces;
+   p.entryW  = dbl ? (const T*) ol.entryW : (const T*) ol.entryWF;
+   p.primInv = pi;
+   p.part    = part;
+   p.x       = (const T*) xv;
+   p.in1     = (const T*) bv;
+   p.out0    = (T*) xv;
+   p.out1    = (T*) xv;
+   p.a       = T( relax );
+   p.items   = g.flatItems;
+   p.nItems  = nFace > 0 ? g.nFlatItems : 0;
+   p.chunks  = g.chunks;
+   p.nChunks = nLow;
+   p.prims   = m.prims;
+   p.entries = m.entries;
+   p.gC      = p.nSegs > 0 ? xcdGrid( ( ( p.nSegs + 3 ) / 4 ) * p.nC ) : 0;
+   p.gF      = p.nItems > 0 ? xcdGrid( ceilDiv( p.nItems, 4 ) ) : 0;
+   p.cinfo   = m.cellInfo;
+   p.sorM    = m.nColours;
+   const int gL = nLow > 0 ? xcdGrid( ceilDiv( nLow, kBlock / 64 ) ) : 0;
+   for ( int q = 0; q < m.nColours; ++q )
+   {
+      p.sorK = backwards ? m.nColours - 1 - q : q;
+      if ( p.gC + p.gF + gL > 0 )
+         hipLaunchKernelGGL( ( kSweep< T, MODE_SOR, 2, 1, 2 > ), dim3( p.gC + p.gF + gL ), dim3( kBlock ), 0, s, p );
+      if ( nFace > 0 )
+         hipLaunchKernelGGL( ( kFaceCombineSor< T > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
+                             g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, m.mesh->nV() + m.mesh->nE(), part, pi,
+                             m.cellInfo, p.sorK, p.sorM, L, g.stride, flag, (T*) xv, (const T*) bv, T( relax ) );
+   }
+}
+
+void launchSor( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, int dtype, void* x, const void* b, double relax,
+                int flag, bool backwards, hipStream_t s )
+{
+   if ( dtype == 8 )
+      sorT< double >( m, g, ol, L, x, b, relax, flag, backwards, s );
+   else
+      sorT< float >( m, g, ol, L, x, b, relax, flag, backwards, s );
 }
 
 template < typename T >

@@ -2,6 +2,7 @@
 // code written against HyTeG's operator/solver API reads the same:
 //   Operator::apply(src, dst, level, flag, updateType)          operators/Operator.hpp:73-80
 //   WeightedJacobiSmoothable::smooth_jac(dst, rhs, tmp, relax, level, flag)   solvers/Smoothables.hpp:32-43
+//   SORSmoothable::smooth_sor(dst, rhs, relax, level, flag)      solvers/Smoothables.hpp:72-98
 //   Solver<Op>::solve(A, x, b, level)                            solvers/Solver.hpp:26-36
 //   ProlongationOperator::prolongate / prolongateAndAdd          gridtransferoperators/ProlongationOperator.hpp:30-39
 //   RestrictionOperator::restrict                                gridtransferoperators/RestrictionOperator.hpp:29-35
@@ -198,7 +199,7 @@ class MatrixFreeOperator
    {
       h_.reset( op, []( hyg_op p ) { hyg_op_destroy( p ); } );
    }
-
+   void* stream() const { return stream_; }
 
  private:
    std::shared_ptr< PrimitiveStorage > storage_;
@@ -215,6 +216,21 @@ class P1ConstantLaplaceOperator : public MatrixFreeOperator< P1Function< ValueTy
    P1ConstantLaplaceOperator( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )
    : MatrixFreeOperator< P1Function< ValueType > >( storage, minLevel, maxLevel, make( storage, minLevel, maxLevel ) )
    {}
+   // SORSmoothable::smooth_sor / smooth_sor_backwards, smooth_gs (P1Operator.hpp:325-423), multi-colour order
+   void smooth_sor( const P1Function< ValueType >& dst, const P1Function< ValueType >& rhs, double relax, int level, DoFType flag,
+                    bool backwards = false ) const
+   {
+      check( hyg_smooth_sor( this->handle(), dst.handle(), rhs.handle(), relax, level, flag, backwards ? 1 : 0, this->stream() ) );
+   }
+   void smooth_sor_backwards( const P1Function< ValueType >& dst, const P1Function< ValueType >& rhs, double relax, int level,
+                              DoFType flag ) const
+   {
+      smooth_sor( dst, rhs, relax, level, flag, true );
+   }
+   void smooth_gs( const P1Function< ValueType >& dst, const P1Function< ValueType >& rhs, int level, DoFType flag ) const
+   {
+      smooth_sor( dst, rhs, 1.0, level, flag, false );
+   }
 
  private:
    static hyg_op make( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel )

@@ -167,6 +167,18 @@ int hyg_apply_vector( hyg_op op, int n, const hyg_func* srcs, const hyg_func* ds
  * P1Operator.hpp:433-451): dst = src + relax * D^-1 (rhs - A src), fused into one pass. */
 int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double relax, int level, int flag,
                     void* stream );
+/* SORSmoothable::smooth_sor / SORBackwardsSmoothable::smooth_sor_backwards (solvers/Smoothables.hpp:72-98;
+ * P1Operator.hpp:343-423): one SOR sweep on dst in place, x_p += relax D^-1 (rhs - A x)_p. The reference
+ * visits vertices, edges, faces and cells in that order, each lexicographically; here the points are visited
+ * by colour (multi-colour Gauss-Seidel): the macro-vertices get a greedy colouring (vertex order, smallest
+ * colour free among macro-edge neighbours; hyg_mesh_sor_colours), a lattice point with barycentric lattice
+ * coordinates b_i over the vertices v_i of its cell gets the colour (sum_i c(v_i) b_i) mod ncolours, and
+ * points of one colour, which the stencil never couples, are updated together; backwards = 1 takes the
+ * colours in reverse order. Constant P1 operator only. */
+int hyg_smooth_sor( hyg_op op, hyg_func dst, hyg_func rhs, double relax, int level, int flag, int backwards, void* stream );
+/* The SOR colouring of a mesh: the number of colours and, if vertex_colours is not null, the colour of every
+ * macro-vertex (nV entries). No counterpart in the reference (its SOR is lexicographic). */
+int hyg_mesh_sor_colours( hyg_mesh mesh, int* ncolours, int* vertex_colours );
 /* OperatorWithInverseDiagonal::getInverseDiagonalValues (Smoothables.hpp:99-106) written into `dst`. */
 int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream );
 /* residual r = b - A x on `flag` (GeometricMultigridSolver.hpp:185-186 apply + assign), fused. */
@@ -199,8 +211,9 @@ int hyg_cheb_coefficients( int order, double lower, double upper, double* out );
 int hyg_power_iteration( hyg_op op, hyg_func x, hyg_func tmp, int iterations, int level, double* radius, void* stream );
 
 /* GeometricMultigridSolver (solvers/GeometricMultigridSolver.hpp:42-235) with a WeightedJacobiSmoother
- * (smoother = 0, solvers/WeightedJacobiSmoother.hpp:42-62) or ChebyshevSmoother (smoother = 1,
- * ChebyshevSmoother.hpp:48-97) and a CGSolver coarse solver (solvers/CGSolver.hpp:66-134).
+ * (smoother = 0, solvers/WeightedJacobiSmoother.hpp:42-62), ChebyshevSmoother (smoother = 1,
+ * ChebyshevSmoother.hpp:48-97) or SORSmoother (smoother = 2, solvers/SORSmoother.hpp:38-50, forward multi-colour
+ * sweeps as hyg_smooth_sor, constant P1 operator) and a CGSolver coarse solver (solvers/CGSolver.hpp:66-134).
  * iparams = {min_level, max_level, pre, post, increment, smoother, cheb_order, cg_max_iter}
  * dparams = {relax, cheb_lower, cheb_upper, cg_tolerance}
  * The solver works on functions of the operator's value type (HYG_F64 or HYG_F32). */

@@ -2261,6 +2261,15 @@ void or_mesh_info( void* mh, int64_t* out )
    out[6]  = std::count( m->fB.begin(), m->fB.end(), 1 );
 }
 
+// macro-cell vertices (4 per cell, the cell's local vertex order)
+void or_mesh_cells( void* mh, int* out )
+{
+   auto* m = static_cast< Mesh* >( mh );
+   for ( size_t c = 0; c < m->Cl.size(); ++c )
+      for ( int k = 0; k < 4; ++k )
+         out[4 * c + k] = m->Cl[c][k];
+}
+
 int64_t or_ndofs( void* mh, int L ) { return numDofs( *static_cast< Mesh* >( mh ), L ); }
 
 // coordinates (3 per DoF) and boundary marker (1 = Dirichlet) in canonical order

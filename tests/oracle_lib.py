@@ -42,6 +42,7 @@ def lib():
             "or_apply": (None, [vp, vp, vp, i, i, i]),
             "or_inv_diag": (None, [vp, i, P(np.float64)]),
             "or_smooth_jac": (None, [vp, vp, vp, vp, d, i, i]),
+            "or_mesh_cells": (None, [vp, P(np.int32)]),
             "or_restrict": (None, [vp, i, i]),
             "or_prolongate": (None, [vp, i, i, i]),
             "or_dot": (d, [vp, vp, i, i]),
@@ -96,6 +97,13 @@ class Mesh:
 
     def ndofs(self, L):
         return lib().or_ndofs(self.h, L)
+
+    def cells(self):
+        """Macro-cell vertex indices, (nC, 4), each cell in its local vertex order."""
+        nc = int(self.info()[3])
+        out = np.zeros(4 * nc, dtype=np.int32)
+        lib().or_mesh_cells(self.h, out)
+        return out.reshape(nc, 4)
 
     def volume(self):
         return lib().or_mesh_volume(self.h)
