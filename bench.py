@@ -430,6 +430,20 @@ def main():
     ms_copy = timed(hip, st, copy_once, 20) / 20
     copy_gbs = 16.0 * nval / (ms_copy / 1e3) / 1e9
 
+    # one multi-colour SOR sweep (relax 1, in place) for comparison with the fused Jacobi sweep
+    ncol = mesh.sor_colours()[0]
+
+    def sor_once():
+        hyg.check(lib.hyg_smooth_sor(op.h, x.h, y.h, 1.0, L, hyg.INNER, 0, st))
+
+    def jac_once():
+        hyg.check(lib.hyg_smooth_jac(op.h, y.h, x.h, x.h, 2.0 / 3.0, L, hyg.INNER, st))
+
+    sor_once()
+    ms_sor = timed(hip, st, sor_once, 5) / 5
+    jac_once()
+    ms_jac = timed(hip, st, jac_once, 10) / 10
+
     out = {
         "metric": "GDoF/s matrix-free apply + time-to-1e-10 V-cycle solve, mixed vs FP64, 1 GPU",
         "value": round(gdofs, 3),
@@ -452,6 +466,9 @@ def main():
         "copy_reference": {"bytes_per_launch": 16.0 * nval, "ms": round(ms_copy, 5), "GB/s": round(copy_gbs, 1)},
         "cell_kernel_config": {"apply": "rows %d x points %d" % op.config(L, hyg.F64, 0)[1:],
                                "tuned_per_mode": tuned},
+        "smoothers_f64": {"jacobi_sweep_ms": round(ms_jac, 4),
+                          "sor_sweep_ms": round(ms_sor, 4), "sor_colours": ncol,
+                          "note": "multi-colour SOR: one sweep-kernel pass plus a face pass per colour"},
     }
 
     # ---------------- time to 1e-10: FP64 vs mixed precision
