@@ -295,11 +295,14 @@ int smoothImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, hipStream_t s
       c.a    = g->relax;
       return stencilImpl( g->op, L, g->dtype, c, st );
    }
-   if ( g->smoother == 2 )
+   if ( g->smoother >= 2 )
    {
-      // SORSmoother::solve: smooth_sor(x, b, relax, level, flag), multi-colour order
+      // SORSmoother::solve: smooth_sor(x, b, relax, level, flag), multi-colour order; SymmetricSORSmoother::solve
+      // follows it with smooth_sor_backwards
       MeshCtx& mc = g->op->mesh->ctx;
       launchSor( mc, mc.geom( L ), g->op->levels[L], L, g->dtype, x->data[L], b->data[L], g->relax, kGmgFlag, false, st );
+      if ( g->smoother == 3 )
+         launchSor( mc, mc.geom( L ), g->op->levels[L], L, g->dtype, x->data[L], b->data[L], g->relax, kGmgFlag, true, st );
       return launched( "sor" );
    }
    // ChebyshevSmoother::solve, fused: t0 = D^-1 (b - A x), y = x + c0 t0; t_k = D^-1 A t_{k-1}, y += c_k t_k; x = y
@@ -1411,9 +1414,9 @@ int hyg_gmg_create( hyg_op op, int dtype, const int* ip, const double* dp, hyg_g
       return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
    if ( lat( op, ip[0] ) < op->minL || lat( op, ip[1] ) > op->maxL || ip[0] > ip[1] )
       return fail( HYG_ERR_ARG, "solver levels outside the operator's range" );
-   if ( ip[5] < 0 || ip[5] > 2 )
-      return fail( HYG_ERR_ARG, "smoother must be 0 (weighted Jacobi), 1 (Chebyshev) or 2 (SOR)" );
-   if ( ip[5] == 2 && op->kind != 0 )
+   if ( ip[5] < 0 || ip[5] > 3 )
+      return fail( HYG_ERR_ARG, "smoother must be 0 (weighted Jacobi), 1 (Chebyshev), 2 (SOR) or 3 (symmetric SOR)" );
+   if ( ip[5] >= 2 && op->kind != 0 )
       return fail( HYG_ERR_ARG, "the SOR smoother is implemented for the constant P1 operator" );
    if ( ip[5] == 1 && ( ip[6] < 1 || ip[6] > 5 ) )
       return fail( HYG_ERR_ARG, "Chebyshev order must be 1..5" );

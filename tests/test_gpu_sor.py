@@ -133,12 +133,14 @@ def test_gmg_with_sor_smoother_beats_jacobi():
     b = hyg.Function(gm, 0, L)
     b.upload(L, bv)
     rate = {}
-    for sm_, relax in (("sor", 1.0), ("jacobi", 2.0 / 3.0)):
+    for sm_, relax in (("sor", 1.0), ("ssor", 1.0), ("jacobi", 2.0 / 3.0)):
         x = hyg.Function(gm, 0, L)
         res = hyg.GMG(op, 0, L, pre=2, post=2, smoother=sm_, relax=relax).solve(x, b, L, 8)
         rate[sm_] = (res[-1] / res[-5]) ** 0.25
     print("asymptotic rates", rate)
     assert rate["sor"] < 0.35 and rate["sor"] < rate["jacobi"], rate
+    # a symmetric sweep does a forward and a backward sweep: at least as good as the forward one
+    assert rate["ssor"] <= rate["sor"], rate
 
 
 def test_gmg_gauss_seidel_octahedron_reference_bound():
