@@ -38,6 +38,7 @@ def _sig(lib):
         "hyg_mesh_create_from_msh": (i, [C.c_char_p, pvp]),
         "hyg_mesh_create_spherical_shell": (i, [i, i, d, d, pvp]),
         "hyg_mesh_create_symmetric_cuboid": (i, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i, pvp]),
+        "hyg_mesh_create_cuboid": (i, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i, pvp]),
         "hyg_mesh_destroy": (i, [vp]),
         "hyg_mesh_info": (i, [vp, P(np.int64)]),
         "hyg_mesh_num_dofs": (i, [vp, i, C.POINTER(C.c_int64)]),
@@ -128,8 +129,8 @@ def _out():
 
 
 class Mesh:
-    """A Gmsh v2.2 file (name or path), the spherical shell ("shell", ntan, nrad, rmin, rmax) or the symmetric
-    cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz)."""
+    """A Gmsh v2.2 file (name or path), the spherical shell ("shell", ntan, nrad, rmin, rmax), the symmetric
+    cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz) or the six-tet cuboid ("cuboid", same arguments)."""
 
     def __init__(self, name, *args):
         init()
@@ -138,11 +139,12 @@ class Mesh:
             self.path = None
             ntan, nrad, rmin, rmax = args
             check(load().hyg_mesh_create_spherical_shell(ntan, nrad, rmin, rmax, C.byref(h)))
-        elif name == "symmetric_cuboid":
+        elif name in ("symmetric_cuboid", "cuboid"):
             self.path = None
             lo, hi, nx, ny, nz = args
             lo3, hi3 = (C.c_double * 3)(*lo), (C.c_double * 3)(*hi)
-            check(load().hyg_mesh_create_symmetric_cuboid(lo3, hi3, nx, ny, nz, C.byref(h)))
+            fn = load().hyg_mesh_create_symmetric_cuboid if name == "symmetric_cuboid" else load().hyg_mesh_create_cuboid
+            check(fn(lo3, hi3, nx, ny, nz, C.byref(h)))
         else:
             self.path = name if os.path.isabs(name) else os.path.join(MESHES, name)
             check(load().hyg_mesh_create_from_msh(self.path.encode(), C.byref(h)))

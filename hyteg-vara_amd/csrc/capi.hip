@@ -579,6 +579,30 @@ int hyg_mesh_create_symmetric_cuboid( const double lower[3], const double upper[
    return HYG_OK;
 }
 
+int hyg_mesh_create_cuboid( const double lower[3], const double upper[3], int nx, int ny, int nz, hyg_mesh* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !out || !lower || !upper )
+      return fail( HYG_ERR_ARG, "null argument" );
+   std::string err;
+   auto        m = meshCuboid( lower, upper, nx, ny, nz, err );
+   if ( !m )
+      return fail( HYG_ERR_ARG, err );
+   auto* h = new hyg_mesh_s;
+   h->ctx.mesh   = std::move( m );
+   h->ctx.device = g_device;
+   try
+   {
+      h->ctx.buildTopology();
+   } catch ( const std::exception& e )
+   {
+      delete h;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   *out = h;
+   return HYG_OK;
+}
+
 int hyg_mesh_destroy( hyg_mesh mesh )
 {
    delete mesh;

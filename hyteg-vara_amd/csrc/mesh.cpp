@@ -333,6 +333,41 @@ std::unique_ptr< Mesh > meshSphericalShell( int ntan, int nrad, double rmin, dou
    return buildMesh( nodes, cellKeys, "spherical shell", err );
 }
 
+// MeshInfo::meshCuboid( lowerLeftFront, upperRightBack, nx, ny, nz ) (mesh/MeshGenCuboid.cpp:35-174): the
+// (nx+1)(ny+1)(nz+1) grid points, numbered x fastest, then y, then z, each cube of the grid split into the same six
+// tetrahedra (the reference's tNode table over the cube corners 0..7 = (0,0,0) (1,0,0) (1,1,0) (0,1,0) and the
+// same four one layer up), cell vertices in that table's order.
+std::unique_ptr< Mesh > meshCuboid( const double lo[3], const double hi[3], int nx, int ny, int nz, std::string& err )
+{
+   if ( nx < 1 || ny < 1 || nz < 1 )
+   {
+      err = "cuboid needs at least one cube in every direction";
+      return nullptr;
+   }
+   static const int corner[8][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 },
+                                     { 0, 0, 1 }, { 1, 0, 1 }, { 1, 1, 1 }, { 0, 1, 1 } };
+   static const int split[6][4]  = { { 4, 5, 7, 3 }, { 0, 3, 1, 4 }, { 4, 1, 5, 3 }, { 5, 6, 7, 2 }, { 1, 3, 2, 5 }, { 7, 2, 3, 5 } };
+   const double     h[3] = { ( hi[0] - lo[0] ) / double( nx ), ( hi[1] - lo[1] ) / double( ny ), ( hi[2] - lo[2] ) / double( nz ) };
+   auto             gid  = [&]( int i, int j, int k ) { return ( long( k ) * ( ny + 1 ) + j ) * ( nx + 1 ) + i; };
+   std::map< long, std::array< double, 3 > > nodes;
+   for ( int k = 0; k <= nz; ++k )
+      for ( int j = 0; j <= ny; ++j )
+         for ( int i = 0; i <= nx; ++i )
+            nodes[gid( i, j, k )] = { lo[0] + double( i ) * h[0], lo[1] + double( j ) * h[1], lo[2] + double( k ) * h[2] };
+   std::map< std::vector< long >, int > cellKeys;
+   for ( int k = 0; k < nz; ++k )
+      for ( int j = 0; j < ny; ++j )
+         for ( int i = 0; i < nx; ++i )
+            for ( const auto& t : split )
+            {
+               std::vector< long > key;
+               for ( int c : t )
+                  key.push_back( gid( i + corner[c][0], j + corner[c][1], k + corner[c][2] ) );
+               cellKeys[key] = 0;
+            }
+   return buildMesh( nodes, cellKeys, "cuboid", err );
+}
+
 // MeshInfo::meshSymmetricCuboid( lowerLeftFront, upperRightBack, nx, ny, nz ) (mesh/MeshGenSymmetricCuboid.cpp:35-253):
 // nx x ny x nz sub-cubes of 24 tetrahedra around the centres of their six faces. Sub-cubes are visited z-major,
 // then y, then x; a sub-cube's 14 vertices (8 corners, 6 face centres) take the ID of the same point in an

@@ -69,6 +69,9 @@ int hyg_mesh_create_spherical_shell( int ntan, int nrad, double rmin, double rma
  * face centres), point-symmetric about the centre, followed by the same storage setup and boundary flags
  * as above (the whole surface is Dirichlet). */
 int hyg_mesh_create_symmetric_cuboid( const double lower[3], const double upper[3], int nx, int ny, int nz, hyg_mesh* out );
+/* MeshInfo::meshCuboid(lowerLeftFront, upperRightBack, nx, ny, nz) (mesh/MeshGenCuboid.cpp:35-174), six tetrahedra
+ * per grid cube, followed by the same storage setup and boundary flags as hyg_mesh_create_from_msh. */
+int hyg_mesh_create_cuboid( const double lower[3], const double upper[3], int nx, int ny, int nz, hyg_mesh* out );
 int hyg_mesh_destroy( hyg_mesh mesh );
 /* out[0..6] = #vertices, #edges, #faces, #cells, #boundary vertices, #boundary edges, #boundary faces */
 int hyg_mesh_info( hyg_mesh mesh, int64_t* out );

@@ -2228,6 +2228,39 @@ extern "C" {
 
 void* or_mesh_load( const char* path ) { return loadGmsh( path ).release(); }
 void* or_mesh_shell( int ntan, int nrad, double rmin, double rmax ) { return makeShell( ntan, nrad, rmin, rmax ).release(); }
+// MeshInfo::meshCuboid (mesh/MeshGenCuboid.cpp:35-174): grid points numbered x fastest; every grid cube split into
+// six tetrahedra given by the reference's corner table (corners counter-clockwise in the bottom layer, then the
+// top layer)
+void* or_mesh_cuboid( const double* lo, const double* hi, int nx, int ny, int nz )
+{
+   if ( nx < 1 || ny < 1 || nz < 1 )
+      return nullptr;
+   const int                      tNode[6][4] = { { 4, 5, 7, 3 }, { 0, 3, 1, 4 }, { 4, 1, 5, 3 }, { 5, 6, 7, 2 }, { 1, 3, 2, 5 }, { 7, 2, 3, 5 } };
+   const int                      dx[8] = { 0, 1, 1, 0, 0, 1, 1, 0 }, dy[8] = { 0, 0, 1, 1, 0, 0, 1, 1 }, dz[8] = { 0, 0, 0, 0, 1, 1, 1, 1 };
+   const idx                      sx = 1, sy = nx + 1, sz = idx( nx + 1 ) * ( ny + 1 );
+   std::map< idx, P3 >            nodes;
+   std::set< std::vector< idx > > cells;
+   for ( idx v = 0; v < sz * ( nz + 1 ); ++v )
+   {
+      const int i = int( v % sy ), j = int( ( v / sy ) % ( ny + 1 ) ), k = int( v / sz );
+      nodes[v]    = { lo[0] + i * ( ( hi[0] - lo[0] ) / nx ), lo[1] + j * ( ( hi[1] - lo[1] ) / ny ), lo[2] + k * ( ( hi[2] - lo[2] ) / nz ) };
+   }
+   for ( int k = 0; k < nz; ++k )
+      for ( int j = 0; j < ny; ++j )
+         for ( int i = 0; i < nx; ++i )
+            for ( int t = 0; t < 6; ++t )
+            {
+               std::vector< idx > cv( 4 );
+               for ( int q = 0; q < 4; ++q )
+               {
+                  const int c = tNode[t][q];
+                  cv[q]       = ( i + dx[c] ) * sx + ( j + dy[c] ) * sy + ( k + dz[c] ) * sz;
+               }
+               cells.insert( cv );
+            }
+   return buildMesh( nodes, cells ).release();
+}
+
 void* or_mesh_symmetric_cuboid( const double* lo, const double* hi, int nx, int ny, int nz )
 {
    return ( nx < 1 || ny < 1 || nz < 1 ) ? nullptr : makeSymmetricCuboid( lo, hi, nx, ny, nz ).release();

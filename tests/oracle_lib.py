@@ -43,6 +43,7 @@ def lib():
             "or_inv_diag": (None, [vp, i, P(np.float64)]),
             "or_smooth_jac": (None, [vp, vp, vp, vp, d, i, i]),
             "or_mesh_cells": (None, [vp, P(np.int32)]),
+            "or_mesh_cuboid": (vp, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i]),
             "or_restrict": (None, [vp, i, i]),
             "or_prolongate": (None, [vp, i, i, i]),
             "or_dot": (d, [vp, vp, i, i]),
@@ -67,8 +68,8 @@ def lib():
 
 
 class Mesh:
-    """A Gmsh v2.2 file (name or path), the spherical shell ("shell", ntan, nrad, rmin, rmax) or the symmetric
-    cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz)."""
+    """A Gmsh v2.2 file (name or path), the spherical shell ("shell", ntan, nrad, rmin, rmax), the symmetric
+    cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz) or the six-tet cuboid ("cuboid", same arguments)."""
 
     def __init__(self, name, *args):
         if name == "shell":
@@ -77,10 +78,11 @@ class Mesh:
             if not self.h:
                 raise RuntimeError("bad spherical shell parameters")
             return
-        if name == "symmetric_cuboid":
+        if name in ("symmetric_cuboid", "cuboid"):
             self.path = None
             lo, hi, nx, ny, nz = args
-            self.h = lib().or_mesh_symmetric_cuboid((C.c_double * 3)(*lo), (C.c_double * 3)(*hi), nx, ny, nz)
+            fn = lib().or_mesh_symmetric_cuboid if name == "symmetric_cuboid" else lib().or_mesh_cuboid
+            self.h = fn((C.c_double * 3)(*lo), (C.c_double * 3)(*hi), nx, ny, nz)
             if not self.h:
                 raise RuntimeError("bad symmetric cuboid parameters")
             return

@@ -1,14 +1,17 @@
-"""GPU side of MeshInfo::meshSymmetricCuboid (mesh/MeshGenSymmetricCuboid.cpp:35-253): the product's generator
-(vertex IDs from a map of sub-cube points) against the oracle's restatement of the reference's per-sub-cube vertex
-map: identical primitive counts, DoF coordinates (bit for bit) and boundary flags; then the P1 apply within 1e-13
-relative l-inf and a V-cycle residual history within 1e-10 per cycle on that mesh."""
+"""GPU side of MeshInfo::meshSymmetricCuboid (mesh/MeshGenSymmetricCuboid.cpp:35-253) and MeshInfo::meshCuboid
+(mesh/MeshGenCuboid.cpp:35-174): the product's generators against the oracle's restatements (written
+independently): identical primitive counts, DoF coordinates (bit for bit) and boundary flags; then the P1 apply
+within 1e-13 relative l-inf and a V-cycle residual history within 1e-10 per cycle on those meshes."""
 import numpy as np
 import pytest
 
 import oracle_lib as ol
 
 pytestmark = pytest.mark.gpu
-CASES = [((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1, 1, 1), ((-1.0, 0.5, 2.0), (2.0, 1.5, 2.75), 3, 1, 2)]
+CASES = [("symmetric_cuboid", ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1, 1, 1)),
+         ("symmetric_cuboid", ((-1.0, 0.5, 2.0), (2.0, 1.5, 2.75), 3, 1, 2)),
+         ("cuboid", ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1, 1, 1)),
+         ("cuboid", ((-1.0, 0.5, 2.0), (2.0, 1.5, 2.75), 3, 2, 2))]
 
 
 def rel_inf(a, b):
@@ -19,7 +22,7 @@ def rel_inf(a, b):
 @pytest.fixture(scope="module")
 def meshes(hyg):
     hyg.init()
-    return [(hyg.Mesh("symmetric_cuboid", *c), ol.Mesh("symmetric_cuboid", *c)) for c in CASES]
+    return [(hyg.Mesh(kind, *c), ol.Mesh(kind, *c)) for kind, c in CASES]
 
 
 @pytest.mark.parametrize("k", range(len(CASES)))
@@ -50,8 +53,9 @@ def test_p1_apply(hyg, meshes, k):
         assert rel_inf(d.download(L), od.get(L)) <= 1e-13, flag
 
 
-def test_vcycle_history(hyg, meshes):
-    gm, om = meshes[1]
+@pytest.mark.parametrize("k", [1, 3])
+def test_vcycle_history(hyg, meshes, k):
+    gm, om = meshes[k]
     minL, maxL, cycles = 0, 4, 5
     n = gm.ndofs(maxL)
     _, bnd = gm.coords(maxL)

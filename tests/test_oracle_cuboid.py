@@ -51,3 +51,37 @@ def test_single_cube_vertex_numbering():
     want = [(0, 0, 0), (2, 0, 0), (1, 1, 0), (0, 2, 0), (2, 2, 0), (1, 0, 1), (0, 1, 1), (2, 1, 1), (1, 2, 1),
             (0, 0, 2), (2, 0, 2), (1, 1, 2), (0, 2, 2), (2, 2, 2)]
     assert np.array_equal(xyz[:14], np.array(want, dtype=float))
+
+
+# MeshInfo::meshCuboid (mesh/MeshGenCuboid.cpp:35-174): six tetrahedra per grid cube, the grid points as vertices
+CASES6 = [((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1, 1, 1), ((-1.0, 0.5, 2.0), (2.0, 1.5, 2.75), 3, 2, 2)]
+
+
+@pytest.mark.parametrize("case", CASES6)
+def test_cuboid_counts_euler_volume(case):
+    lo, hi, nx, ny, nz = case
+    m = ol.Mesh("cuboid", lo, hi, nx, ny, nz)
+    nv, ne, nf, nc, bv, be, bf = [int(v) for v in m.info()]
+    assert nv == (nx + 1) * (ny + 1) * (nz + 1)
+    assert nc == 6 * nx * ny * nz
+    assert nv - ne + nf - nc == 1
+    # every boundary square of the grid is split into two triangles; the surface is a sphere
+    assert bf == 2 * 2 * (nx * ny + ny * nz + nx * nz)
+    assert bv - be + bf == 2
+    vol = np.prod(np.array(hi) - np.array(lo))
+    assert abs(m.volume() - vol) <= 1e-13 * vol
+
+
+@pytest.mark.parametrize("case", CASES6)
+def test_cuboid_operator_kills_linears(case):
+    # P1LaplaceOperator3DTest.cpp:40-150 on this mesh: A applied to a linear function vanishes on inner DoFs
+    m = ol.Mesh("cuboid", *case)
+    L = 3
+    xyz, bnd = m.coords(L)
+    u = 0.3 + 1.7 * xyz[:, 0] - 0.4 * xyz[:, 1] + 2.2 * xyz[:, 2]
+    op = ol.Operator(m, L, L)
+    s, d = ol.Function(m, L, L), ol.Function(m, L, L)
+    s.set(L, u)
+    op.apply(s, d, L, ol.INNER, False)
+    r = d.get(L)
+    assert np.abs(r[~bnd]).max() <= 1e-11 * np.abs(u).max()
