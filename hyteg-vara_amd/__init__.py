@@ -39,6 +39,7 @@ def _sig(lib):
         "hyg_mesh_create_spherical_shell": (i, [i, i, d, d, pvp]),
         "hyg_mesh_create_symmetric_cuboid": (i, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i, pvp]),
         "hyg_mesh_create_cuboid": (i, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i, pvp]),
+        "hyg_mesh_create_cubed_domain": (i, [P(np.int32), i, i, pvp]),
         "hyg_mesh_destroy": (i, [vp]),
         "hyg_mesh_info": (i, [vp, P(np.int64)]),
         "hyg_mesh_num_dofs": (i, [vp, i, C.POINTER(C.c_int64)]),
@@ -130,7 +131,8 @@ def _out():
 
 class Mesh:
     """A Gmsh v2.2 file (name or path), the spherical shell ("shell", ntan, nrad, rmin, rmax), the symmetric
-    cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz) or the six-tet cuboid ("cuboid", same arguments)."""
+    cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz) the six-tet cuboid ("cuboid", same arguments) or a cubed domain ("cubed_domain", integer cube positions,
+    cube type 0 or 1)."""
 
     def __init__(self, name, *args):
         init()
@@ -145,6 +147,11 @@ class Mesh:
             lo3, hi3 = (C.c_double * 3)(*lo), (C.c_double * 3)(*hi)
             fn = load().hyg_mesh_create_symmetric_cuboid if name == "symmetric_cuboid" else load().hyg_mesh_create_cuboid
             check(fn(lo3, hi3, nx, ny, nz, C.byref(h)))
+        elif name == "cubed_domain":
+            self.path = None
+            cubes, cube_type = args
+            cb = np.ascontiguousarray(np.asarray(cubes, dtype=np.int32).reshape(-1))
+            check(load().hyg_mesh_create_cubed_domain(cb, len(cb) // 3, cube_type, C.byref(h)))
         else:
             self.path = name if os.path.isabs(name) else os.path.join(MESHES, name)
             check(load().hyg_mesh_create_from_msh(self.path.encode(), C.byref(h)))

@@ -603,6 +603,30 @@ int hyg_mesh_create_cuboid( const double lower[3], const double upper[3], int nx
    return HYG_OK;
 }
 
+int hyg_mesh_create_cubed_domain( const int* cubes, int ncubes, int cube_type, hyg_mesh* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !out || !cubes )
+      return fail( HYG_ERR_ARG, "null argument" );
+   std::string err;
+   auto        m = meshCubedDomain( cubes, ncubes, cube_type, err );
+   if ( !m )
+      return fail( HYG_ERR_ARG, err );
+   auto* h = new hyg_mesh_s;
+   h->ctx.mesh   = std::move( m );
+   h->ctx.device = g_device;
+   try
+   {
+      h->ctx.buildTopology();
+   } catch ( const std::exception& e )
+   {
+      delete h;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   *out = h;
+   return HYG_OK;
+}
+
 int hyg_mesh_destroy( hyg_mesh mesh )
 {
    delete mesh;

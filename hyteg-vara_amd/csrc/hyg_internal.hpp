@@ -472,6 +472,7 @@ std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err );
 std::unique_ptr< Mesh > meshSphericalShell( int ntan, int nrad, double rmin, double rmax, std::string& err );
 std::unique_ptr< Mesh > meshSymmetricCuboid( const double lo[3], const double hi[3], int nx, int ny, int nz, std::string& err );
 std::unique_ptr< Mesh > meshCuboid( const double lo[3], const double hi[3], int nx, int ny, int nz, std::string& err );
+std::unique_ptr< Mesh > meshCubedDomain( const int* cubes, int nCubes, int cubeType, std::string& err );
 
 // element matrices of the six micro-tetrahedron types of a cell with vertices cv at level L, and the
 // one-sided 15-point stencil of that cell at lattice point (qx, qy, qz) (w[0..14])

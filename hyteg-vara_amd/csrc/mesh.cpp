@@ -368,6 +368,80 @@ std::unique_ptr< Mesh > meshCuboid( const double lo[3], const double hi[3], int 
    return buildMesh( nodes, cellKeys, "cuboid", err );
 }
 
+// MeshInfo::meshCubedDomain( cubeCoordinates, cubeType ) (mesh/MeshGenCubedDomain.cpp:37-383): unit cubes at the
+// given integer positions, each split into 6 tetrahedra (type 0, the cuboid's corner table) or 24 around its face
+// centres (type 1, the symmetric cuboid's table). Vertex IDs follow the reference's numbering over the bounding
+// box of the cubes: the corner grid x fastest, then (type 1) the centres of the xy-, xz- and yz-faces, each family
+// over its own grid.
+std::unique_ptr< Mesh > meshCubedDomain( const int* cubes, int nCubes, int cubeType, std::string& err )
+{
+   if ( nCubes < 1 || ( cubeType != 0 && cubeType != 1 ) )
+   {
+      err = "cubed domain needs at least one cube and cube type 0 or 1";
+      return nullptr;
+   }
+   int lo[3] = { cubes[0], cubes[1], cubes[2] }, hi[3] = { cubes[0], cubes[1], cubes[2] };
+   for ( int q = 0; q < nCubes; ++q )
+      for ( int c = 0; c < 3; ++c )
+      {
+         lo[c] = std::min( lo[c], cubes[3 * q + c] );
+         hi[c] = std::max( hi[c], cubes[3 * q + c] );
+      }
+   const long NX = hi[0] + 1 - lo[0], NY = hi[1] + 1 - lo[1], NZ = hi[2] + 1 - lo[2]; // cubes per direction
+   const long nCorner = ( NX + 1 ) * ( NY + 1 ) * ( NZ + 1 ), nXY = NX * NY * ( NZ + 1 ), nXZ = NX * NZ * ( NY + 1 );
+   // ID of a point given in half-cube units (h = 2 x for a grid coordinate x)
+   auto idOfHalf = [&]( int hx, int hy, int hz ) -> long {
+      const long x = hx >> 1, y = hy >> 1, z = hz >> 1; // lower grid coordinate
+      const long i = x - lo[0], j = y - lo[1], k = z - lo[2];
+      const int  odd = ( hx & 1 ) | ( ( hy & 1 ) << 1 ) | ( ( hz & 1 ) << 2 );
+      switch ( odd )
+      {
+      case 0: return ( k * ( NY + 1 ) + j ) * ( NX + 1 ) + i;                       // corner
+      case 3: return nCorner + ( k * NY + j ) * NX + i;                             // xy-face centre
+      case 5: return nCorner + nXY + ( k * ( NY + 1 ) + j ) * NX + i;                // xz-face centre
+      default: return nCorner + nXY + nXZ + ( k * NY + j ) * ( NX + 1 ) + i;        // yz-face centre (odd == 6)
+      }
+   };
+   static const int corner[8][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 },
+                                     { 0, 0, 1 }, { 1, 0, 1 }, { 1, 1, 1 }, { 0, 1, 1 } };
+   static const int split6[6][4] = { { 4, 5, 7, 3 }, { 0, 3, 1, 4 }, { 4, 1, 5, 3 }, { 5, 6, 7, 2 }, { 1, 3, 2, 5 }, { 7, 2, 3, 5 } };
+   static const int lp[14][3]    = { { 0, 0, 0 }, { 2, 0, 0 }, { 1, 1, 0 }, { 0, 2, 0 }, { 2, 2, 0 }, { 1, 0, 1 }, { 0, 1, 1 },
+                                     { 2, 1, 1 }, { 1, 2, 1 }, { 0, 0, 2 }, { 2, 0, 2 }, { 1, 1, 2 }, { 0, 2, 2 }, { 2, 2, 2 } };
+   static const int split24[24][4] = { { 0, 1, 2, 5 },    { 0, 3, 2, 6 },    { 1, 2, 4, 7 },    { 2, 4, 3, 8 },   { 2, 0, 5, 6 },
+                                       { 2, 1, 5, 7 },    { 2, 3, 6, 8 },    { 2, 4, 7, 8 },    { 0, 9, 5, 6 },   { 1, 10, 5, 7 },
+                                       { 3, 12, 6, 8 },   { 4, 13, 7, 8 },   { 2, 5, 6, 8 },    { 2, 5, 7, 8 },   { 11, 5, 6, 8 },
+                                       { 11, 5, 7, 8 },   { 11, 9, 5, 6 },   { 11, 10, 5, 7 },  { 11, 12, 6, 8 }, { 11, 13, 7, 8 },
+                                       { 11, 9, 10, 5 },  { 11, 9, 12, 6 },  { 11, 10, 13, 7 }, { 11, 12, 13, 8 } };
+   std::map< long, std::array< double, 3 > > nodes;
+   std::map< std::vector< long >, int >      cellKeys;
+   for ( int q = 0; q < nCubes; ++q )
+   {
+      const int x = cubes[3 * q], y = cubes[3 * q + 1], z = cubes[3 * q + 2];
+      auto      vert = [&]( int hx, int hy, int hz ) {
+         const long id = idOfHalf( 2 * x + hx, 2 * y + hy, 2 * z + hz );
+         nodes[id]     = { double( x ) + 0.5 * hx, double( y ) + 0.5 * hy, double( z ) + 0.5 * hz };
+         return id;
+      };
+      if ( cubeType == 0 )
+         for ( const auto& t : split6 )
+         {
+            std::vector< long > key;
+            for ( int c : t )
+               key.push_back( vert( 2 * corner[c][0], 2 * corner[c][1], 2 * corner[c][2] ) );
+            cellKeys[key] = 0;
+         }
+      else
+         for ( const auto& t : split24 )
+         {
+            std::vector< long > key;
+            for ( int c : t )
+               key.push_back( vert( lp[c][0], lp[c][1], lp[c][2] ) );
+            cellKeys[key] = 0;
+         }
+   }
+   return buildMesh( nodes, cellKeys, "cubed domain", err );
+}
+
 // MeshInfo::meshSymmetricCuboid( lowerLeftFront, upperRightBack, nx, ny, nz ) (mesh/MeshGenSymmetricCuboid.cpp:35-253):
 // nx x ny x nz sub-cubes of 24 tetrahedra around the centres of their six faces. Sub-cubes are visited z-major,
 // then y, then x; a sub-cube's 14 vertices (8 corners, 6 face centres) take the ID of the same point in an

@@ -72,6 +72,10 @@ int hyg_mesh_create_symmetric_cuboid( const double lower[3], const double upper[
 /* MeshInfo::meshCuboid(lowerLeftFront, upperRightBack, nx, ny, nz) (mesh/MeshGenCuboid.cpp:35-174), six tetrahedra
  * per grid cube, followed by the same storage setup and boundary flags as hyg_mesh_create_from_msh. */
 int hyg_mesh_create_cuboid( const double lower[3], const double upper[3], int nx, int ny, int nz, hyg_mesh* out );
+/* MeshInfo::meshCubedDomain(cubeCoordinates, cubeType) (mesh/MeshGenCubedDomain.cpp:37-383): unit cubes at the
+ * integer positions cubes[3 i .. 3 i + 2] (i < ncubes), split into 6 (cube_type 0) or 24 (cube_type 1) tetrahedra,
+ * followed by the same storage setup and boundary flags as hyg_mesh_create_from_msh. */
+int hyg_mesh_create_cubed_domain( const int* cubes, int ncubes, int cube_type, hyg_mesh* out );
 int hyg_mesh_destroy( hyg_mesh mesh );
 /* out[0..6] = #vertices, #edges, #faces, #cells, #boundary vertices, #boundary edges, #boundary faces */
 int hyg_mesh_info( hyg_mesh mesh, int64_t* out );

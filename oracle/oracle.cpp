@@ -48,6 +48,7 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <climits>
 #include <map>
 #include <memory>
 #include <set>
@@ -2258,6 +2259,87 @@ void* or_mesh_cuboid( const double* lo, const double* hi, int nx, int ny, int nz
                }
                cells.insert( cv );
             }
+   return buildMesh( nodes, cells ).release();
+}
+
+// MeshInfo::meshCubedDomain (mesh/MeshGenCubedDomain.cpp:37-383): unit cubes at integer positions; type 0 splits a
+// cube into six tets with the cuboid's corner table, type 1 into 24 around its face centres (local vertices
+// 0..13 as in the symmetric cuboid). Vertex IDs per local vertex (the reference's symmCubeID / indexInCube).
+void* or_mesh_cubed_domain( const int* cubes, int n, int type )
+{
+   if ( n < 1 || type < 0 || type > 1 )
+      return nullptr;
+   std::set< std::array< int, 3 > > cs;
+   for ( int q = 0; q < n; ++q )
+      cs.insert( { cubes[3 * q], cubes[3 * q + 1], cubes[3 * q + 2] } );
+   int xmin = INT_MAX, ymin = INT_MAX, zmin = INT_MAX, xmax = INT_MIN, ymax = INT_MIN, zmax = INT_MIN;
+   for ( const auto& c : cs )
+   {
+      xmin = std::min( xmin, c[0] ), ymin = std::min( ymin, c[1] ), zmin = std::min( zmin, c[2] );
+      xmax = std::max( xmax, c[0] + 1 ), ymax = std::max( ymax, c[1] + 1 ), zmax = std::max( zmax, c[2] + 1 );
+   }
+   const idx mx = xmax - xmin, my = ymax - ymin, mz = zmax - zmin;
+   auto      inCube = [&]( int x, int y, int z, idx nx, idx ny ) { return idx( z - zmin ) * nx * ny + idx( y - ymin ) * nx + idx( x - xmin ); };
+   const idx nCorner = ( mx + 1 ) * ( my + 1 ) * ( mz + 1 ), nXY = mx * my * ( mz + 1 ), nXZ = mx * mz * ( my + 1 );
+   auto      symmId = [&]( int x, int y, int z, int lv ) -> idx {
+      switch ( lv )
+      {
+      case 0: return inCube( x, y, z, mx + 1, my + 1 );
+      case 1: return inCube( x + 1, y, z, mx + 1, my + 1 );
+      case 3: return inCube( x, y + 1, z, mx + 1, my + 1 );
+      case 4: return inCube( x + 1, y + 1, z, mx + 1, my + 1 );
+      case 9: return inCube( x, y, z + 1, mx + 1, my + 1 );
+      case 10: return inCube( x + 1, y, z + 1, mx + 1, my + 1 );
+      case 12: return inCube( x, y + 1, z + 1, mx + 1, my + 1 );
+      case 13: return inCube( x + 1, y + 1, z + 1, mx + 1, my + 1 );
+      case 2: return nCorner + inCube( x, y, z, mx, my );
+      case 11: return nCorner + inCube( x, y, z + 1, mx, my );
+      case 5: return nCorner + nXY + inCube( x, y, z, mx, my + 1 );
+      case 8: return nCorner + nXY + inCube( x, y + 1, z, mx, my + 1 );
+      case 6: return nCorner + nXY + nXZ + inCube( x, y, z, mx + 1, my );
+      default: return nCorner + nXY + nXZ + inCube( x + 1, y, z, mx + 1, my ); // 7
+      }
+   };
+   const P3  local[14] = { { 0, 0, 0 },       { 1, 0, 0 }, { 0.5, 0.5, 0 }, { 0, 1, 0 }, { 1, 1, 0 },     { 0.5, 0, 0.5 }, { 0, 0.5, 0.5 },
+                           { 1, 0.5, 0.5 },   { 0.5, 1, 0.5 }, { 0, 0, 1 }, { 1, 0, 1 }, { 0.5, 0.5, 1 }, { 0, 1, 1 },     { 1, 1, 1 } };
+   const int cellV[24][4] = { { 0, 1, 2, 5 },   { 0, 3, 2, 6 },   { 1, 2, 4, 7 },    { 2, 4, 3, 8 },    { 2, 0, 5, 6 },   { 2, 1, 5, 7 },
+                              { 2, 3, 6, 8 },   { 2, 4, 7, 8 },   { 0, 9, 5, 6 },    { 1, 10, 5, 7 },   { 3, 12, 6, 8 },  { 4, 13, 7, 8 },
+                              { 2, 5, 6, 8 },   { 2, 5, 7, 8 },   { 11, 5, 6, 8 },   { 11, 5, 7, 8 },   { 11, 9, 5, 6 },  { 11, 10, 5, 7 },
+                              { 11, 12, 6, 8 }, { 11, 13, 7, 8 }, { 11, 9, 10, 5 },  { 11, 9, 12, 6 },  { 11, 10, 13, 7 }, { 11, 12, 13, 8 } };
+   const int tNode[6][4] = { { 4, 5, 7, 3 }, { 0, 3, 1, 4 }, { 4, 1, 5, 3 }, { 5, 6, 7, 2 }, { 1, 3, 2, 5 }, { 7, 2, 3, 5 } };
+   const int dx[8] = { 0, 1, 1, 0, 0, 1, 1, 0 }, dy[8] = { 0, 0, 1, 1, 0, 0, 1, 1 }, dz[8] = { 0, 0, 0, 0, 1, 1, 1, 1 };
+   std::map< idx, P3 >            nodes;
+   std::set< std::vector< idx > > cells;
+   for ( const auto& c : cs )
+   {
+      if ( type == 0 )
+      {
+         for ( int a = 0; a <= 1; ++a )
+            for ( int b = 0; b <= 1; ++b )
+               for ( int d = 0; d <= 1; ++d )
+                  nodes.emplace( inCube( c[0] + a, c[1] + b, c[2] + d, mx + 1, my + 1 ), P3{ double( c[0] + a ), double( c[1] + b ), double( c[2] + d ) } );
+         for ( int t = 0; t < 6; ++t )
+         {
+            std::vector< idx > cv;
+            for ( int q = 0; q < 4; ++q )
+               cv.push_back( inCube( c[0] + dx[tNode[t][q]], c[1] + dy[tNode[t][q]], c[2] + dz[tNode[t][q]], mx + 1, my + 1 ) );
+            cells.insert( cv );
+         }
+      }
+      else
+      {
+         for ( int lv = 0; lv < 14; ++lv )
+            nodes.emplace( symmId( c[0], c[1], c[2], lv ),
+                           P3{ double( c[0] ) + local[lv][0], double( c[1] ) + local[lv][1], double( c[2] ) + local[lv][2] } );
+         for ( int t = 0; t < 24; ++t )
+         {
+            std::vector< idx > cv;
+            for ( int q = 0; q < 4; ++q )
+               cv.push_back( symmId( c[0], c[1], c[2], cellV[t][q] ) );
+            cells.insert( cv );
+         }
+      }
+   }
    return buildMesh( nodes, cells ).release();
 }
 

@@ -44,6 +44,7 @@ def lib():
             "or_smooth_jac": (None, [vp, vp, vp, vp, d, i, i]),
             "or_mesh_cells": (None, [vp, P(np.int32)]),
             "or_mesh_cuboid": (vp, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i]),
+            "or_mesh_cubed_domain": (vp, [P(np.int32), i, i]),
             "or_restrict": (None, [vp, i, i]),
             "or_prolongate": (None, [vp, i, i, i]),
             "or_dot": (d, [vp, vp, i, i]),
@@ -77,6 +78,14 @@ class Mesh:
             self.h = lib().or_mesh_shell(*args)
             if not self.h:
                 raise RuntimeError("bad spherical shell parameters")
+            return
+        if name == "cubed_domain":
+            self.path = None
+            cubes, cube_type = args
+            cb = np.ascontiguousarray(np.asarray(cubes, dtype=np.int32).reshape(-1))
+            self.h = lib().or_mesh_cubed_domain(cb, len(cb) // 3, cube_type)
+            if not self.h:
+                raise RuntimeError("bad cubed domain parameters")
             return
         if name in ("symmetric_cuboid", "cuboid"):
             self.path = None

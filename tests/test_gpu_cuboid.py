@@ -1,5 +1,5 @@
-"""GPU side of MeshInfo::meshSymmetricCuboid (mesh/MeshGenSymmetricCuboid.cpp:35-253) and MeshInfo::meshCuboid
-(mesh/MeshGenCuboid.cpp:35-174): the product's generators against the oracle's restatements (written
+"""GPU side of MeshInfo::meshSymmetricCuboid (mesh/MeshGenSymmetricCuboid.cpp:35-253), MeshInfo::meshCuboid
+(mesh/MeshGenCuboid.cpp:35-174) and MeshInfo::meshCubedDomain (mesh/MeshGenCubedDomain.cpp:37-383): the product's generators against the oracle's restatements (written
 independently): identical primitive counts, DoF coordinates (bit for bit) and boundary flags; then the P1 apply
 within 1e-13 relative l-inf and a V-cycle residual history within 1e-10 per cycle on those meshes."""
 import numpy as np
@@ -11,7 +11,9 @@ pytestmark = pytest.mark.gpu
 CASES = [("symmetric_cuboid", ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1, 1, 1)),
          ("symmetric_cuboid", ((-1.0, 0.5, 2.0), (2.0, 1.5, 2.75), 3, 1, 2)),
          ("cuboid", ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1, 1, 1)),
-         ("cuboid", ((-1.0, 0.5, 2.0), (2.0, 1.5, 2.75), 3, 2, 2))]
+         ("cuboid", ((-1.0, 0.5, 2.0), (2.0, 1.5, 2.75), 3, 2, 2)),
+         ("cubed_domain", ([(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1)], 0)),
+         ("cubed_domain", ([(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1)], 1))]
 
 
 def rel_inf(a, b):
@@ -53,7 +55,7 @@ def test_p1_apply(hyg, meshes, k):
         assert rel_inf(d.download(L), od.get(L)) <= 1e-13, flag
 
 
-@pytest.mark.parametrize("k", [1, 3])
+@pytest.mark.parametrize("k", [1, 3, 4, 5])
 def test_vcycle_history(hyg, meshes, k):
     gm, om = meshes[k]
     minL, maxL, cycles = 0, 4, 5

@@ -85,3 +85,39 @@ def test_cuboid_operator_kills_linears(case):
     op.apply(s, d, L, ol.INNER, False)
     r = d.get(L)
     assert np.abs(r[~bnd]).max() <= 1e-11 * np.abs(u).max()
+
+
+# MeshInfo::meshCubedDomain (mesh/MeshGenCubedDomain.cpp:37-383): an L-shaped block of unit cubes and a column
+LSHAPE = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1)]
+COLUMN = [(-1, 2, 0), (-1, 2, 1), (-1, 2, 2)]
+
+
+@pytest.mark.parametrize("cubes", [LSHAPE, COLUMN])
+@pytest.mark.parametrize("cube_type", [0, 1])
+def test_cubed_domain_counts_euler_volume(cubes, cube_type):
+    m = ol.Mesh("cubed_domain", cubes, cube_type)
+    nv, ne, nf, nc, bv, be, bf = [int(v) for v in m.info()]
+    corners = len({(x + a, y + b, z + c) for x, y, z in cubes for a in (0, 1) for b in (0, 1) for c in (0, 1)})
+    if cube_type == 0:
+        assert nv == corners
+    else:
+        faces = {(2 * x + a, 2 * y + b, 2 * z + c) for x, y, z in cubes
+                 for a, b, c in ((1, 1, 0), (1, 1, 2), (1, 0, 1), (1, 2, 1), (0, 1, 1), (2, 1, 1))}
+        assert nv == corners + len(faces)
+    assert nc == (6 if cube_type == 0 else 24) * len(cubes)
+    assert nv - ne + nf - nc == 1
+    assert bv - be + bf == 2
+    assert abs(m.volume() - len(cubes)) <= 1e-13 * len(cubes)
+
+
+@pytest.mark.parametrize("cube_type", [0, 1])
+def test_cubed_domain_operator_kills_linears(cube_type):
+    m = ol.Mesh("cubed_domain", LSHAPE, cube_type)
+    L = 3
+    xyz, bnd = m.coords(L)
+    u = -0.7 + 0.9 * xyz[:, 0] + 1.3 * xyz[:, 1] - 2.1 * xyz[:, 2]
+    op = ol.Operator(m, L, L)
+    s, d = ol.Function(m, L, L), ol.Function(m, L, L)
+    s.set(L, u)
+    op.apply(s, d, L, ol.INNER, False)
+    assert np.abs(d.get(L)[~bnd]).max() <= 1e-11 * np.abs(u).max()
