@@ -116,6 +116,17 @@ int checkSpace( const hyg_op_s* op, const hyg_func_s* f )
    return HYG_OK;
 }
 
+// an operator and a function it is applied to: same mesh (the kernels walk the operator's cells over the
+// function's arrays) and same space
+int checkOpFunc( const hyg_op_s* op, const hyg_func_s* f )
+{
+   if ( !op || !f )
+      return fail( HYG_ERR_ARG, "null operator or function handle" );
+   if ( f->mesh != op->mesh )
+      return fail( HYG_ERR_ARG, "function and operator live on different meshes" );
+   return checkSpace( op, f );
+}
+
 i64 levelValues( const hyg_func_s* f, int L ) { return i64( f->mesh->ctx.mesh->nC() ) * cellStride( L ); }
 
 int createFunc( hyg_mesh_s* mesh, int minL, int maxL, int dtype, hyg_func_s** out, int space = 0 )
@@ -123,6 +134,8 @@ int createFunc( hyg_mesh_s* mesh, int minL, int maxL, int dtype, hyg_func_s** ou
    auto* f  = new ( std::nothrow ) hyg_func_s;
    if ( !f )
       return fail( HYG_ERR_ARG, "out of host memory" );
+   static uint64_t nextGen = 0;
+   f->gen   = ++nextGen;
    f->lat   = space;
    f->mesh  = mesh;
    f->minL  = minL;
@@ -260,7 +273,7 @@ struct hyg_gmg_s
    double*               cgPartials   = nullptr;
    hipGraph_t            graph        = nullptr;
    hipGraphExec_t        graphExec    = nullptr;
-   hyg_func_s*           graphX = nullptr, *graphB = nullptr;
+   uint64_t              graphX = 0, graphB = 0; // creation numbers of the captured x and b
    int                   graphLevel   = -1;
    ~hyg_gmg_s()
    {
@@ -1250,7 +1263,7 @@ int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double 
    level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    HYG_TRY( checkPair( src, dst, level ) );
-   HYG_TRY( checkSpace( op, src ) );
+   HYG_TRY( checkOpFunc( op, src ) );
    HYG_TRY( checkPair( src, rhs, level ) );
    if ( src == dst || rhs == dst )
       return fail( HYG_ERR_ARG, "smooth_jac requires dst different from src and rhs" );
@@ -1269,7 +1282,7 @@ int hyg_smooth_sor( hyg_op op, hyg_func dst, hyg_func rhs, double relax, int lev
    level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    HYG_TRY( checkPair( dst, rhs, level ) );
-   HYG_TRY( checkSpace( op, dst ) );
+   HYG_TRY( checkOpFunc( op, dst ) );
    if ( op->kind != 0 )
       return fail( HYG_ERR_ARG, "smooth_sor is implemented for the constant P1 operator" );
    if ( dst == rhs )
@@ -1298,7 +1311,7 @@ int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream )
    level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    HYG_TRY( checkLevel( dst, level ) );
-   HYG_TRY( checkSpace( op, dst ) );
+   HYG_TRY( checkOpFunc( op, dst ) );
    MeshCtx& mc = op->mesh->ctx;
    if ( op->kind >= 1 )
    {
@@ -1319,7 +1332,7 @@ int hyg_residual( hyg_op op, hyg_func r, hyg_func b, hyg_func x, int level, int 
 {
    level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
-   HYG_TRY( checkSpace( op, x ) );
+   HYG_TRY( checkOpFunc( op, x ) );
    HYG_TRY( checkPair( r, b, level ) );
    HYG_TRY( checkPair( r, x, level ) );
    if ( r == x )
@@ -1422,7 +1435,7 @@ int hyg_power_iteration( hyg_op op, hyg_func x, hyg_func tmp, int iterations, in
    level          = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
    HYG_TRY( checkPair( x, tmp, level ) );
-   HYG_TRY( checkSpace( op, x ) );
+   HYG_TRY( checkOpFunc( op, x ) );
    if ( x == tmp || !radius || iterations < 1 )
       return fail( HYG_ERR_ARG, "bad argument" );
    hipStream_t st = S( stream );
@@ -1518,10 +1531,10 @@ int hyg_gmg_vcycle_graph( hyg_gmg g, hyg_func x, hyg_func b, int level, int cg_i
    if ( !g )
       return fail( HYG_ERR_ARG, "null solver handle" );
    level = lat( g->op, level );
-   HYG_TRY( checkSpace( g->op, x ) );
-   HYG_TRY( checkSpace( g->op, b ) );
-   HYG_TRY( checkLevel( x, level ) );
-   HYG_TRY( checkLevel( b, level ) );
+   HYG_TRY( checkPair( x, b, level ) );
+   HYG_TRY( checkOpFunc( g->op, x ) );
+   if ( x == b )
+      return fail( HYG_ERR_ARG, "x and b must differ" );
    if ( !stream )
       return fail( HYG_ERR_ARG, "capturing a V-cycle needs a stream other than the default stream" );
    if ( level < g->minL || level > g->maxL || x->dtype != g->dtype || b->dtype != g->dtype || x->minL > g->minL ||
@@ -1530,7 +1543,9 @@ int hyg_gmg_vcycle_graph( hyg_gmg g, hyg_func x, hyg_func b, int level, int cg_i
    if ( cg_iterations < 1 )
       return fail( HYG_ERR_ARG, "cg_iterations must be positive" );
    hipStream_t st = S( stream );
-   if ( !g->graphExec || g->graphX != x || g->graphB != b || g->graphLevel != level || g->cgGraphIters != cg_iterations )
+   // the cache key uses the creation numbers of x and b, never their addresses: a destroyed function's handle
+   // address can be reused by a new function whose device arrays differ
+   if ( !g->graphExec || g->graphX != x->gen || g->graphB != b->gen || g->graphLevel != level || g->cgGraphIters != cg_iterations )
    {
       if ( g->graphExec )
          (void) hipGraphExecDestroy( g->graphExec );
@@ -1564,8 +1579,8 @@ int hyg_gmg_vcycle_graph( hyg_gmg g, hyg_func x, hyg_func b, int level, int cg_i
       }
       g->graph      = graph;
       g->graphExec  = exec;
-      g->graphX     = x;
-      g->graphB     = b;
+      g->graphX     = x->gen;
+      g->graphB     = b->gen;
       g->graphLevel = level;
    }
    HYG_HIP( hipGraphLaunch( g->graphExec, st ) );
@@ -1615,9 +1630,11 @@ int hyg_ir_solve( hyg_op op64, hyg_gmg g32, hyg_func x, hyg_func b, int level, i
    if ( !g32 || g32->dtype != HYG_F32 || g32->maxL != level )
       return fail( HYG_ERR_ARG, "defect correction needs an FP32 solver whose max level is the solve level" );
    HYG_TRY( checkPair( x, b, level ) );
-   HYG_TRY( checkSpace( op64, x ) );
+   HYG_TRY( checkOpFunc( op64, x ) );
    if ( g32->op->lat != op64->lat )
       return fail( HYG_ERR_ARG, "P1 and P2 operators and solvers cannot be combined" );
+   if ( g32->op->mesh != op64->mesh )
+      return fail( HYG_ERR_ARG, "the FP64 operator and the FP32 solver live on different meshes" );
    if ( x->dtype != HYG_F64 )
       return fail( HYG_ERR_ARG, "defect correction works on FP64 x and b" );
    if ( !res || !steps || inner < 1 || max_outer < 0 )

@@ -549,6 +549,7 @@ struct hyg_func_s
    hyg_mesh_s*           mesh;
    int                   minL, maxL, dtype; // lattice levels (a P2 function of level L holds lattice L+1)
    int                   lat = 0;           // 0: P1 function, 1: P2 function
+   uint64_t              gen = 0;           // creation number, unique over the process (cache keys)
    std::vector< void* >  data; // per level (nullptr below minL)
    size_t                elem() const { return dtype == 8 ? 8 : 4; }
 };

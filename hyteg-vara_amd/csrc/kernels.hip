@@ -276,7 +276,9 @@ __global__ __launch_bounds__( kBlock ) void kProlongate( const PlaneTile* __rest
    for ( int u = 0; u < kPtsPerThread; ++u )
    {
       slotAt( c, tile, 1 << Lf, strideF, tile.s0 + int( threadIdx.x ) + kBlock * u, sp[u] );
-      on[u]        = sp[u].ok && ( int( ( bw >> ( 4 * sp[u].slot ) ) & 15u ) & flag );
+      // cell-interior points are prolongated whatever the flag (the reference's cell loop has no flag test,
+      // P1toP1LinearProlongation.cpp:208-260); points of macro-faces, -edges and -vertices follow the flag
+      on[u]        = sp[u].ok && ( sp[u].slot == 0 || ( int( ( bw >> ( 4 * sp[u].slot ) ) & 15u ) & flag ) );
       const int pc = ( sp[u].x & 1 ) | ( ( sp[u].y & 1 ) << 1 ) | ( ( sp[u].z & 1 ) << 2 );
       const int ex = EX[pc], ey = EY[pc], ez = EZ[pc];
       va[u] = on[u] ? cs[cidx( wc, ( sp[u].x - ex ) >> 1, ( sp[u].y - ey ) >> 1, ( sp[u].z - ez ) >> 1 )] : T( 0 );
@@ -494,7 +496,9 @@ static void restrictT( const MeshCtx& m, const LevelGeom& gc, int Lc, void* coar
 {
    const int nC      = m.mesh->nC();
    const i64 strideF = cellStride( Lc + 1 );
-   if ( ( flag & 1 ) && gc.nInteriorTiles > 0 )
+   // cell-interior coarse points are restricted whatever the flag (the reference's cell loop has no flag test,
+   // P1toP1LinearRestriction.cpp:181-260)
+   if ( gc.nInteriorTiles > 0 )
    {
       const int nb = gc.nInteriorTiles * nC;
       hipLaunchKernelGGL( ( kRestrictCell< T > ), dim3( xcdGrid( nb ) ), dim3( kBlock ), 0, st, gc.tiles, gc.interiorTile0,

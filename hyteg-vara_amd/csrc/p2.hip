@@ -672,8 +672,10 @@ template < typename T >
 __global__ __launch_bounds__( kBlock ) void kP2ProlongRows( int Lf, int nC, i64 strideF, i64 strideC, int flag, T* __restrict__ fine,
                                                             const T* __restrict__ coarse, int add )
 {
+   // cell-interior points are prolongated whatever the flag (the reference's cell loop has no flag test,
+   // P2toP2QuadraticProlongation.cpp:217-424)
    const int Nf = 1 << Lf, W = Nf - 3; // fine rows: a triangle with W rows
-   if ( !( flag & 1 ) || W <= 0 )
+   if ( W <= 0 )
       return;
    const int rows = W * ( W + 1 ) / 2, nbx = ( rows + 3 ) / 4, nb = nbx * nC;
    const int lb   = remapXcd( int( blockIdx.x ), nb );

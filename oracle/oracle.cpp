@@ -2536,6 +2536,17 @@ double or_p2k_value( int kind, double k0, double alpha, double beta, uint64_t se
    return kf( { x[0], x[1], x[2] }, eid );
 }
 
+// the 11-point rule of p2_div_k_grad_affine_q4: barycentric coordinates lam[4 q + v] and weights w[q]
+void or_p2_quadrature( double* lam, double* w )
+{
+   for ( int q = 0; q < 11; ++q )
+   {
+      for ( int v = 0; v < 4; ++v )
+         lam[4 * q + v] = XG4_LAMBDA[q][v];
+      w[q] = XG4_W[q];
+   }
+}
+
 // P2 div-k-grad element matrix (11-point rule) of the tetrahedron x (12 coordinates), coefficient P2KField kind 1
 // (polynomial) -> 100 values, row major
 void or_p2_divkgrad_element_poly( const double* x, double* out )

@@ -55,13 +55,14 @@ def _lattice(N):
 
 
 class RefinedMesh:
-    """Uniformly refined macro mesh at level L with globally numbered micro-vertices."""
+    """Uniformly refined macro mesh at level L with globally numbered micro-vertices. `path` is a Gmsh file or a
+    (nodes, tets) pair: nodes maps a vertex id to its coordinates, tets lists the cells' vertex ids in local order."""
 
     def __init__(self, path, L, scale_key=None):
-        nodes, tets = read_gmsh(path)
+        nodes, tets = read_gmsh(path) if isinstance(path, str) else path
         self.N = N = 1 << L
         self.L = L
-        keyscale = 4 * N if scale_key is None else scale_key
+        keyscale = 4 * N if scale_key in (None, "tree") else scale_key
         allkeys = []
         allxyz = []
         elems = []
@@ -84,7 +85,19 @@ class RefinedMesh:
                 elems.append(ids + off)
             off += len(lat)
         keys = np.concatenate(allkeys)
-        uniq, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+        if scale_key == "tree":
+            # coordinates that are not dyadic (curved meshes): identify micro-vertices that coincide up to
+            # rounding by a radius search instead of exact keys
+            from scipy.sparse.csgraph import connected_components
+            from scipy.spatial import cKDTree
+            pts = np.concatenate(allxyz)
+            pairs = cKDTree(pts).query_pairs(r=1e-10, output_type="ndarray")
+            adj = sp.coo_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(len(pts), len(pts)))
+            _, lab = connected_components(adj, directed=False)
+            _, first, inv = np.unique(lab, return_index=True, return_inverse=True)
+            uniq = keys[first]
+        else:
+            uniq, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
         inv = inv.reshape(-1)
         self.keys = uniq
         self.xyz = np.concatenate(allxyz)[first]  # coordinates of the first cell that produced the vertex
@@ -97,6 +110,15 @@ class RefinedMesh:
         uf, cnt = np.unique(faces, axis=0, return_counts=True)
         self.boundary = np.zeros(self.n, dtype=bool)
         self.boundary[uf[cnt == 1].reshape(-1)] = True
+
+    def index_of(self, xyz):
+        """Global indices of the given coordinates (nearest micro-vertex, which must coincide)."""
+        from scipy.spatial import cKDTree
+        if not hasattr(self, "_tree"):
+            self._tree = cKDTree(self.xyz)
+        d, i = self._tree.query(np.asarray(xyz).reshape(-1, 3))
+        assert d.max() < 1e-9
+        return i
 
     def key_index(self, xyz):
         """Global indices of the given coordinates."""
@@ -221,23 +243,63 @@ def p2_element_exact(X):
     return vol[:, None, None] * K
 
 
+def tet_gauss_rule(n):
+    """Collapsed-coordinate Gauss rule on the reference tetrahedron (n^3 points; the collapse multiplies the
+    integrand by (1 - a)^2 (1 - b), so polynomials of total degree 2n - 3 are integrated exactly):
+    barycentric coordinates (npts, 4) and weights summing to 1/6."""
+    g, w = np.polynomial.legendre.leggauss(n)
+    g, w = (g + 1) / 2, w / 2
+    lam, wts = [], []
+    for a, wa in zip(g, w):
+        for b, wb in zip(g, w):
+            for c, wc in zip(g, w):
+                p = (a, b * (1 - a), c * (1 - a) * (1 - b))
+                lam.append((1 - sum(p), p[0], p[1], p[2]))
+                wts.append(wa * wb * wc * (1 - a) ** 2 * (1 - b))
+    return np.array(lam), np.array(wts)
+
+
+def p2_element_varcoef(X, kfun, lam, wts):
+    """P2 matrices of -div(k grad) on tetrahedra X (ne, 4, 3) with the quadrature rule (lam, wts):
+    K_e = sum_q w_q |det J_e| k(x_q) grad phi(lam_q) grad phi(lam_q)^T, k evaluated at the physical points."""
+    J = X[:, 1:, :] - X[:, :1, :]
+    g = np.empty((len(X), 4, 3))
+    g[:, 1:, :] = np.linalg.inv(J).transpose(0, 2, 1)
+    g[:, 0, :] = -g[:, 1:, :].sum(axis=1)
+    adet = np.abs(np.linalg.det(J))
+    K = np.zeros((len(X), 10, 10))
+    for lq, wq in zip(lam, wts):
+        # gradients of the ten P2 basis functions at lq: vertices (4 l_v - 1) g_v, edges 4 (l_a g_b + l_b g_a)
+        d = [(4 * lq[v] - 1) * g[:, v, :] for v in range(4)]
+        d += [4 * (lq[a] * g[:, b, :] + lq[b] * g[:, a, :]) for a, b in P2_EDGES]
+        d = np.stack(d, axis=1)                                    # (ne, 10, 3)
+        xq = np.einsum("v,evc->ec", lq, X)                         # physical point of every element
+        K += (wq * adet * kfun(xq))[:, None, None] * np.einsum("eic,ejc->eij", d, d)
+    return K
+
+
 class P2Model:
     """P2 of level L on the lattice of level L+1 (global numbering by exact coordinates)."""
 
-    def __init__(self, path, L):
-        self.coarse = RefinedMesh(path, L)
-        self.lat = RefinedMesh(path, L + 1)
+    def __init__(self, path, L, scale_key=None):
+        self.coarse = RefinedMesh(path, L, scale_key)
+        self.lat = RefinedMesh(path, L + 1, scale_key)
         self.n = self.lat.n
         self.boundary = self.lat.boundary
         X = self.coarse.xyz[self.coarse.tets]
         nodes = np.concatenate([X, np.stack([(X[:, p] + X[:, q]) / 2 for p, q in P2_EDGES], axis=1)], axis=1)
         self.X = X
+        if scale_key == "tree":
+            self.nodes = self.lat.index_of(nodes).reshape(-1, 10)
+            return
         lookup = {tuple(r): i for i, r in enumerate(self.lat.keys)}
         k = np.rint(nodes.reshape(-1, 3) * self.lat.keyscale).astype(np.int64)
         self.nodes = np.array([lookup[tuple(r)] for r in k], dtype=np.int64).reshape(-1, 10)
 
-    def stiffness(self):
-        K = p2_element_exact(self.X)
+    def stiffness(self, kfun=None, rule=None):
+        """Global P2 matrix of -Laplace, or of -div(k grad) with kfun (points (n, 3) -> values) under `rule`
+        = (barycentric points, weights)."""
+        K = p2_element_exact(self.X) if kfun is None else p2_element_varcoef(self.X, kfun, *rule)
         rows = np.repeat(self.nodes, 10, axis=1).reshape(-1)
         cols = np.tile(self.nodes, (1, 10)).reshape(-1)
         return sp.csr_matrix((K.reshape(-1), (rows, cols)), shape=(self.n, self.n))

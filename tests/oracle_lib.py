@@ -55,6 +55,7 @@ def lib():
             "or_op_new_p2": (vp, [vp, i, i, i]),
             "or_op_new_p2_divkgrad": (vp, [vp, i, i, i, i, d, d, d, C.c_uint64]),
             "or_p2_divkgrad_element_poly": (None, [P(np.float64), P(np.float64)]),
+            "or_p2_quadrature": (None, [P(np.float64), P(np.float64)]),
             "or_p2k_value": (d, [i, d, d, d, C.c_uint64, P(np.float64), C.c_uint64]),
             "or_p2_element": (None, [P(np.float64), P(np.float64)]),
             "or_p2_restrict": (None, [vp, i, i]),
@@ -199,6 +200,13 @@ def p2_divkgrad_element_poly(x):
     out = np.zeros(100)
     lib().or_p2_divkgrad_element_poly(np.ascontiguousarray(x, dtype=np.float64).ravel(), out)
     return out.reshape(10, 10)
+
+
+def p2_quadrature():
+    """The oracle's 11-point rule: barycentric coordinates (11, 4) and weights (11,)."""
+    lam, w = np.zeros(44), np.zeros(11)
+    lib().or_p2_quadrature(lam, w)
+    return lam.reshape(11, 4), w
 
 
 def p2_element(x):

@@ -105,7 +105,8 @@ def test_transfer_parity(hyg, meshes, name, Lc):
     u, old, v = rnd(nc, 7), rnd(nf, 8), rnd(nf, 9)
     f = hyg.Function(gm, Lc, Lc + 1, space="p2")
     of = ol.Function(om, Lc + 1, Lc + 2)
-    for flag, upd in ((hyg.INNER | hyg.NEUMANN, hyg.REPLACE), (hyg.ALL, hyg.ADD), (hyg.INNER | hyg.NEUMANN, hyg.ADD)):
+    for flag, upd in ((hyg.INNER | hyg.NEUMANN, hyg.REPLACE), (hyg.ALL, hyg.ADD), (hyg.INNER | hyg.NEUMANN, hyg.ADD),
+                      (hyg.DIRICHLET, hyg.REPLACE), (hyg.DIRICHLET, hyg.ADD)):
         f.upload(Lc, u)
         f.upload(Lc + 1, old)
         of.set(Lc + 1, u)
@@ -113,7 +114,8 @@ def test_transfer_parity(hyg, meshes, name, Lc):
         f.prolongate(Lc, flag, upd)
         ol.p2_prolongate(of, Lc + 1, flag, upd == hyg.ADD)
         assert rel_inf(f.download(Lc + 1), of.get(Lc + 2)) <= TOL64, (flag, upd)
-    for flag in (hyg.ALL, hyg.INNER | hyg.NEUMANN):
+    # DIRICHLET alone: boundary points and (as in the reference, whose cell loop takes no flag) cell interiors
+    for flag in (hyg.ALL, hyg.INNER | hyg.NEUMANN, hyg.DIRICHLET):
         f.upload(Lc + 1, v)
         f.upload(Lc, u)
         of.set(Lc + 2, v)

@@ -111,11 +111,13 @@ def test_smooth_jac_and_residual_parity(hyg, meshes, name, L):
 
 @pytest.mark.parametrize("name", MESHES)
 @pytest.mark.parametrize("Lf", [1, 2, 3, 4, 5])
-@pytest.mark.parametrize("flag", ["inner", "all"])
+@pytest.mark.parametrize("flag", ["inner", "all", "dirichlet"])
 def test_transfer_parity(hyg, meshes, name, Lf, flag):
+    """With a flag that excludes Inner the cell-interior points are still transferred (the reference's cell
+    loops take no flag) and only the macro-face/-edge/-vertex points follow the flag."""
     gm, om = meshes[name]
     Lc = Lf - 1
-    fl = hyg.INNER if flag == "inner" else hyg.ALL
+    fl = {"inner": hyg.INNER, "all": hyg.ALL, "dirichlet": hyg.DIRICHLET}[flag]
     vc, vf = rnd(gm.ndofs(Lc), 21), rnd(gm.ndofs(Lf), 22)
     for op_name in ("restrict", "prolong_replace", "prolong_add"):
         g = hyg.Function(gm, Lc, Lf)
