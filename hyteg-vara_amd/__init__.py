@@ -54,6 +54,7 @@ def _sig(lib):
         "hyg_add": (i, [vp, i, P(np.float64), pvp, i, i, vp]),
         "hyg_mult_elementwise": (i, [vp, vp, vp, i, i, vp]),
         "hyg_dot": (i, [vp, vp, i, i, C.POINTER(C.c_double), vp]),
+        "hyg_dot_noalloc": (i, [vp, vp, i, i, vp, vp]),
         "hyg_convert": (i, [vp, vp, i, vp]),
         "hyg_op_create_p1_const": (i, [vp, i, i, pvp]),
         "hyg_op_create_p1_divkgrad": (i, [vp, i, i, d, P(np.float64), i, pvp]),

@@ -19,10 +19,19 @@ void primEntriesPos( const LowerEntry& e, int N, int i, int j, int& qx, int& qy,
 
 using namespace hyg;
 
+// partial sums of hyg_dot_noalloc (HYG_NOALLOC_DOT_PARTIALS values, 2 MiB per device)
+static __device__ __align__( 256 ) double scratch[HYG_NOALLOC_DOT_PARTIALS];
+
 namespace {
 
 thread_local std::string g_err;
 int                      g_device = -1;
+
+// Fixed reduction scratch of hyg_dot_noalloc: one partial sum per dot block. Every device the library runs on
+// holds its own copy until the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (kept in g_scratchDev by hyg_init).
+constexpr int kMaxDevices      = 64;
+double*       g_scratchDev[kMaxDevices] = {};
 
 // per-process reduction scratch (dot products); calls that overlap on one device race on it
 struct DotScratch
@@ -516,6 +525,12 @@ int hyg_init( int device )
    if ( device < 0 || device >= n )
       return fail( HYG_ERR_ARG, "device index out of range" );
    HYG_HIP( hipSetDevice( device ) );
+   if ( device < kMaxDevices && !g_scratchDev[device] )
+   {
+      void* p = nullptr;
+      HYG_HIP( hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) ) );
+      g_scratchDev[device] = static_cast< double* >( p );
+   }
    g_device = device;
    return HYG_OK;
 }
@@ -868,6 +883,27 @@ int hyg_dot( hyg_func a, hyg_func b, int level, int flag, double* out, void* str
    if ( !out )
       return fail( HYG_ERR_ARG, "null output" );
    return dotImpl( a, b, level, flag, out, S( stream ) );
+}
+
+int hyg_dot_noalloc( hyg_func a, hyg_func b, int level, int flag, double* d_out, void* stream )
+{
+   level = lat( a, level );
+   HYG_TRY( checkPair( a, b, level ) );
+   if ( !d_out )
+      return fail( HYG_ERR_ARG, "null output" );
+   MeshCtx& mc = a->mesh->ctx;
+   // the level tables exist (built when the functions were created), so geom() allocates nothing here
+   LevelGeom& g      = mc.geom( level );
+   const int  needed = dotPartialsNeeded( mc, g );
+   if ( needed > HYG_NOALLOC_DOT_PARTIALS )
+      return fail( HYG_ERR_ARG, "hyg_dot_noalloc: this level needs " + std::to_string( needed ) + " partial sums, more than the " +
+                                    std::to_string( HYG_NOALLOC_DOT_PARTIALS ) + " of the fixed scratch (use hyg_dot)" );
+   double* part = mc.device >= 0 && mc.device < kMaxDevices ? g_scratchDev[mc.device] : nullptr;
+   if ( !part )
+      return fail( HYG_ERR_STATE, "hyg_dot_noalloc: hyg_init was not called for the device of this mesh" );
+   // every block writes its own partial sum, so nothing needs clearing between calls
+   launchDot( mc, g, level, a->dtype, a->data[level], b->data[level], flag, part, d_out, S( stream ) );
+   return launched( "dot_noalloc" );
 }
 
 int hyg_convert( hyg_func dst, hyg_func src, int level, void* stream )

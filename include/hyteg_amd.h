@@ -114,6 +114,15 @@ int hyg_add( hyg_func dst, int n, const double* scalars, const hyg_func* srcs, i
 int hyg_mult_elementwise( hyg_func dst, hyg_func a, hyg_func b, int level, int flag, void* stream );
 /* VertexDoFFunction::dotGlobal(other, level, flag) (VertexDoFFunction.cpp:1668-1753); synchronises `stream`. */
 int hyg_dot( hyg_func a, hyg_func b, int level, int flag, double* out, void* stream );
+/* No counterpart in the reference. hyg_dot for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the same kernels and the same
+ * result, written asynchronously to the caller's device address d_out (one double) on `stream`, with no
+ * host synchronisation. The block partial sums go to a fixed scratch array of HYG_NOALLOC_DOT_PARTIALS
+ * doubles (2 MiB) that every device holds for the life of the process (its address is resolved by
+ * hyg_init); a level needing more partial sums is rejected with HYG_ERR_ARG. Every call on a device
+ * shares that device's scratch: calls that overlap on one device (two streams, two threads) race. */
+#define HYG_NOALLOC_DOT_PARTIALS 262144
+int hyg_dot_noalloc( hyg_func a, hyg_func b, int level, int flag, double* d_out, void* stream );
 /* P1Function<T>::copyFrom(P1Function<U>) between value types (VertexDoFFunction.hpp:511-530, FunctionMemory.hpp:117) */
 int hyg_convert( hyg_func dst, hyg_func src, int level, void* stream );
 
