@@ -55,6 +55,13 @@ def _sig(lib):
         "hyg_mult_elementwise": (i, [vp, vp, vp, i, i, vp]),
         "hyg_dot": (i, [vp, vp, i, i, C.POINTER(C.c_double), vp]),
         "hyg_dot_noalloc": (i, [vp, vp, i, i, vp, vp]),
+        "hyg_batch_create": (i, [i, pvp, pvp, pvp, pvp]),
+        "hyg_batch_destroy": (i, [vp]),
+        "hyg_batch_set_config": (i, [vp, i]),
+        "hyg_batch_vcycle": (i, [vp, i, vp]),
+        "hyg_batch_apply": (i, [vp, pvp, pvp, i, vp]),
+        "hyg_batch_residual_norms": (i, [vp, P(np.float64), vp]),
+        "hyg_batch_ir_solve": (i, [vp, pvp, pvp, i, i, d, i, P(np.float64), C.POINTER(C.c_int), vp]),
         "hyg_convert": (i, [vp, vp, i, vp]),
         "hyg_op_create_p1_const": (i, [vp, i, i, pvp]),
         "hyg_op_create_p1_divkgrad": (i, [vp, i, i, d, P(np.float64), i, pvp]),
@@ -391,6 +398,47 @@ def ir_solve(op64, gmg32, x, b, L, inner=1, max_outer=30, rel_tol=1e-10, stream=
     steps = C.c_int()
     check(load().hyg_ir_solve(op64.h, gmg32.h, x.h, b.h, L, inner, max_outer, rel_tol, res, C.byref(steps), stream))
     return res[: steps.value + 1]
+
+
+class Batch:
+    """Independent problems solved together: every grid operation of their V-cycles is one launch (hyg_batch_*).
+    solvers[i] solves problem i on x[i], b[i] at its max level."""
+
+    def __init__(self, solvers, xs, bs):
+        n = len(solvers)
+        arr = lambda objs: (C.c_void_p * n)(*[o.h.value if isinstance(o.h, C.c_void_p) else o.h for o in objs])
+        self._keep = (solvers, xs, bs)
+        self.n = n
+        h = _out()
+        check(load().hyg_batch_create(n, arr(solvers), arr(xs), arr(bs), C.byref(h)))
+        self.h = h
+        self._arr = arr
+
+    def set_config(self, cfg):
+        check(load().hyg_batch_set_config(self.h, cfg))
+
+    def vcycle(self, use_graph=False, stream=None):
+        check(load().hyg_batch_vcycle(self.h, 1 if use_graph else 0, stream))
+
+    def apply(self, srcs, dsts, flag=INNER, stream=None):
+        check(load().hyg_batch_apply(self.h, self._arr(srcs), self._arr(dsts), flag, stream))
+
+    def residual_norms(self, stream=None):
+        out = np.zeros(self.n)
+        check(load().hyg_batch_residual_norms(self.h, out, stream))
+        return out
+
+    def ir_solve(self, x64, b64, inner=1, max_outer=30, rel_tol=1e-10, use_graph=False, stream=None):
+        """Defect correction of every problem; returns the residual histories (n x (steps + 1))."""
+        res = np.zeros(self.n * (max_outer + 1))
+        steps = C.c_int()
+        check(load().hyg_batch_ir_solve(self.h, self._arr(x64), self._arr(b64), inner, max_outer, rel_tol,
+                                        1 if use_graph else 0, res, C.byref(steps), stream))
+        return res.reshape(self.n, max_outer + 1)[:, : steps.value + 1]
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.hyg_batch_destroy(self.h)
 
 
 def fourier_field(seed, modes=32, kmax=8, std=1.0):

@@ -1713,3 +1713,553 @@ int hyg_ir_solve( hyg_op op64, hyg_gmg g32, hyg_func x, hyg_func b, int level, i
 }
 
 } // extern "C"
+
+// ================================================================== batched independent problems
+// A batch runs the V-cycles (and the defect-correction steps around them) of many independent problems
+// together: every grid operation is one persistent launch over all problems (BatchPlan). Problem i has its own
+// solver (operator, levels, smoother scalars), mesh and functions; the cycle parameters (value type, smoother
+// kind and order, pre/post/increment) are those of the first solver and must be shared by all.
+struct hyg_batch_s
+{
+   std::vector< hyg_gmg_s* >  g;
+   std::vector< hyg_func_s* > x, b;
+   int                        dtype = 8, cfg = 1;
+   int                        depth = 0; // largest number of levels below the top
+   std::unique_ptr< BatchPlan > cycle;  // one V-cycle of every problem
+   std::unique_ptr< BatchPlan > norms;  // residual b - A x at the top level into the solvers' tmp, its dot
+   hipGraphExec_t             cycleExec = nullptr;
+   double*                    partials  = nullptr; // dot partial sums, per problem at offset partOff[i]
+   double*                    results   = nullptr; // one per problem
+   double*                    host      = nullptr; // pinned copy of results
+   // defect correction of the FP64 problems (x64, b64) around this FP32 batch: keyed by the creation numbers
+   std::vector< uint64_t >    irKey;
+   std::vector< hyg_func_s* > r64, t64;
+   std::unique_ptr< BatchPlan > irPre, irPost; // residual + convert + zero; correction + residual + dot
+   hipGraphExec_t             irExec  = nullptr;
+   int                        irInner = 0;
+   // batched apply plans keyed by the creation numbers of (src, dst) of every problem and the flag
+   std::vector< uint64_t >    applyKey;
+   std::unique_ptr< BatchPlan > applyPlan;
+   ~hyg_batch_s()
+   {
+      if ( cycleExec )
+         (void) hipGraphExecDestroy( cycleExec );
+      if ( irExec )
+         (void) hipGraphExecDestroy( irExec );
+      for ( hyg_func_s* f : r64 )
+         destroyFunc( f );
+      for ( hyg_func_s* f : t64 )
+         destroyFunc( f );
+      (void) hipFree( partials );
+      (void) hipFree( results );
+      if ( host )
+         (void) hipHostFree( host );
+   }
+};
+
+namespace {
+
+int levelOf( const hyg_batch_s* bt, size_t i, int d ) { return bt->g[i]->maxL - d; }
+
+// inner DoFs of level L (cell interiors and points of primitives flagged Inner)
+i64 innerDofs( const MeshCtx& mc, int L )
+{
+   i64 n = 0;
+   forEachCanonical( mc, L, [&]( i64, int p, int, int, int ) {
+      if ( p < 0 || ( mc.primsHost[p].bc & HYG_INNER ) )
+         ++n;
+   } );
+   return n;
+}
+
+BatchPlan::Item stencilItem( hyg_gmg_s* g, int L, int mode, const void* x, const void* in1, void* out0, void* out1, double a )
+{
+   MeshCtx&        mc = g->op->mesh->ctx;
+   BatchPlan::Item it{ &mc, &mc.geom( L ), g->op, L, StencilCall{} };
+   it.c.mode = mode;
+   it.c.flag = kGmgFlag;
+   it.c.x    = x;
+   it.c.in1  = in1;
+   it.c.out0 = out0;
+   it.c.out1 = out1;
+   it.c.a    = a;
+   return it;
+}
+
+// one smoothing step (WeightedJacobiSmoother or ChebyshevSmoother::solve, as smoothImpl) of the problems in `act` at depth d
+void batchSmooth( hyg_batch_s* bt, BatchPlan& plan, const std::vector< size_t >& act, int d )
+{
+   const int dt = bt->dtype;
+   if ( bt->g[0]->smoother == 0 )
+   {
+      std::vector< BatchPlan::Vec >  cp;
+      std::vector< BatchPlan::Item > it;
+      for ( size_t i : act )
+      {
+         hyg_gmg_s* g = bt->g[i];
+         const int  L = levelOf( bt, i, d );
+         MeshCtx&   mc = g->op->mesh->ctx;
+         BatchPlan::Vec v{ &mc, &mc.geom( L ), L, HYG_ALL, 1 };
+         v.dst    = g->jt->data[L];
+         v.src[0] = bt->x[i]->data[L];
+         v.a[0]   = 1.0;
+         cp.push_back( v );
+         it.push_back( stencilItem( g, L, MODE_JACOBI, g->jt->data[L], bt->b[i]->data[L], bt->x[i]->data[L], nullptr, g->relax ) );
+      }
+      plan.vec( dt, cp );
+      plan.stencil( dt, MODE_JACOBI, it );
+      return;
+   }
+   const int K = bt->g[0]->chebOrder;
+   {
+      std::vector< BatchPlan::Item > it;
+      for ( size_t i : act )
+      {
+         hyg_gmg_s* g = bt->g[i];
+         const int  L = levelOf( bt, i, d );
+         it.push_back( stencilItem( g, L, MODE_CHEB0, bt->x[i]->data[L], bt->b[i]->data[L], g->t1->data[L], g->cy->data[L], g->cheb[0] ) );
+      }
+      plan.stencil( dt, MODE_CHEB0, it );
+   }
+   bool swapped = false; // t1 -> t2 -> t1 ... as smoothImpl's tin / tout
+   for ( int k = 1; k < K; ++k )
+   {
+      std::vector< BatchPlan::Item > it;
+      for ( size_t i : act )
+      {
+         hyg_gmg_s*  g    = bt->g[i];
+         const int   L    = levelOf( bt, i, d );
+         hyg_func_s* tin  = swapped ? g->t2 : g->t1;
+         hyg_func_s* tout = swapped ? g->t1 : g->t2;
+         it.push_back( stencilItem( g, L, MODE_CHEBK, tin->data[L], g->cy->data[L], tout->data[L],
+                                    k == K - 1 ? bt->x[i]->data[L] : g->cy->data[L], g->cheb[k] ) );
+      }
+      plan.stencil( dt, MODE_CHEBK, it );
+      swapped = !swapped;
+   }
+   if ( K == 1 )
+   {
+      std::vector< BatchPlan::Vec > cp;
+      for ( size_t i : act )
+      {
+         hyg_gmg_s* g  = bt->g[i];
+         const int  L  = levelOf( bt, i, d );
+         MeshCtx&   mc = g->op->mesh->ctx;
+         BatchPlan::Vec v{ &mc, &mc.geom( L ), L, kGmgFlag, 1 };
+         v.dst    = bt->x[i]->data[L];
+         v.src[0] = g->cy->data[L];
+         v.a[0]   = 1.0;
+         cp.push_back( v );
+      }
+      plan.vec( dt, cp );
+   }
+}
+
+// GeometricMultigridSolver::solveRecursively (as vcycleImpl) for every problem: depth d acts on level maxL_i - d of the
+// problems that still have a coarser level; a problem's coarse level has no inner DoFs (checked at creation), so its
+// coarse solve changes nothing and is left out
+void buildCycle( hyg_batch_s* bt, BatchPlan& plan )
+{
+   const int dt   = bt->dtype;
+   auto      act  = [&]( int d ) {
+      std::vector< size_t > a;
+      for ( size_t i = 0; i < bt->g.size(); ++i )
+         if ( bt->g[i]->maxL - d > bt->g[i]->minL )
+            a.push_back( i );
+      return a;
+   };
+   const hyg_gmg_s* g0 = bt->g[0];
+   for ( int d = 0; d < bt->depth; ++d )
+   {
+      const auto A = act( d );
+      for ( int s = 0; s < g0->pre + g0->inc * d; ++s )
+         batchSmooth( bt, plan, A, d );
+      std::vector< BatchPlan::Item >     res;
+      std::vector< BatchPlan::Transfer > rst;
+      std::vector< BatchPlan::Vec >      vv;
+      for ( size_t i : A )
+      {
+         hyg_gmg_s* g  = bt->g[i];
+         const int  L  = levelOf( bt, i, d );
+         MeshCtx&   mc = g->op->mesh->ctx;
+         res.push_back( stencilItem( g, L, MODE_RESIDUAL, bt->x[i]->data[L], bt->b[i]->data[L], g->tmp->data[L], nullptr, 0.0 ) );
+         rst.push_back( BatchPlan::Transfer{ &mc, &mc.geom( L - 1 ), L - 1, kGmgFlag, false, g->tmp->data[L - 1], g->tmp->data[L] } );
+         BatchPlan::Vec cb{ &mc, &mc.geom( L - 1 ), L - 1, kGmgFlag, 1 };
+         cb.dst    = bt->b[i]->data[L - 1];
+         cb.src[0] = g->tmp->data[L - 1];
+         cb.a[0]   = 1.0;
+         vv.push_back( cb );
+         BatchPlan::Vec zx{ &mc, &mc.geom( L - 1 ), L - 1, HYG_ALL, 0 };
+         zx.dst = bt->x[i]->data[L - 1];
+         vv.push_back( zx );
+      }
+      plan.stencil( dt, MODE_RESIDUAL, res );
+      plan.restrictLevels( dt, rst );
+      plan.vec( dt, vv );
+   }
+   for ( int d = bt->depth - 1; d >= 0; --d )
+   {
+      const auto                         A = act( d );
+      std::vector< BatchPlan::Transfer > pro;
+      for ( size_t i : A )
+      {
+         hyg_gmg_s* g  = bt->g[i];
+         const int  L  = levelOf( bt, i, d );
+         MeshCtx&   mc = g->op->mesh->ctx;
+         pro.push_back( BatchPlan::Transfer{ &mc, &mc.geom( L ), L, kGmgFlag, true, bt->x[i]->data[L], bt->x[i]->data[L - 1] } );
+      }
+      plan.prolongate( dt, pro );
+      for ( int s = 0; s < g0->post + g0->inc * d; ++s )
+         batchSmooth( bt, plan, A, d );
+   }
+}
+
+// r = b - A x at the top level of every problem (into r[i]), then the per-problem dot r . r into bt->results
+void buildNorms( hyg_batch_s* bt, BatchPlan& plan, int dt, const std::vector< hyg_func_s* >& xs, const std::vector< hyg_func_s* >& bs,
+                 const std::vector< hyg_func_s* >& rs )
+{
+   std::vector< BatchPlan::Item >     res;
+   std::vector< BatchPlan::Vec >      dots;
+   std::vector< BatchPlan::DotFinal > fin;
+   double*                            part = bt->partials;
+   for ( size_t i = 0; i < bt->g.size(); ++i )
+   {
+      hyg_gmg_s* g  = bt->g[i];
+      const int  L  = g->maxL;
+      MeshCtx&   mc = g->op->mesh->ctx;
+      res.push_back( stencilItem( g, L, MODE_RESIDUAL, xs[i]->data[L], bs[i]->data[L], rs[i]->data[L], nullptr, 0.0 ) );
+      BatchPlan::Vec v{ &mc, &mc.geom( L ), L, kGmgFlag, 0 };
+      v.dot      = true;
+      v.src[0]   = rs[i]->data[L];
+      v.src[1]   = rs[i]->data[L];
+      v.partials = part;
+      dots.push_back( v );
+      const int np = dotPartialsNeeded( mc, mc.geom( L ) );
+      fin.push_back( BatchPlan::DotFinal{ part, np, bt->results + i } );
+      part += np;
+   }
+   plan.stencil( dt, MODE_RESIDUAL, res );
+   plan.vec( dt, dots );
+   plan.dotFinal( fin );
+}
+
+int batchLaunchGraph( const BatchPlan& plan, hipGraphExec_t* exec, size_t from, size_t to, hipStream_t st )
+{
+   if ( !*exec )
+   {
+      hipGraph_t graph = nullptr;
+      HYG_HIP( hipStreamBeginCapture( st, hipStreamCaptureModeThreadLocal ) );
+      plan.launch( st, from, to );
+      const hipError_t e = hipStreamEndCapture( st, &graph );
+      if ( e != hipSuccess )
+         return fail( HYG_ERR_HIP, std::string( "capturing the batched cycle: " ) + hipGetErrorString( e ) );
+      const hipError_t e2 = hipGraphInstantiate( exec, graph, nullptr, nullptr, 0 );
+      (void) hipGraphDestroy( graph );
+      if ( e2 != hipSuccess )
+      {
+         *exec = nullptr;
+         return fail( HYG_ERR_HIP, std::string( "instantiating the batched cycle: " ) + hipGetErrorString( e2 ) );
+      }
+   }
+   HYG_HIP( hipGraphLaunch( *exec, st ) );
+   return HYG_OK;
+}
+
+int batchPlanCycle( hyg_batch_s* bt )
+{
+   bt->cycle.reset( new BatchPlan );
+   bt->cycle->cfg = bt->cfg;
+   buildCycle( bt, *bt->cycle );
+   HYG_HIP( bt->cycle->upload() );
+   if ( bt->cycleExec )
+   {
+      (void) hipGraphExecDestroy( bt->cycleExec );
+      bt->cycleExec = nullptr;
+   }
+   std::vector< hyg_func_s* > tmps;
+   for ( hyg_gmg_s* g : bt->g )
+      tmps.push_back( g->tmp );
+   bt->norms.reset( new BatchPlan );
+   bt->norms->cfg = bt->cfg;
+   buildNorms( bt, *bt->norms, bt->dtype, bt->x, bt->b, tmps );
+   HYG_HIP( bt->norms->upload() );
+   bt->irKey.clear(); // rebuilt with the new configuration on the next defect-correction call
+   bt->applyKey.clear();
+   return HYG_OK;
+}
+
+int batchReadNorms( hyg_batch_s* bt, double* out, hipStream_t st )
+{
+   const size_t n = bt->g.size();
+   HYG_HIP( hipMemcpyAsync( bt->host, bt->results, n * sizeof( double ), hipMemcpyDeviceToHost, st ) );
+   HYG_HIP( hipStreamSynchronize( st ) );
+   for ( size_t i = 0; i < n; ++i )
+   {
+      out[i] = std::sqrt( bt->host[i] );
+      if ( !std::isfinite( out[i] ) )
+         return fail( HYG_ERR_NUMERIC, "non-finite residual norm in problem " + std::to_string( i ) );
+   }
+   return HYG_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int hyg_batch_create( int n, const hyg_gmg* solvers, const hyg_func* x, const hyg_func* b, hyg_batch* out )
+{
+   HYG_TRY( checkInit() );
+   if ( n < 1 || !solvers || !x || !b || !out )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   const hyg_gmg_s* g0 = solvers[0];
+   if ( !g0 )
+      return fail( HYG_ERR_ARG, "null solver handle" );
+   for ( int i = 0; i < n; ++i )
+   {
+      hyg_gmg_s* g = solvers[i];
+      const std::string who = "problem " + std::to_string( i ) + ": ";
+      HYG_TRY( checkGmgArgs( g, x[i], b[i], g ? g->maxL : 0 ) );
+      if ( g->dtype != g0->dtype || g->smoother != g0->smoother || g->pre != g0->pre || g->post != g0->post || g->inc != g0->inc ||
+           ( g->smoother == 1 && g->chebOrder != g0->chebOrder ) )
+         return fail( HYG_ERR_ARG, who + "value type, smoother and cycle shape must be those of problem 0" );
+      if ( g->smoother > 1 )
+         return fail( HYG_ERR_ARG, who + "batched cycles take the weighted Jacobi or the Chebyshev smoother" );
+      if ( g->op->lat != 0 || g->op->kind > 1 )
+         return fail( HYG_ERR_ARG, who + "batched cycles take P1 constant and P1 elementwise operators" );
+      if ( g->op->kind == 1 && !g->op->stored )
+         return fail( HYG_ERR_ARG, who + "an elementwise operator in a batch must keep its local matrices (hyg_op_store_local_matrices)" );
+      if ( innerDofs( g->op->mesh->ctx, g->minL ) != 0 )
+         return fail( HYG_ERR_ARG, who + "the coarse level has inner DoFs (batched cycles have no coarse solve)" );
+   }
+   auto* bt  = new hyg_batch_s;
+   bt->dtype = g0->dtype;
+   for ( int i = 0; i < n; ++i )
+   {
+      bt->g.push_back( solvers[i] );
+      bt->x.push_back( x[i] );
+      bt->b.push_back( b[i] );
+      bt->depth = std::max( bt->depth, solvers[i]->maxL - solvers[i]->minL );
+   }
+   size_t np = 0;
+   for ( hyg_gmg_s* g : bt->g )
+   {
+      MeshCtx& mc = g->op->mesh->ctx;
+      np += size_t( dotPartialsNeeded( mc, mc.geom( g->maxL ) ) );
+   }
+   int rc = HYG_OK;
+   if ( hipMalloc( &bt->partials, np * sizeof( double ) ) != hipSuccess || hipMalloc( &bt->results, n * sizeof( double ) ) != hipSuccess ||
+        hipHostMalloc( (void**) &bt->host, n * sizeof( double ), hipHostMallocDefault ) != hipSuccess )
+      rc = fail( HYG_ERR_HIP, "allocating the batch's reduction buffers" );
+   if ( rc == HYG_OK )
+      rc = batchPlanCycle( bt );
+   if ( rc != HYG_OK )
+   {
+      delete bt;
+      return rc;
+   }
+   *out = bt;
+   return HYG_OK;
+}
+
+int hyg_batch_destroy( hyg_batch bt )
+{
+   delete bt;
+   return HYG_OK;
+}
+
+int hyg_batch_set_config( hyg_batch bt, int config )
+{
+   if ( !bt || config < 0 || config > 2 )
+      return fail( HYG_ERR_ARG, "batch configuration must be 0, 1 or 2" );
+   if ( config == bt->cfg )
+      return HYG_OK;
+   bt->cfg = config;
+   return batchPlanCycle( bt );
+}
+
+int hyg_batch_vcycle( hyg_batch bt, int use_graph, void* stream )
+{
+   if ( !bt )
+      return fail( HYG_ERR_ARG, "null batch handle" );
+   hipStream_t st = S( stream );
+   if ( use_graph )
+      return batchLaunchGraph( *bt->cycle, &bt->cycleExec, 0, size_t( -1 ), st );
+   bt->cycle->launch( st );
+   return launched( "batched V-cycle" );
+}
+
+int hyg_batch_residual_norms( hyg_batch bt, double* out, void* stream )
+{
+   if ( !bt || !out )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   bt->norms->launch( S( stream ) );
+   HYG_TRY( launched( "batched residual" ) );
+   return batchReadNorms( bt, out, S( stream ) );
+}
+
+int hyg_batch_apply( hyg_batch bt, const hyg_func* src, const hyg_func* dst, int flag, void* stream )
+{
+   if ( !bt || !src || !dst )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   const size_t            n = bt->g.size();
+   std::vector< uint64_t > key{ uint64_t( flag ) };
+   for ( size_t i = 0; i < n; ++i )
+   {
+      const int L = bt->g[i]->maxL;
+      HYG_TRY( checkPair( src[i], dst[i], L ) );
+      HYG_TRY( checkOpFunc( bt->g[i]->op, src[i] ) );
+      if ( src[i]->dtype != bt->dtype )
+         return fail( HYG_ERR_ARG, "functions must have the batch's value type" );
+      key.push_back( src[i]->gen );
+      key.push_back( dst[i]->gen );
+   }
+   if ( key != bt->applyKey )
+   {
+      std::vector< BatchPlan::Item > it;
+      for ( size_t i = 0; i < n; ++i )
+      {
+         BatchPlan::Item item = stencilItem( bt->g[i], bt->g[i]->maxL, MODE_APPLY, src[i]->data[bt->g[i]->maxL], nullptr,
+                                             dst[i]->data[bt->g[i]->maxL], nullptr, 0.0 );
+         item.c.flag          = flag;
+         it.push_back( item );
+      }
+      bt->applyPlan.reset( new BatchPlan );
+      bt->applyPlan->cfg = bt->cfg;
+      bt->applyPlan->stencil( bt->dtype, MODE_APPLY, it );
+      HYG_HIP( bt->applyPlan->upload() );
+      bt->applyKey = key;
+   }
+   bt->applyPlan->launch( S( stream ) );
+   return launched( "batched apply" );
+}
+
+int hyg_batch_ir_solve( hyg_batch bt, const hyg_func* x64, const hyg_func* b64, int inner, int max_outer, double rel_tol,
+                        int use_graph, double* res, int* steps, void* stream )
+{
+   if ( !bt || !x64 || !b64 || !res || !steps || inner < 1 || max_outer < 0 )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   if ( bt->dtype != HYG_F32 )
+      return fail( HYG_ERR_ARG, "defect correction runs FP32 batched cycles" );
+   const size_t            n = bt->g.size();
+   std::vector< uint64_t > key{ uint64_t( inner ) };
+   for ( size_t i = 0; i < n; ++i )
+   {
+      hyg_gmg_s* g = bt->g[i];
+      HYG_TRY( checkPair( x64[i], b64[i], g->maxL ) );
+      HYG_TRY( checkOpFunc( g->op, x64[i] ) );
+      if ( x64[i]->dtype != HYG_F64 )
+         return fail( HYG_ERR_ARG, "defect correction works on FP64 x and b" );
+      key.push_back( x64[i]->gen );
+      key.push_back( b64[i]->gen );
+   }
+   hipStream_t st = S( stream );
+   if ( key != bt->irKey )
+   {
+      // FP64 residual and correction of every problem at its top level
+      if ( bt->r64.empty() )
+         for ( hyg_gmg_s* g : bt->g )
+         {
+            hyg_func_s *r = nullptr, *t = nullptr;
+            HYG_TRY( createFunc( g->op->mesh, g->maxL, g->maxL, HYG_F64, &r ) );
+            bt->r64.push_back( r );
+            HYG_TRY( createFunc( g->op->mesh, g->maxL, g->maxL, HYG_F64, &t ) );
+            bt->t64.push_back( t );
+         }
+      std::vector< hyg_func_s* > xs( x64, x64 + n ), bs( b64, b64 + n );
+      // before the cycles: b32 = (float) r64, x32 = 0; after: t64 = (double) x32, x64 += t64, r64 = b64 - A x64, r64 . r64
+      bt->irPre.reset( new BatchPlan );
+      bt->irPost.reset( new BatchPlan );
+      bt->irPre->cfg = bt->irPost->cfg = bt->cfg;
+      std::vector< BatchPlan::Convert > toF, toD;
+      std::vector< BatchPlan::Vec >     zero, add;
+      for ( size_t i = 0; i < n; ++i )
+      {
+         hyg_gmg_s* g  = bt->g[i];
+         const int  L  = g->maxL;
+         MeshCtx&   mc = g->op->mesh->ctx;
+         const i64  nv = levelValues( bt->x[i], L );
+         toF.push_back( BatchPlan::Convert{ nv, bt->b[i]->data[L], bt->r64[i]->data[L] } );
+         BatchPlan::Vec z{ &mc, &mc.geom( L ), L, HYG_ALL, 0 };
+         z.dst = bt->x[i]->data[L];
+         zero.push_back( z );
+         toD.push_back( BatchPlan::Convert{ nv, bt->t64[i]->data[L], bt->x[i]->data[L] } );
+         BatchPlan::Vec a{ &mc, &mc.geom( L ), L, kGmgFlag, 1 };
+         a.acc    = true;
+         a.dst    = x64[i]->data[L];
+         a.src[0] = bt->t64[i]->data[L];
+         a.a[0]   = 1.0;
+         add.push_back( a );
+      }
+      bt->irPre->convert( HYG_F32, HYG_F64, toF );
+      bt->irPre->vec( HYG_F32, zero );
+      bt->irPost->convert( HYG_F64, HYG_F32, toD );
+      bt->irPost->vec( HYG_F64, add );
+      buildNorms( bt, *bt->irPost, HYG_F64, xs, bs, bt->r64 );
+      HYG_HIP( bt->irPre->upload() );
+      HYG_HIP( bt->irPost->upload() );
+      if ( bt->irExec )
+      {
+         (void) hipGraphExecDestroy( bt->irExec );
+         bt->irExec = nullptr;
+      }
+      bt->irKey   = key;
+      bt->irInner = inner;
+   }
+   // initial residual: the last four steps of irPost (residual sweep and combine, dot partial sums, final sums)
+   const size_t nPost = bt->irPost->steps.size();
+   bt->irPost->launch( st, nPost - 4, nPost );
+   HYG_TRY( launched( "batched residual" ) );
+   std::vector< double > r( n );
+   HYG_TRY( batchReadNorms( bt, r.data(), st ) );
+   const size_t stride = size_t( max_outer ) + 1;
+   for ( size_t i = 0; i < n; ++i )
+      res[i * stride] = r[i];
+   std::vector< double > r0 = r;
+   *steps = 0;
+   for ( int it = 0; it < max_outer; ++it )
+   {
+      bool done = true;
+      for ( size_t i = 0; i < n; ++i )
+         done = done && ( r0[i] == 0.0 || r[i] / r0[i] <= rel_tol );
+      if ( done )
+         break;
+      if ( use_graph )
+      {
+         if ( !bt->irExec )
+         {
+            hipGraph_t graph = nullptr;
+            HYG_HIP( hipStreamBeginCapture( st, hipStreamCaptureModeThreadLocal ) );
+            bt->irPre->launch( st );
+            for ( int k = 0; k < inner; ++k )
+               bt->cycle->launch( st );
+            bt->irPost->launch( st );
+            const hipError_t e = hipStreamEndCapture( st, &graph );
+            if ( e != hipSuccess )
+               return fail( HYG_ERR_HIP, std::string( "capturing the defect-correction step: " ) + hipGetErrorString( e ) );
+            const hipError_t e2 = hipGraphInstantiate( &bt->irExec, graph, nullptr, nullptr, 0 );
+            (void) hipGraphDestroy( graph );
+            if ( e2 != hipSuccess )
+            {
+               bt->irExec = nullptr;
+               return fail( HYG_ERR_HIP, std::string( "instantiating the defect-correction step: " ) + hipGetErrorString( e2 ) );
+            }
+         }
+         HYG_HIP( hipGraphLaunch( bt->irExec, st ) );
+      }
+      else
+      {
+         bt->irPre->launch( st );
+         for ( int k = 0; k < inner; ++k )
+            bt->cycle->launch( st );
+         bt->irPost->launch( st );
+         HYG_TRY( launched( "batched defect correction" ) );
+      }
+      HYG_TRY( batchReadNorms( bt, r.data(), st ) );
+      for ( size_t i = 0; i < n; ++i )
+         res[i * stride + it + 1] = r[i];
+      *steps = it + 1;
+   }
+   return HYG_OK;
+}
+
+} // extern "C"

@@ -17,212 +17,10 @@
 //   3. adds the one-sided sums of face, edge and vertex points over their cells and writes every
 //      replica (kFaceCombine, kLowCombine).
 // The inverse diagonal (per point, since it depends on k) is built the same way at operator setup.
-#include "face_combine.hpp"
+
+#include "elementwise.hpp"
 
 namespace hyg {
-
-// stencil direction (C W E N S NW SE TC TW TS TSE BC BN BE BNW) from vertex k to vertex j of type t
-constexpr int kMCDir[6][4][4] = { { { 0, 2, 3, 7 }, { 1, 0, 5, 8 }, { 4, 6, 0, 9 }, { 11, 13, 12, 0 } },
-                                  { { 0, 3, 5, 7 }, { 4, 0, 1, 9 }, { 6, 2, 0, 10 }, { 11, 12, 14, 0 } },
-                                  { { 0, 5, 7, 8 }, { 6, 0, 10, 9 }, { 11, 14, 0, 1 }, { 13, 12, 2, 0 } },
-                                  { { 0, 7, 8, 9 }, { 11, 0, 1, 4 }, { 13, 2, 0, 6 }, { 12, 3, 5, 0 } },
-                                  { { 0, 5, 1, 14 }, { 6, 0, 4, 11 }, { 2, 3, 0, 12 }, { 10, 7, 9, 0 } },
-                                  { { 0, 2, 10, 7 }, { 1, 0, 9, 8 }, { 14, 12, 0, 5 }, { 11, 13, 6, 0 } } };
-// offset of vertex k of type t from the anchor, encoded x + 2y + 4z
-constexpr int kMCOff[6][4] = { { 0, 1, 2, 4 }, { 1, 3, 2, 5 }, { 1, 2, 5, 4 }, { 3, 7, 6, 5 }, { 5, 6, 4, 2 }, { 2, 3, 5, 6 } };
-// 6 x the quadrature weights (centroid first), so that kb = 1 for k = 1
-constexpr double kQW6[5] = { -0.8, 0.45, 0.45, 0.45, 0.45 };
-
-__device__ __forceinline__ void sincosT( double a, double& s, double& c ) { sincos( a, &s, &c ); }
-__device__ __forceinline__ void sincosT( float a, float& s, float& c ) { sincosf( a, &s, &c ); }
-__device__ __forceinline__ double expT( double a ) { return exp( a ); }
-__device__ __forceinline__ float  expT( float a ) { return expf( a ); }
-
-// ------------------------------------------------------------------ 1. coefficient integrals
-// One thread per anchor (x, y, z), x+y+z <= N-1, of one macro-cell; kb[c][t][anchor] for the six types.
-// Dynamic LDS: nModes x 30 x (cos, sin) of the quadrature offsets, then nModes x (theta0, gx, gy, gz, a).
-template < typename T >
-__global__ __launch_bounds__( kBlock ) void kKBar( int L, int tilesPerCell, const ElemCell* __restrict__ cells,
-                                                   const double* __restrict__ modes, int nModes, double k0, T* __restrict__ kb )
-{
-   extern __shared__ double ldsD[];
-   const int c    = int( blockIdx.x ) / tilesPerCell;
-   const int tile = int( blockIdx.x ) - c * tilesPerCell;
-   const int N    = 1 << L;
-   T*        tab  = reinterpret_cast< T* >( ldsD + 5 * nModes );
-   double*   mp   = ldsD;
-   const ElemCell& ec = cells[c];
-   for ( int m = int( threadIdx.x ); m < nModes; m += kBlock )
-   {
-      const double kx = modes[5 * m], ky = modes[5 * m + 1], kz = modes[5 * m + 2];
-      const double tp = 2.0 * M_PI;
-      mp[5 * m]       = tp * ( kx * ec.v0[0] + ky * ec.v0[1] + kz * ec.v0[2] ) + modes[5 * m + 3];
-      for ( int k = 0; k < 3; ++k )
-         mp[5 * m + 1 + k] = tp * ( kx * ec.a[0][k] + ky * ec.a[1][k] + kz * ec.a[2][k] );
-      mp[5 * m + 4] = modes[5 * m + 4];
-   }
-   __syncthreads();
-   for ( int i = int( threadIdx.x ); i < 30 * nModes; i += kBlock )
-   {
-      const int m = i / 30, tq = i - 30 * m, t = tq / 5, q = tq - 5 * t;
-      // lattice offset of quadrature point q of type t from the anchor
-      double d[3] = { 0.0, 0.0, 0.0 };
-      for ( int v = 0; v < 4; ++v )
-      {
-         const double lam = q == 0 ? 0.25 : ( v == q - 1 ? 0.5 : 1.0 / 6.0 );
-         for ( int k = 0; k < 3; ++k )
-            d[k] += lam * kMC[t][v][k];
-      }
-      const double ang = mp[5 * m + 1] * d[0] + mp[5 * m + 2] * d[1] + mp[5 * m + 3] * d[2];
-      double       sn, cs;
-      sincos( ang, &sn, &cs );
-      tab[2 * i]     = T( cs );
-      tab[2 * i + 1] = T( sn );
-   }
-   __syncthreads();
-   const int S = S3i( N );
-   const int s = tile * kBlock + int( threadIdx.x );
-   if ( s >= S )
-      return;
-   int x, y, z;
-   tetXYZ( N, s, x, y, z );
-   T g[30];
-#pragma unroll
-   for ( int k = 0; k < 30; ++k )
-      g[k] = T( 0 );
-   for ( int m = 0; m < nModes; ++m )
-   {
-      const T th = T( mp[5 * m] + mp[5 * m + 1] * x + mp[5 * m + 2] * y + mp[5 * m + 3] * z );
-      T       sn, cs;
-      sincosT( th, sn, cs );
-      const T  am = T( mp[5 * m + 4] );
-      const T  ac = am * cs, as = am * sn;
-      const T* tb = tab + 60 * m;
-#pragma unroll
-      for ( int k = 0; k < 30; ++k )
-         g[k] += ac * tb[2 * k] - as * tb[2 * k + 1];
-   }
-   T* out = kb + i64( c ) * 6 * S;
-#pragma unroll
-   for ( int t = 0; t < 6; ++t )
-   {
-      T v = T( 0 );
-#pragma unroll
-      for ( int q = 0; q < 5; ++q )
-         v += T( kQW6[q] ) * expT( g[5 * t + q] );
-      if ( x + y + z + kMCMax[t] <= N )
-         out[t * S + s] = T( k0 ) * v;
-   }
-}
-
-// ------------------------------------------------------------------ 2. stencil application per cell
-// One thread per point of the closed macro-cell. DIAG: the diagonal entry instead of the product.
-template < typename T, int MODE, bool DIAG >
-__global__ __launch_bounds__( kBlock ) void kElemApply( int L, int tilesPerCell, i64 stride, const ElemCell* __restrict__ cells,
-                                                        const T* __restrict__ kb, const CellInfo* __restrict__ cinfo,
-                                                        const CellFaceSlot* __restrict__ cfs, const LowerPrim* __restrict__ prims,
-                                                        const LowerEntry* __restrict__ entries, T* __restrict__ part,
-                                                        T* __restrict__ lowPart, const T* __restrict__ dinv, int flag,
-                                                        const T* __restrict__ x, const T* __restrict__ in1,
-                                                        T* __restrict__ out0, T* __restrict__ out1, T a )
-{
-   const int c    = int( blockIdx.x ) / tilesPerCell;
-   const int tile = int( blockIdx.x ) - c * tilesPerCell;
-   const int N = 1 << L, w = N + 1, NP = N + 1;
-   const int s = tile * kBlock + int( threadIdx.x );
-   if ( s >= S3i( w ) )
-      return;
-   int px, py, pz;
-   tetXYZ( w, s, px, py, pz );
-   const int slot = classifySlot( N, px, py, pz );
-   if ( slot == 0 && !( flag & 1 ) )
-      return;
-   const ElemCell& ec = cells[c];
-   const int       SN = S3i( N );
-   const T*        kc = kb + i64( c ) * 6 * SN;
-   // anchors p - o for the eight offsets o in {0,1}^3 (encoded x + 2y + 4z)
-   int  aIdx[8];
-   bool aOk[8];
-#pragma unroll
-   for ( int o = 0; o < 8; ++o )
-   {
-      const int ax = px - ( o & 1 ), ay = py - ( ( o >> 1 ) & 1 ), az = pz - ( ( o >> 2 ) & 1 );
-      aOk[o]       = ax >= 0 && ay >= 0 && az >= 0;
-      aIdx[o]      = aOk[o] ? cidx( N, ax, ay, az ) : 0;
-   }
-   T W[NDIR];
-#pragma unroll
-   for ( int d = 0; d < NDIR; ++d )
-      W[d] = T( 0 );
-#pragma unroll
-   for ( int t = 0; t < 6; ++t )
-#pragma unroll
-      for ( int k = 0; k < 4; ++k )
-      {
-         const int o   = kMCOff[t][k];
-         const int as  = px + py + pz - ( ( o & 1 ) + ( ( o >> 1 ) & 1 ) + ( ( o >> 2 ) & 1 ) );
-         const bool ok = aOk[o] && as + kMCMax[t] <= N;
-         const T    kv = ok ? kc[t * SN + aIdx[o]] : T( 0 );
-         if constexpr ( DIAG )
-            W[0] += kv * T( ec.mt[t][5 * k] );
-         else
-#pragma unroll
-            for ( int j = 0; j < 4; ++j )
-               W[kMCDir[t][k][j]] += kv * T( ec.mt[t][4 * k + j] );
-      }
-   const i64 q = i64( c ) * stride + cidx( w, px, py, pz );
-   T         val;
-   if constexpr ( DIAG )
-      val = W[0];
-   else
-   {
-      const int Wz = w - pz;
-      const int tc = T2i( Wz ) - py, bc = T2i( Wz + 1 ) - py, nn = Wz - py;
-      const int off[NDIR] = { 0, -1, 1, nn, -( nn + 1 ), nn - 1, -nn, tc, tc - 1, tc - nn, tc - nn + 1, -bc, -bc + nn + 1, -bc + 1, -bc + nn };
-      val = T( 0 );
-#pragma unroll
-      for ( int d = 0; d < NDIR; ++d )
-         val += W[d] * x[q + ( W[d] != T( 0 ) ? off[d] : 0 )];
-   }
-   if ( slot == 0 )
-   {
-      if constexpr ( DIAG )
-         out0[q] = val;
-      else
-      {
-         T i1 = T( 0 ), r0, r1;
-         if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-            i1 = in1[q];
-         if constexpr ( MODE == MODE_APPLY_ADD )
-            i1 = out0[q];
-         epilogue< T, MODE >( val, x[q], i1, i1, dinv[q], a, r0, r1 );
-         out0[q] = r0;
-         if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-            out1[q] = r1;
-      }
-   }
-   else if ( slot <= 4 )
-   {
-      const CellFaceSlot fs = cfs[4 * c + slot - 1];
-      if ( fs.bc & flag )
-         part[i64( fs.block ) * NP * NP + facePartIndex( slot - 1, N, px, py, pz )] = val;
-   }
-   else
-   {
-      const CellInfo& ci = cinfo[c];
-      if ( prims[ci.prim[slot]].bc & flag )
-      {
-         const int e = ci.entry[slot];
-         int       i = 0;
-         if ( slot <= 10 )
-         {
-            const int b[4] = { N - px - py - pz, px, py, pz };
-            i              = b[entries[e].l1];
-         }
-         lowPart[i64( e ) * NP + i] = val;
-      }
-   }
-}
 
 // ------------------------------------------------------------------ launch
 template < typename T >

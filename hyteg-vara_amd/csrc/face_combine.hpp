@@ -8,7 +8,7 @@
 namespace hyg {
 
 template < typename T, int MODE >
-__global__ __launch_bounds__( kBlock ) void kFaceCombine( const LowerChunk* __restrict__ chunks, int nChunks,
+__device__ __forceinline__ void bFaceCombine( int bid, const LowerChunk* __restrict__ chunks, int nChunks,
                                                           const LowerPrim* __restrict__ prims,
                                                           const LowerEntry* __restrict__ entries, int firstFacePrim,
                                                           const T* __restrict__ part, const T* __restrict__ primInv,
@@ -17,7 +17,7 @@ __global__ __launch_bounds__( kBlock ) void kFaceCombine( const LowerChunk* __re
                                                           const T* __restrict__ in1, T* __restrict__ out0, T* __restrict__ out1, T a )
 {
    const int nb = ( nChunks + 3 ) / 4;
-   const int ci = remapXcd( int( blockIdx.x ), nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
+   const int ci = remapXcd( bid, nb ) * ( kBlock / 64 ) + __builtin_amdgcn_readfirstlane( int( threadIdx.x >> 6 ) );
    if ( ci >= nChunks )
       return;
    const LowerChunk ch = chunks[ci];
@@ -52,6 +52,18 @@ __global__ __launch_bounds__( kBlock ) void kFaceCombine( const LowerChunk* __re
       if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
          out1[o[k]] = r1;
    }
+}
+
+template < typename T, int MODE >
+__global__ __launch_bounds__( kBlock ) void kFaceCombine( const LowerChunk* __restrict__ chunks, int nChunks,
+                                                          const LowerPrim* __restrict__ prims,
+                                                          const LowerEntry* __restrict__ entries, int firstFacePrim,
+                                                          const T* __restrict__ part, const T* __restrict__ primInv,
+                                                          const T* __restrict__ dinvArr, int L,
+                                                          i64 stride, int flag, const T* __restrict__ x,
+                                                          const T* __restrict__ in1, T* __restrict__ out0, T* __restrict__ out1, T a )
+{
+   bFaceCombine< T, MODE >( int( blockIdx.x ), chunks, nChunks, prims, entries, firstFacePrim, part, primInv, dinvArr, L, stride, flag, x, in1, out0, out1, a );
 }
 
 // colour of point (i, j) of a macro-face, -edge or -vertex in the multi-colour SOR ordering: the sum over the
@@ -134,13 +146,13 @@ __device__ __forceinline__ void tetXYZ( int w, int s, int& x, int& y, int& z )
 
 // ------------------------------------------------------------------ edges and vertices: combine
 template < typename T, int MODE >
-__global__ __launch_bounds__( kBlock ) void kLowCombine( const LowerChunk* __restrict__ chunks, int nChunks,
+__device__ __forceinline__ void bLowCombine( int bid, const LowerChunk* __restrict__ chunks, int nChunks,
                                                          const LowerPrim* __restrict__ prims, const LowerEntry* __restrict__ entries,
                                                          const T* __restrict__ lowPart, const T* __restrict__ dinv, int L,
                                                          i64 stride, int flag, const T* __restrict__ x, const T* __restrict__ in1,
                                                          T* __restrict__ out0, T* __restrict__ out1, T a )
 {
-   const int ci = int( blockIdx.x ) * ( kBlock / 64 ) + int( threadIdx.x >> 6 );
+   const int ci = bid * ( kBlock / 64 ) + int( threadIdx.x >> 6 );
    if ( ci >= nChunks )
       return;
    const LowerChunk ch   = chunks[ci];
@@ -173,6 +185,16 @@ __global__ __launch_bounds__( kBlock ) void kLowCombine( const LowerChunk* __res
       if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
          out1[o] = r1;
    }
+}
+
+template < typename T, int MODE >
+__global__ __launch_bounds__( kBlock ) void kLowCombine( const LowerChunk* __restrict__ chunks, int nChunks,
+                                                         const LowerPrim* __restrict__ prims, const LowerEntry* __restrict__ entries,
+                                                         const T* __restrict__ lowPart, const T* __restrict__ dinv, int L,
+                                                         i64 stride, int flag, const T* __restrict__ x, const T* __restrict__ in1,
+                                                         T* __restrict__ out0, T* __restrict__ out1, T a )
+{
+   bLowCombine< T, MODE >( int( blockIdx.x ), chunks, nChunks, prims, entries, lowPart, dinv, L, stride, flag, x, in1, out0, out1, a );
 }
 
 // 1 / d of every value (0 stays 0: padding and unused slots)

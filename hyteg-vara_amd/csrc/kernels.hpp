@@ -88,4 +88,90 @@ void launchRestrict( const MeshCtx& m, const LevelGeom& gc, int Lc, int dtype, v
 void launchProlongate( const MeshCtx& m, const LevelGeom& gf, int Lf, int dtype, void* fine, const void* coarse, int flag,
                        bool add, hipStream_t st );
 
+// ------------------------------------------------------------------ batched problems
+// A sequence of batched grid operations (steps). Each step applies one operation to every problem of a batch
+// (each at its own level, mesh, operator and functions) in one persistent launch over a task table; the
+// builders append steps on the host, upload() copies every task table to the device once, launch() issues
+// steps [from, to) in order (capturable: no allocation, copy or memset).
+struct BatchPlan
+{
+   enum Phase : int
+   {
+      STENCIL,
+      COMBINE,
+      TRANSFER,
+      VEC,
+      DOTFINAL,
+      CONVERT
+   };
+   struct Step
+   {
+      int                 phase = 0, mode = 0, dtype = 8, nTasks = 0, total = 0;
+      std::vector< char > host;
+      void*               dev = nullptr;
+   };
+   // one problem of a stencil operation (constant operators: kind 0, elementwise with stored integrals: kind 1)
+   struct Item
+   {
+      const MeshCtx*  m;
+      LevelGeom*      g;
+      const hyg_op_s* op;
+      int             L;
+      StencilCall     c;
+   };
+   // restriction to level L (from L+1) or prolongation to level L (from L-1)
+   struct Transfer
+   {
+      const MeshCtx*   m;
+      const LevelGeom* g; // geometry of level L
+      int              L, flag;
+      bool             add;
+      void*            dst;
+      const void*      src;
+   };
+   // linear combination (dot = false) or dot-product partial sums of src[0] . src[1] into `partials`
+   struct Vec
+   {
+      const MeshCtx*   m;
+      const LevelGeom* g;
+      int              L, flag, nsrc = 0;
+      bool             acc = false, dot = false;
+      void*            dst = nullptr;
+      const void*      src[3] = { nullptr, nullptr, nullptr };
+      double           a[3]   = { 0, 0, 0 };
+      double           value  = 0;
+      double*          partials = nullptr;
+   };
+   struct DotFinal
+   {
+      const double* partials;
+      int           n;
+      double*       result;
+   };
+   struct Convert
+   {
+      i64         n;
+      void*       dst;
+      const void* src;
+   };
+
+   std::vector< Step > steps;
+   void*               dev = nullptr;
+   int                 cfg = 1; // cell-interior configuration of the constant-operator sweeps (0, 1 or 2)
+
+   static bool modeSupported( int mode );
+   void        stencil( int dtype, int mode, const std::vector< Item >& items ); // sweep step + combine step
+   void        restrictLevels( int dtype, const std::vector< Transfer >& items );
+   void        prolongate( int dtype, const std::vector< Transfer >& items );
+   void        vec( int dtype, const std::vector< Vec >& items );
+   void        dotFinal( const std::vector< DotFinal >& items );
+   void        convert( int dstType, int srcType, const std::vector< Convert >& items );
+   hipError_t  upload();
+   void        launch( hipStream_t st, size_t from = 0, size_t to = size_t( -1 ) ) const;
+   BatchPlan() = default;
+   BatchPlan( const BatchPlan& ) = delete;
+   BatchPlan& operator=( const BatchPlan& ) = delete;
+   ~BatchPlan();
+};
+
 } // namespace hyg

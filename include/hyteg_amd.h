@@ -46,6 +46,7 @@ typedef struct hyg_mesh_s* hyg_mesh;
 typedef struct hyg_func_s* hyg_func;
 typedef struct hyg_op_s*   hyg_op;
 typedef struct hyg_gmg_s*  hyg_gmg;
+typedef struct hyg_batch_s* hyg_batch;
 
 /* Last error message of the calling thread ("" if none). */
 const char* hyg_last_error( void );
@@ -259,6 +260,35 @@ int hyg_gmg_solve_tol( hyg_gmg g, hyg_func x, hyg_func b, int level, int max_cyc
  * (res[0] initial); *steps = number of outer steps taken. */
 int hyg_ir_solve( hyg_op op64, hyg_gmg g32, hyg_func x, hyg_func b, int level, int inner, int max_outer,
                   double rel_tol, double* res, int* steps, void* stream );
+
+/* ---------------------------------------------------------------- batched independent problems
+ * No counterpart in the reference, which runs one GeometricMultigridSolver::solve (GeometricMultigridSolver.hpp:
+ * 136-235) per problem and one Operator::apply (p1functionspace/P1Operator.hpp:196-324) per call. A batch holds n
+ * independent problems: problem i is solved by solvers[i] (its operator, mesh, levels and smoother scalars) on
+ * x[i], b[i] at the solver's max level. Every grid operation of a cycle is one persistent launch over all problems
+ * (a stencil operation is two: the sweep and the face/edge/vertex combine), each problem at its own level; per
+ * problem the operations, their order and their arithmetic are those of hyg_gmg_vcycle. Requirements: P1 constant
+ * or elementwise operators (the latter with hyg_op_store_local_matrices applied), the weighted Jacobi or the
+ * Chebyshev smoother, the same value type, smoother order and pre/post/increment in every solver, and coarse
+ * levels without inner DoFs (the batched cycle has no coarse solve). The handles must outlive the batch. */
+int hyg_batch_create( int n, const hyg_gmg* solvers, const hyg_func* x, const hyg_func* b, hyg_batch* out );
+int hyg_batch_destroy( hyg_batch bt );
+/* Cell-interior kernel configuration of the constant-operator sweeps (0: 1 row, 1: 2 rows, 2: 4 rows per
+ * wavefront); every batched launch uses one configuration. */
+int hyg_batch_set_config( hyg_batch bt, int config );
+/* One V-cycle of every problem; use_graph = 1 captures the batched cycle into a hipGraph on the first call (on
+ * `stream`, which must not be the default stream) and replays it afterwards. */
+int hyg_batch_vcycle( hyg_batch bt, int use_graph, void* stream );
+/* dst[i] = A_i src[i] at problem i's top level on `flag` (Operator::apply, Replace), all problems in one launch. */
+int hyg_batch_apply( hyg_batch bt, const hyg_func* src, const hyg_func* dst, int flag, void* stream );
+/* out[i] = ||b_i - A_i x_i|| over Inner|Neumann DoFs at the top level (batched residual and dot). */
+int hyg_batch_residual_norms( hyg_batch bt, double* out, void* stream );
+/* Mixed-precision defect correction of every problem (as hyg_ir_solve): FP64 x64[i], b64[i]; each outer step
+ * r = b - A x in FP64, `inner` batched FP32 V-cycles of this (FP32) batch on A c = r, x += c. Outer steps run until
+ * every problem has ||r||/||r0|| <= rel_tol or after max_outer steps; use_graph = 1 replays the whole outer step
+ * as one hipGraph. res[i * (max_outer + 1) + k] = FP64 residual norm of problem i after step k. */
+int hyg_batch_ir_solve( hyg_batch bt, const hyg_func* x64, const hyg_func* b64, int inner, int max_outer, double rel_tol,
+                        int use_graph, double* res, int* steps, void* stream );
 
 #ifdef __cplusplus
 }
