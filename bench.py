@@ -117,9 +117,104 @@ def cpu_baseline(mesh_name, L, target_s, inner):
         el = time.perf_counter() - t0
         if el >= target_s or reps >= 200:
             break
-    return {"value": inner * reps / el / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
-            "sample": f"{mesh_name} L{L} FP64 P1 constant apply (inner DoFs, incl. downward halo refresh), "
-                      f"{reps} applies in {el:.1f} s on {threads} OpenMP threads"}
+    out = {"value": inner * reps / el / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
+           "sample": f"{mesh_name} L{L} FP64 P1 constant apply (inner DoFs, incl. downward halo refresh), "
+                     f"{reps} applies in {el:.1f} s on {threads} OpenMP threads"}
+    # the same apply on one thread (a bounded sample of a few applies)
+    try:
+        gomp = C.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        reps1, t0 = 0, time.perf_counter()
+        while True:
+            op.apply(s, d, L, ol.INNER)
+            reps1 += 1
+            el1 = time.perf_counter() - t0
+            if el1 >= target_s / 2 or reps1 >= 50:
+                break
+        gomp.omp_set_num_threads(threads)
+        out["single_thread"] = {"value": inner * reps1 / el1 / 1e9, "unit": "GDoF/s", "cores": 1,
+                                "sample": f"{reps1} applies in {el1:.1f} s"}
+    except OSError as e:
+        out["single_thread"] = {"value": None, "error": str(e)}
+    out["host"] = host_info()
+    return out
+
+
+def host_info():
+    """CPU model, NUMA layout and the oracle's compiler flags of the host the CPU baseline ran on."""
+    info = {"cpu_model": None, "logical_cpus": os.cpu_count(), "numa_nodes": None,
+            "compiler_flags": "g++ -O3 -march=x86-64-v3 -std=c++17 -fopenmp (oracle/Makefile)"}
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    base = "/sys/devices/system/node"
+    try:
+        nodes = sorted(d for d in os.listdir(base) if d.startswith("node") and d[4:].isdigit())
+        info["numa_nodes"] = {}
+        for nd in nodes:
+            with open(os.path.join(base, nd, "cpulist")) as fh:
+                info["numa_nodes"][nd] = fh.read().strip()
+    except OSError:
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    return info
+
+
+def graph_solves(hyg, hip, st, op, mesh, L, bv, kw, every, max_cycles=60):
+    """Time to 1e-10 of one problem, FP64 against mixed precision, both as captured hipGraphs with the same
+    residual-check cadence: FP64 replays `every` captured V-cycles between FP64 residual norms; the mixed solve
+    replays one captured defect-correction step (FP64 residual, `every` FP32 V-cycles, FP64 update) per norm.
+    Both go through single-problem batches (hyg_batch_*), whose captured cycle and defect-correction step are the
+    whole operation sequence of hyg_gmg_vcycle / hyg_ir_solve."""
+    g64 = hyg.GMG(op, 0, L, hyg.F64, **kw)
+    g32 = hyg.GMG(op, 0, L, hyg.F32, **kw)
+    xf, bf = hyg.Function(mesh, 0, L), hyg.Function(mesh, 0, L)
+    x32, b32 = hyg.Function(mesh, 0, L, hyg.F32), hyg.Function(mesh, 0, L, hyg.F32)
+    xt, bt = hyg.Function(mesh, L, L), hyg.Function(mesh, L, L)
+    bf.upload(L, bv)
+    bt.upload(L, bv)
+    b64 = hyg.Batch([g64], [xf], [bf])
+    b32b = hyg.Batch([g32], [x32], [b32])
+
+    def fp64_solve():
+        r0 = b64.residual_norms(st)[0]
+        hist, c = [r0], 0
+        while hist[-1] > 1e-10 * r0 and c < max_cycles:
+            for _ in range(every):
+                b64.vcycle(True, st)
+            c += every
+            hist.append(b64.residual_norms(st)[0])
+        return hist, c
+
+    # warm-up: captures both graphs
+    fp64_solve_once = lambda: (b64.vcycle(True, st), b64.residual_norms(st))
+    fp64_solve_once()
+    b32b.ir_solve([xt], [bt], inner=every, max_outer=1, rel_tol=1e-10, use_graph=True, stream=st)
+    xf.interpolate(0.0, L)
+    xt.interpolate(0.0, L)
+    hip.sync(st)
+    t0 = time.perf_counter()
+    h64, c64 = fp64_solve()
+    t64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hmx = b32b.ir_solve([xt], [bt], inner=every, max_outer=max_cycles // every + 1, rel_tol=1e-10, use_graph=True, stream=st)[0]
+    tmx = time.perf_counter() - t0
+    u64, umx = xf.download(L), xt.download(L)
+    return {"captured": "whole V-cycle (FP64) / whole defect-correction step (mixed) replayed as one hipGraph",
+            "residual_check_every_cycles": every,
+            "fp64": {"seconds": round(t64, 5), "cycles": c64, "rel_residual": float(h64[-1] / h64[0])},
+            "mixed": {"seconds": round(tmx, 5), "outer_steps": len(hmx) - 1, "inner_cycles": every,
+                      "rel_residual": float(hmx[-1] / hmx[0]),
+                      "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
+            "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
 
 
 def bench_divkgrad(hyg, hip, st, steps, level, seed):
@@ -154,6 +249,22 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
     # coefficient evaluation: per lattice anchor 32 modes x (3 FMA phase + 30 x 2 FMA) + 30 exp + 6 x 5 FMA
     anchors = int(mesh.info()[3]) * ((1 << L) * ((1 << L) + 1) * ((1 << L) + 2) // 6)
     out["kbar_flops_per_apply"] = anchors * (32 * (6 + 120) + 60)
+    # the stated field: 32 modes of amplitude 1.0 each (g has standard deviation 4); the apply does the same work
+    # whatever the amplitude, only the solve depends on it (it diverges for amplitude 1: see the solve below, std 0.5)
+    op1 = hyg.P1DivKGradOperator(mesh, 0, L, 1.0, hyg.fourier_field(seed, modes=32, kmax=8, std=4.0))
+    for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):
+        x, y = hyg.Function(mesh, L, L, dt), hyg.Function(mesh, L, L, dt)
+        x.upload(L, rng.uniform(-1, 1, n))
+
+        def ap1():
+            hyg.check(lib.hyg_apply(op1.h, x.h, y.h, L, hyg.INNER, hyg.REPLACE, st))
+
+        for _ in range(2):
+            ap1()
+        ms = timed(hip, st, ap1, steps) / steps
+        out[f"apply_amplitude1_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3),
+                                          "field": "32 modes, amplitude 1.0 each (std of g = 4), seed %d" % seed}
+    del op1
     # a second operator with the coefficient integrals kept (computeAndStoreLocalElementMatrices)
     ops = hyg.P1DivKGradOperator(mesh, 0, L, 1.0, modes)
     ops.store_local_matrices()
@@ -168,6 +279,25 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
             aps()
         ms = timed(hip, st, aps, steps) / steps
         out[f"apply_stored_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3)}
+    # VALU use. Executed flops: kKBar as counted above; kElemApply per point of a macro-cell 24 x 4 FMA to
+    # assemble the stencil from the incident elements' integrals + 15 FMA to apply it (222 flops). The kKBar time
+    # is the on-the-fly apply minus the stored one (the rest is the same launches). The reference's work per apply
+    # (p1_div_k_grad_affine_q3.hpp:85-95: integrateAll 309 adds + 463 muls + 2 divs per micro-element, plus a 4 x 4
+    # gather-multiply-scatter, 32 flops) is reported as an equivalent rate.
+    pts = int(mesh.info()[3]) * ((1 << L) + 1) * ((1 << L) + 2) * ((1 << L) + 3) // 6
+    micro = int(mesh.info()[3]) * (1 << L) ** 3
+    valu = {}
+    for name, peak in (("f64", 78.6e12), ("f32", 157.3e12)):
+        t_on, t_st = out[f"apply_{name}"]["ms"] / 1e3, out[f"apply_stored_{name}"]["ms"] / 1e3
+        t_kbar = max(t_on - t_st, 1e-9)
+        valu[name] = {"kKBar_tflops": round(out["kbar_flops_per_apply"] / t_kbar / 1e12, 2),
+                      "kKBar_valu_frac": round(out["kbar_flops_per_apply"] / t_kbar / peak, 4),
+                      "stored_apply_tflops": round(222.0 * pts / t_st / 1e12, 2),
+                      "stored_apply_valu_frac": round(222.0 * pts / t_st / peak, 4),
+                      "reference_equivalent_gflops_on_the_fly": round((774 + 32) * micro / t_on / 1e9, 1)}
+    out["valu"] = valu
+    out["valu_note"] = ("peaks: FP64 78.6, FP32 157.3 TFLOP/s (dense vector); executed flops of this build's kernels; "
+                        "reference_equivalent = the reference's per-element flop count x micro-elements / time")
     b = hyg.Function(mesh, 0, L)
     bv = rng.uniform(-1.0, 1.0, n)
     bv[bnd] = 0.0
@@ -193,6 +323,7 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
     tmx = time.perf_counter() - t0
     u64, umx = x64.download(L), xmx.download(L)
     out["solve"] = {"cycle": f"V(3,3) Chebyshev order 2, rho(D^-1 A) = {rho:.5f}",
+                    "field": "std 0.5 (the amplitude-1 field diverges with rediscretised coarse operators)",
                     "fp64": {"seconds": round(t64, 4), "cycles": len(r64) - 1, "rel_residual": float(r64[-1] / r64[0])},
                     "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "inner_cycles": 1, "rel_residual": float(rmx[-1] / rmx[0]),
                               "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
@@ -213,6 +344,130 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
                            "mixed": {"seconds": round(tmx, 4), "outer_steps": len(rmx) - 1, "inner_cycles": 1,
                                      "rel_residual": float(rmx[-1] / rmx[0])},
                            "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
+    return out
+
+
+def bench_batched(hyg, hip, st, steps, nprob, seed):
+    """Independent problems solved together (hyg_batch_*): nprob problems drawn from one seed, each on the 6-tet
+    (Kuhn) or the 24-tet cube at a level drawn from {4, 5, 6, 7}, with the constant Laplacian or -div(k grad u), k
+    the 32-mode log-normal field of the variable-coefficient workload (its own seed per problem, local matrices
+    stored). Every grid operation of all problems is one persistent launch. Reported: the batched FP64 apply
+    (aggregate GDoF/s, roofline), the captured batched FP32 V-cycle and the host time of one graph launch, and the
+    time to 1e-10 of every problem: batched FP64 V(3,3) Chebyshev cycles against batched mixed precision (FP32
+    cycles inside FP64 defect correction), both captured, the residual checked after every 3 cycles in both; and
+    the same mixed solves run one problem after another without batching."""
+    rng = np.random.default_rng(seed)
+    meshes = {name: hyg.Mesh(name) for name in ("cube_kuhn_6el.msh", "cube_24el.msh")}
+    every = 3
+    probs = []
+    for i in range(nprob):
+        L = int(rng.integers(4, 8))
+        name = ("cube_kuhn_6el.msh", "cube_24el.msh")[int(rng.integers(0, 2))]
+        field = bool(rng.integers(0, 2))
+        mesh = meshes[name]
+        if field:
+            op = hyg.P1DivKGradOperator(mesh, 0, L, 1.0, hyg.fourier_field(seed * 1000 + i, modes=32, kmax=8, std=0.5))
+            op.store_local_matrices()
+        else:
+            op = hyg.P1ConstantOperator(mesh, 0, L)
+        Lr = min(L, 5)
+        pv, pt = hyg.Function(mesh, Lr, Lr), hyg.Function(mesh, Lr, Lr)
+        pv.upload(Lr, np.random.default_rng(5).uniform(-1, 1, mesh.ndofs(Lr)))
+        rho = op.power_iteration(pv, pt, 40, Lr)
+        kw = dict(pre=3, post=3, smoother="cheb", cheb_order=2, cheb_bounds=(0.3 * rho, 1.2 * rho))
+        n = mesh.ndofs(L)
+        bnd = mesh.boundary(L)
+        bv = rng.uniform(-1.0, 1.0, n)
+        bv[bnd] = 0.0
+        p = dict(L=L, mesh=name, field=field, op=op, inner=int((~bnd).sum()), bv=bv,
+                 g64=hyg.GMG(op, 0, L, hyg.F64, **kw), g32=hyg.GMG(op, 0, L, hyg.F32, **kw),
+                 xf=hyg.Function(mesh, 0, L), bf=hyg.Function(mesh, 0, L),
+                 x32=hyg.Function(mesh, 0, L, hyg.F32), b32=hyg.Function(mesh, 0, L, hyg.F32),
+                 xt=hyg.Function(mesh, L, L), bt=hyg.Function(mesh, L, L),
+                 src=hyg.Function(mesh, L, L), dst=hyg.Function(mesh, L, L))
+        p["bf"].upload(L, bv)
+        p["bt"].upload(L, bv)
+        p["src"].upload(L, rng.uniform(-1.0, 1.0, n))
+        probs.append(p)
+    inner = sum(p["inner"] for p in probs)
+    # tiles: the cell-kernel configuration of every (mesh, level, value type, mode), tuned on the mesh; each
+    # batched sweep takes the one of its largest problem
+    tuned = {}
+    for name, mesh in meshes.items():
+        opt = hyg.P1ConstantOperator(mesh, 4, 7)
+        for lev in range(4, 8):
+            for dt, dn in ((hyg.F64, "f64"), (hyg.F32, "f32")):
+                tuned[f"{name[:-4]}_L{lev}_{dn}"] = opt.tune(lev, dt, reps=3, stream=st)
+    b64 = hyg.Batch([p["g64"] for p in probs], [p["xf"] for p in probs], [p["bf"] for p in probs])
+    b32 = hyg.Batch([p["g32"] for p in probs], [p["x32"] for p in probs], [p["b32"] for p in probs])
+    out = {"workload": f"{nprob} independent P1 problems (seed {seed}): levels from {{4,5,6,7}}, 6-tet Kuhn or 24-tet cube, "
+                       "constant or 32-mode log-normal k (std 0.5, stored local matrices)",
+           "problems": {"levels": [p["L"] for p in probs], "cube24": sum(p["mesh"] == "cube_24el.msh" for p in probs),
+                        "variable_k": sum(p["field"] for p in probs)},
+           "dofs_inner_total": inner, "cell_config": "per batched sweep, tuned for its largest problem", "tuned": tuned}
+    # batched FP64 apply: all problems in one sweep launch and one combine launch
+    srcs, dsts = [p["src"] for p in probs], [p["dst"] for p in probs]
+    b64.apply(srcs, dsts, hyg.INNER, st)
+    ms = timed(hip, st, lambda: b64.apply(srcs, dsts, hyg.INNER, st), steps) / steps
+    gbs = 16.0 * inner / (ms / 1e3) / 1e9
+    out["apply_f64"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3),
+                        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                     "frac": round(gbs / PEAK_HBM_GBS, 4)}}
+    # captured batched FP32 cycle: device time per replay and host time to launch the graph
+    ms_graph = timed(hip, st, lambda: b32.vcycle(True, st), 5) / 5
+    ms_eager = timed(hip, st, lambda: b32.vcycle(False, st), 5) / 5
+    hip.sync(st)
+    t0 = time.perf_counter()
+    b32.vcycle(True, st)
+    launch_us = (time.perf_counter() - t0) * 1e6
+    hip.sync(st)
+    out["vcycle_f32"] = {"graph_ms": round(ms_graph, 3), "eager_batched_ms": round(ms_eager, 3),
+                         "graph_launch_host_us": round(launch_us, 1)}
+    # time to 1e-10 of every problem
+    for p in probs:
+        p["xf"].interpolate(0.0, p["L"])
+    b64.vcycle(True, st)
+    b64.residual_norms(st)
+    b32.ir_solve([p["xt"] for p in probs], [p["bt"] for p in probs], inner=every, max_outer=1, rel_tol=1e-10,
+                 use_graph=True, stream=st)
+    for p in probs:
+        p["xf"].interpolate(0.0, p["L"])
+        p["xt"].interpolate(0.0, p["L"])
+    hip.sync(st)
+    t0 = time.perf_counter()
+    r0 = b64.residual_norms(st)
+    r, c64 = r0, 0
+    while np.any(r > 1e-10 * r0) and c64 < 60:
+        for _ in range(every):
+            b64.vcycle(True, st)
+        c64 += every
+        r = b64.residual_norms(st)
+    t64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hmx = b32.ir_solve([p["xt"] for p in probs], [p["bt"] for p in probs], inner=every, max_outer=20, rel_tol=1e-10,
+                       use_graph=True, stream=st)
+    tmx = time.perf_counter() - t0
+    err = max(float(np.abs(p["xt"].download(p["L"]) - p["xf"].download(p["L"])).max() /
+                    np.abs(p["xf"].download(p["L"])).max()) for p in probs)
+    # the same mixed solves one problem at a time (hyg_ir_solve, eager launches); a first call with no outer step
+    # creates each solver's defect-correction work functions outside the timed region
+    for p in probs:
+        p["xt"].interpolate(0.0, p["L"])
+        hyg.ir_solve(p["op"], p["g32"], p["xt"], p["bt"], p["L"], inner=every, max_outer=0, rel_tol=1e-10, stream=st)
+    hip.sync(st)
+    t0 = time.perf_counter()
+    for p in probs:
+        hyg.ir_solve(p["op"], p["g32"], p["xt"], p["bt"], p["L"], inner=every, max_outer=20, rel_tol=1e-10, stream=st)
+    tseq = time.perf_counter() - t0
+    out["solve"] = {"cycle": "V(3,3) Chebyshev order 2 per problem (bounds 0.3/1.2 x its power-iteration rho), coarse level 0",
+                    "residual_check_every_cycles": every,
+                    "fp64_batched_graph": {"seconds": round(t64, 4), "cycles": c64, "max_rel_residual": float(np.max(r / r0))},
+                    "mixed_batched_graph": {"seconds": round(tmx, 4), "outer_steps": hmx.shape[1] - 1, "inner_cycles": every,
+                                            "max_rel_residual": float(np.max(hmx[:, -1] / hmx[:, 0])),
+                                            "max_rel_error_vs_fp64": err},
+                    "mixed_one_problem_at_a_time_eager": {"seconds": round(tseq, 4)},
+                    "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
+                    "speedup_batched_vs_one_at_a_time": round(tseq / tmx, 3)}
     return out
 
 
@@ -306,7 +561,13 @@ def bench_shell(hyg, hip, st, steps, level, seed, solve_cycles):
 
         ap()
         ms = timed(hip, st, ap, steps) / steps
-        out[f"vector_apply_{name}"] = {"ms": round(ms, 3), "GDoF/s": round(3 * inner / (ms / 1e3) / 1e9, 3)}
+        peak = 78.6e12 if dt == hyg.F64 else 157.3e12
+        micro = int(mesh.info()[3]) * (1 << L) ** 3
+        out[f"vector_apply_{name}"] = {"ms": round(ms, 3), "GDoF/s": round(3 * inner / (ms / 1e3) / 1e9, 3),
+                                       # ~900 FMA per point and component (11-point element rows assembled per point)
+                                       "valu_frac": round(3 * inner * 1800.0 / (ms / 1e3) / peak, 4),
+                                       # the reference's integrateAll: 4118 adds + 5862 muls per P2 micro-element and component
+                                       "reference_equivalent_tflops": round(3 * micro * 9982.0 / (ms / 1e3) / 1e12, 2)}
         del xs, ys
     # the same operator with its per-point stencils stored (computeAndStoreLocalElementMatrices): the solves use it
     hip.sync(st)
@@ -373,8 +634,10 @@ def main():
     ap.add_argument("--no-p2", action="store_true", help="skip the P2 workload")
     ap.add_argument("--no-shell", action="store_true", help="skip the spherical-shell P2 vector Laplacian workload")
     ap.add_argument("--shell-level", type=int, default=5)
-    ap.add_argument("--shell-cycles", type=int, default=8, help="cycle cap of the shell solves (1e-10 needs ~60)")
+    ap.add_argument("--shell-cycles", type=int, default=80, help="cycle cap of the shell solves to 1e-10")
     ap.add_argument("--p2-level", type=int, default=7)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched independent problems")
+    ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--ir-inner", type=int, default=3, help="FP32 V-cycles per FP64 residual in the mixed solves")
     args = ap.parse_args()
     global IR_INNER
@@ -513,11 +776,17 @@ def main():
                       "rel_residual": float(resmx[-1] / resmx[0]),
                       "rel_error_vs_fp64": float(np.abs(umx - u64).max() / np.abs(u64).max())},
             "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
+            "note": "eager launches; FP64 residual norm after every FP64 cycle, mixed after every outer step",
         }
+        out["solve_graph"] = graph_solves(hyg, hip, st, op, mesh, L, bv, kw, IR_INNER)
 
     # ---------------- variable-coefficient workload
     if not args.no_divkgrad:
         out["divkgrad"] = bench_divkgrad(hyg, hip, st, max(3, args.steps // 10), args.divkgrad_level, 2024)
+
+    # ---------------- batched independent problems
+    if not args.no_batched:
+        out["batched"] = bench_batched(hyg, hip, st, max(3, args.steps // 10), args.batch_problems, 2027)
 
     # ---------------- P2 workload
     if not args.no_p2:

@@ -36,6 +36,7 @@ def _sig(lib):
         "hyg_last_error": (C.c_char_p, []),
         "hyg_init": (i, [i]),
         "hyg_mesh_create_from_msh": (i, [C.c_char_p, pvp]),
+        "hyg_mesh_create_from_msh_tags": (i, [C.c_char_p, i, pvp]),
         "hyg_mesh_create_spherical_shell": (i, [i, i, d, d, pvp]),
         "hyg_mesh_create_symmetric_cuboid": (i, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i, pvp]),
         "hyg_mesh_create_cuboid": (i, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i, pvp]),
@@ -55,6 +56,10 @@ def _sig(lib):
         "hyg_mult_elementwise": (i, [vp, vp, vp, i, i, vp]),
         "hyg_dot": (i, [vp, vp, i, i, C.POINTER(C.c_double), vp]),
         "hyg_dot_noalloc": (i, [vp, vp, i, i, vp, vp]),
+        "hyg_halo_layout": (i, [vp, i, P(np.int64)]),
+        "hyg_halo_pack": (i, [vp, i, vp, vp]),
+        "hyg_halo_unpack": (i, [vp, i, vp, vp]),
+        "hyg_halo_sum_replicas": (i, [vp, i, i, vp]),
         "hyg_batch_create": (i, [i, pvp, pvp, pvp, pvp]),
         "hyg_batch_destroy": (i, [vp]),
         "hyg_batch_set_config": (i, [vp, i]),
@@ -142,7 +147,8 @@ class Mesh:
     cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz) the six-tet cuboid ("cuboid", same arguments) or a cubed domain ("cubed_domain", integer cube positions,
     cube type 0 or 1)."""
 
-    def __init__(self, name, *args):
+    def __init__(self, name, *args, tags=False):
+        """tags=True (Gmsh files): the file's physical tags are the mesh boundary flags (create0123BC mapping)."""
         init()
         h = _out()
         if name == "shell":
@@ -162,7 +168,7 @@ class Mesh:
             check(load().hyg_mesh_create_cubed_domain(cb, len(cb) // 3, cube_type, C.byref(h)))
         else:
             self.path = name if os.path.isabs(name) else os.path.join(MESHES, name)
-            check(load().hyg_mesh_create_from_msh(self.path.encode(), C.byref(h)))
+            check(load().hyg_mesh_create_from_msh_tags(self.path.encode(), 1 if tags else 0, C.byref(h)))
         self.h = h
 
     def info(self):
@@ -398,6 +404,57 @@ def ir_solve(op64, gmg32, x, b, L, inner=1, max_outer=30, rel_tol=1e-10, stream=
     steps = C.c_int()
     check(load().hyg_ir_solve(op64.h, gmg32.h, x.h, b.h, L, inner, max_outer, rel_tol, res, C.byref(steps), stream))
     return res[: steps.value + 1]
+
+
+def _hipdev(nbytes):
+    """A device buffer (hipMalloc) for calls that take device pointers; freed by _hipfree."""
+    hip = C.CDLL("libamdhip64.so")
+    p = C.c_void_p()
+    if hip.hipMalloc(C.byref(p), C.c_size_t(max(nbytes, 8))) != 0:
+        raise HygError(ERR_HIP, "hipMalloc failed")
+    return hip, p
+
+
+def halo_layout(mesh, L):
+    """Element offsets of the reference's per-primitive memory with ghost layers (vertices, edges, faces)."""
+    nv, ne, nf = (int(v) for v in mesh.info()[:3])
+    off = np.zeros(nv + ne + nf + 1, dtype=np.int64)
+    check(load().hyg_halo_layout(mesh.h, L, off))
+    return off
+
+
+def halo_pack(f, L):
+    """The reference-layout memory of level L of a P1 function, copied to the host (float64)."""
+    n = int(halo_layout(f.mesh, L)[-1])
+    es = 8 if f.dtype == F64 else 4
+    hip, p = _hipdev(n * es)
+    try:
+        check(load().hyg_halo_pack(f.h, L, p, None))
+        out = np.zeros(n, dtype=np.float64 if es == 8 else np.float32)
+        hip.hipDeviceSynchronize()
+        if hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), p, C.c_size_t(n * es), 2) != 0:
+            raise HygError(ERR_HIP, "hipMemcpy failed")
+        return out.astype(np.float64)
+    finally:
+        hip.hipFree(p)
+
+
+def halo_unpack(f, L, mem):
+    """Own values of every primitive in `mem` (reference layout) into all replicas of the function."""
+    es = 8 if f.dtype == F64 else 4
+    a = np.ascontiguousarray(mem, dtype=np.float64 if es == 8 else np.float32)
+    hip, p = _hipdev(a.nbytes)
+    try:
+        if hip.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) != 0:
+            raise HygError(ERR_HIP, "hipMemcpy failed")
+        check(load().hyg_halo_unpack(f.h, L, p, None))
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(p)
+
+
+def halo_sum_replicas(f, L, flag=ALL):
+    check(load().hyg_halo_sum_replicas(f.h, L, flag, None))
 
 
 class Batch:

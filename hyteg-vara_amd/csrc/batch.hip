@@ -4,10 +4,11 @@
 // prolongation, a pointwise update, a dot product or a value-type conversion) applied to every problem of
 // the batch, each at its own level, on its own mesh, with its own operator and functions. The step's task
 // table holds one entry per problem (two for operations made of two independent kernels): the kernel
-// arguments of that problem and the range of virtual blocks it takes. A fixed grid of kBatchGrid blocks
-// walks the virtual blocks of all tasks (block b takes b, b + grid, ...), finds the task of each by binary
-// search on the range starts, and runs the same kernel body the single-problem launch runs with the same
-// block index, so every problem's arithmetic and order of operations are those of the unbatched call.
+// arguments of that problem and the range of virtual blocks it takes. Each block finds the task of its virtual
+// block by binary search on the range starts and runs the same kernel body the single-problem launch runs with
+// the same block index, so every problem's arithmetic and order of operations are those of the unbatched call.
+// The stencil sweeps launch one block per virtual block; the lighter operations run a fixed grid of kBatchGrid
+// blocks that walks the virtual blocks (block b takes b, b + grid, ...).
 #include "elementwise.hpp"
 #include "kernels_dev.hpp"
 #include "sweep.hpp"
@@ -41,13 +42,19 @@ struct ElemArgs
    T                   a;
 };
 
-// kind 0: constant-coefficient sweep (cell interiors, flat faces, edges and vertices); kind 1: elementwise operator
+// constant-coefficient sweep (cell interiors, flat faces, edges and vertices) of one problem
 template < typename T >
-struct StencilTask
+struct SweepTask
 {
    int            kind, first;
    SweepArgs< T > sw;
-   ElemArgs< T >  el;
+};
+// elementwise operator (stored coefficient integrals) of one problem
+template < typename T >
+struct ElemTask
+{
+   int           kind, first;
+   ElemArgs< T > el;
 };
 
 // kind 0: macro-face points from the one-sided sums; kind 1: macro-edge and -vertex points (elementwise operator)
@@ -138,22 +145,30 @@ __device__ __forceinline__ int findTask( const Task* __restrict__ t, int n, int 
    return lo;
 }
 
+// The two stencil kernels are separate launches (a batch with both operator kinds issues both): one kernel
+// holding both bodies needs the register budget of both at once (253 VGPRs against 128-214 and 120) and runs
+// at a quarter of the occupancy. Their grids take one block per work item of every problem, as the
+// single-problem launches do; the lighter operations below walk their work with a fixed grid.
 template < typename T, int MODE, int R, int P, int D >
-__global__ __launch_bounds__( kBlock ) void kBatchStencil( const StencilTask< T >* __restrict__ tasks, int nTasks, int total )
+__global__ __launch_bounds__( kBlock ) void kBatchSweep( const SweepTask< T >* __restrict__ tasks, int nTasks, int total )
 {
-   for ( int vb = int( blockIdx.x ); vb < total; vb += int( gridDim.x ) )
-   {
-      const StencilTask< T >& t = tasks[findTask( tasks, nTasks, vb )];
-      const int               b = vb - t.first;
-      if ( t.kind == 0 )
-         bSweep< T, MODE, R, P, D >( b, t.sw );
-      else
-      {
-         const ElemArgs< T >& e = t.el;
-         bElemApply< T, MODE, false >( b, e.L, e.tilesPerCell, e.stride, e.cells, e.kb, e.cinfo, e.cfs, e.prims, e.entries, e.part,
-                                       e.lowPart, e.dinv, e.flag, e.x, e.in1, e.out0, e.out1, e.a );
-      }
-   }
+   const int vb = int( blockIdx.x );
+   if ( vb >= total )
+      return;
+   const SweepTask< T >& t = tasks[findTask( tasks, nTasks, vb )];
+   bSweep< T, MODE, R, P, D >( vb - t.first, t.sw );
+}
+
+template < typename T, int MODE >
+__global__ __launch_bounds__( kBlock ) void kBatchElem( const ElemTask< T >* __restrict__ tasks, int nTasks, int total )
+{
+   const int vb = int( blockIdx.x );
+   if ( vb >= total )
+      return;
+   const ElemTask< T >& t = tasks[findTask( tasks, nTasks, vb )];
+   const ElemArgs< T >& e = t.el;
+   bElemApply< T, MODE, false >( vb - t.first, e.L, e.tilesPerCell, e.stride, e.cells, e.kb, e.cinfo, e.cfs, e.prims, e.entries, e.part,
+                                 e.lowPart, e.dinv, e.flag, e.x, e.in1, e.out0, e.out1, e.a );
 }
 
 template < typename T, int MODE >
@@ -288,11 +303,13 @@ template < typename T >
 void addStencilT( BatchPlan& plan, int mode, const std::vector< BatchPlan::Item >& items, int cfg )
 {
    constexpr bool   dbl = std::is_same< T, double >::value;
-   BatchPlan::Step  st{}, cb{};
+   BatchPlan::Step  st{}, el{}, cb{};
    st.phase = BatchPlan::STENCIL;
+   st.cfg   = cfg;
+   el.phase = BatchPlan::ELEM;
    cb.phase = BatchPlan::COMBINE;
-   st.mode = cb.mode = mode;
-   st.dtype = cb.dtype = dbl ? 8 : 4;
+   st.mode = el.mode = cb.mode = mode;
+   st.dtype = el.dtype = cb.dtype = dbl ? 8 : 4;
    for ( const auto& it : items )
    {
       const MeshCtx&          m  = *it.m;
@@ -302,7 +319,8 @@ void addStencilT( BatchPlan& plan, int mode, const std::vector< BatchPlan::Item 
       const StencilCall&      c  = it.c;
       const int               nC = m.mesh->nC(), N = 1 << it.L, w = N + 1;
       const int               nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
-      StencilTask< T >        t{};
+      SweepTask< T >          t{};
+      ElemTask< T >           u{};
       CombineTask< T >        f{};
       f.chunks        = g.chunks + g.firstFaceChunk;
       f.nChunks       = nFace;
@@ -331,8 +349,8 @@ void addStencilT( BatchPlan& plan, int mode, const std::vector< BatchPlan::Item 
       }
       else
       {
-         ElemArgs< T >& e = t.el;
-         t.kind           = 1;
+         ElemArgs< T >& e = u.el;
+         u.kind           = 1;
          e.L              = it.L;
          e.tilesPerCell   = ceilDiv( S3i( w ), kBlock );
          e.stride         = g.stride;
@@ -351,7 +369,7 @@ void addStencilT( BatchPlan& plan, int mode, const std::vector< BatchPlan::Item 
          e.out0           = (T*) c.out0;
          e.out1           = (T*) c.out1;
          e.a              = T( c.a );
-         appendTask( st, t, nC * e.tilesPerCell );
+         appendTask( el, u, nC * e.tilesPerCell );
          f.kind    = 0;
          f.part    = e.part;
          f.primInv = nullptr;
@@ -366,6 +384,7 @@ void addStencilT( BatchPlan& plan, int mode, const std::vector< BatchPlan::Item 
       }
    }
    plan.steps.push_back( std::move( st ) );
+   plan.steps.push_back( std::move( el ) );
    plan.steps.push_back( std::move( cb ) );
 }
 
@@ -473,32 +492,52 @@ void addVecT( BatchPlan& plan, const std::vector< BatchPlan::Vec >& items )
 template < typename T, int MODE, int R, int D >
 void launchStencilCfg( const BatchPlan::Step& s, hipStream_t st )
 {
-   const int grid = std::min( s.total, kBatchGrid );
-   hipLaunchKernelGGL( ( kBatchStencil< T, MODE, R, 1, D > ), dim3( grid ), dim3( kBlock ), 0, st, (const StencilTask< T >*) s.dev,
+   hipLaunchKernelGGL( ( kBatchSweep< T, MODE, R, 1, D > ), dim3( s.total ), dim3( kBlock ), 0, st, (const SweepTask< T >*) s.dev,
                        s.nTasks, s.total );
 }
 
+// the seven cell-interior configurations (rows per wavefront R, planes in flight D) of kSweep
 template < typename T, int MODE >
-void launchStencilMode( const BatchPlan::Step& s, int cfg, hipStream_t st )
+void launchStencilMode( const BatchPlan::Step& s, hipStream_t st )
 {
-   switch ( cfg )
+   switch ( s.cfg )
    {
    case 0: launchStencilCfg< T, MODE, 1, 2 >( s, st ); break;
+   case 1: launchStencilCfg< T, MODE, 2, 2 >( s, st ); break;
    case 2: launchStencilCfg< T, MODE, 4, 2 >( s, st ); break;
-   default: launchStencilCfg< T, MODE, 2, 2 >( s, st ); break;
+   case 3: launchStencilCfg< T, MODE, 1, 4 >( s, st ); break;
+   case 4: launchStencilCfg< T, MODE, 2, 4 >( s, st ); break;
+   case 5: launchStencilCfg< T, MODE, 4, 4 >( s, st ); break;
+   default: launchStencilCfg< T, MODE, 8, 2 >( s, st ); break;
    }
 }
 
 template < typename T >
-void launchStencilStep( const BatchPlan::Step& s, int cfg, hipStream_t st )
+void launchElemStep( const BatchPlan::Step& s, hipStream_t st )
+{
+   const dim3 g( s.total ), b( kBlock );
+   switch ( s.mode )
+   {
+   case MODE_APPLY: hipLaunchKernelGGL( ( kBatchElem< T, MODE_APPLY > ), g, b, 0, st, (const ElemTask< T >*) s.dev, s.nTasks, s.total ); break;
+   case MODE_RESIDUAL:
+      hipLaunchKernelGGL( ( kBatchElem< T, MODE_RESIDUAL > ), g, b, 0, st, (const ElemTask< T >*) s.dev, s.nTasks, s.total );
+      break;
+   case MODE_JACOBI: hipLaunchKernelGGL( ( kBatchElem< T, MODE_JACOBI > ), g, b, 0, st, (const ElemTask< T >*) s.dev, s.nTasks, s.total ); break;
+   case MODE_CHEB0: hipLaunchKernelGGL( ( kBatchElem< T, MODE_CHEB0 > ), g, b, 0, st, (const ElemTask< T >*) s.dev, s.nTasks, s.total ); break;
+   default: hipLaunchKernelGGL( ( kBatchElem< T, MODE_CHEBK > ), g, b, 0, st, (const ElemTask< T >*) s.dev, s.nTasks, s.total ); break;
+   }
+}
+
+template < typename T >
+void launchStencilStep( const BatchPlan::Step& s, hipStream_t st )
 {
    switch ( s.mode )
    {
-   case MODE_APPLY: launchStencilMode< T, MODE_APPLY >( s, cfg, st ); break;
-   case MODE_RESIDUAL: launchStencilMode< T, MODE_RESIDUAL >( s, cfg, st ); break;
-   case MODE_JACOBI: launchStencilMode< T, MODE_JACOBI >( s, cfg, st ); break;
-   case MODE_CHEB0: launchStencilMode< T, MODE_CHEB0 >( s, cfg, st ); break;
-   default: launchStencilMode< T, MODE_CHEBK >( s, cfg, st ); break;
+   case MODE_APPLY: launchStencilMode< T, MODE_APPLY >( s, st ); break;
+   case MODE_RESIDUAL: launchStencilMode< T, MODE_RESIDUAL >( s, st ); break;
+   case MODE_JACOBI: launchStencilMode< T, MODE_JACOBI >( s, st ); break;
+   case MODE_CHEB0: launchStencilMode< T, MODE_CHEB0 >( s, st ); break;
+   default: launchStencilMode< T, MODE_CHEBK >( s, st ); break;
    }
 }
 
@@ -526,7 +565,7 @@ bool BatchPlan::modeSupported( int mode )
    return mode == MODE_APPLY || mode == MODE_RESIDUAL || mode == MODE_JACOBI || mode == MODE_CHEB0 || mode == MODE_CHEBK;
 }
 
-void BatchPlan::stencil( int dtype, int mode, const std::vector< Item >& items )
+void BatchPlan::stencil( int dtype, int mode, const std::vector< Item >& items, int cfg )
 {
    if ( dtype == 8 )
       addStencilT< double >( *this, mode, items, cfg );
@@ -629,9 +668,15 @@ void BatchPlan::launch( hipStream_t st, size_t from, size_t to ) const
       {
       case STENCIL:
          if ( s.dtype == 8 )
-            launchStencilStep< double >( s, cfg, st );
+            launchStencilStep< double >( s, st );
          else
-            launchStencilStep< float >( s, cfg, st );
+            launchStencilStep< float >( s, st );
+         break;
+      case ELEM:
+         if ( s.dtype == 8 )
+            launchElemStep< double >( s, st );
+         else
+            launchElemStep< float >( s, st );
          break;
       case COMBINE:
          if ( s.dtype == 8 )

@@ -535,13 +535,15 @@ int hyg_init( int device )
    return HYG_OK;
 }
 
-int hyg_mesh_create_from_msh( const char* path, hyg_mesh* out )
+int hyg_mesh_create_from_msh( const char* path, hyg_mesh* out ) { return hyg_mesh_create_from_msh_tags( path, 0, out ); }
+
+int hyg_mesh_create_from_msh_tags( const char* path, int keep_tags, hyg_mesh* out )
 {
    HYG_TRY( checkInit() );
    if ( !path || !out )
       return fail( HYG_ERR_ARG, "null argument" );
    std::string err;
-   auto        m = readGmsh( path, err );
+   auto        m = readGmsh( path, err, keep_tags != 0 );
    if ( !m )
       return fail( HYG_ERR_IO, err );
    auto* h = new hyg_mesh_s;
@@ -904,6 +906,68 @@ int hyg_dot_noalloc( hyg_func a, hyg_func b, int level, int flag, double* d_out,
    // every block writes its own partial sum, so nothing needs clearing between calls
    launchDot( mc, g, level, a->dtype, a->data[level], b->data[level], flag, part, d_out, S( stream ) );
    return launched( "dot_noalloc" );
+}
+
+int hyg_halo_layout( hyg_mesh mesh, int level, int64_t* offsets )
+{
+   if ( !mesh || !offsets || level < 0 || level > 10 )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   const auto off = haloLayout( *mesh->ctx.mesh, level );
+   std::copy( off.begin(), off.end(), offsets );
+   return HYG_OK;
+}
+
+namespace {
+int haloArgs( hyg_func f, int level, const void* buf )
+{
+   HYG_TRY( checkLevel( f, level ) );
+   if ( f->lat != 0 )
+      return fail( HYG_ERR_ARG, "reference-layout halos are defined for P1 functions" );
+   if ( !buf )
+      return fail( HYG_ERR_ARG, "null buffer" );
+   return HYG_OK;
+}
+const HaloMaps* haloOf( hyg_func f, int level )
+{
+   try
+   {
+      return &f->mesh->ctx.halo( level );
+   } catch ( const std::exception& e )
+   {
+      fail( HYG_ERR_HIP, std::string( "building the halo tables: " ) + e.what() );
+      return nullptr;
+   }
+}
+} // namespace
+
+int hyg_halo_pack( hyg_func f, int level, void* out, void* stream )
+{
+   HYG_TRY( haloArgs( f, level, out ) );
+   const HaloMaps* h = haloOf( f, level );
+   if ( !h )
+      return HYG_ERR_HIP;
+   launchHaloGather( *h, f->dtype, f->data[level], out, S( stream ) );
+   return launched( "halo pack" );
+}
+
+int hyg_halo_unpack( hyg_func f, int level, const void* in, void* stream )
+{
+   HYG_TRY( haloArgs( f, level, in ) );
+   const HaloMaps* h = haloOf( f, level );
+   if ( !h )
+      return HYG_ERR_HIP;
+   launchHaloScatter( *h, f->dtype, in, f->data[level], S( stream ) );
+   return launched( "halo unpack" );
+}
+
+int hyg_halo_sum_replicas( hyg_func f, int level, int flag, void* stream )
+{
+   HYG_TRY( haloArgs( f, level, f ) );
+   const HaloMaps* h = haloOf( f, level );
+   if ( !h )
+      return HYG_ERR_HIP;
+   launchHaloSum( *h, f->dtype, flag, f->data[level], S( stream ) );
+   return launched( "halo sum" );
 }
 
 int hyg_convert( hyg_func dst, hyg_func src, int level, void* stream )
@@ -1723,7 +1787,8 @@ struct hyg_batch_s
 {
    std::vector< hyg_gmg_s* >  g;
    std::vector< hyg_func_s* > x, b;
-   int                        dtype = 8, cfg = 1;
+   int                        dtype = 8;
+   int                        cfg   = -1; // forced cell-interior configuration of the sweeps, -1: per step (tuned)
    int                        depth = 0; // largest number of levels below the top
    std::unique_ptr< BatchPlan > cycle;  // one V-cycle of every problem
    std::unique_ptr< BatchPlan > norms;  // residual b - A x at the top level into the solvers' tmp, its dot
@@ -1737,6 +1802,7 @@ struct hyg_batch_s
    std::unique_ptr< BatchPlan > irPre, irPost; // residual + convert + zero; correction + residual + dot
    hipGraphExec_t             irExec  = nullptr;
    int                        irInner = 0;
+   size_t                     irNormStart = 0; // first step of irPost's residual norm
    // batched apply plans keyed by the creation numbers of (src, dst) of every problem and the flag
    std::vector< uint64_t >    applyKey;
    std::unique_ptr< BatchPlan > applyPlan;
@@ -1760,6 +1826,31 @@ struct hyg_batch_s
 namespace {
 
 int levelOf( const hyg_batch_s* bt, size_t i, int d ) { return bt->g[i]->maxL - d; }
+
+// cell-interior configuration of a batched sweep: the batch's forced one, else the one chosen for the largest
+// problem of the step (hyg_op_tune, HYG_CELL_CFG or the default) at its level, value type and mode
+int stepConfig( const hyg_batch_s* bt, const std::vector< BatchPlan::Item >& items, int dtype, int mode )
+{
+   if ( bt->cfg >= 0 )
+      return bt->cfg;
+   const BatchPlan::Item* big = nullptr;
+   i64                    most = -1;
+   for ( const auto& it : items )
+   {
+      const i64 n = i64( it.m->mesh->nC() ) * it.g->stride;
+      if ( it.op->kind == 0 && n > most )
+      {
+         most = n;
+         big  = &it;
+      }
+   }
+   return big ? cellConfig( *big->g, dtype, mode ) : 1;
+}
+
+void addStencil( hyg_batch_s* bt, BatchPlan& plan, int dtype, int mode, const std::vector< BatchPlan::Item >& items )
+{
+   plan.stencil( dtype, mode, items, stepConfig( bt, items, dtype, mode ) );
+}
 
 // inner DoFs of level L (cell interiors and points of primitives flagged Inner)
 i64 innerDofs( const MeshCtx& mc, int L )
@@ -1807,7 +1898,7 @@ void batchSmooth( hyg_batch_s* bt, BatchPlan& plan, const std::vector< size_t >&
          it.push_back( stencilItem( g, L, MODE_JACOBI, g->jt->data[L], bt->b[i]->data[L], bt->x[i]->data[L], nullptr, g->relax ) );
       }
       plan.vec( dt, cp );
-      plan.stencil( dt, MODE_JACOBI, it );
+      addStencil( bt, plan, dt, MODE_JACOBI, it );
       return;
    }
    const int K = bt->g[0]->chebOrder;
@@ -1819,7 +1910,7 @@ void batchSmooth( hyg_batch_s* bt, BatchPlan& plan, const std::vector< size_t >&
          const int  L = levelOf( bt, i, d );
          it.push_back( stencilItem( g, L, MODE_CHEB0, bt->x[i]->data[L], bt->b[i]->data[L], g->t1->data[L], g->cy->data[L], g->cheb[0] ) );
       }
-      plan.stencil( dt, MODE_CHEB0, it );
+      addStencil( bt, plan, dt, MODE_CHEB0, it );
    }
    bool swapped = false; // t1 -> t2 -> t1 ... as smoothImpl's tin / tout
    for ( int k = 1; k < K; ++k )
@@ -1834,7 +1925,7 @@ void batchSmooth( hyg_batch_s* bt, BatchPlan& plan, const std::vector< size_t >&
          it.push_back( stencilItem( g, L, MODE_CHEBK, tin->data[L], g->cy->data[L], tout->data[L],
                                     k == K - 1 ? bt->x[i]->data[L] : g->cy->data[L], g->cheb[k] ) );
       }
-      plan.stencil( dt, MODE_CHEBK, it );
+      addStencil( bt, plan, dt, MODE_CHEBK, it );
       swapped = !swapped;
    }
    if ( K == 1 )
@@ -1893,7 +1984,7 @@ void buildCycle( hyg_batch_s* bt, BatchPlan& plan )
          zx.dst = bt->x[i]->data[L - 1];
          vv.push_back( zx );
       }
-      plan.stencil( dt, MODE_RESIDUAL, res );
+      addStencil( bt, plan, dt, MODE_RESIDUAL, res );
       plan.restrictLevels( dt, rst );
       plan.vec( dt, vv );
    }
@@ -1938,7 +2029,7 @@ void buildNorms( hyg_batch_s* bt, BatchPlan& plan, int dt, const std::vector< hy
       fin.push_back( BatchPlan::DotFinal{ part, np, bt->results + i } );
       part += np;
    }
-   plan.stencil( dt, MODE_RESIDUAL, res );
+   addStencil( bt, plan, dt, MODE_RESIDUAL, res );
    plan.vec( dt, dots );
    plan.dotFinal( fin );
 }
@@ -1968,7 +2059,6 @@ int batchLaunchGraph( const BatchPlan& plan, hipGraphExec_t* exec, size_t from, 
 int batchPlanCycle( hyg_batch_s* bt )
 {
    bt->cycle.reset( new BatchPlan );
-   bt->cycle->cfg = bt->cfg;
    buildCycle( bt, *bt->cycle );
    HYG_HIP( bt->cycle->upload() );
    if ( bt->cycleExec )
@@ -1980,7 +2070,6 @@ int batchPlanCycle( hyg_batch_s* bt )
    for ( hyg_gmg_s* g : bt->g )
       tmps.push_back( g->tmp );
    bt->norms.reset( new BatchPlan );
-   bt->norms->cfg = bt->cfg;
    buildNorms( bt, *bt->norms, bt->dtype, bt->x, bt->b, tmps );
    HYG_HIP( bt->norms->upload() );
    bt->irKey.clear(); // rebuilt with the new configuration on the next defect-correction call
@@ -2069,8 +2158,9 @@ int hyg_batch_destroy( hyg_batch bt )
 
 int hyg_batch_set_config( hyg_batch bt, int config )
 {
-   if ( !bt || config < 0 || config > 2 )
-      return fail( HYG_ERR_ARG, "batch configuration must be 0, 1 or 2" );
+   if ( !bt || config < -1 || config >= kNumCellCfgs )
+      return fail( HYG_ERR_ARG, "batch configuration must be -1 (per level, as tuned) or a cell-kernel configuration 0.." +
+                                    std::to_string( kNumCellCfgs - 1 ) );
    if ( config == bt->cfg )
       return HYG_OK;
    bt->cfg = config;
@@ -2124,8 +2214,7 @@ int hyg_batch_apply( hyg_batch bt, const hyg_func* src, const hyg_func* dst, int
          it.push_back( item );
       }
       bt->applyPlan.reset( new BatchPlan );
-      bt->applyPlan->cfg = bt->cfg;
-      bt->applyPlan->stencil( bt->dtype, MODE_APPLY, it );
+      addStencil( bt, *bt->applyPlan, bt->dtype, MODE_APPLY, it );
       HYG_HIP( bt->applyPlan->upload() );
       bt->applyKey = key;
    }
@@ -2169,7 +2258,6 @@ int hyg_batch_ir_solve( hyg_batch bt, const hyg_func* x64, const hyg_func* b64, 
       // before the cycles: b32 = (float) r64, x32 = 0; after: t64 = (double) x32, x64 += t64, r64 = b64 - A x64, r64 . r64
       bt->irPre.reset( new BatchPlan );
       bt->irPost.reset( new BatchPlan );
-      bt->irPre->cfg = bt->irPost->cfg = bt->cfg;
       std::vector< BatchPlan::Convert > toF, toD;
       std::vector< BatchPlan::Vec >     zero, add;
       for ( size_t i = 0; i < n; ++i )
@@ -2194,6 +2282,7 @@ int hyg_batch_ir_solve( hyg_batch bt, const hyg_func* x64, const hyg_func* b64, 
       bt->irPre->vec( HYG_F32, zero );
       bt->irPost->convert( HYG_F64, HYG_F32, toD );
       bt->irPost->vec( HYG_F64, add );
+      bt->irNormStart = bt->irPost->steps.size();
       buildNorms( bt, *bt->irPost, HYG_F64, xs, bs, bt->r64 );
       HYG_HIP( bt->irPre->upload() );
       HYG_HIP( bt->irPost->upload() );
@@ -2205,9 +2294,8 @@ int hyg_batch_ir_solve( hyg_batch bt, const hyg_func* x64, const hyg_func* b64, 
       bt->irKey   = key;
       bt->irInner = inner;
    }
-   // initial residual: the last four steps of irPost (residual sweep and combine, dot partial sums, final sums)
-   const size_t nPost = bt->irPost->steps.size();
-   bt->irPost->launch( st, nPost - 4, nPost );
+   // initial residual: the norm steps at the end of irPost (residual, dot partial sums, final sums)
+   bt->irPost->launch( st, bt->irNormStart );
    HYG_TRY( launched( "batched residual" ) );
    std::vector< double > r( n );
    HYG_TRY( batchReadNorms( bt, r.data(), st ) );

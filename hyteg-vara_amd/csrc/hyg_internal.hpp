@@ -461,14 +461,17 @@ struct Mesh
    std::vector< std::array< int, 4 > >    cF;
    std::vector< std::array< int, 6 > >    cE;
    std::vector< std::vector< int > >      vC, eC, fC;
-   std::vector< uint8_t >                 vB, eB, fB;
+   std::vector< uint8_t >                 vB, eB, fB; // mesh boundary flags (0 inner, 1 Dirichlet, 2 Neumann, 3 free-slip)
    int nV() const { return (int) X.size(); }
    int nE() const { return (int) E.size(); }
    int nF() const { return (int) F.size(); }
    int nC() const { return (int) Cv.size(); }
 };
 
-std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err );
+std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err, bool keepTags = false );
+// DoF type of a mesh boundary flag under BoundaryCondition::create0123BC (boundary/BoundaryConditions.cpp:26-33):
+// 1 Dirichlet, 2 Neumann, 3 free-slip, every other flag Inner
+inline int bcOfFlag( int flag ) { return flag == 1 ? 2 : flag == 2 ? 4 : flag == 3 ? 8 : 1; }
 std::unique_ptr< Mesh > meshSphericalShell( int ntan, int nrad, double rmin, double rmax, std::string& err );
 std::unique_ptr< Mesh > meshSymmetricCuboid( const double lo[3], const double hi[3], int nx, int ny, int nz, std::string& err );
 std::unique_ptr< Mesh > meshCuboid( const double lo[3], const double hi[3], int nx, int ny, int nz, std::string& err );
@@ -515,6 +518,26 @@ struct LevelGeom
    int                                      firstFaceChunk = 0; // chunks [firstFaceChunk, nChunks) are face rows
 };
 
+// Reference-layout halo tables of one level (csrc/halo.cpp): gather[k] = offset in a function's level array of element
+// k of the reference's lower-dimensional primitive memory (own values and ghost layers); for the nOwn own values,
+// ownSrc = their element index, replica[ownRow[j] .. ownRow[j+1]) the offsets of all replicas, ownPrimBc the DoF type;
+// sumOwn lists the own values of points inside their primitive (each point once: edge ends and face borders belong
+// to the vertices and edges)
+struct HaloMaps
+{
+   i64  size = 0;
+   int  nOwn = 0, nSum = 0;
+   int* sumOwn = nullptr;
+   i64* gather = nullptr;
+   int* ownRow = nullptr;
+   i64* ownSrc = nullptr;
+   i64* replica = nullptr;
+   int* ownPrimBc = nullptr;
+   ~HaloMaps();
+};
+// element offsets of the reference's per-primitive memory (vertices, edges, faces in storage order; size nV+nE+nF+1)
+std::vector< i64 > haloLayout( const Mesh& m, int L );
+
 struct MeshCtx
 {
    std::unique_ptr< Mesh >     mesh;
@@ -531,6 +554,8 @@ struct MeshCtx
    P2Tables*                   p2tab    = nullptr; // device copy, built on first use
    int                         nColours = 0;       // colours of the macro-vertex colouring (multi-colour SOR)
    std::vector< int >          vertexColours;      // colour of every macro-vertex
+   std::vector< std::unique_ptr< HaloMaps > > haloMaps; // per level, built on first use
+   const HaloMaps&             halo( int L );
    void                        buildTopology(); // prims, entries, cell info (host + device)
    LevelGeom&                  geom( int L );
    const P2Tables*             p2Tables();

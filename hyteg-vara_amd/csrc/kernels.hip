@@ -28,6 +28,87 @@ __global__ __launch_bounds__( kFinalBlock ) void kDotFinal( const double* __rest
       result[0] = v;
 }
 
+// ------------------------------------------------------------------ reference-layout halos
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kHaloGather( const i64* __restrict__ map, i64 n, const T* __restrict__ f, T* __restrict__ out )
+{
+   for ( i64 k = i64( blockIdx.x ) * kBlock + threadIdx.x; k < n; k += i64( gridDim.x ) * kBlock )
+      out[k] = f[map[k]];
+}
+
+// each point takes the own value of the primitive it lies inside (vertex, edge interior, face interior), as after the
+// reference's downward chain vertex -> edge -> face -> cell, where lower-dimensional values overwrite the borders
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kHaloScatter( const int* __restrict__ sumOwn, int nSum, const int* __restrict__ row,
+                                                          const i64* __restrict__ src, const i64* __restrict__ rep,
+                                                          const T* __restrict__ in, T* __restrict__ f )
+{
+   for ( int s = int( blockIdx.x ) * kBlock + int( threadIdx.x ); s < nSum; s += int( gridDim.x ) * kBlock )
+   {
+      const int j = sumOwn[s];
+      const T   v = in[src[j]];
+      for ( int r = row[j]; r < row[j + 1]; ++r )
+         f[rep[r]] = v;
+   }
+}
+
+// communicateAdditively<Cell, Face/Edge/Vertex> followed by the downward copy: each replica set holds per-cell
+// contributions, their sum becomes the value of every replica
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kHaloSum( const int* __restrict__ sumOwn, int nSum, const int* __restrict__ row,
+                                                      const i64* __restrict__ rep, const int* __restrict__ bc, int flag,
+                                                      T* __restrict__ f )
+{
+   for ( int s = int( blockIdx.x ) * kBlock + int( threadIdx.x ); s < nSum; s += int( gridDim.x ) * kBlock )
+   {
+      const int j = sumOwn[s];
+      if ( !( bc[j] & flag ) )
+         continue;
+      T v = T( 0 );
+      for ( int r = row[j]; r < row[j + 1]; ++r )
+         v += f[rep[r]];
+      for ( int r = row[j]; r < row[j + 1]; ++r )
+         f[rep[r]] = v;
+   }
+}
+
+void launchHaloGather( const HaloMaps& h, int dtype, const void* f, void* out, hipStream_t st )
+{
+   const int grid = int( std::min< i64 >( ( h.size + kBlock - 1 ) / kBlock, 16384 ) );
+   if ( h.size == 0 )
+      return;
+   if ( dtype == 8 )
+      hipLaunchKernelGGL( kHaloGather< double >, dim3( grid ), dim3( kBlock ), 0, st, h.gather, h.size, (const double*) f, (double*) out );
+   else
+      hipLaunchKernelGGL( kHaloGather< float >, dim3( grid ), dim3( kBlock ), 0, st, h.gather, h.size, (const float*) f, (float*) out );
+}
+
+void launchHaloScatter( const HaloMaps& h, int dtype, const void* in, void* f, hipStream_t st )
+{
+   const int grid = std::min( ( h.nSum + kBlock - 1 ) / kBlock, 16384 );
+   if ( h.nSum == 0 )
+      return;
+   if ( dtype == 8 )
+      hipLaunchKernelGGL( kHaloScatter< double >, dim3( grid ), dim3( kBlock ), 0, st, h.sumOwn, h.nSum, h.ownRow, h.ownSrc, h.replica,
+                          (const double*) in, (double*) f );
+   else
+      hipLaunchKernelGGL( kHaloScatter< float >, dim3( grid ), dim3( kBlock ), 0, st, h.sumOwn, h.nSum, h.ownRow, h.ownSrc, h.replica,
+                          (const float*) in, (float*) f );
+}
+
+void launchHaloSum( const HaloMaps& h, int dtype, int flag, void* f, hipStream_t st )
+{
+   const int grid = std::min( ( h.nSum + kBlock - 1 ) / kBlock, 16384 );
+   if ( h.nSum == 0 )
+      return;
+   if ( dtype == 8 )
+      hipLaunchKernelGGL( kHaloSum< double >, dim3( grid ), dim3( kBlock ), 0, st, h.sumOwn, h.nSum, h.ownRow, h.replica, h.ownPrimBc,
+                          flag, (double*) f );
+   else
+      hipLaunchKernelGGL( kHaloSum< float >, dim3( grid ), dim3( kBlock ), 0, st, h.sumOwn, h.nSum, h.ownRow, h.replica, h.ownPrimBc,
+                          flag, (float*) f );
+}
+
 // ------------------------------------------------------------------ launchers
 template < typename T >
 static void lincombT( const MeshCtx& m, const LevelGeom& g, int L, void* dst, int nsrc, const double* s, const void* const* src,

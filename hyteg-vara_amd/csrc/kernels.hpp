@@ -88,6 +88,12 @@ void launchRestrict( const MeshCtx& m, const LevelGeom& gc, int Lc, int dtype, v
 void launchProlongate( const MeshCtx& m, const LevelGeom& gf, int Lf, int dtype, void* fine, const void* coarse, int flag,
                        bool add, hipStream_t st );
 
+// reference-layout halos (HaloMaps): out[k] = f[gather[k]]; every replica of own value j = in[ownSrc[j]]; every replica
+// of own value j (DoF type in flag) = the sum of its replicas
+void launchHaloGather( const HaloMaps& h, int dtype, const void* f, void* out, hipStream_t st );
+void launchHaloScatter( const HaloMaps& h, int dtype, const void* in, void* f, hipStream_t st );
+void launchHaloSum( const HaloMaps& h, int dtype, int flag, void* f, hipStream_t st );
+
 // ------------------------------------------------------------------ batched problems
 // A sequence of batched grid operations (steps). Each step applies one operation to every problem of a batch
 // (each at its own level, mesh, operator and functions) in one persistent launch over a task table; the
@@ -97,7 +103,8 @@ struct BatchPlan
 {
    enum Phase : int
    {
-      STENCIL,
+      STENCIL, // constant-operator sweeps
+      ELEM,    // elementwise-operator applies
       COMBINE,
       TRANSFER,
       VEC,
@@ -107,6 +114,7 @@ struct BatchPlan
    struct Step
    {
       int                 phase = 0, mode = 0, dtype = 8, nTasks = 0, total = 0;
+      int                 cfg   = 1; // sweep steps: cell-interior configuration (index of kCellRows / kCellDepth)
       std::vector< char > host;
       void*               dev = nullptr;
    };
@@ -157,10 +165,10 @@ struct BatchPlan
 
    std::vector< Step > steps;
    void*               dev = nullptr;
-   int                 cfg = 1; // cell-interior configuration of the constant-operator sweeps (0, 1 or 2)
 
    static bool modeSupported( int mode );
-   void        stencil( int dtype, int mode, const std::vector< Item >& items ); // sweep step + combine step
+   // sweep step (constant operators, cell-interior configuration cfg), elementwise step, combine step
+   void        stencil( int dtype, int mode, const std::vector< Item >& items, int cfg );
    void        restrictLevels( int dtype, const std::vector< Transfer >& items );
    void        prolongate( int dtype, const std::vector< Transfer >& items );
    void        vec( int dtype, const std::vector< Vec >& items );

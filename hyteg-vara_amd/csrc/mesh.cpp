@@ -8,7 +8,9 @@
 // fixes the micro-tetrahedra of the uniform refinement, so it must follow the file exactly.
 // Boundary primitives follow setMeshBoundaryFlagsOnBoundary(1, 0, true)
 // (SetupPrimitiveStorage.cpp:781-792, 895-945): faces with one incident cell, and every
-// edge and vertex of such a face; cells are always inner.
+// edge and vertex of such a face; cells are always inner. With keepTags the mesh boundary flags
+// are the file's instead (MeshGenFromGmshFile.cpp:100-116, 160-200): a line or triangle element
+// gives its edge or face the element's first (physical) tag, every other primitive gets 0.
 #include "hyg_internal.hpp"
 
 #include <algorithm>
@@ -25,7 +27,13 @@ static const int kCellEdgeVerts[6][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 }, { 0, 3 }
 std::unique_ptr< Mesh > buildMesh( const std::map< long, std::array< double, 3 > >& nodes,
                                    const std::map< std::vector< long >, int >& cellKeys, const std::string& what, std::string& err );
 
-std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err )
+// mesh boundary flags of the file's tagged edges and faces (node id lists as written in the file)
+struct GmshTags
+{
+   std::map< std::vector< long >, int > edges, faces;
+};
+
+std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err, bool keepTags )
 {
    std::ifstream in( path );
    if ( !in )
@@ -72,14 +80,17 @@ std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err )
    long ne = 0;
    in >> ne;
    std::map< std::vector< long >, int > cellKeys; // ordered like the reference's CellContainer
+   GmshTags                             tags;
    for ( long k = 0; k < ne; ++k )
    {
-      long id, type, ntags;
+      long id, type, ntags, first = 0;
       in >> id >> type >> ntags;
       for ( long t = 0; t < ntags; ++t )
       {
          long tag;
          in >> tag;
+         if ( t == 0 )
+            first = tag;
       }
       int nv = type == 15 ? 1 : type == 1 ? 2 : type == 2 ? 3 : type == 4 ? 4 : -1;
       if ( nv < 0 || !in )
@@ -100,8 +111,60 @@ std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err )
             }
          cellKeys[vs] = 0;
       }
+      else if ( type == 1 )
+         tags.edges[vs] = int( first );
+      else if ( type == 2 )
+         tags.faces[vs] = int( first );
    }
-   return buildMesh( nodes, cellKeys, path, err );
+   auto m = buildMesh( nodes, cellKeys, path, err );
+   if ( m && keepTags )
+   {
+      // the file's flags replace the geometric rule; like MeshInfo::addEdge / addFace the first entry of a
+      // sorted vertex set (in the order of the parsed containers) keeps its flag
+      std::map< long, int > vid; // vertex index = rank of the node id, as in buildMesh
+      for ( auto& kv : nodes )
+      {
+         const int idx = int( vid.size() );
+         vid[kv.first] = idx;
+      }
+      std::map< std::array< int, 2 >, int > eIdx;
+      std::map< std::array< int, 3 >, int > fIdx;
+      for ( int e = 0; e < m->nE(); ++e )
+         eIdx[m->E[e]] = e;
+      for ( int f = 0; f < m->nF(); ++f )
+         fIdx[m->F[f]] = f;
+      m->vB.assign( m->nV(), 0 );
+      m->eB.assign( m->nE(), 0 );
+      m->fB.assign( m->nF(), 0 );
+      std::vector< bool > eSet( m->nE(), false ), fSet( m->nF(), false );
+      for ( auto& kv : tags.edges )
+      {
+         if ( !vid.count( kv.first[0] ) || !vid.count( kv.first[1] ) )
+            continue;
+         std::array< int, 2 > key{ vid[kv.first[0]], vid[kv.first[1]] };
+         std::sort( key.begin(), key.end() );
+         auto it = eIdx.find( key );
+         if ( it != eIdx.end() && !eSet[it->second] )
+         {
+            m->eB[it->second] = uint8_t( kv.second );
+            eSet[it->second]  = true;
+         }
+      }
+      for ( auto& kv : tags.faces )
+      {
+         if ( !vid.count( kv.first[0] ) || !vid.count( kv.first[1] ) || !vid.count( kv.first[2] ) )
+            continue;
+         std::array< int, 3 > key{ vid[kv.first[0]], vid[kv.first[1]], vid[kv.first[2]] };
+         std::sort( key.begin(), key.end() );
+         auto it = fIdx.find( key );
+         if ( it != fIdx.end() && !fSet[it->second] )
+         {
+            m->fB[it->second] = uint8_t( kv.second );
+            fSet[it->second]  = true;
+         }
+      }
+   }
+   return m;
 }
 
 // Primitives of a tetrahedral mesh given by vertex IDs -> coordinates and cells (vertex ID lists in the

@@ -137,7 +137,7 @@ void MeshCtx::buildTopology()
    auto addPrim = [&]( int kind, int bnd, const std::vector< int >& cells, const int* pv, int nv ) {
       LowerPrim p;
       p.kind   = kind;
-      p.bc     = bnd ? 2 : 1; // create0123BC: boundary flag 1 -> DirichletBoundary, 0 -> Inner
+      p.bc     = bcOfFlag( bnd ); // create0123BC: mesh flag -> DoF type
       p.nCells = (int) cells.size();
       p.entry0 = (int) entriesHost.size();
       primsHost.push_back( p );
@@ -218,7 +218,7 @@ void MeshCtx::buildTopology()
       for ( int k = 0; k < 15; ++k )
          ci.entry[k] = -1;
       auto setSlot = [&]( int slot, int prim, int bnd, const std::vector< int >& cells ) {
-         ci.bcword |= uint64_t( bnd ? 2 : 1 ) << ( 4 * slot );
+         ci.bcword |= uint64_t( bcOfFlag( bnd ) ) << ( 4 * slot );
          if ( cells.front() == c )
             ci.ownmask |= 1u << slot;
          ci.prim[slot]  = prim;

@@ -60,6 +60,12 @@ int hyg_init( int device );
  * (primitivestorage/SetupPrimitiveStorage.cpp:40-365, 781-792) and PrimitiveStorage.
  * Boundary primitives get mesh flag 1 = Dirichlet (boundary/BoundaryConditions.hpp create0123BC). */
 int hyg_mesh_create_from_msh( const char* path, hyg_mesh* out );
+/* MeshInfo::fromGmshFile + SetupPrimitiveStorage with the file's physical tags as mesh boundary flags
+ * (mesh/MeshGenFromGmshFile.cpp:100-116, 160-200; keep_tags = 1: no setMeshBoundaryFlagsOnBoundary): the edge of
+ * a line element and the face of a triangle element get the element's first tag, every other primitive 0; flags
+ * map to DoF types as BoundaryCondition::create0123BC (boundary/BoundaryConditions.cpp:26-33): 1 Dirichlet,
+ * 2 Neumann, 3 free-slip, others Inner. keep_tags = 0 is hyg_mesh_create_from_msh. */
+int hyg_mesh_create_from_msh_tags( const char* path, int keep_tags, hyg_mesh* out );
 /* Replaces MeshInfo::meshSphericalShell( ntan, nrad, rmin, rmax ) (mesh/MeshGenSphericalShell.cpp, default
  * SHELLMESH_CLASSIC: arc-midpoint refinement of the icosahedron's diamonds, ntan - 1 a power of two,
  * nrad >= 2 equidistant radii) followed by the same storage setup and boundary flags as above: the inner
@@ -126,6 +132,30 @@ int hyg_dot( hyg_func a, hyg_func b, int level, int flag, double* out, void* str
 int hyg_dot_noalloc( hyg_func a, hyg_func b, int level, int flag, double* d_out, void* stream );
 /* P1Function<T>::copyFrom(P1Function<U>) between value types (VertexDoFFunction.hpp:511-530, FunctionMemory.hpp:117) */
 int hyg_convert( hyg_func dst, hyg_func src, int level, void* stream );
+
+/* ---------------------------------------------------------------- halos in the reference's layout
+ * The library keeps macro-face, -edge and -vertex values as replicas in every incident cell array, so no halo
+ * exchange precedes an operator. These calls produce and consume the reference's per-primitive memory of one level
+ * instead (p1functionspace/VertexDoFMemory.hpp:44-64), for code that holds data in that layout or a multi-GPU
+ * extension that ships halos. Primitives in storage order (vertices, edges, faces); per primitive its own values,
+ * then its ghost layers as VertexDoFFunction::communicate fills them (VertexDoFPackInfo.hpp: edge -> vertex
+ * :204-245, face -> edge with the face's cell ghost layers :295-408, cell -> face :508-660):
+ *   vertex: own, then per incident edge the edge point next to the vertex;
+ *   edge:   own w = N+1 values from its lower-id vertex, per incident face the N points one row into the face,
+ *           per incident cell the N-1 points one step towards both other cell vertices;
+ *   face:   own w(w+1)/2 values, per incident cell (cell 0: VERTEX_TC layer, cell 1: VERTEX_BC) the N(N+1)/2 points
+ *           one layer into that cell.
+ * hyg_halo_layout writes nV+nE+nF+1 element offsets (the last is the total). hyg_halo_pack gathers that memory
+ * into `out` (device, the function's value type); hyg_halo_unpack writes the own values in `in` into all replicas
+ * (the downward communication vertex -> edge -> face -> cell, VertexDoFPackInfo.hpp:161-505: a point takes the value
+ * of the primitive it lies inside, so vertex values win over edge ends and edge values over face borders);
+ * hyg_halo_sum_replicas replaces every replica of a point on a macro-face, -edge or -vertex whose DoF type is in
+ * flag by the sum over its cells (communicateAdditively<Cell, Face/Edge/Vertex>, VertexDoFFunction.hpp:389-415,
+ * VertexDoFAdditivePackInfo.hpp:378-571, followed by the downward copy). P1 functions. */
+int hyg_halo_layout( hyg_mesh mesh, int level, int64_t* offsets );
+int hyg_halo_pack( hyg_func f, int level, void* out, void* stream );
+int hyg_halo_unpack( hyg_func f, int level, const void* in, void* stream );
+int hyg_halo_sum_replicas( hyg_func f, int level, int flag, void* stream );
 
 /* ---------------------------------------------------------------- operators
  * Replaces P1ConstantLaplaceOperator(storage, minLevel, maxLevel)
@@ -202,8 +232,9 @@ int hyg_inverse_diagonal( hyg_op op, hyg_func dst, int level, void* stream );
 int hyg_residual( hyg_op op, hyg_func r, hyg_func b, hyg_func x, int level, int flag, void* stream );
 
 /* Stencil kernel configurations (no counterpart in the reference, whose generated kernels have one
- * fixed loop nest). The cell-interior kernel marches z-planes with R rows per wavefront and P points
- * per lane; configuration k has R = {1,2,4,1,2,4}[k], P = {1,1,1,2,2,2}[k]. Modes: 0 apply,
+ * fixed loop nest). The cell-interior kernel marches z-planes with R rows per wavefront and D planes of
+ * loads in flight, one point per lane; configuration k has R = {1,2,4,1,2,4,8}[k], D = {2,2,2,4,4,4,2}[k]
+ * (hyg_op_get_config reports R as `rows` and 1 as `points`). Modes: 0 apply,
  * 1 apply-add, 2 residual, 3 Jacobi sweep, 4 first Chebyshev step, 5 later Chebyshev steps.
  * hyg_op_tune times every configuration for every mode at `level` on the operator's mesh (`reps`
  * sweeps each, on scratch functions it allocates and frees) and keeps the fastest; chosen[0..5]
@@ -273,8 +304,9 @@ int hyg_ir_solve( hyg_op op64, hyg_gmg g32, hyg_func x, hyg_func b, int level, i
  * levels without inner DoFs (the batched cycle has no coarse solve). The handles must outlive the batch. */
 int hyg_batch_create( int n, const hyg_gmg* solvers, const hyg_func* x, const hyg_func* b, hyg_batch* out );
 int hyg_batch_destroy( hyg_batch bt );
-/* Cell-interior kernel configuration of the constant-operator sweeps (0: 1 row, 1: 2 rows, 2: 4 rows per
- * wavefront); every batched launch uses one configuration. */
+/* Cell-interior kernel configuration of the constant-operator sweeps: -1 (the default) takes, for every batched
+ * sweep, the configuration chosen for its largest problem's level, value type and mode (hyg_op_tune, see
+ * hyg_op_set_config); 0..6 forces that configuration of hyg_op_set_config for every batched sweep. */
 int hyg_batch_set_config( hyg_batch bt, int config );
 /* One V-cycle of every problem; use_graph = 1 captures the batched cycle into a hipGraph on the first call (on
  * `stream`, which must not be the default stream) and replays it afterwards. */

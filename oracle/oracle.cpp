@@ -143,7 +143,10 @@ static const int CELL_EDGE_VERTS[6][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 }, { 0, 3 
 
 static std::unique_ptr< Mesh > buildMesh( const std::map< idx, P3 >& nodes, const std::set< std::vector< idx > >& cellsGmsh );
 
-static std::unique_ptr< Mesh > loadGmsh( const std::string& path )
+// keepTags: MeshInfo::fromGmshFile's own flags (MeshGenFromGmshFile.cpp:100-116, 160-200) instead of
+// setMeshBoundaryFlagsOnBoundary: the parsed line and triangle elements carry their first tag into the
+// EdgeContainer / FaceContainer (addEdge / addFace keep the first entry of a sorted vertex set), all else 0.
+static std::unique_ptr< Mesh > loadGmsh( const std::string& path, bool keepTags = false )
 {
    std::ifstream in( path );
    if ( !in )
@@ -167,6 +170,7 @@ static std::unique_ptr< Mesh > loadGmsh( const std::string& path )
    in >> ne;
    std::set< std::vector< idx > > cellsGmsh; // keyed like MeshInfo::CellContainer
    std::vector< std::vector< idx > > cellList;
+   std::map< std::vector< idx >, idx > parsedEdges, parsedFaces; // file-order vertex lists -> physical tag
    for ( idx i = 0; i < ne; ++i )
    {
       idx id, type, ntags;
@@ -182,8 +186,40 @@ static std::unique_ptr< Mesh > loadGmsh( const std::string& path )
          in >> v;
       if ( type == 4 )
          cellsGmsh.insert( vs );
+      else if ( type == 1 )
+         parsedEdges[vs] = ntags > 0 ? tags[0] : 0;
+      else if ( type == 2 )
+         parsedFaces[vs] = ntags > 0 ? tags[0] : 0;
    }
-   return buildMesh( nodes, cellsGmsh );
+   auto m = buildMesh( nodes, cellsGmsh );
+   if ( m && keepTags )
+   {
+      std::map< idx, int > rank;
+      int                  r = 0;
+      for ( auto& kv : nodes )
+         rank[kv.first] = r++;
+      std::fill( m->vB.begin(), m->vB.end(), 0 );
+      std::fill( m->eB.begin(), m->eB.end(), 0 );
+      std::fill( m->fB.begin(), m->fB.end(), 0 );
+      std::set< size_t > doneE, doneF;
+      for ( auto& [vs, tag] : parsedEdges )
+      {
+         std::array< int, 2 > key{ rank.at( vs[0] ), rank.at( vs[1] ) };
+         std::sort( key.begin(), key.end() );
+         for ( size_t e = 0; e < m->E.size(); ++e )
+            if ( m->E[e][0] == key[0] && m->E[e][1] == key[1] && doneE.insert( e ).second )
+               m->eB[e] = int( tag );
+      }
+      for ( auto& [vs, tag] : parsedFaces )
+      {
+         std::array< int, 3 > key{ rank.at( vs[0] ), rank.at( vs[1] ), rank.at( vs[2] ) };
+         std::sort( key.begin(), key.end() );
+         for ( size_t f = 0; f < m->F.size(); ++f )
+            if ( m->F[f][0] == key[0] && m->F[f][1] == key[1] && m->F[f][2] == key[2] && doneF.insert( f ).second )
+               m->fB[f] = int( tag );
+      }
+   }
+   return m;
 }
 
 // primitives of a MeshInfo given by its vertex container (ID -> coordinates) and cell container (vertex lists)
@@ -282,8 +318,18 @@ static std::unique_ptr< Mesh > buildMesh( const std::map< idx, P3 >& nodes, cons
    return m;
 }
 
-// create0123BC: mesh flag 1 -> Dirichlet, 0 -> Inner
-inline int bcType( int onBoundary ) { return onBoundary ? DirichletBoundary : Inner; }
+// BoundaryCondition::create0123BC (boundary/BoundaryConditions.cpp:26-33): mesh flag 1 Dirichlet, 2 Neumann,
+// 3 free-slip; any other flag takes the default, Inner
+inline int bcType( int meshFlag )
+{
+   switch ( meshFlag )
+   {
+   case 1: return DirichletBoundary;
+   case 2: return NeumannBoundary;
+   case 3: return FreeslipBoundary;
+   default: return Inner;
+   }
+}
 
 // Cell coordinates of a point of a lower-dimensional primitive given by global vertex ids pv[0..nv-1]
 // and barycentric weights bw[0..nv-1] (sum N); VertexDoFMacroCell / Cell local vertex maps.
@@ -2225,9 +2271,157 @@ static std::unique_ptr< Mesh > makeShell( idx ntan, idx nrad, double rmin, doubl
    return buildMesh( nodes, cells );
 }
 
+// ---------------------------------------------------------------- reference memory with ghost layers
+// The per-primitive vertex-DoF memory of one level (p1functionspace/VertexDoFMemory.hpp:44-64) as
+// VertexDoFFunction::communicate fills it: the owned arrays after the downward chain, the vertex ghost entries from
+// packEdgeForVertex (VertexDoFPackInfo.hpp:204-245), the edge ghost rows from packFaceForEdge / unpackEdgeFromFace
+// (:295-408: the face row next to the edge, then the face's cell ghost rows into the edge's cell slots), and the face
+// ghost layers from packCellForFace / unpackFaceFromCell (:508-565) with the cell boundary iterator of
+// indexing/MacroCellIndexing.cpp:160-300 (start at vertex0, offset towards the fourth vertex, rows along
+// vertex0 -> vertex1 wrapping along vertex0 -> vertex2).
+namespace halo {
+// cell-local unit step from local vertex a towards local vertex b in (x, y, z) lattice coordinates
+inline I3 step( int a, int b )
+{
+   static const I3 pos[4] = { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } };
+   return { pos[b][0] - pos[a][0], pos[b][1] - pos[a][1], pos[b][2] - pos[a][2] };
+}
+// points of CellBoundaryIterator( w, v0, v1, v2, offset ) in iteration order
+inline std::vector< I3 > cellBoundary( idx w, int v0, int v1, int v2, idx off )
+{
+   const idx N = w - 1;
+   int       v3 = 6 - v0 - v1 - v2;
+   I3        start{ 0, 0, 0 };
+   if ( v0 > 0 )
+      start[v0 - 1] = N;
+   const I3 toward3 = step( v0, v3 ), d1 = step( v0, v1 ), d2 = step( v0, v2 );
+   for ( int k = 0; k < 3; ++k )
+      start[k] += off * toward3[k];
+   std::vector< I3 > pts;
+   const idx         len = w - off;
+   for ( idx r = 0; r < len; ++r )
+      for ( idx q = 0; q + r < len; ++q )
+         pts.push_back( { start[0] + q * d1[0] + r * d2[0], start[1] + q * d1[1] + r * d2[1], start[2] + q * d1[2] + r * d2[2] } );
+   return pts;
+}
+// face-local (x, y) points one row into a face of width `width` from its local edge le (0: v0-v1, 1: v0-v2, 2: v1-v2),
+// walked from the given edge vertex end (fromFirst: from the pair's first vertex), `count` points
+inline std::vector< std::array< idx, 2 > > faceRow( idx width, int le, bool fromFirst, idx count )
+{
+   std::vector< std::array< idx, 2 > > pts;
+   const idx                           N = width - 1;
+   for ( idx i = 0; i < count; ++i )
+   {
+      const idx j = fromFirst ? i : count - 1 - i;
+      if ( le == 0 )
+         pts.push_back( { j, 1 } );
+      else if ( le == 1 )
+         pts.push_back( { 1, j } );
+      else
+         pts.push_back( { N - 1 - j, j } );
+   }
+   return pts;
+}
+} // namespace halo
+
+template < typename T >
+static void haloPack( const Function< T >& F, int L, double* out )
+{
+   F.commDown( L );
+   const Mesh& m = *F.m;
+   const idx   N = idx( 1 ) << L, w = N + 1;
+   const int   nV = int( m.X.size() ), nE = int( m.E.size() ), nF = int( m.F.size() );
+   std::vector< std::vector< int > > vEdges( nV ), eFaces( nE );
+   for ( int e = 0; e < nE; ++e )
+   {
+      vEdges[m.E[e][0]].push_back( e );
+      vEdges[m.E[e][1]].push_back( e );
+   }
+   auto edgeOf = [&]( int a, int b ) {
+      for ( int e = 0; e < nE; ++e )
+         if ( ( m.E[e][0] == a && m.E[e][1] == b ) || ( m.E[e][0] == b && m.E[e][1] == a ) )
+            return e;
+      return -1;
+   };
+   for ( int f = 0; f < nF; ++f )
+   {
+      const auto& t = m.F[f];
+      for ( int e : { edgeOf( t[0], t[1] ), edgeOf( t[0], t[2] ), edgeOf( t[1], t[2] ) } )
+         eFaces[e].push_back( f );
+   }
+   auto local = [&]( int c, int v ) {
+      for ( int l = 0; l < 4; ++l )
+         if ( m.Cl[c][l] == v )
+            return l;
+      return -1;
+   };
+   // the ghost triangle of face f from its cell c, as face memory (fidx(N, x, y) order)
+   auto faceGhost = [&]( int f, int c ) {
+      const auto& t   = m.F[f];
+      const auto  pts = halo::cellBoundary( w, local( c, t[0] ), local( c, t[1] ), local( c, t[2] ), 1 );
+      std::vector< double > g;
+      for ( const auto& q : pts )
+         g.push_back( double( F.c[L][c][cidx( w, q[0], q[1], q[2] )] ) );
+      return g;
+   };
+   size_t o = 0;
+   for ( int v = 0; v < nV; ++v )
+   {
+      out[o++] = double( F.v[L][v][0] );
+      for ( int e : vEdges[v] )
+         out[o++] = double( F.e[L][e][m.E[e][0] == v ? 1 : N - 1] );
+   }
+   for ( int e = 0; e < nE; ++e )
+   {
+      const size_t base = o;
+      for ( idx i = 0; i <= N; ++i )
+         out[o++] = double( F.e[L][e][i] );
+      std::vector< int > cells;
+      for ( size_t c = 0; c < m.Cl.size(); ++c )
+         for ( int k = 0; k < 6; ++k )
+            if ( m.cE[c][k] == e )
+               cells.push_back( int( c ) );
+      const size_t cellBase = base + size_t( w ) + eFaces[e].size() * size_t( w - 1 );
+      for ( int f : eFaces[e] )
+      {
+         const auto& t = m.F[f];
+         int         le = 0, first = t[0];
+         if ( edgeOf( t[0], t[1] ) == e )
+            le = 0, first = t[0];
+         else if ( edgeOf( t[0], t[2] ) == e )
+            le = 1, first = t[0];
+         else
+            le = 2, first = t[1];
+         const bool fromFirst = m.E[e][0] == first;
+         for ( const auto& q : halo::faceRow( w, le, fromFirst, N ) )
+            out[o++] = double( F.f[L][f][fidx( w, q[0], q[1] )] );
+         // the face's cell ghost rows go to the edge's slots of those cells
+         for ( size_t k = 0; k < m.fC[f].size(); ++k )
+         {
+            const int  c    = m.fC[f][k];
+            const auto g    = faceGhost( f, c );
+            const int  slot = int( std::find( cells.begin(), cells.end(), c ) - cells.begin() );
+            idx        i    = 0;
+            for ( const auto& q : halo::faceRow( N, le, fromFirst, N - 1 ) )
+               out[cellBase + size_t( slot ) * size_t( w - 2 ) + size_t( i++ )] = g[size_t( fidx( N, q[0], q[1] ) )];
+         }
+      }
+      o = cellBase + cells.size() * size_t( w - 2 );
+   }
+   for ( int f = 0; f < nF; ++f )
+   {
+      for ( size_t k = 0; k < F.f[L][f].size(); ++k )
+         out[o++] = double( F.f[L][f][k] );
+      for ( int c : m.fC[f] )
+         for ( double g : faceGhost( f, c ) )
+            out[o++] = g;
+   }
+}
+
 extern "C" {
 
 void* or_mesh_load( const char* path ) { return loadGmsh( path ).release(); }
+void* or_mesh_load_tags( const char* path ) { return loadGmsh( path, true ).release(); }
 void* or_mesh_shell( int ntan, int nrad, double rmin, double rmax ) { return makeShell( ntan, nrad, rmin, rmax ).release(); }
 // MeshInfo::meshCuboid (mesh/MeshGenCuboid.cpp:35-174): grid points numbered x fastest; every grid cube split into
 // six tetrahedra given by the reference's corner table (corners counter-clockwise in the bottom layer, then the
@@ -2438,6 +2632,26 @@ void* or_func_new( void* mh, int minL, int maxL, int prec )
    return h;
 }
 void or_func_free( void* h ) { delete static_cast< OrFunc* >( h ); }
+
+// edge and face vertex lists in storage order
+void or_mesh_prims( void* mh, int* edges, int* faces )
+{
+   const auto& m = *static_cast< Mesh* >( mh );
+   for ( size_t e = 0; e < m.E.size(); ++e )
+      edges[2 * e] = m.E[e][0], edges[2 * e + 1] = m.E[e][1];
+   for ( size_t f = 0; f < m.F.size(); ++f )
+      faces[3 * f] = m.F[f][0], faces[3 * f + 1] = m.F[f][1], faces[3 * f + 2] = m.F[f][2];
+}
+
+// the reference's lower-dimensional vertex-DoF memory of level L with ghost layers (vertices, edges, faces)
+void or_halo_pack( void* fh, int L, double* out )
+{
+   auto* h = static_cast< OrFunc* >( fh );
+   if ( h->prec == 4 )
+      haloPack( *h->f, L, out );
+   else
+      haloPack( *h->d, L, out );
+}
 
 void or_func_set( void* fh, int L, const double* canon )
 {

@@ -24,6 +24,9 @@ def lib():
         P = np.ctypeslib.ndpointer
         sig = {
             "or_mesh_load": (vp, [C.c_char_p]),
+            "or_mesh_load_tags": (vp, [C.c_char_p]),
+            "or_mesh_prims": (None, [vp, P(np.int32), P(np.int32)]),
+            "or_halo_pack": (None, [vp, i, P(np.float64)]),
             "or_mesh_shell": (vp, [i, i, d, d]),
             "or_mesh_symmetric_cuboid": (vp, [C.POINTER(C.c_double), C.POINTER(C.c_double), i, i, i]),
             "or_mesh_volume": (d, [vp]),
@@ -73,7 +76,7 @@ class Mesh:
     """A Gmsh v2.2 file (name or path), the spherical shell ("shell", ntan, nrad, rmin, rmax), the symmetric
     cuboid ("symmetric_cuboid", lower xyz, upper xyz, nx, ny, nz) or the six-tet cuboid ("cuboid", same arguments)."""
 
-    def __init__(self, name, *args):
+    def __init__(self, name, *args, tags=False):
         if name == "shell":
             self.path = None
             self.h = lib().or_mesh_shell(*args)
@@ -98,7 +101,7 @@ class Mesh:
             return
         path = name if os.path.isabs(name) else os.path.join(MESHES, name)
         self.path = path
-        self.h = lib().or_mesh_load(path.encode())
+        self.h = (lib().or_mesh_load_tags if tags else lib().or_mesh_load)(path.encode())
         if not self.h:
             raise RuntimeError("cannot load mesh " + path)
 
@@ -116,6 +119,14 @@ class Mesh:
         out = np.zeros(4 * nc, dtype=np.int32)
         lib().or_mesh_cells(self.h, out)
         return out.reshape(nc, 4)
+
+    def prims(self):
+        """Edge (nE, 2) and face (nF, 3) vertex lists in storage order."""
+        nv, ne, nf = (int(v) for v in self.info()[:3])
+        e = np.zeros(2 * ne, dtype=np.int32)
+        f = np.zeros(3 * nf, dtype=np.int32)
+        lib().or_mesh_prims(self.h, e, f)
+        return e.reshape(ne, 2), f.reshape(nf, 3)
 
     def volume(self):
         return lib().or_mesh_volume(self.h)

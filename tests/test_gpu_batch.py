@@ -2,7 +2,8 @@
 
 A batch mixes the 24-tet and the Kuhn 6-tet cube, levels 2..5, the constant Laplacian and -div(k grad u) with a
 seeded Fourier-mode coefficient (local matrices stored). Checked per problem:
-  * the batched FP64 apply against the oracle (1e-13 relative l-inf);
+  * the batched FP64 apply against the oracle (1e-13 relative l-inf), with the per-level tuned and with forced
+    cell-kernel configurations;
   * batched FP64 Chebyshev V(3,3) and Jacobi V(2,2) cycles: residual history against the oracle's GMG (1e-10
     relative per cycle), and the iterate bit for bit equal to the unbatched hyg_gmg_vcycle (the batched launch runs
     the same kernel bodies with the same decomposition);
@@ -58,7 +59,8 @@ def rhs(gm, L, seed):
     return b0
 
 
-def test_batched_apply_matches_the_oracle(hyg, meshes):
+@pytest.mark.parametrize("cfg", [-1, 0, 3, 6])
+def test_batched_apply_matches_the_oracle(hyg, meshes, cfg):
     ops, oops = make_ops(hyg, meshes)
     gs, xs, bs, srcs, dsts, data = [], [], [], [], [], []
     for k, ((name, L, _), op) in enumerate(zip(PROBLEMS, ops)):
@@ -73,8 +75,9 @@ def test_batched_apply_matches_the_oracle(hyg, meshes):
         dsts.append(d)
         data.append(v)
     bt = hyg.Batch(gs, xs, bs)
+    bt.set_config(cfg)
     for flag in (hyg.INNER, hyg.ALL):
-        print(f"batched apply, flag {flag}", flush=True)
+        print(f"batched apply, configuration {cfg}, flag {flag}", flush=True)
         bt.apply(srcs, dsts, flag)
         for k, ((name, L, field), oop) in enumerate(zip(PROBLEMS, oops)):
             _, om = meshes[name]
