@@ -70,6 +70,8 @@ def _sig(lib):
         "hyg_convert": (i, [vp, vp, i, vp]),
         "hyg_op_create_p1_const": (i, [vp, i, i, pvp]),
         "hyg_op_create_p1_divkgrad": (i, [vp, i, i, d, P(np.float64), i, pvp]),
+        "hyg_op_create_p1_divkgrad_poly": (i, [vp, i, i, i, P(np.float64), pvp]),
+        "hyg_op_interpolate_stencils": (i, [vp, i, i]),
         "hyg_op_create_p2_const": (i, [vp, i, i, pvp]),
         "hyg_op_create_p2_divkgrad": (i, [vp, i, i, i, d, d, d, C.c_uint64, pvp]),
         "hyg_op_store_local_matrices": (i, [vp]),
@@ -311,19 +313,28 @@ class P1ConstantOperator:
 
 
 class P1DivKGradOperator(P1ConstantOperator):
-    """-div(k grad u), k = k0 exp(sum_m a_m cos(2 pi kappa_m . x + phi_m)), modes: M x 5 array."""
+    """-div(k grad u), k = k0 exp(sum_m a_m cos(2 pi kappa_m . x + phi_m)), modes: M x 5 array; or, with
+    poly = coefficients of the monomials of degree <= poly_degree (MonomialBasis3D order), the polynomial k."""
 
-    def __init__(self, mesh, minL, maxL, k0=1.0, modes=None):
+    def __init__(self, mesh, minL, maxL, k0=1.0, modes=None, poly=None, poly_degree=None):
         self.mesh = mesh
-        md = np.zeros((1, 5)) if modes is None or len(modes) == 0 else np.ascontiguousarray(modes, dtype=np.float64)
-        nm = 0 if modes is None else len(modes)
         h = _out()
-        check(load().hyg_op_create_p1_divkgrad(mesh.h, minL, maxL, float(k0), md, nm, C.byref(h)))
+        if poly is not None:
+            c = np.ascontiguousarray(poly, dtype=np.float64)
+            check(load().hyg_op_create_p1_divkgrad_poly(mesh.h, minL, maxL, int(poly_degree), c, C.byref(h)))
+        else:
+            md = np.zeros((1, 5)) if modes is None or len(modes) == 0 else np.ascontiguousarray(modes, dtype=np.float64)
+            nm = 0 if modes is None else len(modes)
+            check(load().hyg_op_create_p1_divkgrad(mesh.h, minL, maxL, float(k0), md, nm, C.byref(h)))
         self.h = h
 
     def store_local_matrices(self):
         """Keep the coefficient integrals of every micro-element (computeAndStoreLocalElementMatrices)."""
         check(load().hyg_op_store_local_matrices(self.h))
+
+    def interpolate_stencils(self, degree, interpolation_level):
+        """P1SurrogateOperator::interpolateStencils: polynomial cell-interior stencils from then on."""
+        check(load().hyg_op_interpolate_stencils(self.h, degree, interpolation_level))
 
 
 class P2ConstantOperator(P1ConstantOperator):

@@ -1056,6 +1056,78 @@ int hyg_op_create_p1_divkgrad( hyg_mesh mesh, int min_level, int max_level, doub
    return HYG_OK;
 }
 
+int hyg_op_create_p1_divkgrad_poly( hyg_mesh mesh, int min_level, int max_level, int degree, const double* coeffs, hyg_op* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh || !out || min_level < 0 || max_level < min_level || max_level > 10 || degree < 0 || degree > 12 || !coeffs )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   const int nc = ( degree + 1 ) * ( degree + 2 ) * ( degree + 3 ) / 6;
+   for ( int i = 0; i < nc; ++i )
+      if ( !std::isfinite( coeffs[i] ) )
+         return fail( HYG_ERR_ARG, "non-finite polynomial coefficient" );
+   auto* op     = new hyg_op_s;
+   op->mesh     = mesh;
+   op->minL     = min_level;
+   op->maxL     = max_level;
+   op->kind     = 1;
+   op->kPolyDeg = degree;
+   op->kPolyHost.assign( coeffs, coeffs + nc );
+   try
+   {
+      for ( int L = min_level; L <= max_level; ++L )
+         mesh->ctx.geom( L );
+      assembleP1DivKGrad( *op );
+      for ( int L = min_level; L <= max_level; ++L )
+         launchElementwiseInverseDiagonal( mesh->ctx, mesh->ctx.geom( L ), *op, L, nullptr );
+   } catch ( const std::exception& e )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   if ( int rc = launched( "elementwise operator setup" ) )
+   {
+      delete op;
+      return rc;
+   }
+   if ( hipDeviceSynchronize() != hipSuccess )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, "elementwise operator setup failed" );
+   }
+   *out = op;
+   return HYG_OK;
+}
+
+int hyg_op_interpolate_stencils( hyg_op op, int degree, int interpolation_level )
+{
+   HYG_TRY( checkInit() );
+   if ( !op || op->kind != 1 )
+      return fail( HYG_ERR_ARG, "surrogate stencils are for the variable-coefficient P1 operator" );
+   if ( degree < 0 || degree > 12 )
+      return fail( HYG_ERR_ARG, "surrogate polynomial degree must be in 0..12" );
+   if ( interpolation_level < 2 )
+      return fail( HYG_ERR_ARG, "the interpolation level must have interior points (level >= 2)" );
+   MeshCtx& mc = op->mesh->ctx;
+   try
+   {
+      if ( !op->stored )
+         launchElementwiseStore( mc, *op, nullptr );
+      if ( int rc = launched( "coefficient integrals" ) )
+         return rc;
+      surrogateInterpolate( mc, *op, degree, interpolation_level );
+      for ( int L = op->minL; L <= op->maxL; ++L )
+         launchElementwiseInverseDiagonal( mc, mc.geom( L ), *op, L, nullptr );
+   } catch ( const std::exception& e )
+   {
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   if ( int rc = launched( "surrogate setup" ) )
+      return rc;
+   if ( hipDeviceSynchronize() != hipSuccess )
+      return fail( HYG_ERR_HIP, "surrogate setup failed" );
+   return HYG_OK;
+}
+
 int hyg_op_create_p2_const( hyg_mesh mesh, int min_level, int max_level, hyg_op* out )
 {
    HYG_TRY( checkInit() );
@@ -2117,6 +2189,8 @@ int hyg_batch_create( int n, const hyg_gmg* solvers, const hyg_func* x, const hy
          return fail( HYG_ERR_ARG, who + "batched cycles take P1 constant and P1 elementwise operators" );
       if ( g->op->kind == 1 && !g->op->stored )
          return fail( HYG_ERR_ARG, who + "an elementwise operator in a batch must keep its local matrices (hyg_op_store_local_matrices)" );
+      if ( g->op->surDegree >= 0 )
+         return fail( HYG_ERR_ARG, who + "surrogate operators are not batched" );
       if ( innerDofs( g->op->mesh->ctx, g->minL ) != 0 )
          return fail( HYG_ERR_ARG, who + "the coarse level has inner DoFs (batched cycles have no coarse solve)" );
    }

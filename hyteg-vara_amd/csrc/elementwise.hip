@@ -28,7 +28,9 @@ static void launchKBar( const hyg_op_s& op, int L, int nC, T* kb, hipStream_t s 
 {
    const int N  = 1 << L;
    const int tK = ceilDiv( S3i( N ), kBlock );
-   if ( tK > 0 && S3i( N ) > 0 )
+   if ( tK > 0 && S3i( N ) > 0 && op.kPolyDeg >= 0 )
+      hipLaunchKernelGGL( ( kKBarPoly< T > ), dim3( nC * tK ), dim3( kBlock ), 0, s, L, tK, op.levels[L].elem, op.kPoly, op.kPolyDeg, kb );
+   else if ( tK > 0 && S3i( N ) > 0 )
    {
       const size_t lds = size_t( op.nModes ) * ( 5 * sizeof( double ) + 60 * sizeof( T ) );
       hipLaunchKernelGGL( ( kKBar< T > ), dim3( nC * tK ), dim3( kBlock ), lds, s, L, tK, op.levels[L].elem, op.modes, op.nModes, op.k0,
@@ -65,10 +67,18 @@ static void elemT( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, co
    // 1. coefficient integrals of all micro-cells (kept from hyg_op_store_local_matrices when stored)
    if ( !op.stored )
       launchKBar< T >( op, L, nC, kb, s );
-   // 2. per-cell stencils (interior results, one-sided sums of face, edge and vertex points)
+   // 2. per-cell stencils (interior results, one-sided sums of face, edge and vertex points); the surrogate operator
+   //    does its interior points with the polynomial stencils
    const int tA = ceilDiv( S3i( w ), kBlock );
-   hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem, kb,
-                       m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ) );
+   if ( !DIAG && surrogateActive( op, L ) )
+   {
+      hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG, true > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem, kb,
+                          m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ) );
+      launchSurrogateInterior( m, g, op, L, std::is_same< T, double >::value ? 8 : 4, c, s );
+   }
+   else
+      hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem, kb,
+                          m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ) );
    // 3. face, edge and vertex values
    const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
    if ( nFace > 0 )
@@ -125,6 +135,8 @@ void launchElementwiseInverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg
    c.out0 = ol.dinv;
    elemT< double, MODE_APPLY, true >( m, g, op, L, c, s );
    hipLaunchKernelGGL( ( kInvertAll< double > ), dim3( 1024 ), dim3( kBlock ), 0, s, n, ol.dinv );
+   if ( surrogateActive( op, L ) )
+      launchSurrogateInverseDiagonal( m, g, op, L, s );
    launchConvert( n, 4, ol.dinvF, 8, ol.dinv, s );
 }
 

@@ -99,31 +99,69 @@ __global__ __launch_bounds__( kBlock ) void kKBar( int L, int tilesPerCell, cons
    }
 }
 
-// ------------------------------------------------------------------ 2. stencil application per cell
-// One thread per point of the closed macro-cell. DIAG: the diagonal entry instead of the product.
-template < typename T, int MODE, bool DIAG >
-__device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i64 stride, const ElemCell* __restrict__ cells,
-                                                        const T* __restrict__ kb, const CellInfo* __restrict__ cinfo,
-                                                        const CellFaceSlot* __restrict__ cfs, const LowerPrim* __restrict__ prims,
-                                                        const LowerEntry* __restrict__ entries, T* __restrict__ part,
-                                                        T* __restrict__ lowPart, const T* __restrict__ dinv, int flag,
-                                                        const T* __restrict__ x, const T* __restrict__ in1,
-                                                        T* __restrict__ out0, T* __restrict__ out1, T a )
+// The same integrals for a polynomial coefficient k(x) = sum_i poly[i] x^a y^b z^c over the monomials of degree <= deg
+// in the order of the reference's MonomialBasis3D (by total degree; within a degree by decreasing a, then b). One
+// thread per anchor; the polynomial is evaluated at the physical position of each of the 30 quadrature points.
+template < typename T >
+__global__ __launch_bounds__( kBlock ) void kKBarPoly( int L, int tilesPerCell, const ElemCell* __restrict__ cells,
+                                                       const double* __restrict__ poly, int deg, T* __restrict__ kb )
 {
-   const int c    = bid / tilesPerCell;
-   const int tile = bid - c * tilesPerCell;
-   const int N = 1 << L, w = N + 1, NP = N + 1;
-   const int s = tile * kBlock + int( threadIdx.x );
-   if ( s >= S3i( w ) )
+   const int c    = int( blockIdx.x ) / tilesPerCell;
+   const int tile = int( blockIdx.x ) - c * tilesPerCell;
+   const int N    = 1 << L;
+   const int S    = S3i( N );
+   const int s    = tile * kBlock + int( threadIdx.x );
+   if ( s >= S )
       return;
-   int px, py, pz;
-   tetXYZ( w, s, px, py, pz );
-   const int slot = classifySlot( N, px, py, pz );
-   if ( slot == 0 && !( flag & 1 ) )
-      return;
-   const ElemCell& ec = cells[c];
-   const int       SN = S3i( N );
-   const T*        kc = kb + i64( c ) * 6 * SN;
+   int x, y, z;
+   tetXYZ( N, s, x, y, z );
+   const ElemCell& ec  = cells[c];
+   T*              out = kb + i64( c ) * 6 * S;
+   for ( int t = 0; t < 6; ++t )
+   {
+      if ( x + y + z + kMCMax[t] > N )
+         continue;
+      double v = 0.0;
+      for ( int q = 0; q < 5; ++q )
+      {
+         double d[3] = { double( x ), double( y ), double( z ) };
+         for ( int u = 0; u < 4; ++u )
+         {
+            const double lam = q == 0 ? 0.25 : ( u == q - 1 ? 0.5 : 1.0 / 6.0 );
+            for ( int k = 0; k < 3; ++k )
+               d[k] += lam * kMC[t][u][k];
+         }
+         double X[3];
+         for ( int r = 0; r < 3; ++r )
+            X[r] = ec.v0[r] + ec.a[r][0] * d[0] + ec.a[r][1] * d[1] + ec.a[r][2] * d[2];
+         double kq = 0.0;
+         int    i  = 0;
+         for ( int dg = 0; dg <= deg; ++dg )
+            for ( int ea = dg; ea >= 0; --ea )
+               for ( int eb = dg - ea; eb >= 0; --eb, ++i )
+               {
+                  double mv = 1.0;
+                  for ( int e = 0; e < ea; ++e )
+                     mv *= X[0];
+                  for ( int e = 0; e < eb; ++e )
+                     mv *= X[1];
+                  for ( int e = 0; e < dg - ea - eb; ++e )
+                     mv *= X[2];
+                  kq += poly[i] * mv;
+               }
+         v += kQW6[q] * kq;
+      }
+      out[t * S + s] = T( v );
+   }
+}
+
+// ------------------------------------------------------------------ 2. stencil application per cell
+// Stencil weights W[d] (C W E N S NW SE TC TW TS TSE BC BN BE BNW) of lattice point (px, py, pz) of a macro-cell
+// from the coefficient integrals kc of its micro-cells (the incident ones that lie in the cell); DIAG: W[0] only.
+template < typename T, bool DIAG >
+__device__ __forceinline__ void elemWeights( const ElemCell& ec, const T* __restrict__ kc, int N, int px, int py, int pz, T W[NDIR] )
+{
+   const int SN = S3i( N );
    // anchors p - o for the eight offsets o in {0,1}^3 (encoded x + 2y + 4z)
    int  aIdx[8];
    bool aOk[8];
@@ -134,7 +172,6 @@ __device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i6
       aOk[o]       = ax >= 0 && ay >= 0 && az >= 0;
       aIdx[o]      = aOk[o] ? cidx( N, ax, ay, az ) : 0;
    }
-   T W[NDIR];
 #pragma unroll
    for ( int d = 0; d < NDIR; ++d )
       W[d] = T( 0 );
@@ -154,6 +191,34 @@ __device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i6
             for ( int j = 0; j < 4; ++j )
                W[kMCDir[t][k][j]] += kv * T( ec.mt[t][4 * k + j] );
       }
+}
+
+// One thread per point of the closed macro-cell. DIAG: the diagonal entry instead of the product. BND: points on the
+// cell's faces, edges and vertices only (the surrogate operator does the interior points itself).
+template < typename T, int MODE, bool DIAG, bool BND = false >
+__device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i64 stride, const ElemCell* __restrict__ cells,
+                                                        const T* __restrict__ kb, const CellInfo* __restrict__ cinfo,
+                                                        const CellFaceSlot* __restrict__ cfs, const LowerPrim* __restrict__ prims,
+                                                        const LowerEntry* __restrict__ entries, T* __restrict__ part,
+                                                        T* __restrict__ lowPart, const T* __restrict__ dinv, int flag,
+                                                        const T* __restrict__ x, const T* __restrict__ in1,
+                                                        T* __restrict__ out0, T* __restrict__ out1, T a )
+{
+   const int c    = bid / tilesPerCell;
+   const int tile = bid - c * tilesPerCell;
+   const int N = 1 << L, w = N + 1, NP = N + 1;
+   const int s = tile * kBlock + int( threadIdx.x );
+   if ( s >= S3i( w ) )
+      return;
+   int px, py, pz;
+   tetXYZ( w, s, px, py, pz );
+   const int slot = classifySlot( N, px, py, pz );
+   if ( slot == 0 && ( BND || !( flag & 1 ) ) )
+      return;
+   const ElemCell& ec = cells[c];
+   const T*        kc = kb + i64( c ) * 6 * S3i( N );
+   T               W[NDIR];
+   elemWeights< T, DIAG >( ec, kc, N, px, py, pz, W );
    const i64 q = i64( c ) * stride + cidx( w, px, py, pz );
    T         val;
    if constexpr ( DIAG )
@@ -208,7 +273,7 @@ __device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i6
    }
 }
 
-template < typename T, int MODE, bool DIAG >
+template < typename T, int MODE, bool DIAG, bool BND = false >
 __global__ __launch_bounds__( kBlock ) void kElemApply( int L, int tilesPerCell, i64 stride, const ElemCell* __restrict__ cells,
                                                         const T* __restrict__ kb, const CellInfo* __restrict__ cinfo,
                                                         const CellFaceSlot* __restrict__ cfs, const LowerPrim* __restrict__ prims,
@@ -217,7 +282,7 @@ __global__ __launch_bounds__( kBlock ) void kElemApply( int L, int tilesPerCell,
                                                         const T* __restrict__ x, const T* __restrict__ in1,
                                                         T* __restrict__ out0, T* __restrict__ out1, T a )
 {
-   bElemApply< T, MODE, DIAG >( int( blockIdx.x ), L, tilesPerCell, stride, cells, kb, cinfo, cfs, prims, entries, part, lowPart, dinv, flag, x, in1, out0, out1, a );
+   bElemApply< T, MODE, DIAG, BND >( int( blockIdx.x ), L, tilesPerCell, stride, cells, kb, cinfo, cfs, prims, entries, part, lowPart, dinv, flag, x, in1, out0, out1, a );
 }
 
 } // namespace hyg

@@ -602,6 +602,14 @@ struct hyg_op_s
    std::vector< double > modesHost; // nModes x 5: kappa_x, kappa_y, kappa_z, phi, a
    double*               modes  = nullptr;
    int                   nModes = 0;
+   // kind 1 with a polynomial coefficient (kPolyDeg >= 0): k = sum_i kPoly[i] x^a y^b z^c, monomials of degree <=
+   // kPolyDeg in the reference's MonomialBasis3D order (device copy; the Fourier fields above are unused)
+   int                   kPolyDeg = -1;
+   std::vector< double > kPolyHost;
+   double*               kPoly = nullptr;
+   // kind 1 after hyg_op_interpolate_stencils (P1SurrogateOperator): the polynomial degree of the cell-interior
+   // stencils (Level::surPoly), -1 without
+   int                   surDegree = -1;
    struct Level
    {
       std::vector< double > cellStHost;  // nC x 16: 15 weights, inverse centre weight
@@ -645,6 +653,12 @@ struct hyg_op_s
       // point and one-sided stencils of every cell-face point (per cell p2StoredCellSize(N) values)
       double*               p2s     = nullptr;
       float*                p2sF    = nullptr;
+      // surrogate (kind 1 after hyg_op_interpolate_stencils): per cell the polynomial of every stencil direction
+      // (15 x monomial count, direction-major) in the cell-local coordinates (x, y, z) / N, and the (y, z) of the
+      // cell-interior rows
+      double*               surPoly = nullptr;
+      int*                  surRows = nullptr;
+      int                   surNRows = 0;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

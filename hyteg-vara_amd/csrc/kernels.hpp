@@ -41,6 +41,14 @@ void launchElementwise( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int 
                         hipStream_t s );
 void launchElementwiseInverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );
 void launchElementwiseStore( const MeshCtx& m, hyg_op_s& op, hipStream_t s );
+// Surrogate stencils of the elementwise operator (P1SurrogateOperator): fit the cell-interior stencils of every
+// level by polynomials of degree deg from the samples of the interpolation level (needs the stored coefficient
+// integrals; synchronous), whether level L uses them, the interior part of one stencil operation, and the interior
+// inverse diagonal (written into op.levels[L].dinv)
+void surrogateInterpolate( const MeshCtx& m, hyg_op_s& op, int deg, int interpolationLevel );
+bool surrogateActive( const hyg_op_s& op, int L );
+void launchSurrogateInterior( const MeshCtx& m, const LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c, hipStream_t s );
+void launchSurrogateInverseDiagonal( const MeshCtx& m, const LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );
 // P2 operator (kind 2) at lattice level L: the stencil modes, the inverse diagonal, and the quadratic transfers
 // (fine lattice Lf from Lf-1; coarse lattice Lc from Lc+1)
 void launchP2( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c, hipStream_t s );

@@ -531,6 +531,8 @@ void assembleP1DivKGrad( hyg_op_s& op )
    if ( !op.modesHost.empty() )
       op.modes = upload( op.modesHost );
    op.nModes = int( op.modesHost.size() / 5 );
+   if ( op.kPolyDeg >= 0 )
+      op.kPoly = upload( op.kPolyHost );
    for ( int L = op.minL; L <= op.maxL; ++L )
    {
       hyg_op_s::Level& ol = op.levels[L];
@@ -932,6 +934,9 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.p2fstF );
       (void) hipFree( l.p2s );
       (void) hipFree( l.p2sF );
+      (void) hipFree( l.surPoly );
+      (void) hipFree( l.surRows );
    }
    (void) hipFree( modes );
+   (void) hipFree( kPoly );
 }

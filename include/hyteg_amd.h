@@ -173,6 +173,21 @@ int hyg_op_destroy( hyg_op op );
  * in FP64 and FP32. */
 int hyg_op_create_p1_divkgrad( hyg_mesh mesh, int min_level, int max_level, double k0, const double* modes, int n_modes,
                                hyg_op* out );
+/* The same operator with a polynomial coefficient k(x) = sum_i coeffs[i] m_i(x) over the (degree+1)(degree+2)(degree+3)/6
+ * monomials m_i = x^a y^b z^c of degree <= degree (0..12) in the order of the reference's MonomialBasis3D
+ * (polynomial/MonomialBasis3D.hpp: by total degree, within a degree by decreasing a, then decreasing b), as the
+ * forms::p1_div_k_grad_affine_q3(k, k) of tests/hyteg/operators/P1SurrogateOperatorTest.cpp:37-45 take it. */
+int hyg_op_create_p1_divkgrad_poly( hyg_mesh mesh, int min_level, int max_level, int degree, const double* coeffs, hyg_op* out );
+/* Replaces P1SurrogateOperator< p1_div_k_grad_affine_q3 >::interpolateStencils(polyDegree, interpolationLevel)
+ * (p1functionspace/P1SurrogateOperator.hpp:97-110, 232-335) on a variable-coefficient P1 operator: from then on the
+ * operator is the surrogate. On every level L the 15 stencil weights of the interior points of each macro-cell are
+ * polynomials of degree `degree` (0..12) in the cell-local coordinates (x, y, z) / 2^L, least-squares fits
+ * (LSQPInterpolator3D, column-pivoted QR) to the exact stencils at the interior points of level
+ * min(L, interpolation_level) (>= 2) scaled to level L; face, edge and vertex points keep the exact stencils. Applies,
+ * smoothers, residuals, the inverse diagonal (interior: 1 / the polynomial centre weight) and GMG use the surrogate.
+ * Keeps the coefficient integrals (as hyg_op_store_local_matrices) for the face, edge and vertex points. Calling it
+ * again refits. Surrogate operators cannot be batched. */
+int hyg_op_interpolate_stencils( hyg_op op, int degree, int interpolation_level );
 /* Replaces P2ConstantLaplaceOperator(storage, minLevel, maxLevel) (P2ConstantOperator< p2_tet_diffusion >,
  * p2functionspace/P2ConstantOperator.cpp:99-113, with its vertex/edge sub-operators and
  * computeInverseDiagonalOperatorValues), applied element by element with the element matrices of the six

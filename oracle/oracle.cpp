@@ -47,6 +47,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <fstream>
 #include <climits>
 #include <map>
@@ -976,18 +977,45 @@ void commAdditiveCellToLower( Function< T >& fn, int L, int excludeFlag, bool ze
 }
 
 // ---------------------------------------------------------------- variable-coefficient elementwise operator
+// MonomialBasis3D (polynomial/MonomialBasis3D.hpp): monomials x^a y^b z^c ordered by total degree, within a degree
+// by decreasing a, then decreasing b. Exponents of basis function i.
+inline int monomialCount3( int deg ) { return ( deg + 1 ) * ( deg + 2 ) * ( deg + 3 ) / 6; }
+inline std::array< int, 3 > monomialExponents3( int i )
+{
+   for ( int d = 0;; ++d )
+      for ( int a = d; a >= 0; --a )
+         for ( int b = d - a; b >= 0; --b )
+            if ( i-- == 0 )
+               return { a, b, d - a - b };
+}
+inline double monomial3( int i, const P3& x )
+{
+   const auto e = monomialExponents3( i );
+   return std::pow( x[0], e[0] ) * std::pow( x[1], e[1] ) * std::pow( x[2], e[2] );
+}
+
 // Coefficient k(x) of -div(k grad u): kind 0: k0 exp( sum_m a_m cos(2 pi kappa_m . x + phi_m) ),
-// kind 1: tanh(alpha (x - 1/2)) + 2 (the coefficient of ElementwiseDivKGradCGConvergenceTest.cpp:60-62).
+// kind 1: tanh(alpha (x - 1/2)) + 2 (the coefficient of ElementwiseDivKGradCGConvergenceTest.cpp:60-62),
+// kind 2: a polynomial sum_i poly_i m_i(x) in the monomial basis above (the coefficients k1..k4 of
+// tests/hyteg/operators/P1SurrogateOperatorTest.cpp:76-112 are of this kind).
 struct KField
 {
    int                                   kind  = 0;
    double                                k0    = 1.0;
    double                                alpha = 2.0;
    std::vector< std::array< double, 5 > > modes; // kappa_x, kappa_y, kappa_z, phi, a
+   std::vector< double >                 poly;
    double operator()( const P3& x ) const
    {
       if ( kind == 1 )
          return std::tanh( alpha * ( x[0] - 0.5 ) ) + 2.0;
+      if ( kind == 2 )
+      {
+         double v = 0.0;
+         for ( size_t i = 0; i < poly.size(); ++i )
+            v += poly[i] * monomial3( int( i ), x );
+         return v;
+      }
       double g = 0.0;
       for ( auto& md : modes )
          g += md[4] * std::cos( 2.0 * M_PI * ( md[0] * x[0] + md[1] * x[1] + md[2] * x[2] ) + md[3] );
@@ -1167,6 +1195,245 @@ struct P1ElementwiseDivKGrad : OperatorBase< T >
          T& v = at( D, L, kind, p, i );
          v    = T( 1.0 ) / v;
       } );
+   }
+};
+
+// Least squares by Householder QR with column pivoting (the reference solves its fits with Eigen's
+// colPivHouseholderQr().solve, polynomial/LSQPInterpolator.hpp:166-180): a is m x n row-major; every right-hand
+// side in rhs (length m) is replaced by its solution (length n). Pivot: the remaining column of largest norm (first
+// of equals); the numerical rank counts |R_kk| > eps min(m, n) |R_00|; columns beyond it get zero (the basic
+// solution of a rank-deficient system).
+inline void lsqColPiv( std::vector< double > a, int m, int n, std::vector< std::vector< double > >& rhs )
+{
+   std::vector< int >    perm( n );
+   std::vector< double > tau( n, 0.0 );
+   for ( int j = 0; j < n; ++j )
+      perm[j] = j;
+   const int kmax = std::min( m, n );
+   int       rank = 0;
+   double    r00  = 0.0;
+   for ( int k = 0; k < kmax; ++k )
+   {
+      int    jb = k;
+      double nb = -1.0;
+      for ( int j = k; j < n; ++j )
+      {
+         double s = 0.0;
+         for ( int i = k; i < m; ++i )
+            s += a[size_t( i ) * n + j] * a[size_t( i ) * n + j];
+         if ( s > nb )
+         {
+            nb = s;
+            jb = j;
+         }
+      }
+      if ( jb != k )
+      {
+         for ( int i = 0; i < m; ++i )
+            std::swap( a[size_t( i ) * n + k], a[size_t( i ) * n + jb] );
+         std::swap( perm[k], perm[jb] );
+      }
+      const double nx = std::sqrt( nb );
+      if ( k == 0 )
+         r00 = nx;
+      if ( nx <= std::numeric_limits< double >::epsilon() * kmax * r00 || nx == 0.0 )
+         break;
+      ++rank;
+      const double alpha = a[size_t( k ) * n + k];
+      const double beta  = alpha >= 0.0 ? -nx : nx;
+      tau[k]             = ( beta - alpha ) / beta;
+      const double sc    = 1.0 / ( alpha - beta );
+      for ( int i = k + 1; i < m; ++i )
+         a[size_t( i ) * n + k] *= sc;
+      a[size_t( k ) * n + k] = beta;
+      for ( int j = k + 1; j < n; ++j )
+      {
+         double wv = a[size_t( k ) * n + j];
+         for ( int i = k + 1; i < m; ++i )
+            wv += a[size_t( i ) * n + k] * a[size_t( i ) * n + j];
+         wv *= tau[k];
+         a[size_t( k ) * n + j] -= wv;
+         for ( int i = k + 1; i < m; ++i )
+            a[size_t( i ) * n + j] -= wv * a[size_t( i ) * n + k];
+      }
+   }
+   for ( auto& b : rhs )
+   {
+      for ( int k = 0; k < rank; ++k )
+      {
+         double wv = b[k];
+         for ( int i = k + 1; i < m; ++i )
+            wv += a[size_t( i ) * n + k] * b[i];
+         wv *= tau[k];
+         b[k] -= wv;
+         for ( int i = k + 1; i < m; ++i )
+            b[i] -= wv * a[size_t( i ) * n + k];
+      }
+      std::vector< double > z( n, 0.0 );
+      for ( int k = rank - 1; k >= 0; --k )
+      {
+         double v = b[k];
+         for ( int j = k + 1; j < rank; ++j )
+            v -= a[size_t( k ) * n + j] * z[j];
+         z[k] = v / a[size_t( k ) * n + k];
+      }
+      b.assign( n, 0.0 );
+      for ( int j = 0; j < n; ++j )
+         b[perm[j]] = z[j];
+   }
+}
+
+// P1SurrogateOperator< p1_div_k_grad_affine_q3 > (p1functionspace/P1SurrogateOperator.hpp): after
+// interpolateStencils( degree, interpolationLevel ) (:232-335) every stencil direction of the interior points of a
+// macro-cell is a polynomial of the given degree in the cell-local coordinates (i, j, k) / N, fitted by least
+// squares to the variable stencils (P1Elements3D::assembleP1LocalStencilNew_new, P1Operator.hpp:2134-2161) at the
+// interior points of the interpolation level (the level's own points scaled by 2^(L - interpolation level)).
+// Applies, the inverse diagonal (P1Operator.hpp:645-915 with assemble_stencil_cell) and everything built on them use
+// the polynomials at the interior points and the variable stencils at face, edge and vertex points (:404-469).
+template < typename T >
+struct P1SurrogateDivKGrad : P1ElementwiseDivKGrad< T >
+{
+   using P1ElementwiseDivKGrad< T >::P1ElementwiseDivKGrad;
+   using P1ElementwiseDivKGrad< T >::k;
+   int degree = -1;
+   // [level][cell]: NDIR x monomialCount3(degree) coefficients, direction-major
+   std::vector< std::vector< std::vector< double > > > poly;
+
+   // variable stencil of lattice point (x, y, z) of cell c: rows of the element matrices of the incident micro-cells
+   void variableStencil( int c, int L, idx x, idx y, idx z, double st[NDIR] ) const
+   {
+      const idx N = idx( 1 ) << L;
+      for ( int d = 0; d < NDIR; ++d )
+         st[d] = 0.0;
+      for ( int t = 0; t < 6; ++t )
+         for ( int v = 0; v < 4; ++v )
+         {
+            const idx ax = x - MICRO_CELL[t][v][0], ay = y - MICRO_CELL[t][v][1], az = z - MICRO_CELL[t][v][2];
+            P3        xc[4];
+            bool      inside = true;
+            for ( int u = 0; u < 4; ++u )
+            {
+               const idx a = ax + MICRO_CELL[t][u][0], b = ay + MICRO_CELL[t][u][1], e = az + MICRO_CELL[t][u][2];
+               if ( !inCell( N, a, b, e ) )
+               {
+                  inside = false;
+                  break;
+               }
+               xc[u] = microCoord( *this->m, c, L, a, b, e );
+            }
+            if ( !inside )
+               continue;
+            double el[4][4];
+            divKGradElement( xc, k, el );
+            for ( int u = 0; u < 4; ++u )
+            {
+               const int o[3] = { MICRO_CELL[t][u][0] - MICRO_CELL[t][v][0], MICRO_CELL[t][u][1] - MICRO_CELL[t][v][1],
+                                  MICRO_CELL[t][u][2] - MICRO_CELL[t][v][2] };
+               for ( int d = 0; d < NDIR; ++d )
+                  if ( DOFF[d][0] == o[0] && DOFF[d][1] == o[1] && DOFF[d][2] == o[2] )
+                     st[d] += el[v][u];
+            }
+         }
+   }
+
+   void interpolateStencils( int deg, int interpolationLevel )
+   {
+      degree      = deg;
+      const int nc = monomialCount3( deg );
+      poly.assign( this->maxL + 1, {} );
+      for ( int L = std::max( this->minL, 1 ); L <= this->maxL; ++L )
+      {
+         const int    Li = std::min( L, interpolationLevel ), sh = L - Li;
+         const idx    Ni = idx( 1 ) << Li;
+         const double hi = 1.0 / double( Ni );
+         std::vector< std::array< idx, 3 > > pts;
+         for ( idx z = 1; z + 3 <= Ni; ++z )
+            for ( idx y = 1; y + z + 2 <= Ni; ++y )
+               for ( idx x = 1; x + y + z + 1 <= Ni; ++x )
+                  pts.push_back( { x, y, z } );
+         const int             np = int( pts.size() );
+         std::vector< double > a( size_t( np ) * nc );
+         for ( int r = 0; r < np; ++r )
+            for ( int i = 0; i < nc; ++i )
+               a[size_t( r ) * nc + i] = monomial3( i, P3{ pts[r][0] * hi, pts[r][1] * hi, pts[r][2] * hi } );
+         poly[L].assign( this->m->Cl.size(), std::vector< double >( size_t( NDIR ) * nc, 0.0 ) );
+         for ( int c = 0; c < (int) this->m->Cl.size(); ++c )
+         {
+            std::vector< std::vector< double > > rhs( NDIR, std::vector< double >( np ) );
+            for ( int r = 0; r < np; ++r )
+            {
+               double st[NDIR];
+               variableStencil( c, L, pts[r][0] << sh, pts[r][1] << sh, pts[r][2] << sh, st );
+               for ( int d = 0; d < NDIR; ++d )
+                  rhs[d][r] = st[d];
+            }
+            if ( np > 0 )
+               lsqColPiv( a, np, nc, rhs );
+            for ( int d = 0; d < NDIR; ++d )
+               for ( int i = 0; i < nc && np > 0; ++i )
+                  poly[L][c][size_t( d ) * nc + i] = rhs[d][i];
+         }
+         surrogateInverseDiagonal( L );
+      }
+   }
+
+   double evalDir( int L, int c, int d, idx x, idx y, idx z ) const
+   {
+      const int    nc = monomialCount3( degree );
+      const double h  = 1.0 / double( idx( 1 ) << L );
+      const P3     p{ x * h, y * h, z * h };
+      double       v = 0.0;
+      for ( int i = 0; i < nc; ++i )
+         v += poly[L][c][size_t( d ) * nc + i] * monomial3( i, p );
+      return v;
+   }
+   bool active( int L ) const { return degree >= 0 && L < (int) poly.size() && !poly[L].empty(); }
+
+   void apply( const Function< T >& src, Function< T >& dst, int L, int flag, bool add ) const override
+   {
+      if ( !active( L ) )
+      {
+         P1ElementwiseDivKGrad< T >::apply( src, dst, L, flag, add );
+         return;
+      }
+      const idx N = idx( 1 ) << L, w = N + 1;
+      // interior values before the apply (the element loops add into them)
+      std::vector< std::vector< T > > keep;
+      if ( add && ( flag & Inner ) )
+         for ( const auto& dc : dst.c[L] )
+            keep.push_back( dc );
+      P1ElementwiseDivKGrad< T >::apply( src, dst, L, flag, add );
+      if ( !( flag & Inner ) )
+         return;
+      src.commDown( L );
+#pragma omp parallel for schedule( dynamic )
+      for ( idx c = 0; c < (idx) this->m->Cl.size(); ++c )
+      {
+         const auto& sc = src.c[L][c];
+         auto&       dc = dst.c[L][c];
+         for ( idx z = 1; z + 3 <= N; ++z )
+            for ( idx y = 1; y + z + 2 <= N; ++y )
+               for ( idx x = 1; x + y + z + 1 <= N; ++x )
+               {
+                  double s = 0.0;
+                  for ( int d = 0; d < NDIR; ++d )
+                     s += evalDir( L, (int) c, d, x, y, z ) * double( sc[cidx( w, x + DOFF[d][0], y + DOFF[d][1], z + DOFF[d][2] )] );
+                  const idx q = cidx( w, x, y, z );
+                  dc[q]       = ( add ? keep[c][q] : T( 0 ) ) + T( s );
+               }
+      }
+   }
+
+   // interior points: 1 / the polynomial centre weight; elsewhere the variable stencil's
+   void surrogateInverseDiagonal( int L )
+   {
+      auto&     D = *this->invDiag;
+      const idx N = idx( 1 ) << L, w = N + 1;
+      for ( idx c = 0; c < (idx) this->m->Cl.size(); ++c )
+         for ( idx z = 1; z + 3 <= N; ++z )
+            for ( idx y = 1; y + z + 2 <= N; ++y )
+               for ( idx x = 1; x + y + z + 1 <= N; ++x )
+                  D.c[L][c][cidx( w, x, y, z )] = T( 1.0 / evalDir( L, (int) c, V_C, x, y, z ) );
    }
 };
 
@@ -2698,10 +2965,67 @@ void* or_op_new_divkgrad( void* mh, int minL, int maxL, int prec, int kind, doub
    o->prec = prec;
    o->m    = static_cast< Mesh* >( mh );
    if ( prec == 4 )
-      o->f = std::make_unique< P1ElementwiseDivKGrad< float > >( o->m, minL, maxL, kf );
+      o->f = std::make_unique< P1SurrogateDivKGrad< float > >( o->m, minL, maxL, kf );
    else
-      o->d = std::make_unique< P1ElementwiseDivKGrad< double > >( o->m, minL, maxL, kf );
+      o->d = std::make_unique< P1SurrogateDivKGrad< double > >( o->m, minL, maxL, kf );
    return o;
+}
+
+// the same operator with the polynomial coefficient k = sum_i coeffs[i] m_i(x) (monomial basis, ncoef terms)
+void* or_op_new_divkgrad_poly( void* mh, int minL, int maxL, int prec, const double* coeffs, int ncoef )
+{
+   KField kf;
+   kf.kind = 2;
+   kf.poly.assign( coeffs, coeffs + ncoef );
+   auto* o = new OrOp;
+   o->prec = prec;
+   o->m    = static_cast< Mesh* >( mh );
+   if ( prec == 4 )
+      o->f = std::make_unique< P1SurrogateDivKGrad< float > >( o->m, minL, maxL, kf );
+   else
+      o->d = std::make_unique< P1SurrogateDivKGrad< double > >( o->m, minL, maxL, kf );
+   return o;
+}
+
+// P1SurrogateOperator::interpolateStencils( degree, interpolationLevel ) on a div-k-grad operator of this file
+int or_op_interpolate_stencils( void* oh, int degree, int interpolationLevel )
+{
+   auto* o = static_cast< OrOp* >( oh );
+   if ( o->prec == 4 )
+   {
+      auto* s = dynamic_cast< P1SurrogateDivKGrad< float >* >( o->f.get() );
+      if ( !s )
+         return 1;
+      s->interpolateStencils( degree, interpolationLevel );
+   }
+   else
+   {
+      auto* s = dynamic_cast< P1SurrogateDivKGrad< double >* >( o->d.get() );
+      if ( !s )
+         return 1;
+      s->interpolateStencils( degree, interpolationLevel );
+   }
+   return 0;
+}
+
+// the fitted polynomials of cell c at level L (NDIR x monomialCount3(degree), direction-major), FP64 operators
+int or_op_surrogate_poly( void* oh, int L, int c, double* out )
+{
+   auto* s = dynamic_cast< P1SurrogateDivKGrad< double >* >( static_cast< OrOp* >( oh )->d.get() );
+   if ( !s || !s->active( L ) )
+      return 1;
+   std::copy( s->poly[L][c].begin(), s->poly[L][c].end(), out );
+   return 0;
+}
+
+// the variable stencil (element rows) of lattice point (x, y, z) of cell c at level L, FP64 operators
+int or_op_variable_stencil( void* oh, int L, int c, long x, long y, long z, double* out15 )
+{
+   auto* s = dynamic_cast< P1SurrogateDivKGrad< double >* >( static_cast< OrOp* >( oh )->d.get() );
+   if ( !s )
+      return 1;
+   s->variableStencil( c, L, x, y, z, out15 );
+   return 0;
 }
 
 // P2ConstantOperator< p2_tet_diffusion > on P2 levels minL..maxL; functions used with it hold lattice levels

@@ -41,6 +41,10 @@ def lib():
             "or_op_new": (vp, [vp, i, i, i]),
             "or_op_free": (None, [vp]),
             "or_op_new_divkgrad": (vp, [vp, i, i, i, i, d, d, P(np.float64), i]),
+            "or_op_new_divkgrad_poly": (vp, [vp, i, i, i, P(np.float64), i]),
+            "or_op_interpolate_stencils": (i, [vp, i, i]),
+            "or_op_surrogate_poly": (i, [vp, i, i, P(np.float64)]),
+            "or_op_variable_stencil": (i, [vp, i, i, C.c_long, C.c_long, C.c_long, P(np.float64)]),
             "or_op_cell_stencil": (None, [vp, i, i, P(np.float64)]),
             "or_apply": (None, [vp, vp, vp, i, i, i]),
             "or_inv_diag": (None, [vp, i, P(np.float64)]),
@@ -236,16 +240,36 @@ def p2_prolongate(f, src_lat, flag=INNER, add=False):
 
 
 class DivKGradOperator(Operator):
-    """P1 elementwise -div(k grad) operator; kfield = ("fourier", k0, modes[M x 5]) or ("tanh", alpha)."""
+    """P1 elementwise -div(k grad) operator; kfield = ("fourier", k0, modes[M x 5]), ("tanh", alpha) or
+    ("poly", coefficients in the monomial basis). interpolate_stencils(degree, level) turns it into the surrogate."""
 
     def __init__(self, mesh, minL, maxL, kfield, prec=8):
         self.mesh = mesh
-        if kfield[0] == "fourier":
+        self.degree = -1
+        if kfield[0] == "poly":
+            c = np.ascontiguousarray(np.asarray(kfield[1], dtype=np.float64))
+            self.h = lib().or_op_new_divkgrad_poly(mesh.h, minL, maxL, prec, c, len(c))
+        elif kfield[0] == "fourier":
             modes = np.ascontiguousarray(np.asarray(kfield[2], dtype=np.float64).reshape(-1, 5))
             self.h = lib().or_op_new_divkgrad(mesh.h, minL, maxL, prec, 0, float(kfield[1]), 0.0,
                                                modes if len(modes) else np.zeros((1, 5)), len(modes))
         else:
             self.h = lib().or_op_new_divkgrad(mesh.h, minL, maxL, prec, 1, 1.0, float(kfield[1]), np.zeros((1, 5)), 0)
+
+    def interpolate_stencils(self, degree, level):
+        assert lib().or_op_interpolate_stencils(self.h, degree, level) == 0
+        self.degree = degree
+
+    def surrogate_poly(self, L, c):
+        nc = (self.degree + 1) * (self.degree + 2) * (self.degree + 3) // 6
+        out = np.zeros(15 * nc)
+        assert lib().or_op_surrogate_poly(self.h, L, c, out) == 0
+        return out.reshape(15, nc)
+
+    def variable_stencil(self, L, c, x, y, z):
+        out = np.zeros(15)
+        assert lib().or_op_variable_stencil(self.h, L, c, x, y, z, out) == 0
+        return out
 
 
 def random_field(seed, modes=32, kmax=8, std=1.0):
