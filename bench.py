@@ -279,6 +279,39 @@ def bench_divkgrad(hyg, hip, st, steps, level, seed):
             aps()
         ms = timed(hip, st, aps, steps) / steps
         out[f"apply_stored_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3)}
+    # surrogate operator (P1SurrogateOperator::interpolateStencils): degree-4 polynomial stencils of the cell
+    # interiors fitted on level min(L, 5), on a smooth field (3 modes, |kappa| <= 1, std 0.2) that such polynomials
+    # approximate; the interior apply reads x and writes y only (16 B per DoF), face/edge/vertex points use the stored
+    # integrals. Error: relative l-inf against the variable operator's apply of the same vector.
+    smooth = hyg.fourier_field(seed, modes=3, kmax=1, std=0.2)
+    opv = hyg.P1DivKGradOperator(mesh, L, L, 1.0, smooth)
+    opv.store_local_matrices()
+    opq = hyg.P1DivKGradOperator(mesh, L, L, 1.0, smooth)
+    t0 = time.perf_counter()
+    opq.interpolate_stencils(4, min(L, 5))
+    sur = {"degree": 4, "interpolation_level": min(L, 5), "field": f"3 Fourier modes, |kappa| <= 1, std 0.2, seed {seed}",
+           "setup_seconds": round(time.perf_counter() - t0, 3)}
+    xv = rng.uniform(-1, 1, n)
+    for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):
+        x, y = hyg.Function(mesh, L, L, dt), hyg.Function(mesh, L, L, dt)
+        x.upload(L, xv)
+
+        def apq():
+            hyg.check(lib.hyg_apply(opq.h, x.h, y.h, L, hyg.INNER, hyg.REPLACE, st))
+
+        for _ in range(2):
+            apq()
+        ms = timed(hip, st, apq, steps) / steps
+        sur[f"apply_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3),
+                                "GB/s_alg": round((16.0 if dt == hyg.F64 else 8.0) * inner / (ms / 1e3) / 1e9, 1)}
+        if dt == hyg.F64:
+            yq = y.download(L)
+            yv = hyg.Function(mesh, L, L)
+            opv.apply(x, yv, L, hyg.INNER)
+            yr = yv.download(L)
+            sur["rel_error_vs_variable"] = float(np.abs(yq - yr).max() / np.abs(yr).max())
+    out["surrogate"] = sur
+    del opv, opq
     # VALU use. Executed flops: kKBar as counted above; kElemApply per point of a macro-cell 24 x 4 FMA to
     # assemble the stencil from the incident elements' integrals + 15 FMA to apply it (222 flops). The kKBar time
     # is the on-the-fly apply minus the stored one (the rest is the same launches). The reference's work per apply

@@ -72,8 +72,10 @@ static void elemT( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, co
    const int tA = ceilDiv( S3i( w ), kBlock );
    if ( !DIAG && surrogateActive( op, L ) )
    {
-      hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG, true > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem, kb,
-                          m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ) );
+      const int tB = ceilDiv( ol.surNBnd, kBlock );
+      hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, ol.elem, kb,
+                          m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ), ol.surBnd,
+                          ol.surNBnd );
       launchSurrogateInterior( m, g, op, L, std::is_same< T, double >::value ? 8 : 4, c, s );
    }
    else

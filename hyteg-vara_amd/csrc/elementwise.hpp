@@ -193,8 +193,9 @@ __device__ __forceinline__ void elemWeights( const ElemCell& ec, const T* __rest
       }
 }
 
-// One thread per point of the closed macro-cell. DIAG: the diagonal entry instead of the product. BND: points on the
-// cell's faces, edges and vertices only (the surrogate operator does the interior points itself).
+// One thread per point of the closed macro-cell. DIAG: the diagonal entry instead of the product. BND: the points
+// on the cell's faces, edges and vertices only, listed in bnd[0 .. nBnd) by their index in the cell array (the
+// surrogate operator does the interior points itself).
 template < typename T, int MODE, bool DIAG, bool BND = false >
 __device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i64 stride, const ElemCell* __restrict__ cells,
                                                         const T* __restrict__ kb, const CellInfo* __restrict__ cinfo,
@@ -202,18 +203,25 @@ __device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i6
                                                         const LowerEntry* __restrict__ entries, T* __restrict__ part,
                                                         T* __restrict__ lowPart, const T* __restrict__ dinv, int flag,
                                                         const T* __restrict__ x, const T* __restrict__ in1,
-                                                        T* __restrict__ out0, T* __restrict__ out1, T a )
+                                                        T* __restrict__ out0, T* __restrict__ out1, T a,
+                                                        const int* __restrict__ bnd = nullptr, int nBnd = 0 )
 {
    const int c    = bid / tilesPerCell;
    const int tile = bid - c * tilesPerCell;
    const int N = 1 << L, w = N + 1, NP = N + 1;
-   const int s = tile * kBlock + int( threadIdx.x );
-   if ( s >= S3i( w ) )
+   int       s = tile * kBlock + int( threadIdx.x );
+   if constexpr ( BND )
+   {
+      if ( s >= nBnd )
+         return;
+      s = bnd[s];
+   }
+   else if ( s >= S3i( w ) )
       return;
    int px, py, pz;
    tetXYZ( w, s, px, py, pz );
    const int slot = classifySlot( N, px, py, pz );
-   if ( slot == 0 && ( BND || !( flag & 1 ) ) )
+   if ( slot == 0 && !( flag & 1 ) )
       return;
    const ElemCell& ec = cells[c];
    const T*        kc = kb + i64( c ) * 6 * S3i( N );
@@ -280,9 +288,11 @@ __global__ __launch_bounds__( kBlock ) void kElemApply( int L, int tilesPerCell,
                                                         const LowerEntry* __restrict__ entries, T* __restrict__ part,
                                                         T* __restrict__ lowPart, const T* __restrict__ dinv, int flag,
                                                         const T* __restrict__ x, const T* __restrict__ in1,
-                                                        T* __restrict__ out0, T* __restrict__ out1, T a )
+                                                        T* __restrict__ out0, T* __restrict__ out1, T a,
+                                                        const int* __restrict__ bnd = nullptr, int nBnd = 0 )
 {
-   bElemApply< T, MODE, DIAG, BND >( int( blockIdx.x ), L, tilesPerCell, stride, cells, kb, cinfo, cfs, prims, entries, part, lowPart, dinv, flag, x, in1, out0, out1, a );
+   bElemApply< T, MODE, DIAG, BND >( int( blockIdx.x ), L, tilesPerCell, stride, cells, kb, cinfo, cfs, prims, entries, part, lowPart, dinv,
+                                     flag, x, in1, out0, out1, a, bnd, nBnd );
 }
 
 } // namespace hyg

@@ -654,11 +654,11 @@ struct hyg_op_s
       double*               p2s     = nullptr;
       float*                p2sF    = nullptr;
       // surrogate (kind 1 after hyg_op_interpolate_stencils): per cell the polynomial of every stencil direction
-      // (15 x monomial count, direction-major) in the cell-local coordinates (x, y, z) / N, and the (y, z) of the
-      // cell-interior rows
+      // (15 x monomial count, direction-major) in the cell-local coordinates (x, y, z) / N, and the cell-array indices
+      // of the points on a cell's faces, edges and vertices (the points the exact stencils keep)
       double*               surPoly = nullptr;
-      int*                  surRows = nullptr;
-      int                   surNRows = 0;
+      int*                  surBnd  = nullptr;
+      int                   surNBnd = 0;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

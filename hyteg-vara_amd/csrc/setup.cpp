@@ -935,7 +935,7 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.p2s );
       (void) hipFree( l.p2sF );
       (void) hipFree( l.surPoly );
-      (void) hipFree( l.surRows );
+      (void) hipFree( l.surBnd );
    }
    (void) hipFree( modes );
    (void) hipFree( kPoly );

@@ -13,10 +13,10 @@
 //   * the sample stencils come from the device (kSurSample: the element assembly of the elementwise operator from
 //     the stored coefficient integrals), the fit runs on the host (one column-pivoted Householder QR of the
 //     sample matrix per level, shared by all cells and directions);
-//   * kSurrogate applies the polynomial stencils: one wave per interior row (y, z) of a macro-cell first restricts
-//     the 15 polynomials to the row (their coefficients in x, in LDS), then every lane evaluates them at its point
-//     by Horner and applies the stencil with the mode's epilogue. The reference evaluates by forward differences
-//     along the row (polynomial/PolynomialEvaluator.hpp); both evaluate the same polynomial, in another rounding;
+//   * kSurrogate applies the polynomial stencils, one block per interior z-plane of a macro-cell: the polynomials are
+//     restricted to the plane, then to pairs of rows, and every lane evaluates its row's polynomials in x by Horner
+//     and applies the stencil with the mode's epilogue. The reference evaluates by forward differences along the
+//     row (polynomial/PolynomialEvaluator.hpp); both evaluate the same polynomial, in another rounding;
 //   * the inverse diagonal of the interior points is 1 / the polynomial centre weight (P1Operator.hpp:645-915 with
 //     assemble_stencil_cell).
 #include "elementwise.hpp"
@@ -62,84 +62,128 @@ __global__ __launch_bounds__( kBlock ) void kSurSample( int L, int nPts, int til
       o[d] = W[d];
 }
 
-// Cell-interior points of the surrogate operator. Block = kSurWaves waves, wave = one interior row (rows[2r] = y,
-// rows[2r+1] = z) of cell blockIdx / rowBlocks. DIAG: writes 1 / centre weight into diagOut instead.
-template < typename T, int MODE, bool DIAG >
-__global__ __launch_bounds__( 64 * kSurWaves ) void kSurrogate( int L, int deg, int nRows, int rowBlocks, const int* __restrict__ rows,
-                                                               const double* __restrict__ poly, i64 stride, int flag,
-                                                               const T* __restrict__ dinv, const T* __restrict__ x,
-                                                               const T* __restrict__ in1, T* __restrict__ out0, T* __restrict__ out1,
-                                                               T a, double* __restrict__ diagOut )
+// Cell-interior points of the surrogate operator. Block = one interior z-plane (z = 1 .. N-3) of a macro-cell:
+//   1. the block restricts the 15 direction polynomials to the plane, pz[d][(a, b)] = sum_c P_d[a, b, c] z^c (LDS);
+//   2. the plane's interior rows y = 1 .. M (M = N-2-z, row y has M+1-y points) are taken in folded pairs
+//      (y, M+1-y), whose lengths add up to M+1, so that a wave's lanes cover both rows of a pair without gaps; per
+//      pair the wave restricts the plane polynomials to its two rows, c1[row][d][a] = sum_b pz[d][(a, b)] y^b (LDS),
+//      and every lane evaluates its row's polynomials at its point by Horner and applies the stencil.
+// DIAG: writes 1 / the centre weight into diagOut instead. The LDS of each wave is written and read by that wave
+// only (its LDS instructions execute in order), so no barrier is needed after step 1.
+// DEG >= 0: the degree at compile time (all polynomial loops unrolled, so the LDS reads of the 15 Horner chains
+// are issued together); DEG = -1: any degree up to kSurMaxDeg, given by degRun.
+template < typename T, int MODE, bool DIAG, int DEG >
+__global__ __launch_bounds__( 64 * kSurWaves ) void kSurrogate( int L, int degRun, int planes, const double* __restrict__ poly, i64 stride,
+                                                               int flag, const T* __restrict__ dinv, const T* __restrict__ x,
+                                                               const T* __restrict__ in1, T* __restrict__ out0, T* __restrict__ out1, T a,
+                                                               double* __restrict__ diagOut )
 {
-   __shared__ double c1[kSurWaves][NDIR * ( kSurMaxDeg + 1 )];
-   const int  c    = int( blockIdx.x ) / rowBlocks;
-   const int  wv   = int( threadIdx.x ) >> 6, lane = int( threadIdx.x ) & 63;
-   const int  r    = ( int( blockIdx.x ) - c * rowBlocks ) * kSurWaves + wv;
-   const bool live = r < nRows;
-   const int  N = 1 << L, w = N + 1, nc = surCoefs( deg ), q1 = deg + 1;
+   const int deg = DEG >= 0 ? DEG : degRun;
+   __shared__ double pz[NDIR * ( kSurMaxDeg + 1 ) * ( kSurMaxDeg + 2 ) / 2];
+   __shared__ double c1[kSurWaves][2][NDIR * ( kSurMaxDeg + 1 )];
+   const int    c  = int( blockIdx.x ) / planes;
+   const int    z  = 1 + int( blockIdx.x ) - c * planes;
+   const int    wv = int( threadIdx.x ) >> 6, lane = int( threadIdx.x ) & 63;
+   const int    N = 1 << L, w = N + 1, nc = surCoefs( deg ), q1 = deg + 1, n2 = q1 * ( q1 + 1 ) / 2;
    const double h  = 1.0 / double( N );
-   const int    py = live ? rows[2 * r] : 1, pz = live ? rows[2 * r + 1] : 1;
-   if ( live )
    {
-      // coefficients in x of the row's restriction: sum over b, c of P[a, b, c] y^b z^c (Horner in y and z)
-      const double  Y = py * h, Z = pz * h;
+      const double  Z = z * h;
       const double* P = poly + i64( c ) * NDIR * nc;
-      for ( int it = lane; it < NDIR * q1; it += 64 )
+      for ( int it = int( threadIdx.x ); it < NDIR * n2; it += 64 * kSurWaves )
       {
-         const int d = it / q1, aa = it - d * q1;
-         double    s = 0.0;
-         for ( int b = deg - aa; b >= 0; --b )
+         const int d = it / n2;
+         int       j = it - d * n2, aa = 0; // j = index of (aa, b) with aa + b <= deg, aa-major
+         while ( j > deg - aa )
          {
-            double t = 0.0;
-            for ( int cc = deg - aa - b; cc >= 0; --cc )
-               t = t * Z + P[d * nc + surMonomial( aa, b, cc )];
-            s = s * Y + t;
+            j -= deg - aa + 1;
+            ++aa;
          }
-         c1[wv][it] = s;
+         const int b = j;
+         double    t = 0.0;
+#pragma unroll
+         for ( int cc = kSurMaxDeg; cc >= 0; --cc )
+            if ( cc <= deg - aa - b )
+               t = t * Z + P[d * nc + surMonomial( aa, b, cc )];
+         pz[it] = t;
       }
    }
    __syncthreads();
-   if ( !live || !( flag & 1 ) )
+   if ( !( flag & 1 ) )
       return;
-   const int len = N - 1 - py - pz; // interior points x = 1 .. len
-   const int Wz = w - pz, tc = T2i( Wz ) - py, bc = T2i( Wz + 1 ) - py, nn = Wz - py;
-   const int off[NDIR] = { 0, -1, 1, nn, -( nn + 1 ), nn - 1, -nn, tc, tc - 1, tc - nn, tc - nn + 1, -bc, -bc + nn + 1, -bc + 1, -bc + nn };
-   const i64 rowBase   = i64( c ) * stride + cidx( w, 0, py, pz );
-   for ( int x0 = 1; x0 <= len; x0 += 64 )
+   const int M = N - 2 - z; // interior rows y = 1 .. M of the plane, row y has M + 1 - y points
+   const int nPairs = ( M + 1 ) / 2;
+   const int nIt    = NDIR * q1;
+   for ( int k = wv; k < nPairs; k += kSurWaves )
    {
-      const int px = x0 + lane;
-      if ( px > len )
-         break;
-      const double X = px * h;
-      double       Wd[NDIR];
+      const int y1 = 1 + k, y2 = M - k;
+      const int len1 = M - k, total = y1 == y2 ? len1 : M + 1;
+      // restriction of the plane polynomials to rows y1 and y2 (Horner in y)
+      for ( int it = lane; it < 2 * nIt; it += 64 )
+      {
+         const int    row = it >= nIt ? 1 : 0, rem = it - row * nIt;
+         const int    d = rem / q1, aa = rem - d * q1;
+         const double Y = ( row ? y2 : y1 ) * h;
+         const double* pd = pz + d * n2 + aa * ( deg + 1 ) - aa * ( aa - 1 ) / 2;
+         double        t  = 0.0;
 #pragma unroll
-      for ( int d = 0; d < NDIR; ++d )
-      {
-         double v = 0.0;
-         for ( int aa = deg; aa >= 0; --aa )
-            v = v * X + c1[wv][d * q1 + aa];
-         Wd[d] = v;
+         for ( int b = ( DEG >= 0 ? DEG : kSurMaxDeg ); b >= 0; --b )
+            if ( b <= deg - aa )
+               t = t * Y + pd[b];
+         c1[wv][row][rem] = t;
       }
-      const i64 q = rowBase + px;
-      if constexpr ( DIAG )
-         diagOut[q] = 1.0 / Wd[0];
-      else
+      __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+      for ( int p0 = 0; p0 < total; p0 += 64 )
       {
-         T val = T( 0 );
+         const int p = p0 + lane;
+         if ( p >= total )
+            break;
+         const int    row = p < len1 ? 0 : 1;
+         const int    px = 1 + ( row ? p - len1 : p ), py = row ? y2 : y1;
+         const double X  = px * h;
+         const double* cr = c1[wv][row];
+         double        Wd[NDIR];
 #pragma unroll
          for ( int d = 0; d < NDIR; ++d )
-            val += T( Wd[d] ) * x[q + off[d]];
-         T i1 = T( 0 ), r0, r1;
-         if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-            i1 = in1[q];
-         if constexpr ( MODE == MODE_APPLY_ADD )
-            i1 = out0[q];
-         epilogue< T, MODE >( val, x[q], i1, i1, MODE == MODE_APPLY || MODE == MODE_APPLY_ADD || MODE == MODE_RESIDUAL ? T( 0 ) : dinv[q], a,
-                              r0, r1 );
-         out0[q] = r0;
-         if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
-            out1[q] = r1;
+         {
+            double v = 0.0;
+            if constexpr ( DEG >= 0 )
+            {
+#pragma unroll
+               for ( int aa = DEG; aa >= 0; --aa )
+                  v = v * X + cr[d * q1 + aa];
+            }
+            else
+               for ( int aa = deg; aa >= 0; --aa )
+                  v = v * X + cr[d * q1 + aa];
+            Wd[d] = v;
+         }
+         const i64 q = i64( c ) * stride + cidx( w, px, py, z );
+         if constexpr ( DIAG )
+            diagOut[q] = 1.0 / Wd[0];
+         else
+         {
+            const int Wz = w - z, tc = T2i( Wz ) - py, bc = T2i( Wz + 1 ) - py, nn = Wz - py;
+            const int off[NDIR] = { 0, -1, 1, nn, -( nn + 1 ), nn - 1, -nn, tc, tc - 1, tc - nn, tc - nn + 1, -bc, -bc + nn + 1, -bc + 1, -bc + nn };
+            T         val = T( 0 );
+#pragma unroll
+            for ( int d = 0; d < NDIR; ++d )
+               val += T( Wd[d] ) * x[q + off[d]];
+            T i1 = T( 0 ), r0, r1;
+            if constexpr ( MODE == MODE_RESIDUAL || MODE == MODE_JACOBI || MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+               i1 = in1[q];
+            if constexpr ( MODE == MODE_APPLY_ADD )
+               i1 = out0[q];
+            epilogue< T, MODE >( val, x[q], i1, i1, dinv[q], a, r0, r1 );
+            out0[q] = r0;
+            if constexpr ( MODE == MODE_CHEB0 || MODE == MODE_CHEBK )
+               out1[q] = r1;
+         }
       }
+      __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
    }
 }
 
@@ -240,16 +284,32 @@ T* toDeviceArray( const std::vector< T >& h )
    return static_cast< T* >( d );
 }
 
+template < typename T, int MODE, bool DIAG, int DEG >
+void surLaunchDeg( const MeshCtx& m, const LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, double* diagOut, hipStream_t s )
+{
+   const hyg_op_s::Level& ol     = op.levels[L];
+   const int              planes = ( 1 << L ) - 3;
+   constexpr bool         dbl    = std::is_same< T, double >::value;
+   const T*               dv     = dbl ? (const T*) ol.dinv : (const T*) ol.dinvF;
+   hipLaunchKernelGGL( ( kSurrogate< T, MODE, DIAG, DEG > ), dim3( m.mesh->nC() * planes ), dim3( 64 * kSurWaves ), 0, s, L,
+                       op.surDegree, planes, ol.surPoly, g.stride, c.flag, dv, (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1,
+                       T( c.a ), diagOut );
+}
+
 template < typename T, int MODE, bool DIAG >
 void surLaunch( const MeshCtx& m, const LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, double* diagOut, hipStream_t s )
 {
-   const hyg_op_s::Level& ol = op.levels[L];
-   const int              rb = ceilDiv( ol.surNRows, kSurWaves );
-   constexpr bool         dbl = std::is_same< T, double >::value;
-   const T*               dv  = dbl ? (const T*) ol.dinv : (const T*) ol.dinvF;
-   hipLaunchKernelGGL( ( kSurrogate< T, MODE, DIAG > ), dim3( m.mesh->nC() * rb ), dim3( 64 * kSurWaves ), 0, s, L, op.surDegree,
-                       ol.surNRows, rb, ol.surRows, ol.surPoly, g.stride, c.flag, dv, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
-                       (T*) c.out1, T( c.a ), diagOut );
+   switch ( op.surDegree )
+   {
+   case 0: surLaunchDeg< T, MODE, DIAG, 0 >( m, g, op, L, c, diagOut, s ); break;
+   case 1: surLaunchDeg< T, MODE, DIAG, 1 >( m, g, op, L, c, diagOut, s ); break;
+   case 2: surLaunchDeg< T, MODE, DIAG, 2 >( m, g, op, L, c, diagOut, s ); break;
+   case 3: surLaunchDeg< T, MODE, DIAG, 3 >( m, g, op, L, c, diagOut, s ); break;
+   case 4: surLaunchDeg< T, MODE, DIAG, 4 >( m, g, op, L, c, diagOut, s ); break;
+   case 5: surLaunchDeg< T, MODE, DIAG, 5 >( m, g, op, L, c, diagOut, s ); break;
+   case 6: surLaunchDeg< T, MODE, DIAG, 6 >( m, g, op, L, c, diagOut, s ); break;
+   default: surLaunchDeg< T, MODE, DIAG, -1 >( m, g, op, L, c, diagOut, s ); break;
+   }
 }
 
 template < typename T >
@@ -297,20 +357,19 @@ void surrogateInterpolate( const MeshCtx& mc, hyg_op_s& op, int deg, int interpo
    {
       hyg_op_s::Level& ol = op.levels[L];
       (void) hipFree( ol.surPoly );
-      (void) hipFree( ol.surRows );
-      ol.surPoly  = nullptr;
-      ol.surRows  = nullptr;
-      ol.surNRows = 0;
-      const int N = 1 << L;
+      (void) hipFree( ol.surBnd );
+      ol.surPoly = nullptr;
+      ol.surBnd  = nullptr;
+      ol.surNBnd = 0;
+      const int N = 1 << L, w = N + 1;
       if ( N < 4 )
          continue; // no interior points
-      std::vector< int > rows;
-      for ( int z = 1; z <= N - 3; ++z )
-         for ( int y = 1; y <= N - 2 - z; ++y )
-         {
-            rows.push_back( y );
-            rows.push_back( z );
-         }
+      std::vector< int > bnd;
+      for ( int z = 0; z <= N; ++z )
+         for ( int y = 0; y + z <= N; ++y )
+            for ( int x = 0; x + y + z <= N; ++x )
+               if ( x == 0 || y == 0 || z == 0 || x + y + z == N )
+                  bnd.push_back( cidx( w, x, y, z ) );
       // sample points: the interior points of the interpolation level, scaled to level L
       const int Li = std::min( L, interpolationLevel ), Ni = 1 << Li, sh = L - Li;
       std::vector< int >    pts;
@@ -355,9 +414,9 @@ void surrogateInterpolate( const MeshCtx& mc, hyg_op_s& op, int deg, int interpo
             for ( int p = 0; p < np; ++p )
                rhs[( size_t( c ) * NDIR + d ) * np + p] = st[( size_t( c ) * np + p ) * NDIR + d];
       lsqColumnPivoted( vand, np, nc, rhs, nC * NDIR );
-      ol.surPoly  = toDeviceArray( rhs ); // [cell][direction][monomial]
-      ol.surRows  = toDeviceArray( rows );
-      ol.surNRows = int( rows.size() / 2 );
+      ol.surPoly = toDeviceArray( rhs ); // [cell][direction][monomial]
+      ol.surBnd  = toDeviceArray( bnd );
+      ol.surNBnd = int( bnd.size() );
    }
    op.surDegree = deg;
 }

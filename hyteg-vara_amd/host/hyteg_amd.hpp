@@ -251,6 +251,11 @@ class P1ElementwiseDivKGradOperator : public MatrixFreeOperator< P1Function< Val
                                   const std::vector< std::array< double, 5 > >& modes )
    : MatrixFreeOperator< P1Function< ValueType > >( storage, minLevel, maxLevel, make( storage, minLevel, maxLevel, k0, modes ) )
    {}
+   // polynomial coefficient k = sum_i coeffs[i] m_i(x), monomials of degree <= degree in MonomialBasis3D order
+   P1ElementwiseDivKGradOperator( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel, int degree,
+                                  const std::vector< double >& coeffs )
+   : MatrixFreeOperator< P1Function< ValueType > >( storage, minLevel, maxLevel, makePoly( storage, minLevel, maxLevel, degree, coeffs ) )
+   {}
    void computeAndStoreLocalElementMatrices() const { check( hyg_op_store_local_matrices( this->handle() ) ); }
 
  private:
@@ -261,6 +266,29 @@ class P1ElementwiseDivKGradOperator : public MatrixFreeOperator< P1Function< Val
       check( hyg_op_create_p1_divkgrad( storage->handle(), minLevel, maxLevel, k0, modes.empty() ? nullptr : modes[0].data(),
                                         int( modes.size() ), &op ) );
       return op;
+   }
+   static hyg_op makePoly( const std::shared_ptr< PrimitiveStorage >& storage, int minLevel, int maxLevel, int degree,
+                           const std::vector< double >& coeffs )
+   {
+      if ( int( coeffs.size() ) != ( degree + 1 ) * ( degree + 2 ) * ( degree + 3 ) / 6 )
+         throw std::runtime_error( "polynomial coefficient count does not match the degree" );
+      hyg_op op = nullptr;
+      check( hyg_op_create_p1_divkgrad_poly( storage->handle(), minLevel, maxLevel, degree, coeffs.data(), &op ) );
+      return op;
+   }
+};
+
+// P1SurrogateOperator< p1_div_k_grad_affine_q3 > (p1functionspace/P1SurrogateOperator.hpp): constructed like the
+// elementwise operator; interpolateStencils( polyDegree, interpolationLevel ) switches the cell interiors to the
+// fitted polynomial stencils.
+template < typename ValueType >
+class P1SurrogateAffineDivkGradOperator : public P1ElementwiseDivKGradOperator< ValueType >
+{
+ public:
+   using P1ElementwiseDivKGradOperator< ValueType >::P1ElementwiseDivKGradOperator;
+   void interpolateStencils( int polyDegree, int interpolationLevel ) const
+   {
+      check( hyg_op_interpolate_stencils( this->handle(), polyDegree, interpolationLevel ) );
    }
 };
 
