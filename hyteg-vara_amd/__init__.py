@@ -84,6 +84,7 @@ def _sig(lib):
         "hyg_op_get_config": (i, [vp, i, i, i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "hyg_smooth_jac": (i, [vp, vp, vp, vp, d, i, i, vp]),
         "hyg_smooth_sor": (i, [vp, vp, vp, d, i, i, i, vp]),
+        "hyg_smooth_sor_ordered": (i, [vp, vp, vp, d, i, i, i, vp]),
         "hyg_mesh_sor_colours": (i, [vp, C.POINTER(C.c_int), P(np.int32)]),
         "hyg_inverse_diagonal": (i, [vp, vp, i, vp]),
         "hyg_residual": (i, [vp, vp, vp, vp, i, i, vp]),
@@ -281,6 +282,10 @@ class P1ConstantOperator:
 
     def smooth_sor(self, dst, rhs, relax, L, flag=INNER, backwards=False):
         check(load().hyg_smooth_sor(self.h, dst.h, rhs.h, relax, L, flag, 1 if backwards else 0, None))
+
+    def smooth_sor_ordered(self, dst, rhs, relax, L, flag=INNER, backwards=False, stream=None):
+        """smooth_sor in the reference's own (lexicographic, primitive by primitive) order."""
+        check(load().hyg_smooth_sor_ordered(self.h, dst.h, rhs.h, relax, L, flag, 1 if backwards else 0, stream))
 
     def inverse_diagonal(self, dst, L):
         check(load().hyg_inverse_diagonal(self.h, dst.h, L, None))

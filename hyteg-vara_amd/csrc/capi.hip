@@ -1465,6 +1465,28 @@ int hyg_smooth_sor( hyg_op op, hyg_func dst, hyg_func rhs, double relax, int lev
    return launched( "sor" );
 }
 
+int hyg_smooth_sor_ordered( hyg_op op, hyg_func dst, hyg_func rhs, double relax, int level, int flag, int backwards, void* stream )
+{
+   level = lat( op, level );
+   HYG_TRY( checkOpLevel( op, level ) );
+   HYG_TRY( checkPair( dst, rhs, level ) );
+   HYG_TRY( checkOpFunc( op, dst ) );
+   if ( op->kind != 0 )
+      return fail( HYG_ERR_ARG, "smooth_sor is implemented for the constant P1 operator" );
+   if ( dst == rhs )
+      return fail( HYG_ERR_ARG, "smooth_sor requires dst different from rhs" );
+   MeshCtx& mc = op->mesh->ctx;
+   try
+   {
+      launchSorOrdered( mc, mc.geom( level ), op->levels[level], level, dst->dtype, dst->data[level], rhs->data[level], relax, flag,
+                        backwards != 0, S( stream ) );
+   } catch ( const std::exception& e )
+   {
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   return launched( "ordered sor" );
+}
+
 int hyg_mesh_sor_colours( hyg_mesh mesh, int* ncolours, int* vertex_colours )
 {
    HYG_TRY( checkInit() );

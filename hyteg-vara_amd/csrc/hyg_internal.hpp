@@ -579,6 +579,28 @@ struct hyg_func_s
    size_t                elem() const { return dtype == 8 ? 8 : 4; }
 };
 
+// Tables of the reference-order SOR of one level, built on first use: the lower-dimensional primitives by kind,
+// the face points by anti-diagonal i + 2j and the cell-interior points by hyperplane x + 2y + 3z (with offsets), and
+// the snapshot buffers of the function before a sweep (FP64, FP32).
+struct SorOrderTables
+{
+   int*               vertices = nullptr;
+   int*               edges    = nullptr;
+   int*               faces    = nullptr;
+   int                nVertices = 0, nEdges = 0, nFaces = 0;
+   int*               facePts  = nullptr; // i | j << 16
+   int*               faceOff  = nullptr;
+   int                nFaceTau = 0;
+   int*               cellPts  = nullptr; // cell-array indices
+   std::vector< int > cellOff;            // host: offsets per hyperplane
+   void*              snap64   = nullptr;
+   void*              snap32   = nullptr;
+   SorOrderTables()                        = default;
+   SorOrderTables( const SorOrderTables& ) = delete;
+   SorOrderTables& operator=( const SorOrderTables& ) = delete;
+   ~SorOrderTables();
+};
+
 struct hyg_op_s
 {
    hyg_mesh_s* mesh;
@@ -659,6 +681,8 @@ struct hyg_op_s
       double*               surPoly = nullptr;
       int*                  surBnd  = nullptr;
       int                   surNBnd = 0;
+      // constant P1 operator: tables of the reference-order SOR (hyg_smooth_sor_ordered), built on first use
+      std::unique_ptr< SorOrderTables > sorOrder;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

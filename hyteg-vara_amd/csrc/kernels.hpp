@@ -35,6 +35,12 @@ void launchStencil( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, i
 void launchSor( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L, int dtype, void* x, const void* b, double relax,
                 int flag, bool backwards, hipStream_t s );
 
+// One SOR sweep of the constant P1 operator in the reference's order (primitives by dimension, each lexicographic;
+// neighbours outside a primitive's closure and of no higher dimension read from a snapshot taken before the sweep).
+// Builds the level's tables and snapshot buffer on first use.
+void launchSorOrdered( const MeshCtx& m, const LevelGeom& g, hyg_op_s::Level& ol, int L, int dtype, void* x, const void* b, double relax,
+                       int flag, bool backwards, hipStream_t s );
+
 // Variable-coefficient elementwise operator (op.kind == 1): one stencil operation, and the inverse
 // diagonal of level L written into op.levels[L].dinv / dinvF (setup).
 void launchElementwise( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c,

@@ -238,6 +238,15 @@ int hyg_smooth_jac( hyg_op op, hyg_func dst, hyg_func rhs, hyg_func src, double 
  * points of one colour, which the stencil never couples, are updated together; backwards = 1 takes the
  * colours in reverse order. Constant P1 operator only. */
 int hyg_smooth_sor( hyg_op op, hyg_func dst, hyg_func rhs, double relax, int level, int flag, int backwards, void* stream );
+/* SORSmoothable::smooth_sor / smooth_sor_backwards in the reference's own order (P1Operator.hpp:343-416): macro-
+ * vertices, edges, faces, cells (backwards: cells, faces, edges, vertices), each primitive lexicographically
+ * (VertexDoFMacroFace.hpp:751-810, generatedKernels/sor_3D_macrocell_P1.cpp), with the reference's ghost-layer
+ * semantics: a point of primitive P reads a neighbour's updated value when the neighbour lies in P's closure or in a
+ * primitive of higher dimension, and its value before the sweep otherwise. x_p = (1 - relax) x_p + relax (rhs_p -
+ * sum_{q != p} a_pq x_q) / a_pp. Same results as the reference up to rounding; parallel over primitives, face
+ * anti-diagonals and cell hyperplanes x + 2y + 3z (one launch per hyperplane, about 3 * 2^level launches), so it is
+ * much slower than hyg_smooth_sor. Allocates a snapshot of the level on first use. Constant P1 operator only. */
+int hyg_smooth_sor_ordered( hyg_op op, hyg_func dst, hyg_func rhs, double relax, int level, int flag, int backwards, void* stream );
 /* The SOR colouring of a mesh: the number of colours and, if vertex_colours is not null, the colour of every
  * macro-vertex (nV entries). No counterpart in the reference (its SOR is lexicographic). */
 int hyg_mesh_sor_colours( hyg_mesh mesh, int* ncolours, int* vertex_colours );

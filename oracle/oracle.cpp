@@ -694,6 +694,160 @@ struct P1ConstantOperator : OperatorBase< T >
       }
    }
 
+   // SORSmoothable::smooth_sor( dst, rhs, relax, level, flag, backwards ) in the reference's order
+   // (P1Operator.hpp:343-416): after the six communications every copy of dst is consistent; then (forward)
+   // macro-vertices, edges, faces and cells are smoothed in turn, each primitive lexicographically (edges i = 1..N-1,
+   // faces j outer / i inner, cells z, y, x: VertexDoFMacroEdge.hpp, VertexDoFMacroFace.hpp:751-810 smoothSOR3D,
+   // generatedKernels/sor_3D_macrocell_P1.cpp), with a downward communication after each phase; backwards the
+   // phases and the visiting order are reversed and the communications go upward. A primitive reads its
+   // neighbours through its own memory: its own values and those of its closure are the current ones, the ghost
+   // layers hold what the last communication left there. Hence the rule used here (dst keeps every copy up to
+   // date, `old` is dst before the sweep): neighbour q of a point of primitive P is read from dst when q lies in P's
+   // closure or in a primitive of higher dimension than P (which the sweep has not reached forward, and whose
+   // updated values the upward communications bring backwards), and from `old` otherwise. Update:
+   // x = (1 - relax) x + relax (b - sum_{q != p} a_pq x_q) / a_pp.
+   void smoothSorOrdered( Function< T >& dst, const Function< T >& rhs, double relax, int L, int flag, bool backwards ) const
+   {
+      dst.commDown( L );
+      rhs.commDown( L );
+      const Function< T > old = dst;
+      const idx           N   = idx( 1 ) << L, w = N + 1;
+      // sum over the incident cells' one-sided stencils, centre excluded, reading each neighbour per the rule;
+      // pv / bw: the point as barycentric weights over the global vertices pv of its primitive (dimension nv - 1)
+      auto offSum = [&]( const LowerStencil& ls, const int* pv, const idx* bw, int nv, double& centre ) {
+         double sum = 0.0;
+         centre     = 0.0;
+         for ( int k = 0; k < ls.nCells; ++k )
+         {
+            const int c = ls.cells[k];
+            const I3  q = cellCoords( *m, c, pv, bw, nv );
+            centre += ls.w[k][V_C];
+            for ( int d = 1; d < NDIR; ++d )
+            {
+               if ( ls.w[k][d] == 0.0 )
+                  continue;
+               const idx x = q[0] + DOFF[d][0], y = q[1] + DOFF[d][1], z = q[2] + DOFF[d][2];
+               if ( !inCell( N, x, y, z ) )
+                  continue;
+               const idx b[4] = { N - x - y - z, x, y, z };
+               int       nz   = 0;
+               bool      sub  = true;
+               for ( int l = 0; l < 4; ++l )
+                  if ( b[l] != 0 )
+                  {
+                     ++nz;
+                     bool in = false;
+                     for ( int k2 = 0; k2 < nv; ++k2 )
+                        in = in || m->Cl[c][l] == pv[k2];
+                     sub = sub && in;
+                  }
+               const bool cur = nz > nv || sub;
+               sum += ls.w[k][d] * double( ( cur ? dst : old ).c[L][c][cidx( w, x, y, z )] );
+            }
+         }
+         return sum;
+      };
+      // new value of a lower-dimensional point, written into its owned array slot and every cell copy
+      auto updLower = [&]( const LowerStencil& ls, const int* pv, const idx* bw, int nv, T& own, T rhsv ) {
+         double     centre;
+         const double s  = offSum( ls, pv, bw, nv, centre );
+         const T      nv_ = T( ( 1.0 - relax ) * double( own ) + relax * ( double( rhsv ) - s ) * ( 1.0 / centre ) );
+         own              = nv_;
+         for ( int k = 0; k < ls.nCells; ++k )
+         {
+            const I3 q = cellCoords( *m, ls.cells[k], pv, bw, nv );
+            dst.c[L][ls.cells[k]][cidx( w, q[0], q[1], q[2] )] = nv_;
+         }
+      };
+      auto vertices = [&]() {
+         for ( size_t vi = 0; vi < m->X.size(); ++vi )
+         {
+            if ( !( bcType( m->vB[vi] ) & flag ) )
+               continue;
+            int pv[1] = { (int) vi };
+            idx bw[1] = { N };
+            updLower( vSt[L][vi], pv, bw, 1, dst.v[L][vi][0], rhs.v[L][vi][0] );
+         }
+      };
+      auto edges = [&]() {
+         if ( L < 1 )
+            return;
+         for ( size_t ei = 0; ei < m->E.size(); ++ei )
+         {
+            if ( !( bcType( m->eB[ei] ) & flag ) )
+               continue;
+            int pv[2] = { m->E[ei][0], m->E[ei][1] };
+            for ( idx s = 1; s <= N - 1; ++s )
+            {
+               const idx i     = backwards ? N - s : s;
+               idx       bw[2] = { N - i, i };
+               updLower( eSt[L][ei], pv, bw, 2, dst.e[L][ei][i], rhs.e[L][ei][i] );
+            }
+         }
+      };
+      auto faces = [&]() {
+         if ( L < 2 )
+            return;
+         std::vector< std::array< idx, 2 > > pts;
+         for ( idx j = 1; j <= N - 2; ++j )
+            for ( idx i = 1; i + j <= N - 1; ++i )
+               pts.push_back( { i, j } );
+         if ( backwards )
+            std::reverse( pts.begin(), pts.end() );
+         for ( size_t fi = 0; fi < m->F.size(); ++fi )
+         {
+            if ( !( bcType( m->fB[fi] ) & flag ) )
+               continue;
+            int pv[3] = { m->F[fi][0], m->F[fi][1], m->F[fi][2] };
+            for ( const auto& ij : pts )
+            {
+               idx bw[3] = { N - ij[0] - ij[1], ij[0], ij[1] };
+               updLower( fSt[L][fi], pv, bw, 3, dst.f[L][fi][fidx( w, ij[0], ij[1] )], rhs.f[L][fi][fidx( w, ij[0], ij[1] )] );
+            }
+         }
+      };
+      auto cells = [&]() {
+         if ( L < 2 || !( flag & Inner ) )
+            return;
+         std::vector< std::array< idx, 3 > > pts;
+         for ( idx z = 1; z <= N - 3; ++z )
+            for ( idx y = 1; y + z <= N - 2; ++y )
+               for ( idx x = 1; x + y + z <= N - 1; ++x )
+                  pts.push_back( { x, y, z } );
+         if ( backwards )
+            std::reverse( pts.begin(), pts.end() );
+         for ( size_t ci = 0; ci < m->Cl.size(); ++ci )
+         {
+            auto&       a  = dst.c[L][ci];
+            const auto& r  = rhs.c[L][ci];
+            const auto& st = cellSt[L][ci];
+            for ( const auto& p : pts )
+            {
+               double sum = 0.0;
+               for ( int d = 1; d < NDIR; ++d )
+                  sum += st[d] * double( a[cidx( w, p[0] + DOFF[d][0], p[1] + DOFF[d][1], p[2] + DOFF[d][2] )] );
+               const idx q = cidx( w, p[0], p[1], p[2] );
+               a[q]        = T( ( 1.0 - relax ) * double( a[q] ) + relax * ( double( r[q] ) - sum ) * ( 1.0 / st[V_C] ) );
+            }
+         }
+      };
+      if ( backwards )
+      {
+         cells();
+         faces();
+         edges();
+         vertices();
+      }
+      else
+      {
+         vertices();
+         edges();
+         faces();
+         cells();
+      }
+      dst.commDown( L );
+   }
+
    // generated kernel apply_3D_macrocell_vertexdof_to_vertexdof_{replace,add}.cpp:32-87, one z-plane:
    // the 15 coefficients are hoisted, the row offsets computed once per (y, z) row.
    static void applyCellPlane( const T* s, T* d, const std::array< double, NDIR >& st, int L, idx z, bool add )
@@ -3030,6 +3184,29 @@ int or_op_variable_stencil( void* oh, int L, int c, long x, long y, long z, doub
 
 // P2ConstantOperator< p2_tet_diffusion > on P2 levels minL..maxL; functions used with it hold lattice levels
 // minL+1..maxL+1 and every level argument of the operator calls below is a lattice level
+// SORSmoothable::smooth_sor in the reference's order (constant-coefficient operators of or_op_new)
+int or_smooth_sor_ordered( void* oh, void* dh, void* rh, double relax, int L, int flag, int backwards )
+{
+   auto* o = static_cast< OrOp* >( oh );
+   if ( o->prec == 4 )
+   {
+      auto* op = dynamic_cast< P1ConstantOperator< float >* >( o->f.get() );
+      if ( !op )
+         return 1;
+      op->smoothSorOrdered( fget< float >( static_cast< OrFunc* >( dh ) ), fget< float >( static_cast< OrFunc* >( rh ) ), relax, L, flag,
+                            backwards != 0 );
+   }
+   else
+   {
+      auto* op = dynamic_cast< P1ConstantOperator< double >* >( o->d.get() );
+      if ( !op )
+         return 1;
+      op->smoothSorOrdered( fget< double >( static_cast< OrFunc* >( dh ) ), fget< double >( static_cast< OrFunc* >( rh ) ), relax, L,
+                            flag, backwards != 0 );
+   }
+   return 0;
+}
+
 void* or_op_new_p2( void* mh, int minL, int maxL, int prec )
 {
    auto* o = new OrOp;

@@ -42,6 +42,7 @@ def lib():
             "or_op_free": (None, [vp]),
             "or_op_new_divkgrad": (vp, [vp, i, i, i, i, d, d, P(np.float64), i]),
             "or_op_new_divkgrad_poly": (vp, [vp, i, i, i, P(np.float64), i]),
+            "or_smooth_sor_ordered": (i, [vp, vp, vp, d, i, i, i]),
             "or_op_interpolate_stencils": (i, [vp, i, i]),
             "or_op_surrogate_poly": (i, [vp, i, i, P(np.float64)]),
             "or_op_variable_stencil": (i, [vp, i, i, C.c_long, C.c_long, C.c_long, P(np.float64)]),

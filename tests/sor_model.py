@@ -75,3 +75,52 @@ def sor_sweep(apply, dinv, x, b, relax, colours, m, mask, backwards=False):
         sel = mask & (colours == k)
         x[sel] += relax * dinv[sel] * r[sel]
     return x
+
+
+# ------------------------------------------------------------------ SOR in the reference's order
+def canonical_primitives(om, L):
+    """For every DoF in the canonical order (macro-vertices, then the interior points of every edge, face and cell,
+    each primitive in storage order): the set of global macro-vertex ids of the primitive it belongs to."""
+    N = 1 << L
+    edges, faces = om.prims()
+    cells = om.cells()
+    nv = int(om.info()[0])
+    out = [frozenset([v]) for v in range(nv)]
+    for e in edges:
+        out += [frozenset(int(v) for v in e)] * (N - 1)
+    nf = (N - 1) * (N - 2) // 2 if N >= 3 else 0
+    for f in faces:
+        out += [frozenset(int(v) for v in f)] * nf
+    nc = (N - 1) * (N - 2) * (N - 3) // 6 if N >= 4 else 0
+    for c in cells:
+        out += [frozenset(int(v) for v in c)] * nc
+    assert len(out) == om.ndofs(L)
+    return out
+
+
+def ordered_sor(A, x, b, relax, prims, update, backwards=False, closure_rule=True):
+    """One SOR sweep in the order of SORSmoothable::smooth_sor (P1Operator.hpp:343-416), written on the global
+    matrix A (canonical order): points are visited in canonical order (backwards: reversed), which visits vertices,
+    edges, faces and cells in turn and every primitive lexicographically. A point p of primitive P (vertex set
+    prims[p]) reads neighbour q's current value when q's primitive lies in P's closure (prims[q] a subset of prims[p])
+    or has a higher dimension, and the value before the sweep otherwise (the reference reads those from ghost
+    layers filled before the phase). Points with update[p] False keep their values. closure_rule=False reads every
+    neighbour's current value (plain Gauss-Seidel in the same order).
+    x_p <- (1 - relax) x_p + relax (b_p - sum_{q != p} a_pq x_q) / a_pp."""
+    A = A.tocsr()
+    old = np.array(x, dtype=np.float64)
+    x = old.copy()
+    order = range(len(x) - 1, -1, -1) if backwards else range(len(x))
+    for p in order:
+        if not update[p]:
+            continue
+        s, c = 0.0, 0.0
+        for k in range(A.indptr[p], A.indptr[p + 1]):
+            q, a = A.indices[k], A.data[k]
+            if q == p:
+                c = a
+                continue
+            cur = not closure_rule or len(prims[q]) > len(prims[p]) or prims[q] <= prims[p]
+            s += a * (x[q] if cur else old[q])
+        x[p] = (1.0 - relax) * x[p] + relax * (b[p] - s) / c
+    return x
