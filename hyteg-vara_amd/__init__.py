@@ -385,7 +385,7 @@ class GMG:
     def __init__(self, op, minL, maxL, dtype=F64, pre=2, post=2, inc=0, smoother="jacobi", relax=2.0 / 3.0,
                  cheb_order=2, cheb_bounds=(0.0, 0.0), cg_max_iter=10000, cg_tol=1e-16):
         self.op, self.maxL = op, maxL
-        ip = np.array([minL, maxL, pre, post, inc, {"jacobi": 0, "cheb": 1, "sor": 2, "ssor": 3}[smoother], cheb_order, cg_max_iter],
+        ip = np.array([minL, maxL, pre, post, inc, {"jacobi": 0, "cheb": 1, "sor": 2, "ssor": 3, "sor_ordered": 4, "ssor_ordered": 5}[smoother], cheb_order, cg_max_iter],
                       dtype=np.int32)
         dp = np.array([relax, cheb_bounds[0], cheb_bounds[1], cg_tol])
         h = _out()

@@ -317,6 +317,15 @@ int smoothImpl( hyg_gmg_s* g, hyg_func_s* x, hyg_func_s* b, int L, hipStream_t s
       c.a    = g->relax;
       return stencilImpl( g->op, L, g->dtype, c, st );
    }
+   if ( g->smoother >= 4 )
+   {
+      // the same in the reference's own order (smooth_sor_ordered)
+      MeshCtx& mc = g->op->mesh->ctx;
+      launchSorOrdered( mc, mc.geom( L ), g->op->levels[L], L, g->dtype, x->data[L], b->data[L], g->relax, kGmgFlag, false, st );
+      if ( g->smoother == 5 )
+         launchSorOrdered( mc, mc.geom( L ), g->op->levels[L], L, g->dtype, x->data[L], b->data[L], g->relax, kGmgFlag, true, st );
+      return launched( "ordered sor" );
+   }
    if ( g->smoother >= 2 )
    {
       // SORSmoother::solve: smooth_sor(x, b, relax, level, flag), multi-colour order; SymmetricSORSmoother::solve
@@ -1669,8 +1678,9 @@ int hyg_gmg_create( hyg_op op, int dtype, const int* ip, const double* dp, hyg_g
       return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
    if ( lat( op, ip[0] ) < op->minL || lat( op, ip[1] ) > op->maxL || ip[0] > ip[1] )
       return fail( HYG_ERR_ARG, "solver levels outside the operator's range" );
-   if ( ip[5] < 0 || ip[5] > 3 )
-      return fail( HYG_ERR_ARG, "smoother must be 0 (weighted Jacobi), 1 (Chebyshev), 2 (SOR) or 3 (symmetric SOR)" );
+   if ( ip[5] < 0 || ip[5] > 5 )
+      return fail( HYG_ERR_ARG, "smoother must be 0 (weighted Jacobi), 1 (Chebyshev), 2 (SOR), 3 (symmetric SOR), 4 (SOR in the "
+                                "reference's order) or 5 (symmetric SOR in the reference's order)" );
    if ( ip[5] >= 2 && op->kind != 0 )
       return fail( HYG_ERR_ARG, "the SOR smoother is implemented for the constant P1 operator" );
    if ( ip[5] == 1 && ( ip[6] < 1 || ip[6] > 5 ) )
@@ -1692,6 +1702,20 @@ int hyg_gmg_create( hyg_op op, int dtype, const int* ip, const double* dp, hyg_g
    g->cgTol     = dp[3];
    if ( g->smoother == 1 )
       hyg_cheb_coefficients( g->chebOrder, g->chebLo, g->chebHi, g->cheb );
+   if ( g->smoother >= 4 )
+   {
+      // tables and snapshots now, so that neither a cycle nor its capture allocates
+      MeshCtx& mc = op->mesh->ctx;
+      try
+      {
+         for ( int L = g->minL; L <= g->maxL; ++L )
+            prepareSorOrdered( mc, mc.geom( L ), op->levels[L], L, dtype );
+      } catch ( const std::exception& e )
+      {
+         delete g;
+         return fail( HYG_ERR_HIP, e.what() );
+      }
+   }
    hyg_mesh_s* m  = op->mesh;
    int         rc = HYG_OK;
    for ( hyg_func_s** f : { &g->tmp, &g->jt, &g->t1, &g->t2, &g->cy, &g->cgp, &g->cgz, &g->cgap, &g->cgr } )

@@ -40,6 +40,8 @@ void launchSor( const MeshCtx& m, LevelGeom& g, const hyg_op_s::Level& ol, int L
 // Builds the level's tables and snapshot buffer on first use.
 void launchSorOrdered( const MeshCtx& m, const LevelGeom& g, hyg_op_s::Level& ol, int L, int dtype, void* x, const void* b, double relax,
                        int flag, bool backwards, hipStream_t s );
+// builds the tables and the snapshot buffer of one level and value type (so that later sweeps allocate nothing)
+void prepareSorOrdered( const MeshCtx& m, const LevelGeom& g, hyg_op_s::Level& ol, int L, int dtype );
 
 // Variable-coefficient elementwise operator (op.kind == 1): one stencil operation, and the inverse
 // diagonal of level L written into op.levels[L].dinv / dinvF (setup).

@@ -276,23 +276,24 @@ static void sorOrderedT( const MeshCtx& m, const LevelGeom& g, hyg_op_s::Level& 
    }
 }
 
-void launchSorOrdered( const MeshCtx& m, const LevelGeom& g, hyg_op_s::Level& ol, int L, int dtype, void* x, const void* b, double relax,
-                       int flag, bool backwards, hipStream_t s )
+void prepareSorOrdered( const MeshCtx& m, const LevelGeom& g, hyg_op_s::Level& ol, int L, int dtype )
 {
    SorOrderTables& t = sorOrderTables( m, ol, L );
    const size_t    n = size_t( m.mesh->nC() ) * size_t( g.stride ) + size_t( kTailSlack );
+   void**          p = dtype == 8 ? &t.snap64 : &t.snap32;
+   if ( !*p && hipMalloc( p, n * ( dtype == 8 ? sizeof( double ) : sizeof( float ) ) ) != hipSuccess )
+      throw std::runtime_error( "hipMalloc failed for the SOR snapshot" );
+}
+
+void launchSorOrdered( const MeshCtx& m, const LevelGeom& g, hyg_op_s::Level& ol, int L, int dtype, void* x, const void* b, double relax,
+                       int flag, bool backwards, hipStream_t s )
+{
+   prepareSorOrdered( m, g, ol, L, dtype );
+   SorOrderTables& t = *ol.sorOrder;
    if ( dtype == 8 )
-   {
-      if ( !t.snap64 && hipMalloc( &t.snap64, n * sizeof( double ) ) != hipSuccess )
-         throw std::runtime_error( "hipMalloc failed for the SOR snapshot" );
       sorOrderedT< double >( m, g, ol, t, L, (double*) x, (const double*) b, (double*) t.snap64, relax, flag, backwards, s );
-   }
    else
-   {
-      if ( !t.snap32 && hipMalloc( &t.snap32, n * sizeof( float ) ) != hipSuccess )
-         throw std::runtime_error( "hipMalloc failed for the SOR snapshot" );
       sorOrderedT< float >( m, g, ol, t, L, (float*) x, (const float*) b, (float*) t.snap32, float( relax ), flag, backwards, s );
-   }
 }
 
 } // namespace hyg

@@ -286,7 +286,9 @@ int hyg_power_iteration( hyg_op op, hyg_func x, hyg_func tmp, int iterations, in
  * (smoother = 0, solvers/WeightedJacobiSmoother.hpp:42-62), ChebyshevSmoother (smoother = 1,
  * ChebyshevSmoother.hpp:48-97), SORSmoother (smoother = 2, solvers/SORSmoother.hpp:38-50, forward multi-colour
  * sweeps as hyg_smooth_sor) or SymmetricSORSmoother (smoother = 3, solvers/SymmetricSORSmoother.hpp:37-60, a
- * forward then a backward sweep), SOR for the constant P1 operator only, and a CGSolver coarse solver
+ * forward then a backward sweep); smoother = 4 and 5 are the same two in the reference's own visiting order
+ * (hyg_smooth_sor_ordered, same iterates as the reference up to rounding); SOR for the constant P1 operator only,
+ * and a CGSolver coarse solver
  * (solvers/CGSolver.hpp:66-134).
  * iparams = {min_level, max_level, pre, post, increment, smoother, cheb_order, cg_max_iter}
  * dparams = {relax, cheb_lower, cheb_upper, cg_tolerance}

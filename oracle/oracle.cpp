@@ -2247,6 +2247,17 @@ struct Solvers
    {
       if ( smoother == 0 )
          jacobiSolve( x, b, L );
+      else if ( smoother >= 4 )
+      {
+         // SORSmoother / SymmetricSORSmoother::solve (solvers/SORSmoother.hpp:38-50, SymmetricSORSmoother.hpp:37-60)
+         // with smooth_sor in the reference's order
+         const auto* c = dynamic_cast< const P1ConstantOperator< T >* >( A );
+         if ( !c )
+            throw std::invalid_argument( "SOR smoothing needs the constant P1 operator" );
+         c->smoothSorOrdered( x, b, relax, L, gmgFlag, false );
+         if ( smoother == 5 )
+            c->smoothSorOrdered( x, b, relax, L, gmgFlag, true );
+      }
       else
          chebyshevSolve( x, b, L );
    }

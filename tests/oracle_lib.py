@@ -294,7 +294,8 @@ def cheb_coefficients(order, lower, upper):
 
 def gmg(op, x, b, minL, maxL, cycles, pre=2, post=2, inc=0, smoother="jacobi", relax=2.0 / 3.0,
         cheb_order=2, cheb_bounds=(0.0, 0.0), cg_max_iter=10000, cg_tol=1e-16):
-    ip = np.array([minL, maxL, pre, post, inc, 0 if smoother == "jacobi" else 1, cheb_order, cg_max_iter], dtype=np.int32)
+    kind = {"jacobi": 0, "cheb": 1, "sor_ordered": 4, "ssor_ordered": 5}[smoother]
+    ip = np.array([minL, maxL, pre, post, inc, kind, cheb_order, cg_max_iter], dtype=np.int32)
     dp = np.array([relax, cheb_bounds[0], cheb_bounds[1], cg_tol])
     res = np.zeros(cycles + 1)
     lib().or_gmg(op.h, x.h, b.h, ip, dp, cycles, res)

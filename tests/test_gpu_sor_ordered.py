@@ -102,3 +102,43 @@ def test_level6_invariants(hyg, meshes):
     op.apply(x, ax, L, hyg.INNER)
     res = b0 - ax.download(L)
     assert np.abs(res[last]).max() <= 1e-12 * np.abs(b0).max()
+
+
+def test_reference_convergence_test_in_reference_order(hyg):
+    # P1GMG3DConvergenceTest.cpp:36-125 with the reference's own Gauss-Seidel order: regular_octahedron_8el.msh,
+    # levels 0..3, V(3,3), u random in the interior and sin(x) sinh(y) z on the boundary, f = 0; every cycle reduces
+    # the squared residual by a factor below 3.2e-2, and the residual history is the oracle's (1e-10 per cycle)
+    name, minL, L, cycles = "regular_octahedron_8el.msh", 0, 3, 4
+    gm, om = hyg.Mesh(name), ol.Mesh(name)
+    xyz, bnd = gm.coords(L)
+    u0 = np.random.default_rng(0).uniform(0.0, 1.0, len(xyz))
+    u0[bnd] = np.sin(xyz[bnd, 0]) * np.sinh(xyz[bnd, 1]) * xyz[bnd, 2]
+    op = hyg.P1ConstantOperator(gm, minL, L)
+    x, b = hyg.Function(gm, minL, L), hyg.Function(gm, minL, L)
+    x.upload(L, u0)
+    print("GMG with ordered Gauss-Seidel", flush=True)
+    res = hyg.GMG(op, minL, L, pre=3, post=3, smoother="sor_ordered", relax=1.0).solve(x, b, L, cycles)
+    sq = (np.array(res[1:]) / np.array(res[:-1])) ** 2
+    assert np.all(sq < 3.2e-2), sq
+    ox, ob = ol.Function(om, minL, L), ol.Function(om, minL, L)
+    ox.set(L, u0)
+    ref = ol.gmg(ol.Operator(om, minL, L), ox, ob, minL, L, cycles, pre=3, post=3, smoother="sor_ordered", relax=1.0)
+    assert np.all(np.abs(res - ref) <= 1e-10 * ref), (res, ref)
+
+
+@pytest.mark.parametrize("name", ["cube_24el.msh", "cube_kuhn_6el.msh"])
+def test_symmetric_ordered_sor_vcycles_match_the_oracle(hyg, name):
+    minL, L, cycles = 0, 4, 3
+    gm, om = hyg.Mesh(name), ol.Mesh(name)
+    _, bnd = gm.coords(L)
+    b0 = np.random.default_rng(33).uniform(-1, 1, len(bnd))
+    b0[bnd] = 0.0
+    op = hyg.P1ConstantOperator(gm, minL, L)
+    x, b = hyg.Function(gm, minL, L), hyg.Function(gm, minL, L)
+    b.upload(L, b0)
+    print(f"GMG with symmetric ordered SOR on {name}", flush=True)
+    res = hyg.GMG(op, minL, L, pre=2, post=2, smoother="ssor_ordered", relax=1.2).solve(x, b, L, cycles)
+    ox, ob = ol.Function(om, minL, L), ol.Function(om, minL, L)
+    ob.set(L, b0)
+    ref = ol.gmg(ol.Operator(om, minL, L), ox, ob, minL, L, cycles, pre=2, post=2, smoother="ssor_ordered", relax=1.2)
+    assert np.all(np.abs(res - ref) <= 1e-10 * ref), (res, ref)
