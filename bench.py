@@ -531,6 +531,13 @@ def bench_p2(hyg, hip, st, steps, level, seed):
     op = hyg.P2ConstantOperator(mesh, 0, L)
     out = {"workload": f"P2 -Laplace, cube_24el.msh P2 level {L} ({n} DoFs)", "dofs_inner": inner,
            "algorithmic_bytes_per_dof_f64": 16, "flops_per_dof": 150}
+    # row-kernel configuration (segment length x planes in flight) per level, value type and mode
+    tuned = {}
+    for lv in range(max(1, L - 2), L + 1):
+        for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):
+            tuned[f"L{lv}_{name}"] = op.tune(lv, dt, reps=3, stream=st)
+    out["row_kernel_config"] = {"tuned": tuned, "meaning": "config k: z-segments of (8, 16, 32, 64)[k // 2] planes, "
+                                "k % 2 + 1 planes in flight; modes apply, add, residual, Jacobi, Chebyshev first, next"}
     rng = np.random.default_rng(seed)
     for dt, name in ((hyg.F64, "f64"), (hyg.F32, "f32")):
         x, y = hyg.Function(mesh, L, L, dt, space="p2"), hyg.Function(mesh, L, L, dt, space="p2")
@@ -542,8 +549,9 @@ def bench_p2(hyg, hip, st, steps, level, seed):
         for _ in range(2):
             ap()
         ms = timed(hip, st, ap, steps) / steps
-        out[f"apply_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3),
-                                "GB/s": round(inner * (16 if dt == hyg.F64 else 8) / (ms / 1e3) / 1e9, 1)}
+        gbs = inner * (16 if dt == hyg.F64 else 8) / (ms / 1e3) / 1e9
+        out[f"apply_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3), "GB/s": round(gbs, 1),
+                                "roofline_frac": round(gbs / PEAK_HBM_GBS, 4)}
         del x, y
     b = hyg.Function(mesh, 0, L, space="p2")
     bv = rng.uniform(-1.0, 1.0, n)
@@ -749,8 +757,13 @@ def main():
     def jac_once():
         hyg.check(lib.hyg_smooth_jac(op.h, y.h, x.h, x.h, 2.0 / 3.0, L, hyg.INNER, st))
 
+    def sor_ordered_once():
+        hyg.check(lib.hyg_smooth_sor_ordered(op.h, x.h, y.h, 1.0, L, hyg.INNER, 0, st))
+
     sor_once()
     ms_sor = timed(hip, st, sor_once, 5) / 5
+    sor_ordered_once()
+    ms_sor_ord = timed(hip, st, sor_ordered_once, 3) / 3
     jac_once()
     ms_jac = timed(hip, st, jac_once, 10) / 10
 
@@ -778,7 +791,10 @@ def main():
                                "tuned_per_mode": tuned},
         "smoothers_f64": {"jacobi_sweep_ms": round(ms_jac, 4),
                           "sor_sweep_ms": round(ms_sor, 4), "sor_colours": ncol,
-                          "note": "multi-colour SOR: one sweep-kernel pass plus a face pass per colour"},
+                          "sor_reference_order_sweep_ms": round(ms_sor_ord, 4),
+                          "note": "multi-colour SOR: one sweep-kernel pass plus a face pass per colour; reference "
+                                  "order: vertices, edges, faces (anti-diagonals), then one launch per cell hyperplane "
+                                  "x + 2y + 3z"},
     }
 
     # ---------------- time to 1e-10: FP64 vs mixed precision

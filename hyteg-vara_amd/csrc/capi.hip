@@ -1273,8 +1273,8 @@ int hyg_op_tune( hyg_op op, int level, int dtype, int reps, int* chosen, void* s
 {
    level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
-   if ( op->kind != 0 )
-      return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operator only" );
+   if ( op->kind != 0 && op->kind != 2 )
+      return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operators only" );
    if ( dtype != HYG_F64 && dtype != HYG_F32 )
       return fail( HYG_ERR_ARG, "dtype must be HYG_F64 or HYG_F32" );
    if ( reps < 1 )
@@ -1287,13 +1287,13 @@ int hyg_op_tune( hyg_op op, int level, int dtype, int reps, int* chosen, void* s
       for ( hyg_func_s* f : { a, b, r, t } )
          destroyFunc( f );
    };
-   int rc = createFunc( op->mesh, level, level, dtype, &a );
+   int rc = createFunc( op->mesh, level, level, dtype, &a, op->lat );
    if ( rc == HYG_OK )
-      rc = createFunc( op->mesh, level, level, dtype, &b );
+      rc = createFunc( op->mesh, level, level, dtype, &b, op->lat );
    if ( rc == HYG_OK )
-      rc = createFunc( op->mesh, level, level, dtype, &r );
+      rc = createFunc( op->mesh, level, level, dtype, &r, op->lat );
    if ( rc == HYG_OK )
-      rc = createFunc( op->mesh, level, level, dtype, &t );
+      rc = createFunc( op->mesh, level, level, dtype, &t, op->lat );
    if ( rc != HYG_OK )
    {
       release();
@@ -1322,14 +1322,25 @@ int hyg_op_tune( hyg_op op, int level, int dtype, int reps, int* chosen, void* s
       c.out1 = t->data[level];
       float best = 0.0f;
       int   bestCfg = 0;
-      for ( int cfg = 0; cfg < kNumCellCfgs && status == HYG_OK; ++cfg )
-      {
-         setCellConfig( g, dtype, mode, cfg );
-         for ( int k = 0; k < 2; ++k )
+      const bool p2   = op->kind == 2;
+      const int  nCfg = p2 ? kNumP2Cfgs : kNumCellCfgs;
+      auto       run  = [&]() {
+         if ( p2 )
+            launchP2( mc, g, *op, level, dtype, c, st );
+         else
             launchStencil( mc, g, op->levels[level], level, dtype, c, st );
+      };
+      for ( int cfg = 0; cfg < nCfg && status == HYG_OK; ++cfg )
+      {
+         if ( p2 )
+            setP2Config( g, dtype, mode, cfg );
+         else
+            setCellConfig( g, dtype, mode, cfg );
+         for ( int k = 0; k < 2; ++k )
+            run();
          (void) hipEventRecord( e0, st );
          for ( int k = 0; k < reps; ++k )
-            launchStencil( mc, g, op->levels[level], level, dtype, c, st );
+            run();
          (void) hipEventRecord( e1, st );
          float ms = 0.0f;
          if ( hipEventSynchronize( e1 ) != hipSuccess || hipEventElapsedTime( &ms, e0, e1 ) != hipSuccess )
@@ -1340,7 +1351,10 @@ int hyg_op_tune( hyg_op op, int level, int dtype, int reps, int* chosen, void* s
             bestCfg = cfg;
          }
       }
-      setCellConfig( g, dtype, mode, bestCfg );
+      if ( p2 )
+         setP2Config( g, dtype, mode, bestCfg );
+      else
+         setCellConfig( g, dtype, mode, bestCfg );
       if ( chosen )
          chosen[mode] = bestCfg;
    }
@@ -1356,11 +1370,15 @@ int hyg_op_set_config( hyg_op op, int level, int dtype, int mode, int config )
 {
    level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
-   if ( op->kind != 0 )
-      return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operator only" );
-   if ( ( dtype != HYG_F64 && dtype != HYG_F32 ) || mode < 0 || mode > 5 || config < 0 || config >= kNumCellCfgs )
+   if ( op->kind != 0 && op->kind != 2 )
+      return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operators only" );
+   if ( ( dtype != HYG_F64 && dtype != HYG_F32 ) || mode < 0 || mode > 5 || config < 0 ||
+        config >= ( op->kind == 2 ? kNumP2Cfgs : kNumCellCfgs ) )
       return fail( HYG_ERR_ARG, "bad dtype, mode or configuration" );
-   setCellConfig( op->mesh->ctx.geom( level ), dtype, mode, config );
+   if ( op->kind == 2 )
+      setP2Config( op->mesh->ctx.geom( level ), dtype, mode, config );
+   else
+      setCellConfig( op->mesh->ctx.geom( level ), dtype, mode, config );
    return HYG_OK;
 }
 
@@ -1368,10 +1386,21 @@ int hyg_op_get_config( hyg_op op, int level, int dtype, int mode, int* config, i
 {
    level = lat( op, level );
    HYG_TRY( checkOpLevel( op, level ) );
-   if ( op->kind != 0 )
-      return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operator only" );
+   if ( op->kind != 0 && op->kind != 2 )
+      return fail( HYG_ERR_ARG, "configurations exist for the constant-coefficient operators only" );
    if ( ( dtype != HYG_F64 && dtype != HYG_F32 ) || mode < 0 || mode > 5 )
       return fail( HYG_ERR_ARG, "bad dtype or mode" );
+   if ( op->kind == 2 )
+   {
+      const int c = p2Config( op->mesh->ctx.geom( level ), dtype, mode );
+      if ( config )
+         *config = c;
+      if ( rows )
+         *rows = kP2Chunks[c / 2];
+      if ( points )
+         *points = c % 2 + 1;
+      return HYG_OK;
+   }
    const int c = cellConfig( op->mesh->ctx.geom( level ), dtype, mode );
    if ( config )
       *config = c;

@@ -116,6 +116,11 @@ constexpr int kZSegChunk = 24; // planes per z-marching segment (short segments 
 // table stores (y0, k, z0, z1) in a ZSeg.
 constexpr int kP2ZSeg   = 124;
 constexpr int kP2ZChunk = 32; // planes per segment where N >= 256 (N = 128: 16, smaller lattices: 8)
+// configurations of the P2 row kernel (tuned by hyg_op_tune): planes per z-segment kP2Chunks[cfg / 2] and planes in
+// flight cfg % 2 + 1
+constexpr int kP2Chunks[4] = { 8, 16, 32, 64 };
+constexpr int kNumP2Cfgs   = 8;
+inline int    p2DefaultConfig( int N ) { return 2 * ( N >= 256 ? 2 : ( N >= 128 ? 1 : 0 ) ); }
 // cell-interior kernel configurations: rows per wavefront, points per lane, planes in flight
 constexpr int kCellRows[]   = { 1, 2, 4, 1, 2, 4, 8 };
 constexpr int kCellPoints[] = { 1, 1, 1, 1, 1, 1, 1 };
@@ -509,9 +514,12 @@ struct LevelGeom
    std::vector< int4 >                      flatItemsHost; // z = 0 / y = 0 face rows: (cell, slot, line, x0)
    int4*                                    flatItems  = nullptr;
    int                                      nFlatItems = 0;
-   std::vector< ZSeg >                      p2ZSegsHost; // P2 constant operator row-pair segments (this lattice level)
-   ZSeg*                                    p2ZSegs  = nullptr;
-   int                                      nP2ZSegs = 0;
+   // P2 constant operator row-pair segments of this lattice level, one table per segment length kP2Chunks[k], and the
+   // chosen row-kernel configuration per value type and mode (-1: p2DefaultConfig)
+   std::vector< ZSeg >                      p2ZSegsHost[4];
+   ZSeg*                                    p2ZSegs[4]  = {};
+   int                                      nP2ZSegs[4] = {};
+   int                                      p2Cfg[2][6] = { { -1, -1, -1, -1, -1, -1 }, { -1, -1, -1, -1, -1, -1 } };
    std::vector< LowerChunk >                chunksHost; // macro-vertex, then -edge, then -face work
    LowerChunk*                              chunks  = nullptr;
    int                                      nChunks = 0;

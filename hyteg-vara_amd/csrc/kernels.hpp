@@ -72,6 +72,10 @@ void launchP2Restrict( const MeshCtx& m, const LevelGeom& gc, const P2Tables* tb
 // cell-interior kernel configuration (index into kCellRows / kCellPoints) of a level, value type and mode
 int  cellConfig( LevelGeom& g, int dtype, int mode );
 void setCellConfig( LevelGeom& g, int dtype, int mode, int cfg );
+// P2 row-kernel configuration (0 .. kNumP2Cfgs-1: segment length kP2Chunks[cfg / 2], cfg % 2 + 1 planes in flight) of
+// a lattice level, value type and mode
+int  p2Config( const LevelGeom& g, int dtype, int mode );
+void setP2Config( LevelGeom& g, int dtype, int mode, int cfg );
 
 // Pointwise linear combination over all cell slots selected by flag:
 // dst = (accumulate ? dst : 0) + sum_k s_k src_k  (nsrc <= 3), or dst = value if nsrc == 0.

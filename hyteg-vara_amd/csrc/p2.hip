@@ -1146,6 +1146,13 @@ __global__ __launch_bounds__( kBlock ) void kP2RestrictLower( const LowerChunk* 
 }
 
 // ------------------------------------------------------------------ launch
+int p2Config( const LevelGeom& g, int dtype, int mode )
+{
+   const int c = g.p2Cfg[dtype == 8 ? 0 : 1][mode];
+   return c >= 0 ? c : p2DefaultConfig( 1 << g.L );
+}
+void setP2Config( LevelGeom& g, int dtype, int mode, int cfg ) { g.p2Cfg[dtype == 8 ? 0 : 1][mode] = cfg; }
+
 template < typename T, int MODE, bool DIAG, bool KQ >
 static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s,
                   bool skipInterior = false )
@@ -1164,11 +1171,19 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
    {
       // constant coefficient: cell interiors by row pairs, cell faces by one-sided stencils, the points on cell
       // edges and vertices element by element
-      const T* stw = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
-      if ( ( c.flag & 1 ) && g.nP2ZSegs > 0 )
-         hipLaunchKernelGGL( ( kP2Rows< T, MODE, 1 > ), dim3( xcdGrid( nC * ceilDiv( g.nP2ZSegs, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
-                             L, nC, g.stride, g.p2ZSegs, g.nP2ZSegs, stw, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
-                             (T*) c.out1, T( c.a ) );
+      const T*  stw  = dbl ? (const T*) ol.p2st : (const T*) ol.p2stF;
+      const int cfg  = p2Config( g, dbl ? 8 : 4, MODE );
+      const int ci   = cfg / 2, nSeg = g.nP2ZSegs[ci];
+      const dim3 grid( xcdGrid( nC * ceilDiv( nSeg, kBlock / 64 ) ) );
+      if ( ( c.flag & 1 ) && nSeg > 0 )
+      {
+         if ( cfg & 1 )
+            hipLaunchKernelGGL( ( kP2Rows< T, MODE, 2 > ), grid, dim3( kBlock ), 0, s, L, nC, g.stride, g.p2ZSegs[ci], nSeg, stw,
+                                (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
+         else
+            hipLaunchKernelGGL( ( kP2Rows< T, MODE, 1 > ), grid, dim3( kBlock ), 0, s, L, nC, g.stride, g.p2ZSegs[ci], nSeg, stw,
+                                (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
+      }
       P2FaceTiles ft;
       ft.start[0] = 0;
       for ( int k = 0; k < 4; ++k )

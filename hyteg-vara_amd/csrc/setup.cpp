@@ -354,18 +354,22 @@ LevelGeom& MeshCtx::geom( int L )
    // pair has interior outputs while z <= N - 1 - x_first - max(y0, 1). Planes are chunked from z = 0 so every
    // segment starts on an even plane (plane 0 itself has no interior output).
    // Coarse lattices take shorter segments: more waves, fewer serial plane steps each.
-   g.p2ZSegsHost.clear();
-   const int p2Chunk = N >= 256 ? kP2ZChunk : ( N >= 128 ? 16 : 8 );
-   for ( int z0 = 0; z0 <= N - 3; z0 += p2Chunk )
-      for ( int k = 0; kP2ZSeg * k <= N - 3; ++k )
-         for ( int y0 = 0; y0 <= N - 3; y0 += 2 )
-         {
-            const int xs = k == 0 ? 1 : kP2ZSeg * k, zEnd = N - xs - std::max( y0, 1 );
-            if ( zEnd > std::max( z0, 1 ) )
-               g.p2ZSegsHost.push_back( { y0, k, z0, std::min( z0 + p2Chunk, zEnd ) } );
-         }
-   g.nP2ZSegs = (int) g.p2ZSegsHost.size();
-   g.p2ZSegs  = upload( g.p2ZSegsHost );
+   for ( int ci = 0; ci < 4; ++ci )
+   {
+      const int p2Chunk = kP2Chunks[ci];
+      auto&     v       = g.p2ZSegsHost[ci];
+      v.clear();
+      for ( int z0 = 0; z0 <= N - 3; z0 += p2Chunk )
+         for ( int k = 0; kP2ZSeg * k <= N - 3; ++k )
+            for ( int y0 = 0; y0 <= N - 3; y0 += 2 )
+            {
+               const int xs = k == 0 ? 1 : kP2ZSeg * k, zEnd = N - xs - std::max( y0, 1 );
+               if ( zEnd > std::max( z0, 1 ) )
+                  v.push_back( { y0, k, z0, std::min( z0 + p2Chunk, zEnd ) } );
+            }
+      g.nP2ZSegs[ci] = (int) v.size();
+      g.p2ZSegs[ci]  = upload( v );
+   }
    // z = 0 and y = 0 faces of every cell: rows of 62 points
    g.flatItemsHost.clear();
    for ( int c = 0; c < mesh->nC(); ++c )
@@ -415,7 +419,8 @@ MeshCtx::~MeshCtx()
       for ( auto* zs : g.zSegs )
          (void) hipFree( zs );
       (void) hipFree( g.flatItems );
-      (void) hipFree( g.p2ZSegs );
+      for ( auto* zs : g.p2ZSegs )
+         (void) hipFree( zs );
       (void) hipFree( g.chunks );
    }
    (void) hipFree( prims );

@@ -260,6 +260,9 @@ int hyg_residual( hyg_op op, hyg_func r, hyg_func b, hyg_func x, int level, int 
  * loads in flight, one point per lane; configuration k has R = {1,2,4,1,2,4,8}[k], D = {2,2,2,4,4,4,2}[k]
  * (hyg_op_get_config reports R as `rows` and 1 as `points`). Modes: 0 apply,
  * 1 apply-add, 2 residual, 3 Jacobi sweep, 4 first Chebyshev step, 5 later Chebyshev steps.
+ * For the P2 constant operator the configurations 0..7 are those of its cell-interior row kernel: z-segments of
+ * {8, 16, 32, 64}[k / 2] planes with k % 2 + 1 planes of loads in flight (hyg_op_get_config reports the segment
+ * length as `rows` and the planes in flight as `points`); default 4 at P2 level >= 7, 2 at level 6, 0 below.
  * hyg_op_tune times every configuration for every mode at `level` on the operator's mesh (`reps`
  * sweeps each, on scratch functions it allocates and frees) and keeps the fastest; chosen[0..5]
  * (may be NULL) receives the choices. The choice is shared by all operators of the mesh. */
