@@ -137,6 +137,13 @@ struct Mesh
    std::vector< std::array< int, 6 > > cE;    // cell-local edge k -> edge id
    std::vector< std::vector< int > >   vC, eC, fC, vE, vF, eF;
    std::vector< int >                  vB, eB, fB; // 1 = on boundary (Dirichlet), 0 = inner
+   // IcosahedralShellMap of every cell (empty: affine cells), see shellMapOf
+   struct ShellMap
+   {
+      std::array< double, 3 > ref, ray, thr, fo;
+      double                  rRef, rRay;
+   };
+   std::vector< ShellMap > shell;
 };
 
 static const int CELL_FACE_VERTS[4][3] = { { 0, 1, 2 }, { 0, 1, 3 }, { 0, 2, 3 }, { 1, 2, 3 } };
@@ -499,6 +506,168 @@ inline P3 microCoord( const Mesh& m, int c, int L, idx x, idx y, idx z )
    for ( int k = 0; k < 3; ++k )
       r[k] = v0[k] + ( v1[k] - v0[k] ) * h * double( x ) + ( v2[k] - v0[k] ) * h * double( y ) + ( v3[k] - v0[k] ) * h * double( z );
    return r;
+}
+
+// ---------------------------------------------------------------- spherical-shell blending
+// IcosahedralShellMap (geometry/IcosahedralShellMap.hpp): classifyTet :281-317, classifyVertices :346-470,
+// findNonSkewTetInPrism :473-515, evalF :92-128. Two vertices of a shell cell lie on one ray from the origin; with
+// one vertex on one sphere and three on the other the lone one is `ref`, its ray partner `ray`, the others `thr` (the
+// later one in local order) and `fo`; a cell with two vertices on each sphere takes the map of the cell across its
+// first local face that does not contain both ray vertices.
+inline Mesh::ShellMap shellMapOf( const Mesh& m, int c, int depth = 0 )
+{
+   std::array< P3, 4 >     v;
+   std::array< double, 4 > rad;
+   double                  lo = 1e300, hi = 0.0;
+   for ( int k = 0; k < 4; ++k )
+   {
+      v[k]   = m.X[m.Cl[c][k]];
+      rad[k] = std::sqrt( v[k][0] * v[k][0] + v[k][1] * v[k][1] + v[k][2] * v[k][2] );
+      lo     = std::min( lo, rad[k] );
+      hi     = std::max( hi, rad[k] );
+   }
+   int nOuter = 0;
+   for ( int k = 0; k < 4; ++k )
+      nOuter += ( hi - rad[k] ) < ( hi - lo ) * 0.001 ? 1 : 0;
+   int a = -1, b = -1;
+   for ( int k = 0; k < 4 && a < 0; ++k )
+      for ( int j = k + 1; j < 4 && a < 0; ++j )
+      {
+         const double cx = v[k][1] * v[j][2] - v[k][2] * v[j][1], cy = v[k][2] * v[j][0] - v[k][0] * v[j][2],
+                      cz = v[k][0] * v[j][1] - v[k][1] * v[j][0], s = rad[k] * rad[j];
+         if ( std::fabs( cx ) / s <= 1e-14 && std::fabs( cy ) / s <= 1e-14 && std::fabs( cz ) / s <= 1e-14 )
+         {
+            a = k;
+            b = j;
+         }
+      }
+   if ( a < 0 || nOuter < 1 || nOuter > 3 || depth > 3 )
+      throw std::runtime_error( "cell is not a spherical-shell cell" );
+   if ( nOuter == 2 )
+   {
+      for ( int f = 0; f < 4; ++f )
+      {
+         bool hasA = false, hasB = false;
+         for ( int t = 0; t < 3; ++t )
+         {
+            hasA = hasA || CELL_FACE_VERTS[f][t] == a;
+            hasB = hasB || CELL_FACE_VERTS[f][t] == b;
+         }
+         if ( hasA && hasB )
+            continue;
+         for ( int o : m.fC[m.cF[c][f]] )
+            if ( o != c )
+               return shellMapOf( m, o, depth + 1 );
+      }
+      throw std::runtime_error( "no non-skew cell next to a skew shell cell" );
+   }
+   int ref = a, ray = b;
+   if ( ( nOuter == 1 && rad[ref] < rad[ray] ) || ( nOuter == 3 && rad[ref] > rad[ray] ) )
+      std::swap( ref, ray );
+   int thr = -1, fo = -1;
+   for ( int k = 0; k < 4; ++k )
+      if ( k != ref && k != ray )
+         thr = k;
+   for ( int k = 0; k < 4; ++k )
+      if ( k != ref && k != ray && k != thr )
+         fo = k;
+   return { v[ref], v[ray], v[thr], v[fo], rad[ref], rad[ray] };
+}
+
+// lambda(x): fraction of the way from the plane (ray, thr, fo) to ref; and its gradient (with the sign of |.|)
+inline double shellLambda( const Mesh::ShellMap& s, const P3& x, double g[3] )
+{
+   double a[3], b[3], d[3], n[3];
+   for ( int k = 0; k < 3; ++k )
+   {
+      a[k] = s.thr[k] - s.ray[k];
+      b[k] = s.fo[k] - s.ray[k];
+      d[k] = s.ref[k] - s.ray[k];
+   }
+   n[0]            = a[1] * b[2] - a[2] * b[1];
+   n[1]            = a[2] * b[0] - a[0] * b[2];
+   n[2]            = a[0] * b[1] - a[1] * b[0];
+   const double vt = d[0] * n[0] + d[1] * n[1] + d[2] * n[2];
+   const double vx = ( x[0] - s.ray[0] ) * n[0] + ( x[1] - s.ray[1] ) * n[1] + ( x[2] - s.ray[2] ) * n[2];
+   const double sg = vx / vt < 0.0 ? -1.0 : 1.0;
+   for ( int k = 0; k < 3; ++k )
+      g[k] = sg * n[k] / vt;
+   return std::fabs( vx / vt );
+}
+
+// evalF: x scaled to the radius r_ray + lambda (r_ref - r_ray)
+inline P3 shellF( const Mesh::ShellMap& s, const P3& x )
+{
+   double       g[3];
+   const double lam = shellLambda( s, x, g );
+   const double r   = s.rRay + lam * ( s.rRef - s.rRay );
+   const double n   = std::sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] );
+   return { x[0] * r / n, x[1] * r / n, x[2] * r / n };
+}
+
+// Jacobian of shellF, written out from the formula: DF = (r/|x|) I + x (x) ((r_ref - r_ray) grad lambda / |x| - r x / |x|^3)
+inline void shellDF( const Mesh::ShellMap& s, const P3& x, double D[3][3] )
+{
+   double       g[3];
+   const double lam = shellLambda( s, x, g );
+   const double r   = s.rRay + lam * ( s.rRef - s.rRay );
+   const double n   = std::sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] );
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j )
+         D[i][j] = ( i == j ? r / n : 0.0 ) + x[i] * ( ( s.rRef - s.rRay ) * g[j] / n - r * x[j] / ( n * n * n ) );
+}
+
+// p1_diffusion_blending_q3::integrateAll (forms/form_hyteg_generated/p1/p1_diffusion_blending_q3.cpp:287-...):
+// a_ij = |det J| sum_q w_q |det DF_q| (DF_q^-T grad lambda_i) . (DF_q^-T grad lambda_j), DF at the affine quadrature
+// points of the 5-point degree-3 rule (centroid -2/15, points (1/2, 1/6, 1/6, 1/6) 3/40)
+inline double baryGrads( const P3* x, double G[4][3] );
+inline void blendingLaplaceElement( const P3* x, const Mesh::ShellMap& s, double a[4][4] )
+{
+   double       G[4][3];
+   const double det = baryGrads( x, G );
+   const double lam[5][4] = { { 0.25, 0.25, 0.25, 0.25 },
+                              { 1.0 / 6, 1.0 / 6, 1.0 / 6, 0.5 },
+                              { 1.0 / 6, 1.0 / 6, 0.5, 1.0 / 6 },
+                              { 1.0 / 6, 0.5, 1.0 / 6, 1.0 / 6 },
+                              { 0.5, 1.0 / 6, 1.0 / 6, 1.0 / 6 } };
+   const double wq[5]     = { -2.0 / 15.0, 3.0 / 40.0, 3.0 / 40.0, 3.0 / 40.0, 3.0 / 40.0 };
+   for ( int i = 0; i < 4; ++i )
+      for ( int j = 0; j < 4; ++j )
+         a[i][j] = 0.0;
+   for ( int q = 0; q < 5; ++q )
+   {
+      P3 xq{ 0, 0, 0 };
+      for ( int v = 0; v < 4; ++v )
+         for ( int c = 0; c < 3; ++c )
+            xq[c] += lam[q][v] * x[v][c];
+      double D[3][3];
+      shellDF( s, xq, D );
+      // inverse of D by cofactors
+      const double dd = D[0][0] * ( D[1][1] * D[2][2] - D[1][2] * D[2][1] ) - D[0][1] * ( D[1][0] * D[2][2] - D[1][2] * D[2][0] ) +
+                        D[0][2] * ( D[1][0] * D[2][1] - D[1][1] * D[2][0] );
+      double Di[3][3];
+      Di[0][0] = ( D[1][1] * D[2][2] - D[1][2] * D[2][1] ) / dd;
+      Di[0][1] = ( D[0][2] * D[2][1] - D[0][1] * D[2][2] ) / dd;
+      Di[0][2] = ( D[0][1] * D[1][2] - D[0][2] * D[1][1] ) / dd;
+      Di[1][0] = ( D[1][2] * D[2][0] - D[1][0] * D[2][2] ) / dd;
+      Di[1][1] = ( D[0][0] * D[2][2] - D[0][2] * D[2][0] ) / dd;
+      Di[1][2] = ( D[0][2] * D[1][0] - D[0][0] * D[1][2] ) / dd;
+      Di[2][0] = ( D[1][0] * D[2][1] - D[1][1] * D[2][0] ) / dd;
+      Di[2][1] = ( D[0][1] * D[2][0] - D[0][0] * D[2][1] ) / dd;
+      Di[2][2] = ( D[0][0] * D[1][1] - D[0][1] * D[1][0] ) / dd;
+      // physical gradients: row vector g_i^T Di
+      double P[4][3];
+      for ( int i = 0; i < 4; ++i )
+         for ( int c = 0; c < 3; ++c )
+            P[i][c] = G[i][0] * Di[0][c] + G[i][1] * Di[1][c] + G[i][2] * Di[2][c];
+      const double s_ = wq[q] * std::fabs( dd );
+      for ( int i = 0; i < 4; ++i )
+         for ( int j = 0; j < 4; ++j )
+            a[i][j] += s_ * ( P[i][0] * P[j][0] + P[i][1] * P[j][1] + P[i][2] * P[j][2] );
+   }
+   for ( int i = 0; i < 4; ++i )
+      for ( int j = 0; j < 4; ++j )
+         a[i][j] *= std::fabs( det );
 }
 
 // calculateStencilInMacroCellForm_new (P1Elements.hpp:463-527): sum row 0 over the micro-tets at q inside cell c.
@@ -1254,6 +1423,13 @@ struct P1ElementwiseDivKGrad : OperatorBase< T >
    using OperatorBase< T >::invDiag;
    KField k;
 
+   // element matrix of a micro-cell of macro-cell c with (affine) vertices xc
+   virtual void element( int c, const P3* xc, double el[4][4] ) const
+   {
+      (void) c;
+      divKGradElement( xc, k, el );
+   }
+
    P1ElementwiseDivKGrad( const Mesh* mesh, int lo, int hi, const KField& kf )
    : k( kf )
    {
@@ -1293,7 +1469,7 @@ struct P1ElementwiseDivKGrad : OperatorBase< T >
                   if ( !inside )
                      continue;
                   double el[4][4];
-                  divKGradElement( xc, k, el );
+                  element( c, xc, el );
                   fn( vi, el );
                }
    }
@@ -1350,6 +1526,23 @@ struct P1ElementwiseDivKGrad : OperatorBase< T >
          v    = T( 1.0 ) / v;
       } );
    }
+};
+
+// P1ElementwiseBlendingLaplaceOperator = P1ElementwiseOperator< p1_diffusion_blending_q3 >
+// (elementwiseoperators/P1ElementwiseOperator.hpp:332) on a mesh with the spherical-shell map: the element loops of
+// the elementwise operator with blendingLaplaceElement and each macro-cell's map.
+template < typename T >
+struct P1ElementwiseBlendingLaplace : P1ElementwiseDivKGrad< T >
+{
+   P1ElementwiseBlendingLaplace( const Mesh* mesh, int lo, int hi )
+   : P1ElementwiseDivKGrad< T >( mesh, lo, hi, KField{} )
+   {
+      if ( mesh->shell.size() != mesh->Cl.size() )
+         throw std::invalid_argument( "the blended operator needs the shell map of every cell" );
+      for ( int L = lo; L <= hi; ++L )
+         this->computeInverseDiagonal( L ); // the base constructor used its own element matrices
+   }
+   void element( int c, const P3* xc, double el[4][4] ) const override { blendingLaplaceElement( xc, this->m->shell[c], el ); }
 };
 
 // Least squares by Householder QR with column pivoting (the reference solves its fits with Eigen's
@@ -3045,12 +3238,65 @@ void or_dof_coords( void* mh, int L, double* xyz, uint8_t* bnd )
          x = microCoord( m, p, L, i, j, k );
          b = 0;
       }
+      // blended coordinates: the map of the edge's / face's first neighbour cell (IcosahedralShellMap( Edge / Face )
+      // takes neighborCells[0]), the cell's own map; macro-vertices keep their positions
+      if ( !m.shell.empty() && kind > 0 )
+         x = shellF( m.shell[kind == 1 ? m.eC[p][0] : ( kind == 2 ? m.fC[p][0] : p )], x );
       xyz[3 * n + 0] = x[0];
       xyz[3 * n + 1] = x[1];
       xyz[3 * n + 2] = x[2];
       if ( bnd )
          bnd[n] = (uint8_t) b;
    } );
+}
+
+// IcosahedralShellMap::setMap on a spherical-shell mesh: every cell gets its map (DoF coordinates become blended)
+int or_mesh_set_shell_map( void* mh )
+{
+   auto& m = *static_cast< Mesh* >( mh );
+   try
+   {
+      std::vector< Mesh::ShellMap > v;
+      for ( int c = 0; c < (int) m.Cl.size(); ++c )
+         v.push_back( shellMapOf( m, c ) );
+      m.shell = v;
+   } catch ( const std::exception& )
+   {
+      return 1;
+   }
+   return 0;
+}
+
+// P1ElementwiseBlendingLaplaceOperator on a mesh with the shell map
+void* or_op_new_blending_laplace( void* mh, int minL, int maxL, int prec )
+{
+   auto* o = new OrOp;
+   o->prec = prec;
+   o->m    = static_cast< Mesh* >( mh );
+   if ( prec == 4 )
+      o->f = std::make_unique< P1ElementwiseBlendingLaplace< float > >( o->m, minL, maxL );
+   else
+      o->d = std::make_unique< P1ElementwiseBlendingLaplace< double > >( o->m, minL, maxL );
+   return o;
+}
+
+// the shell map of cell c: ref, ray, thr, fo (3 each), r_ref, r_ray
+int or_mesh_shell_map( void* mh, int c, double* out14 )
+{
+   const auto& m = *static_cast< Mesh* >( mh );
+   if ( m.shell.empty() )
+      return 1;
+   const auto& s = m.shell[c];
+   for ( int k = 0; k < 3; ++k )
+   {
+      out14[k]     = s.ref[k];
+      out14[3 + k] = s.ray[k];
+      out14[6 + k] = s.thr[k];
+      out14[9 + k] = s.fo[k];
+   }
+   out14[12] = s.rRef;
+   out14[13] = s.rRay;
+   return 0;
 }
 
 void* or_func_new( void* mh, int minL, int maxL, int prec )

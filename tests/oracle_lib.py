@@ -43,6 +43,9 @@ def lib():
             "or_op_new_divkgrad": (vp, [vp, i, i, i, i, d, d, P(np.float64), i]),
             "or_op_new_divkgrad_poly": (vp, [vp, i, i, i, P(np.float64), i]),
             "or_smooth_sor_ordered": (i, [vp, vp, vp, d, i, i, i]),
+            "or_mesh_set_shell_map": (i, [vp]),
+            "or_mesh_shell_map": (i, [vp, i, P(np.float64)]),
+            "or_op_new_blending_laplace": (vp, [vp, i, i, i]),
             "or_op_interpolate_stencils": (i, [vp, i, i]),
             "or_op_surrogate_poly": (i, [vp, i, i, P(np.float64)]),
             "or_op_variable_stencil": (i, [vp, i, i, C.c_long, C.c_long, C.c_long, P(np.float64)]),
@@ -135,6 +138,15 @@ class Mesh:
 
     def volume(self):
         return lib().or_mesh_volume(self.h)
+
+    def set_shell_map(self):
+        """IcosahedralShellMap::setMap: blended cells; coords() returns blended positions from then on."""
+        assert lib().or_mesh_set_shell_map(self.h) == 0
+
+    def shell_map(self, c):
+        out = np.zeros(14)
+        assert lib().or_mesh_shell_map(self.h, c, out) == 0
+        return out[0:3], out[3:6], out[6:9], out[9:12], out[12], out[13]
 
     def coords(self, L):
         n = self.ndofs(L)
@@ -238,6 +250,14 @@ def p2_restrict(f, src_lat, flag=INNER):
 
 def p2_prolongate(f, src_lat, flag=INNER, add=False):
     lib().or_p2_prolongate(f.h, src_lat, flag, int(add))
+
+
+class BlendingLaplaceOperator(Operator):
+    """P1ElementwiseBlendingLaplaceOperator (p1_diffusion_blending_q3) on a mesh with the shell map."""
+
+    def __init__(self, mesh, minL, maxL, prec=8):
+        self.mesh = mesh
+        self.h = lib().or_op_new_blending_laplace(mesh.h, minL, maxL, prec)
 
 
 class DivKGradOperator(Operator):
