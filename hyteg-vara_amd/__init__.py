@@ -72,6 +72,8 @@ def _sig(lib):
         "hyg_op_create_p1_divkgrad": (i, [vp, i, i, d, P(np.float64), i, pvp]),
         "hyg_op_create_p1_divkgrad_poly": (i, [vp, i, i, i, P(np.float64), pvp]),
         "hyg_op_interpolate_stencils": (i, [vp, i, i]),
+        "hyg_mesh_set_shell_map": (i, [vp]),
+        "hyg_op_create_p1_blending_laplace": (i, [vp, i, i, pvp]),
         "hyg_op_create_p2_const": (i, [vp, i, i, pvp]),
         "hyg_op_create_p2_divkgrad": (i, [vp, i, i, i, d, d, d, C.c_uint64, pvp]),
         "hyg_op_store_local_matrices": (i, [vp]),
@@ -190,6 +192,10 @@ class Mesh:
         bnd = np.zeros(n, dtype=np.uint8)
         check(load().hyg_mesh_dof_coords(self.h, L, xyz.ctypes.data, bnd.ctypes.data))
         return xyz.reshape(n, 3), bnd.astype(bool)
+
+    def set_shell_map(self):
+        """IcosahedralShellMap::setMap: blended cells (coords() returns blended positions from then on)."""
+        check(load().hyg_mesh_set_shell_map(self.h))
 
     def sor_colours(self):
         """(number of colours, colour of every macro-vertex) of the multi-colour SOR ordering."""
@@ -340,6 +346,16 @@ class P1DivKGradOperator(P1ConstantOperator):
     def interpolate_stencils(self, degree, interpolation_level):
         """P1SurrogateOperator::interpolateStencils: polynomial cell-interior stencils from then on."""
         check(load().hyg_op_interpolate_stencils(self.h, degree, interpolation_level))
+
+
+class P1BlendingLaplaceOperator(P1ConstantOperator):
+    """P1ElementwiseBlendingLaplaceOperator on a mesh with the shell map (Mesh.set_shell_map)."""
+
+    def __init__(self, mesh, minL, maxL):
+        self.mesh = mesh
+        h = _out()
+        check(load().hyg_op_create_p1_blending_laplace(mesh.h, minL, maxL, C.byref(h)))
+        self.h = h
 
 
 class P2ConstantOperator(P1ConstantOperator):

@@ -733,6 +733,16 @@ int hyg_mesh_dof_coords( hyg_mesh mesh, int level, double* xyz, uint8_t* boundar
             r[a] = m.X[cv[0]][a] + ( m.X[cv[1]][a] - m.X[cv[0]][a] ) * h * i + ( m.X[cv[2]][a] - m.X[cv[0]][a] ) * h * j +
                    ( m.X[cv[3]][a] - m.X[cv[0]][a] ) * h * k;
       }
+      // blended coordinates (hyg_mesh_set_shell_map): the map of the edge's / face's first cell (the reference's
+      // IcosahedralShellMap( Edge / Face ) takes neighborCells[0]) or of the cell itself; macro-vertices stay
+      if ( !mc.shellHost.empty() && !( p >= 0 && mc.primsHost[p].kind == 0 ) )
+      {
+         int cell = -1 - p;
+         if ( p >= 0 )
+            cell = mc.primsHost[p].kind == 1 ? m.eC[p - m.nV()][0] : m.fC[p - m.nV() - m.nE()][0];
+         const double xa[3] = { r[0], r[1], r[2] };
+         shellF( mc.shellHost[cell], xa, r );
+      }
       if ( xyz )
          xyz[3 * n] = r[0], xyz[3 * n + 1] = r[1], xyz[3 * n + 2] = r[2];
       if ( boundary )
@@ -1065,6 +1075,62 @@ int hyg_op_create_p1_divkgrad( hyg_mesh mesh, int min_level, int max_level, doub
    return HYG_OK;
 }
 
+int hyg_mesh_set_shell_map( hyg_mesh mesh )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh )
+      return fail( HYG_ERR_ARG, "null mesh" );
+   try
+   {
+      mesh->ctx.setShellMap();
+   } catch ( const std::runtime_error& e )
+   {
+      return fail( HYG_ERR_ARG, e.what() );
+   }
+   return HYG_OK;
+}
+
+int hyg_op_create_p1_blending_laplace( hyg_mesh mesh, int min_level, int max_level, hyg_op* out )
+{
+   HYG_TRY( checkInit() );
+   if ( !mesh || !out || min_level < 0 || max_level < min_level || max_level > 10 )
+      return fail( HYG_ERR_ARG, "bad argument" );
+   if ( mesh->ctx.shellHost.empty() )
+      return fail( HYG_ERR_ARG, "the blended operator needs the shell map (hyg_mesh_set_shell_map)" );
+   auto* op   = new hyg_op_s;
+   op->mesh   = mesh;
+   op->minL   = min_level;
+   op->maxL   = max_level;
+   op->kind   = 1;
+   op->blend  = true;
+   op->stored = true; // no coefficient integrals: the weights are stored
+   try
+   {
+      for ( int L = min_level; L <= max_level; ++L )
+         mesh->ctx.geom( L );
+      assembleP1DivKGrad( *op );
+      assembleBlendWeights( mesh->ctx, *op, nullptr );
+      for ( int L = min_level; L <= max_level; ++L )
+         launchElementwiseInverseDiagonal( mesh->ctx, mesh->ctx.geom( L ), *op, L, nullptr );
+   } catch ( const std::exception& e )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, e.what() );
+   }
+   if ( int rc = launched( "blended operator setup" ) )
+   {
+      delete op;
+      return rc;
+   }
+   if ( hipDeviceSynchronize() != hipSuccess )
+   {
+      delete op;
+      return fail( HYG_ERR_HIP, "blended operator setup failed" );
+   }
+   *out = op;
+   return HYG_OK;
+}
+
 int hyg_op_create_p1_divkgrad_poly( hyg_mesh mesh, int min_level, int max_level, int degree, const double* coeffs, hyg_op* out )
 {
    HYG_TRY( checkInit() );
@@ -1110,7 +1176,7 @@ int hyg_op_create_p1_divkgrad_poly( hyg_mesh mesh, int min_level, int max_level,
 int hyg_op_interpolate_stencils( hyg_op op, int degree, int interpolation_level )
 {
    HYG_TRY( checkInit() );
-   if ( !op || op->kind != 1 )
+   if ( !op || op->kind != 1 || op->blend )
       return fail( HYG_ERR_ARG, "surrogate stencils are for the variable-coefficient P1 operator" );
    if ( degree < 0 || degree > 12 )
       return fail( HYG_ERR_ARG, "surrogate polynomial degree must be in 0..12" );
@@ -2264,8 +2330,8 @@ int hyg_batch_create( int n, const hyg_gmg* solvers, const hyg_func* x, const hy
          return fail( HYG_ERR_ARG, who + "batched cycles take P1 constant and P1 elementwise operators" );
       if ( g->op->kind == 1 && !g->op->stored )
          return fail( HYG_ERR_ARG, who + "an elementwise operator in a batch must keep its local matrices (hyg_op_store_local_matrices)" );
-      if ( g->op->surDegree >= 0 )
-         return fail( HYG_ERR_ARG, who + "surrogate operators are not batched" );
+      if ( g->op->surDegree >= 0 || g->op->blend )
+         return fail( HYG_ERR_ARG, who + "surrogate and blended operators are not batched" );
       if ( innerDofs( g->op->mesh->ctx, g->minL ) != 0 )
          return fail( HYG_ERR_ARG, who + "the coarse level has inner DoFs (batched cycles have no coarse solve)" );
    }

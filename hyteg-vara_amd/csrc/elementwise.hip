@@ -78,6 +78,10 @@ static void elemT( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, co
                           ol.surNBnd );
       launchSurrogateInterior( m, g, op, L, std::is_same< T, double >::value ? 8 : 4, c, s );
    }
+   else if ( op.blend )
+      hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG, false, true > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem,
+                          kb, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ),
+                          nullptr, 0, dbl ? (const T*) ol.bw : (const T*) ol.bwF );
    else
       hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem, kb,
                           m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ) );

@@ -195,8 +195,9 @@ __device__ __forceinline__ void elemWeights( const ElemCell& ec, const T* __rest
 
 // One thread per point of the closed macro-cell. DIAG: the diagonal entry instead of the product. BND: the points
 // on the cell's faces, edges and vertices only, listed in bnd[0 .. nBnd) by their index in the cell array (the
-// surrogate operator does the interior points itself).
-template < typename T, int MODE, bool DIAG, bool BND = false >
+// surrogate operator does the interior points itself). STW: the point's stencil weights are read from wst
+// ([cell][direction][point], the blended operator) instead of assembled from the coefficient integrals.
+template < typename T, int MODE, bool DIAG, bool BND = false, bool STW = false >
 __device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i64 stride, const ElemCell* __restrict__ cells,
                                                         const T* __restrict__ kb, const CellInfo* __restrict__ cinfo,
                                                         const CellFaceSlot* __restrict__ cfs, const LowerPrim* __restrict__ prims,
@@ -204,7 +205,8 @@ __device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i6
                                                         T* __restrict__ lowPart, const T* __restrict__ dinv, int flag,
                                                         const T* __restrict__ x, const T* __restrict__ in1,
                                                         T* __restrict__ out0, T* __restrict__ out1, T a,
-                                                        const int* __restrict__ bnd = nullptr, int nBnd = 0 )
+                                                        const int* __restrict__ bnd = nullptr, int nBnd = 0,
+                                                        const T* __restrict__ wst = nullptr )
 {
    const int c    = bid / tilesPerCell;
    const int tile = bid - c * tilesPerCell;
@@ -223,10 +225,16 @@ __device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i6
    const int slot = classifySlot( N, px, py, pz );
    if ( slot == 0 && !( flag & 1 ) )
       return;
-   const ElemCell& ec = cells[c];
-   const T*        kc = kb + i64( c ) * 6 * S3i( N );
-   T               W[NDIR];
-   elemWeights< T, DIAG >( ec, kc, N, px, py, pz, W );
+   T W[NDIR];
+   if constexpr ( STW )
+   {
+      const T* wc = wst + i64( c ) * NDIR * S3i( w ) + s;
+#pragma unroll
+      for ( int d = 0; d < NDIR; ++d )
+         W[d] = ( DIAG && d > 0 ) ? T( 0 ) : wc[i64( d ) * S3i( w )];
+   }
+   else
+      elemWeights< T, DIAG >( cells[c], kb + i64( c ) * 6 * S3i( N ), N, px, py, pz, W );
    const i64 q = i64( c ) * stride + cidx( w, px, py, pz );
    T         val;
    if constexpr ( DIAG )
@@ -281,7 +289,7 @@ __device__ __forceinline__ void bElemApply( int bid, int L, int tilesPerCell, i6
    }
 }
 
-template < typename T, int MODE, bool DIAG, bool BND = false >
+template < typename T, int MODE, bool DIAG, bool BND = false, bool STW = false >
 __global__ __launch_bounds__( kBlock ) void kElemApply( int L, int tilesPerCell, i64 stride, const ElemCell* __restrict__ cells,
                                                         const T* __restrict__ kb, const CellInfo* __restrict__ cinfo,
                                                         const CellFaceSlot* __restrict__ cfs, const LowerPrim* __restrict__ prims,
@@ -289,10 +297,11 @@ __global__ __launch_bounds__( kBlock ) void kElemApply( int L, int tilesPerCell,
                                                         T* __restrict__ lowPart, const T* __restrict__ dinv, int flag,
                                                         const T* __restrict__ x, const T* __restrict__ in1,
                                                         T* __restrict__ out0, T* __restrict__ out1, T a,
-                                                        const int* __restrict__ bnd = nullptr, int nBnd = 0 )
+                                                        const int* __restrict__ bnd = nullptr, int nBnd = 0,
+                                                        const T* __restrict__ wst = nullptr )
 {
-   bElemApply< T, MODE, DIAG, BND >( int( blockIdx.x ), L, tilesPerCell, stride, cells, kb, cinfo, cfs, prims, entries, part, lowPart, dinv,
-                                     flag, x, in1, out0, out1, a, bnd, nBnd );
+   bElemApply< T, MODE, DIAG, BND, STW >( int( blockIdx.x ), L, tilesPerCell, stride, cells, kb, cinfo, cfs, prims, entries, part, lowPart,
+                                          dinv, flag, x, in1, out0, out1, a, bnd, nBnd, wst );
 }
 
 } // namespace hyg

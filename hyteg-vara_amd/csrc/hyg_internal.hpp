@@ -473,6 +473,42 @@ struct Mesh
    int nC() const { return (int) Cv.size(); }
 };
 
+// Spherical-shell blending of one macro-cell (the reference's IcosahedralShellMap, geometry/IcosahedralShellMap.hpp):
+// lambda(x) = |(x - ray) . n / vt| with n = (thr - ray) x (fo - ray), vt = (ref - ray) . n is the fraction of the way
+// from the plane (ray, thr, fo) to ref, and F(x) = x (rRay + lambda (rRef - rRay)) / |x|.
+struct ShellMap
+{
+   double ref[3], ray[3], thr[3], fo[3];
+   double rRef, rRay;
+   double n[3], vt;
+};
+__host__ __device__ inline double shellLambda( const ShellMap& s, const double x[3], double g[3] )
+{
+   const double v  = ( ( x[0] - s.ray[0] ) * s.n[0] + ( x[1] - s.ray[1] ) * s.n[1] + ( x[2] - s.ray[2] ) * s.n[2] ) / s.vt;
+   const double sg = v < 0.0 ? -1.0 : 1.0;
+   for ( int k = 0; k < 3; ++k )
+      g[k] = sg * s.n[k] / s.vt;
+   return v < 0.0 ? -v : v;
+}
+__host__ __device__ inline void shellF( const ShellMap& s, const double x[3], double y[3] )
+{
+   double       g[3];
+   const double r = s.rRay + shellLambda( s, x, g ) * ( s.rRef - s.rRay );
+   const double q = r / sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] );
+   for ( int k = 0; k < 3; ++k )
+      y[k] = x[k] * q;
+}
+// Jacobian of shellF: DF = (r / |x|) I + x (r' grad lambda / |x| - r x / |x|^3)^T, r' = rRef - rRay
+__host__ __device__ inline void shellDF( const ShellMap& s, const double x[3], double D[3][3] )
+{
+   double       g[3];
+   const double r  = s.rRay + shellLambda( s, x, g ) * ( s.rRef - s.rRay );
+   const double nx = sqrt( x[0] * x[0] + x[1] * x[1] + x[2] * x[2] );
+   for ( int i = 0; i < 3; ++i )
+      for ( int j = 0; j < 3; ++j )
+         D[i][j] = ( i == j ? r / nx : 0.0 ) + x[i] * ( ( s.rRef - s.rRay ) * g[j] / nx - r * x[j] / ( nx * nx * nx ) );
+}
+
 std::unique_ptr< Mesh > readGmsh( const std::string& path, std::string& err, bool keepTags = false );
 // DoF type of a mesh boundary flag under BoundaryCondition::create0123BC (boundary/BoundaryConditions.cpp:26-33):
 // 1 Dirichlet, 2 Neumann, 3 free-slip, every other flag Inner
@@ -564,6 +600,10 @@ struct MeshCtx
    std::vector< int >          vertexColours;      // colour of every macro-vertex
    std::vector< std::unique_ptr< HaloMaps > > haloMaps; // per level, built on first use
    const HaloMaps&             halo( int L );
+   // spherical-shell blending (hyg_mesh_set_shell_map): the map of every cell, host and device; empty: affine cells
+   std::vector< ShellMap >     shellHost;
+   ShellMap*                   shell = nullptr;
+   void                        setShellMap(); // throws when a cell is not a shell cell
    void                        buildTopology(); // prims, entries, cell info (host + device)
    LevelGeom&                  geom( int L );
    const P2Tables*             p2Tables();
@@ -640,6 +680,8 @@ struct hyg_op_s
    // kind 1 after hyg_op_interpolate_stencils (P1SurrogateOperator): the polynomial degree of the cell-interior
    // stencils (Level::surPoly), -1 without
    int                   surDegree = -1;
+   // kind 1 as the blended Laplacian of a shell mesh (P1ElementwiseBlendingLaplaceOperator): weights in Level::bw
+   bool                  blend = false;
    struct Level
    {
       std::vector< double > cellStHost;  // nC x 16: 15 weights, inverse centre weight
@@ -691,6 +733,10 @@ struct hyg_op_s
       int                   surNBnd = 0;
       // constant P1 operator: tables of the reference-order SOR (hyg_smooth_sor_ordered), built on first use
       std::unique_ptr< SorOrderTables > sorOrder;
+      // blended Laplacian (kind 1 with blend): the 15 one-sided stencil weights of every point of every cell, as
+      // summed from the cell's micro-elements (per cell, direction-major: [cell][direction][point]), FP64 and FP32
+      double*               bw  = nullptr;
+      float*                bwF = nullptr;
    };
    std::vector< Level > levels;
    ~hyg_op_s();

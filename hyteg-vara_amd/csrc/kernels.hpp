@@ -54,6 +54,8 @@ void launchElementwiseStore( const MeshCtx& m, hyg_op_s& op, hipStream_t s );
 // integrals; synchronous), whether level L uses them, the interior part of one stencil operation, and the interior
 // inverse diagonal (written into op.levels[L].dinv)
 void surrogateInterpolate( const MeshCtx& m, hyg_op_s& op, int deg, int interpolationLevel );
+// blended Laplacian of a shell mesh (op.blend): the stencil weights of every level (Level::bw, bwF)
+void assembleBlendWeights( const MeshCtx& m, hyg_op_s& op, hipStream_t s );
 bool surrogateActive( const hyg_op_s& op, int L );
 void launchSurrogateInterior( const MeshCtx& m, const LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall& c, hipStream_t s );
 void launchSurrogateInverseDiagonal( const MeshCtx& m, const LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );

@@ -424,6 +424,7 @@ MeshCtx::~MeshCtx()
       (void) hipFree( g.chunks );
    }
    (void) hipFree( prims );
+   (void) hipFree( shell );
    (void) hipFree( p2tab );
    (void) hipFree( cellFaces );
    (void) hipFree( entries );
@@ -940,6 +941,8 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.p2s );
       (void) hipFree( l.p2sF );
       (void) hipFree( l.surPoly );
+      (void) hipFree( l.bw );
+      (void) hipFree( l.bwF );
       (void) hipFree( l.surBnd );
    }
    (void) hipFree( modes );

@@ -173,6 +173,21 @@ int hyg_op_destroy( hyg_op op );
  * in FP64 and FP32. */
 int hyg_op_create_p1_divkgrad( hyg_mesh mesh, int min_level, int max_level, double k0, const double* modes, int n_modes,
                                hyg_op* out );
+/* Replaces IcosahedralShellMap::setMap(setupStorage) (geometry/IcosahedralShellMap.hpp:40-128, 281-515) on a
+ * spherical-shell mesh: every cell gets the shell map (two vertices on one ray: the lone vertex of the sphere with one
+ * vertex is `ref`, its ray partner `ray`; cells with two vertices on each sphere take the map of the cell across their
+ * first local face without both ray vertices), F(x) = x (r_ray + lambda(x) (r_ref - r_ray)) / |x| with lambda the
+ * fraction of the way from the plane (ray, thr, for) to ref. hyg_mesh_dof_coords returns blended positions from
+ * then on (edge and face points: the map of their first cell); operators created afterwards are unaffected except
+ * hyg_op_create_p1_blending_laplace. HYG_ERR_ARG if a cell is not a shell cell. */
+int hyg_mesh_set_shell_map( hyg_mesh mesh );
+/* Replaces P1ElementwiseBlendingLaplaceOperator(storage, minLevel, maxLevel) (P1ElementwiseOperator<
+ * forms::p1_diffusion_blending_q3 >, elementwiseoperators/P1ElementwiseOperator.hpp:332; form
+ * p1_diffusion_blending_q3.cpp:287-...) on a mesh with the shell map: element matrices
+ * a_ij = |det J| sum_q w_q |det DF_q| (DF_q^-T g_i).(DF_q^-T g_j) over the 5-point degree-3 rule. The 15 stencil weights
+ * of every cell point are summed once at creation (15 FP64 + 15 FP32 values per point and cell); supports what
+ * hyg_op_create_p1_divkgrad supports except surrogates and batches. */
+int hyg_op_create_p1_blending_laplace( hyg_mesh mesh, int min_level, int max_level, hyg_op* out );
 /* The same operator with a polynomial coefficient k(x) = sum_i coeffs[i] m_i(x) over the (degree+1)(degree+2)(degree+3)/6
  * monomials m_i = x^a y^b z^c of degree <= degree (0..12) in the order of the reference's MonomialBasis3D
  * (polynomial/MonomialBasis3D.hpp: by total degree, within a degree by decreasing a, then decreasing b), as the
