@@ -585,6 +585,38 @@ def bench_p2(hyg, hip, st, steps, level, seed):
     return out
 
 
+def bench_blending(hyg, hip, st, steps, level):
+    """Blended P1 Laplacian (P1ElementwiseBlendingLaplaceOperator, p1_diffusion_blending_q3) on the configuration-5
+    spherical shell (5, 2, [0.55, 1]) with the IcosahedralShellMap: setup (the per-point stencil weights, summed on
+    the device from the blended element matrices) and apply throughput; the apply streams x, y and 15 weights per
+    point and cell (136 B per inner DoF in FP64)."""
+    mesh = hyg.Mesh("shell", 5, 2, 0.55, 1.0)
+    mesh.set_shell_map()
+    L = level
+    n = mesh.ndofs(L)
+    inner = int((~mesh.boundary(L)).sum())
+    lib = hyg.load()
+    t0 = time.perf_counter()
+    op = hyg.P1BlendingLaplaceOperator(mesh, L, L)
+    out = {"workload": f"P1 blended Laplacian, spherical shell (5, 2, [0.55, 1]) level {L}", "dofs_inner": inner,
+           "setup_seconds": round(time.perf_counter() - t0, 3)}
+    rng = np.random.default_rng(11)
+    for dt, name, b in ((hyg.F64, "f64", 136.0), (hyg.F32, "f32", 68.0)):
+        x, y = hyg.Function(mesh, L, L, dt), hyg.Function(mesh, L, L, dt)
+        x.upload(L, rng.uniform(-1, 1, n))
+
+        def ap():
+            hyg.check(lib.hyg_apply(op.h, x.h, y.h, L, hyg.INNER, hyg.REPLACE, st))
+
+        for _ in range(2):
+            ap()
+        ms = timed(hip, st, ap, steps) / steps
+        gbs = b * inner / (ms / 1e3) / 1e9
+        out[f"apply_{name}"] = {"ms": round(ms, 4), "GDoF/s": round(inner / (ms / 1e3) / 1e9, 3), "GB/s_alg": round(gbs, 1),
+                                "roofline_frac": round(gbs / PEAK_HBM_GBS, 4)}
+    return out
+
+
 def bench_shell(hyg, hip, st, steps, level, seed, solve_cycles):
     """Spherical shell ntan = 5, nrad = 2, r in [0.55, 1] (960 macro-cells), P2 level 5: the vector Laplacian
     -div(eta grad u_i), i = 1..3, with eta(x) = exp(2.99 (1 - |x|)) (1 + 0.1 xi_e), xi_e uniform in [-1, 1) per
@@ -857,6 +889,7 @@ def main():
 
     # ---------------- spherical shell, P2 vector Laplacian with variable viscosity
     if not args.no_shell:
+        out["shell_blending"] = bench_blending(hyg, hip, st, max(5, args.steps // 10), 6)
         out["shell_p2"] = bench_shell(hyg, hip, st, max(2, args.steps // 25), args.shell_level, 2026, args.shell_cycles)
 
     # ---------------- CPU baseline (oracle), rank 0, N = 1
