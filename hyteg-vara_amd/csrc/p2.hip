@@ -1421,6 +1421,7 @@ static void p2RestrictT( const MeshCtx& m, const LevelGeom& gc, const P2Tables* 
       const int M     = p2InteriorExtent( 1 << Lc, k );
       ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
    }
+   // the cell interiors take no flag, as in P2toP2QuadraticRestriction.cpp:146 (every cell, whatever the flag)
    if ( ct.start[8] > 0 )
       hipLaunchKernelGGL( ( kP2RestrictInt< T > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, st, Lc, gc.stride, strideF, ct, (T*) coarse,
                           (const T*) fine );
