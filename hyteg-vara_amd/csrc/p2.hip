@@ -19,7 +19,7 @@
 // M_t,q = w_q |det J| grad phi(l_q) grad phi(l_q)^T of the cell's type t (affine map: the same for every
 // micro-cell of that type) and the coefficient k_e,q at the quadrature points, evaluated once at operator
 // setup (kP2KQ). A point assembles the rows it needs on the fly.
-#include "face_combine.hpp"
+#include "p2_common.hpp"
 
 #include <utility>
 
@@ -48,10 +48,11 @@ __device__ __forceinline__ double p2CoefAt( const P2Coef& k, double x, double y,
 }
 
 // One thread per anchor of the micro-cell lattice (level L-1) of a macro-cell; the valid types of that anchor.
-// Element id (6 c + t) S3(Nc + 1) + index(Nc + 1, anchor); kq[11 id + q].
+// Element id (6 c + t) S3(Nc + 1) + index(Nc + 1, anchor); kq[q kqStride + id], kqStride = 6 nC S3(Nc + 1), so that
+// threads of consecutive anchors read consecutive values of one quadrature point.
 template < typename T >
 __global__ __launch_bounds__( kBlock ) void kP2KQ( int L, int tilesPerCell, const ElemCell* __restrict__ cells, P2Coef coef,
-                                                   T* __restrict__ kq )
+                                                   T* __restrict__ kq, i64 kqStride )
 {
    const int c    = int( blockIdx.x ) / tilesPerCell;
    const int tile = int( blockIdx.x ) - c * tilesPerCell;
@@ -82,17 +83,13 @@ __global__ __launch_bounds__( kBlock ) void kP2KQ( int L, int tilesPerCell, cons
 #pragma unroll
          for ( int r = 0; r < 3; ++r )
             x[r] = kXG4Lambda[q][0] * X[0][r] + kXG4Lambda[q][1] * X[1][r] + kXG4Lambda[q][2] * X[2][r] + kXG4Lambda[q][3] * X[3][r];
-         kq[11 * eid + q] = T( p2CoefAt( coef, x[0], x[1], x[2], eid ) );
+         kq[i64( q ) * kqStride + i64( eid )] = T( p2CoefAt( coef, x[0], x[1], x[2], eid ) );
       }
    }
 }
 
 // ------------------------------------------------------------------ operator application
 // KQ: variable coefficient (el = 11 quadrature-point matrices per cell and type, kq per micro-cell)
-__host__ __device__ constexpr int triCount( int n )
-{
-   return n > 0 ? n * ( n + 1 ) / 2 : 0;
-}
 // points on the cell edges and vertices of a cell lattice with N intervals per edge: the four vertices, then
 // N - 1 points of each edge e0 = (v0, v1) ... e5 = (v2, v3)
 constexpr int kP2EvLanes = 4; // lanes per point of the edge/vertex kernel
@@ -129,7 +126,7 @@ __device__ __forceinline__ bool edgeVertexXYZ( int N, int s, int& x, int& y, int
 // points on cell edges and vertices, kP2EvLanes lanes per point (each sums every kP2EvLanes-th incident micro-cell)
 template < typename T, int MODE, bool DIAG, bool KQ, bool SKIPIN = false >
 __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i64 stride, const T* __restrict__ el,
-                                                      const T* __restrict__ kq,
+                                                      const T* __restrict__ kq, i64 kqStride,
                                                       const P2Tables* __restrict__ tb, const CellInfo* __restrict__ cinfo,
                                                       const CellFaceSlot* __restrict__ cfs, const LowerPrim* __restrict__ prims,
                                                       const LowerEntry* __restrict__ entries, T* __restrict__ part,
@@ -175,11 +172,11 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
       if constexpr ( KQ )
       {
          const T* M  = ec + 1100 * t + 10 * l; // M[100 q + j]: row l of quadrature-point matrix q
-         const T* kv = kq + 11 * ( i64( c * 6 + t ) * SA + cidx( Nc + 1, ax, ay, az ) );
+         const T* kv = kq + i64( c * 6 + t ) * SA + cidx( Nc + 1, ax, ay, az );
          T        kk[11];
 #pragma unroll
          for ( int q = 0; q < 11; ++q )
-            kk[q] = kv[q];
+            kk[q] = kv[i64( q ) * kqStride];
          if constexpr ( DIAG )
          {
 #pragma unroll
@@ -266,20 +263,6 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
 // class; the sum goes to the face's partial-sum block like the element kernel's. On each face the points split
 // into four sub-lattices by the parities of their face coordinates (u, v) = (2 i + 2 - ku, 2 j + 2 - kv), u, v >= 1,
 // u + v <= N - 1, and each sub-lattice has one parity class, so the offsets are known at compile time.
-__host__ __device__ constexpr int p2FaceSubCount( int N, int ku, int kv )
-{
-   return N - 5 + ku + kv < 0 ? 0 : triCount( ( N - 5 + ku + kv ) / 2 + 1 );
-}
-struct P2FaceTiles
-{
-   int start[5]; // first tile of each (ku, kv) = (s & 1, s >> 1) sub-lattice within a face; start[4] = per face
-};
-
-// parity class of the points of face F whose face coordinates have parities (KU, KV)
-__host__ __device__ constexpr int p2FaceClass( int F, int KU, int KV )
-{
-   return ( F == 2 ? 0 : KU ) | ( ( F == 0 || F == 3 ? KV : ( F == 2 ? KU : 0 ) ) << 1 ) | ( ( F == 0 ? 0 : ( F == 3 ? ( KU + KV ) & 1 : KV ) ) << 2 );
-}
 
 // Stored stencils of the variable-coefficient operator: per cell, the merged stencils of the cell-interior
 // points class by class ([e][point of the class sub-lattice], the class-kappa points p = 2 q + kappa of the cell
@@ -615,16 +598,6 @@ __global__ __launch_bounds__( kBlock ) void kP2Rows( int L, int nC, i64 stride, 
 // ------------------------------------------------------------------ transfers: cell-interior points
 // Interior fine points of one residue class r = P mod 4 share one interpolation stencil (coarse offsets and
 // weights known at compile time); interior coarse points of one parity class share one restriction stencil.
-// interior points of a lattice with N intervals of parity class cls: p = 2 q + cls, q' = q - (1 - cls), |q'| <= M
-__host__ __device__ constexpr int p2InteriorExtent( int N, int cls )
-{
-   const int k = ( cls & 1 ) + ( ( cls >> 1 ) & 1 ) + ( ( cls >> 2 ) & 1 );
-   return ( N - 1 - k ) / 2 - ( 3 - k );
-}
-struct P2ClassTiles
-{
-   int start[65]; // first tile of each class within a cell; start[nClasses] = tiles per cell
-};
 
 // Interior fine points row by row: one wave per fine row (y, z), y, z >= 1, y + z <= Nf - 2; lane l holds the four
 // consecutive points x = 4 q + r, r = 0..3, q = 64 j + l in pass j. All points of the wave share (y mod 4, z mod 4),
@@ -770,15 +743,6 @@ __global__ __launch_bounds__( kBlock ) void kP2RestrictInt( int Lc, i64 strideC,
 // Stored stencils, cell-interior points one class per wave (the class sub-lattice, p2InteriorExtent): each
 // thread gathers its point's neighbours and reads its own weights ([e][sub-lattice point], consecutive threads
 // read consecutive weights), then the mode's epilogue; the inverse diagonal is the inverse centre weight.
-// up to three functions handled by one stored-stencil pass (the components of a vector function)
-template < typename T >
-struct P2Comps
-{
-   const T* x[3];
-   const T* in1[3];
-   T*       out0[3];
-   T*       out1[3];
-};
 
 template < typename T, int MODE, int CLS, int NC >
 __device__ __forceinline__ void p2StoredIntPt( int c, int s, int N, i64 stride, const T* __restrict__ sw, const P2Comps< T >& cp, T a )
@@ -857,91 +821,53 @@ __global__ __launch_bounds__( kBlock ) void kP2StoredInt( int L, i64 stride, P2C
       ++cls;
    p2StoredIntDispatch< T, MODE, 0, NC >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << L, stride, sw, cp, a );
 }
-inline P2ClassTiles p2InteriorTiles( int N )
-{
-   P2ClassTiles ct;
-   ct.start[0] = 0;
-   for ( int k = 0; k < 8; ++k )
-   {
-      const int M     = p2InteriorExtent( N, k );
-      ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
-   }
-   return ct;
-}
 
 // ------------------------------------------------------------------ stored stencils (variable coefficient)
 // Setup: the stencil of a point is the sum, over the micro-cells that have it as node l (on the cell's side of
 // face F for a face point), of row l of sum_q k_q M_{t,q}, keyed by the offset to the other node.
-constexpr int p2OffsetIndex( const P2StencilOffsets& o, int dx, int dy, int dz )
-{
-   for ( int e = 0; e < o.n; ++e )
-      if ( o.d[e][0] == dx && o.d[e][1] == dy && o.d[e][2] == dz )
-         return e;
-   return -1;
-}
-// offset index of node j seen from node l of micro-cell type t, for points of class CLS (face F)
-struct P2OffsetTable
-{
-   int e[6][10][10] = {};
-};
-constexpr P2OffsetTable p2OffsetTable( int cls, int f )
-{
-   P2OffsetTable          tb;
-   const P2StencilOffsets off = p2StencilOffsets( cls, f );
-   for ( int t = 0; t < 6; ++t )
-      for ( int l = 0; l < 10; ++l )
-         for ( int j = 0; j < 10; ++j )
-            tb.e[t][l][j] = p2OffsetIndex( off, p2NodeOffset( t, j, 0 ) - p2NodeOffset( t, l, 0 ), p2NodeOffset( t, j, 1 ) - p2NodeOffset( t, l, 1 ),
-                                           p2NodeOffset( t, j, 2 ) - p2NodeOffset( t, l, 2 ) );
-   return tb;
-}
+// Stores the stencil of a point (p2StencilOffsets(CLS, F), entry e at out[e estride]): the sum, over the micro-cells
+// that have the point as node l, of row l of sum_q k_q M_{t,q} (rows staged in LDS by p2StageRows), accumulated in
+// place in the point's own output slots (one micro-cell per loop iteration, in the same order as the on-the-fly
+// assembly), so no per-point stencil array is held.
 template < int CLS, int F >
-__device__ __forceinline__ void p2StorePoint( int c, int N, int px, int py, int pz, const double* __restrict__ el,
-                                              const double* __restrict__ kq, double* __restrict__ out, i64 estride )
+__device__ __forceinline__ void p2StorePoint( int c, int N, int px, int py, int pz, const double* __restrict__ kq, i64 kqStride,
+                                              double* __restrict__ out, i64 estride, const double* __restrict__ Ms )
 {
    constexpr P2StencilOffsets off = p2StencilOffsets( CLS, F );
-   constexpr P2OffsetTable    tab = p2OffsetTable( CLS, F );
-   typedef const double __attribute__( ( address_space( 4 ) ) ) CD;
-   const CD* ec = (const CD*) ( el + i64( c ) * 6600 );
-   const int Nc = N >> 1, SA = S3i( Nc + 1 );
-   double    W[off.n];
 #pragma unroll
    for ( int e = 0; e < off.n; ++e )
-      W[e] = 0.0;
+      out[i64( e ) * estride] = 0.0;
+   const int          Nc = N >> 1, SA = S3i( Nc + 1 );
+   const P2KQEntries& E  = kP2KQTab.c[CLS][F + 1];
+#pragma unroll 1
+   for ( int k = 0; k < E.n; ++k )
+   {
+      const P2KQEntry& e  = E.e[k];
+      const int        ax = ( px - e.nd[0] ) >> 1, ay = ( py - e.nd[1] ) >> 1, az = ( pz - e.nd[2] ) >> 1;
+      const double*    kv = kq + i64( c * 6 + e.t ) * SA + cidx( Nc + 1, ax, ay, az );
+      double           kk[11];
 #pragma unroll
-   for ( int t = 0; t < 6; ++t )
+      for ( int q = 0; q < 11; ++q )
+         kk[q] = kv[i64( q ) * kqStride];
 #pragma unroll
-      for ( int l = 0; l < 10; ++l )
+      for ( int jj = 0; jj < 10; ++jj )
       {
-         if ( p2NodeClass( t, l ) != CLS || !p2OnCellSide( F, t, l ) )
-            continue;
-         const int     ax = ( px - p2NodeOffset( t, l, 0 ) ) >> 1, ay = ( py - p2NodeOffset( t, l, 1 ) ) >> 1,
-                       az = ( pz - p2NodeOffset( t, l, 2 ) ) >> 1;
-         const double* kv = kq + 11 * ( i64( c * 6 + t ) * SA + cidx( Nc + 1, ax, ay, az ) );
-         double        kk[11];
+         const double* row = Ms + ( e.ord * 10 + jj ) * kP2RowPad;
+         double        r   = 0.0;
 #pragma unroll
          for ( int q = 0; q < 11; ++q )
-            kk[q] = kv[q];
-#pragma unroll
-         for ( int j = 0; j < 10; ++j )
-         {
-            double r = 0.0;
-#pragma unroll
-            for ( int q = 0; q < 11; ++q )
-               r += kk[q] * ec[1100 * t + 100 * q + 10 * l + j];
-            W[tab.e[t][l][j]] += r;
-         }
+            r += kk[q] * row[q];
+         out[i64( e.slot[jj] ) * estride] += r;
       }
-#pragma unroll
-   for ( int e = 0; e < off.n; ++e )
-      out[i64( e ) * estride] = W[e];
+   }
 }
 template < int CLS >
 __device__ __forceinline__ void p2StoreIntPt( int c, int s, int N, const double* __restrict__ el, const double* __restrict__ kq,
-                                              double* __restrict__ sw )
+                                              i64 kqStride, double* __restrict__ sw, double* __restrict__ Ms )
 {
    constexpr int kx = CLS & 1, ky = ( CLS >> 1 ) & 1, kz = ( CLS >> 2 ) & 1;
-   const int     M  = p2InteriorExtent( N, CLS );
+   p2StageRows< double, CLS >( el, c, Ms );
+   const int M = p2InteriorExtent( N, CLS );
    if ( M < 0 || s >= S3i( M + 1 ) )
       return;
    int qx, qy, qz;
@@ -950,31 +876,34 @@ __device__ __forceinline__ void p2StoreIntPt( int c, int s, int N, const double*
    const int px = 2 * ( qx + 1 - kx ) + kx, py = 2 * ( qy + 1 - ky ) + ky, pz = 2 * ( qz + 1 - kz ) + kz;
    const int SW = p2SubWidth( N, CLS );
    double*   o  = sw + i64( c ) * p2StoredCellSize( N ) + p2StoredIntBase( N, CLS ) + cidx( SW, px >> 1, py >> 1, pz >> 1 );
-   p2StorePoint< CLS, -1 >( c, N, px, py, pz, el, kq, o, S3i( SW ) );
+   p2StorePoint< CLS, -1 >( c, N, px, py, pz, kq, kqStride, o, S3i( SW ), Ms );
 }
 template < int CLS >
-__device__ __forceinline__ void p2StoreIntDispatch( int cls, int c, int s, int N, const double* el, const double* kq, double* sw )
+__device__ __forceinline__ void p2StoreIntDispatch( int cls, int c, int s, int N, const double* el, const double* kq, i64 kqStride,
+                                                    double* sw, double* Ms )
 {
    if ( cls == CLS )
-      p2StoreIntPt< CLS >( c, s, N, el, kq, sw );
+      p2StoreIntPt< CLS >( c, s, N, el, kq, kqStride, sw, Ms );
    else if constexpr ( CLS < 7 )
-      p2StoreIntDispatch< CLS + 1 >( cls, c, s, N, el, kq, sw );
+      p2StoreIntDispatch< CLS + 1 >( cls, c, s, N, el, kq, kqStride, sw, Ms );
 }
 __global__ __launch_bounds__( kBlock ) void kP2StoreInt( int L, P2ClassTiles ct, const double* __restrict__ el,
-                                                         const double* __restrict__ kq, double* __restrict__ sw )
+                                                         const double* __restrict__ kq, i64 kqStride, double* __restrict__ sw )
 {
-   const int per = ct.start[8];
-   const int c   = int( blockIdx.x ) / per;
-   const int t   = int( blockIdx.x ) - c * per;
-   int       cls = 0;
+   __shared__ double Ms[kP2MaxPairs * 10 * kP2RowPad];
+   const int         per = ct.start[8];
+   const int         c   = int( blockIdx.x ) / per;
+   const int         t   = int( blockIdx.x ) - c * per;
+   int               cls = 0;
    while ( cls < 7 && t >= ct.start[cls + 1] )
       ++cls;
-   p2StoreIntDispatch< 0 >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << L, el, kq, sw );
+   p2StoreIntDispatch< 0 >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << L, el, kq, kqStride, sw, Ms );
 }
 template < int F, int KU, int KV >
 __device__ __forceinline__ void p2StoreFacePt( int c, int s, int N, const double* __restrict__ el, const double* __restrict__ kq,
-                                               double* __restrict__ sw )
+                                               i64 kqStride, double* __restrict__ sw, double* __restrict__ Ms )
 {
+   p2StageRows< double, p2FaceClass( F, KU, KV ) >( el, c, Ms );
    const int M = ( N - 5 + KU + KV ) / 2;
    if ( s >= triCount( M + 1 ) )
       return;
@@ -983,10 +912,10 @@ __device__ __forceinline__ void p2StoreFacePt( int c, int s, int N, const double
    const int u = 2 * i + 2 - KU, v = 2 * j + 2 - KV;
    const int px = F == 2 ? 0 : u, py = F == 0 || F == 3 ? v : ( F == 2 ? u : 0 ), pz = F == 0 ? 0 : ( F == 3 ? N - u - v : v );
    double*   o  = sw + i64( c ) * p2StoredCellSize( N ) + p2StoredFaceBase( N, F, KU | ( KV << 1 ) ) + s;
-   p2StorePoint< p2FaceClass( F, KU, KV ), F >( c, N, px, py, pz, el, kq, o, triCount( M + 1 ) );
+   p2StorePoint< p2FaceClass( F, KU, KV ), F >( c, N, px, py, pz, kq, kqStride, o, triCount( M + 1 ), Ms );
 }
 __global__ __launch_bounds__( kBlock ) void kP2StoreFace( int L, P2FaceTiles ft, const double* __restrict__ el,
-                                                          const double* __restrict__ kq, double* __restrict__ sw )
+                                                          const double* __restrict__ kq, i64 kqStride, double* __restrict__ sw )
 {
    const int per = ft.start[4];
    const int c   = int( blockIdx.x ) / ( 4 * per );
@@ -998,8 +927,9 @@ __global__ __launch_bounds__( kBlock ) void kP2StoreFace( int L, P2FaceTiles ft,
       ++sub;
    const int s = ( r - ft.start[sub] ) * kBlock + int( threadIdx.x );
    const int N = 1 << L;
+   __shared__ double Ms[kP2MaxPairs * 10 * kP2RowPad];
 #define HYG_P2_STORE_FACE( F, SUB ) \
-   case 4 * F + SUB: p2StoreFacePt< F, ( SUB & 1 ), ( SUB >> 1 ) >( c, s, N, el, kq, sw ); break;
+   case 4 * F + SUB: p2StoreFacePt< F, ( SUB & 1 ), ( SUB >> 1 ) >( c, s, N, el, kq, kqStride, sw, Ms ); break;
    switch ( 4 * f + sub )
    {
       HYG_P2_STORE_FACE( 0, 0 )
@@ -1017,10 +947,11 @@ __global__ __launch_bounds__( kBlock ) void kP2StoreFace( int L, P2FaceTiles ft,
       HYG_P2_STORE_FACE( 3, 0 )
       HYG_P2_STORE_FACE( 3, 1 )
       HYG_P2_STORE_FACE( 3, 2 )
-   default: p2StoreFacePt< 3, 1, 1 >( c, s, N, el, kq, sw ); break;
+   default: p2StoreFacePt< 3, 1, 1 >( c, s, N, el, kq, kqStride, sw, Ms ); break;
    }
 #undef HYG_P2_STORE_FACE
 }
+
 
 // ------------------------------------------------------------------ prolongation
 // Value at point p (fine lattice, N = 2^Lf) of the quadratic interpolant of the coarse function cs
@@ -1153,16 +1084,81 @@ int p2Config( const LevelGeom& g, int dtype, int mode )
 }
 void setP2Config( LevelGeom& g, int dtype, int mode, int cfg ) { g.p2Cfg[dtype == 8 ? 0 : 1][mode] = cfg; }
 
+// element-wise coefficient values per quadrature point are kq[q kqStride + element]; per-component partial sums of
+// the variable-coefficient operator lie partStride / lowStride apart
+static i64 p2KqStride( const MeshCtx& m, int L ) { return i64( m.mesh->nC() ) * 6 * S3i( ( 1 << ( L - 1 ) ) + 1 ); }
+static i64 p2PartStride( const MeshCtx& m, int L ) { return i64( m.mesh->nF() ) * 2 * i64( ( 1 << L ) + 1 ) * ( ( 1 << L ) + 1 ); }
+static i64 p2LowStride( const MeshCtx& m, int L ) { return i64( m.entriesHost.size() ) * ( ( 1 << L ) + 1 ); }
+
+template < typename T, int MODE >
+static void p2Combines( const MeshCtx& m, const LevelGeom& g, int L, const StencilCall& c, T* part, T* low, const T* dv, hipStream_t s )
+{
+   const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
+   if ( nFace > 0 )
+      hipLaunchKernelGGL( ( kFaceCombine< T, MODE > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
+                          g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, m.mesh->nV() + m.mesh->nE(), part,
+                          (const T*) nullptr, dv, L, g.stride, c.flag, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
+                          (T*) c.out1, T( c.a ) );
+   if ( nLow > 0 )
+      hipLaunchKernelGGL( ( kLowCombine< T, MODE > ), dim3( ceilDiv( nLow, kBlock / 64 ) ), dim3( kBlock ), 0, s, g.chunks, nLow,
+                          m.prims, m.entries, low, dv, L, g.stride, c.flag, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
+                          (T*) c.out1, T( c.a ) );
+}
+
+// variable coefficient assembled on the fly for n = 1 or 3 components (one assembly per point for all of them)
+template < typename T, int MODE >
+static void p2KQOnTheFly( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall* cs, int n, hipStream_t s )
+{
+   constexpr bool         dbl  = std::is_same< T, double >::value;
+   const hyg_op_s::Level& ol   = op.levels[L];
+   const int              nC   = m.mesh->nC(), N = 1 << L;
+   const T*               el   = dbl ? (const T*) ol.p2el : (const T*) ol.p2elF;
+   const T*               kq   = dbl ? (const T*) ol.p2kq : (const T*) ol.p2kqF;
+   T*                     part = dbl ? (T*) ol.facePart : (T*) ol.facePartF;
+   T*                     low  = dbl ? (T*) ol.lowPart : (T*) ol.lowPartF;
+   const T*               dv   = dbl ? (const T*) ol.dinv : (const T*) ol.dinvF;
+   const i64              kqs = p2KqStride( m, L ), ps = p2PartStride( m, L ), ls = p2LowStride( m, L );
+   const int              flag = cs[0].flag;
+   P2Comps< T >           cp{};
+   for ( int k = 0; k < n; ++k )
+   {
+      cp.x[k]    = (const T*) cs[k].x;
+      cp.in1[k]  = (const T*) cs[k].in1;
+      cp.out0[k] = (T*) cs[k].out0;
+      cp.out1[k] = (T*) cs[k].out1;
+   }
+   launchP2KQPoints< T >( L, nC, g.stride, el, kq, kqs, dv, cp, n, MODE, T( cs[0].a ), m.cellFaces, flag, part, ps, s );
+   const int tB = ceilDiv( i64( p2EdgeVertexCount( N ) ) * kP2EvLanes, kBlock );
+   for ( int k = 0; k < n; ++k )
+   {
+      const StencilCall& c = cs[k];
+      hipLaunchKernelGGL( ( kP2Apply< T, MODE, false, true, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq, kqs,
+                          m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part + k * ps, low + k * ls, dv, c.flag,
+                          (const T*) c.x, (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
+   }
+   for ( int k = 0; k < n; ++k )
+      p2Combines< T, MODE >( m, g, L, cs[k], part + k * ps, low + k * ls, dv, s );
+}
+
 template < typename T, int MODE, bool DIAG, bool KQ >
 static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, const StencilCall& c, hipStream_t s,
                   bool skipInterior = false )
 {
+   if constexpr ( KQ && !DIAG )
+   {
+      if ( !op.levels[L].p2s )
+      {
+         p2KQOnTheFly< T, MODE >( m, g, op, L, &c, 1, s );
+         return;
+      }
+   }
    constexpr bool         dbl  = std::is_same< T, double >::value;
    const hyg_op_s::Level& ol   = op.levels[L];
    const int              nC   = m.mesh->nC();
    const int              w    = ( 1 << L ) + 1;
    const T*               el   = dbl ? (const T*) ol.p2el : (const T*) ol.p2elF;
    const T*               kq   = dbl ? (const T*) ol.p2kq : (const T*) ol.p2kqF;
+   const i64              kqs  = KQ ? p2KqStride( m, L ) : 0;
    T*                     part = dbl ? (T*) ol.facePart : (T*) ol.facePartF;
    T*                     low  = dbl ? (T*) ol.lowPart : (T*) ol.lowPartF;
    const T*               dv   = DIAG ? nullptr : ( dbl ? (const T*) ol.dinv : (const T*) ol.dinvF );
@@ -1193,7 +1189,7 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
          hipLaunchKernelGGL( ( kP2Face< T, false > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, fsw, m.cellFaces,
                              c.flag, part, (const T*) c.x );
       const int tB = ceilDiv( i64( p2EdgeVertexCount( 1 << L ) ) * kP2EvLanes, kBlock );
-      hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq,
+      hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq, kqs,
                           m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
                           (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
    }
@@ -1216,24 +1212,15 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
          hipLaunchKernelGGL( ( kP2Face< T, true > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, sw, m.cellFaces,
                              c.flag, part, (const T*) c.x );
       const int tB = ceilDiv( i64( p2EdgeVertexCount( 1 << L ) ) * kP2EvLanes, kBlock );
-      hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq,
+      hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq, kqs,
                           m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
                           (const T*) c.in1, (T*) c.out0, (T*) c.out1, T( c.a ) );
    }
    else
-   hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, el, kq, m.p2tab,
+   hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, el, kq, kqs, m.p2tab,
                        m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x, (const T*) c.in1,
                        (T*) c.out0, (T*) c.out1, T( c.a ) );
-   const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
-   if ( nFace > 0 )
-      hipLaunchKernelGGL( ( kFaceCombine< T, MODE > ), dim3( xcdGrid( ceilDiv( nFace, kBlock / 64 ) ) ), dim3( kBlock ), 0, s,
-                          g.chunks + g.firstFaceChunk, nFace, m.prims, m.entries, m.mesh->nV() + m.mesh->nE(), part,
-                          (const T*) nullptr, dv, L, g.stride, c.flag, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
-                          (T*) c.out1, T( c.a ) );
-   if ( nLow > 0 )
-      hipLaunchKernelGGL( ( kLowCombine< T, MODE > ), dim3( ceilDiv( nLow, kBlock / 64 ) ), dim3( kBlock ), 0, s, g.chunks, nLow,
-                          m.prims, m.entries, low, dv, L, g.stride, c.flag, (const T*) c.x, (const T*) c.in1, (T*) c.out0,
-                          (T*) c.out1, T( c.a ) );
+   p2Combines< T, MODE >( m, g, L, c, part, low, dv, s );
 }
 
 template < typename T, int MODE, bool DIAG >
@@ -1311,6 +1298,17 @@ static void p2VectorT( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L
 
 void launchP2Vector( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, int dtype, const StencilCall* cs, int n, hipStream_t s )
 {
+   if ( op.kind == 3 && !op.levels[L].p2s && n == 3 && ( cs[0].mode == MODE_APPLY || cs[0].mode == MODE_APPLY_ADD ) )
+   {
+      // one on-the-fly assembly per point for the three components
+      if ( dtype == 8 )
+         cs[0].mode == MODE_APPLY ? p2KQOnTheFly< double, MODE_APPLY >( m, g, op, L, cs, n, s )
+                                  : p2KQOnTheFly< double, MODE_APPLY_ADD >( m, g, op, L, cs, n, s );
+      else
+         cs[0].mode == MODE_APPLY ? p2KQOnTheFly< float, MODE_APPLY >( m, g, op, L, cs, n, s )
+                                  : p2KQOnTheFly< float, MODE_APPLY_ADD >( m, g, op, L, cs, n, s );
+      return;
+   }
    const bool fused = op.kind == 3 && op.levels[L].p2s && n >= 1 && n <= 3 &&
                       ( cs[0].mode == MODE_APPLY || cs[0].mode == MODE_APPLY_ADD );
    if ( !fused )
@@ -1332,8 +1330,8 @@ void launchP2Coefficients( const MeshCtx& m, const hyg_op_s& op, int L, hipStrea
    const int              nC = m.mesh->nC();
    const int              tA = ceilDiv( S3i( ( 1 << ( L - 1 ) ) + 1 ), kBlock );
    const P2Coef           k{ op.p2k.kind, op.p2k.k0, op.p2k.alpha, op.p2k.beta, op.p2k.seed };
-   hipLaunchKernelGGL( ( kP2KQ< double > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, ol.elem, k, ol.p2kq );
-   hipLaunchKernelGGL( ( kP2KQ< float > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, ol.elem, k, ol.p2kqF );
+   hipLaunchKernelGGL( ( kP2KQ< double > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, ol.elem, k, ol.p2kq, p2KqStride( m, L ) );
+   hipLaunchKernelGGL( ( kP2KQ< float > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, ol.elem, k, ol.p2kqF, p2KqStride( m, L ) );
 }
 
 // stored stencils of the variable-coefficient P2 operator at lattice level L (FP64, and an FP32 copy)
@@ -1358,13 +1356,13 @@ void launchP2StoreStencils( const MeshCtx& m, hyg_op_s& op, int L, hipStream_t s
       ct.start[k + 1] = ct.start[k] + ( M >= 0 ? ceilDiv( S3i( M + 1 ), kBlock ) : 0 );
    }
    if ( ct.start[8] > 0 )
-      hipLaunchKernelGGL( kP2StoreInt, dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, ct, ol.p2el, ol.p2kq, ol.p2s );
+      hipLaunchKernelGGL( kP2StoreInt, dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, ct, ol.p2el, ol.p2kq, p2KqStride( m, L ), ol.p2s );
    P2FaceTiles ft;
    ft.start[0] = 0;
    for ( int k = 0; k < 4; ++k )
       ft.start[k + 1] = ft.start[k] + ceilDiv( p2FaceSubCount( N, k & 1, k >> 1 ), kBlock );
    if ( ft.start[4] > 0 )
-      hipLaunchKernelGGL( kP2StoreFace, dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, ft, ol.p2el, ol.p2kq, ol.p2s );
+      hipLaunchKernelGGL( kP2StoreFace, dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, ft, ol.p2el, ol.p2kq, p2KqStride( m, L ), ol.p2s );
    launchConvert( n, 4, ol.p2sF, 8, ol.p2s, s );
 }
 

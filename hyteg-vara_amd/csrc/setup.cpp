@@ -900,10 +900,11 @@ void assembleP2DivKGrad( hyg_op_s& op )
       const size_t nFunc   = size_t( m.nC() ) * size_t( cellStride( L ) ) + size_t( kTailSlack );
       ol.p2kq      = static_cast< double* >( alloc( nKq * sizeof( double ) ) );
       ol.p2kqF     = static_cast< float* >( alloc( nKq * sizeof( float ) ) );
-      ol.lowPart   = static_cast< double* >( alloc( nLow * sizeof( double ) ) );
-      ol.lowPartF  = static_cast< float* >( alloc( nLow * sizeof( float ) ) );
-      ol.facePart  = static_cast< double* >( alloc( nFace * sizeof( double ) ) );
-      ol.facePartF = static_cast< float* >( alloc( nFace * sizeof( float ) ) );
+      // partial sums for up to three components (a vector function's components applied in one pass)
+      ol.lowPart   = static_cast< double* >( alloc( 3 * nLow * sizeof( double ) ) );
+      ol.lowPartF  = static_cast< float* >( alloc( 3 * nLow * sizeof( float ) ) );
+      ol.facePart  = static_cast< double* >( alloc( 3 * nFace * sizeof( double ) ) );
+      ol.facePartF = static_cast< float* >( alloc( 3 * nFace * sizeof( float ) ) );
       ol.dinv      = static_cast< double* >( alloc( nFunc * sizeof( double ) ) );
       ol.dinvF     = static_cast< float* >( alloc( nFunc * sizeof( float ) ) );
    }

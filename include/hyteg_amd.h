@@ -238,7 +238,9 @@ int hyg_apply( hyg_op op, hyg_func src, hyg_func dst, int level, int flag, int u
 /* VectorLaplaceOperator::apply (operators/VectorLaplaceOperator.hpp:36-66): the scalar operator applied to
  * each of n <= 3 component functions (same value type, distinct sources and destinations). For the
  * variable-coefficient P2 operator with stored stencils the cell interiors of all components are done in one
- * pass that reads every stored weight once; otherwise the components are applied one after the other. */
+ * pass that reads every stored weight once; without stored stencils and n = 3, each cell-interior and cell-face
+ * point's stencil is assembled once for the three components; otherwise the components are applied one after the
+ * other. Every component's result equals its single-component apply bit for bit. */
 int hyg_apply_vector( hyg_op op, int n, const hyg_func* srcs, const hyg_func* dsts, int level, int flag, int update, void* stream );
 /* WeightedJacobiSmoothable::smooth_jac(dst, rhs, src, relax, level, flag) (solvers/Smoothables.hpp:32-43;
  * P1Operator.hpp:433-451): dst = src + relax * D^-1 (rhs - A src), fused into one pass. */
