@@ -35,6 +35,11 @@ __host__ __device__ constexpr int p2InteriorExtent( int N, int cls )
    const int k = ( cls & 1 ) + ( ( cls >> 1 ) & 1 ) + ( ( cls >> 2 ) & 1 );
    return ( N - 1 - k ) / 2 - ( 3 - k );
 }
+// all points of parity class cls of a cell lattice with N intervals: p = 2 q + cls, |q| <= M
+__host__ __device__ constexpr int p2ClassExtent( int N, int cls )
+{
+   return ( N - ( cls & 1 ) - ( ( cls >> 1 ) & 1 ) - ( ( cls >> 2 ) & 1 ) ) / 2;
+}
 struct P2ClassTiles
 {
    int start[65]; // first tile of each class within a cell; start[nClasses] = tiles per cell
@@ -134,7 +139,7 @@ __device__ __forceinline__ void p2StageRows( const T* __restrict__ el, int c, T*
 // from the point ((dz + 2) * 5 + dy + 2 in bits 0..7, dx + 2 in bits 8..10) and its slot in the point's stencil.
 struct P2KQEntry
 {
-   int t = 0, l = 0, ord = 0, nd[3] = {}, node[10] = {}, slot[10] = {}; // slot: index of node j in p2StencilOffsets(cls, f)
+   int t = 0, l = 0, ord = 0, mcmax = 0, nd[3] = {}, node[10] = {}, slot[10] = {}; // slot: index of node j in p2StencilOffsets(cls, f)
 };
 struct P2KQEntries
 {
@@ -160,6 +165,7 @@ constexpr P2KQEntryTables p2KQEntryTables()
                   continue;
                P2KQEntry& e = L.e[L.n++];
                e.t          = t;
+               e.mcmax      = p2CellMaxSum( t );
                e.l          = l;
                e.ord        = p2PairOrdinal( cls, t, l );
                for ( int k = 0; k < 3; ++k )

@@ -22,7 +22,7 @@ namespace hyg {
 template < typename T, int CLS, int F >
 __device__ __forceinline__ void p2KQDirect( int c, int N, int px, int py, int pz, const T* __restrict__ kq, i64 kqStride,
                                             const T* __restrict__ Ms, const int ( &rb )[5][5], i64 base, const P2Comps< T >& cp, int nc,
-                                            T ( &val )[3] )
+                                            T ( &val )[3], bool onFace = false )
 {
    const int          Nc = N >> 1, SA = S3i( Nc + 1 );
    const P2KQEntries& E  = kP2KQTab.c[CLS][F + 1];
@@ -35,8 +35,11 @@ __device__ __forceinline__ void p2KQDirect( int c, int N, int px, int py, int pz
    {
       const P2KQEntry& e  = E.e[k];
       const int        ax = ( px - e.nd[0] ) >> 1, ay = ( py - e.nd[1] ) >> 1, az = ( pz - e.nd[2] ) >> 1;
-      const T*         kv = kq + i64( c * 6 + e.t ) * SA + cidx( Nc + 1, ax, ay, az );
-      T                kk[11];
+      // a point on a cell face takes the micro-cells inside the cell (its one-sided sum)
+      if ( onFace && ( ax < 0 || ay < 0 || az < 0 || ax + ay + az + e.mcmax > Nc ) )
+         continue;
+      const T* kv = kq + i64( c * 6 + e.t ) * SA + cidx( Nc + 1, ax, ay, az );
+      T        kk[11];
 #pragma unroll
       for ( int q = 0; q < 11; ++q )
          kk[q] = kv[i64( q ) * kqStride];
@@ -109,19 +112,34 @@ __device__ __forceinline__ void p2EmitInterior( int mode, int nc, const T ( &val
          cp.out1[k][q] = r1;
    }
 }
+// All points of class CLS of a cell that are not on a cell edge: the class sub-lattice {2 q + kappa : 2 |q| + |kappa| <= N}
+// (p2ClassExtent), so the points on the x = 0 and diagonal faces sit in the waves of their rows' interior points and
+// are read and written with them. Interior points get the mode's epilogue, face points store their one-sided sums.
 template < typename T, int CLS >
 __device__ __forceinline__ void p2KQIntPt( int c, int s, int N, i64 stride, const T* __restrict__ el, const T* __restrict__ kq,
                                            i64 kqStride, const T* __restrict__ dinv, const P2Comps< T >& cp, int nc, int mode, T a,
-                                           T* __restrict__ Ms )
+                                           T* __restrict__ Ms, const CellFaceSlot* __restrict__ cfs, int flag, T* __restrict__ part,
+                                           i64 partStride )
 {
    constexpr int kx = CLS & 1, ky = ( CLS >> 1 ) & 1, kz = ( CLS >> 2 ) & 1;
    p2StageRows< T, CLS >( el, c, Ms );
-   const int M = p2InteriorExtent( N, CLS );
-   if ( M < 0 || s >= S3i( M + 1 ) )
+   const int M = p2ClassExtent( N, CLS );
+   if ( s >= S3i( M + 1 ) )
       return;
    int qx, qy, qz;
    tetXYZ( M + 1, s, qx, qy, qz );
-   const int px = 2 * ( qx + 1 - kx ) + kx, py = 2 * ( qy + 1 - ky ) + ky, pz = 2 * ( qz + 1 - kz ) + kz;
+   const int px = 2 * qx + kx, py = 2 * qy + ky, pz = 2 * qz + kz;
+   const int slot = classifySlot( N, px, py, pz );
+   if ( slot > 4 || ( slot == 0 && !( flag & 1 ) ) ) // cell edges and vertices: kP2Apply
+      return;
+   i64 pi = 0;
+   if ( slot > 0 )
+   {
+      const CellFaceSlot fs = cfs[4 * c + slot - 1];
+      if ( !( fs.bc & flag ) )
+         return;
+      pi = i64( fs.block ) * ( N + 1 ) * ( N + 1 ) + facePartIndex( slot - 1, N, px, py, pz );
+   }
    const int w = N + 1;
    int       rb[5][5];
 #pragma unroll
@@ -134,22 +152,35 @@ __device__ __forceinline__ void p2KQIntPt( int c, int s, int N, i64 stride, cons
          rb[dz + 2][dy + 2] = pb + rowStart( Wz, py + dy );
    }
    T val[3] = { T( 0 ), T( 0 ), T( 0 ) };
-   p2KQDirect< T, CLS, -1 >( c, N, px, py, pz, kq, kqStride, Ms, rb, i64( c ) * stride + px, cp, nc, val );
-   p2EmitInterior< T >( mode, nc, val, i64( c ) * stride + rb[2][2] + px, cp, dinv, a );
+   p2KQDirect< T, CLS, -1 >( c, N, px, py, pz, kq, kqStride, Ms, rb, i64( c ) * stride + px, cp, nc, val, slot > 0 );
+   if ( slot == 0 )
+      p2EmitInterior< T >( mode, nc, val, i64( c ) * stride + rb[2][2] + px, cp, dinv, a );
+   else
+   {
+#pragma unroll
+      for ( int k = 0; k < 3; ++k )
+      {
+         if ( k >= nc )
+            break;
+         part[i64( k ) * partStride + pi] = val[k];
+      }
+   }
 }
 template < typename T, int CLS >
 __device__ __forceinline__ void p2KQIntDispatch( int cls, int c, int s, int N, i64 stride, const T* el, const T* kq, i64 kqStride,
-                                                 const T* dinv, const P2Comps< T >& cp, int nc, int mode, T a, T* Ms )
+                                                 const T* dinv, const P2Comps< T >& cp, int nc, int mode, T a, T* Ms,
+                                                 const CellFaceSlot* cfs, int flag, T* part, i64 partStride )
 {
    if ( cls == CLS )
-      p2KQIntPt< T, CLS >( c, s, N, stride, el, kq, kqStride, dinv, cp, nc, mode, a, Ms );
+      p2KQIntPt< T, CLS >( c, s, N, stride, el, kq, kqStride, dinv, cp, nc, mode, a, Ms, cfs, flag, part, partStride );
    else if constexpr ( CLS < 7 )
-      p2KQIntDispatch< T, CLS + 1 >( cls, c, s, N, stride, el, kq, kqStride, dinv, cp, nc, mode, a, Ms );
+      p2KQIntDispatch< T, CLS + 1 >( cls, c, s, N, stride, el, kq, kqStride, dinv, cp, nc, mode, a, Ms, cfs, flag, part, partStride );
 }
 template < typename T >
 __global__ __launch_bounds__( kBlock ) void kP2KQInt( int L, i64 stride, P2ClassTiles ct, const T* __restrict__ el,
                                                       const T* __restrict__ kq, i64 kqStride, const T* __restrict__ dinv,
-                                                      P2Comps< T > cp, int nc, int mode, T a )
+                                                      P2Comps< T > cp, int nc, int mode, T a, const CellFaceSlot* __restrict__ cfs,
+                                                      int flag, T* __restrict__ part, i64 partStride )
 {
    __shared__ T Ms[kP2MaxPairs * 10 * kP2RowPad];
    const int    per = ct.start[8];
@@ -159,105 +190,19 @@ __global__ __launch_bounds__( kBlock ) void kP2KQInt( int L, i64 stride, P2Class
    while ( cls < 7 && t >= ct.start[cls + 1] )
       ++cls;
    p2KQIntDispatch< T, 0 >( cls, c, ( t - ct.start[cls] ) * kBlock + int( threadIdx.x ), 1 << L, stride, el, kq, kqStride, dinv, cp,
-                            nc, mode, a, Ms );
+                            nc, mode, a, Ms, cfs, flag, part, partStride );
 }
-template < typename T, int F, int KU, int KV >
-__device__ __forceinline__ void p2KQFacePt( int c, int s, int N, i64 stride, const T* __restrict__ el, const T* __restrict__ kq,
-                                            i64 kqStride, const CellFaceSlot& fs, T* __restrict__ part, i64 partStride,
-                                            const P2Comps< T >& cp, int nc, T* __restrict__ Ms )
-{
-   constexpr int CLS = p2FaceClass( F, KU, KV );
-   p2StageRows< T, CLS >( el, c, Ms );
-   const int M = ( N - 5 + KU + KV ) / 2;
-   if ( s >= triCount( M + 1 ) )
-      return;
-   int i, j;
-   planeXY( M + 1, s, i, j );
-   const int u = 2 * i + 2 - KU, v = 2 * j + 2 - KV;
-   const int px = F == 2 ? 0 : u, py = F == 0 || F == 3 ? v : ( F == 2 ? u : 0 ), pz = F == 0 ? 0 : ( F == 3 ? N - u - v : v );
-   const int w = N + 1;
-   int       rb[5][5];
-#pragma unroll
-   for ( int dz = -2; dz <= 2; ++dz )
-   {
-      const int Wz = w - ( pz + dz );
-      const int pb = S3i( w ) - S3i( Wz );
-#pragma unroll
-      for ( int dy = -2; dy <= 2; ++dy )
-         rb[dz + 2][dy + 2] = pb + rowStart( Wz, py + dy );
-   }
-   T val[3] = { T( 0 ), T( 0 ), T( 0 ) };
-   p2KQDirect< T, CLS, F >( c, N, px, py, pz, kq, kqStride, Ms, rb, i64( c ) * stride + px, cp, nc, val );
-   const i64 pi = i64( fs.block ) * w * w + facePartIndex( F, N, px, py, pz );
-#pragma unroll
-   for ( int k = 0; k < 3; ++k )
-   {
-      if ( k >= nc )
-         break;
-      part[i64( k ) * partStride + pi] = val[k];
-   }
-}
-template < typename T >
-__global__ __launch_bounds__( kBlock ) void kP2KQFace( int L, i64 stride, P2FaceTiles ft, const T* __restrict__ el,
-                                                       const T* __restrict__ kq, i64 kqStride, const CellFaceSlot* __restrict__ cfs,
-                                                       int flag, T* __restrict__ part, i64 partStride, P2Comps< T > cp, int nc )
-{
-   const int per = ft.start[4];
-   const int c   = int( blockIdx.x ) / ( 4 * per );
-   int       r   = int( blockIdx.x ) - c * 4 * per;
-   const int f   = r / per;
-   r -= f * per;
-   int sub = 0;
-   while ( sub < 3 && r >= ft.start[sub + 1] )
-      ++sub;
-   const int          s  = ( r - ft.start[sub] ) * kBlock + int( threadIdx.x );
-   const CellFaceSlot fs = cfs[4 * c + f];
-   if ( !( fs.bc & flag ) ) // the whole block leaves (c, f are the block's)
-      return;
-   __shared__ T Ms[kP2MaxPairs * 10 * kP2RowPad];
-   const int    N = 1 << L;
-#define HYG_P2_KQ_FACE( F, SUB )                                                                                                          \
-   case 4 * F + SUB:                                                                                                                     \
-      p2KQFacePt< T, F, ( SUB & 1 ), ( SUB >> 1 ) >( c, s, N, stride, el, kq, kqStride, fs, part, partStride, cp, nc, Ms );                   \
-      break;
-   switch ( 4 * f + sub )
-   {
-      HYG_P2_KQ_FACE( 0, 0 )
-      HYG_P2_KQ_FACE( 0, 1 )
-      HYG_P2_KQ_FACE( 0, 2 )
-      HYG_P2_KQ_FACE( 0, 3 )
-      HYG_P2_KQ_FACE( 1, 0 )
-      HYG_P2_KQ_FACE( 1, 1 )
-      HYG_P2_KQ_FACE( 1, 2 )
-      HYG_P2_KQ_FACE( 1, 3 )
-      HYG_P2_KQ_FACE( 2, 0 )
-      HYG_P2_KQ_FACE( 2, 1 )
-      HYG_P2_KQ_FACE( 2, 2 )
-      HYG_P2_KQ_FACE( 2, 3 )
-      HYG_P2_KQ_FACE( 3, 0 )
-      HYG_P2_KQ_FACE( 3, 1 )
-      HYG_P2_KQ_FACE( 3, 2 )
-   default: p2KQFacePt< T, 3, 1, 1 >( c, s, N, stride, el, kq, kqStride, fs, part, partStride, cp, nc, Ms ); break;
-   }
-#undef HYG_P2_KQ_FACE
-}
-
 template < typename T >
 void launchP2KQPoints( int L, int nC, i64 stride, const T* el, const T* kq, i64 kqStride, const T* dinv, const P2Comps< T >& cp, int nc,
                        int mode, T a, const CellFaceSlot* cfs, int flag, T* part, i64 partStride, hipStream_t s )
 {
-   const int          N  = 1 << L;
-   const P2ClassTiles ct = p2InteriorTiles( N );
-   if ( ct.start[8] > 0 && ( flag & 1 ) )
-      hipLaunchKernelGGL( ( kP2KQInt< T > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, stride, ct, el, kq, kqStride, dinv, cp, nc,
-                          mode, a );
-   P2FaceTiles ft;
-   ft.start[0] = 0;
-   for ( int k = 0; k < 4; ++k )
-      ft.start[k + 1] = ft.start[k] + ceilDiv( p2FaceSubCount( N, k & 1, k >> 1 ), kBlock );
-   if ( ft.start[4] > 0 )
-      hipLaunchKernelGGL( ( kP2KQFace< T > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, stride, ft, el, kq, kqStride, cfs,
-                          flag, part, partStride, cp, nc );
+   const int    N = 1 << L;
+   P2ClassTiles ct;
+   ct.start[0] = 0;
+   for ( int k = 0; k < 8; ++k )
+      ct.start[k + 1] = ct.start[k] + ceilDiv( S3i( p2ClassExtent( N, k ) + 1 ), kBlock );
+   hipLaunchKernelGGL( ( kP2KQInt< T > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, stride, ct, el, kq, kqStride, dinv, cp, nc,
+                       mode, a, cfs, flag, part, partStride );
 }
 
 } // namespace hyg
