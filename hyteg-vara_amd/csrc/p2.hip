@@ -262,7 +262,9 @@ __global__ __launch_bounds__( kBlock ) void kP2Apply( int L, int tilesPerCell, i
 // Points inside a cell face (not on a cell edge) take the one-sided merged stencil of their face and parity
 // class; the sum goes to the face's partial-sum block like the element kernel's. On each face the points split
 // into four sub-lattices by the parities of their face coordinates (u, v) = (2 i + 2 - ku, 2 j + 2 - kv), u, v >= 1,
-// u + v <= N - 1, and each sub-lattice has one parity class, so the offsets are known at compile time.
+// u + v <= N - 1, and each sub-lattice has one parity class, so the offsets are known at compile time. The cell
+// point of (u, v) is (u, v, 0), (u, 0, v), (0, u, v) and (u, N - u - v, v) on faces 0..3: consecutive threads (u)
+// stay in one z-plane, on the diagonal face too.
 
 // Stored stencils of the variable-coefficient operator: per cell, the merged stencils of the cell-interior
 // points class by class ([e][point of the class sub-lattice], the class-kappa points p = 2 q + kappa of the cell
@@ -314,7 +316,7 @@ __device__ __forceinline__ void p2FacePoint( int c, int s, int N, i64 stride, co
    int i, j;
    planeXY( M + 1, s, i, j );
    const int u = 2 * i + 2 - KU, v = 2 * j + 2 - KV;
-   const int px = F == 2 ? 0 : u, py = F == 0 || F == 3 ? v : ( F == 2 ? u : 0 ), pz = F == 0 ? 0 : ( F == 3 ? N - u - v : v );
+   const int px = F == 2 ? 0 : u, py = F == 0 ? v : ( F == 2 ? u : ( F == 3 ? N - u - v : 0 ) ), pz = F == 0 ? 0 : v;
    const int w = N + 1;
    typedef const T __attribute__( ( address_space( 4 ) ) ) CT;
    // uniform weights of the cell (constant coefficient), or this point's stored stencil
@@ -910,7 +912,7 @@ __device__ __forceinline__ void p2StoreFacePt( int c, int s, int N, const double
    int i, j;
    planeXY( M + 1, s, i, j );
    const int u = 2 * i + 2 - KU, v = 2 * j + 2 - KV;
-   const int px = F == 2 ? 0 : u, py = F == 0 || F == 3 ? v : ( F == 2 ? u : 0 ), pz = F == 0 ? 0 : ( F == 3 ? N - u - v : v );
+   const int px = F == 2 ? 0 : u, py = F == 0 ? v : ( F == 2 ? u : ( F == 3 ? N - u - v : 0 ) ), pz = F == 0 ? 0 : v;
    double*   o  = sw + i64( c ) * p2StoredCellSize( N ) + p2StoredFaceBase( N, F, KU | ( KV << 1 ) ) + s;
    p2StorePoint< p2FaceClass( F, KU, KV ), F >( c, N, px, py, pz, kq, kqStride, o, triCount( M + 1 ), Ms );
 }

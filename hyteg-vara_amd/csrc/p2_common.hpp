@@ -26,7 +26,7 @@ struct P2FaceTiles
 // parity class of the points of face F whose face coordinates have parities (KU, KV)
 __host__ __device__ constexpr int p2FaceClass( int F, int KU, int KV )
 {
-   return ( F == 2 ? 0 : KU ) | ( ( F == 0 || F == 3 ? KV : ( F == 2 ? KU : 0 ) ) << 1 ) | ( ( F == 0 ? 0 : ( F == 3 ? ( KU + KV ) & 1 : KV ) ) << 2 );
+   return ( F == 2 ? 0 : KU ) | ( ( F == 0 ? KV : ( F == 2 ? KU : ( F == 3 ? ( KU + KV ) & 1 : 0 ) ) ) << 1 ) | ( ( F == 0 ? 0 : KV ) << 2 );
 }
 
 // interior points of a lattice with N intervals of parity class cls: p = 2 q + cls, q' = q - (1 - cls), |q'| <= M
