@@ -11,7 +11,8 @@
 //
 // Parallel form of the lexicographic order:
 //   * vertices: independent (they read each other only from the snapshot), one thread each;
-//   * edges: one thread per edge walks i = 1..N-1 (or back), so the update of i-1 precedes the read of it;
+//   * edges: one wave per edge walks i = 1..N-1 (or back), its lanes sharing the terms of each point's sum, so the
+//     update of i-1 precedes the read of it;
 //   * faces: one block per face; in face coordinates the earlier neighbours of (i, j) are (i-1, j), (i, j-1) and
 //     (i+1, j-1), so the points of one anti-diagonal tau = i + 2j are independent and depend only on smaller tau:
 //     the block walks tau with a barrier between steps;
@@ -29,8 +30,10 @@ namespace hyg {
 
 namespace {
 
-// neighbour sum (centre excluded) of point (i, j) of a lower-dimensional primitive with nv vertices, per the rule
-template < typename T >
+// neighbour sum (centre excluded) of point (i, j) of a lower-dimensional primitive with nv vertices, per the rule;
+// with MAXC > 0 the primitive has at most MAXC incident cells and the loops are unrolled, so that every load of the
+// sum is issued before the first one is waited for
+template < typename T, int MAXC = 0 >
 __device__ __forceinline__ T ordNeighbourSum( const LowerPrim& p, const LowerEntry* __restrict__ entries, const T* __restrict__ entryW,
                                               int N, i64 stride, int i, int j, int nv, const T* __restrict__ x,
                                               const T* __restrict__ xo, T& centre )
@@ -38,8 +41,12 @@ __device__ __forceinline__ T ordNeighbourSum( const LowerPrim& p, const LowerEnt
    const int w   = N + 1;
    T         sum = T( 0 );
    centre        = T( 0 );
-   for ( int k = 0; k < p.nCells; ++k )
+   const int nk  = MAXC > 0 ? MAXC : p.nCells;
+#pragma unroll
+   for ( int k = 0; k < nk; ++k )
    {
+      if ( MAXC > 0 && k >= p.nCells )
+         break;
       const LowerEntry e = entries[p.entry0 + k];
       int              qx, qy, qz;
       entryPos( e, N, i, j, qx, qy, qz );
@@ -47,6 +54,7 @@ __device__ __forceinline__ T ordNeighbourSum( const LowerPrim& p, const LowerEnt
       const unsigned pmask = ( 1u << e.l0 ) | ( nv > 1 ? 1u << e.l1 : 0u ) | ( nv > 2 ? 1u << e.l2 : 0u );
       const i64      base  = i64( e.cell ) * stride;
       centre += wv[0];
+#pragma unroll
       for ( int d = 1; d < NDIR; ++d )
       {
          const T wd = wv[d];
@@ -62,14 +70,14 @@ __device__ __forceinline__ T ordNeighbourSum( const LowerPrim& p, const LowerEnt
 }
 
 // new value of point (i, j) of primitive p, written to every replica
-template < typename T >
+template < typename T, int MAXC = 0 >
 __device__ __forceinline__ void ordUpdate( const LowerPrim& p, const LowerEntry* __restrict__ entries, const T* __restrict__ entryW,
                                            int N, i64 stride, int i, int j, int nv, T* __restrict__ x, const T* __restrict__ xo,
                                            const T* __restrict__ b, T relax )
 {
    const int w = N + 1;
    T         centre;
-   const T   s = ordNeighbourSum( p, entries, entryW, N, stride, i, j, nv, x, xo, centre );
+   const T   s = ordNeighbourSum< T, MAXC >( p, entries, entryW, N, stride, i, j, nv, x, xo, centre );
    int       qx, qy, qz;
    entryPos( entries[p.entry0], N, i, j, qx, qy, qz );
    const i64 o0 = i64( entries[p.entry0].cell ) * stride + cidx( w, qx, qy, qz );
@@ -96,22 +104,73 @@ __global__ __launch_bounds__( 64 ) void kSorOrdVertices( const int* __restrict__
       ordUpdate( p, entries, entryW, 1 << L, stride, 0, 0, 1, x, xo, b, relax );
 }
 
-// one thread per edge; every step reads the value the same thread wrote in the step before
+// one wave per edge; at every point of the edge lane 16 k' + d takes term d (0: the centre weight) of the incident
+// cells k = k', k' + 4, ...; the wave's sums give the new value, which the centre lanes write to their cells' replicas
+// before the wave moves on (every step reads what the step before wrote)
+template < typename T >
+__device__ __forceinline__ T waveSum( T v )
+{
+#pragma unroll
+   for ( int m = 32; m >= 1; m >>= 1 )
+      v += __shfl_xor( v, m, 64 );
+   return v;
+}
 template < typename T >
 __global__ __launch_bounds__( 64 ) void kSorOrdEdges( const int* __restrict__ list, int n, const LowerPrim* __restrict__ prims,
                                                        const LowerEntry* __restrict__ entries, const T* __restrict__ entryW, int L,
                                                        i64 stride, int flag, int backwards, T* __restrict__ x, const T* __restrict__ xo,
                                                        const T* __restrict__ b, T relax )
 {
-   const int t = int( blockIdx.x ) * 64 + int( threadIdx.x );
+   const int t = int( blockIdx.x );
    if ( t >= n )
       return;
    const LowerPrim p = prims[list[t]];
    if ( !( p.bc & flag ) )
       return;
-   const int N = 1 << L;
+   const int      N = 1 << L, w = N + 1;
+   const int      lane = int( threadIdx.x ), k0 = lane >> 4, d = lane & 15;
+   constexpr int  nv   = 2;
    for ( int s = 1; s <= N - 1; ++s )
-      ordUpdate( p, entries, entryW, N, stride, backwards ? N - s : s, 0, 2, x, xo, b, relax );
+   {
+      const int i   = backwards ? N - s : s;
+      T         sum = T( 0 ), cen = T( 0 );
+      for ( int k = k0; k < p.nCells; k += 4 )
+      {
+         const LowerEntry e = entries[p.entry0 + k];
+         int              qx, qy, qz;
+         entryPos( e, N, i, 0, qx, qy, qz );
+         const T* wv = entryW + i64( p.entry0 + k ) * 16;
+         if ( d == 0 )
+            cen += wv[0];
+         else if ( d < NDIR )
+         {
+            const T   wd = wv[d];
+            const int ax = qx + dirX( d ), ay = qy + dirY( d ), az = qz + dirZ( d );
+            if ( wd != T( 0 ) && inCell( N, ax, ay, az ) )
+            {
+               const unsigned pmask = ( 1u << e.l0 ) | ( 1u << e.l1 );
+               const unsigned nz    = ( N - ax - ay - az != 0 ? 1u : 0u ) | ( ax != 0 ? 2u : 0u ) | ( ay != 0 ? 4u : 0u ) | ( az != 0 ? 8u : 0u );
+               const bool     cur   = __popc( nz ) > nv || ( nz & ~pmask ) == 0u;
+               sum += wd * ( cur ? x : xo )[i64( e.cell ) * stride + cidx( w, ax, ay, az )];
+            }
+         }
+      }
+      sum = waveSum( sum );
+      cen = waveSum( cen );
+      int qx, qy, qz;
+      entryPos( entries[p.entry0], N, i, 0, qx, qy, qz );
+      const i64 o0 = i64( entries[p.entry0].cell ) * stride + cidx( w, qx, qy, qz );
+      const T   v  = ( T( 1 ) - relax ) * x[o0] + relax * ( b[o0] - sum ) * ( T( 1 ) / cen );
+      if ( d == 0 )
+         for ( int k = k0; k < p.nCells; k += 4 )
+         {
+            const LowerEntry e = entries[p.entry0 + k];
+            entryPos( e, N, i, 0, qx, qy, qz );
+            x[i64( e.cell ) * stride + cidx( w, qx, qy, qz )] = v;
+         }
+      // the next point reads these replicas
+      __threadfence_block();
+   }
 }
 
 // one block per face; pts[off[tau] .. off[tau+1]) = the face points (i | j << 16) with i + 2j = tau
@@ -132,7 +191,7 @@ __global__ __launch_bounds__( 256 ) void kSorOrdFaces( const int* __restrict__ l
       for ( int k = off[tau] + int( threadIdx.x ); k < off[tau + 1]; k += 256 )
       {
          const int ij = pts[k];
-         ordUpdate( p, entries, entryW, N, stride, ij & 0xffff, ij >> 16, 3, x, xo, b, relax );
+         ordUpdate< T, 2 >( p, entries, entryW, N, stride, ij & 0xffff, ij >> 16, 3, x, xo, b, relax ); // a face has <= 2 cells
       }
       __syncthreads();
    }
@@ -237,7 +296,7 @@ static void sorOrderedT( const MeshCtx& m, const LevelGeom& g, hyg_op_s::Level& 
    };
    auto edges = [&]() {
       if ( L >= 1 && t.nEdges > 0 )
-         hipLaunchKernelGGL( ( kSorOrdEdges< T > ), dim3( ceilDiv( t.nEdges, 64 ) ), dim3( 64 ), 0, s, t.edges, t.nEdges, m.prims, m.entries,
+         hipLaunchKernelGGL( ( kSorOrdEdges< T > ), dim3( t.nEdges ), dim3( 64 ), 0, s, t.edges, t.nEdges, m.prims, m.entries,
                              entryW, L, g.stride, flag, int( backwards ), x, (const T*) xo, b, relax );
    };
    auto faces = [&]() {
