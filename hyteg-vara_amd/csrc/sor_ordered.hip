@@ -184,14 +184,60 @@ __global__ __launch_bounds__( 256 ) void kSorOrdFaces( const int* __restrict__ l
    const LowerPrim p = prims[list[blockIdx.x]];
    if ( !( p.bc & flag ) )
       return;
-   const int N = 1 << L;
+   const int N = 1 << L, w = N + 1;
+   // the face's (at most two) cells and their stencils, loaded once for the whole sweep of the face
+   const int  nc = p.nCells < 2 ? p.nCells : 2;
+   LowerEntry E[2];
+   T          Wt[2][NDIR];
+#pragma unroll
+   for ( int k = 0; k < 2; ++k )
+   {
+      E[k] = entries[p.entry0 + ( k < nc ? k : 0 )];
+#pragma unroll
+      for ( int d = 0; d < NDIR; ++d )
+         Wt[k][d] = entryW[i64( p.entry0 + ( k < nc ? k : 0 ) ) * 16 + d];
+   }
    for ( int s = 0; s < nTau; ++s )
    {
       const int tau = backwards ? nTau - 1 - s : s;
-      for ( int k = off[tau] + int( threadIdx.x ); k < off[tau + 1]; k += 256 )
+      for ( int kk = off[tau] + int( threadIdx.x ); kk < off[tau + 1]; kk += 256 )
       {
-         const int ij = pts[k];
-         ordUpdate< T, 2 >( p, entries, entryW, N, stride, ij & 0xffff, ij >> 16, 3, x, xo, b, relax ); // a face has <= 2 cells
+         const int ij = pts[kk], i = ij & 0xffff, j = ij >> 16;
+         // the same terms in the same order as ordNeighbourSum / ordUpdate
+         T sum = T( 0 ), centre = T( 0 );
+#pragma unroll
+         for ( int k = 0; k < 2; ++k )
+         {
+            if ( k >= nc )
+               break;
+            int qx, qy, qz;
+            entryPos( E[k], N, i, j, qx, qy, qz );
+            const unsigned pmask = ( 1u << E[k].l0 ) | ( 1u << E[k].l1 ) | ( 1u << E[k].l2 );
+            const i64      base  = i64( E[k].cell ) * stride;
+            centre += Wt[k][0];
+#pragma unroll
+            for ( int d = 1; d < NDIR; ++d )
+            {
+               const int ax = qx + dirX( d ), ay = qy + dirY( d ), az = qz + dirZ( d );
+               if ( Wt[k][d] == T( 0 ) || !inCell( N, ax, ay, az ) )
+                  continue;
+               const unsigned nz  = ( N - ax - ay - az != 0 ? 1u : 0u ) | ( ax != 0 ? 2u : 0u ) | ( ay != 0 ? 4u : 0u ) | ( az != 0 ? 8u : 0u );
+               const bool     cur = __popc( nz ) > 3 || ( nz & ~pmask ) == 0u;
+               sum += Wt[k][d] * ( cur ? x : xo )[base + cidx( w, ax, ay, az )];
+            }
+         }
+         int qx, qy, qz;
+         entryPos( E[0], N, i, j, qx, qy, qz );
+         const i64 o0 = i64( E[0].cell ) * stride + cidx( w, qx, qy, qz );
+         const T   v  = ( T( 1 ) - relax ) * x[o0] + relax * ( b[o0] - sum ) * ( T( 1 ) / centre );
+#pragma unroll
+         for ( int k = 0; k < 2; ++k )
+         {
+            if ( k >= nc )
+               break;
+            entryPos( E[k], N, i, j, qx, qy, qz );
+            x[i64( E[k].cell ) * stride + cidx( w, qx, qy, qz )] = v;
+         }
       }
       __syncthreads();
    }
