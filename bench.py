@@ -137,7 +137,51 @@ def cpu_baseline(mesh_name, L, target_s, inner):
     except OSError as e:
         out["single_thread"] = {"value": None, "error": str(e)}
     out["host"] = host_info()
+    out["config1_solve"] = config1_cpu(ol, threads)
     return out
+
+
+def config1_problem(ndofs, bnd):
+    """Right-hand side of the 6-tet Kuhn cube problem at level 5: f ~ U[-1, 1] per DoF (seed 2021), zero on the
+    Dirichlet boundary."""
+    b0 = np.random.default_rng(2021).uniform(-1.0, 1.0, ndofs)
+    b0[bnd] = 0.0
+    return b0
+
+
+def config1_cpu(ol, threads):
+    """The reference's CPU-runnable case timed with the oracle on this host: -Laplace u = f on the 6-tet Kuhn cube,
+    level 5, zero Dirichlet, 10 V(2,2) cycles with weighted Jacobi (omega = 2/3), coarse level 0 by CG."""
+    m, L = ol.Mesh("cube_kuhn_6el.msh"), 5
+    _, bnd = m.coords(L)
+    op = ol.Operator(m, 0, L)
+    x, b = ol.Function(m, 0, L), ol.Function(m, 0, L)
+    b.set(L, config1_problem(len(bnd), bnd))
+    t0 = time.perf_counter()
+    res = ol.gmg(op, x, b, 0, L, 10, pre=2, post=2, smoother="jacobi", relax=2.0 / 3.0)
+    el = time.perf_counter() - t0
+    return {"seconds": round(el, 4), "cycles": 10, "rel_residual": float(res[-1] / res[0]), "threads": threads}
+
+
+def bench_config1(hyg, hip, st):
+    """The same 10 V(2,2) Jacobi cycles on the 6-tet Kuhn cube at level 5 on the GPU (eager, residual norm after every
+    cycle)."""
+    gm, L = hyg.Mesh("cube_kuhn_6el.msh"), 5
+    _, bnd = gm.coords(L)
+    b0 = config1_problem(len(bnd), bnd)
+    op = hyg.P1ConstantOperator(gm, 0, L)
+    g = hyg.GMG(op, 0, L, pre=2, post=2, smoother="jacobi", relax=2.0 / 3.0)
+    x, b = hyg.Function(gm, 0, L), hyg.Function(gm, 0, L)
+    b.upload(L, b0)
+    g.solve(x, b, L, 10)  # warm-up
+    x.upload(L, np.zeros(len(bnd)))
+    hip.sync(st)
+    t0 = time.perf_counter()
+    res = g.solve(x, b, L, 10)
+    el = time.perf_counter() - t0
+    return {"workload": "-Laplace u = f, cube_kuhn_6el.msh level 5 (35,937 DoFs), zero Dirichlet, f ~ U[-1,1] (seed 2021), "
+                        "10 V(2,2) cycles, weighted Jacobi 2/3, coarse level 0 CG",
+            "seconds": round(el, 4), "cycles": 10, "rel_residual": float(res[-1] / res[0])}
 
 
 def host_info():
@@ -900,6 +944,9 @@ def main():
     if not args.no_shell:
         out["shell_blending"] = bench_blending(hyg, hip, st, max(5, args.steps // 10), 6)
         out["shell_p2"] = bench_shell(hyg, hip, st, max(2, args.steps // 25), args.shell_level, 2026, args.shell_cycles)
+
+    # ---------------- the reference's CPU-runnable case on the GPU
+    out["config1"] = bench_config1(hyg, hip, st)
 
     # ---------------- CPU baseline (oracle), rank 0, N = 1
     if not args.no_cpu:

@@ -10,7 +10,8 @@
 // coordinates (1/2, 1/6, 1/6, 1/6)). Each apply
 //   1. evaluates kb_e for every micro-cell (kKBar): per lattice point the phases of the Fourier modes of
 //      k are formed once and the 30 quadrature points of the six cell types are reached by angle
-//      addition with per-cell tables of cos/sin of the quadrature offsets;
+//      addition with per-cell tables of cos/sin of the quadrature offsets (kKBarTables, once per level at
+//      operator creation; read with scalar loads);
 //   2. assembles at every point of every macro-cell the 15 weights of its stencil from the (up to) 24
 //      incident elements of that cell and applies them (kElemApply); interior points get the epilogue,
 //      points on faces, edges and vertices store the cell's one-sided sum;
@@ -22,6 +23,48 @@
 
 namespace hyg {
 
+// ------------------------------------------------------------------ coefficient tables
+// Tables of one level, built once (one block per cell): per mode the phase at the cell's vertex 0, its steps
+// along the three lattice axes and the amplitude (phase[(c nModes + m) 5 + ..]), and cos/sin of the phase offsets of
+// the 30 quadrature points of the six types from their anchor (tab[(c nModes + m) 60 + 2 (5 t + q) + {0, 1}]).
+__global__ __launch_bounds__( kBlock ) void kKBarTables( const ElemCell* __restrict__ cells, const double* __restrict__ modes, int nModes,
+                                                         double* __restrict__ phase, double* __restrict__ tab, float* __restrict__ tabF )
+{
+   const int       c  = int( blockIdx.x );
+   const ElemCell& ec = cells[c];
+   double*         mp = phase + i64( c ) * nModes * 5;
+   for ( int m = int( threadIdx.x ); m < nModes; m += kBlock )
+   {
+      const double kx = modes[5 * m], ky = modes[5 * m + 1], kz = modes[5 * m + 2];
+      const double tp = 2.0 * M_PI;
+      mp[5 * m]       = tp * ( kx * ec.v0[0] + ky * ec.v0[1] + kz * ec.v0[2] ) + modes[5 * m + 3];
+      for ( int k = 0; k < 3; ++k )
+         mp[5 * m + 1 + k] = tp * ( kx * ec.a[0][k] + ky * ec.a[1][k] + kz * ec.a[2][k] );
+      mp[5 * m + 4] = modes[5 * m + 4];
+   }
+   __syncthreads();
+   for ( int i = int( threadIdx.x ); i < 30 * nModes; i += kBlock )
+   {
+      const int m = i / 30, tq = i - 30 * m, t = tq / 5, q = tq - 5 * t;
+      // lattice offset of quadrature point q of type t from the anchor
+      double d[3] = { 0.0, 0.0, 0.0 };
+      for ( int v = 0; v < 4; ++v )
+      {
+         const double lam = q == 0 ? 0.25 : ( v == q - 1 ? 0.5 : 1.0 / 6.0 );
+         for ( int k = 0; k < 3; ++k )
+            d[k] += lam * kMC[t][v][k];
+      }
+      const double ang = mp[5 * m + 1] * d[0] + mp[5 * m + 2] * d[1] + mp[5 * m + 3] * d[2];
+      double       sn, cs;
+      sincos( ang, &sn, &cs );
+      const i64 o  = ( i64( c ) * nModes ) * 60 + 2 * i;
+      tab[o]       = cs;
+      tab[o + 1]   = sn;
+      tabF[o]      = float( cs );
+      tabF[o + 1]  = float( sn );
+   }
+}
+
 // ------------------------------------------------------------------ launch
 template < typename T >
 static void launchKBar( const hyg_op_s& op, int L, int nC, T* kb, hipStream_t s )
@@ -32,10 +75,20 @@ static void launchKBar( const hyg_op_s& op, int L, int nC, T* kb, hipStream_t s 
       hipLaunchKernelGGL( ( kKBarPoly< T > ), dim3( nC * tK ), dim3( kBlock ), 0, s, L, tK, op.levels[L].elem, op.kPoly, op.kPolyDeg, kb );
    else if ( tK > 0 && S3i( N ) > 0 )
    {
-      const size_t lds = size_t( op.nModes ) * ( 5 * sizeof( double ) + 60 * sizeof( T ) );
-      hipLaunchKernelGGL( ( kKBar< T > ), dim3( nC * tK ), dim3( kBlock ), lds, s, L, tK, op.levels[L].elem, op.modes, op.nModes, op.k0,
+      const hyg_op_s::Level& ol = op.levels[L];
+      const T*               tb = std::is_same< T, double >::value ? (const T*) ol.kbTab : (const T*) ol.kbTabF;
+      hipLaunchKernelGGL( ( kKBar< T > ), dim3( nC * tK ), dim3( kBlock ), 0, s, L, tK, (const double*) ol.kbPhase, tb, op.nModes, op.k0,
                           kb );
    }
+}
+
+// per-level tables of the Fourier coefficient (phases and quadrature-offset cos/sin), once at operator setup
+void launchKBarTables( const MeshCtx& m, const hyg_op_s& op, int L, hipStream_t s )
+{
+   const hyg_op_s::Level& ol = op.levels[L];
+   if ( op.nModes > 0 )
+      hipLaunchKernelGGL( kKBarTables, dim3( m.mesh->nC() ), dim3( kBlock ), 0, s, ol.elem, op.modes, op.nModes, ol.kbPhase, ol.kbTab,
+                          ol.kbTabF );
 }
 
 // coefficient integrals of every level in both value types, kept for the applies that follow
@@ -132,6 +185,8 @@ void launchElementwise( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int 
 
 void launchElementwiseInverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s )
 {
+   // the coefficient tables of the level (called once per level when the operator is created)
+   launchKBarTables( m, op, L, s );
    const hyg_op_s::Level& ol = op.levels[L];
    const i64              n  = i64( m.mesh->nC() ) * g.stride + kTailSlack;
    (void) hipMemsetAsync( ol.dinv, 0, size_t( n ) * sizeof( double ), s );

@@ -23,49 +23,21 @@ __device__ __forceinline__ double expT( double a ) { return exp( a ); }
 __device__ __forceinline__ float  expT( float a ) { return expf( a ); }
 
 // ------------------------------------------------------------------ 1. coefficient integrals
-// One thread per anchor (x, y, z), x+y+z <= N-1, of one macro-cell; kb[c][t][anchor] for the six types.
-// Dynamic LDS: nModes x 30 x (cos, sin) of the quadrature offsets, then nModes x (theta0, gx, gy, gz, a).
+// One thread per anchor (x, y, z), x+y+z <= N-1, of one macro-cell; kb[c][t][anchor] for the six types. The tables
+// of the block's cell are wave-uniform: they are read with scalar loads and feed the multiply-adds directly.
 template < typename T >
-__global__ __launch_bounds__( kBlock ) void kKBar( int L, int tilesPerCell, const ElemCell* __restrict__ cells,
-                                                   const double* __restrict__ modes, int nModes, double k0, T* __restrict__ kb )
+__global__ __launch_bounds__( kBlock ) void kKBar( int L, int tilesPerCell, const double* __restrict__ phase, const T* __restrict__ tab,
+                                                   int nModes, double k0, T* __restrict__ kb )
 {
-   extern __shared__ double ldsD[];
+   typedef const double __attribute__( ( address_space( 4 ) ) ) CD;
+   typedef const T __attribute__( ( address_space( 4 ) ) ) CT;
    const int c    = int( blockIdx.x ) / tilesPerCell;
    const int tile = int( blockIdx.x ) - c * tilesPerCell;
    const int N    = 1 << L;
-   T*        tab  = reinterpret_cast< T* >( ldsD + 5 * nModes );
-   double*   mp   = ldsD;
-   const ElemCell& ec = cells[c];
-   for ( int m = int( threadIdx.x ); m < nModes; m += kBlock )
-   {
-      const double kx = modes[5 * m], ky = modes[5 * m + 1], kz = modes[5 * m + 2];
-      const double tp = 2.0 * M_PI;
-      mp[5 * m]       = tp * ( kx * ec.v0[0] + ky * ec.v0[1] + kz * ec.v0[2] ) + modes[5 * m + 3];
-      for ( int k = 0; k < 3; ++k )
-         mp[5 * m + 1 + k] = tp * ( kx * ec.a[0][k] + ky * ec.a[1][k] + kz * ec.a[2][k] );
-      mp[5 * m + 4] = modes[5 * m + 4];
-   }
-   __syncthreads();
-   for ( int i = int( threadIdx.x ); i < 30 * nModes; i += kBlock )
-   {
-      const int m = i / 30, tq = i - 30 * m, t = tq / 5, q = tq - 5 * t;
-      // lattice offset of quadrature point q of type t from the anchor
-      double d[3] = { 0.0, 0.0, 0.0 };
-      for ( int v = 0; v < 4; ++v )
-      {
-         const double lam = q == 0 ? 0.25 : ( v == q - 1 ? 0.5 : 1.0 / 6.0 );
-         for ( int k = 0; k < 3; ++k )
-            d[k] += lam * kMC[t][v][k];
-      }
-      const double ang = mp[5 * m + 1] * d[0] + mp[5 * m + 2] * d[1] + mp[5 * m + 3] * d[2];
-      double       sn, cs;
-      sincos( ang, &sn, &cs );
-      tab[2 * i]     = T( cs );
-      tab[2 * i + 1] = T( sn );
-   }
-   __syncthreads();
-   const int S = S3i( N );
-   const int s = tile * kBlock + int( threadIdx.x );
+   const CD* mp   = (const CD*) ( phase + i64( c ) * nModes * 5 );
+   const CT* tc   = (const CT*) ( tab + i64( c ) * nModes * 60 );
+   const int S    = S3i( N );
+   const int s    = tile * kBlock + int( threadIdx.x );
    if ( s >= S )
       return;
    int x, y, z;
@@ -79,9 +51,9 @@ __global__ __launch_bounds__( kBlock ) void kKBar( int L, int tilesPerCell, cons
       const T th = T( mp[5 * m] + mp[5 * m + 1] * x + mp[5 * m + 2] * y + mp[5 * m + 3] * z );
       T       sn, cs;
       sincosT( th, sn, cs );
-      const T  am = T( mp[5 * m + 4] );
-      const T  ac = am * cs, as = am * sn;
-      const T* tb = tab + 60 * m;
+      const T   am = T( mp[5 * m + 4] );
+      const T   ac = am * cs, as = am * sn;
+      const CT* tb = tc + 60 * m;
 #pragma unroll
       for ( int k = 0; k < 30; ++k )
          g[k] += ac * tb[2 * k] - as * tb[2 * k + 1];

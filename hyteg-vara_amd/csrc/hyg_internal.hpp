@@ -702,6 +702,11 @@ struct hyg_op_s
       hyg::ElemCell*        elem    = nullptr;
       double*               kb      = nullptr;
       float*                kbF     = nullptr;
+      // Fourier coefficient: per cell and mode the phase terms (theta0, gx, gy, gz, amplitude) and the cos/sin of the
+      // 30 quadrature offsets of the six micro-cell types (built once at operator setup, read by kKBar)
+      double*               kbPhase = nullptr;
+      double*               kbTab   = nullptr;
+      float*                kbTabF  = nullptr;
       double*               lowPart = nullptr;
       float*                lowPartF = nullptr;
       double*               dinv    = nullptr;

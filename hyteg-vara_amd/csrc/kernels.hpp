@@ -49,6 +49,7 @@ void launchElementwise( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int 
                         hipStream_t s );
 void launchElementwiseInverseDiagonal( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, hipStream_t s );
 void launchElementwiseStore( const MeshCtx& m, hyg_op_s& op, hipStream_t s );
+void launchKBarTables( const MeshCtx& m, const hyg_op_s& op, int L, hipStream_t s );
 // Surrogate stencils of the elementwise operator (P1SurrogateOperator): fit the cell-interior stencils of every
 // level by polynomials of degree deg from the samples of the interpolation level (needs the stored coefficient
 // integrals; synchronous), whether level L uses them, the interior part of one stencil operation, and the interior

@@ -573,6 +573,13 @@ void assembleP1DivKGrad( hyg_op_s& op )
       ol.facePartF = static_cast< float* >( alloc( nFace * sizeof( float ) ) );
       ol.dinv      = static_cast< double* >( alloc( nFunc * sizeof( double ) ) );
       ol.dinvF     = static_cast< float* >( alloc( nFunc * sizeof( float ) ) );
+      if ( op.nModes > 0 )
+      {
+         const size_t nm = size_t( m.nC() ) * size_t( op.nModes );
+         ol.kbPhase      = static_cast< double* >( alloc( nm * 5 * sizeof( double ) ) );
+         ol.kbTab        = static_cast< double* >( alloc( nm * 60 * sizeof( double ) ) );
+         ol.kbTabF       = static_cast< float* >( alloc( nm * 60 * sizeof( float ) ) );
+      }
    }
 }
 
@@ -927,6 +934,9 @@ hyg_op_s::~hyg_op_s()
       (void) hipFree( l.elem );
       (void) hipFree( l.kb );
       (void) hipFree( l.kbF );
+      (void) hipFree( l.kbPhase );
+      (void) hipFree( l.kbTab );
+      (void) hipFree( l.kbTabF );
       (void) hipFree( l.lowPart );
       (void) hipFree( l.lowPartF );
       (void) hipFree( l.dinv );
