@@ -180,12 +180,17 @@ template < typename T >
 __global__ __launch_bounds__( kBlock ) void kP2KQInt( int L, i64 stride, P2ClassTiles ct, const T* __restrict__ el,
                                                       const T* __restrict__ kq, i64 kqStride, const T* __restrict__ dinv,
                                                       P2Comps< T > cp, int nc, int mode, T a, const CellFaceSlot* __restrict__ cfs,
-                                                      int flag, T* __restrict__ part, i64 partStride )
+                                                      int flag, T* __restrict__ part, i64 partStride, int nC )
 {
    __shared__ T Ms[kP2MaxPairs * 10 * kP2RowPad];
    const int    per = ct.start[8];
-   const int    c   = int( blockIdx.x ) / per;
-   const int    t   = int( blockIdx.x ) - c * per;
+   // each XCD takes a contiguous range of cells (the hardware deals workgroups out round-robin over the 8 XCDs), so
+   // a cell's coefficient values and source stay in one XCD's L2 while its class blocks run
+   const int b = remapXcd( int( blockIdx.x ), nC * per );
+   if ( b >= nC * per ) // the whole block (grid padded to a multiple of 8)
+      return;
+   const int c = b / per;
+   const int t = b - c * per;
    int          cls = 0;
    while ( cls < 7 && t >= ct.start[cls + 1] )
       ++cls;
@@ -201,8 +206,8 @@ void launchP2KQPoints( int L, int nC, i64 stride, const T* el, const T* kq, i64 
    ct.start[0] = 0;
    for ( int k = 0; k < 8; ++k )
       ct.start[k + 1] = ct.start[k] + ceilDiv( S3i( p2ClassExtent( N, k ) + 1 ), kBlock );
-   hipLaunchKernelGGL( ( kP2KQInt< T > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, stride, ct, el, kq, kqStride, dinv, cp, nc,
-                       mode, a, cfs, flag, part, partStride );
+   hipLaunchKernelGGL( ( kP2KQInt< T > ), dim3( xcdGrid( nC * ct.start[8] ) ), dim3( kBlock ), 0, s, L, stride, ct, el, kq, kqStride, dinv,
+                       cp, nc, mode, a, cfs, flag, part, partStride, nC );
 }
 
 } // namespace hyg
