@@ -132,12 +132,13 @@ static void elemT( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, co
       launchSurrogateInterior( m, g, op, L, std::is_same< T, double >::value ? 8 : 4, c, s );
    }
    else if ( op.blend )
-      hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG, false, true > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem,
-                          kb, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ),
-                          nullptr, 0, dbl ? (const T*) ol.bw : (const T*) ol.bwF );
+      hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG, false, true > ), dim3( xcdGrid( nC * tA ) ), dim3( kBlock ), 0, s, L, tA, g.stride,
+                          ol.elem, kb, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ),
+                          nullptr, 0, dbl ? (const T*) ol.bw : (const T*) ol.bwF, nC * tA );
    else
-      hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG > ), dim3( nC * tA ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem, kb,
-                          m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ) );
+      hipLaunchKernelGGL( ( kElemApply< T, MODE, DIAG > ), dim3( xcdGrid( nC * tA ) ), dim3( kBlock ), 0, s, L, tA, g.stride, ol.elem, kb,
+                          m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, x, in1, out0, out1, T( c.a ), nullptr, 0,
+                          nullptr, nC * tA );
    // 3. face, edge and vertex values
    const int nLow = g.firstFaceChunk, nFace = g.nChunks - g.firstFaceChunk;
    if ( nFace > 0 )

@@ -270,10 +270,15 @@ __global__ __launch_bounds__( kBlock ) void kElemApply( int L, int tilesPerCell,
                                                         const T* __restrict__ x, const T* __restrict__ in1,
                                                         T* __restrict__ out0, T* __restrict__ out1, T a,
                                                         const int* __restrict__ bnd = nullptr, int nBnd = 0,
-                                                        const T* __restrict__ wst = nullptr )
+                                                        const T* __restrict__ wst = nullptr, int nb = 0 )
 {
-   bElemApply< T, MODE, DIAG, BND, STW >( int( blockIdx.x ), L, tilesPerCell, stride, cells, kb, cinfo, cfs, prims, entries, part, lowPart,
-                                          dinv, flag, x, in1, out0, out1, a, bnd, nBnd, wst );
+   // nb > 0: the grid is padded to a multiple of 8 and each XCD takes a contiguous range of cells (the hardware deals
+   // workgroups out round-robin over the XCDs), so a cell's coefficients and source stay in one XCD's L2
+   const int b = nb > 0 ? remapXcd( int( blockIdx.x ), nb ) : int( blockIdx.x );
+   if ( nb > 0 && b >= nb )
+      return;
+   bElemApply< T, MODE, DIAG, BND, STW >( b, L, tilesPerCell, stride, cells, kb, cinfo, cfs, prims, entries, part, lowPart, dinv, flag, x,
+                                          in1, out0, out1, a, bnd, nBnd, wst );
 }
 
 } // namespace hyg

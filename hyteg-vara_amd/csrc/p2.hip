@@ -813,11 +813,15 @@ __device__ __forceinline__ void p2StoredIntDispatch( int cls, int c, int s, int 
 }
 template < typename T, int MODE, int NC >
 __global__ __launch_bounds__( kBlock ) void kP2StoredInt( int L, i64 stride, P2ClassTiles ct, const T* __restrict__ sw, P2Comps< T > cp,
-                                                          T a )
+                                                          T a, int nC )
 {
    const int per = ct.start[8];
-   const int c   = int( blockIdx.x ) / per;
-   const int t   = int( blockIdx.x ) - c * per;
+   // each XCD takes a contiguous range of cells (grid padded to a multiple of 8), keeping a cell's source in one L2
+   const int b = remapXcd( int( blockIdx.x ), nC * per );
+   if ( b >= nC * per )
+      return;
+   const int c = b / per;
+   const int t = b - c * per;
    int       cls = 0;
    while ( cls < 7 && t >= ct.start[cls + 1] )
       ++cls;
@@ -1203,8 +1207,8 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
       if ( ct.start[8] > 0 && ( c.flag & 1 ) && !skipInterior )
       {
          const P2Comps< T > cp{ { (const T*) c.x }, { (const T*) c.in1 }, { (T*) c.out0 }, { (T*) c.out1 } };
-         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 1 > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp,
-                             T( c.a ) );
+         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 1 > ), dim3( xcdGrid( nC * ct.start[8] ) ), dim3( kBlock ), 0, s, L, g.stride, ct, sw,
+                             cp, T( c.a ), nC );
       }
       P2FaceTiles ft;
       ft.start[0] = 0;
@@ -1288,11 +1292,14 @@ static void p2VectorT( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L
       const T*  sw = dbl ? (const T*) ol.p2s : (const T*) ol.p2sF;
       const int nC = m.mesh->nC();
       if ( n == 3 )
-         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 3 > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp, T( 0 ) );
+         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 3 > ), dim3( xcdGrid( nC * ct.start[8] ) ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp,
+                             T( 0 ), nC );
       else if ( n == 2 )
-         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 2 > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp, T( 0 ) );
+         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 2 > ), dim3( xcdGrid( nC * ct.start[8] ) ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp,
+                             T( 0 ), nC );
       else
-         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 1 > ), dim3( nC * ct.start[8] ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp, T( 0 ) );
+         hipLaunchKernelGGL( ( kP2StoredInt< T, MODE, 1 > ), dim3( xcdGrid( nC * ct.start[8] ) ), dim3( kBlock ), 0, s, L, g.stride, ct, sw, cp,
+                             T( 0 ), nC );
    }
    for ( int k = 0; k < n; ++k )
       p2TK< T, MODE, false, true >( m, g, op, L, cs[k], s, true );
