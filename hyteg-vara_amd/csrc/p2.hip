@@ -344,11 +344,16 @@ __device__ __forceinline__ void p2FacePoint( int c, int s, int N, i64 stride, co
 template < typename T, bool STORED >
 __global__ __launch_bounds__( kBlock ) void kP2Face( int L, i64 stride, P2FaceTiles ft, const T* __restrict__ fst,
                                                      const CellFaceSlot* __restrict__ cfs, int flag, T* __restrict__ part,
-                                                     const T* __restrict__ x )
+                                                     const T* __restrict__ x, int nC )
 {
    const int per = ft.start[4];
-   const int c   = int( blockIdx.x ) / ( 4 * per );
-   int       r   = int( blockIdx.x ) - c * 4 * per;
+   // nC > 0: each XCD takes a contiguous range of cells (grid padded to a multiple of 8), keeping a cell's data in one
+   // L2 (pays off for the stored stencils of many cells; the constant operator's launch keeps the plain order)
+   const int b = nC > 0 ? remapXcd( int( blockIdx.x ), nC * 4 * per ) : int( blockIdx.x );
+   if ( nC > 0 && b >= nC * 4 * per )
+      return;
+   const int c = b / ( 4 * per );
+   int       r = b - c * 4 * per;
    const int f   = r / per;
    r -= f * per;
    int sub = 0;
@@ -1193,7 +1198,7 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
       const T* fsw = dbl ? (const T*) ol.p2fst : (const T*) ol.p2fstF;
       if ( ft.start[4] > 0 )
          hipLaunchKernelGGL( ( kP2Face< T, false > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, fsw, m.cellFaces,
-                             c.flag, part, (const T*) c.x );
+                             c.flag, part, (const T*) c.x, 0 );
       const int tB = ceilDiv( i64( p2EdgeVertexCount( 1 << L ) ) * kP2EvLanes, kBlock );
       hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq, kqs,
                           m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
@@ -1215,8 +1220,8 @@ static void p2TK( const MeshCtx& m, LevelGeom& g, const hyg_op_s& op, int L, con
       for ( int k = 0; k < 4; ++k )
          ft.start[k + 1] = ft.start[k] + ceilDiv( p2FaceSubCount( 1 << L, k & 1, k >> 1 ), kBlock );
       if ( ft.start[4] > 0 )
-         hipLaunchKernelGGL( ( kP2Face< T, true > ), dim3( nC * 4 * ft.start[4] ), dim3( kBlock ), 0, s, L, g.stride, ft, sw, m.cellFaces,
-                             c.flag, part, (const T*) c.x );
+         hipLaunchKernelGGL( ( kP2Face< T, true > ), dim3( xcdGrid( nC * 4 * ft.start[4] ) ), dim3( kBlock ), 0, s, L, g.stride, ft, sw,
+                             m.cellFaces, c.flag, part, (const T*) c.x, nC );
       const int tB = ceilDiv( i64( p2EdgeVertexCount( 1 << L ) ) * kP2EvLanes, kBlock );
       hipLaunchKernelGGL( ( kP2Apply< T, MODE, DIAG, KQ, true > ), dim3( nC * tB ), dim3( kBlock ), 0, s, L, tB, g.stride, el, kq, kqs,
                           m.p2tab, m.cellInfo, m.cellFaces, m.prims, m.entries, part, low, dv, c.flag, (const T*) c.x,
