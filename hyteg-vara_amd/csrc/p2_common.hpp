@@ -35,10 +35,12 @@ __host__ __device__ constexpr int p2InteriorExtent( int N, int cls )
    const int k = ( cls & 1 ) + ( ( cls >> 1 ) & 1 ) + ( ( cls >> 2 ) & 1 );
    return ( N - 1 - k ) / 2 - ( 3 - k );
 }
-// all points of parity class cls of a cell lattice with N intervals: p = 2 q + cls, |q| <= M
+// all points of parity class cls of a cell lattice with N intervals: p = 2 q + cls, |q| <= M; M = -1 when the class
+// has no point (N = 2, cls = 7: (1, 1, 1) lies outside the cell; integer division would round -1/2 up to 0)
 __host__ __device__ constexpr int p2ClassExtent( int N, int cls )
 {
-   return ( N - ( cls & 1 ) - ( ( cls >> 1 ) & 1 ) - ( ( cls >> 2 ) & 1 ) ) / 2;
+   const int r = N - ( cls & 1 ) - ( ( cls >> 1 ) & 1 ) - ( ( cls >> 2 ) & 1 );
+   return r < 0 ? -1 : r / 2;
 }
 struct P2ClassTiles
 {

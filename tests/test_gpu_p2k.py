@@ -56,7 +56,8 @@ def test_p1_apply_on_the_shell(hyg, meshes):
     assert rel_inf(d.download(L), od.get(L)) <= TOL64
 
 
-@pytest.mark.parametrize("name,L", [("cube_24el.msh", 1), ("cube_24el.msh", 3), ("shell", 1), ("shell", 2)])
+@pytest.mark.parametrize("name,L", [("cube_24el.msh", 0), ("cube_24el.msh", 1), ("cube_24el.msh", 3), ("shell", 0), ("shell", 1),
+                                    ("shell", 2)])
 @pytest.mark.parametrize("coef", ["exp", "poly"])
 @pytest.mark.parametrize("stored", [False, True])
 def test_apply_diagonal_smoother_parity(hyg, meshes, name, L, coef, stored):
